@@ -1,0 +1,110 @@
+/*
+ * kapdqmc.h — C ABI of the MI355X projector-DQMC library (libkapdqmc_hip.so).
+ *
+ * The library replaces the numeric bodies of the reference's Julia entry points for the
+ * projector-DQMC inner loop. Every call returns an int status (KDQ_OK or a negative
+ * code); kdq_last_error() gives the message. A handle owns one HIP stream and a batch of
+ * independent Markov chains that advance together; calls on distinct handles may run on
+ * different host threads, a handle itself is not re-entrant.
+ *
+ * Data crossing the boundary uses the reference's layouts: column-major matrices, field
+ * arrays as UInt8 with values 1..4, complex numbers interleaved (ComplexF64).
+ */
+#ifndef KAPDQMC_H
+#define KAPDQMC_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KDQ_OK 0
+#define KDQ_EARG (-1)    /* invalid argument or shape */
+#define KDQ_EHIP (-2)    /* HIP runtime error (allocation, launch, device) */
+#define KDQ_ENUM (-3)    /* numerical failure (singular overlap, non-finite G) */
+#define KDQ_ESTATE (-4)  /* call not valid in the handle's current state */
+
+enum { KDQ_MODEL_TU = 0, KDQ_MODEL_TV = 1, KDQ_MODEL_TUV = 2 };
+enum { KDQ_LATTICE_HC120 = 0, KDQ_LATTICE_HC60 = 1, KDQ_LATTICE_SQ90 = 2, KDQ_LATTICE_SQ45 = 3 };
+enum { KDQ_PREC_FP64 = 0, KDQ_PREC_MIXED = 1 };
+
+/* Model constants of one chain: the fields of tU_Hubbard_Para_ (src/tU/model.jl:14-37)
+ * that the sweep reads. All matrices are column-major, ns x ns (Pt: ns x ns/2). */
+typedef struct kdq_model_desc {
+    int32_t kind;            /* KDQ_MODEL_* */
+    int32_t ns;              /* model.Ns */
+    int32_t nt;              /* model.Nt */
+    int32_t n_nodes;         /* length(model.nodes) */
+    const int64_t *nodes;    /* model.nodes */
+    int32_t lattice;         /* KDQ_LATTICE_* (model.Lattice) */
+    int32_t site[2];         /* model.site */
+    double ht;               /* model.Ht */
+    const double *alpha;     /* model.α, length nt */
+    const double *eta;       /* model.η, length 4 */
+    const double *gamma;     /* model.γ, length 4 */
+    const double *K;         /* model.K (real part; flux = 0) */
+    const double *eK;        /* model.eK */
+    const double *eKinv;     /* model.eKinv */
+    const double *halfeK;    /* model.HalfeK */
+    const double *halfeKinv; /* model.HalfeKinv */
+    const double *Pt;        /* model.Pt (real) */
+} kdq_model_desc;
+
+/* Per-chain run settings. The random stream is Philox4x32-10 keyed by `seed`, with the
+ * chain id and the sweep counter in the counter words, so a run is bit-reproducible
+ * and resumable from (fields, sweep counter). */
+typedef struct kdq_run_desc {
+    uint64_t seed;
+    uint32_t chain;          /* stream id (< 2^24) */
+    uint32_t sweep0;         /* sweep counter of the first sweep */
+    int32_t precision;       /* KDQ_PREC_* */
+    int32_t delay;           /* local-update block size k (1..64); 1 = Sherman-Morrison */
+} kdq_run_desc;
+
+typedef struct kdq_handle_s *kdq_handle;
+
+/* Creates a batch of `nchains` chains on the current HIP device. All chains of a batch
+ * must share nt and nodes; ns, couplings and seeds may differ.
+ * Replaces: the buffer set-up at the top of phy_update (src/tU/phy_update.jl:7-37). */
+int kdq_create(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t nchains,
+               kdq_handle *out);
+int kdq_destroy(kdq_handle h);
+const char *kdq_last_error(kdq_handle h);   /* h may be NULL: last global error */
+int kdq_num_chains(kdq_handle h);
+
+/* Field arrays s[ns, nt] (UInt8, values 1..4, column-major) of chain `c`.
+ * set/get correspond to passing `s` into and out of phy_update (phy_update.jl:3,151). */
+int kdq_set_fields(kdq_handle h, int32_t c, const uint8_t *s);
+int kdq_get_fields(kdq_handle h, int32_t c, uint8_t *s);
+/* Initial_s (src/tU/GreenMatrix.jl:1-8) from the chain's Philox stream. */
+int kdq_initial_s(kdq_handle h);
+
+/* phy_update(path, model, s, Sweeps, record) (src/tU/phy_update.jl:3-152) for every chain.
+ * rows (may be NULL when record == 0) receives nchains x sweeps x 6 doubles
+ * [Ek Eu CDW0 CDW1 SDW0 SDW1] averaged per sweep, the row the reference appends to its CSV
+ * (phy_update.jl:144). The caller keeps the CSV writing. */
+int kdq_phy_update(kdq_handle h, int32_t sweeps, int32_t record, double *rows);
+
+/* ---- fine-grained entry points (parity checks, custom drivers) ---- */
+/* Current equal-time G of chain c as ns x ns ComplexF64 (interleaved, column-major). */
+int kdq_get_G(kdq_handle h, int32_t c, double *G);
+int kdq_set_G(kdq_handle h, int32_t c, const double *G);
+/* WrapKV!(…, D, G, "Forward"|"Backward", "B") with D = cis(±α[lt] η[s[:,lt]])
+ * (src/tU/GreenMatrix.jl:56-86, as called at phy_update.jl:61-64 and :123-126). */
+int kdq_wrap(kdq_handle h, int32_t lt, int32_t forward);
+/* UpdatePhyLayer! at slice lt (src/tU/phy_update.jl:256-266) with draws from the Philox
+ * stream of sweep `sweep` and pass direction dir (0 forward, 1 backward). */
+int kdq_update_layer(kdq_handle h, int32_t lt, int32_t dir, uint32_t sweep);
+/* Rebuilds the U_L chain and sets G = get_G!(BL[1], BR[1]) (phy_update.jl:39-50). */
+int kdq_init_chain(kdq_handle h);
+/* phy_measure(model, lt, s, G) (src/tU/phy_update.jl:154-254) of the current G;
+ * out receives nchains x 6 doubles. */
+int kdq_measure(kdq_handle h, int32_t lt, double *out);
+
+/* Counters: out[0] = proposals, out[1] = accepted, per chain (nchains x 2). */
+int kdq_get_counters(kdq_handle h, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,85 @@
+"""ctypes binding of libkapdqmc_hip.so (the C ABI declared in include/kapdqmc.h).
+
+The library is built in-tree by ``kapdqmc_amd.build.build()``; importing the product API
+without it raises immediately — there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libkapdqmc_hip.so")
+
+KDQ_OK, KDQ_EARG, KDQ_EHIP, KDQ_ENUM, KDQ_ESTATE = 0, -1, -2, -3, -4
+MODEL_TU, MODEL_TV, MODEL_TUV = 0, 1, 2
+LATTICE_CODE = {"HoneyComb120": 0, "HoneyComb60": 1, "SQUARE90": 2, "SQUARE45": 3}
+PREC_FP64, PREC_MIXED = 0, 1
+
+# Every symbol include/kapdqmc.h declares (checked by tests/test_capi.py).
+EXPORTS = ("kdq_create", "kdq_destroy", "kdq_last_error", "kdq_num_chains", "kdq_set_fields",
+           "kdq_get_fields", "kdq_initial_s", "kdq_phy_update", "kdq_get_G", "kdq_set_G", "kdq_wrap",
+           "kdq_update_layer", "kdq_init_chain", "kdq_measure", "kdq_get_counters")
+
+
+class KdqError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class ModelDesc(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("ns", C.c_int32), ("nt", C.c_int32), ("n_nodes", C.c_int32),
+                ("nodes", C.POINTER(C.c_int64)), ("lattice", C.c_int32), ("site", C.c_int32 * 2),
+                ("ht", C.c_double), ("alpha", C.POINTER(C.c_double)), ("eta", C.POINTER(C.c_double)),
+                ("gamma", C.POINTER(C.c_double)), ("K", C.POINTER(C.c_double)),
+                ("eK", C.POINTER(C.c_double)), ("eKinv", C.POINTER(C.c_double)),
+                ("halfeK", C.POINTER(C.c_double)), ("halfeKinv", C.POINTER(C.c_double)),
+                ("Pt", C.POINTER(C.c_double))]
+
+
+class RunDesc(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("chain", C.c_uint32), ("sweep0", C.c_uint32),
+                ("precision", C.c_int32), ("delay", C.c_int32)]
+
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Load the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is missing: build it with kapdqmc_amd.build.build() "
+                          "(the product path has no CPU fallback)")
+    L = C.CDLL(path)
+    vp, i32, u32, dp = C.c_void_p, C.c_int32, C.c_uint32, C.POINTER(C.c_double)
+    L.kdq_create.argtypes = [C.POINTER(ModelDesc), C.POINTER(RunDesc), i32, C.POINTER(vp)]
+    L.kdq_destroy.argtypes = [vp]
+    L.kdq_last_error.argtypes = [vp]
+    L.kdq_last_error.restype = C.c_char_p
+    L.kdq_num_chains.argtypes = [vp]
+    L.kdq_set_fields.argtypes = [vp, i32, C.POINTER(C.c_uint8)]
+    L.kdq_get_fields.argtypes = [vp, i32, C.POINTER(C.c_uint8)]
+    L.kdq_initial_s.argtypes = [vp]
+    L.kdq_phy_update.argtypes = [vp, i32, i32, dp]
+    L.kdq_get_G.argtypes = [vp, i32, dp]
+    L.kdq_set_G.argtypes = [vp, i32, dp]
+    L.kdq_wrap.argtypes = [vp, i32, i32]
+    L.kdq_update_layer.argtypes = [vp, i32, i32, u32]
+    L.kdq_init_chain.argtypes = [vp]
+    L.kdq_measure.argtypes = [vp, i32, dp]
+    L.kdq_get_counters.argtypes = [vp, dp]
+    for name in EXPORTS:
+        if name != "kdq_last_error":
+            getattr(L, name).restype = C.c_int
+    _lib = L
+    return L
+
+
+def check(rc: int, handle=None):
+    if rc != KDQ_OK:
+        msg = load().kdq_last_error(handle).decode(errors="replace")
+        raise KdqError(rc, msg)

@@ -1,0 +1,121 @@
+// Shared device/host definitions of the MI355X projector-DQMC library.
+//
+// Complex matrices are stored planar (a real plane and an imaginary plane, each
+// column-major with leading dimension ld) because CDNA4 matrix cores have no complex
+// type: a real x complex product is two real GEMMs sharing one operand and a
+// complex x complex product is four. Every kernel is batched over independent chains
+// (blockIdx.z = chain) and reads per-chain sizes and pointers from a ChainDev table in
+// device memory, so chains of different lattice size share one launch.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define KDQ_DEV __device__ __forceinline__
+#define KDQ_HD __host__ __device__ __forceinline__
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+// One operand of a batched GEMM: pointers of chain c in a device table.
+struct Opnd {
+    double *re;
+    double *im;  // nullptr for real operands
+    int ld;
+    int pad;
+};
+
+// Per-chain constants and state pointers (device table, one entry per chain).
+struct ChainDev {
+    int ns, np, nt, pad0;
+    const double *alpha;   // [nt]
+    double eta[4];
+    double gamma[4];
+    uint8_t *s;            // ns x nt column-major, values 1..4
+    double ht;
+    int lattice, L1, L2, pad1;
+    const double *K;       // ns x ns
+    uint64_t seed;
+    uint32_t chain_id, pad2;
+    // local-update scratch
+    int *acc_cnt;          // [1]
+    int *acc_idx;          // [64]
+    double *W;             // 2 planes of 64 x 64 (re, im), ld 64
+    double *counters;      // [2]: proposals, accepted
+    double *meas;          // [8]: accumulated observables + count
+};
+
+enum { DIM_NS = 0, DIM_NP = 1 };
+
+KDQ_DEV int dim_of(const ChainDev &c, int code) { return code == DIM_NS ? c.ns : c.np; }
+
+// Diagonal factor applied to GEMM rows, columns or the contraction index.
+// kind 0: none; kind 1: t-U phase cis(sign * alpha[lt-1] * eta[s[i, lt-1] - 1]).
+struct DiagSpec {
+    int kind;
+    int lt;    // 1-based slice
+    int sign;  // +1 or -1
+    int pad;
+};
+
+KDQ_DEV void diag_value(const ChainDev &c, const DiagSpec &d, int i, double &re, double &im) {
+    if (d.kind == 1) {
+        int sv = c.s[(size_t)i + (size_t)(d.lt - 1) * c.ns];
+        double x = d.sign * c.alpha[d.lt - 1] * c.eta[sv - 1];
+        sincos(x, &im, &re);
+    } else {
+        re = 1.0;
+        im = 0.0;
+    }
+}
+
+KDQ_HD void cmul(double ar, double ai, double br, double bi, double &cr, double &ci) {
+    double r = ar * br - ai * bi;
+    double i = ar * bi + ai * br;
+    cr = r;
+    ci = i;
+}
+
+// ---------------------------------------------------------------- Philox4x32-10
+// Identical stream layout to the CPU restatement: key = {seed_lo, seed_hi},
+// ctr = {elem, slice_tag, sweep, (stream << 24) | chain}.
+KDQ_HD void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
+                          uint32_t k1, uint32_t out[4]) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        uint32_t n1 = (uint32_t)p1;
+        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        uint32_t n3 = (uint32_t)p0;
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+// Proposal draw: choice in {0,1,2} (index into the other three field values, increasing
+// order) and a uniform double in [0,1).
+KDQ_HD void proposal_draw(uint64_t seed, uint32_t chain, uint32_t stream, uint32_t sweep,
+                          uint32_t slice_tag, uint32_t elem, int &choice, double &u) {
+    uint32_t x[4];
+    philox4x32_10(elem, slice_tag, sweep, (stream << 24) | (chain & 0xFFFFFFu), (uint32_t)seed,
+                  (uint32_t)(seed >> 32), x);
+    choice = (int)(((uint64_t)x[0] * 3u) >> 32);
+    u = ((double)(x[1] >> 5) * 67108864.0 + (double)(x[2] >> 6)) * (1.0 / 9007199254740992.0);
+}
+
+KDQ_HD int init_field_value(uint64_t seed, uint32_t chain, uint32_t stream, uint32_t slice,
+                            uint32_t elem) {
+    uint32_t x[4];
+    philox4x32_10(elem, slice, 0xFFFFFFFFu, (stream << 24) | (chain & 0xFFFFFFu), (uint32_t)seed,
+                  (uint32_t)(seed >> 32), x);
+    return 1 + (int)(((uint64_t)x[0] * 4u) >> 32);
+}
+
+// ------------------------------------------------------------------ reductions
+KDQ_DEV double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}

@@ -1,0 +1,174 @@
+"""Reference-shaped entry points over the HIP library.
+
+``Initial_s`` and ``phy_update`` keep the signatures and the argument meaning of the
+reference's generics (src/KAPDQMC.jl:12-16, src/tU/GreenMatrix.jl:1-8,
+src/tU/phy_update.jl:3-152): ``phy_update`` mutates and returns ``s`` and appends one CSV
+row per sweep when ``record`` is true. ``DQMCBatch`` is the handle-level API (a batch of
+independent chains advancing together on one GPU).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import _lib as L
+from .rng import initial_fields
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class DQMCBatch:
+    """A batch of t-U chains on the current GPU.
+
+    models: list of ``tU_Hubbard_Para_`` (same Nt and nodes); seeds: list of int.
+    delay: local-update block size k (1 = one Sherman-Morrison update per flip).
+    """
+
+    def __init__(self, models, seeds, chains=None, delay: int = 32, precision: str = "fp64",
+                 sweep0: int = 0):
+        self.L = L.load()
+        if not isinstance(models, (list, tuple)):
+            models = [models]
+        n = len(models)
+        seeds = list(seeds) if isinstance(seeds, (list, tuple)) else [seeds] * n
+        chains = list(range(n)) if chains is None else list(chains)
+        if precision != "fp64":
+            raise ValueError("only precision='fp64' is available")
+        self.models = list(models)
+        self._keep = []
+        descs = (L.ModelDesc * n)()
+        runs = (L.RunDesc * n)()
+        for c, m in enumerate(models):
+            if getattr(m, "kind", "tU") != "tU":
+                raise ValueError("DQMCBatch drives the t-U model")
+            f = lambda a: np.asfortranarray(np.asarray(a, dtype=np.float64))
+            arrs = dict(alpha=f(m.α), eta=f(m.η), gamma=f(m.γ), K=f(np.real(m.K)), eK=f(np.real(m.eK)),
+                        eKinv=f(np.real(m.eKinv)), halfeK=f(np.real(m.HalfeK)),
+                        halfeKinv=f(np.real(m.HalfeKinv)), Pt=f(np.real(m.Pt)))
+            nodes = np.ascontiguousarray(m.nodes, dtype=np.int64)
+            self._keep += list(arrs.values()) + [nodes]
+            d = descs[c]
+            d.kind, d.ns, d.nt, d.n_nodes = L.MODEL_TU, m.Ns, m.Nt, len(nodes)
+            d.nodes = nodes.ctypes.data_as(C.POINTER(C.c_int64))
+            d.lattice = L.LATTICE_CODE[m.Lattice]
+            d.site[0], d.site[1] = m.site
+            d.ht = m.Ht
+            for k, v in arrs.items():
+                setattr(d, k, _dp(v))
+            r = runs[c]
+            r.seed, r.chain, r.sweep0, r.precision, r.delay = seeds[c], chains[c], sweep0, L.PREC_FP64, delay
+        h = C.c_void_p()
+        L.check(self.L.kdq_create(descs, runs, n, C.byref(h)))
+        self.h = h
+        self.n = n
+        self.seeds, self.chains, self.delay = seeds, chains, delay
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.kdq_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        L.check(rc, self.h)
+
+    # fields ---------------------------------------------------------------
+    def set_fields(self, c: int, s):
+        m = self.models[c]
+        a = np.asfortranarray(np.asarray(s, dtype=np.uint8))
+        if a.shape != (m.Ns, m.Nt):
+            raise ValueError(f"s must have shape {(m.Ns, m.Nt)}")
+        self._chk(self.L.kdq_set_fields(self.h, c, a.ctypes.data_as(C.POINTER(C.c_uint8))))
+
+    def get_fields(self, c: int) -> np.ndarray:
+        m = self.models[c]
+        a = np.zeros((m.Ns, m.Nt), dtype=np.uint8, order="F")
+        self._chk(self.L.kdq_get_fields(self.h, c, a.ctypes.data_as(C.POINTER(C.c_uint8))))
+        return a
+
+    def initial_s(self):
+        self._chk(self.L.kdq_initial_s(self.h))
+
+    # sweeps ---------------------------------------------------------------
+    def phy_update(self, sweeps: int, record: bool = True) -> np.ndarray:
+        rows = np.zeros((self.n, sweeps, 6))
+        self._chk(self.L.kdq_phy_update(self.h, sweeps, int(record), _dp(rows)))
+        return rows
+
+    # fine-grained ---------------------------------------------------------
+    def get_G(self, c: int) -> np.ndarray:
+        n = self.models[c].Ns
+        buf = np.zeros(2 * n * n)
+        self._chk(self.L.kdq_get_G(self.h, c, _dp(buf)))
+        return buf.view(np.complex128).reshape(n, n).T.copy()
+
+    def set_G(self, c: int, G):
+        n = self.models[c].Ns
+        buf = np.ascontiguousarray(np.asarray(G, dtype=np.complex128).T).reshape(-1).view(np.float64).copy()
+        self._chk(self.L.kdq_set_G(self.h, c, _dp(buf)))
+
+    def wrap(self, lt: int, forward: bool):
+        self._chk(self.L.kdq_wrap(self.h, lt, int(forward)))
+
+    def update_layer(self, lt: int, direction: int, sweep: int):
+        self._chk(self.L.kdq_update_layer(self.h, lt, direction, sweep))
+
+    def init_chain(self):
+        self._chk(self.L.kdq_init_chain(self.h))
+
+    def measure(self, lt: int) -> np.ndarray:
+        out = np.zeros((self.n, 6))
+        self._chk(self.L.kdq_measure(self.h, lt, _dp(out)))
+        return out
+
+    def counters(self) -> np.ndarray:
+        out = np.zeros((self.n, 2))
+        self._chk(self.L.kdq_get_counters(self.h, _dp(out)))
+        return out
+
+
+def Initial_s(model, seed: int = 0, chain: int = 0) -> np.ndarray:
+    """Initial_s (src/tU/GreenMatrix.jl:1-8): UInt8 [Ns, Nt], i.i.d. uniform over {1..4},
+    drawn from the library's Philox stream instead of a MersenneTwister."""
+    return initial_fields(seed, chain, model.Ns, model.Nt)
+
+
+def _lattice_tag(lattice: str) -> str:
+    tags = {"SQUARE90": "□90", "SQUARE45": "□45", "HoneyComb60": "HC", "HoneyComb120": "HC120"}
+    if lattice not in tags:
+        raise ValueError(f"Lattice: {lattice} is not allowed !")
+    return tags[lattice]
+
+
+def phy_update(path: str, model, s: np.ndarray, Sweeps: int, record: bool = False, *, seed: int = 0,
+               chain: int = 0, delay: int = 32, sweep0: int = 0) -> np.ndarray:
+    """phy_update (src/tU/phy_update.jl:3-152) on the GPU. Mutates ``s`` in place, returns it,
+    and appends one row [Ek Eu CDW0 CDW1 SDW0 SDW1] per sweep to the reference's CSV file
+    name (phy_update.jl:23-27,141-149) when ``record`` is true."""
+    b = DQMCBatch([model], [seed], [chain], delay=delay, sweep0=sweep0)
+    try:
+        b.set_fields(0, s)
+        rows = b.phy_update(Sweeps, record)[0]
+        s[...] = b.get_fields(0)
+    finally:
+        b.close()
+    if record:
+        name = _lattice_tag(model.Lattice)
+        if model.Θquench == 0.0:
+            fn = f"tUphy{name}_t{model.Ht}U{model.Hu1}size{model.site}Δt{model.Δt}Θ{model.Θrelax}BS{model.BatchSize}.csv"
+        else:
+            fn = (f"tUphy{name}_t{model.Ht}U{model.Hu1}_{model.Hu2}size{model.site}Δt{model.Δt}"
+                  f"Θ{model.Θrelax}_{model.Θquench}BS{model.BatchSize}.csv")
+        with open(os.path.join(path, fn), "a") as fh:
+            for r in rows:
+                fh.write(",".join(repr(float(x)) for x in r) + "\n")
+    return s

@@ -1,0 +1,277 @@
+"""ORACLE — test infrastructure only.
+
+ctypes front end of the FP64 CPU restatement in oracle/*.c (see orc.h for what each
+function restates and how it is pinned). Only tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg may import this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liborc_kdq.so")
+LATTICE_CODE = {"HoneyComb120": 0, "HoneyComb60": 1, "SQUARE90": 2, "SQUARE45": 3}
+
+_lib = None
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with its Makefile (gcc only)."""
+    if force and os.path.exists(LIB_PATH):
+        os.remove(LIB_PATH)
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = C.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+def _declare(L):
+    L.orc_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+    L.orc_draw.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                           _ip, _dp]
+    L.orc_init_value.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]
+    L.orc_init_value.restype = C.c_int
+    L.orc_xy_i.argtypes = [_ip, C.c_int, C.c_int]
+    L.orc_xy_i.restype = C.c_int
+    L.orc_nn2idx.argtypes = [C.c_int, _ip, C.c_int, _ip]
+    L.orc_nn2idx.restype = C.c_int
+    L.orc_nnK.argtypes = [C.c_int, _ip, _dp]
+    L.orc_nnidx_F.argtypes = [C.c_int, _ip, _ip]
+    L.orc_jacobi_eigh.argtypes = [C.c_int, _dp, _dp, _dp]
+    L.orc_zqr_q.argtypes = [C.c_int, C.c_int, _dp, C.c_int, _dp]
+    L.orc_zgetrf.argtypes = [C.c_int, _dp, C.c_int, _ip]
+    L.orc_zgetrf.restype = C.c_int
+    L.orc_zgetrs.argtypes = [C.c_int, C.c_int, _dp, C.c_int, _ip, _dp, C.c_int]
+    L.orc_zinv.argtypes = [C.c_int, _dp, C.c_int]
+    L.orc_zinv.restype = C.c_int
+    L.orc_get_G.argtypes = [C.c_int, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, _dp]
+    L.orc_tu_new.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int64), C.c_int, _ip,
+                             C.c_double, C.c_double, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                             C.c_uint64, C.c_uint32]
+    L.orc_tu_new.restype = C.c_void_p
+    L.orc_tu_free.argtypes = [C.c_void_p]
+    L.orc_tu_initial_s.argtypes = [C.c_void_p]
+    L.orc_tu_init_chain.argtypes = [C.c_void_p]
+    L.orc_tu_phy_update.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp]
+    L.orc_tu_Gtau.argtypes = [C.c_void_p, C.c_int, _dp]
+    L.orc_tu_WrapKV.argtypes = [C.c_void_p, _dp, _dp, C.c_int, C.c_char]
+    L.orc_tu_update_layer.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    L.orc_tu_measure.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+    L.orc_tu_BM_F.argtypes = [C.c_void_p, C.c_int, _dp]
+
+
+def _p(a, t=C.c_double):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def _fortran(a, dtype):
+    return np.asfortranarray(np.asarray(a, dtype=dtype))
+
+
+# ----------------------------------------------------------------------------- RNG
+def philox(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib().orc_philox4x32_10(c, k, o)
+    return list(o)
+
+
+def draw(seed, chain, stream, sweep, slice_tag, elem):
+    ch = C.c_int()
+    u = C.c_double()
+    lib().orc_draw(seed, chain, stream, sweep, slice_tag, elem, C.byref(ch), C.byref(u))
+    return ch.value, u.value
+
+
+# ------------------------------------------------------------------------ geometry
+def nn2idx(lattice, site, idx):
+    s = (C.c_int * 2)(*site)
+    nn = (C.c_int * 4)()
+    z = lib().orc_nn2idx(LATTICE_CODE[lattice], s, idx, nn)
+    return list(nn)[:z]
+
+
+def nnK(lattice, site):
+    ns = 2 * site[0] * site[1]
+    K = np.zeros((ns, ns), order="F")
+    lib().orc_nnK(LATTICE_CODE[lattice], (C.c_int * 2)(*site), _p(K))
+    return K
+
+
+# ------------------------------------------------------------------ linear algebra
+def eigh(A):
+    A = _fortran(A, np.float64)
+    n = A.shape[0]
+    w = np.zeros(n)
+    V = np.zeros((n, n), order="F")
+    lib().orc_jacobi_eigh(n, _p(A), _p(w), _p(V))
+    return w, V
+
+
+def qr_q(A):
+    A = _fortran(A, np.complex128).copy(order="F")
+    m, n = A.shape
+    lr = C.c_double()
+    lib().orc_zqr_q(m, n, _p(A), m, C.byref(lr))
+    return A, lr.value
+
+
+def lu_solve(A, B):
+    A = _fortran(A, np.complex128).copy(order="F")
+    B = _fortran(B, np.complex128).copy(order="F")
+    n = A.shape[0]
+    ipiv = np.zeros(n, dtype=np.int32)
+    info = lib().orc_zgetrf(n, _p(A), n, _p(ipiv, C.c_int))
+    lib().orc_zgetrs(n, B.shape[1], _p(A), n, _p(ipiv, C.c_int), _p(B), n)
+    return B, info
+
+
+def inv(A):
+    A = _fortran(A, np.complex128).copy(order="F")
+    info = lib().orc_zinv(A.shape[0], _p(A), A.shape[0])
+    return A, info
+
+
+def get_G(BL, BR):
+    BL = _fortran(BL, np.complex128)
+    BR = _fortran(BR, np.complex128)
+    np_, ns = BL.shape
+    G = np.zeros((ns, ns), dtype=np.complex128, order="F")
+    ld = C.c_double()
+    lib().orc_get_G(ns, np_, _p(BL), np_, _p(BR), ns, _p(G), C.byref(ld))
+    return G, ld.value
+
+
+# -------------------------------------------------------------------------- t-U
+class OracleTU:
+    """Oracle state for one t-U chain built from a host model (kapdqmc_amd.models)."""
+
+    def __init__(self, model, seed: int = 0, chain: int = 0):
+        self.model = model
+        L = lib()
+        f = lambda a: _fortran(a, np.float64)
+        self._keep = [f(model.α), f(model.η), f(model.γ),
+                      f(model.K), f(model.eK), f(model.eKinv), f(model.HalfeK), f(model.HalfeKinv),
+                      f(model.Pt), np.ascontiguousarray(model.nodes, dtype=np.int64)]
+        a, eta, gam, K, eK, eKi, hK, hKi, Pt, nodes = self._keep
+        site = (C.c_int * 2)(*model.site)
+        self.h = L.orc_tu_new(model.Ns, model.Nt, model.BatchSize, len(nodes),
+                              nodes.ctypes.data_as(C.POINTER(C.c_int64)), LATTICE_CODE[model.Lattice],
+                              site, model.Ht, model.Δt, _p(a), _p(eta), _p(gam), _p(K), _p(eK),
+                              _p(eKi), _p(hK), _p(hKi), _p(Pt), seed, chain)
+        self.ns, self.nt = model.Ns, model.Nt
+
+    def __del__(self):
+        try:
+            lib().orc_tu_free(self.h)
+        except Exception:
+            pass
+
+    # state access through the C struct layout
+    class _S(C.Structure):
+        _fields_ = [("ns", C.c_int), ("np", C.c_int), ("nt", C.c_int), ("bs", C.c_int),
+                    ("nn_nodes", C.c_int), ("lattice", C.c_int), ("site", C.c_int * 2),
+                    ("ht", C.c_double), ("dt", C.c_double), ("nodes", C.POINTER(C.c_int)),
+                    ("alpha", _dp), ("eta", C.c_double * 4), ("gamma", C.c_double * 4),
+                    ("K", _dp), ("eK", _dp), ("eKinv", _dp), ("HalfeK", _dp), ("HalfeKinv", _dp),
+                    ("Pt", _dp), ("s", C.POINTER(C.c_uint8)), ("G", _dp), ("BLs", _dp), ("BRs", _dp),
+                    ("tmpNN", _dp), ("tmpNN2", _dp), ("BM", _dp), ("tmpNn", _dp), ("tmpnN", _dp),
+                    ("tmpN", _dp), ("idx", C.c_int), ("seed", C.c_uint64), ("chain", C.c_uint32),
+                    ("sweep", C.c_uint32), ("n_accept", C.c_longlong), ("n_propose", C.c_longlong)]
+
+    @property
+    def _st(self):
+        return C.cast(self.h, C.POINTER(self._S)).contents
+
+    def initial_s(self):
+        lib().orc_tu_initial_s(self.h)
+
+    @property
+    def s(self):
+        st = self._st
+        return np.ctypeslib.as_array(st.s, shape=(self.nt * self.ns,)).reshape(self.nt, self.ns).T.copy()
+
+    @s.setter
+    def s(self, value):
+        st = self._st
+        arr = np.ctypeslib.as_array(st.s, shape=(self.nt * self.ns,))
+        arr[:] = np.asarray(value, dtype=np.uint8).T.reshape(-1)
+
+    @property
+    def G(self):
+        st = self._st
+        raw = np.ctypeslib.as_array(st.G, shape=(2 * self.ns * self.ns,))
+        return raw.view(np.complex128).reshape(self.ns, self.ns).T.copy()
+
+    @G.setter
+    def G(self, value):
+        st = self._st
+        raw = np.ctypeslib.as_array(st.G, shape=(2 * self.ns * self.ns,)).view(np.complex128)
+        raw[:] = np.asarray(value, dtype=np.complex128).T.reshape(-1)
+
+    @property
+    def sweep(self):
+        return self._st.sweep
+
+    @sweep.setter
+    def sweep(self, v):
+        C.cast(self.h, C.POINTER(self._S)).contents.sweep = v
+
+    @property
+    def acceptance(self):
+        st = self._st
+        return st.n_accept / max(st.n_propose, 1)
+
+    @property
+    def idx(self):
+        return self._st.idx
+
+    def init_chain(self):
+        lib().orc_tu_init_chain(self.h)
+
+    def phy_update(self, sweeps: int, record: bool = True):
+        rows = np.zeros((sweeps, 6))
+        lib().orc_tu_phy_update(self.h, sweeps, int(record), _p(rows))
+        return rows
+
+    def Gtau(self, tau: int):
+        out = np.zeros((self.ns, self.ns), dtype=np.complex128, order="F")
+        lib().orc_tu_Gtau(self.h, tau, _p(out))
+        return out
+
+    def wrap(self, G, D, forward: bool, LR: str):
+        Gf = _fortran(G, np.complex128).copy(order="F")
+        Df = np.ascontiguousarray(D, dtype=np.complex128)
+        lib().orc_tu_WrapKV(self.h, _p(Gf), _p(Df), int(forward), LR.encode())
+        return Gf
+
+    def update_layer(self, lt: int, direction: int):
+        lib().orc_tu_update_layer(self.h, lt, direction)
+
+    def measure(self, lt: int, G):
+        Gf = _fortran(G, np.complex128)
+        out = np.zeros(6)
+        lib().orc_tu_measure(self.h, lt, _p(Gf), _p(out))
+        return out
+
+    def BM(self, idx: int):
+        out = np.zeros((self.ns, self.ns), dtype=np.complex128, order="F")
+        lib().orc_tu_BM_F(self.h, idx, _p(out))
+        return out

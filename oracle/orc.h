@@ -1,0 +1,96 @@
+/*
+ * ORACLE — test infrastructure only.
+ *
+ * FP64 CPU restatement of the KAPDQMC (Julia) projector-DQMC algorithm, used as the
+ * parity checker for the HIP library. Only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg may load it; the product path never calls it.
+ *
+ * Conventions: all matrices are column-major (as in Julia), complex matrices are
+ * interleaved `double complex` (as Julia's ComplexF64). Field values are 1-based
+ * (1..4) exactly as the reference's UInt8 arrays. Every function cites the
+ * reference file:line it restates.
+ *
+ * Parity status per function (checks live in tests/test_oracle_*.py):
+ *   geometry            pinned: K symmetric, coordination, sublattice structure,
+ *                       hand-checked 3x3 neighbour tables
+ *   jacobi eigensolver  pinned: numpy.linalg.eigh
+ *   householder QR, LU  pinned: numpy/scipy (QR span, LU solve residual, det)
+ *   philox4x32-10       pinned: Random123 known-answer vectors
+ *   model constants     pinned: closed forms (HS identity, eK*eKinv = I)
+ *   wrap / B blocks     pinned: brute-force unstabilised products (Gtau)
+ *   get_r / Gupdate     pinned: brute-force determinant ratio + recomputed G
+ *   get_G               pinned: projector identities, free-fermion closed form
+ *   phy_measure         pinned: hand-evaluated formulas on a fixed G (incl. the
+ *                       reference's discarded `+adjoint(...)` statements)
+ *   phy_update          pinned: composition of the above + G vs Gtau after sweeps
+ */
+#ifndef KDQ_ORACLE_H
+#define KDQ_ORACLE_H
+#include <complex.h>
+#include <stdint.h>
+
+typedef double complex zdouble;
+
+/* ---------------- random numbers (shared specification with the HIP library) ------ */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+/* one proposal draw: choice in {0,1,2} and uniform in [0,1) */
+void orc_draw(uint64_t seed, uint32_t chain, uint32_t stream, uint32_t sweep, uint32_t slice_tag,
+              uint32_t elem, int *choice, double *u);
+/* initial-field value in {1,2,3,4} */
+int orc_init_value(uint64_t seed, uint32_t chain, uint32_t stream, uint32_t slice, uint32_t elem);
+
+/* ---------------- geometry (src/public/Geometry.jl) ---------------------------------- */
+enum { ORC_HC120 = 0, ORC_HC60 = 1, ORC_SQ90 = 2, ORC_SQ45 = 3 };
+int orc_xy_i(const int site[2], int x, int y);                 /* Geometry.jl:8-13 */
+void orc_i_xy(const int site[2], int i, int *x, int *y);       /* Geometry.jl:18-21 */
+int orc_nn2idx(int lattice, const int site[2], int idx, int nn[4]); /* Geometry.jl:74-134 */
+void orc_nnK(int lattice, const int site[2], double *K);        /* Geometry.jl:178-207 */
+void orc_nnidx_F(int lattice, const int site[2], int *nnidx);  /* Geometry.jl:24-66 (HC) */
+int orc_area_index(int lattice, const int site[2], const int a0[2], const int a1[2], int *index);
+
+/* ---------------- dense linear algebra ---------------------------------------------- */
+void orc_zgemm(char ta, char tb, int m, int n, int k, zdouble alpha, const zdouble *A, int lda,
+               const zdouble *B, int ldb, zdouble beta, zdouble *C, int ldc);
+void orc_jacobi_eigh(int n, const double *A, double *w, double *V);
+void orc_zqr_q(int m, int n, zdouble *A, int lda, double *logabsr);   /* geqrf+orgqr */
+int orc_zgetrf(int n, zdouble *A, int lda, int *ipiv);                  /* getrf */
+void orc_zgetrs(int n, int nrhs, const zdouble *A, int lda, const int *ipiv, zdouble *B, int ldb);
+zdouble orc_zdet(int n, const zdouble *A, int lda);
+int orc_zinv(int n, zdouble *A, int lda);                               /* getrf+getri */
+void orc_get_G(int ns, int np, const zdouble *BL, int ldbl, const zdouble *BR, int ldbr,
+               zdouble *G, double *logabsdet);
+
+/* ---------------- t-U model (src/tU) ------------------------------------------------- */
+typedef struct orc_tu {
+    int ns, np, nt, bs, nn_nodes, lattice, site[2];
+    double ht, dt;
+    int *nodes;             /* [nn_nodes] */
+    double *alpha;          /* [nt] */
+    double eta[4], gamma[4];
+    zdouble *K, *eK, *eKinv, *HalfeK, *HalfeKinv, *Pt;   /* ns x ns, Pt ns x np */
+    uint8_t *s;             /* ns x nt, values 1..4 */
+    zdouble *G, *BLs, *BRs; /* G ns x ns; BLs np x ns x NN; BRs ns x np x NN */
+    zdouble *tmpNN, *tmpNN2, *BM, *tmpNn, *tmpnN, *tmpN;
+    int idx;                /* current node index (1-based, as the reference) */
+    uint64_t seed;
+    uint32_t chain, sweep;  /* RNG stream position */
+    long long n_accept, n_propose;
+} orc_tu;
+
+orc_tu *orc_tu_new(int ns, int nt, int bs, int nn_nodes, const int64_t *nodes, int lattice,
+                   const int site[2], double ht, double dt, const double *alpha,
+                   const double *eta, const double *gamma, const double *K, const double *eK,
+                   const double *eKinv, const double *HalfeK, const double *HalfeKinv,
+                   const double *Pt, uint64_t seed, uint32_t chain);
+void orc_tu_free(orc_tu *m);
+void orc_tu_initial_s(orc_tu *m);
+void orc_tu_BM_F(orc_tu *m, int idx, zdouble *BM);
+void orc_tu_BMinv_F(orc_tu *m, int idx, zdouble *BM);
+void orc_tu_WrapKV(orc_tu *m, zdouble *G, const zdouble *D_in, int forward, char LR);
+void orc_tu_update_layer(orc_tu *m, int lt, int dir);
+void orc_tu_measure(orc_tu *m, int lt, const zdouble *G, double out[6]);
+void orc_tu_init_chain(orc_tu *m);
+void orc_tu_phy_update(orc_tu *m, int sweeps, int record, double *rows);
+void orc_tu_Gtau(orc_tu *m, int tau, zdouble *out);
+
+#endif
