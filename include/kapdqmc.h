@@ -104,6 +104,26 @@ int kdq_measure(kdq_handle h, int32_t lt, double *out);
 /* Counters: out[0] = proposals, out[1] = accepted, per chain (nchains x 2). */
 int kdq_get_counters(kdq_handle h, double *out);
 
+/* ---- kernel timing (no counterpart in the reference; used by the benchmark) ----
+ * Launches of the kernel families whose bit is set in `mask` are bracketed by HIP events
+ * on the handle's stream; kdq_profile_read returns the summed device time (ms) and the
+ * launch count per family (arrays of KDQ_PROF_FAMILIES). kdq_profile resets both. */
+enum {
+    KDQ_PROF_WRAP = 0,       /* the two GEMMs of each sweep WrapKV! */
+    KDQ_PROF_CHAIN = 1,      /* per-slice propagation of U_R / U_L between nodes */
+    KDQ_PROF_GETG = 2,       /* GEMMs of get_G! */
+    KDQ_PROF_DECIDE = 3,     /* block Metropolis decisions */
+    KDQ_PROF_FLUSH = 4,      /* rank-k flush of accepted flips */
+    KDQ_PROF_QR = 5,         /* Householder QR */
+    KDQ_PROF_INV = 6,        /* overlap inversion + log|det| */
+    KDQ_PROF_MEAS_WRAP = 7,  /* wraps and HalfeK sandwich of phy_measure */
+    KDQ_PROF_MEASURE = 8,    /* observable reduction */
+    KDQ_PROF_GEMM_OTHER = 9,
+    KDQ_PROF_FAMILIES = 10
+};
+int kdq_profile(kdq_handle h, uint32_t mask);
+int kdq_profile_read(kdq_handle h, double *ms, int64_t *launches);
+
 #ifdef __cplusplus
 }
 #endif

@@ -19,7 +19,10 @@ PREC_FP64, PREC_MIXED = 0, 1
 # Every symbol include/kapdqmc.h declares (checked by tests/test_capi.py).
 EXPORTS = ("kdq_create", "kdq_destroy", "kdq_last_error", "kdq_num_chains", "kdq_set_fields",
            "kdq_get_fields", "kdq_initial_s", "kdq_phy_update", "kdq_get_G", "kdq_set_G", "kdq_wrap",
-           "kdq_update_layer", "kdq_init_chain", "kdq_measure", "kdq_get_counters")
+           "kdq_update_layer", "kdq_init_chain", "kdq_measure", "kdq_get_counters", "kdq_profile",
+           "kdq_profile_read")
+PROF_FAMILIES = ("wrap", "chain", "get_G", "decide", "flush", "qr", "inv", "meas_wrap", "measure",
+                 "gemm_other")
 
 
 class KdqError(RuntimeError):
@@ -72,6 +75,8 @@ def load(path: str = LIB_PATH):
     L.kdq_init_chain.argtypes = [vp]
     L.kdq_measure.argtypes = [vp, i32, dp]
     L.kdq_get_counters.argtypes = [vp, dp]
+    L.kdq_profile.argtypes = [vp, u32]
+    L.kdq_profile_read.argtypes = [vp, dp, C.POINTER(C.c_int64)]
     for name in EXPORTS:
         if name != "kdq_last_error":
             getattr(L, name).restype = C.c_int

@@ -64,6 +64,13 @@ struct kdq_handle_s {
     uint32_t sweep = 0;    // sweep counter of the next sweep
     std::string err;
     double *h_meas = nullptr;  // pinned host staging
+    // per-family HIP-event timing (kdq_profile): launches of enabled families are
+    // bracketed by events on the handle's stream and harvested in batches.
+    uint32_t prof_mask = 0;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
+    double prof_ms[KDQ_PROF_FAMILIES] = {};
+    int64_t prof_n[KDQ_PROF_FAMILIES] = {};
 
     ~kdq_handle_s() {
         if (stream) hipStreamSynchronize(stream);
@@ -71,6 +78,11 @@ struct kdq_handle_s {
             for (void *p : c.allocs) hipFree(p);
         for (void *p : dev_misc) hipFree(p);
         if (h_meas) hipHostFree(h_meas);
+        for (auto &p : ev_pending) {
+            hipEventDestroy(p.second.first);
+            hipEventDestroy(p.second.second);
+        }
+        for (hipEvent_t e : ev_pool) hipEventDestroy(e);
         if (stream) hipStreamDestroy(stream);
     }
 };
@@ -92,6 +104,52 @@ int fail(kdq_handle h, int code, const std::string &msg) {
     set_global_error(msg);
     return code;
 }
+
+hipEvent_t take_event(kdq_handle h) {
+    if (!h->ev_pool.empty()) {
+        hipEvent_t e = h->ev_pool.back();
+        h->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    hipEventCreate(&e);
+    return e;
+}
+
+// Moves finished event pairs into the accumulators (synchronises on the last one).
+void harvest(kdq_handle h) {
+    if (h->ev_pending.empty()) return;
+    hipEventSynchronize(h->ev_pending.back().second.second);
+    for (auto &p : h->ev_pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, p.second.first, p.second.second) == hipSuccess) {
+            h->prof_ms[p.first] += ms;
+            h->prof_n[p.first] += 1;
+        }
+        h->ev_pool.push_back(p.second.first);
+        h->ev_pool.push_back(p.second.second);
+    }
+    h->ev_pending.clear();
+}
+
+struct ProfScope {
+    kdq_handle h;
+    int fam;
+    hipEvent_t a = nullptr;
+    ProfScope(kdq_handle h_, int fam_) : h(h_), fam(fam_) {
+        if (h->prof_mask & (1u << fam)) {
+            a = take_event(h);
+            hipEventRecord(a, h->stream);
+        }
+    }
+    ~ProfScope() {
+        if (!a) return;
+        hipEvent_t b = take_event(h);
+        hipEventRecord(b, h->stream);
+        h->ev_pending.push_back({fam, {a, b}});
+        if (h->ev_pending.size() >= 4096) harvest(h);
+    }
+};
 
 template <class T>
 hipError_t dalloc(ChainHost &c, T **p, size_t count) {
@@ -128,7 +186,8 @@ GemmArgs gargs(kdq_handle h, const Opnd *A, const Opnd *B, const Opnd *C, int md
 
 int maxdim(kdq_handle h, int code) { return code == DIM_NS ? h->max_ns : h->max_np; }
 
-int gemm(kdq_handle h, bool ac, bool bc, int opa, int opb, const GemmArgs &g) {
+int gemm(kdq_handle h, bool ac, bool bc, int opa, int opb, const GemmArgs &g, int fam = KDQ_PROF_GEMM_OTHER) {
+    ProfScope ps(h, fam);
     HIPCHK(launch_gemm(h->stream, h->nch, maxdim(h, g.mdim), maxdim(h, g.ndim), ac, bc, opa, opb, g));
     return KDQ_OK;
 }
@@ -136,22 +195,22 @@ int gemm(kdq_handle h, bool ac, bool bc, int opa, int opb, const GemmArgs &g) {
 DiagSpec phase(int lt, int sign) { return DiagSpec{1, lt, sign, 0}; }
 
 // G <- D eK G eKinv D⁻¹ (forward) or eKinv D⁻¹ G D eK (backward), D = cis(α η_s) at slice lt.
-int wrap_B(kdq_handle h, Opnd *Gt, Opnd *Tt, int lt, bool forward) {
+int wrap_B(kdq_handle h, Opnd *Gt, Opnd *Tt, int lt, bool forward, int fam = KDQ_PROF_WRAP) {
     if (forward) {
         GemmArgs g1 = gargs(h, h->t_eK, Gt, Tt, DIM_NS, DIM_NS, DIM_NS);
-        if (int r = gemm(h, false, true, 0, 0, g1)) return r;
+        if (int r = gemm(h, false, true, 0, 0, g1, fam)) return r;
         GemmArgs g2 = gargs(h, Tt, h->t_eKinv, Gt, DIM_NS, DIM_NS, DIM_NS);
         g2.rowS = phase(lt, +1);
         g2.colS = phase(lt, -1);
-        return gemm(h, true, false, 0, 0, g2);
+        return gemm(h, true, false, 0, 0, g2, fam);
     }
     // D' = cis(-α η): T = eKinv · (D' G) · D'⁻¹, then G = T eK
     GemmArgs g1 = gargs(h, h->t_eKinv, Gt, Tt, DIM_NS, DIM_NS, DIM_NS);
     g1.inS = phase(lt, -1);
     g1.colS = phase(lt, +1);
-    if (int r = gemm(h, false, true, 0, 0, g1)) return r;
+    if (int r = gemm(h, false, true, 0, 0, g1, fam)) return r;
     GemmArgs g2 = gargs(h, Tt, h->t_eK, Gt, DIM_NS, DIM_NS, DIM_NS);
-    return gemm(h, true, false, 0, 0, g2);
+    return gemm(h, true, false, 0, 0, g2, fam);
 }
 
 // dst = QR_Q(B_block(idx) · src) for U_R: slices nodes[idx-1]+1 .. nodes[idx] (1-based idx)
@@ -162,11 +221,12 @@ int propagate_R(kdq_handle h, Opnd *src, Opnd *dst, int idx) {
         Opnd *out = (lt == hi) ? dst : ((cur == h->t_P0) ? h->t_P1 : h->t_P0);
         GemmArgs g = gargs(h, h->t_eK, cur, out, DIM_NS, DIM_NP, DIM_NS);
         g.rowS = phase(lt, +1);
-        if (int r = gemm(h, false, true, 0, 0, g)) return r;
+        if (int r = gemm(h, false, true, 0, 0, g, KDQ_PROF_CHAIN)) return r;
         cur = out;
     }
     if (lo > hi) HIPCHK(launch_copy_mat(h->stream, h->nch, h->max_ns, h->max_np, h->d_chains, src, dst, DIM_NS, DIM_NP, true));
     QRArgs q{h->d_chains, dst, DIM_NS, DIM_NP};
+    ProfScope ps(h, KDQ_PROF_QR);
     HIPCHK(launch_qr(h->stream, h->nch, q));
     return KDQ_OK;
 }
@@ -180,11 +240,12 @@ int propagate_L(kdq_handle h, Opnd *src, Opnd *dst, int idx) {
         Opnd *out = (lt == lo) ? dst : ((cur == h->t_P0) ? h->t_P1 : h->t_P0);
         GemmArgs g = gargs(h, h->t_eK, cur, out, DIM_NS, DIM_NP, DIM_NS);
         g.inS = phase(lt, -1);
-        if (int r = gemm(h, false, true, 1, 0, g)) return r;
+        if (int r = gemm(h, false, true, 1, 0, g, KDQ_PROF_CHAIN)) return r;
         cur = out;
     }
     if (lo > hi) HIPCHK(launch_copy_mat(h->stream, h->nch, h->max_ns, h->max_np, h->d_chains, src, dst, DIM_NS, DIM_NP, true));
     QRArgs q{h->d_chains, dst, DIM_NS, DIM_NP};
+    ProfScope ps(h, KDQ_PROF_QR);
     HIPCHK(launch_qr(h->stream, h->nch, q));
     return KDQ_OK;
 }
@@ -193,15 +254,18 @@ int propagate_L(kdq_handle h, Opnd *src, Opnd *dst, int idx) {
 int get_G(kdq_handle h, int idx, int *bad = nullptr) {
     Opnd *ULH = h->t_ULH[idx - 1], *UR = h->t_UR[idx - 1];
     GemmArgs g1 = gargs(h, ULH, UR, h->t_M, DIM_NP, DIM_NP, DIM_NS);
-    if (int r = gemm(h, true, true, 2, 0, g1)) return r;
+    if (int r = gemm(h, true, true, 2, 0, g1, KDQ_PROF_GETG)) return r;
     InvArgs iv{h->d_chains, h->t_M, DIM_NP, 0, h->d_logdet, h->d_info};
-    HIPCHK(launch_inv(h->stream, h->nch, iv));
+    {
+        ProfScope ps(h, KDQ_PROF_INV);
+        HIPCHK(launch_inv(h->stream, h->nch, iv));
+    }
     GemmArgs g2 = gargs(h, h->t_M, ULH, h->t_X, DIM_NP, DIM_NS, DIM_NP);
-    if (int r = gemm(h, true, true, 0, 2, g2)) return r;
+    if (int r = gemm(h, true, true, 0, 2, g2, KDQ_PROF_GETG)) return r;
     GemmArgs g3 = gargs(h, UR, h->t_X, h->t_G[h->cur], DIM_NS, DIM_NS, DIM_NP);
     g3.mode = 1;
     (void)bad;
-    return gemm(h, true, true, 0, 0, g3);
+    return gemm(h, true, true, 0, 0, g3, KDQ_PROF_GETG);
 }
 
 int init_chain(kdq_handle h) {
@@ -221,9 +285,15 @@ int update_layer(kdq_handle h, int lt, int dir, uint32_t sweep) {
         a.chains = h->d_chains;
         a.G = h->t_G[h->cur];
         a.lt = lt; a.dir = dir; a.s0 = s0; a.kb = kb; a.sweep = sweep;
-        HIPCHK(launch_decide_tu(h->stream, h->nch, a));
+        {
+            ProfScope ps(h, KDQ_PROF_DECIDE);
+            HIPCHK(launch_decide_tu(h->stream, h->nch, a));
+        }
         FlushArgs f{h->d_chains, h->t_G[h->cur], h->t_G[1 - h->cur]};
-        HIPCHK(launch_flush(h->stream, h->nch, h->max_ns, f));
+        {
+            ProfScope ps(h, KDQ_PROF_FLUSH);
+            HIPCHK(launch_flush(h->stream, h->nch, h->max_ns, f));
+        }
         h->cur ^= 1;
     }
     return KDQ_OK;
@@ -235,16 +305,17 @@ int measure(kdq_handle h, int lt) {
     const int nt = h->nt;
     if (lt > nt / 2.0) {
         for (int t = lt; t >= nt / 2 + 1; --t)
-            if (int r = wrap_B(h, h->t_Gm, h->t_Tm, t, false)) return r;
+            if (int r = wrap_B(h, h->t_Gm, h->t_Tm, t, false, KDQ_PROF_MEAS_WRAP)) return r;
     } else {
         for (int t = lt + 1; t <= nt / 2; ++t)
-            if (int r = wrap_B(h, h->t_Gm, h->t_Tm, t, true)) return r;
+            if (int r = wrap_B(h, h->t_Gm, h->t_Tm, t, true, KDQ_PROF_MEAS_WRAP)) return r;
     }
     GemmArgs g1 = gargs(h, h->t_hK, h->t_Gm, h->t_Tm, DIM_NS, DIM_NS, DIM_NS);
-    if (int r = gemm(h, false, true, 0, 0, g1)) return r;
+    if (int r = gemm(h, false, true, 0, 0, g1, KDQ_PROF_MEAS_WRAP)) return r;
     GemmArgs g2 = gargs(h, h->t_Tm, h->t_hKinv, h->t_Gm, DIM_NS, DIM_NS, DIM_NS);
-    if (int r = gemm(h, true, false, 0, 0, g2)) return r;
+    if (int r = gemm(h, true, false, 0, 0, g2, KDQ_PROF_MEAS_WRAP)) return r;
     MeasArgs m{h->d_chains, h->t_Gm};
+    ProfScope ps(h, KDQ_PROF_MEASURE);
     HIPCHK(launch_measure_tu(h->stream, h->nch, m));
     return KDQ_OK;
 }
@@ -580,6 +651,28 @@ int kdq_measure(kdq_handle h, int32_t lt, double *out) {
     HIPCHK(launch_zero_meas(h->stream, h->nch, h->d_chains));
     if (int r = measure(h, lt)) return r;
     return read_meas(h, out);
+}
+
+int kdq_profile(kdq_handle h, uint32_t mask) {
+    if (!h) return fail(h, KDQ_EARG, "kdq_profile: null handle");
+    harvest(h);
+    h->prof_mask = mask;
+    for (int f = 0; f < KDQ_PROF_FAMILIES; ++f) {
+        h->prof_ms[f] = 0.0;
+        h->prof_n[f] = 0;
+    }
+    return KDQ_OK;
+}
+
+int kdq_profile_read(kdq_handle h, double *ms, int64_t *launches) {
+    if (!h || !ms || !launches) return fail(h, KDQ_EARG, "kdq_profile_read: bad argument");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    harvest(h);
+    for (int f = 0; f < KDQ_PROF_FAMILIES; ++f) {
+        ms[f] = h->prof_ms[f];
+        launches[f] = h->prof_n[f];
+    }
+    return KDQ_OK;
 }
 
 int kdq_get_counters(kdq_handle h, double *out) {

@@ -130,6 +130,20 @@ class DQMCBatch:
         self._chk(self.L.kdq_measure(self.h, lt, _dp(out)))
         return out
 
+    def profile(self, families=()):
+        """Enable HIP-event timing of the named kernel families (see _lib.PROF_FAMILIES)."""
+        mask = 0
+        for f in families:
+            mask |= 1 << L.PROF_FAMILIES.index(f)
+        self._chk(self.L.kdq_profile(self.h, mask))
+
+    def profile_read(self) -> dict:
+        """{family: (total device ms, launches)} since the last profile() call."""
+        ms = np.zeros(len(L.PROF_FAMILIES))
+        n = np.zeros(len(L.PROF_FAMILIES), dtype=np.int64)
+        self._chk(self.L.kdq_profile_read(self.h, _dp(ms), n.ctypes.data_as(C.POINTER(C.c_int64))))
+        return {f: (float(ms[i]), int(n[i])) for i, f in enumerate(L.PROF_FAMILIES)}
+
     def counters(self) -> np.ndarray:
         out = np.zeros((self.n, 2))
         self._chk(self.L.kdq_get_counters(self.h, _dp(out)))

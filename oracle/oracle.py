@@ -74,6 +74,29 @@ def _declare(L):
     L.orc_tu_update_layer.argtypes = [C.c_void_p, C.c_int, C.c_int]
     L.orc_tu_measure.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
     L.orc_tu_BM_F.argtypes = [C.c_void_p, C.c_int, _dp]
+    L.orc_tu_forward_slices.argtypes = [C.c_void_p, C.c_int]
+    L.orc_tv_new.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int64), C.c_int, _ip,
+                             C.c_double, _dp, _dp, _dp, _ip, _dp, _dp, _dp, _dp, _dp, _dp,
+                             C.c_uint64, C.c_uint32]
+    L.orc_tv_new.restype = C.c_void_p
+    L.orc_tv_free.argtypes = [C.c_void_p]
+    L.orc_tv_initial_s.argtypes = [C.c_void_p]
+    L.orc_tv_init_chain.argtypes = [C.c_void_p]
+    L.orc_tv_phy_update.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp]
+    L.orc_tv_update_family.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
+    L.orc_tv_measure.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+    L.orc_tv_Gtau.argtypes = [C.c_void_p, C.c_int, _dp]
+    L.orc_tv_BM_F.argtypes = [C.c_void_p, C.c_int, _dp]
+    L.orc_tuv_new.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int64), C.c_int, _ip,
+                              C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, _ip,
+                              _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, C.c_uint64, C.c_uint32]
+    L.orc_tuv_new.restype = C.c_void_p
+    L.orc_tuv_free.argtypes = [C.c_void_p]
+    L.orc_tuv_initial_s.argtypes = [C.c_void_p]
+    L.orc_tuv_init_chain.argtypes = [C.c_void_p]
+    L.orc_tuv_phy_update.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp]
+    L.orc_tuv_update_layer.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    L.orc_tuv_measure.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
 
 
 def _p(a, t=C.c_double):
@@ -251,6 +274,9 @@ class OracleTU:
         lib().orc_tu_phy_update(self.h, sweeps, int(record), _p(rows))
         return rows
 
+    def forward_slices(self, nslices: int):
+        lib().orc_tu_forward_slices(self.h, nslices)
+
     def Gtau(self, tau: int):
         out = np.zeros((self.ns, self.ns), dtype=np.complex128, order="F")
         lib().orc_tu_Gtau(self.h, tau, _p(out))
@@ -274,4 +300,193 @@ class OracleTU:
     def BM(self, idx: int):
         out = np.zeros((self.ns, self.ns), dtype=np.complex128, order="F")
         lib().orc_tu_BM_F(self.h, idx, _p(out))
+        return out
+
+
+# -------------------------------------------------------------------------- t-V
+class OracleTV:
+    """Oracle state for one spinless t-V chain (src/tV)."""
+
+    class _S(C.Structure):
+        _fields_ = [("ns", C.c_int), ("np", C.c_int), ("nt", C.c_int), ("bs", C.c_int),
+                    ("nn_nodes", C.c_int), ("lattice", C.c_int), ("site", C.c_int * 2), ("nb", C.c_int),
+                    ("ht", C.c_double), ("nodes", C.POINTER(C.c_int)), ("alpha", _dp),
+                    ("eta", C.c_double * 4), ("gamma", C.c_double * 4), ("bx", _ip), ("by", _ip),
+                    ("K", _dp), ("eK", _dp), ("eKinv", _dp), ("HalfeK", _dp), ("HalfeKinv", _dp),
+                    ("Pt", _dp), ("UV", _dp), ("s", C.POINTER(C.c_uint8)), ("G", _dp), ("BLs", _dp),
+                    ("BRs", _dp), ("T", _dp), ("T2", _dp), ("BM", _dp), ("tNn", _dp), ("tnN", _dp),
+                    ("tN", _dp), ("idx", C.c_int), ("seed", C.c_uint64), ("chain", C.c_uint32),
+                    ("sweep", C.c_uint32), ("n_accept", C.c_longlong), ("n_propose", C.c_longlong),
+                    ("n_negative", C.c_longlong)]
+
+    def __init__(self, model, seed: int = 0, chain: int = 0):
+        self.model = model
+        f = lambda a: _fortran(a, np.float64)
+        bonds = np.ascontiguousarray(np.transpose(model.nnidx, (1, 0, 2)).reshape(-1, 2), dtype=np.int32)
+        self._keep = [f(model.α), f(model.η), f(model.γ), bonds, f(model.K), f(model.eK), f(model.eKinv),
+                      f(model.HalfeK), f(model.HalfeKinv), f(model.Pt),
+                      np.ascontiguousarray(model.nodes, dtype=np.int64)]
+        a, eta, gam, bnd, K, eK, eKi, hK, hKi, Pt, nodes = self._keep
+        self.h = lib().orc_tv_new(model.Ns, model.Nt, model.BatchSize, len(nodes),
+                                  nodes.ctypes.data_as(C.POINTER(C.c_int64)), LATTICE_CODE[model.Lattice],
+                                  (C.c_int * 2)(*model.site), model.Ht, _p(a), _p(eta), _p(gam),
+                                  _p(bnd, C.c_int), _p(K), _p(eK), _p(eKi), _p(hK), _p(hKi), _p(Pt), seed, chain)
+        self.ns, self.nt, self.nb = model.Ns, model.Nt, model.Ns // 2
+
+    def __del__(self):
+        try:
+            lib().orc_tv_free(self.h)
+        except Exception:
+            pass
+
+    @property
+    def _st(self):
+        return C.cast(self.h, C.POINTER(self._S)).contents
+
+    def initial_s(self):
+        lib().orc_tv_initial_s(self.h)
+
+    @property
+    def s(self):
+        """UInt8 [Ns/2, 3, Nt] as the reference's tV field array."""
+        raw = np.ctypeslib.as_array(self._st.s, shape=(self.nb * 3 * self.nt,))
+        return raw.reshape(self.nt, 3, self.nb).transpose(2, 1, 0).copy()
+
+    @s.setter
+    def s(self, value):
+        raw = np.ctypeslib.as_array(self._st.s, shape=(self.nb * 3 * self.nt,))
+        raw[:] = np.asarray(value, dtype=np.uint8).transpose(2, 1, 0).reshape(-1)
+
+    @property
+    def G(self):
+        raw = np.ctypeslib.as_array(self._st.G, shape=(self.ns * self.ns,))
+        return raw.reshape(self.ns, self.ns).T.copy()
+
+    @G.setter
+    def G(self, value):
+        raw = np.ctypeslib.as_array(self._st.G, shape=(self.ns * self.ns,))
+        raw[:] = np.asarray(value, dtype=np.float64).T.reshape(-1)
+
+    @property
+    def idx(self):
+        return self._st.idx
+
+    @property
+    def acceptance(self):
+        st = self._st
+        return st.n_accept / max(st.n_propose, 1)
+
+    def init_chain(self):
+        lib().orc_tv_init_chain(self.h)
+
+    def phy_update(self, sweeps: int, record: bool = True):
+        rows = np.zeros((sweeps, 10))
+        lib().orc_tv_phy_update(self.h, sweeps, int(record), _p(rows))
+        return rows
+
+    def update_family(self, lt: int, direction: int, j: int):
+        lib().orc_tv_update_family(self.h, lt, direction, j)
+
+    def measure(self, lt: int, G):
+        Gf = _fortran(G, np.float64)
+        out = np.zeros(10)
+        lib().orc_tv_measure(self.h, lt, _p(Gf), _p(out))
+        return out
+
+    def Gtau(self, tau: int):
+        out = np.zeros((self.ns, self.ns), order="F")
+        lib().orc_tv_Gtau(self.h, tau, _p(out))
+        return out
+
+
+# ------------------------------------------------------------------------- t-U-V
+class OracleTUV:
+    """Oracle state for one t-U-V chain (src/tUV)."""
+
+    class _S(C.Structure):
+        _fields_ = [("ns", C.c_int), ("np", C.c_int), ("nt", C.c_int), ("bs", C.c_int),
+                    ("nn_nodes", C.c_int), ("lattice", C.c_int), ("site", C.c_int * 2), ("nbd", C.c_int),
+                    ("ht", C.c_double), ("U", C.c_double), ("V", C.c_double), ("a", C.c_double),
+                    ("is_complex", C.c_int), ("nodes", C.POINTER(C.c_int)), ("eta", C.c_double * 4),
+                    ("gamma", C.c_double * 4), ("bx", _ip), ("by", _ip), ("K", _dp), ("eK", _dp),
+                    ("eKinv", _dp), ("HalfeK", _dp), ("HalfeKinv", _dp), ("Pt", _dp),
+                    ("s", C.POINTER(C.c_uint8)), ("G", _dp), ("BLs", _dp), ("BRs", _dp), ("T", _dp),
+                    ("T2", _dp), ("BM", _dp), ("tNn", _dp), ("tnN", _dp), ("tN", _dp), ("idx", C.c_int),
+                    ("seed", C.c_uint64), ("chain", C.c_uint32), ("sweep", C.c_uint32),
+                    ("n_accept", C.c_longlong), ("n_propose", C.c_longlong)]
+
+    def __init__(self, model, seed: int = 0, chain: int = 0):
+        self.model = model
+        f = lambda a: _fortran(a, np.float64)
+        z = model.nnidx.shape[1]
+        # linear (column-major) order of nnidx[Ns, z]
+        bonds = np.ascontiguousarray(np.transpose(model.nnidx, (1, 0, 2)).reshape(-1, 2), dtype=np.int32)
+        self._keep = [bonds, f(model.η), f(model.γ), f(model.K), f(model.eK), f(model.eKinv),
+                      f(model.HalfeK), f(model.HalfeKinv), f(model.Pt),
+                      np.ascontiguousarray(model.nodes, dtype=np.int64)]
+        bnd, eta, gam, K, eK, eKi, hK, hKi, Pt, nodes = self._keep
+        self.h = lib().orc_tuv_new(model.Ns, model.Nt, model.BatchSize, len(nodes),
+                                   nodes.ctypes.data_as(C.POINTER(C.c_int64)), LATTICE_CODE[model.Lattice],
+                                   (C.c_int * 2)(*model.site), model.Ht, model.U, model.V, model.a,
+                                   int(model.Type is complex), z, _p(bnd, C.c_int), _p(eta), _p(gam), _p(K),
+                                   _p(eK), _p(eKi), _p(hK), _p(hKi), _p(Pt), seed, chain)
+        self.ns, self.nt, self.nbd = model.Ns, model.Nt, model.Ns * z
+
+    def __del__(self):
+        try:
+            lib().orc_tuv_free(self.h)
+        except Exception:
+            pass
+
+    @property
+    def _st(self):
+        return C.cast(self.h, C.POINTER(self._S)).contents
+
+    def initial_s(self):
+        lib().orc_tuv_initial_s(self.h)
+
+    @property
+    def s(self):
+        raw = np.ctypeslib.as_array(self._st.s, shape=(self.nbd * self.nt,))
+        return raw.reshape(self.nt, self.nbd).T.copy()
+
+    @s.setter
+    def s(self, value):
+        raw = np.ctypeslib.as_array(self._st.s, shape=(self.nbd * self.nt,))
+        raw[:] = np.asarray(value, dtype=np.uint8).T.reshape(-1)
+
+    @property
+    def G(self):
+        raw = np.ctypeslib.as_array(self._st.G, shape=(2 * self.ns * self.ns,))
+        return raw.view(np.complex128).reshape(self.ns, self.ns).T.copy()
+
+    @G.setter
+    def G(self, value):
+        raw = np.ctypeslib.as_array(self._st.G, shape=(2 * self.ns * self.ns,)).view(np.complex128)
+        raw[:] = np.asarray(value, dtype=np.complex128).T.reshape(-1)
+
+    @property
+    def idx(self):
+        return self._st.idx
+
+    @property
+    def acceptance(self):
+        st = self._st
+        return st.n_accept / max(st.n_propose, 1)
+
+    def init_chain(self):
+        lib().orc_tuv_init_chain(self.h)
+
+    def phy_update(self, sweeps: int, record: bool = True):
+        rows = np.zeros((sweeps, 7))
+        lib().orc_tuv_phy_update(self.h, sweeps, int(record), _p(rows))
+        return rows
+
+    def update_layer(self, lt: int, direction: int):
+        lib().orc_tuv_update_layer(self.h, lt, direction)
+
+    def measure(self, lt: int, G):
+        Gf = _fortran(G, np.complex128)
+        out = np.zeros(7)
+        lib().orc_tuv_measure(self.h, lt, _p(Gf), _p(out))
         return out

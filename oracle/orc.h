@@ -92,5 +92,67 @@ void orc_tu_measure(orc_tu *m, int lt, const zdouble *G, double out[6]);
 void orc_tu_init_chain(orc_tu *m);
 void orc_tu_phy_update(orc_tu *m, int sweeps, int record, double *rows);
 void orc_tu_Gtau(orc_tu *m, int tau, zdouble *out);
+void orc_tu_forward_slices(orc_tu *m, int nslices);
+
+/* ---------------- spinless t-V model (src/tV) ---------------------------------------- */
+typedef struct orc_tv {
+    int ns, np, nt, bs, nn_nodes, lattice, site[2], nb;  /* nb = ns/2 bonds per family */
+    double ht;
+    int *nodes;
+    double *alpha;
+    double eta[4], gamma[4];
+    int *bx, *by;           /* [nb*3] 0-based sites of bond (i, family j) at i + j*nb */
+    double *K, *eK, *eKinv, *HalfeK, *HalfeKinv, *Pt, *UV;  /* UV: ns x ns x 3 */
+    uint8_t *s;             /* [nb, 3, nt] column-major */
+    double *G, *BLs, *BRs, *T, *T2, *BM, *tNn, *tnN, *tN;
+    int idx;
+    uint64_t seed;
+    uint32_t chain, sweep;
+    long long n_accept, n_propose, n_negative;
+} orc_tv;
+
+orc_tv *orc_tv_new(int ns, int nt, int bs, int nn_nodes, const int64_t *nodes, int lattice,
+                   const int site[2], double ht, const double *alpha, const double *eta,
+                   const double *gamma, const int *bonds, const double *K, const double *eK,
+                   const double *eKinv, const double *HalfeK, const double *HalfeKinv,
+                   const double *Pt, uint64_t seed, uint32_t chain);
+void orc_tv_free(orc_tv *m);
+void orc_tv_initial_s(orc_tv *m);
+void orc_tv_BM_F(orc_tv *m, int idx, double *BM);
+void orc_tv_update_family(orc_tv *m, int lt, int dir, int j);
+void orc_tv_measure(orc_tv *m, int lt, const double *G, double out[10]);
+void orc_tv_init_chain(orc_tv *m);
+void orc_tv_phy_update(orc_tv *m, int sweeps, int record, double *rows);
+void orc_tv_Gtau(orc_tv *m, int tau, double *out);
+
+/* ---------------- t-U-V model (src/tUV) ---------------------------------------------- */
+typedef struct orc_tuv {
+    int ns, np, nt, bs, nn_nodes, lattice, site[2], nbd;   /* nbd = ns*z directed bonds */
+    double ht, U, V, a;
+    int is_complex;
+    int *nodes;
+    double eta[4], gamma[4];
+    int *bx, *by;           /* [nbd] 0-based (x, y) of directed bond i (linear index of nnidx) */
+    zdouble *K, *eK, *eKinv, *HalfeK, *HalfeKinv, *Pt;
+    uint8_t *s;             /* [nbd, nt] */
+    zdouble *G, *BLs, *BRs, *T, *T2, *BM, *tNn, *tnN, *tN;
+    int idx;
+    uint64_t seed;
+    uint32_t chain, sweep;
+    long long n_accept, n_propose;
+} orc_tuv;
+
+orc_tuv *orc_tuv_new(int ns, int nt, int bs, int nn_nodes, const int64_t *nodes, int lattice,
+                     const int site[2], double ht, double U, double V, double a, int is_complex,
+                     int z, const int *bonds, const double *eta, const double *gamma, const double *K,
+                     const double *eK, const double *eKinv, const double *HalfeK,
+                     const double *HalfeKinv, const double *Pt, uint64_t seed, uint32_t chain);
+void orc_tuv_free(orc_tuv *m);
+void orc_tuv_initial_s(orc_tuv *m);
+void orc_tuv_BM_F(orc_tuv *m, int idx, zdouble *BM);
+void orc_tuv_update_layer(orc_tuv *m, int lt, int dir);
+void orc_tuv_measure(orc_tuv *m, int lt, const zdouble *G, double out[7]);
+void orc_tuv_init_chain(orc_tuv *m);
+void orc_tuv_phy_update(orc_tuv *m, int sweeps, int record, double *rows);
 
 #endif

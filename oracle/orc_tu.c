@@ -340,6 +340,30 @@ void orc_tu_phy_update(orc_tu *m, int sweeps, int record, double *rows) {
     free(D);
 }
 
+/* Bounded sample of phy_update for the CPU baseline: chain initialisation, then the
+ * forward pass over slices 1..nslices exactly as in the sweep (wrap, local update and
+ * the stabilisation at every node), without measurements. */
+void orc_tu_forward_slices(orc_tu *m, int nslices) {
+    int n = m->ns, np = m->np;
+    zdouble *D = zalloc(n);
+    orc_tu_init_chain(m);
+    for (int lt = 1; lt <= nslices && lt <= m->nt; ++lt) {
+        for (int i = 0; i < n; ++i) D[i] = cis(m->alpha[lt - 1] * m->eta[m->s[IX(i, lt - 1, n)] - 1]);
+        orc_tu_WrapKV(m, m->G, D, 1, 'B');
+        orc_tu_update_layer(m, lt, 0);
+        if (is_node(m, lt)) {
+            m->idx += 1;
+            int idx = m->idx;
+            orc_tu_BM_F(m, idx - 1, m->BM);
+            orc_zgemm('N', 'N', n, np, n, 1.0, m->BM, n, BR(idx - 1), n, 0.0, m->tmpNn, n);
+            orc_zqr_q(n, np, m->tmpNn, n, NULL);
+            memcpy(BR(idx), m->tmpNn, sizeof(zdouble) * (size_t)n * np);
+            orc_get_G(n, np, BL(idx), np, BR(idx), n, m->G, NULL);
+        }
+    }
+    free(D);
+}
+
 /* Gτ (GreenMatrix.jl:91-123): equal-time G at slice tau from products with a QR every
  * BatchSize slices — the reference's own debug cross-check. */
 void orc_tu_Gtau(orc_tu *m, int tau, zdouble *out) {
