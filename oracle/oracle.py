@@ -97,6 +97,17 @@ def _declare(L):
     L.orc_tuv_phy_update.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp]
     L.orc_tuv_update_layer.argtypes = [C.c_void_p, C.c_int, C.c_int]
     L.orc_tuv_measure.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+    L.orc_scee_new.argtypes = [C.c_void_p, C.c_int, _ip, C.c_int, _ip, C.c_double, C.c_int,
+                               C.POINTER(C.c_uint8), C.POINTER(C.c_uint8)]
+    L.orc_scee_new.restype = C.c_void_p
+    L.orc_scee_free.argtypes = [C.c_void_p]
+    L.orc_scee_init.argtypes = [C.c_void_p]
+    L.orc_scee_sweeps.argtypes = [C.c_void_p, C.c_int, _dp]
+    L.orc_scee_forward_slice.argtypes = [C.c_void_p, C.c_int, C.c_uint32]
+    L.orc_scee_get_fields.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_uint8)]
+    L.orc_scee_get_G4.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp]
+    L.orc_scee_detg.argtypes = [C.c_void_p, C.c_int]
+    L.orc_scee_detg.restype = C.c_double
 
 
 def _p(a, t=C.c_double):
@@ -490,3 +501,48 @@ class OracleTUV:
         out = np.zeros(7)
         lib().orc_tuv_measure(self.h, lt, _p(Gf), _p(out))
         return out
+
+
+# ---------------------------------------------------------- t-U swap-replica Rényi-2
+class OracleSCEE:
+    """ctrl_SCEEicr of the t-U model (src/tU/SCEE.jl) on two replica fields.
+    indexA / indexB are 1-based site lists (as area_index returns)."""
+
+    def __init__(self, model, indexA, indexB, lam: float, nlam: int, s1, s2, seed: int = 0, chain: int = 0):
+        self.tu = OracleTU(model, seed=seed, chain=chain)
+        self.ns, self.nt = model.Ns, model.Nt
+        self._keep = [np.ascontiguousarray(indexA, dtype=np.int32), np.ascontiguousarray(indexB, dtype=np.int32),
+                      np.asfortranarray(s1, dtype=np.uint8), np.asfortranarray(s2, dtype=np.uint8)]
+        iA, iB, a1, a2 = self._keep
+        self.h = lib().orc_scee_new(self.tu.h, len(iA), _p(iA, C.c_int), len(iB), _p(iB, C.c_int), lam, nlam,
+                                    _p(a1, C.c_uint8), _p(a2, C.c_uint8))
+
+    def __del__(self):
+        try:
+            lib().orc_scee_free(self.h)
+        except Exception:
+            pass
+
+    def init(self):
+        lib().orc_scee_init(self.h)
+
+    def sweeps(self, n: int):
+        O = np.zeros(n)
+        lib().orc_scee_sweeps(self.h, n, _p(O))
+        return O
+
+    def forward_slice(self, lt: int, sweep: int = 0):
+        lib().orc_scee_forward_slice(self.h, lt, sweep)
+
+    def fields(self, r: int):
+        out = np.zeros((self.ns, self.nt), dtype=np.uint8, order="F")
+        lib().orc_scee_get_fields(self.h, r, _p(out, C.c_uint8))
+        return out
+
+    def G4(self, r: int):
+        mats = [np.zeros((self.ns, self.ns), dtype=np.complex128, order="F") for _ in range(4)]
+        lib().orc_scee_get_G4(self.h, r, *[_p(x) for x in mats])
+        return mats   # Gt, G0, Gt0, G0t
+
+    def detg(self, which: int):
+        return lib().orc_scee_detg(self.h, which)

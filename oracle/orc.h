@@ -155,4 +155,16 @@ void orc_tuv_measure(orc_tuv *m, int lt, const zdouble *G, double out[7]);
 void orc_tuv_init_chain(orc_tuv *m);
 void orc_tuv_phy_update(orc_tuv *m, int sweeps, int record, double *rows);
 
+/* ---------------- t-U swap-replica Rényi-2 (src/tU/SCEE.jl) --------------------------- */
+typedef struct orc_scee orc_scee;
+orc_scee *orc_scee_new(orc_tu *m, int nA, const int *indexA, int nB, const int *indexB, double lambda,
+                       int nlam, const uint8_t *s1, const uint8_t *s2);
+void orc_scee_free(orc_scee *e);
+void orc_scee_init(orc_scee *e);
+void orc_scee_sweeps(orc_scee *e, int sweeps, double *O);
+void orc_scee_forward_slice(orc_scee *e, int lt, uint32_t sweep);
+void orc_scee_get_fields(orc_scee *e, int r, uint8_t *s);
+void orc_scee_get_G4(orc_scee *e, int r, zdouble *Gt, zdouble *G0, zdouble *Gt0, zdouble *G0t);
+double orc_scee_detg(orc_scee *e, int which);
+
 #endif
