@@ -28,7 +28,8 @@ enum { KDQ_MODEL_TU = 0, KDQ_MODEL_TV = 1, KDQ_MODEL_TUV = 2 };
 enum { KDQ_LATTICE_HC120 = 0, KDQ_LATTICE_HC60 = 1, KDQ_LATTICE_SQ90 = 2, KDQ_LATTICE_SQ45 = 3 };
 enum { KDQ_PREC_FP64 = 0, KDQ_PREC_MIXED = 1 };
 
-/* Model constants of one chain: the fields of tU_Hubbard_Para_ (src/tU/model.jl:14-37)
+/* Model constants of one chain: the fields of tU_Hubbard_Para_ (src/tU/model.jl:14-37),
+ * tV_Hubbard_Para_ (src/tV/model.jl:3-28) or tUV_Hubbard_Para_ (src/tUV/model.jl:3-27)
  * that the sweep reads. All matrices are column-major, ns x ns (Pt: ns x ns/2). */
 typedef struct kdq_model_desc {
     int32_t kind;            /* KDQ_MODEL_* */
@@ -48,6 +49,13 @@ typedef struct kdq_model_desc {
     const double *halfeK;    /* model.HalfeK */
     const double *halfeKinv; /* model.HalfeKinv */
     const double *Pt;        /* model.Pt (real) */
+    /* t-V and t-U-V only (ignored for t-U) */
+    int32_t z;               /* coordination number (3 on the honeycomb lattice) */
+    const int32_t *bonds;    /* model.nnidx as 1-based (x, y) pairs in column-major order:
+                                t-V: ns/2 x z bonds (family j = column), t-U-V: ns x z directed bonds */
+    double a;                /* t-U-V: model.a */
+    int32_t is_complex;      /* t-U-V: model.Type == ComplexF64 */
+    double U, V;             /* t-U-V: model.U, model.V (observable prefactors) */
 } kdq_model_desc;
 
 /* Per-chain run settings. The random stream is Philox4x32-10 keyed by `seed`, with the
@@ -58,13 +66,14 @@ typedef struct kdq_run_desc {
     uint32_t chain;          /* stream id (< 2^24) */
     uint32_t sweep0;         /* sweep counter of the first sweep */
     int32_t precision;       /* KDQ_PREC_* */
-    int32_t delay;           /* local-update block size k (1..64); 1 = Sherman-Morrison */
+    int32_t delay;           /* local-update block size k: sites (t-U, 1..64) or field elements
+                                (t-V / t-U-V, 1..32); 1 = one Sherman-Morrison update per flip */
 } kdq_run_desc;
 
 typedef struct kdq_handle_s *kdq_handle;
 
 /* Creates a batch of `nchains` chains on the current HIP device. All chains of a batch
- * must share nt and nodes; ns, couplings and seeds may differ.
+ * must share the model kind, nt and nodes; ns, couplings and seeds may differ.
  * Replaces: the buffer set-up at the top of phy_update (src/tU/phy_update.jl:7-37). */
 int kdq_create(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t nchains,
                kdq_handle *out);
@@ -72,33 +81,42 @@ int kdq_destroy(kdq_handle h);
 const char *kdq_last_error(kdq_handle h);   /* h may be NULL: last global error */
 int kdq_num_chains(kdq_handle h);
 
-/* Field arrays s[ns, nt] (UInt8, values 1..4, column-major) of chain `c`.
+/* Number of observables per measurement row: 6 (t-U), 10 (t-V), 7 (t-U-V). */
+int kdq_num_obs(kdq_handle h);
+
+/* Field arrays (UInt8, values 1..4, column-major) of chain `c`: t-U s[ns, nt],
+ * t-V s[ns/2, 3, nt], t-U-V s[ns*z, nt].
  * set/get correspond to passing `s` into and out of phy_update (phy_update.jl:3,151). */
 int kdq_set_fields(kdq_handle h, int32_t c, const uint8_t *s);
 int kdq_get_fields(kdq_handle h, int32_t c, uint8_t *s);
 /* Initial_s (src/tU/GreenMatrix.jl:1-8) from the chain's Philox stream. */
 int kdq_initial_s(kdq_handle h);
 
-/* phy_update(path, model, s, Sweeps, record) (src/tU/phy_update.jl:3-152) for every chain.
- * rows (may be NULL when record == 0) receives nchains x sweeps x 6 doubles
- * [Ek Eu CDW0 CDW1 SDW0 SDW1] averaged per sweep, the row the reference appends to its CSV
- * (phy_update.jl:144). The caller keeps the CSV writing. */
+/* phy_update(path, model, s, Sweeps, record) for every chain (t-U: src/tU/phy_update.jl:3-152,
+ * t-V: src/tV/phy_update.jl:1-188, t-U-V: src/tUV/phy_update.jl:1-186).
+ * rows (may be NULL when record == 0) receives nchains x sweeps x kdq_num_obs doubles averaged
+ * per sweep, the row the reference appends to its CSV: t-U [Ek Eu CDW0 CDW1 SDW0 SDW1]
+ * (phy_update.jl:144), t-V [Ek Ev R0[4] R1[4]] (:178), t-U-V [Ek Eu Ev CDW0 CDW1 SDW0 SDW1]
+ * (:178). The caller keeps the CSV writing. */
 int kdq_phy_update(kdq_handle h, int32_t sweeps, int32_t record, double *rows);
 
 /* ---- fine-grained entry points (parity checks, custom drivers) ---- */
 /* Current equal-time G of chain c as ns x ns ComplexF64 (interleaved, column-major). */
 int kdq_get_G(kdq_handle h, int32_t c, double *G);
 int kdq_set_G(kdq_handle h, int32_t c, const double *G);
-/* WrapKV!(…, D, G, "Forward"|"Backward", "B") with D = cis(±α[lt] η[s[:,lt]])
- * (src/tU/GreenMatrix.jl:56-86, as called at phy_update.jl:61-64 and :123-126). */
+/* The full wrap of one slice as the sweep applies it: t-U / t-U-V WrapKV!(…, "B")
+ * (src/tU/GreenMatrix.jl:56-86, src/tUV/GreenMatrix.jl:92-122); t-V the kinetic wrap and the
+ * three WrapV!(…, "B") bond rotations (src/tV/phy_update.jl:60-71 and :131-142). */
 int kdq_wrap(kdq_handle h, int32_t lt, int32_t forward);
-/* UpdatePhyLayer! at slice lt (src/tU/phy_update.jl:256-266) with draws from the Philox
- * stream of sweep `sweep` and pass direction dir (0 forward, 1 backward). */
+/* UpdatePhyLayer! at slice lt (src/tU/phy_update.jl:256-266, src/tUV/phy_update.jl:188-203;
+ * t-V: all three bond families in sweep order, src/tV/phy_update.jl:190-205) with draws from
+ * the Philox stream of sweep `sweep` and pass direction dir (0 forward, 1 backward). */
 int kdq_update_layer(kdq_handle h, int32_t lt, int32_t dir, uint32_t sweep);
 /* Rebuilds the U_L chain and sets G = get_G!(BL[1], BR[1]) (phy_update.jl:39-50). */
 int kdq_init_chain(kdq_handle h);
-/* phy_measure(model, lt, s, G) (src/tU/phy_update.jl:154-254) of the current G;
- * out receives nchains x 6 doubles. */
+/* phy_measure(model, lt, s, G) of the current G (src/tU/phy_update.jl:154-254,
+ * src/tV/phy_update.jl:207-314, src/tUV/phy_update.jl:205-355); out receives
+ * nchains x kdq_num_obs doubles. */
 int kdq_measure(kdq_handle h, int32_t lt, double *out);
 
 /* Counters: out[0] = proposals, out[1] = accepted, per chain (nchains x 2). */

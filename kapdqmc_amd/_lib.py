@@ -17,7 +17,7 @@ LATTICE_CODE = {"HoneyComb120": 0, "HoneyComb60": 1, "SQUARE90": 2, "SQUARE45": 
 PREC_FP64, PREC_MIXED = 0, 1
 
 # Every symbol include/kapdqmc.h declares (checked by tests/test_capi.py).
-EXPORTS = ("kdq_create", "kdq_destroy", "kdq_last_error", "kdq_num_chains", "kdq_set_fields",
+EXPORTS = ("kdq_create", "kdq_destroy", "kdq_last_error", "kdq_num_chains", "kdq_num_obs", "kdq_set_fields",
            "kdq_get_fields", "kdq_initial_s", "kdq_phy_update", "kdq_get_G", "kdq_set_G", "kdq_wrap",
            "kdq_update_layer", "kdq_init_chain", "kdq_measure", "kdq_get_counters", "kdq_profile",
            "kdq_profile_read")
@@ -38,7 +38,8 @@ class ModelDesc(C.Structure):
                 ("gamma", C.POINTER(C.c_double)), ("K", C.POINTER(C.c_double)),
                 ("eK", C.POINTER(C.c_double)), ("eKinv", C.POINTER(C.c_double)),
                 ("halfeK", C.POINTER(C.c_double)), ("halfeKinv", C.POINTER(C.c_double)),
-                ("Pt", C.POINTER(C.c_double))]
+                ("Pt", C.POINTER(C.c_double)), ("z", C.c_int32), ("bonds", C.POINTER(C.c_int32)),
+                ("a", C.c_double), ("is_complex", C.c_int32), ("U", C.c_double), ("V", C.c_double)]
 
 
 class RunDesc(C.Structure):
@@ -64,6 +65,7 @@ def load(path: str = LIB_PATH):
     L.kdq_last_error.argtypes = [vp]
     L.kdq_last_error.restype = C.c_char_p
     L.kdq_num_chains.argtypes = [vp]
+    L.kdq_num_obs.argtypes = [vp]
     L.kdq_set_fields.argtypes = [vp, i32, C.POINTER(C.c_uint8)]
     L.kdq_get_fields.argtypes = [vp, i32, C.POINTER(C.c_uint8)]
     L.kdq_initial_s.argtypes = [vp]

@@ -40,15 +40,27 @@ struct ChainDev {
     int *acc_idx;          // [64]
     double *W;             // 2 planes of 64 x 64 (re, im), ld 64
     double *counters;      // [2]: proposals, accepted
-    double *meas;          // [8]: accumulated observables + count
+    double *meas;          // [KDQ_MEAS_SLOTS]: accumulated observables, count in the last slot
+    // model kind and field layout
+    int kind;              // 0 t-U, 1 t-V, 2 t-U-V
+    int n_elem;            // field elements per slice (t-U: ns, t-V: 3 ns/2, t-U-V: ns z)
+    int is_complex;        // t-U-V: HS phases exp(i·) (U > 0) or exp(·) (U < 0)
+    int nb;                // t-V: bonds per family (ns/2)
+    const int *ex, *ey;    // [n_elem] 0-based sites of element e (t-V, t-U-V)
+    const int *inc;        // t-U-V: [ns * 6] incidence, entry = elem * 2 + (0: coefficient 1, 1: coefficient a)
+    double a, U, V;        // t-U-V constants (a = V/g), couplings used by the observables
 };
+
+#define KDQ_MEAS_SLOTS 16
 
 enum { DIM_NS = 0, DIM_NP = 1 };
 
 KDQ_DEV int dim_of(const ChainDev &c, int code) { return code == DIM_NS ? c.ns : c.np; }
 
 // Diagonal factor applied to GEMM rows, columns or the contraction index.
-// kind 0: none; kind 1: t-U phase cis(sign * alpha[lt-1] * eta[s[i, lt-1] - 1]).
+// kind 0: none; kind 1: t-U phase cis(sign * alpha[lt-1] * eta[s[i, lt-1] - 1]);
+// kind 2: t-U-V site factor of the directed-bond field, x = sign * Σ_b coef_b η[s_b],
+//         exp(i x) (complex chains) or exp(x) (real chains).
 struct DiagSpec {
     int kind;
     int lt;    // 1-based slice
@@ -58,9 +70,26 @@ struct DiagSpec {
 
 KDQ_DEV void diag_value(const ChainDev &c, const DiagSpec &d, int i, double &re, double &im) {
     if (d.kind == 1) {
-        int sv = c.s[(size_t)i + (size_t)(d.lt - 1) * c.ns];
+        int sv = c.s[(size_t)i + (size_t)(d.lt - 1) * c.n_elem];
         double x = d.sign * c.alpha[d.lt - 1] * c.eta[sv - 1];
         sincos(x, &im, &re);
+    } else if (d.kind == 2) {
+        const uint8_t *sl = c.s + (size_t)(d.lt - 1) * c.n_elem;
+        double x1 = 0.0, x2 = 0.0;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const int e = c.inc[i * 6 + k];
+            const double v = c.eta[sl[e >> 1] - 1];
+            if (e & 1) x2 += v;
+            else x1 += v;
+        }
+        const double x = d.sign * (x1 + c.a * x2);
+        if (c.is_complex) {
+            sincos(x, &im, &re);
+        } else {
+            re = exp(x);
+            im = 0.0;
+        }
     } else {
         re = 1.0;
         im = 0.0;

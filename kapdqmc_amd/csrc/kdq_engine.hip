@@ -32,7 +32,8 @@ struct ChainHost {
     int ns = 0, np = 0;
     std::vector<void *> allocs;
     double *eK, *eKinv, *hK, *hKinv, *K, *Pt, *alpha;
-    double *G[2], *T, *Gm, *Tm, *P0, *P1, *M, *X;
+    double *G[2], *T, *Gm, *Tm, *P0, *P1, *P2, *M, *X;
+    int *ex = nullptr, *ey = nullptr, *inc = nullptr;
     std::vector<double *> UR, ULH;
     uint8_t *s;
     int *acc_cnt, *acc_idx, *info;
@@ -47,6 +48,11 @@ struct Table {  // one logical operand across chains
 
 struct kdq_handle_s {
     int nch = 0, nt = 0, nn = 0, max_ns = 0, max_np = 0, delay = 16;
+    int kind = KDQ_MODEL_TU, nobs = 6, max_elem = 0, max_nb = 0;
+    // rank-2 block tables (t-V: per family, t-U-V: one group)
+    BlockDesc *d_blocks = nullptr;
+    int *d_block_offset = nullptr, *d_nblocks = nullptr;
+    int max_blocks = 0;
     std::vector<int64_t> nodes;
     std::vector<ChainHost> ch;
     std::vector<kdq_model_desc> md;
@@ -54,7 +60,7 @@ struct kdq_handle_s {
     hipStream_t stream = nullptr;
     ChainDev *d_chains = nullptr;
     // operand tables
-    Opnd *t_eK, *t_eKinv, *t_hK, *t_hKinv, *t_Pt, *t_G[2], *t_T, *t_Gm, *t_Tm, *t_P0, *t_P1, *t_M, *t_X;
+    Opnd *t_eK, *t_eKinv, *t_hK, *t_hKinv, *t_Pt, *t_G[2], *t_T, *t_Gm, *t_Tm, *t_P0, *t_P1, *t_P2, *t_M, *t_X;
     std::vector<Opnd *> t_UR, t_ULH;
     std::vector<void *> dev_misc;
     double *d_logdet = nullptr;   // [nch] log|det(U_L U_R)| of the last get_G
@@ -93,6 +99,7 @@ namespace {
     do {                                                                                 \
         hipError_t e_ = (expr);                                                          \
         if (e_ != hipSuccess) {                                                          \
+            (void)hipGetLastError(); /* do not leave the error sticky for later checks */ \
             h->err = std::string(#expr) + ": " + hipGetErrorString(e_);                  \
             set_global_error(h->err);                                                    \
             return KDQ_EHIP;                                                             \
@@ -192,7 +199,24 @@ int gemm(kdq_handle h, bool ac, bool bc, int opa, int opb, const GemmArgs &g, in
     return KDQ_OK;
 }
 
+// HS diagonal of slice lt for the site-diagonal models (t-U: kind 1, t-U-V: kind 2)
+DiagSpec hs_diag(kdq_handle h, int lt, int sign) { return DiagSpec{h->kind == KDQ_MODEL_TU ? 1 : 2, lt, sign, 0}; }
 DiagSpec phase(int lt, int sign) { return DiagSpec{1, lt, sign, 0}; }
+
+int bondrot(kdq_handle h, Opnd *X, int family, int lt, int sign, int mode, int ncol_dim, int fam_prof) {
+    ProfScope ps(h, fam_prof);
+    BondRotArgs b{h->d_chains, X, family, lt, sign, mode, ncol_dim, 0};
+    HIPCHK(launch_bondrot(h->stream, h->nch, h->max_nb, mode == 0 ? h->max_nb : maxdim(h, ncol_dim), b));
+    return KDQ_OK;
+}
+
+// t-V kinetic wrap: G <- eK G eKinv (forward) or eKinv G eK (backward)
+int wrap_K(kdq_handle h, Opnd *Gt, Opnd *Tt, bool forward, int fam) {
+    GemmArgs g1 = gargs(h, forward ? h->t_eK : h->t_eKinv, Gt, Tt, DIM_NS, DIM_NS, DIM_NS);
+    if (int r = gemm(h, false, true, 0, 0, g1, fam)) return r;
+    GemmArgs g2 = gargs(h, Tt, forward ? h->t_eKinv : h->t_eK, Gt, DIM_NS, DIM_NS, DIM_NS);
+    return gemm(h, true, false, 0, 0, g2, fam);
+}
 
 // G <- D eK G eKinv D⁻¹ (forward) or eKinv D⁻¹ G D eK (backward), D = cis(α η_s) at slice lt.
 int wrap_B(kdq_handle h, Opnd *Gt, Opnd *Tt, int lt, bool forward, int fam = KDQ_PROF_WRAP) {
@@ -200,14 +224,14 @@ int wrap_B(kdq_handle h, Opnd *Gt, Opnd *Tt, int lt, bool forward, int fam = KDQ
         GemmArgs g1 = gargs(h, h->t_eK, Gt, Tt, DIM_NS, DIM_NS, DIM_NS);
         if (int r = gemm(h, false, true, 0, 0, g1, fam)) return r;
         GemmArgs g2 = gargs(h, Tt, h->t_eKinv, Gt, DIM_NS, DIM_NS, DIM_NS);
-        g2.rowS = phase(lt, +1);
-        g2.colS = phase(lt, -1);
+        g2.rowS = hs_diag(h, lt, +1);
+        g2.colS = hs_diag(h, lt, -1);
         return gemm(h, true, false, 0, 0, g2, fam);
     }
-    // D' = cis(-α η): T = eKinv · (D' G) · D'⁻¹, then G = T eK
+    // D' = D(-): T = eKinv · (D' G) · D'⁻¹, then G = T eK
     GemmArgs g1 = gargs(h, h->t_eKinv, Gt, Tt, DIM_NS, DIM_NS, DIM_NS);
-    g1.inS = phase(lt, -1);
-    g1.colS = phase(lt, +1);
+    g1.inS = hs_diag(h, lt, -1);
+    g1.colS = hs_diag(h, lt, +1);
     if (int r = gemm(h, false, true, 0, 0, g1, fam)) return r;
     GemmArgs g2 = gargs(h, Tt, h->t_eK, Gt, DIM_NS, DIM_NS, DIM_NS);
     return gemm(h, true, false, 0, 0, g2, fam);
@@ -220,8 +244,11 @@ int propagate_R(kdq_handle h, Opnd *src, Opnd *dst, int idx) {
     for (int lt = lo; lt <= hi; ++lt) {
         Opnd *out = (lt == hi) ? dst : ((cur == h->t_P0) ? h->t_P1 : h->t_P0);
         GemmArgs g = gargs(h, h->t_eK, cur, out, DIM_NS, DIM_NP, DIM_NS);
-        g.rowS = phase(lt, +1);
+        if (h->kind != KDQ_MODEL_TV) g.rowS = hs_diag(h, lt, +1);
         if (int r = gemm(h, false, true, 0, 0, g, KDQ_PROF_CHAIN)) return r;
+        if (h->kind == KDQ_MODEL_TV)   // B(lt) = V1 V2 V3 eK: V3 first
+            for (int j = 2; j >= 0; --j)
+                if (int r = bondrot(h, out, j, lt, +1, 1, DIM_NP, KDQ_PROF_CHAIN)) return r;
         cur = out;
     }
     if (lo > hi) HIPCHK(launch_copy_mat(h->stream, h->nch, h->max_ns, h->max_np, h->d_chains, src, dst, DIM_NS, DIM_NP, true));
@@ -238,9 +265,19 @@ int propagate_L(kdq_handle h, Opnd *src, Opnd *dst, int idx) {
     Opnd *cur = src;
     for (int lt = hi; lt >= lo; --lt) {
         Opnd *out = (lt == lo) ? dst : ((cur == h->t_P0) ? h->t_P1 : h->t_P0);
-        GemmArgs g = gargs(h, h->t_eK, cur, out, DIM_NS, DIM_NP, DIM_NS);
-        g.inS = phase(lt, -1);
-        if (int r = gemm(h, false, true, 1, 0, g, KDQ_PROF_CHAIN)) return r;
+        if (h->kind == KDQ_MODEL_TV) {
+            // B(lt)ᴴ = eKᵀ V3 V2 V1 (V symmetric, real): rotate rows of a copy, then eKᵀ
+            Opnd *tmp = h->t_P2;
+            HIPCHK(launch_copy_mat(h->stream, h->nch, h->max_ns, h->max_np, h->d_chains, cur, tmp, DIM_NS, DIM_NP, true));
+            for (int j = 0; j < 3; ++j)
+                if (int r = bondrot(h, tmp, j, lt, +1, 1, DIM_NP, KDQ_PROF_CHAIN)) return r;
+            GemmArgs g = gargs(h, h->t_eK, tmp, out, DIM_NS, DIM_NP, DIM_NS);
+            if (int r = gemm(h, false, true, 1, 0, g, KDQ_PROF_CHAIN)) return r;
+        } else {
+            GemmArgs g = gargs(h, h->t_eK, cur, out, DIM_NS, DIM_NP, DIM_NS);
+            g.inS = hs_diag(h, lt, -1);
+            if (int r = gemm(h, false, true, 1, 0, g, KDQ_PROF_CHAIN)) return r;
+        }
         cur = out;
     }
     if (lo > hi) HIPCHK(launch_copy_mat(h->stream, h->nch, h->max_ns, h->max_np, h->d_chains, src, dst, DIM_NS, DIM_NP, true));
@@ -278,13 +315,39 @@ int init_chain(kdq_handle h) {
     return get_G(h, 1);
 }
 
+// Rank-2 blocks of one family (t-V) or of the whole slice (t-U-V, family 0).
+int update_family_r2(kdq_handle h, int lt, int dir, uint32_t sweep, int family) {
+    for (int b = 0; b < h->max_blocks; ++b) {
+        DecideR2Args a{};
+        a.chains = h->d_chains;
+        a.G = h->t_G[h->cur];
+        a.blocks = h->d_blocks;
+        a.block_offset = h->d_block_offset;
+        a.nblocks = h->d_nblocks;
+        a.lt = lt; a.dir = dir; a.family = family; a.block = b; a.sweep = sweep;
+        {
+            ProfScope ps(h, KDQ_PROF_DECIDE);
+            HIPCHK(launch_decide_r2(h->stream, h->nch, a));
+        }
+        FlushArgs f{h->d_chains, h->t_G[h->cur], h->t_G[1 - h->cur]};
+        {
+            ProfScope ps(h, KDQ_PROF_FLUSH);
+            HIPCHK(launch_flush(h->stream, h->nch, h->max_ns, f));
+        }
+        h->cur ^= 1;
+    }
+    return KDQ_OK;
+}
+
 int update_layer(kdq_handle h, int lt, int dir, uint32_t sweep) {
+    if (h->kind == KDQ_MODEL_TUV) return update_family_r2(h, lt, dir, sweep, 0);
     const int kb = h->delay;
     for (int s0 = 0; s0 < h->max_ns; s0 += kb) {
         DecideArgs a{};
         a.chains = h->d_chains;
         a.G = h->t_G[h->cur];
         a.lt = lt; a.dir = dir; a.s0 = s0; a.kb = kb; a.sweep = sweep;
+        // t-U only; t-V layers go through slice_tv
         {
             ProfScope ps(h, KDQ_PROF_DECIDE);
             HIPCHK(launch_decide_tu(h->stream, h->nch, a));
@@ -299,16 +362,48 @@ int update_layer(kdq_handle h, int lt, int dir, uint32_t sweep) {
     return KDQ_OK;
 }
 
+// t-V forward slice (src/tV/phy_update.jl:60-73): kinetic wrap, then for families 3,2,1
+// the bond rotation of the family and its rank-2 updates.
+int slice_tv_forward(kdq_handle h, Opnd *Gt, Opnd *Tt, int lt, bool update, uint32_t sweep, int fam) {
+    if (int r = wrap_K(h, Gt, Tt, true, fam)) return r;
+    for (int j = 2; j >= 0; --j) {
+        Opnd *cur = update ? h->t_G[h->cur] : Gt;
+        if (int r = bondrot(h, cur, j, lt, +1, 0, DIM_NS, fam)) return r;
+        if (update)
+            if (int r = update_family_r2(h, lt, 0, sweep, j)) return r;
+    }
+    return KDQ_OK;
+}
+
+// t-V backward slice (src/tV/phy_update.jl:131-142): for families 1,2,3 the rank-2 updates
+// and the inverse bond rotation, then the inverse kinetic wrap.
+int slice_tv_backward(kdq_handle h, Opnd *Gt, Opnd *Tt, int lt, bool update, uint32_t sweep, int fam) {
+    for (int j = 0; j < 3; ++j) {
+        if (update)
+            if (int r = update_family_r2(h, lt, 1, sweep, j)) return r;
+        Opnd *cur = update ? h->t_G[h->cur] : Gt;
+        if (int r = bondrot(h, cur, j, lt, -1, 0, DIM_NS, fam)) return r;
+    }
+    return wrap_K(h, update ? h->t_G[h->cur] : Gt, Tt, false, fam);
+}
+
+// One slice wrap of a G copy (no updates) in either direction, any model.
+int wrap_slice(kdq_handle h, Opnd *Gt, Opnd *Tt, int lt, bool forward, int fam) {
+    if (h->kind == KDQ_MODEL_TV)
+        return forward ? slice_tv_forward(h, Gt, Tt, lt, false, 0, fam) : slice_tv_backward(h, Gt, Tt, lt, false, 0, fam);
+    return wrap_B(h, Gt, Tt, lt, forward, fam);
+}
+
 // phy_measure: wrap a copy of G to τ = Nt/2, HalfeK sandwich, observables (accumulated).
 int measure(kdq_handle h, int lt) {
     HIPCHK(launch_copy_mat(h->stream, h->nch, h->max_ns, h->max_ns, h->d_chains, h->t_G[h->cur], h->t_Gm, DIM_NS, DIM_NS, true));
     const int nt = h->nt;
     if (lt > nt / 2.0) {
         for (int t = lt; t >= nt / 2 + 1; --t)
-            if (int r = wrap_B(h, h->t_Gm, h->t_Tm, t, false, KDQ_PROF_MEAS_WRAP)) return r;
+            if (int r = wrap_slice(h, h->t_Gm, h->t_Tm, t, false, KDQ_PROF_MEAS_WRAP)) return r;
     } else {
         for (int t = lt + 1; t <= nt / 2; ++t)
-            if (int r = wrap_B(h, h->t_Gm, h->t_Tm, t, true, KDQ_PROF_MEAS_WRAP)) return r;
+            if (int r = wrap_slice(h, h->t_Gm, h->t_Tm, t, true, KDQ_PROF_MEAS_WRAP)) return r;
     }
     GemmArgs g1 = gargs(h, h->t_hK, h->t_Gm, h->t_Tm, DIM_NS, DIM_NS, DIM_NS);
     if (int r = gemm(h, false, true, 0, 0, g1, KDQ_PROF_MEAS_WRAP)) return r;
@@ -316,8 +411,14 @@ int measure(kdq_handle h, int lt) {
     if (int r = gemm(h, true, false, 0, 0, g2, KDQ_PROF_MEAS_WRAP)) return r;
     MeasArgs m{h->d_chains, h->t_Gm};
     ProfScope ps(h, KDQ_PROF_MEASURE);
-    HIPCHK(launch_measure_tu(h->stream, h->nch, m));
+    if (h->kind == KDQ_MODEL_TU) HIPCHK(launch_measure_tu(h->stream, h->nch, m));
+    else HIPCHK(launch_measure_model(h->stream, h->nch, h->kind, m));
     return KDQ_OK;
+}
+
+int elems(kdq_handle h, int c) {
+    const int n = h->ch[c].ns;
+    return h->kind == KDQ_MODEL_TU ? n : (h->kind == KDQ_MODEL_TV ? 3 * (n / 2) : 3 * n);
 }
 
 bool is_node(kdq_handle h, int t) {
@@ -326,13 +427,14 @@ bool is_node(kdq_handle h, int t) {
     return false;
 }
 
-int read_meas(kdq_handle h, double *dst /* nch x 6 */) {
+int read_meas(kdq_handle h, double *dst /* nch x nobs */) {
+    const int S = KDQ_MEAS_SLOTS, no = h->nobs;
     for (int c = 0; c < h->nch; ++c)
-        HIPCHK(hipMemcpyAsync(h->h_meas + 8 * c, h->ch[c].meas, 8 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipMemcpyAsync(h->h_meas + S * c, h->ch[c].meas, S * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     for (int c = 0; c < h->nch; ++c) {
-        const double cnt = h->h_meas[8 * c + 6];
-        for (int k = 0; k < 6; ++k) dst[6 * c + k] = cnt > 0 ? h->h_meas[8 * c + k] / cnt : 0.0;
+        const double cnt = h->h_meas[S * c + S - 1];
+        for (int k = 0; k < no; ++k) dst[no * c + k] = cnt > 0 ? h->h_meas[S * c + k] / cnt : 0.0;
     }
     return KDQ_OK;
 }
@@ -347,17 +449,31 @@ int check_health(kdq_handle h) {
     return KDQ_OK;
 }
 
+// Measurement windows: t-U measures 0 <= Θidx - idx <= 1 going forward and
+// 0 <= idx - Θidx <= 1 going backward (src/tU/phy_update.jl:69,112); t-V and t-U-V measure
+// |idx - Θidx| <= 1 in both passes (src/tV/phy_update.jl:93,144; src/tUV/phy_update.jl:82,127).
+bool in_window(kdq_handle h, int thidx, bool forward) {
+    const int d = h->idx - thidx;
+    if (h->kind == KDQ_MODEL_TU) return forward ? (d <= 0 && d >= -1) : (d >= 0 && d <= 1);
+    return d >= -1 && d <= 1;
+}
+
 int phy_update(kdq_handle h, int sweeps, int record, double *rows) {
-    const int NN = h->nn, nt = h->nt, thidx = NN / 2 + 1;
+    const int NN = h->nn, nt = h->nt, thidx = NN / 2 + 1, no = h->nobs;
     if (int r = init_chain(h)) return r;
-    std::vector<double> tmp(6 * h->nch);
+    std::vector<double> tmp(no * h->nch);
+    const bool tv = h->kind == KDQ_MODEL_TV;
     for (int sw = 0; sw < sweeps; ++sw) {
         const uint32_t sweep = h->sweep;
         if (record) HIPCHK(launch_zero_meas(h->stream, h->nch, h->d_chains));
         for (int lt = 1; lt <= nt; ++lt) {
-            if (int r = wrap_B(h, h->t_G[h->cur], h->t_T, lt, true)) return r;
-            if (int r = update_layer(h, lt, 0, sweep)) return r;
-            if (record && thidx - h->idx >= 0 && thidx - h->idx <= 1)
+            if (tv) {
+                if (int r = slice_tv_forward(h, h->t_G[h->cur], h->t_T, lt, true, sweep, KDQ_PROF_WRAP)) return r;
+            } else {
+                if (int r = wrap_B(h, h->t_G[h->cur], h->t_T, lt, true)) return r;
+                if (int r = update_layer(h, lt, 0, sweep)) return r;
+            }
+            if (record && in_window(h, thidx, true))
                 if (int r = measure(h, lt)) return r;
             if (is_node(h, lt)) {
                 h->idx += 1;
@@ -366,10 +482,16 @@ int phy_update(kdq_handle h, int sweeps, int record, double *rows) {
             }
         }
         for (int lt = nt; lt >= 1; --lt) {
-            if (int r = update_layer(h, lt, 1, sweep)) return r;
-            if (record && h->idx - thidx >= 0 && h->idx - thidx <= 1)
-                if (int r = measure(h, lt)) return r;
-            if (int r = wrap_B(h, h->t_G[h->cur], h->t_T, lt, false)) return r;
+            if (tv) {
+                if (int r = slice_tv_backward(h, h->t_G[h->cur], h->t_T, lt, true, sweep, KDQ_PROF_WRAP)) return r;
+                if (record && in_window(h, thidx, false))
+                    if (int r = measure(h, lt - 1)) return r;
+            } else {
+                if (int r = update_layer(h, lt, 1, sweep)) return r;
+                if (record && in_window(h, thidx, false))
+                    if (int r = measure(h, lt)) return r;
+                if (int r = wrap_B(h, h->t_G[h->cur], h->t_T, lt, false)) return r;
+            }
             if (is_node(h, lt - 1)) {
                 h->idx -= 1;
                 if (int r = propagate_L(h, h->t_ULH[h->idx], h->t_ULH[h->idx - 1], h->idx)) return r;
@@ -379,7 +501,7 @@ int phy_update(kdq_handle h, int sweeps, int record, double *rows) {
         if (record && rows) {
             if (int r = read_meas(h, tmp.data())) return r;
             for (int c = 0; c < h->nch; ++c)
-                for (int k = 0; k < 6; ++k) rows[((size_t)c * sweeps + sw) * 6 + k] = tmp[6 * c + k];
+                for (int k = 0; k < no; ++k) rows[((size_t)c * sweeps + sw) * no + k] = tmp[no * c + k];
         }
         h->sweep += 1;
     }
@@ -402,10 +524,15 @@ int kdq_create(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t n
     *out = nullptr;
     for (int c = 0; c < nchains; ++c) {
         const kdq_model_desc &m = models[c];
-        if (m.kind != KDQ_MODEL_TU) return fail(nullptr, KDQ_EARG, "kdq_create: only the t-U model is available in this build");
-        if (m.ns < 2 || m.ns % 2 || m.nt < 1 || m.n_nodes < 2 || !m.nodes || !m.alpha || !m.eta || !m.gamma ||
-            !m.K || !m.eK || !m.eKinv || !m.halfeK || !m.halfeKinv || !m.Pt)
+        if (m.kind < KDQ_MODEL_TU || m.kind > KDQ_MODEL_TUV) return fail(nullptr, KDQ_EARG, "kdq_create: unknown model kind");
+        if (m.kind != models[0].kind) return fail(nullptr, KDQ_EARG, "kdq_create: all chains of a batch must share the model kind");
+        if (m.ns < 2 || m.ns % 2 || m.nt < 1 || m.n_nodes < 2 || !m.nodes || (!m.alpha && m.kind != KDQ_MODEL_TUV) ||
+            !m.eta || !m.gamma || !m.K || !m.eK || !m.eKinv || !m.halfeK || !m.halfeKinv || !m.Pt)
             return fail(nullptr, KDQ_EARG, "kdq_create: invalid model description for chain " + std::to_string(c));
+        if (m.kind != KDQ_MODEL_TU && (m.z != 3 || !m.bonds || m.lattice > KDQ_LATTICE_HC60))
+            return fail(nullptr, KDQ_EARG, "kdq_create: t-V and t-U-V need honeycomb bonds (z = 3)");
+        const int kmax = m.kind == KDQ_MODEL_TU ? 64 : 32;
+        if (runs[c].delay > kmax) return fail(nullptr, KDQ_EARG, "kdq_create: delay above the block limit of the model");
         if (m.ns / 2 > 256) return fail(nullptr, KDQ_EARG, "kdq_create: ns/2 above 256 is not supported");
         if (m.nt != models[0].nt || m.n_nodes != models[0].n_nodes)
             return fail(nullptr, KDQ_EARG, "kdq_create: all chains of a batch must share nt and nodes");
@@ -423,6 +550,8 @@ int kdq_create(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t n
     h->nn = models[0].n_nodes;
     h->nodes.assign(models[0].nodes, models[0].nodes + h->nn);
     h->delay = runs[0].delay;
+    h->kind = models[0].kind;
+    h->nobs = h->kind == KDQ_MODEL_TU ? 6 : (h->kind == KDQ_MODEL_TV ? 10 : 7);
     h->md.assign(models, models + nchains);
     h->rd.assign(runs, runs + nchains);
     h->sweep = runs[0].sweep0;
@@ -434,7 +563,9 @@ int kdq_create(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t n
         return r;
     }
     std::vector<ChainDev> cd(nchains);
-    std::vector<Opnd> v_eK, v_eKinv, v_hK, v_hKinv, v_Pt, v_G0, v_G1, v_T, v_Gm, v_Tm, v_P0, v_P1, v_M, v_X;
+    std::vector<Opnd> v_eK, v_eKinv, v_hK, v_hKinv, v_Pt, v_G0, v_G1, v_T, v_Gm, v_Tm, v_P0, v_P1, v_P2, v_M, v_X;
+    std::vector<BlockDesc> blk_all;
+    std::vector<int> blk_offset, blk_count;
     std::vector<std::vector<Opnd>> v_UR(h->nn), v_ULH(h->nn);
     for (int c = 0; c < nchains; ++c) {
         const kdq_model_desc &m = models[c];
@@ -459,6 +590,7 @@ int kdq_create(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t n
         ok &= dalloc(C, &C.Tm, 2 * nn) == hipSuccess;
         ok &= dalloc(C, &C.P0, 2 * nP) == hipSuccess;
         ok &= dalloc(C, &C.P1, 2 * nP) == hipSuccess;
+        ok &= dalloc(C, &C.P2, 2 * nP) == hipSuccess;
         ok &= dalloc(C, &C.M, 2 * pp) == hipSuccess;
         ok &= dalloc(C, &C.X, 2 * nP) == hipSuccess;
         C.UR.resize(h->nn);
@@ -467,7 +599,44 @@ int kdq_create(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t n
             ok &= dalloc(C, &C.UR[k], 2 * nP) == hipSuccess;
             ok &= dalloc(C, &C.ULH[k], 2 * nP) == hipSuccess;
         }
-        ok &= dalloc(C, &C.s, (size_t)n * m.nt) == hipSuccess;
+        const int n_elem = m.kind == KDQ_MODEL_TU ? n : (m.kind == KDQ_MODEL_TV ? 3 * (n / 2) : n * m.z);
+        h->max_elem = std::max(h->max_elem, n_elem);
+        h->max_nb = std::max(h->max_nb, n / 2);
+        ok &= dalloc(C, &C.s, (size_t)n_elem * m.nt) == hipSuccess;
+        std::vector<int> hx, hy, hinc;
+        if (m.kind != KDQ_MODEL_TU) {
+            hx.resize(n_elem);
+            hy.resize(n_elem);
+            for (int e = 0; e < n_elem; ++e) {
+                hx[e] = m.bonds[2 * e] - 1;
+                hy[e] = m.bonds[2 * e + 1] - 1;
+                if (hx[e] < 0 || hx[e] >= n || hy[e] < 0 || hy[e] >= n || hx[e] == hy[e]) {
+                    int r = fail(h, KDQ_EARG, "kdq_create: bond list out of range");
+                    delete h;
+                    return r;
+                }
+            }
+            ok &= dalloc(C, &C.ex, n_elem) == hipSuccess;
+            ok &= dalloc(C, &C.ey, n_elem) == hipSuccess;
+            if (m.kind == KDQ_MODEL_TUV) {
+                std::vector<int> cnt(n, 0);
+                hinc.assign((size_t)n * 6, 0);
+                for (int e = 0; e < n_elem; ++e) {
+                    for (int w = 0; w < 2; ++w) {
+                        const int x = w ? hy[e] : hx[e];
+                        if (cnt[x] >= 6) { ok = false; break; }
+                        hinc[(size_t)x * 6 + cnt[x]++] = e * 2 + w;
+                    }
+                }
+                for (int x = 0; x < n; ++x) ok &= cnt[x] == 6;
+                if (!ok) {
+                    int r = fail(h, KDQ_EARG, "kdq_create: every site must be on 3 + 3 directed bonds");
+                    delete h;
+                    return r;
+                }
+                ok &= dalloc(C, &C.inc, (size_t)n * 6) == hipSuccess;
+            }
+        }
         ok &= dalloc(C, &C.acc_cnt, 1) == hipSuccess;
         ok &= dalloc(C, &C.acc_idx, 64) == hipSuccess;
         ok &= dalloc(C, &C.info, 1) == hipSuccess;
@@ -486,7 +655,47 @@ int kdq_create(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t n
         hipMemcpy(C.hKinv, m.halfeKinv, nn * 8, hipMemcpyHostToDevice);
         hipMemcpy(C.K, m.K, nn * 8, hipMemcpyHostToDevice);
         hipMemcpy(C.Pt, m.Pt, nP * 8, hipMemcpyHostToDevice);
-        hipMemcpy(C.alpha, m.alpha, (size_t)m.nt * 8, hipMemcpyHostToDevice);
+        if (m.alpha) hipMemcpy(C.alpha, m.alpha, (size_t)m.nt * 8, hipMemcpyHostToDevice);
+        else {
+            std::vector<double> ones((size_t)m.nt, 1.0);
+            hipMemcpy(C.alpha, ones.data(), (size_t)m.nt * 8, hipMemcpyHostToDevice);
+        }
+        if (C.ex) {
+            hipMemcpy(C.ex, hx.data(), sizeof(int) * hx.size(), hipMemcpyHostToDevice);
+            hipMemcpy(C.ey, hy.data(), sizeof(int) * hy.size(), hipMemcpyHostToDevice);
+        }
+        if (C.inc) hipMemcpy(C.inc, hinc.data(), sizeof(int) * hinc.size(), hipMemcpyHostToDevice);
+        // rank-2 blocks: t-V per family (bonds i of family j are elements j*nb + i), t-U-V over the slice
+        if (m.kind != KDQ_MODEL_TU) {
+            const int kb = runs[c].delay, nb = n / 2;
+            const int nfam = m.kind == KDQ_MODEL_TV ? 3 : 1;
+            const int per = m.kind == KDQ_MODEL_TV ? nb : n_elem;
+            const int nblk = (per + kb - 1) / kb;
+            blk_offset.push_back((int)blk_all.size());
+            blk_count.push_back(nblk);
+            h->max_blocks = std::max(h->max_blocks, nblk);
+            for (int f = 0; f < nfam; ++f)
+                for (int b = 0; b < nblk; ++b) {
+                    BlockDesc bd;
+                    memset(&bd, 0, sizeof(bd));
+                    bd.e0 = f * per + b * kb;
+                    bd.ne = std::min(kb, per - b * kb);
+                    for (int q = 0; q < bd.ne; ++q) {
+                        const int e = bd.e0 + q;
+                        const int xs[2] = {hx[e], hy[e]};
+                        uint8_t sl[2];
+                        for (int w = 0; w < 2; ++w) {
+                            int k = 0;
+                            while (k < bd.nslot && bd.sites[k] != xs[w]) ++k;
+                            if (k == bd.nslot) bd.sites[bd.nslot++] = xs[w];
+                            sl[w] = (uint8_t)k;
+                        }
+                        bd.sl0[q] = sl[0];
+                        bd.sl1[q] = sl[1];
+                    }
+                    blk_all.push_back(bd);
+                }
+        }
 
         ChainDev &d = cd[c];
         memset(&d, 0, sizeof(d));
@@ -501,13 +710,20 @@ int kdq_create(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t n
         d.chain_id = runs[c].chain;
         d.acc_cnt = C.acc_cnt; d.acc_idx = C.acc_idx; d.W = C.W;
         d.counters = C.counters; d.meas = C.meas;
+        d.kind = m.kind;
+        d.n_elem = n_elem;
+        d.is_complex = m.kind == KDQ_MODEL_TUV ? (m.is_complex ? 1 : 0) : 1;
+        d.nb = n / 2;
+        d.ex = C.ex; d.ey = C.ey; d.inc = C.inc;
+        d.a = m.kind == KDQ_MODEL_TUV ? m.a : 0.0;
+        d.U = m.U; d.V = m.V;
 
         v_eK.push_back(real(C.eK, n)); v_eKinv.push_back(real(C.eKinv, n));
         v_hK.push_back(real(C.hK, n)); v_hKinv.push_back(real(C.hKinv, n));
         v_Pt.push_back(real(C.Pt, n));
         v_G0.push_back(cplx(C.G[0], n, nn)); v_G1.push_back(cplx(C.G[1], n, nn));
         v_T.push_back(cplx(C.T, n, nn)); v_Gm.push_back(cplx(C.Gm, n, nn)); v_Tm.push_back(cplx(C.Tm, n, nn));
-        v_P0.push_back(cplx(C.P0, n, nP)); v_P1.push_back(cplx(C.P1, n, nP));
+        v_P0.push_back(cplx(C.P0, n, nP)); v_P1.push_back(cplx(C.P1, n, nP)); v_P2.push_back(cplx(C.P2, n, nP));
         v_M.push_back(cplx(C.M, np, pp)); v_X.push_back(cplx(C.X, np, nP));
         for (int k = 0; k < h->nn; ++k) {
             v_UR[k].push_back(cplx(C.UR[k], n, nP));
@@ -531,7 +747,21 @@ int kdq_create(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t n
     r |= make_table(h, &h->t_Pt, v_Pt);
     r |= make_table(h, &h->t_G[0], v_G0); r |= make_table(h, &h->t_G[1], v_G1);
     r |= make_table(h, &h->t_T, v_T); r |= make_table(h, &h->t_Gm, v_Gm); r |= make_table(h, &h->t_Tm, v_Tm);
-    r |= make_table(h, &h->t_P0, v_P0); r |= make_table(h, &h->t_P1, v_P1);
+    r |= make_table(h, &h->t_P0, v_P0); r |= make_table(h, &h->t_P1, v_P1); r |= make_table(h, &h->t_P2, v_P2);
+    if (!blk_all.empty()) {
+        void *q1 = nullptr, *q2 = nullptr, *q3 = nullptr;
+        if (hipMalloc(&q1, sizeof(BlockDesc) * blk_all.size()) == hipSuccess) h->dev_misc.push_back(q1); else r = 1;
+        if (hipMalloc(&q2, sizeof(int) * nchains) == hipSuccess) h->dev_misc.push_back(q2); else r = 1;
+        if (hipMalloc(&q3, sizeof(int) * nchains) == hipSuccess) h->dev_misc.push_back(q3); else r = 1;
+        if (!r) {
+            hipMemcpy(q1, blk_all.data(), sizeof(BlockDesc) * blk_all.size(), hipMemcpyHostToDevice);
+            hipMemcpy(q2, blk_offset.data(), sizeof(int) * nchains, hipMemcpyHostToDevice);
+            hipMemcpy(q3, blk_count.data(), sizeof(int) * nchains, hipMemcpyHostToDevice);
+        }
+        h->d_blocks = (BlockDesc *)q1;
+        h->d_block_offset = (int *)q2;
+        h->d_nblocks = (int *)q3;
+    }
     r |= make_table(h, &h->t_M, v_M); r |= make_table(h, &h->t_X, v_X);
     h->t_UR.resize(h->nn);
     h->t_ULH.resize(h->nn);
@@ -547,7 +777,7 @@ int kdq_create(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t n
         h->d_info = (int *)q2;
         if (!r) hipMemset(q2, 0, sizeof(int) * nchains);
     }
-    if (r || hipHostMalloc((void **)&h->h_meas, sizeof(double) * 8 * nchains, hipHostMallocDefault) != hipSuccess) {
+    if (r || hipHostMalloc((void **)&h->h_meas, sizeof(double) * KDQ_MEAS_SLOTS * nchains, hipHostMallocDefault) != hipSuccess) {
         int rr = fail(h, KDQ_EHIP, "kdq_create: table allocation failed");
         delete h;
         return rr;
@@ -568,9 +798,11 @@ int kdq_destroy(kdq_handle h) {
 
 int kdq_num_chains(kdq_handle h) { return h ? h->nch : 0; }
 
+int kdq_num_obs(kdq_handle h) { return h ? h->nobs : 0; }
+
 int kdq_set_fields(kdq_handle h, int32_t c, const uint8_t *s) {
     if (!h || !s || c < 0 || c >= h->nch) return fail(h, KDQ_EARG, "kdq_set_fields: bad argument");
-    const size_t n = (size_t)h->ch[c].ns * h->nt;
+    const size_t n = (size_t)elems(h, c) * h->nt;
     for (size_t i = 0; i < n; ++i)
         if (s[i] < 1 || s[i] > 4) return fail(h, KDQ_EARG, "kdq_set_fields: field values must be 1..4");
     HIPCHK(hipMemcpyAsync(h->ch[c].s, s, n, hipMemcpyHostToDevice, h->stream));
@@ -580,14 +812,14 @@ int kdq_set_fields(kdq_handle h, int32_t c, const uint8_t *s) {
 
 int kdq_get_fields(kdq_handle h, int32_t c, uint8_t *s) {
     if (!h || !s || c < 0 || c >= h->nch) return fail(h, KDQ_EARG, "kdq_get_fields: bad argument");
-    HIPCHK(hipMemcpyAsync(s, h->ch[c].s, (size_t)h->ch[c].ns * h->nt, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(s, h->ch[c].s, (size_t)elems(h, c) * h->nt, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     return KDQ_OK;
 }
 
 int kdq_initial_s(kdq_handle h) {
     if (!h) return fail(h, KDQ_EARG, "kdq_initial_s: null handle");
-    HIPCHK(launch_initial_s(h->stream, h->nch, h->max_ns, h->nt, h->d_chains));
+    HIPCHK(launch_initial_s(h->stream, h->nch, h->max_elem, h->nt, h->d_chains));
     HIPCHK(hipStreamSynchronize(h->stream));
     return KDQ_OK;
 }
@@ -628,14 +860,19 @@ int kdq_set_G(kdq_handle h, int32_t c, const double *G) {
 
 int kdq_wrap(kdq_handle h, int32_t lt, int32_t forward) {
     if (!h || lt < 1 || lt > h->nt) return fail(h, KDQ_EARG, "kdq_wrap: bad argument");
-    if (int r = wrap_B(h, h->t_G[h->cur], h->t_T, lt, forward != 0)) return r;
+    if (int r = wrap_slice(h, h->t_G[h->cur], h->t_T, lt, forward != 0, KDQ_PROF_WRAP)) return r;
     HIPCHK(hipStreamSynchronize(h->stream));
     return KDQ_OK;
 }
 
 int kdq_update_layer(kdq_handle h, int32_t lt, int32_t dir, uint32_t sweep) {
     if (!h || lt < 1 || lt > h->nt || (dir != 0 && dir != 1)) return fail(h, KDQ_EARG, "kdq_update_layer: bad argument");
-    if (int r = update_layer(h, lt, dir, sweep)) return r;
+    if (h->kind == KDQ_MODEL_TV) {
+        for (int k = 0; k < 3; ++k)   // forward pass visits families 3,2,1, backward 1,2,3
+            if (int r = update_family_r2(h, lt, dir, sweep, dir == 0 ? 2 - k : k)) return r;
+    } else if (int r = update_layer(h, lt, dir, sweep)) {
+        return r;
+    }
     HIPCHK(hipStreamSynchronize(h->stream));
     return KDQ_OK;
 }

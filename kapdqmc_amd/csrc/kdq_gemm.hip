@@ -3,9 +3,8 @@
 // Tile 64 x 64 x 16, 256 threads = 4 wave64s in a 2 x 2 arrangement, each wave owning a
 // 32 x 32 block of C as 2 x 2 MFMA 16x16 blocks. Operands are staged through LDS as
 // k-major tiles. The MFMA is fed with op(B) as its first operand and op(A) as its second,
-// so each lane's four accumulator registers hold four consecutive *columns* of one row
-// group and every store instruction writes 16 consecutive rows: coalesced stores into the
-// column-major output.
+// so each lane's four accumulator registers hold four *columns* of one row and every store
+// instruction writes 16 consecutive rows: coalesced stores into the column-major output.
 //
 // Diagonal factors of the projector-DQMC algebra are fused: the HS phases of a time slice
 // (cis(±α η_s)) scale rows/columns in the epilogue or the contraction index while the
@@ -125,7 +124,9 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
         __syncthreads();
     }
 
-    // ---- epilogue: lane holds C[i][j..j+3] for i = row, j = 4*(lane/16) + v
+    // ---- epilogue. For v_mfma_f64_16x16x4f64 lane l, register v holds D[(l/16) + 4v][l%16]
+    // of the 16x16 block (checked on gfx950); D = op(B)ᵀ op(A)ᵀ here, so the lane writes
+    // C[i][j] with i = l%16 and j = l/16 + 4v: 16 consecutive rows per store.
     const Opnd Cm = g.C[c];
     Opnd Ci;
     if (g.mode == 2) Ci = g.Cin[c];
@@ -139,7 +140,7 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
         for (int ni = 0; ni < 2; ++ni)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int j = j0 + wn * 32 + ni * 16 + 4 * (lane >> 4) + v;
+                const int j = j0 + wn * 32 + ni * 16 + (lane >> 4) + 4 * v;
                 if (j >= N) continue;
                 double cr = accR[mi][ni][v], ci = OC ? accI[mi][ni][v] : 0.0;
                 if (g.rowS.kind) cmul(cr, ci, rsr, rsi, cr, ci);

@@ -63,10 +63,38 @@ struct MeasArgs {
 };
 hipError_t launch_measure_tu(hipStream_t st, int nchains, const MeasArgs &m);
 
+// Rank-2 local update (t-V bond flips, t-U-V directed-bond flips) of one block.
+struct BlockDesc {
+    int e0, ne, nslot, pad;   // field elements [e0, e0 + ne) within the slice, touched sites
+    int sites[64];            // global site of each local slot
+    uint8_t sl0[32], sl1[32]; // local slots of the two sites of each element
+};
+struct DecideR2Args {
+    const ChainDev *chains;
+    const Opnd *G;
+    const BlockDesc *blocks;
+    const int *block_offset;  // [nchains] first block of the chain
+    const int *nblocks;       // [nchains] blocks per family of the chain
+    int lt, dir, family, block;
+    uint32_t sweep;
+    int pad;
+};
+hipError_t launch_decide_r2(hipStream_t st, int nchains, const DecideR2Args &a);
+
+// t-V bond rotations of one family (mode 0: X <- M X M⁻¹, mode 1: X <- M X rows only).
+struct BondRotArgs {
+    const ChainDev *chains;
+    const Opnd *X;
+    int family, lt, sign, mode, ncol_dim, pad;
+};
+hipError_t launch_bondrot(hipStream_t st, int nchains, int max_nb, int max_cols, const BondRotArgs &a);
+// Observables of t-V (kind 1) and t-U-V (kind 2).
+hipError_t launch_measure_model(hipStream_t st, int nchains, int kind, const MeasArgs &m);
+
 // Small helpers.
 hipError_t launch_copy_mat(hipStream_t st, int nchains, int max_m, int max_n, const ChainDev *chains,
                            const Opnd *src, const Opnd *dst, int mdim, int ndim, bool cplx);
 hipError_t launch_real_to_cplx(hipStream_t st, int nchains, int max_m, int max_n, const ChainDev *chains,
                                const Opnd *src, const Opnd *dst, int mdim, int ndim);
-hipError_t launch_initial_s(hipStream_t st, int nchains, int max_ns, int nt, const ChainDev *chains);
+hipError_t launch_initial_s(hipStream_t st, int nchains, int max_elem, int nt, const ChainDev *chains);
 hipError_t launch_zero_meas(hipStream_t st, int nchains, const ChainDev *chains);

@@ -272,14 +272,14 @@ __global__ void k_initial_s(const ChainDev *chains, int nt) {
     const int c = blockIdx.y;
     const ChainDev &ch = chains[c];
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= ch.ns * nt) return;
-    const int i = e % ch.ns, lt = e / ch.ns;
+    if (e >= ch.n_elem * nt) return;
+    const int i = e % ch.n_elem, lt = e / ch.n_elem;
     ch.s[e] = (uint8_t)init_field_value(ch.seed, ch.chain_id, 0, (uint32_t)lt, (uint32_t)i);
 }
 
 __global__ void k_zero_meas(const ChainDev *chains) {
     const ChainDev &ch = chains[blockIdx.x];
-    if (threadIdx.x < 8) ch.meas[threadIdx.x] = 0.0;
+    if (threadIdx.x < KDQ_MEAS_SLOTS) ch.meas[threadIdx.x] = 0.0;
 }
 
 }  // namespace
@@ -308,8 +308,8 @@ hipError_t launch_real_to_cplx(hipStream_t st, int nchains, int max_m, int max_n
     return hipGetLastError();
 }
 
-hipError_t launch_initial_s(hipStream_t st, int nchains, int max_ns, int nt, const ChainDev *chains) {
-    dim3 grid((max_ns * nt + 255) / 256, nchains);
+hipError_t launch_initial_s(hipStream_t st, int nchains, int max_elem, int nt, const ChainDev *chains) {
+    dim3 grid((max_elem * nt + 255) / 256, nchains);
     hipLaunchKernelGGL(k_initial_s, grid, dim3(256), 0, st, chains, nt);
     return hipGetLastError();
 }

@@ -107,7 +107,7 @@ __global__ __launch_bounds__(NT) void k_measure_tu(MeasArgs a) {
         ch.meas[3] += out[3] / n2;
         ch.meas[4] += out[4] / n2;
         ch.meas[5] += out[5] / n2;
-        ch.meas[6] += 1.0;
+        ch.meas[KDQ_MEAS_SLOTS - 1] += 1.0;
     }
 }
 

@@ -55,7 +55,7 @@ __global__ __launch_bounds__(256) void k_decide_tu(DecideArgs a) {
         proposal_draw(ch.seed, ch.chain_id, 0, a.sweep, tag, (uint32_t)(s0 + tid), chv, u);
         chs[tid] = chv;
         us[tid] = u;
-        sv[tid] = ch.s[(size_t)(s0 + tid) + (size_t)(lt - 1) * n];
+        sv[tid] = ch.s[(size_t)(s0 + tid) + (size_t)(lt - 1) * ch.n_elem];
     }
     if (tid < 16) {
         const int si = tid >> 2, sx = tid & 3;
@@ -191,7 +191,7 @@ __global__ __launch_bounds__(256) void k_decide_tu(DecideArgs a) {
         ch.W[t + u * KB] = Wr[t][u];
         ch.W[KB * KB + t + u * KB] = Wi[t][u];
     }
-    if (tid < kb) ch.s[(size_t)(s0 + tid) + (size_t)(lt - 1) * n] = (uint8_t)sv[tid];
+    if (tid < kb) ch.s[(size_t)(s0 + tid) + (size_t)(lt - 1) * ch.n_elem] = (uint8_t)sv[tid];
 }
 
 // Gout[I,J] = Gin[I,J] - Gin[I,A] · (W · Y[:,J]),  Y[t, j] = δ(A_t, j) - Gin[A_t, j].

@@ -21,11 +21,25 @@ def _dp(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
-class DQMCBatch:
-    """A batch of t-U chains on the current GPU.
+KIND_CODE = {"tU": L.MODEL_TU, "tV": L.MODEL_TV, "tUV": L.MODEL_TUV}
+NOBS = {"tU": 6, "tV": 10, "tUV": 7}
 
-    models: list of ``tU_Hubbard_Para_`` (same Nt and nodes); seeds: list of int.
-    delay: local-update block size k (1 = one Sherman-Morrison update per flip).
+
+def field_shape(m):
+    """Shape of the reference's field array: t-U (Ns, Nt), t-V (Ns/2, 3, Nt), t-U-V (Ns z, Nt)."""
+    if m.kind == "tU":
+        return (m.Ns, m.Nt)
+    if m.kind == "tV":
+        return (m.Ns // 2, m.nnidx.shape[1], m.Nt)
+    return (m.Ns * m.nnidx.shape[1], m.Nt)
+
+
+class DQMCBatch:
+    """A batch of chains of one model kind (t-U, t-V or t-U-V) on the current GPU.
+
+    models: list of model structs (same kind, Nt and nodes); seeds: list of int.
+    delay: local-update block size k in sites (t-U, ≤ 64) or field elements (t-V / t-U-V,
+    ≤ 32); 1 = one Sherman-Morrison update per flip.
     """
 
     def __init__(self, models, seeds, chains=None, delay: int = 32, precision: str = "fp64",
@@ -39,20 +53,32 @@ class DQMCBatch:
         if precision != "fp64":
             raise ValueError("only precision='fp64' is available")
         self.models = list(models)
+        self.kind = getattr(models[0], "kind", "tU")
+        self.nobs = NOBS[self.kind]
         self._keep = []
         descs = (L.ModelDesc * n)()
         runs = (L.RunDesc * n)()
         for c, m in enumerate(models):
-            if getattr(m, "kind", "tU") != "tU":
-                raise ValueError("DQMCBatch drives the t-U model")
+            if getattr(m, "kind", "tU") != self.kind:
+                raise ValueError("all chains of a batch must share the model kind")
             f = lambda a: np.asfortranarray(np.asarray(a, dtype=np.float64))
-            arrs = dict(alpha=f(m.α), eta=f(m.η), gamma=f(m.γ), K=f(np.real(m.K)), eK=f(np.real(m.eK)),
+            arrs = dict(eta=f(m.η), gamma=f(m.γ), K=f(np.real(m.K)), eK=f(np.real(m.eK)),
                         eKinv=f(np.real(m.eKinv)), halfeK=f(np.real(m.HalfeK)),
                         halfeKinv=f(np.real(m.HalfeKinv)), Pt=f(np.real(m.Pt)))
+            if self.kind != "tUV":
+                arrs["alpha"] = f(m.α)
             nodes = np.ascontiguousarray(m.nodes, dtype=np.int64)
             self._keep += list(arrs.values()) + [nodes]
             d = descs[c]
-            d.kind, d.ns, d.nt, d.n_nodes = L.MODEL_TU, m.Ns, m.Nt, len(nodes)
+            d.kind, d.ns, d.nt, d.n_nodes = KIND_CODE[self.kind], m.Ns, m.Nt, len(nodes)
+            if self.kind != "tU":
+                # model.nnidx in column-major (linear) order as 1-based (x, y) pairs
+                bonds = np.ascontiguousarray(np.transpose(m.nnidx, (1, 0, 2)).reshape(-1, 2), dtype=np.int32)
+                self._keep.append(bonds)
+                d.z = m.nnidx.shape[1]
+                d.bonds = bonds.ctypes.data_as(C.POINTER(C.c_int32))
+            if self.kind == "tUV":
+                d.a, d.is_complex, d.U, d.V = m.a, int(m.Type is complex), m.U, m.V
             d.nodes = nodes.ctypes.data_as(C.POINTER(C.c_int64))
             d.lattice = L.LATTICE_CODE[m.Lattice]
             d.site[0], d.site[1] = m.site
@@ -85,13 +111,13 @@ class DQMCBatch:
     def set_fields(self, c: int, s):
         m = self.models[c]
         a = np.asfortranarray(np.asarray(s, dtype=np.uint8))
-        if a.shape != (m.Ns, m.Nt):
-            raise ValueError(f"s must have shape {(m.Ns, m.Nt)}")
+        if a.shape != field_shape(m):
+            raise ValueError(f"s must have shape {field_shape(m)}")
         self._chk(self.L.kdq_set_fields(self.h, c, a.ctypes.data_as(C.POINTER(C.c_uint8))))
 
     def get_fields(self, c: int) -> np.ndarray:
         m = self.models[c]
-        a = np.zeros((m.Ns, m.Nt), dtype=np.uint8, order="F")
+        a = np.zeros(field_shape(m), dtype=np.uint8, order="F")
         self._chk(self.L.kdq_get_fields(self.h, c, a.ctypes.data_as(C.POINTER(C.c_uint8))))
         return a
 
@@ -100,7 +126,7 @@ class DQMCBatch:
 
     # sweeps ---------------------------------------------------------------
     def phy_update(self, sweeps: int, record: bool = True) -> np.ndarray:
-        rows = np.zeros((self.n, sweeps, 6))
+        rows = np.zeros((self.n, sweeps, self.nobs))
         self._chk(self.L.kdq_phy_update(self.h, sweeps, int(record), _dp(rows)))
         return rows
 
@@ -126,7 +152,7 @@ class DQMCBatch:
         self._chk(self.L.kdq_init_chain(self.h))
 
     def measure(self, lt: int) -> np.ndarray:
-        out = np.zeros((self.n, 6))
+        out = np.zeros((self.n, self.nobs))
         self._chk(self.L.kdq_measure(self.h, lt, _dp(out)))
         return out
 
@@ -151,13 +177,20 @@ class DQMCBatch:
 
 
 def Initial_s(model, seed: int = 0, chain: int = 0) -> np.ndarray:
-    """Initial_s (src/tU/GreenMatrix.jl:1-8): UInt8 [Ns, Nt], i.i.d. uniform over {1..4},
-    drawn from the library's Philox stream instead of a MersenneTwister."""
-    return initial_fields(seed, chain, model.Ns, model.Nt)
+    """Initial_s (src/tU/GreenMatrix.jl:1-8, src/tV/GreenMatrix.jl:1-10,
+    src/tUV/GreenMatrix.jl:1-8): UInt8 field array of the model's shape, i.i.d. uniform over
+    {1..4}, drawn from the library's Philox stream instead of a MersenneTwister."""
+    shape = field_shape(model)
+    n_elem = int(np.prod(shape[:-1]))
+    flat = initial_fields(seed, chain, n_elem, model.Nt)          # [n_elem, Nt]
+    return np.asfortranarray(flat.reshape(shape, order="F"))
 
 
-def _lattice_tag(lattice: str) -> str:
-    tags = {"SQUARE90": "□90", "SQUARE45": "□45", "HoneyComb60": "HC", "HoneyComb120": "HC120"}
+def _lattice_tag(lattice: str, kind: str = "tU") -> str:
+    if kind == "tU":
+        tags = {"SQUARE90": "□90", "SQUARE45": "□45", "HoneyComb60": "HC", "HoneyComb120": "HC120"}
+    else:
+        tags = {"SQUARE": "□", "HoneyComb60": "HC", "HoneyComb120": "HC120"}
     if lattice not in tags:
         raise ValueError(f"Lattice: {lattice} is not allowed !")
     return tags[lattice]
@@ -165,9 +198,9 @@ def _lattice_tag(lattice: str) -> str:
 
 def phy_update(path: str, model, s: np.ndarray, Sweeps: int, record: bool = False, *, seed: int = 0,
                chain: int = 0, delay: int = 32, sweep0: int = 0) -> np.ndarray:
-    """phy_update (src/tU/phy_update.jl:3-152) on the GPU. Mutates ``s`` in place, returns it,
-    and appends one row [Ek Eu CDW0 CDW1 SDW0 SDW1] per sweep to the reference's CSV file
-    name (phy_update.jl:23-27,141-149) when ``record`` is true."""
+    """phy_update on the GPU (t-U: src/tU/phy_update.jl:3-152, t-V: src/tV/phy_update.jl:1-188,
+    t-U-V: src/tUV/phy_update.jl:1-186). Mutates ``s`` in place, returns it, and appends one
+    row of observables per sweep to the reference's CSV file name when ``record`` is true."""
     b = DQMCBatch([model], [seed], [chain], delay=delay, sweep0=sweep0)
     try:
         b.set_fields(0, s)
@@ -176,12 +209,20 @@ def phy_update(path: str, model, s: np.ndarray, Sweeps: int, record: bool = Fals
     finally:
         b.close()
     if record:
-        name = _lattice_tag(model.Lattice)
-        if model.Θquench == 0.0:
+        kind = getattr(model, "kind", "tU")
+        name = _lattice_tag(model.Lattice, kind)
+        if kind == "tU" and model.Θquench == 0.0:
             fn = f"tUphy{name}_t{model.Ht}U{model.Hu1}size{model.site}Δt{model.Δt}Θ{model.Θrelax}BS{model.BatchSize}.csv"
-        else:
+        elif kind == "tU":
             fn = (f"tUphy{name}_t{model.Ht}U{model.Hu1}_{model.Hu2}size{model.site}Δt{model.Δt}"
                   f"Θ{model.Θrelax}_{model.Θquench}BS{model.BatchSize}.csv")
+        elif kind == "tV" and len(np.unique(model.α)) == 1:
+            fn = f"tVphy{name}_t{model.Ht}V{model.Hv1}size{model.site}Δt{model.Δt}Θ{model.Θrelax}BS{model.BatchSize}.csv"
+        elif kind == "tV":
+            fn = (f"tVphy{name}_t{model.Ht}V{model.Hv1}_{model.Hv2}size{model.site}Δt{model.Δt}"
+                  f"Θ{model.Θrelax}_{model.Θquench}BS{model.BatchSize}.csv")
+        else:
+            fn = f"tUVphy{name}_t{model.Ht}U{model.U}V{model.V}size{model.site}Δt{model.Δt}Θ{model.Θ}BS{model.BatchSize}.csv"
         with open(os.path.join(path, fn), "a") as fh:
             for r in rows:
                 fh.write(",".join(repr(float(x)) for x in r) + "\n")
