@@ -65,7 +65,7 @@ struct DiagSpec {
     int kind;
     int lt;    // 1-based slice
     int sign;  // +1 or -1
-    int pad;
+    int conj;  // 1: complex conjugate of the value (D^H for the left chain)
 };
 
 KDQ_DEV void diag_value(const ChainDev &c, const DiagSpec &d, int i, double &re, double &im) {
@@ -94,6 +94,7 @@ KDQ_DEV void diag_value(const ChainDev &c, const DiagSpec &d, int i, double &re,
         re = 1.0;
         im = 0.0;
     }
+    if (d.conj) im = -im;
 }
 
 KDQ_HD void cmul(double ar, double ai, double br, double bi, double &cr, double &ci) {

@@ -200,8 +200,9 @@ int gemm(kdq_handle h, bool ac, bool bc, int opa, int opb, const GemmArgs &g, in
 }
 
 // HS diagonal of slice lt for the site-diagonal models (t-U: kind 1, t-U-V: kind 2)
-DiagSpec hs_diag(kdq_handle h, int lt, int sign) { return DiagSpec{h->kind == KDQ_MODEL_TU ? 1 : 2, lt, sign, 0}; }
-DiagSpec phase(int lt, int sign) { return DiagSpec{1, lt, sign, 0}; }
+DiagSpec hs_diag(kdq_handle h, int lt, int sign, int conj = 0) {
+    return DiagSpec{h->kind == KDQ_MODEL_TU ? 1 : 2, lt, sign, conj};
+}
 
 int bondrot(kdq_handle h, Opnd *X, int family, int lt, int sign, int mode, int ncol_dim, int fam_prof) {
     ProfScope ps(h, fam_prof);
@@ -259,6 +260,7 @@ int propagate_R(kdq_handle h, Opnd *src, Opnd *dst, int idx) {
 }
 
 // dst = QR_Q((U_L · B_block(idx))ᴴ) with U_L stored as U_Lᴴ: U_Lᴴ <- eKᵀ (conj(D_l) ⊙ U_Lᴴ)
+// (t-V: U_Lᴴ <- eKᵀ V3 V2 V1 U_Lᴴ)
 // for l = nodes[idx] down to nodes[idx-1]+1.
 int propagate_L(kdq_handle h, Opnd *src, Opnd *dst, int idx) {
     const int lo = (int)h->nodes[idx - 1] + 1, hi = (int)h->nodes[idx];
@@ -275,7 +277,7 @@ int propagate_L(kdq_handle h, Opnd *src, Opnd *dst, int idx) {
             if (int r = gemm(h, false, true, 1, 0, g, KDQ_PROF_CHAIN)) return r;
         } else {
             GemmArgs g = gargs(h, h->t_eK, cur, out, DIM_NS, DIM_NP, DIM_NS);
-            g.inS = hs_diag(h, lt, -1);
+            g.inS = hs_diag(h, lt, +1, 1);   // conj(D): D⁻¹ for unimodular phases, D for real ones
             if (int r = gemm(h, false, true, 1, 0, g, KDQ_PROF_CHAIN)) return r;
         }
         cur = out;
@@ -642,7 +644,7 @@ int kdq_create(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t n
         ok &= dalloc(C, &C.info, 1) == hipSuccess;
         ok &= dalloc(C, &C.W, 2 * 64 * 64) == hipSuccess;
         ok &= dalloc(C, &C.counters, 2) == hipSuccess;
-        ok &= dalloc(C, &C.meas, 8) == hipSuccess;
+        ok &= dalloc(C, &C.meas, KDQ_MEAS_SLOTS) == hipSuccess;
         ok &= dalloc(C, &C.logdet, 1) == hipSuccess;
         if (!ok) {
             int r = fail(h, KDQ_EHIP, "kdq_create: device allocation failed");
