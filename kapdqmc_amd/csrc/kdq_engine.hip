@@ -32,7 +32,7 @@ struct ChainHost {
     int ns = 0, np = 0;
     std::vector<void *> allocs;
     double *eK, *eKinv, *hK, *hKinv, *K, *Pt, *alpha;
-    double *G[2], *T, *Gm, *Tm, *P0, *P1, *P2, *M, *X;
+    double *G[2], *T, *Gm, *Tm, *P0, *P1, *P2, *M, *X, *S;
     int *ex = nullptr, *ey = nullptr, *inc = nullptr;
     std::vector<double *> UR, ULH;
     uint8_t *s;
@@ -60,11 +60,13 @@ struct kdq_handle_s {
     hipStream_t stream = nullptr;
     ChainDev *d_chains = nullptr;
     // operand tables
-    Opnd *t_eK, *t_eKinv, *t_hK, *t_hKinv, *t_Pt, *t_G[2], *t_T, *t_Gm, *t_Tm, *t_P0, *t_P1, *t_P2, *t_M, *t_X;
+    Opnd *t_eK, *t_eKinv, *t_hK, *t_hKinv, *t_Pt, *t_G[2], *t_T, *t_Gm, *t_Tm, *t_P0, *t_P1, *t_P2, *t_M, *t_X, *t_S;
     std::vector<Opnd *> t_UR, t_ULH;
     std::vector<void *> dev_misc;
     double *d_logdet = nullptr;   // [nch] log|det(U_L U_R)| of the last get_G
     int *d_info = nullptr;        // [nch] sticky: nonzero after a singular overlap
+    int *d_qinfo = nullptr;       // [nch] sticky: nonzero after a rank-deficient propagator
+    int *d_piv = nullptr;         // [nch][max_np] LU pivots of the overlap
     int cur = 0;           // which G buffer is current
     int idx = 1;           // current node index (1-based, as the reference)
     uint32_t sweep = 0;    // sweep counter of the next sweep
@@ -238,6 +240,24 @@ int wrap_B(kdq_handle h, Opnd *Gt, Opnd *Tt, int lt, bool forward, int fam = KDQ
     return gemm(h, true, false, 0, 0, g2, fam);
 }
 
+// U <- orthonormal basis of span(U) for the N x Np propagator, in place: two Cholesky-QR
+// passes (S = UᴴU, S + σI = L Lᴴ, U <- U L⁻ᴴ), the first shifted by σ = c·u·‖U‖_F².
+int orthonormalise(kdq_handle h, Opnd *U) {
+    const double u = 1.1102230246251565e-16;
+    const double n = h->max_ns, k = h->max_np;
+    const double shift = 11.0 * (n * k + k * (k + 1.0)) * u;
+    for (int pass = 0; pass < 2; ++pass) {
+        GemmArgs g = gargs(h, U, U, h->t_S, DIM_NP, DIM_NP, DIM_NS);
+        if (int r = gemm(h, true, true, 2, 0, g, KDQ_PROF_QR)) return r;
+        ProfScope ps(h, KDQ_PROF_QR);
+        CholArgs ca{h->d_chains, h->t_S, DIM_NP, 0, pass == 0 ? shift : 0.0, h->d_qinfo};
+        HIPCHK(launch_chol(h->stream, h->nch, h->max_np, ca));
+        TrsmArgs ta{h->d_chains, U, h->t_S, DIM_NS, DIM_NP};
+        HIPCHK(launch_trsm_rlh(h->stream, h->nch, h->max_ns, h->max_np, ta));
+    }
+    return KDQ_OK;
+}
+
 // dst = QR_Q(B_block(idx) · src) for U_R: slices nodes[idx-1]+1 .. nodes[idx] (1-based idx)
 int propagate_R(kdq_handle h, Opnd *src, Opnd *dst, int idx) {
     const int lo = (int)h->nodes[idx - 1] + 1, hi = (int)h->nodes[idx];
@@ -253,10 +273,7 @@ int propagate_R(kdq_handle h, Opnd *src, Opnd *dst, int idx) {
         cur = out;
     }
     if (lo > hi) HIPCHK(launch_copy_mat(h->stream, h->nch, h->max_ns, h->max_np, h->d_chains, src, dst, DIM_NS, DIM_NP, true));
-    QRArgs q{h->d_chains, dst, DIM_NS, DIM_NP};
-    ProfScope ps(h, KDQ_PROF_QR);
-    HIPCHK(launch_qr(h->stream, h->nch, q));
-    return KDQ_OK;
+    return orthonormalise(h, dst);
 }
 
 // dst = QR_Q((U_L · B_block(idx))ᴴ) with U_L stored as U_Lᴴ: U_Lᴴ <- eKᵀ (conj(D_l) ⊙ U_Lᴴ)
@@ -283,10 +300,7 @@ int propagate_L(kdq_handle h, Opnd *src, Opnd *dst, int idx) {
         cur = out;
     }
     if (lo > hi) HIPCHK(launch_copy_mat(h->stream, h->nch, h->max_ns, h->max_np, h->d_chains, src, dst, DIM_NS, DIM_NP, true));
-    QRArgs q{h->d_chains, dst, DIM_NS, DIM_NP};
-    ProfScope ps(h, KDQ_PROF_QR);
-    HIPCHK(launch_qr(h->stream, h->nch, q));
-    return KDQ_OK;
+    return orthonormalise(h, dst);
 }
 
 // get_G!: G = I - U_R (U_L U_R)⁻¹ U_L into the current G buffer
@@ -294,13 +308,13 @@ int get_G(kdq_handle h, int idx, int *bad = nullptr) {
     Opnd *ULH = h->t_ULH[idx - 1], *UR = h->t_UR[idx - 1];
     GemmArgs g1 = gargs(h, ULH, UR, h->t_M, DIM_NP, DIM_NP, DIM_NS);
     if (int r = gemm(h, true, true, 2, 0, g1, KDQ_PROF_GETG)) return r;
-    InvArgs iv{h->d_chains, h->t_M, DIM_NP, 0, h->d_logdet, h->d_info};
     {
         ProfScope ps(h, KDQ_PROF_INV);
-        HIPCHK(launch_inv(h->stream, h->nch, iv));
+        LuArgs la{h->d_chains, h->t_M, DIM_NP, h->max_np, h->d_piv, h->d_logdet, h->d_info};
+        HIPCHK(launch_lu(h->stream, h->nch, h->max_np, la));
+        LuSolveArgs sa{h->d_chains, h->t_M, ULH, h->t_X, h->d_piv, DIM_NP, DIM_NS, h->max_np, 0};
+        HIPCHK(launch_lu_solve(h->stream, h->nch, h->max_np, h->max_ns, sa));
     }
-    GemmArgs g2 = gargs(h, h->t_M, ULH, h->t_X, DIM_NP, DIM_NS, DIM_NP);
-    if (int r = gemm(h, true, true, 0, 2, g2, KDQ_PROF_GETG)) return r;
     GemmArgs g3 = gargs(h, UR, h->t_X, h->t_G[h->cur], DIM_NS, DIM_NS, DIM_NP);
     g3.mode = 1;
     (void)bad;
@@ -443,11 +457,16 @@ int read_meas(kdq_handle h, double *dst /* nch x nobs */) {
 
 // Reads the sticky singular-overlap flags; KDQ_ENUM names the first failing chain.
 int check_health(kdq_handle h) {
-    std::vector<int> info(h->nch);
+    std::vector<int> info(h->nch), qinfo(h->nch);
     HIPCHK(hipMemcpyAsync(info.data(), h->d_info, sizeof(int) * h->nch, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(qinfo.data(), h->d_qinfo, sizeof(int) * h->nch, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
-    for (int c = 0; c < h->nch; ++c)
+    for (int c = 0; c < h->nch; ++c) {
         if (info[c]) return fail(h, KDQ_ENUM, "singular overlap U_L U_R in chain " + std::to_string(c));
+        if (qinfo[c])
+            return fail(h, KDQ_ENUM, "propagator lost rank between stabilisations in chain " + std::to_string(c) +
+                                         " (reduce BatchSize)");
+    }
     return KDQ_OK;
 }
 
@@ -565,7 +584,7 @@ int kdq_create(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t n
         return r;
     }
     std::vector<ChainDev> cd(nchains);
-    std::vector<Opnd> v_eK, v_eKinv, v_hK, v_hKinv, v_Pt, v_G0, v_G1, v_T, v_Gm, v_Tm, v_P0, v_P1, v_P2, v_M, v_X;
+    std::vector<Opnd> v_eK, v_eKinv, v_hK, v_hKinv, v_Pt, v_G0, v_G1, v_T, v_Gm, v_Tm, v_P0, v_P1, v_P2, v_M, v_X, v_S;
     std::vector<BlockDesc> blk_all;
     std::vector<int> blk_offset, blk_count;
     std::vector<std::vector<Opnd>> v_UR(h->nn), v_ULH(h->nn);
@@ -594,6 +613,7 @@ int kdq_create(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t n
         ok &= dalloc(C, &C.P1, 2 * nP) == hipSuccess;
         ok &= dalloc(C, &C.P2, 2 * nP) == hipSuccess;
         ok &= dalloc(C, &C.M, 2 * pp) == hipSuccess;
+        ok &= dalloc(C, &C.S, 2 * pp) == hipSuccess;
         ok &= dalloc(C, &C.X, 2 * nP) == hipSuccess;
         C.UR.resize(h->nn);
         C.ULH.resize(h->nn);
@@ -727,6 +747,7 @@ int kdq_create(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t n
         v_T.push_back(cplx(C.T, n, nn)); v_Gm.push_back(cplx(C.Gm, n, nn)); v_Tm.push_back(cplx(C.Tm, n, nn));
         v_P0.push_back(cplx(C.P0, n, nP)); v_P1.push_back(cplx(C.P1, n, nP)); v_P2.push_back(cplx(C.P2, n, nP));
         v_M.push_back(cplx(C.M, np, pp)); v_X.push_back(cplx(C.X, np, nP));
+        v_S.push_back(cplx(C.S, np, pp));
         for (int k = 0; k < h->nn; ++k) {
             v_UR[k].push_back(cplx(C.UR[k], n, nP));
             v_ULH[k].push_back(cplx(C.ULH[k], n, nP));
@@ -764,7 +785,7 @@ int kdq_create(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t n
         h->d_block_offset = (int *)q2;
         h->d_nblocks = (int *)q3;
     }
-    r |= make_table(h, &h->t_M, v_M); r |= make_table(h, &h->t_X, v_X);
+    r |= make_table(h, &h->t_M, v_M); r |= make_table(h, &h->t_X, v_X); r |= make_table(h, &h->t_S, v_S);
     h->t_UR.resize(h->nn);
     h->t_ULH.resize(h->nn);
     for (int k = 0; k < h->nn; ++k) {
@@ -772,12 +793,19 @@ int kdq_create(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t n
         r |= make_table(h, &h->t_ULH[k], v_ULH[k]);
     }
     {
-        void *q1 = nullptr, *q2 = nullptr;
+        void *q1 = nullptr, *q2 = nullptr, *q3 = nullptr, *q4 = nullptr;
         if (hipMalloc(&q1, sizeof(double) * nchains) == hipSuccess) h->dev_misc.push_back(q1); else r = 1;
         if (hipMalloc(&q2, sizeof(int) * nchains) == hipSuccess) h->dev_misc.push_back(q2); else r = 1;
+        if (hipMalloc(&q3, sizeof(int) * nchains) == hipSuccess) h->dev_misc.push_back(q3); else r = 1;
+        if (hipMalloc(&q4, sizeof(int) * nchains * (size_t)h->max_np) == hipSuccess) h->dev_misc.push_back(q4); else r = 1;
         h->d_logdet = (double *)q1;
         h->d_info = (int *)q2;
-        if (!r) hipMemset(q2, 0, sizeof(int) * nchains);
+        h->d_qinfo = (int *)q3;
+        h->d_piv = (int *)q4;
+        if (!r) {
+            hipMemset(q2, 0, sizeof(int) * nchains);
+            hipMemset(q3, 0, sizeof(int) * nchains);
+        }
     }
     if (r || hipHostMalloc((void **)&h->h_meas, sizeof(double) * KDQ_MEAS_SLOTS * nchains, hipHostMallocDefault) != hipSuccess) {
         int rr = fail(h, KDQ_EHIP, "kdq_create: table allocation failed");

@@ -1,0 +1,552 @@
+// FP64 factorizations of the stabilisation step (complex, planar storage, column-major).
+//
+// Orthonormalisation of the N x Np propagators (the role of geqrf+orgqr / gerqf+orgrq at
+// src/tU/phy_update.jl:44-45,85-86) by two-pass Cholesky QR: S = Uᴴ U (MFMA GEMM),
+// S + σI = L Lᴴ (k_chol), U <- U L⁻ᴴ (k_trsm_rlh), then once more without the shift. Only
+// span(U) enters G = I - U_R (U_L U_R)⁻¹ U_L (src/public/Gupdate.jl:139-157), and U L⁻ᴴ has
+// exactly the span of U for any invertible L, so the result equals the Householder one up
+// to rounding. The first pass is shifted (σ = c·u·‖U‖_F², Fukaya et al., SIAM J. Sci.
+// Comput. 42 (2020) A477) so that its Cholesky cannot break down for cond(U) < ~1e12; the
+// second pass restores orthonormality to O(u).
+// Overlap solve of get_G!: blocked LU with partial pivoting (k_lu, getrf semantics) and
+// X = M⁻¹ U_L (k_lu_solve, getrs semantics), log|det M| from the diagonal of U.
+//
+// Each factorization runs in one workgroup per chain with its current 16-wide panel in LDS;
+// the triangular solves run over 16-row (16-column) slabs held in LDS, one workgroup each.
+#include "kdq_kernels.h"
+
+namespace {
+
+constexpr int NT = 256;
+constexpr int NB = 16;      // panel / slab width
+constexpr int PS = NB + 1;  // padded LDS row stride of a panel (doubles)
+constexpr int RB = 16;      // rows (right-hand sides) per workgroup of the solves
+
+KDQ_DEV double blk_sum(double v, double *red) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < NT / 64; ++w) s += red[w];
+    return s;
+}
+
+// In-place Cholesky S + σI = L Lᴴ of the Hermitian n x n matrix (lower triangle read and
+// written), σ = shift · trace(S). info (sticky) receives j+1 for a non-positive pivot j.
+__global__ __launch_bounds__(NT) void k_chol(CholArgs a) {
+    const int c = blockIdx.x;
+    const ChainDev &ch = a.chains[c];
+    const int n = dim_of(ch, a.ndim);
+    const Opnd A = a.A[c];
+    const size_t ld = A.ld;
+    double *Re = A.re, *Im = A.im;
+    extern __shared__ double lds[];
+    double *Pr = lds, *Pi = lds + (size_t)n * PS;  // panel rows k0.., [r][q]
+    __shared__ double red[NT / 64];
+    __shared__ int bad;
+    const int tid = threadIdx.x;
+    if (tid == 0) bad = 0;
+    double tr = 0.0;
+    for (int i = tid; i < n; i += NT) tr += Re[i + (size_t)i * ld];
+    const double sigma = a.shift * blk_sum(tr, red);
+    for (int k0 = 0; k0 < n; k0 += NB) {
+        const int kb = min(NB, n - k0), m = n - k0;
+        __syncthreads();
+        for (int e = tid; e < m * kb; e += NT) {
+            const int r = e % m, q = e / m;
+            const size_t o = (k0 + r) + (size_t)(k0 + q) * ld;
+            Pr[r * PS + q] = Re[o] + (r == q ? sigma : 0.0);
+            Pi[r * PS + q] = Im[o];
+        }
+        __syncthreads();
+        for (int j = 0; j < kb; ++j) {
+            const double djj = Pr[j * PS + j];
+            __syncthreads();
+            if (!(djj > 0.0) && tid == 0 && !bad) bad = k0 + j + 1;
+            const double d = sqrt(fmax(djj, 1e-300)), id = 1.0 / d;
+            for (int r = j + tid; r < m; r += NT) {
+                if (r == j) {
+                    Pr[j * PS + j] = d;
+                    Pi[j * PS + j] = 0.0;
+                } else {
+                    Pr[r * PS + j] *= id;
+                    Pi[r * PS + j] *= id;
+                }
+            }
+            __syncthreads();
+            const int nq = kb - j - 1;
+            for (int e = tid; e < (m - j - 1) * nq; e += NT) {
+                const int r = j + 1 + e / nq, q = j + 1 + e % nq;
+                if (r < q) continue;
+                double tr_, ti_;
+                cmul(Pr[r * PS + j], Pi[r * PS + j], Pr[q * PS + j], -Pi[q * PS + j], tr_, ti_);
+                Pr[r * PS + q] -= tr_;
+                Pi[r * PS + q] -= ti_;
+            }
+            __syncthreads();
+        }
+        for (int e = tid; e < m * kb; e += NT) {
+            const int r = e % m, q = e / m;
+            if (r < q) continue;
+            const size_t o = (k0 + r) + (size_t)(k0 + q) * ld;
+            Re[o] = Pr[r * PS + q];
+            Im[o] = Pi[r * PS + q];
+        }
+        // trailing Hermitian update (lower): S[i][j] -= Σ_q P[i][q] conj(P[j][q]), i >= j
+        const int mt = m - kb;
+        for (int e = tid; e < mt * mt; e += NT) {
+            const int i = kb + e % mt, j = kb + e / mt;
+            if (i < j) continue;
+            double sr = 0.0, si = 0.0;
+#pragma unroll 4
+            for (int q = 0; q < kb; ++q) {
+                const double ar = Pr[i * PS + q], ai = Pi[i * PS + q];
+                const double br = Pr[j * PS + q], bi = -Pi[j * PS + q];
+                sr += ar * br - ai * bi;
+                si += ar * bi + ai * br;
+            }
+            const size_t o = (k0 + i) + (size_t)(k0 + j) * ld;
+            Re[o] -= sr;
+            Im[o] -= si;
+        }
+    }
+    __syncthreads();
+    if (tid == 0 && bad && a.info && !a.info[c]) a.info[c] = bad;
+}
+
+// X <- X L⁻ᴴ for the m x n matrix X (in place; L lower, real positive diagonal):
+// X[r][j] = (X[r][j] - Σ_{i<j} X[r][i] conj(L[j][i])) / L[j][j]. One workgroup per 16 rows;
+// the rows live in LDS, the current 16-column slab of L is staged per step.
+__global__ __launch_bounds__(NT) void k_trsm_rlh(TrsmArgs t) {
+    const int c = blockIdx.y;
+    const ChainDev &ch = t.chains[c];
+    const int m = dim_of(ch, t.mdim), n = dim_of(ch, t.ndim);
+    const int r0 = blockIdx.x * RB;
+    if (r0 >= m) return;
+    const int rb = min(RB, m - r0);
+    const Opnd X = t.X[c], L = t.L[c];
+    extern __shared__ double lds[];
+    double *Xr = lds, *Xi = lds + (size_t)n * RB;                      // [j][r]
+    double *Lr = lds + 2 * (size_t)n * RB, *Li = Lr + (size_t)n * PS;  // [jj][i]
+    const int tid = threadIdx.x;
+    for (int e = tid; e < n * RB; e += NT) {
+        const int r = e % RB, j = e / RB;
+        double vr = 0.0, vi = 0.0;
+        if (r < rb) {
+            const size_t o = (size_t)(r0 + r) + (size_t)j * X.ld;
+            vr = X.re[o];
+            vi = X.im[o];
+        }
+        Xr[e] = vr;
+        Xi[e] = vi;
+    }
+    for (int b0 = 0; b0 < n; b0 += NB) {
+        const int bw = min(NB, n - b0), rest = n - b0;
+        __syncthreads();
+        for (int e = tid; e < rest * bw; e += NT) {
+            const int jj = e % rest, i = e / rest;
+            const size_t o = (size_t)(b0 + jj) + (size_t)(b0 + i) * L.ld;
+            Lr[jj * PS + i] = L.re[o];
+            Li[jj * PS + i] = L.im[o];
+        }
+        __syncthreads();
+        if (tid < RB) {
+            const int r = tid;
+            for (int j = 0; j < bw; ++j) {
+                double sr = Xr[(b0 + j) * RB + r], si = Xi[(b0 + j) * RB + r];
+                for (int i = 0; i < j; ++i) {
+                    const double xr = Xr[(b0 + i) * RB + r], xi = Xi[(b0 + i) * RB + r];
+                    const double lr = Lr[j * PS + i], li = -Li[j * PS + i];
+                    sr -= xr * lr - xi * li;
+                    si -= xr * li + xi * lr;
+                }
+                const double id = 1.0 / Lr[j * PS + j];
+                Xr[(b0 + j) * RB + r] = sr * id;
+                Xi[(b0 + j) * RB + r] = si * id;
+            }
+        }
+        __syncthreads();
+        const int nrest = rest - bw;
+        for (int e = tid; e < nrest * RB; e += NT) {
+            const int r = e % RB, jj = bw + e / RB;
+            double sr = 0.0, si = 0.0;
+#pragma unroll 4
+            for (int i = 0; i < bw; ++i) {
+                const double xr = Xr[(b0 + i) * RB + r], xi = Xi[(b0 + i) * RB + r];
+                const double lr = Lr[jj * PS + i], li = -Li[jj * PS + i];
+                sr += xr * lr - xi * li;
+                si += xr * li + xi * lr;
+            }
+            Xr[(b0 + jj) * RB + r] -= sr;
+            Xi[(b0 + jj) * RB + r] -= si;
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < n * RB; e += NT) {
+        const int r = e % RB, j = e / RB;
+        if (r >= rb) continue;
+        const size_t o = (size_t)(r0 + r) + (size_t)j * X.ld;
+        X.re[o] = Xr[e];
+        X.im[o] = Xi[e];
+    }
+}
+
+// In-place LU with partial pivoting of the n x n matrix, P A = L U (getrf semantics: the
+// interchanges are applied to whole rows; pivots stored 0-based, row k swapped with piv[k]).
+// logdet receives log|det A|; info (sticky) receives j+1 for an exactly zero pivot j.
+__global__ __launch_bounds__(NT) void k_lu(LuArgs a) {
+    const int c = blockIdx.x;
+    const ChainDev &ch = a.chains[c];
+    const int n = dim_of(ch, a.ndim);
+    const Opnd A = a.A[c];
+    const size_t ld = A.ld;
+    double *Re = A.re, *Im = A.im;
+    int *piv = a.piv + (size_t)c * a.piv_ld;
+    extern __shared__ double lds[];
+    double *Pr = lds, *Pi = lds + (size_t)n * PS;                      // panel [r][q]
+    double *Ur = lds + 2 * (size_t)n * PS, *Ui = Ur + (size_t)n * PS;  // U12 [col][j]
+    __shared__ double red_v[NT / 64];
+    __shared__ int red_i[NT / 64], lpiv[NB], bad;
+    __shared__ double s_logdet;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) {
+        bad = 0;
+        s_logdet = 0.0;
+    }
+    for (int k0 = 0; k0 < n; k0 += NB) {
+        const int kb = min(NB, n - k0), m = n - k0;
+        __syncthreads();
+        for (int e = tid; e < m * kb; e += NT) {
+            const int r = e % m, q = e / m;
+            const size_t o = (k0 + r) + (size_t)(k0 + q) * ld;
+            Pr[r * PS + q] = Re[o];
+            Pi[r * PS + q] = Im[o];
+        }
+        __syncthreads();
+        for (int j = 0; j < kb; ++j) {
+            // pivot: first index of max |P[r][j]|², r >= j
+            double best = -1.0;
+            int bi = m;
+            for (int r = j + tid; r < m; r += NT) {
+                const double v = Pr[r * PS + j] * Pr[r * PS + j] + Pi[r * PS + j] * Pi[r * PS + j];
+                if (v > best) {
+                    best = v;
+                    bi = r;
+                }
+            }
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                const double ob = __shfl_xor(best, off, 64);
+                const int oi = __shfl_xor(bi, off, 64);
+                if (ob > best || (ob == best && oi < bi)) {
+                    best = ob;
+                    bi = oi;
+                }
+            }
+            if (lane == 0) {
+                red_v[wave] = best;
+                red_i[wave] = bi;
+            }
+            __syncthreads();
+            if (tid == 0) {
+                double b = red_v[0];
+                int p = red_i[0];
+                for (int w = 1; w < NT / 64; ++w)
+                    if (red_v[w] > b || (red_v[w] == b && red_i[w] < p)) {
+                        b = red_v[w];
+                        p = red_i[w];
+                    }
+                lpiv[j] = p;
+                piv[k0 + j] = k0 + p;
+            }
+            __syncthreads();
+            const int p = lpiv[j];
+            if (p != j && tid < kb) {
+                const double ar = Pr[j * PS + tid], ai = Pi[j * PS + tid];
+                Pr[j * PS + tid] = Pr[p * PS + tid];
+                Pi[j * PS + tid] = Pi[p * PS + tid];
+                Pr[p * PS + tid] = ar;
+                Pi[p * PS + tid] = ai;
+            }
+            __syncthreads();
+            const double pr = Pr[j * PS + j], pim = Pi[j * PS + j];
+            const double p2 = pr * pr + pim * pim;
+            if (p2 == 0.0) {   // singular: leave the column, flag (uniform branch)
+                if (tid == 0 && !bad) bad = k0 + j + 1;
+                continue;
+            }
+            const double ir = pr / p2, ii = -pim / p2;
+            for (int r = j + 1 + tid; r < m; r += NT) {
+                double tr_, ti_;
+                cmul(Pr[r * PS + j], Pi[r * PS + j], ir, ii, tr_, ti_);
+                Pr[r * PS + j] = tr_;
+                Pi[r * PS + j] = ti_;
+            }
+            __syncthreads();
+            const int nq = kb - j - 1;
+            for (int e = tid; e < (m - j - 1) * nq; e += NT) {
+                const int r = j + 1 + e / nq, q = j + 1 + e % nq;
+                double tr_, ti_;
+                cmul(Pr[r * PS + j], Pi[r * PS + j], Pr[j * PS + q], Pi[j * PS + q], tr_, ti_);
+                Pr[r * PS + q] -= tr_;
+                Pi[r * PS + q] -= ti_;
+            }
+            __syncthreads();
+        }
+        // panel (L21, U11 and L11) back to memory; log|det| of this panel's pivots
+        for (int e = tid; e < m * kb; e += NT) {
+            const int r = e % m, q = e / m;
+            const size_t o = (k0 + r) + (size_t)(k0 + q) * ld;
+            Re[o] = Pr[r * PS + q];
+            Im[o] = Pi[r * PS + q];
+        }
+        if (tid == 0) {
+            double s = 0.0;
+            for (int j = 0; j < kb; ++j)
+                s += 0.5 * log(Pr[j * PS + j] * Pr[j * PS + j] + Pi[j * PS + j] * Pi[j * PS + j]);
+            s_logdet += s;
+        }
+        // the panel's interchanges on every column outside it (one thread per column)
+        for (int col = tid; col < n; col += NT) {
+            if (col >= k0 && col < k0 + kb) continue;
+            for (int j = 0; j < kb; ++j) {
+                const int p = lpiv[j];
+                if (p == j) continue;
+                const size_t oa = (k0 + j) + (size_t)col * ld, ob = (k0 + p) + (size_t)col * ld;
+                const double ar = Re[oa], ai = Im[oa];
+                Re[oa] = Re[ob];
+                Im[oa] = Im[ob];
+                Re[ob] = ar;
+                Im[ob] = ai;
+            }
+        }
+        __syncthreads();
+        // U12 = L11⁻¹ A12 (unit lower), one thread per column, kept in LDS and stored
+        const int ncol = n - k0 - kb;
+        for (int q = tid; q < ncol; q += NT) {
+            const size_t col = (size_t)(k0 + kb + q) * ld;
+            for (int j = 0; j < kb; ++j) {
+                double sr = Re[k0 + j + col], si = Im[k0 + j + col];
+                for (int i = 0; i < j; ++i) {
+                    double tr_, ti_;
+                    cmul(Pr[j * PS + i], Pi[j * PS + i], Ur[q * PS + i], Ui[q * PS + i], tr_, ti_);
+                    sr -= tr_;
+                    si -= ti_;
+                }
+                Ur[q * PS + j] = sr;
+                Ui[q * PS + j] = si;
+                Re[k0 + j + col] = sr;
+                Im[k0 + j + col] = si;
+            }
+        }
+        __syncthreads();
+        // trailing update A22 -= L21 U12
+        const int mt = m - kb;
+        for (int e = tid; e < mt * ncol; e += NT) {
+            const int i = kb + e % mt, q = e / mt;
+            double sr = 0.0, si = 0.0;
+#pragma unroll 4
+            for (int j = 0; j < kb; ++j) {
+                const double ar = Pr[i * PS + j], ai = Pi[i * PS + j];
+                const double br = Ur[q * PS + j], bi = Ui[q * PS + j];
+                sr += ar * br - ai * bi;
+                si += ar * bi + ai * br;
+            }
+            const size_t o = (k0 + i) + (size_t)(k0 + kb + q) * ld;
+            Re[o] -= sr;
+            Im[o] -= si;
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        if (a.logdet) a.logdet[c] = s_logdet;
+        if (bad && a.info && !a.info[c]) a.info[c] = bad;
+    }
+}
+
+// X = M⁻¹ Cᴴ with the LU factors of M (n x n, from k_lu) and the nr x n matrix C (so X is
+// n x nr): one workgroup per 16 right-hand sides held in LDS. Row interchanges, unit-lower
+// forward and upper backward substitution by 16-column slabs of the factors.
+__global__ __launch_bounds__(NT) void k_lu_solve(LuSolveArgs s) {
+    const int c = blockIdx.y;
+    const ChainDev &ch = s.chains[c];
+    const int n = dim_of(ch, s.ndim), nr = dim_of(ch, s.rdim);
+    const int r0 = blockIdx.x * RB;
+    if (r0 >= nr) return;
+    const int rb = min(RB, nr - r0);
+    const Opnd F = s.LU[c], Cm = s.C[c], X = s.X[c];
+    const int *piv = s.piv + (size_t)c * s.piv_ld;
+    extern __shared__ double lds[];
+    double *Br = lds, *Bi = lds + (size_t)n * RB;                      // [i][q]
+    double *Fr = lds + 2 * (size_t)n * RB, *Fi = Fr + (size_t)n * PS;  // factor slab [row][i]
+    const int tid = threadIdx.x;
+    const size_t ld = F.ld;
+    for (int e = tid; e < n * RB; e += NT) {
+        const int q = e % RB, i = e / RB;
+        double vr = 0.0, vi = 0.0;
+        if (q < rb) {
+            const size_t o = (size_t)(r0 + q) + (size_t)i * Cm.ld;
+            vr = Cm.re[o];
+            vi = -Cm.im[o];
+        }
+        Br[e] = vr;
+        Bi[e] = vi;
+    }
+    __syncthreads();
+    if (tid < RB) {
+        const int q = tid;
+        for (int k = 0; k < n; ++k) {
+            const int p = piv[k];
+            if (p != k) {
+                const double ar = Br[k * RB + q], ai = Bi[k * RB + q];
+                Br[k * RB + q] = Br[p * RB + q];
+                Bi[k * RB + q] = Bi[p * RB + q];
+                Br[p * RB + q] = ar;
+                Bi[p * RB + q] = ai;
+            }
+        }
+    }
+    // forward substitution with the unit lower factor
+    for (int b0 = 0; b0 < n; b0 += NB) {
+        const int bw = min(NB, n - b0), rest = n - b0;
+        __syncthreads();
+        for (int e = tid; e < rest * bw; e += NT) {
+            const int jj = e % rest, i = e / rest;
+            const size_t o = (size_t)(b0 + jj) + (size_t)(b0 + i) * ld;
+            Fr[jj * PS + i] = F.re[o];
+            Fi[jj * PS + i] = F.im[o];
+        }
+        __syncthreads();
+        if (tid < RB) {
+            const int q = tid;
+            for (int j = 1; j < bw; ++j) {
+                double sr = Br[(b0 + j) * RB + q], si = Bi[(b0 + j) * RB + q];
+                for (int i = 0; i < j; ++i) {
+                    double tr_, ti_;
+                    cmul(Fr[j * PS + i], Fi[j * PS + i], Br[(b0 + i) * RB + q], Bi[(b0 + i) * RB + q], tr_, ti_);
+                    sr -= tr_;
+                    si -= ti_;
+                }
+                Br[(b0 + j) * RB + q] = sr;
+                Bi[(b0 + j) * RB + q] = si;
+            }
+        }
+        __syncthreads();
+        for (int e = tid; e < (rest - bw) * RB; e += NT) {
+            const int q = e % RB, jj = bw + e / RB;
+            double sr = 0.0, si = 0.0;
+#pragma unroll 4
+            for (int i = 0; i < bw; ++i) {
+                const double ar = Fr[jj * PS + i], ai = Fi[jj * PS + i];
+                const double br = Br[(b0 + i) * RB + q], bi = Bi[(b0 + i) * RB + q];
+                sr += ar * br - ai * bi;
+                si += ar * bi + ai * br;
+            }
+            Br[(b0 + jj) * RB + q] -= sr;
+            Bi[(b0 + jj) * RB + q] -= si;
+        }
+    }
+    // backward substitution with the upper factor, last slab first
+    const int nslab = (n + NB - 1) / NB;
+    for (int sb = nslab - 1; sb >= 0; --sb) {
+        const int b0 = sb * NB, bw = min(NB, n - b0), top = b0 + bw;
+        __syncthreads();
+        for (int e = tid; e < top * bw; e += NT) {
+            const int jj = e % top, i = e / top;
+            const size_t o = (size_t)jj + (size_t)(b0 + i) * ld;
+            Fr[jj * PS + i] = F.re[o];
+            Fi[jj * PS + i] = F.im[o];
+        }
+        __syncthreads();
+        if (tid < RB) {
+            const int q = tid;
+            for (int j = bw - 1; j >= 0; --j) {
+                double sr = Br[(b0 + j) * RB + q], si = Bi[(b0 + j) * RB + q];
+                for (int i = j + 1; i < bw; ++i) {
+                    double tr_, ti_;
+                    cmul(Fr[(b0 + j) * PS + i], Fi[(b0 + j) * PS + i], Br[(b0 + i) * RB + q],
+                         Bi[(b0 + i) * RB + q], tr_, ti_);
+                    sr -= tr_;
+                    si -= ti_;
+                }
+                const double ur = Fr[(b0 + j) * PS + j], ui = Fi[(b0 + j) * PS + j];
+                const double u2 = ur * ur + ui * ui;
+                double xr, xi;
+                cmul(sr, si, ur / u2, -ui / u2, xr, xi);
+                Br[(b0 + j) * RB + q] = xr;
+                Bi[(b0 + j) * RB + q] = xi;
+            }
+        }
+        __syncthreads();
+        for (int e = tid; e < b0 * RB; e += NT) {
+            const int q = e % RB, jj = e / RB;
+            double sr = 0.0, si = 0.0;
+#pragma unroll 4
+            for (int i = 0; i < bw; ++i) {
+                const double ar = Fr[jj * PS + i], ai = Fi[jj * PS + i];
+                const double br = Br[(b0 + i) * RB + q], bi = Bi[(b0 + i) * RB + q];
+                sr += ar * br - ai * bi;
+                si += ar * bi + ai * br;
+            }
+            Br[jj * RB + q] -= sr;
+            Bi[jj * RB + q] -= si;
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < n * RB; e += NT) {
+        const int i = e % n, q = e / n;
+        if (q >= rb) continue;
+        const size_t o = (size_t)i + (size_t)(r0 + q) * X.ld;
+        X.re[o] = Br[i * RB + q];
+        X.im[o] = Bi[i * RB + q];
+    }
+}
+
+template <class K>
+hipError_t allow_lds(K kern, size_t bytes) {
+    return hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)bytes);
+}
+
+}  // namespace
+
+size_t factor_lds_bytes(int which, int max_n) {
+    const size_t n = (size_t)max_n;
+    switch (which) {
+        case 0: return 2 * n * PS * sizeof(double);                 // k_chol
+        case 1: return (2 * n * RB + 2 * n * PS) * sizeof(double);  // k_trsm_rlh, k_lu_solve
+        case 2: return 4 * n * PS * sizeof(double);                 // k_lu
+        default: return 0;
+    }
+}
+
+hipError_t launch_chol(hipStream_t st, int nchains, int max_n, const CholArgs &a) {
+    const size_t b = factor_lds_bytes(0, max_n);
+    if (hipError_t e = allow_lds(k_chol, b)) return e;
+    hipLaunchKernelGGL(k_chol, dim3(nchains), dim3(NT), b, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_trsm_rlh(hipStream_t st, int nchains, int max_m, int max_n, const TrsmArgs &t) {
+    const size_t b = factor_lds_bytes(1, max_n);
+    if (hipError_t e = allow_lds(k_trsm_rlh, b)) return e;
+    hipLaunchKernelGGL(k_trsm_rlh, dim3((max_m + RB - 1) / RB, nchains), dim3(NT), b, st, t);
+    return hipGetLastError();
+}
+
+hipError_t launch_lu(hipStream_t st, int nchains, int max_n, const LuArgs &a) {
+    const size_t b = factor_lds_bytes(2, max_n);
+    if (hipError_t e = allow_lds(k_lu, b)) return e;
+    hipLaunchKernelGGL(k_lu, dim3(nchains), dim3(NT), b, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_lu_solve(hipStream_t st, int nchains, int max_n, int max_r, const LuSolveArgs &s) {
+    const size_t b = factor_lds_bytes(1, max_n);
+    if (hipError_t e = allow_lds(k_lu_solve, b)) return e;
+    hipLaunchKernelGGL(k_lu_solve, dim3((max_r + RB - 1) / RB, nchains), dim3(NT), b, st, s);
+    return hipGetLastError();
+}

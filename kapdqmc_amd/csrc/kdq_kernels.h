@@ -37,24 +37,49 @@ struct FlushArgs {
 };
 hipError_t launch_flush(hipStream_t st, int nchains, int max_ns, const FlushArgs &f);
 
-// In-place Householder QR with explicit Q of the per-chain m x n complex matrix.
-struct QRArgs {
-    const ChainDev *chains;
-    const Opnd *A;
-    int mdim, ndim;
-};
-hipError_t launch_qr(hipStream_t st, int nchains, const QRArgs &q);
-
-// In-place Gauss-Jordan inverse of the per-chain n x n complex matrix (partial pivoting);
-// logdet (may be null) receives log|det| per chain, info receives 0 or k+1 on a zero pivot.
-struct InvArgs {
+// Cholesky S + shift·trace(S)·I = L Lᴴ in place (lower triangle) of the per-chain n x n
+// Hermitian matrix; info (sticky, may be null) receives j+1 on a non-positive pivot.
+struct CholArgs {
     const ChainDev *chains;
     const Opnd *A;
     int ndim, pad;
+    double shift;
+    int *info;
+};
+hipError_t launch_chol(hipStream_t st, int nchains, int max_n, const CholArgs &a);
+
+// X <- X L⁻ᴴ in place for the m x n matrix X and the lower n x n factor L.
+struct TrsmArgs {
+    const ChainDev *chains;
+    const Opnd *X;
+    const Opnd *L;
+    int mdim, ndim;
+};
+hipError_t launch_trsm_rlh(hipStream_t st, int nchains, int max_m, int max_n, const TrsmArgs &t);
+
+// In-place LU with partial pivoting (getrf semantics, 0-based pivots at piv + chain·piv_ld);
+// logdet (may be null) receives log|det|, info (sticky) j+1 on an exactly zero pivot.
+struct LuArgs {
+    const ChainDev *chains;
+    const Opnd *A;
+    int ndim, piv_ld;
+    int *piv;
     double *logdet;
     int *info;
 };
-hipError_t launch_inv(hipStream_t st, int nchains, const InvArgs &q);
+hipError_t launch_lu(hipStream_t st, int nchains, int max_n, const LuArgs &a);
+
+// X = M⁻¹ Cᴴ from the LU factors of M (n x n) for the per-chain nr x n matrix C.
+struct LuSolveArgs {
+    const ChainDev *chains;
+    const Opnd *LU;
+    const Opnd *C;
+    const Opnd *X;
+    const int *piv;
+    int ndim, rdim, piv_ld, pad;
+};
+hipError_t launch_lu_solve(hipStream_t st, int nchains, int max_n, int max_r, const LuSolveArgs &s);
+size_t factor_lds_bytes(int which, int max_n);
 
 // Observables of phy_measure on the sandwiched G0 (t-U); accumulates into chains[c].meas.
 struct MeasArgs {

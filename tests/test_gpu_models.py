@@ -79,3 +79,29 @@ def test_tv_batched_sizes(dq, oracle_mod):
         assert np.array_equal(b.get_fields(c), o.s)
         assert rel(b.get_G(c).real, o.G) < G_TOL
         assert np.allclose(rb[c], ro, rtol=1e-9, atol=1e-11)
+
+
+
+@pytest.mark.parametrize("bs", [5, 10])
+def test_tv_long_blocks_stabilisation(dq, oracle_mod, bs):
+    """Strong real HS coupling (V=1.5) over long stabilisation blocks: the GPU's
+    orthonormalisation keeps G as accurate as the oracle's Householder QR does. The
+    yardstick is the oracle with BatchSize 2 on the same fields (well conditioned); at
+    BatchSize 10 the Householder result itself is off by ~3e-9 there."""
+    print(f"case: t-V stabilisation BatchSize={bs}", flush=True)
+    m = tv(L=4, V=1.5, theta=2.0, bs=bs)
+    o = oracle_mod.OracleTV(m, seed=5)
+    o.initial_s()
+    o.init_chain()
+    fine = oracle_mod.OracleTV(tv(L=4, V=1.5, theta=2.0, bs=2), seed=5)
+    fine.s = o.s
+    fine.init_chain()
+    err_ref = np.abs(o.G - fine.G).max()
+    b = dq.DQMCBatch([m], [5], [0], delay=16)
+    b.set_fields(0, o.s)
+    b.init_chain()
+    G = b.get_G(0).real
+    err_gpu = np.abs(G - fine.G).max()
+    print(f"  householder err {err_ref:.2e}  gpu err {err_gpu:.2e}", flush=True)
+    assert err_gpu <= 10 * err_ref + 1e-12
+    assert np.abs(G @ G - G).max() < 1e-10
