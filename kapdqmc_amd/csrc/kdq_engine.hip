@@ -345,7 +345,7 @@ int update_family_r2(kdq_handle h, int lt, int dir, uint32_t sweep, int family) 
             ProfScope ps(h, KDQ_PROF_DECIDE);
             HIPCHK(launch_decide_r2(h->stream, h->nch, a));
         }
-        FlushArgs f{h->d_chains, h->t_G[h->cur], h->t_G[1 - h->cur]};
+        FlushArgs f{h->d_chains, h->t_G[h->cur], h->t_G[1 - h->cur], 64, 0};   // two slots per element
         {
             ProfScope ps(h, KDQ_PROF_FLUSH);
             HIPCHK(launch_flush(h->stream, h->nch, h->max_ns, f));
@@ -368,7 +368,7 @@ int update_layer(kdq_handle h, int lt, int dir, uint32_t sweep) {
             ProfScope ps(h, KDQ_PROF_DECIDE);
             HIPCHK(launch_decide_tu(h->stream, h->nch, a));
         }
-        FlushArgs f{h->d_chains, h->t_G[h->cur], h->t_G[1 - h->cur]};
+        FlushArgs f{h->d_chains, h->t_G[h->cur], h->t_G[1 - h->cur], kb, 0};
         {
             ProfScope ps(h, KDQ_PROF_FLUSH);
             HIPCHK(launch_flush(h->stream, h->nch, h->max_ns, f));

@@ -34,6 +34,7 @@ struct FlushArgs {
     const ChainDev *chains;
     const Opnd *Gin;
     const Opnd *Gout;
+    int kmax, pad;   // largest accepted set the preceding decisions can produce (<= 64)
 };
 hipError_t launch_flush(hipStream_t st, int nchains, int max_ns, const FlushArgs &f);
 

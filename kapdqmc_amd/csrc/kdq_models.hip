@@ -40,6 +40,8 @@ __global__ __launch_bounds__(256) void k_decide_r2(DecideR2Args a) {
     __shared__ int chs[KE], sv[KE], acc[KS], gsite[KS];
     __shared__ int sx0[KE], sx1[KE];
     __shared__ double wsh[8];
+    // per (s, s') = 4 si + sx: Δ (2x2) and the weight factor (extra · γ'/γ)
+    __shared__ double dtab_r[16][4], dtab_i[16][4], ftab[16];
 
     const Opnd G = a.G[c];
     const int lt = a.lt;
@@ -56,6 +58,39 @@ __global__ __launch_bounds__(256) void k_decide_r2(DecideR2Args a) {
         G0i[r][q] = G.im[off];
     }
     const uint32_t tag = (uint32_t)lt | ((uint32_t)a.dir << 31);
+    const int kind = ch.kind;
+    const double h = 0.7071067811865476;   // 2^0.5 / 2
+    if (tid >= 128 && tid < 144) {
+        const int si = (tid - 128) >> 2, sx = (tid - 128) & 3;   // 0-based old / new values
+        double Dr[2][2] = {{0.0, 0.0}, {0.0, 0.0}}, Di[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+        double extra = 1.0;
+        if (kind == 1) {   // t-V: Δ = uv diag(e^{v}-1, e^{-v}-1) uv
+            const double v = ch.alpha[lt - 1] * (ch.eta[sx] - ch.eta[si]);
+            const double t0 = exp(v) - 1.0, t1 = exp(-v) - 1.0;
+            const double uv[2][2] = {{-h, -h}, {-h, h}};
+#pragma unroll
+            for (int p = 0; p < 2; ++p)
+#pragma unroll
+                for (int q = 0; q < 2; ++q) Dr[p][q] = (uv[p][0] * t0) * uv[0][q] + (uv[p][1] * t1) * uv[1][q];
+        } else {           // t-U-V: Δ = diag(f(Δs) - 1, f(a Δs) - 1)
+            const double ds = ch.eta[sx] - ch.eta[si];
+            if (ch.is_complex) {
+                double s0, c0, s1, c1;
+                sincos(ds, &s0, &c0);
+                sincos(ch.a * ds, &s1, &c1);
+                Dr[0][0] = c0 - 1.0; Di[0][0] = s0;
+                Dr[1][1] = c1 - 1.0; Di[1][1] = s1;
+            } else {
+                Dr[0][0] = exp(ds) - 1.0;
+                Dr[1][1] = exp(ch.a * ds) - 1.0;
+                extra = exp((ch.eta[si] - ch.eta[sx]) * (ch.a + 1.0));
+            }
+        }
+        const int k = tid - 128;
+        dtab_r[k][0] = Dr[0][0]; dtab_r[k][1] = Dr[0][1]; dtab_r[k][2] = Dr[1][0]; dtab_r[k][3] = Dr[1][1];
+        dtab_i[k][0] = Di[0][0]; dtab_i[k][1] = Di[0][1]; dtab_i[k][2] = Di[1][0]; dtab_i[k][3] = Di[1][1];
+        ftab[k] = extra * (ch.gamma[sx] / ch.gamma[si]);
+    }
     if (tid < ne) {
         int chv;
         double u;
@@ -82,41 +117,15 @@ __global__ __launch_bounds__(256) void k_decide_r2(DecideR2Args a) {
     }
     __syncthreads();
 
-    const double h = 0.7071067811865476;   // 2^0.5 / 2
     int na = 0;
     for (int m = 0; m < ne; ++m) {
         const int si = sv[m];
         const int cidx = chs[m];
         const int sx = cidx + 1 + ((cidx + 1) >= si ? 1 : 0);
-        // Δ (2x2, complex)
+        const int k = (si - 1) * 4 + (sx - 1);
         double Dr[2][2], Di[2][2];
-        double extra = 1.0;
-        if (ch.kind == 1) {   // t-V: Δ = uv diag(e^{v}-1, e^{-v}-1) uv
-            const double v = ch.alpha[lt - 1] * (ch.eta[sx - 1] - ch.eta[si - 1]);
-            const double t0 = exp(v) - 1.0, t1 = exp(-v) - 1.0;
-            const double uv[2][2] = {{-h, -h}, {-h, h}};
-#pragma unroll
-            for (int p = 0; p < 2; ++p)
-#pragma unroll
-                for (int q = 0; q < 2; ++q) {
-                    Dr[p][q] = (uv[p][0] * t0) * uv[0][q] + (uv[p][1] * t1) * uv[1][q];
-                    Di[p][q] = 0.0;
-                }
-        } else {             // t-U-V: Δ = diag(f(Δs) - 1, f(a Δs) - 1)
-            const double ds = ch.eta[sx - 1] - ch.eta[si - 1];
-            if (ch.is_complex) {
-                double s0, c0, s1, c1;
-                sincos(ds, &s0, &c0);
-                sincos(ch.a * ds, &s1, &c1);
-                Dr[0][0] = c0 - 1.0; Di[0][0] = s0;
-                Dr[1][1] = c1 - 1.0; Di[1][1] = s1;
-            } else {
-                Dr[0][0] = exp(ds) - 1.0; Di[0][0] = 0.0;
-                Dr[1][1] = exp(ch.a * ds) - 1.0; Di[1][1] = 0.0;
-                extra = exp((ch.eta[si - 1] - ch.eta[sx - 1]) * (ch.a + 1.0));
-            }
-            Dr[0][1] = Dr[1][0] = Di[0][1] = Di[1][0] = 0.0;
-        }
+        Dr[0][0] = dtab_r[k][0]; Dr[0][1] = dtab_r[k][1]; Dr[1][0] = dtab_r[k][2]; Dr[1][1] = dtab_r[k][3];
+        Di[0][0] = dtab_i[k][0]; Di[0][1] = dtab_i[k][1]; Di[1][0] = dtab_i[k][2]; Di[1][1] = dtab_i[k][3];
         // r = I + Δ (I - G_ss)
         double Omr[2][2], Omi[2][2];
         Omr[0][0] = 1.0 - blkr[m][0]; Omi[0][0] = -blki[m][0];
@@ -139,10 +148,8 @@ __global__ __launch_bounds__(256) void k_decide_r2(DecideR2Args a) {
         cmul(Rr[0][1], Ri[0][1], Rr[1][0], Ri[1][0], tr, ti);
         detr -= tr;
         deti -= ti;
-        double prob;
-        if (ch.kind == 1) prob = detr;
-        else prob = (detr * detr + deti * deti) * extra;
-        prob *= ch.gamma[sx - 1] / ch.gamma[si - 1];
+        double prob = (kind == 1) ? detr : (detr * detr + deti * deti);
+        prob *= ftab[k];
         if (!(us[m] < prob)) continue;
         // w = r⁻¹ Δ
         const double d2 = detr * detr + deti * deti;

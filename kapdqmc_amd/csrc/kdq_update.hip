@@ -5,12 +5,18 @@
 // and each accepted flip applies the rank-1 update of Gupdate! (src/public/Gupdate.jl:5-11)
 //     G <- G - G[:,i] r (e_i - G[i,:]),   r = Δ / (1 + Δ (1 - G_ii)).
 // Here the sites are processed in blocks S of k consecutive sites:
-//   k_decide_tu (one workgroup per chain) loads G[S,S], runs the k sequential proposals
-//   on that k x k submatrix (kept in registers, rank-1 updates only touch S x S), and
-//   builds by bordering the k_a x k_a matrix W with
-//       G_new = G - G[:,A] W (E_A - G[A,:])          (A = accepted sites, in order)
-//   which equals the product of the sequential rank-1 updates exactly.
-//   k_flush applies that rank-|A| correction to the whole N x N matrix (all workgroups).
+//   k_decide_tu (one workgroup per chain) loads G[S,S] into registers, runs the k sequential
+//   proposals on that k x k working submatrix (rank-1 updates restricted to S x S) and
+//   builds the k_a x k_a factor W with
+//       G_new = G - G[:,A] W (E_A - G[A,:]),   W = Γ⁻¹,  Γ = diag(1/Δ_a) + I - G[A,A]
+//   (A = accepted sites in acceptance order), which equals the product of the sequential
+//   rank-1 updates. W grows by bordering: accepting site m with r = 1/(Γ's Schur complement)
+//       W' = [[W + x r y, x r], [r y, r]],   x = W G[A,m],  y = G[m,A] W.
+//   Both vectors follow from the current working matrix Gc without any product:
+//       Gc[A,m] = (I + G[A,A] W) G[A,m] = (D + I) x   ->  x_t = Δ_t/(1+Δ_t) · Gc[a_t, m]
+//       Gc[m,A] = G[m,A] (I - W (I - G[A,A])) = y D  ->  y_u = Δ_u · Gc[m, a_u]
+//   (D = diag(1/Δ_a); m is not in A because every site is visited once per slice).
+//   k_flush applies the rank-|A| correction to the whole N x N matrix (all workgroups).
 #include "kdq_kernels.h"
 
 namespace {
@@ -18,7 +24,16 @@ namespace {
 constexpr int KB = 64;   // maximum block size
 constexpr int LP = KB + 1;
 
-__global__ __launch_bounds__(256) void k_decide_tu(DecideArgs a) {
+// Layout sized to the block: KBT in {8, 16, 32, 64} (smallest >= k), NTH = min(1024, KBT²)
+// threads, thread (r = tid % KBT, cg = tid / KBT) owns row r, columns cg·CPT .. cg·CPT+CPT-1
+// (CPT = KBT²/NTH) of both the working submatrix and W. The accept path is bound by VALU
+// issue, so no wave works on padding: at k = 32 each of 1024 threads owns one entry.
+// Everything a proposal needs except the current diagonal (Δ, γ'/γ, Δ/(1+Δ), the uniform,
+// the new value) is computed for all k sites in parallel before the sequential loop.
+template <int KBT>
+__global__ __launch_bounds__(KBT >= 32 ? 1024 : KBT * KBT) void k_decide_tu(DecideArgs a) {
+    constexpr int NTH = KBT >= 32 ? 1024 : KBT * KBT;
+    constexpr int CPT = KBT * KBT / NTH;
     const int c = blockIdx.x;
     const ChainDev &ch = a.chains[c];
     const int n = ch.ns;
@@ -29,155 +44,138 @@ __global__ __launch_bounds__(256) void k_decide_tu(DecideArgs a) {
         if (tid == 0) *ch.acc_cnt = 0;
         return;
     }
-    __shared__ double G0r[KB][LP], G0i[KB][LP];   // original G[S,S]
-    __shared__ double Wr[KB][LP], Wi[KB][LP];     // Woodbury factor (a x a)
-    __shared__ double colr[KB], coli[KB], rowr[KB], rowi[KB];
-    __shared__ double dgr[KB], dgi[KB];
-    __shared__ double xr[KB], xi[KB], yr[KB], yi[KB];
-    __shared__ double us[KB];
-    __shared__ int chs[KB], accl[KB];
-    __shared__ int sv[KB];
-    __shared__ double dtr[4][4], dti[4][4];
+    __shared__ double colr[KBT], coli[KBT], rowr[KBT], rowi[KBT];   // published column / row
+    __shared__ double dgr[KBT], dgi[KBT];                           // current diagonal
+    __shared__ double dR[KBT], dI[KBT], pg[KBT], us[KBT];           // per site: Δ, γ'/γ, u
+    __shared__ double qxr[KBT], qxi[KBT];                           // per site: Δ/(1+Δ)
+    __shared__ int accl[KBT], sv[KBT];
 
     const Opnd G = a.G[c];
     const int lt = a.lt;
-    // stage G[S,S]
-    for (int e = tid; e < kb * kb; e += 256) {
-        const int r = e % kb, q = e / kb;
-        const size_t off = (size_t)(s0 + r) + (size_t)(s0 + q) * G.ld;
-        G0r[r][q] = G.re[off];
-        G0i[r][q] = G.im[off];
+    const int r = tid % KBT, c0 = (tid / KBT) * CPT;
+    double wr[CPT], wi[CPT], Wr[CPT], Wi[CPT];
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) {
+        const int cc = c0 + q;
+        const bool ok = r < kb && cc < kb;
+        const size_t off = (size_t)(s0 + r) + (size_t)(s0 + cc) * G.ld;
+        wr[q] = ok ? G.re[off] : 0.0;
+        wi[q] = ok ? G.im[off] : 0.0;
+        Wr[q] = 0.0;
+        Wi[q] = 0.0;
     }
     const uint32_t tag = (uint32_t)lt | ((uint32_t)a.dir << 31);
-    if (tid < kb) {
-        int chv;
-        double u;
-        proposal_draw(ch.seed, ch.chain_id, 0, a.sweep, tag, (uint32_t)(s0 + tid), chv, u);
-        chs[tid] = chv;
-        us[tid] = u;
-        sv[tid] = ch.s[(size_t)(s0 + tid) + (size_t)(lt - 1) * ch.n_elem];
+    if (tid < KBT) {
+        accl[tid] = 0;   // entries beyond the accepted count stay valid indices
+        if (tid < kb) {
+            int chv;
+            double u;
+            proposal_draw(ch.seed, ch.chain_id, 0, a.sweep, tag, (uint32_t)(s0 + tid), chv, u);
+            const int si = ch.s[(size_t)(s0 + tid) + (size_t)(lt - 1) * ch.n_elem];
+            const int sx = chv + 1 + ((chv + 1) >= si ? 1 : 0);
+            const double x = ch.alpha[lt - 1] * (ch.eta[sx - 1] - ch.eta[si - 1]);
+            double sn, cs;
+            sincos(x, &sn, &cs);
+            dR[tid] = cs - 1.0;
+            dI[tid] = sn;
+            pg[tid] = ch.gamma[sx - 1] / ch.gamma[si - 1];
+            // Δ/(1+Δ) = 1 - 1/(1+Δ), with 1+Δ = cis(x)
+            qxr[tid] = 1.0 - cs;
+            qxi[tid] = sn;
+            us[tid] = u;
+            sv[tid] = si | (sx << 8);
+        }
     }
-    if (tid < 16) {
-        const int si = tid >> 2, sx = tid & 3;
-        const double x = ch.alpha[lt - 1] * (ch.eta[sx] - ch.eta[si]);
-        double sn, cs;
-        sincos(x, &sn, &cs);
-        dtr[si][sx] = cs - 1.0;
-        dti[si][sx] = sn;
-    }
-    __syncthreads();
-    // working copy: thread owns row r, columns g*16 .. g*16+15
-    const int r = tid & 63, g = tid >> 6;
-    double wr[16], wi[16];
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        const int cc = g * 16 + q;
-        wr[q] = (r < kb && cc < kb) ? G0r[r][cc] : 0.0;
-        wi[q] = (r < kb && cc < kb) ? G0i[r][cc] : 0.0;
-    }
-    if (tid < kb) {
-        dgr[tid] = G0r[tid][tid];
-        dgi[tid] = G0i[tid][tid];
-    }
+    for (int q = 0; q < CPT; ++q)
+        if (r < kb && r == c0 + q) {
+            dgr[r] = wr[q];
+            dgi[r] = wi[q];
+        }
     __syncthreads();
 
     int na = 0;
     for (int m = 0; m < kb; ++m) {
-        const int si = sv[m];
-        const int cidx = chs[m];
-        const int sx = cidx + 1 + ((cidx + 1) >= si ? 1 : 0);
-        const double Dr = dtr[si - 1][sx - 1], Di = dti[si - 1][sx - 1];
+        const double Dr = dR[m], Di = dI[m];
         // p = 1 + Δ (1 - G_mm)
         const double omr = 1.0 - dgr[m], omi = -dgi[m];
         double pr, pi;
         cmul(Dr, Di, omr, omi, pr, pi);
         pr += 1.0;
         const double p2 = pr * pr + pi * pi;
-        const double prob = p2 * (ch.gamma[sx - 1] / ch.gamma[si - 1]);
+        const double prob = p2 * pg[m];
         if (!(us[m] < prob)) continue;
         // r = Δ / p
-        const double rr = (Dr * pr + Di * pi) / p2, ri = (Di * pr - Dr * pi) / p2;
-        // publish column m and row m of the working submatrix
-        if ((m >> 4) == g && r < kb) {
-            colr[r] = wr[m & 15];
-            coli[r] = wi[m & 15];
-        }
-        if (r == m) {
+        const double ip2 = 1.0 / p2;
+        const double rr = (Dr * pr + Di * pi) * ip2, ri = (Di * pr - Dr * pi) * ip2;
+        // publish column m and row m of the working submatrix (state before this flip)
 #pragma unroll
-            for (int q = 0; q < 16; ++q) {
-                rowr[g * 16 + q] = wr[q];
-                rowi[g * 16 + q] = wi[q];
+        for (int q = 0; q < CPT; ++q) {
+            if (r < kb && m == c0 + q) {
+                colr[r] = wr[q];
+                coli[r] = wi[q];
+            }
+            if (r == m) {
+                rowr[c0 + q] = wr[q];
+                rowi[c0 + q] = wi[q];
             }
         }
+        if (tid == 0) {
+            accl[na] = m;
+            sv[m] |= 1 << 16;   // accepted: the new value is written back
+        }
         __syncthreads();
-        // rank-1 update of the working submatrix
         if (r < kb) {
-            double fr, fi;
-            cmul(colr[r], coli[r], rr, ri, fr, fi);
+            // LDS operands first: x_r = Δ/(1+Δ) Gc[a_r, m], y_u = Δ_u Gc[m, a_u]
+            const double clr = colr[r], cli = coli[r];
+            const int ar_ = accl[r];
+            const double cxr = colr[ar_], cxi = coli[ar_], qr = qxr[ar_], qi = qxi[ar_];
+            double rwr[CPT], rwi[CPT], ayr[CPT], ayi[CPT];
 #pragma unroll
-            for (int q = 0; q < 16; ++q) {
-                const int cc = g * 16 + q;
-                const double vr = (cc == m ? 1.0 : 0.0) - rowr[cc], vi = -rowi[cc];
+            for (int q = 0; q < CPT; ++q) {
+                rwr[q] = rowr[c0 + q];
+                rwi[q] = rowi[c0 + q];
+                const int au = accl[c0 + q];
+                cmul(dR[au], dI[au], rowr[au], rowi[au], ayr[q], ayi[q]);
+            }
+            // rank-1 update of the working submatrix
+            double fr, fi;
+            cmul(clr, cli, rr, ri, fr, fi);
+#pragma unroll
+            for (int q = 0; q < CPT; ++q) {
+                const double vr = (c0 + q == m ? 1.0 : 0.0) - rwr[q], vi = -rwi[q];
                 double tr, ti;
                 cmul(fr, fi, vr, vi, tr, ti);
                 wr[q] -= tr;
                 wi[q] -= ti;
             }
-        }
-        // bordering vectors: x = W G0[A, m], y = G0[m, A] W
-        if (tid < na) {
-            double sr = 0.0, sim = 0.0;
-            for (int u = 0; u < na; ++u) {
+            // bordering of W: rows t < na get W + x r y and x r in the new column, row na
+            // gets r y and r
+            const bool old_row = r < na, new_row = r == na;
+            double xr, xi, xrr, xri;
+            cmul(qr, qi, cxr, cxi, xr, xi);
+            cmul(xr, xi, rr, ri, xrr, xri);
+            const double lr = old_row ? xrr : rr, li = old_row ? xri : ri;   // left factor
+#pragma unroll
+            for (int q = 0; q < CPT; ++q) {
+                const int u = c0 + q;
+                const double yr = (u < na) ? ayr[q] : 1.0, yi = (u < na) ? ayi[q] : 0.0;
                 double tr, ti;
-                cmul(Wr[tid][u], Wi[tid][u], G0r[accl[u]][m], G0i[accl[u]][m], tr, ti);
-                sr += tr;
-                sim += ti;
+                cmul(lr, li, yr, yi, tr, ti);
+                const bool keep = old_row && u < na;
+                const bool set = (old_row && u == na) || (new_row && u <= na);
+                Wr[q] = keep ? Wr[q] + tr : (set ? tr : Wr[q]);
+                Wi[q] = keep ? Wi[q] + ti : (set ? ti : Wi[q]);
             }
-            xr[tid] = sr;
-            xi[tid] = sim;
-        } else if (tid >= 128 && tid < 128 + na) {
-            const int u = tid - 128;
-            double sr = 0.0, sim = 0.0;
-            for (int t = 0; t < na; ++t) {
-                double tr, ti;
-                cmul(G0r[m][accl[t]], G0i[m][accl[t]], Wr[t][u], Wi[t][u], tr, ti);
-                sr += tr;
-                sim += ti;
-            }
-            yr[u] = sr;
-            yi[u] = sim;
+#pragma unroll
+            for (int q = 0; q < CPT; ++q)
+                if (r == c0 + q) {
+                    dgr[r] = wr[q];
+                    dgi[r] = wi[q];
+                }
         }
         __syncthreads();
-        // diagonal of the working matrix
-        if (r < kb && (r >> 4) == g) {
-            dgr[r] = wr[r & 15];
-            dgi[r] = wi[r & 15];
-        }
-        // W' = [[W + x r y, x r], [r y, r]]
-        const int na1 = na + 1;
-        for (int e = tid; e < na1 * na1; e += 256) {
-            const int t = e % na1, u = e / na1;
-            if (t < na && u < na) {
-                double tr, ti, vr2, vi2;
-                cmul(xr[t], xi[t], rr, ri, tr, ti);
-                cmul(tr, ti, yr[u], yi[u], vr2, vi2);
-                Wr[t][u] += vr2;
-                Wi[t][u] += vi2;
-            } else if (t < na) {
-                cmul(xr[t], xi[t], rr, ri, Wr[t][u], Wi[t][u]);
-            } else if (u < na) {
-                cmul(rr, ri, yr[u], yi[u], Wr[t][u], Wi[t][u]);
-            } else {
-                Wr[t][u] = rr;
-                Wi[t][u] = ri;
-            }
-        }
-        if (tid == 0) {
-            accl[na] = m;
-            sv[m] = sx;
-        }
-        __syncthreads();
-        na = na1;
+        ++na;
     }
     // outputs
     if (tid == 0) {
@@ -186,47 +184,74 @@ __global__ __launch_bounds__(256) void k_decide_tu(DecideArgs a) {
         ch.counters[1] += na;
     }
     if (tid < na) ch.acc_idx[tid] = s0 + accl[tid];
-    for (int e = tid; e < na * na; e += 256) {
-        const int t = e % na, u = e / na;
-        ch.W[t + u * KB] = Wr[t][u];
-        ch.W[KB * KB + t + u * KB] = Wi[t][u];
+    if (r < na) {
+#pragma unroll
+        for (int q = 0; q < CPT; ++q) {
+            const int u = c0 + q;
+            if (u < na) {
+                ch.W[r + u * KB] = Wr[q];
+                ch.W[KB * KB + r + u * KB] = Wi[q];
+            }
+        }
     }
-    if (tid < kb) ch.s[(size_t)(s0 + tid) + (size_t)(lt - 1) * ch.n_elem] = (uint8_t)sv[tid];
+    if (tid < kb) {
+        const int v = sv[tid];
+        ch.s[(size_t)(s0 + tid) + (size_t)(lt - 1) * ch.n_elem] = (uint8_t)((v >> 16) ? (v >> 8) & 0xff : v & 0xff);
+    }
 }
 
 // Gout[I,J] = Gin[I,J] - Gin[I,A] · (W · Y[:,J]),  Y[t, j] = δ(A_t, j) - Gin[A_t, j].
+// 32 x 32 tiles, 256 threads: thread (ii = tid & 31, jg = tid >> 5) owns row ii, columns
+// 4jg .. 4jg+3. LDS (dynamic, k = maximum block size): W (k x k), Y then Z (k x 32),
+// Gin[I,A] (32 x k).
+constexpr int FT = 32;
 __global__ __launch_bounds__(256) void k_flush(FlushArgs f) {
     const int c = blockIdx.z;
     const ChainDev &ch = f.chains[c];
     const int n = ch.ns;
-    const int i0 = blockIdx.y * 64, j0 = blockIdx.x * 64;
+    const int i0 = blockIdx.y * FT, j0 = blockIdx.x * FT;
     if (i0 >= n || j0 >= n) return;
     const Opnd Gi = f.Gin[c], Go = f.Gout[c];
-    const int na = *ch.acc_cnt;
     const int tid = threadIdx.x;
-    const int ii = tid & 63, jg = tid >> 6;
+    const int ii = tid & 31, jg = tid >> 5;
+    const int i = i0 + ii;
+    // this thread's 4 entries, fetched first so their latency overlaps the staging below
+    double gr[4], gim[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int j = j0 + jg * 4 + q;
+        const bool ok = i < n && j < n;
+        const size_t oi = (size_t)i + (size_t)j * Gi.ld;
+        gr[q] = ok ? Gi.re[oi] : 0.0;
+        gim[q] = ok ? Gi.im[oi] : 0.0;
+    }
+    const int na = *ch.acc_cnt;
     if (na == 0) {
-        for (int q = 0; q < 16; ++q) {
-            const int i = i0 + ii, j = j0 + jg * 16 + q;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int j = j0 + jg * 4 + q;
             if (i < n && j < n) {
-                Go.re[(size_t)i + (size_t)j * Go.ld] = Gi.re[(size_t)i + (size_t)j * Gi.ld];
-                Go.im[(size_t)i + (size_t)j * Go.ld] = Gi.im[(size_t)i + (size_t)j * Gi.ld];
+                Go.re[(size_t)i + (size_t)j * Go.ld] = gr[q];
+                Go.im[(size_t)i + (size_t)j * Go.ld] = gim[q];
             }
         }
         return;
     }
-    __shared__ double Ar[KB][LP], Ai[KB][LP];   // W, later Gin[I, A]
-    __shared__ double Pr[KB][LP], Pi[KB][LP];   // Y[:, J], later Z
+    const int K = f.kmax, KP = K + 1;
+    extern __shared__ double lds[];
+    double *Ar = lds, *Ai = Ar + (size_t)K * KP;             // W [t][u], stride KP
+    double *Pr = Ai + (size_t)K * KP, *Pi = Pr + K * (FT + 1);  // Y, then Z [t][jj]
+    double *Br = Pi + K * (FT + 1), *Bi = Br + FT * KP;        // Gin[I, A] [ii][t]
     __shared__ int aidx[KB];
     if (tid < na) aidx[tid] = ch.acc_idx[tid];
     for (int e = tid; e < na * na; e += 256) {
         const int t = e % na, u = e / na;
-        Ar[t][u] = ch.W[t + u * KB];
-        Ai[t][u] = ch.W[KB * KB + t + u * KB];
+        Ar[t * KP + u] = ch.W[t + u * KB];
+        Ai[t * KP + u] = ch.W[KB * KB + t + u * KB];
     }
     __syncthreads();
-    for (int e = tid; e < na * 64; e += 256) {
-        const int jj = e & 63, t = e >> 6;
+    for (int e = tid; e < na * FT; e += 256) {
+        const int jj = e & (FT - 1), t = e / FT;
         const int j = j0 + jj;
         double yr = 0.0, yi = 0.0;
         if (j < n) {
@@ -234,22 +259,37 @@ __global__ __launch_bounds__(256) void k_flush(FlushArgs f) {
             yr = (aidx[t] == j ? 1.0 : 0.0) - Gi.re[off];
             yi = -Gi.im[off];
         }
-        Pr[t][jj] = yr;
-        Pi[t][jj] = yi;
+        Pr[t * (FT + 1) + jj] = yr;
+        Pi[t * (FT + 1) + jj] = yi;
+    }
+    for (int e = tid; e < na * FT; e += 256) {
+        const int r = e & (FT - 1), t = e / FT;
+        const int ig = i0 + r;
+        double vr = 0.0, vi = 0.0;
+        if (ig < n) {
+            const size_t off = (size_t)ig + (size_t)aidx[t] * Gi.ld;
+            vr = Gi.re[off];
+            vi = Gi.im[off];
+        }
+        Br[r * KP + t] = vr;
+        Bi[r * KP + t] = vi;
     }
     __syncthreads();
-    // Z = W · Y  (thread: column jj = ii, rows t = jg + 4 q)
-    double zr[16], zi[16];
+    // Z = W · Y: outputs (t, jj) spread over the threads, kept in registers until all
+    // reads of Y are done
+    constexpr int ZMAX = (KB * FT) / 256;
+    double zr[ZMAX], zi[ZMAX];
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        const int t = jg + 4 * q;
+    for (int q = 0; q < ZMAX; ++q) {
+        const int e = tid + 256 * q;
+        const int jj = e & (FT - 1), t = e / FT;
         double sr = 0.0, sim = 0.0;
         if (t < na) {
             for (int u = 0; u < na; ++u) {
-                double tr, ti;
-                cmul(Ar[t][u], Ai[t][u], Pr[u][ii], Pi[u][ii], tr, ti);
-                sr += tr;
-                sim += ti;
+                const double ar = Ar[t * KP + u], ai = Ai[t * KP + u];
+                const double br = Pr[u * (FT + 1) + jj], bi = Pi[u * (FT + 1) + jj];
+                sr += ar * br - ai * bi;
+                sim += ar * bi + ai * br;
             }
         }
         zr[q] = sr;
@@ -257,43 +297,32 @@ __global__ __launch_bounds__(256) void k_flush(FlushArgs f) {
     }
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        const int t = jg + 4 * q;
+    for (int q = 0; q < ZMAX; ++q) {
+        const int e = tid + 256 * q;
+        const int jj = e & (FT - 1), t = e / FT;
         if (t < na) {
-            Pr[t][ii] = zr[q];
-            Pi[t][ii] = zi[q];
+            Pr[t * (FT + 1) + jj] = zr[q];
+            Pi[t * (FT + 1) + jj] = zi[q];
         }
-    }
-    // Gin[I, A] into the W space
-    for (int e = tid; e < na * 64; e += 256) {
-        const int r = e & 63, t = e >> 6;
-        const int i = i0 + r;
-        double vr = 0.0, vi = 0.0;
-        if (i < n) {
-            const size_t off = (size_t)i + (size_t)aidx[t] * Gi.ld;
-            vr = Gi.re[off];
-            vi = Gi.im[off];
-        }
-        Ar[r][t] = vr;
-        Ai[r][t] = vi;
     }
     __syncthreads();
-    const int i = i0 + ii;
     if (i >= n) return;
-#pragma unroll 4
-    for (int q = 0; q < 16; ++q) {
-        const int jj = jg * 16 + q, j = j0 + jj;
-        if (j >= n) break;
-        double sr = 0.0, sim = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int jj = jg * 4 + q;
+        double sr = gr[q], sim = gim[q];
         for (int t = 0; t < na; ++t) {
-            double tr, ti;
-            cmul(Ar[ii][t], Ai[ii][t], Pr[t][jj], Pi[t][jj], tr, ti);
-            sr += tr;
-            sim += ti;
+            const double ar = Br[ii * KP + t], ai = Bi[ii * KP + t];
+            const double pr = Pr[t * (FT + 1) + jj], pi = Pi[t * (FT + 1) + jj];
+            sr -= ar * pr - ai * pi;
+            sim -= ar * pi + ai * pr;
         }
-        const size_t oi = (size_t)i + (size_t)j * Gi.ld, oo = (size_t)i + (size_t)j * Go.ld;
-        Go.re[oo] = Gi.re[oi] - sr;
-        Go.im[oo] = Gi.im[oi] - sim;
+        const int j = j0 + jj;
+        if (j < n) {
+            const size_t oo = (size_t)i + (size_t)j * Go.ld;
+            Go.re[oo] = sr;
+            Go.im[oo] = sim;
+        }
     }
 }
 
@@ -301,12 +330,21 @@ __global__ __launch_bounds__(256) void k_flush(FlushArgs f) {
 
 hipError_t launch_decide_tu(hipStream_t st, int nchains, const DecideArgs &a) {
     if (a.kb < 1 || a.kb > KB) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_decide_tu, dim3(nchains), dim3(256), 0, st, a);
+    if (a.kb <= 8) hipLaunchKernelGGL(k_decide_tu<8>, dim3(nchains), dim3(64), 0, st, a);
+    else if (a.kb <= 16) hipLaunchKernelGGL(k_decide_tu<16>, dim3(nchains), dim3(256), 0, st, a);
+    else if (a.kb <= 32) hipLaunchKernelGGL(k_decide_tu<32>, dim3(nchains), dim3(1024), 0, st, a);
+    else hipLaunchKernelGGL(k_decide_tu<64>, dim3(nchains), dim3(1024), 0, st, a);
     return hipGetLastError();
 }
 
 hipError_t launch_flush(hipStream_t st, int nchains, int max_ns, const FlushArgs &f) {
-    const int t = (max_ns + 63) / 64;
-    hipLaunchKernelGGL(k_flush, dim3(t, t, nchains), dim3(256), 0, st, f);
+    if (f.kmax < 1 || f.kmax > KB) return hipErrorInvalidValue;
+    const int t = (max_ns + FT - 1) / FT;
+    const size_t K = (size_t)f.kmax, KP = K + 1;
+    const size_t bytes = (2 * K * KP + 2 * K * (FT + 1) + 2 * FT * KP) * sizeof(double);
+    if (hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_flush),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes))
+        return e;
+    hipLaunchKernelGGL(k_flush, dim3(t, t, nchains), dim3(256), bytes, st, f);
     return hipGetLastError();
 }
