@@ -1,14 +1,18 @@
 #!/usr/bin/env python
-"""Benchmark: projector-DQMC t-U sweep on one MI355X.
+"""Benchmark: projector-DQMC t-U sweeps on one MI355X.
 
-Workload (the plain FP64 GPU sweep of the drop-in path): half-filled t-U Hubbard model on
-a 12 x 12 honeycomb lattice (HoneyComb120, N = 288, Np = 144), U/t = 4, Δτ = 0.1,
-2Θ = 20 (Nt = 200 slices), stabilisation every 10 slices, one Markov chain, FP64
-everywhere, local updates as delayed rank-k blocks (k = --delay), record = true (40
-measurements per sweep as in the reference). A step is one sweep (forward + backward).
+Workload: half-filled t-U Hubbard model on a 12 x 12 honeycomb lattice (HoneyComb120,
+N = 288, Np = 144), U/t = 4, Δτ = 0.1, 2Θ = 20 (Nt = 200 slices), stabilisation every 10
+slices, FP64, local updates as delayed rank-k blocks, record = true (40 measurements per
+sweep as in the reference). A step is one sweep (forward + backward) of every chain.
 
-value = effective independent samples per second of the double occupancy D = Eu/N:
-(sweeps/s) / (2 τ_int), τ_int by Sokal windowing over the timed sweeps.
+Measured configuration (value): --chains independent Markov chains (seeds 1..C) advanced
+together, every kernel launch covering all chains. Reported beside it: the plain
+single-chain FP64 sweep (the drop-in path one reference call corresponds to).
+
+value = effective independent samples per second of the double occupancy D = Eu/N,
+summed over chains: C · (sweeps/s) / (2 τ_int), τ_int from the autocovariance of all
+chains pooled (Sokal windowing).
 """
 from __future__ import annotations
 
@@ -80,12 +84,45 @@ def cpu_baseline(m, tau, nslices: int):
                        f"({sweeps_s:.4f} sweeps/s), ESS/s with the GPU chain's tau_int={tau:.2f}")}
 
 
+def run_batch(m, nch: int, seed0: int, delay: int, warmup: int, steps: int, profile_all: bool):
+    """Warm-up with every kernel family timed, then `steps` timed sweeps with the dominant
+    family (or all) timed by HIP events on the handle's stream."""
+    import numpy as np
+    import torch
+    from kapdqmc_amd.dqmc import DQMCBatch
+    from kapdqmc_amd.rng import initial_fields
+    N = m.Ns
+    b = DQMCBatch([m] * nch, [seed0 + c for c in range(nch)], list(range(nch)), delay=delay)
+    for c in range(nch):
+        b.set_fields(c, initial_fields(seed0 + c, c, N, m.Nt))
+    fams_all = ["wrap", "chain", "get_G", "decide", "flush", "qr", "inv", "meas_wrap", "measure"]
+    b.profile(fams_all)
+    b.phy_update(max(warmup, 1), True)
+    prof_w = b.profile_read()
+    dominant = max(prof_w, key=lambda f: prof_w[f][0])
+    b.profile(list(prof_w) if profile_all else [dominant])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rows = b.phy_update(steps, True)          # [nch, steps, nobs]
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    prof = b.profile_read()
+    cnt = b.counters()
+    breakdown = {f: {"ms_per_sweep": prof_w[f][0] / max(warmup, 1),
+                     "launches_per_sweep": prof_w[f][1] / max(warmup, 1)} for f in prof_w}
+    return {"rows": np.asarray(rows), "dt": dt, "prof": prof, "dominant": dominant,
+            "breakdown": breakdown, "acceptance": float(cnt[:, 1].sum() / max(cnt[:, 0].sum(), 1))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--chains", type=int, default=64, help="independent chains batched per launch")
     ap.add_argument("--delay", type=int, default=32)
+    ap.add_argument("--plain-steps", type=int, default=6, help="sweeps of the single-chain line (0: skip)")
+    ap.add_argument("--plain-delay", type=int, default=16)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-slices", type=int, default=20)
@@ -103,37 +140,28 @@ def main():
         sys.exit(1)
     torch.cuda.set_device(0)
     from kapdqmc_amd import stats
-    from kapdqmc_amd.dqmc import DQMCBatch
-    from kapdqmc_amd.rng import initial_fields
 
     m = model_cfg2()
     N, Np = m.Ns, m.Ns // 2
-    b = DQMCBatch([m], [args.seed], [0], delay=args.delay)
-    b.set_fields(0, initial_fields(args.seed, 0, N, m.Nt))
-
-    # warm-up with every family timed, to find the dominant kernel family
-    b.profile(["wrap", "chain", "get_G", "decide", "flush", "qr", "inv", "meas_wrap", "measure"])
-    b.phy_update(max(args.warmup, 1), True)
-    prof_w = b.profile_read()
-    dominant = max(prof_w, key=lambda f: prof_w[f][0])
-
-    fams = list(prof_w) if args.profile_all else [dominant]
-    b.profile(fams)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    rows = b.phy_update(args.steps, True)[0]
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    prof = b.profile_read()
-    dt = t1 - t0
-    sweeps_s = args.steps / dt
-    D = rows[:, 1] / N
-    tau = stats.tau_int(D)
+    C = args.chains
+    bat = run_batch(m, C, args.seed, args.delay, args.warmup, args.steps, args.profile_all)
+    D = bat["rows"][:, :, 1] / N                     # [C, steps]
+    tau = stats.tau_int_chains(D)
+    sweeps_s = C * args.steps / bat["dt"]            # chain-sweeps per second
     ess_s = sweeps_s / (2 * tau)
-    cnt = b.counters()[0]
 
-    ms, nl = prof[dominant]
+    plain = None
+    if args.plain_steps > 0:
+        one = run_batch(m, 1, args.seed, args.plain_delay, 1, args.plain_steps, False)
+        p_sweeps = args.plain_steps / one["dt"]
+        plain = {"sweeps_per_s": p_sweeps, "ms_per_sweep": one["dt"] / args.plain_steps * 1e3,
+                 "effective_samples_per_s": p_sweeps / (2 * tau), "delay_k": args.plain_delay,
+                 "note": "same Markov chain, so the batched run's tau_int is used"}
+
+    dominant = bat["dominant"]
+    ms, nl = bat["prof"][dominant]
     bound, work, unit = family_work(dominant, N, Np)
+    work *= C                                        # one launch covers every chain
     avg_s = ms / 1e3 / max(nl, 1)
     if bound == "mfma":
         achieved, peak, runit = work / avg_s / 1e12, PEAK_FP64_TFLOPS, "TFLOP/s"
@@ -151,8 +179,6 @@ def main():
         except Exception as e:  # the baseline is reported, never required
             cpu = {"error": repr(e)}
 
-    breakdown = {f: {"ms_per_sweep": prof_w[f][0] / max(args.warmup, 1),
-                     "launches_per_sweep": prof_w[f][1] / max(args.warmup, 1)} for f in prof_w}
     out = {
         "metric": "effective samples/s at N=288 t-U: FP64 plain vs mixed+batched+SLMC, equal error",
         "value": ess_s,
@@ -160,22 +186,25 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": dt / args.steps * 1e3,
+        "ms_per_step": bat["dt"] / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (Philox-seeded HS fields; model constants from the reference formulas)",
-        "config": {"workload": "t-U honeycomb 12x12 N=288 Np=144 U=4 dtau=0.1 2Theta=20 Nt=200 BS=10, "
-                               "plain FP64 sweep, 1 chain, record=true",
-                   "delay_k": args.delay, "variant": "plain FP64 (mixed+batched+SLMC not built yet)"},
+        "config": {"workload": f"t-U honeycomb 12x12 N=288 Np=144 U=4 dtau=0.1 2Theta=20 Nt=200 BS=10, "
+                               f"{C} independent chains batched per launch, FP64, record=true",
+                   "chains": C, "delay_k": args.delay,
+                   "variant": "batched FP64 (mixed precision and SLMC moves not in this line)"},
         "sweeps_per_s": sweeps_s,
         "tau_int_double_occupancy": tau,
         "double_occupancy_mean": float(D.mean()),
-        "acceptance": float(cnt[1] / max(cnt[0], 1)),
+        "acceptance": bat["acceptance"],
+        "plain_fp64_1chain": plain,
+        "ratio_vs_plain": (ess_s / plain["effective_samples_per_s"]) if plain else None,
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "kernel_breakdown_warmup": breakdown,
+        "kernel_breakdown_warmup": bat["breakdown"],
     }
     print(json.dumps(out))
 

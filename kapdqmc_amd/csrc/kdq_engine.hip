@@ -197,7 +197,7 @@ int maxdim(kdq_handle h, int code) { return code == DIM_NS ? h->max_ns : h->max_
 
 int gemm(kdq_handle h, bool ac, bool bc, int opa, int opb, const GemmArgs &g, int fam = KDQ_PROF_GEMM_OTHER) {
     ProfScope ps(h, fam);
-    HIPCHK(launch_gemm(h->stream, h->nch, maxdim(h, g.mdim), maxdim(h, g.ndim), ac, bc, opa, opb, g));
+    HIPCHK(launch_gemm(h->stream, h->nch, maxdim(h, g.mdim), maxdim(h, g.ndim), maxdim(h, g.kdim), ac, bc, opa, opb, g));
     return KDQ_OK;
 }
 

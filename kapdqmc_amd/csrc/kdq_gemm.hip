@@ -1,23 +1,29 @@
 // Batched planar-complex GEMM on CDNA4 FP64 matrix cores (v_mfma_f64_16x16x4f64).
 //
-// Tile 64 x 64 x 16, 256 threads = 4 wave64s in a 2 x 2 arrangement, each wave owning a
-// 32 x 32 block of C as 2 x 2 MFMA 16x16 blocks. Operands are staged through LDS as
-// k-major tiles. The MFMA is fed with op(B) as its first operand and op(A) as its second,
-// so each lane's four accumulator registers hold four *columns* of one row and every store
-// instruction writes 16 consecutive rows: coalesced stores into the column-major output.
+// Tile BM x BN x 16 (BM = BN = 64 or 32), 256 threads = 4 wave64s in a 2 x 2 arrangement,
+// each wave owning a (BM/2) x (BN/2) block of C as MFMA 16x16 blocks. Operands are staged
+// through double-buffered LDS as k-major tiles; the next k-tile is fetched into registers
+// while the current one is multiplied. The MFMA is fed with op(B) as its first operand and
+// op(A) as its second, so each lane's four accumulator registers hold four *columns* of one
+// row and every store instruction writes 16 consecutive rows of the column-major output.
 //
 // Diagonal factors of the projector-DQMC algebra are fused: the HS phases of a time slice
 // (cis(±α η_s)) scale rows/columns in the epilogue or the contraction index while the
 // complex operand is staged, so WrapKV! (src/tU/GreenMatrix.jl:56-86) is two launches.
+// The factors are evaluated once per workgroup into LDS (rows, columns, whole K range).
 #include "kdq_kernels.h"
 
 namespace {
 
-constexpr int BM = 64, BN = 64, BK = 16, LDP = 64 + 2;
+constexpr int BK = 16;
+constexpr int KMAX = 512;   // longest contraction with a diagonal factor (N <= 512)
 
-template <int AC, int BC, int OPA, int OPB>
+template <int AC, int BC, int OPA, int OPB, int BM, int BN>
 __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
     constexpr bool OC = AC || BC;
+    constexpr int WMB = BM / 32, WNB = BN / 32;   // 16x16 MFMA blocks per wave (each dim)
+    constexpr int LPA = BM + 2, LPB = BN + 2;
+    constexpr int EA = (BM * BK) / 256, EB = (BN * BK) / 256;   // staged elements per thread
     const int c = blockIdx.z;
     const ChainDev &ch = g.chains[c];
     const int M = dim_of(ch, g.mdim), N = dim_of(ch, g.ndim), K = dim_of(ch, g.kdim);
@@ -25,39 +31,45 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
     if (i0 >= M || j0 >= N) return;
     const Opnd A = g.A[c], B = g.B[c];
 
-    __shared__ double As[AC ? 2 : 1][BK][LDP];
-    __shared__ double Bs[BC ? 2 : 1][BK][LDP];
-    __shared__ double Sk[2][BK];
+    __shared__ double As[2][AC ? 2 : 1][BK][LPA];
+    __shared__ double Bs[2][BC ? 2 : 1][BK][LPB];
+    __shared__ double Sk[2][KMAX];
+    __shared__ double Sr[2][BM], Sc[2][BN];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
-    v4d accR[2][2], accI[2][2];
+    const bool scale_k = g.inS.kind != 0;
+    // diagonal factors, once per workgroup
+    if (scale_k)
+        for (int k = tid; k < K; k += 256) diag_value(ch, g.inS, k, Sk[0][k], Sk[1][k]);
+    if (g.rowS.kind && tid < BM) {
+        if (i0 + tid < M) diag_value(ch, g.rowS, i0 + tid, Sr[0][tid], Sr[1][tid]);
+        else { Sr[0][tid] = 1.0; Sr[1][tid] = 0.0; }
+    }
+    if (g.colS.kind && tid >= 128 && tid < 128 + BN) {
+        const int t = tid - 128;
+        if (j0 + t < N) diag_value(ch, g.colS, j0 + t, Sc[0][t], Sc[1][t]);
+        else { Sc[0][t] = 1.0; Sc[1][t] = 0.0; }
+    }
+
+    v4d accR[WMB][WNB], accI[WMB][WNB];
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < WMB; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) {
+        for (int b = 0; b < WNB; ++b) {
             accR[a][b] = v4d{0, 0, 0, 0};
             accI[a][b] = v4d{0, 0, 0, 0};
         }
-    const bool scale_k = g.inS.kind != 0;
 
-    for (int k0 = 0; k0 < K; k0 += BK) {
-        if (scale_k) {
-            if (tid < BK) {
-                double sr = 1.0, si = 0.0;
-                if (k0 + tid < K) diag_value(ch, g.inS, k0 + tid, sr, si);
-                Sk[0][tid] = sr;
-                Sk[1][tid] = si;
-            }
-            __syncthreads();
-        }
-        // ---- stage op(A)[i0:i0+64, k0:k0+16] as As[k][i]
+    // register staging of one k-tile: element e = tid + 256 q of op(A)[BM x BK], op(B)[BK x BN]
+    double ra[EA], rai[EA], rb[EB], rbi[EB];
+    auto fetch = [&](int k0) {
 #pragma unroll
-        for (int q = 0; q < (BM * BK) / 256; ++q) {
+        for (int q = 0; q < EA; ++q) {
             const int e = tid + q * 256;
             int ii, kk;
-            if (OPA == 0) { ii = e & 63; kk = e >> 6; }
-            else { kk = e & 15; ii = e >> 4; }
+            if (OPA == 0) { ii = e % BM; kk = e / BM; }
+            else { kk = e % BK; ii = e / BK; }
             const int gi = i0 + ii, gk = k0 + kk;
             double vr = 0.0, vi = 0.0;
             if (gi < M && gk < K) {
@@ -68,17 +80,15 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
                     if (OPA == 2) vi = -vi;
                 }
             }
-            if (AC && scale_k) cmul(vr, vi, Sk[0][kk], Sk[1][kk], vr, vi);
-            As[0][kk][ii] = vr;
-            if (AC) As[1][kk][ii] = vi;
+            ra[q] = vr;
+            rai[q] = vi;
         }
-        // ---- stage op(B)[k0:k0+16, j0:j0+64] as Bs[k][j]
 #pragma unroll
-        for (int q = 0; q < (BN * BK) / 256; ++q) {
+        for (int q = 0; q < EB; ++q) {
             const int e = tid + q * 256;
             int jj, kk;
-            if (OPB == 0) { kk = e & 15; jj = e >> 4; }
-            else { jj = e & 63; kk = e >> 6; }
+            if (OPB == 0) { kk = e % BK; jj = e / BK; }
+            else { jj = e % BN; kk = e / BN; }
             const int gj = j0 + jj, gk = k0 + kk;
             double vr = 0.0, vi = 0.0;
             if (gj < N && gk < K) {
@@ -89,26 +99,61 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
                     if (OPB == 2) vi = -vi;
                 }
             }
-            if (BC && !AC && scale_k) cmul(vr, vi, Sk[0][kk], Sk[1][kk], vr, vi);
-            Bs[0][kk][jj] = vr;
-            if (BC) Bs[1][kk][jj] = vi;
+            rb[q] = vr;
+            rbi[q] = vi;
         }
-        __syncthreads();
+    };
+    auto stash = [&](int buf, int k0) {
+#pragma unroll
+        for (int q = 0; q < EA; ++q) {
+            const int e = tid + q * 256;
+            int ii, kk;
+            if (OPA == 0) { ii = e % BM; kk = e / BM; }
+            else { kk = e % BK; ii = e / BK; }
+            double vr = ra[q], vi = rai[q];
+            if (AC && scale_k && k0 + kk < K) cmul(vr, vi, Sk[0][k0 + kk], Sk[1][k0 + kk], vr, vi);
+            As[buf][0][kk][ii] = vr;
+            if (AC) As[buf][1][kk][ii] = vi;
+        }
+#pragma unroll
+        for (int q = 0; q < EB; ++q) {
+            const int e = tid + q * 256;
+            int jj, kk;
+            if (OPB == 0) { kk = e % BK; jj = e / BK; }
+            else { jj = e % BN; kk = e / BN; }
+            double vr = rb[q], vi = rbi[q];
+            if (BC && !AC && scale_k && k0 + kk < K) cmul(vr, vi, Sk[0][k0 + kk], Sk[1][k0 + kk], vr, vi);
+            Bs[buf][0][kk][jj] = vr;
+            if (BC) Bs[buf][1][kk][jj] = vi;
+        }
+    };
+
+    fetch(0);
+    __syncthreads();   // diagonal factors visible
+    stash(0, 0);
+    __syncthreads();
+    int buf = 0;
+    for (int k0 = 0; k0 < K; k0 += BK) {
+        const bool more = k0 + BK < K;
+        if (more) fetch(k0 + BK);
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 4) {
             const int kr = kk + (lane >> 4), cc = lane & 15;
-            double ar[2], ai[2], br[2], bi[2];
+            double ar[WMB], ai[WMB], br[WNB], bi[WNB];
 #pragma unroll
-            for (int t = 0; t < 2; ++t) {
-                ar[t] = As[0][kr][wm * 32 + t * 16 + cc];
-                ai[t] = AC ? As[1][kr][wm * 32 + t * 16 + cc] : 0.0;
-                br[t] = Bs[0][kr][wn * 32 + t * 16 + cc];
-                bi[t] = BC ? Bs[1][kr][wn * 32 + t * 16 + cc] : 0.0;
+            for (int t = 0; t < WMB; ++t) {
+                ar[t] = As[buf][0][kr][wm * (BM / 2) + t * 16 + cc];
+                ai[t] = AC ? As[buf][AC ? 1 : 0][kr][wm * (BM / 2) + t * 16 + cc] : 0.0;
             }
 #pragma unroll
-            for (int mi = 0; mi < 2; ++mi)
+            for (int t = 0; t < WNB; ++t) {
+                br[t] = Bs[buf][0][kr][wn * (BN / 2) + t * 16 + cc];
+                bi[t] = BC ? Bs[buf][BC ? 1 : 0][kr][wn * (BN / 2) + t * 16 + cc] : 0.0;
+            }
 #pragma unroll
-                for (int ni = 0; ni < 2; ++ni) {
+            for (int mi = 0; mi < WMB; ++mi)
+#pragma unroll
+                for (int ni = 0; ni < WNB; ++ni) {
                     accR[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(br[ni], ar[mi], accR[mi][ni], 0, 0, 0);
                     if (AC && BC) {
                         accR[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(-bi[ni], ai[mi], accR[mi][ni], 0, 0, 0);
@@ -121,7 +166,9 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
                     }
                 }
         }
+        if (more) stash(buf ^ 1, k0 + BK);
         __syncthreads();
+        buf ^= 1;
     }
 
     // ---- epilogue. For v_mfma_f64_16x16x4f64 lane l, register v holds D[(l/16) + 4v][l%16]
@@ -131,24 +178,23 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
     Opnd Ci;
     if (g.mode == 2) Ci = g.Cin[c];
 #pragma unroll
-    for (int mi = 0; mi < 2; ++mi) {
-        const int i = i0 + wm * 32 + mi * 16 + (lane & 15);
+    for (int mi = 0; mi < WMB; ++mi) {
+        const int il = wm * (BM / 2) + mi * 16 + (lane & 15), i = i0 + il;
         if (i >= M) continue;
         double rsr = 1.0, rsi = 0.0;
-        if (g.rowS.kind) diag_value(ch, g.rowS, i, rsr, rsi);
+        if (g.rowS.kind) {
+            rsr = Sr[0][il];
+            rsi = Sr[1][il];
+        }
 #pragma unroll
-        for (int ni = 0; ni < 2; ++ni)
+        for (int ni = 0; ni < WNB; ++ni)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int j = j0 + wn * 32 + ni * 16 + (lane >> 4) + 4 * v;
+                const int jl = wn * (BN / 2) + ni * 16 + (lane >> 4) + 4 * v, j = j0 + jl;
                 if (j >= N) continue;
                 double cr = accR[mi][ni][v], ci = OC ? accI[mi][ni][v] : 0.0;
                 if (g.rowS.kind) cmul(cr, ci, rsr, rsi, cr, ci);
-                if (g.colS.kind) {
-                    double csr, csi;
-                    diag_value(ch, g.colS, j, csr, csi);
-                    cmul(cr, ci, csr, csi, cr, ci);
-                }
+                if (g.colS.kind) cmul(cr, ci, Sc[0][jl], Sc[1][jl], cr, ci);
                 const size_t off = (size_t)i + (size_t)j * Cm.ld;
                 if (g.mode == 1) {
                     cr = (i == j ? 1.0 : 0.0) - cr;
@@ -165,26 +211,34 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
 }
 
 template <int AC, int BC, int OPA, int OPB>
-hipError_t run(hipStream_t st, int nchains, int max_m, int max_n, const GemmArgs &g) {
-    dim3 grid((max_n + BN - 1) / BN, (max_m + BM - 1) / BM, nchains);
-    hipLaunchKernelGGL((k_gemm<AC, BC, OPA, OPB>), grid, dim3(256), 0, st, g);
+hipError_t run(hipStream_t st, int nchains, int max_m, int max_n, int max_k, const GemmArgs &g) {
+    if (g.inS.kind && max_k > KMAX) return hipErrorInvalidValue;
+    // 64 x 64 tiles unless that leaves most compute units idle
+    const long t64 = (long)((max_n + 63) / 64) * ((max_m + 63) / 64) * nchains;
+    if (t64 >= 192) {
+        dim3 grid((max_n + 63) / 64, (max_m + 63) / 64, nchains);
+        hipLaunchKernelGGL((k_gemm<AC, BC, OPA, OPB, 64, 64>), grid, dim3(256), 0, st, g);
+    } else {
+        dim3 grid((max_n + 31) / 32, (max_m + 31) / 32, nchains);
+        hipLaunchKernelGGL((k_gemm<AC, BC, OPA, OPB, 32, 32>), grid, dim3(256), 0, st, g);
+    }
     return hipGetLastError();
 }
 
 }  // namespace
 
-hipError_t launch_gemm(hipStream_t st, int nchains, int max_m, int max_n, bool a_cplx, bool b_cplx,
+hipError_t launch_gemm(hipStream_t st, int nchains, int max_m, int max_n, int max_k, bool a_cplx, bool b_cplx,
                        int opa, int opb, const GemmArgs &g) {
     const int code = (a_cplx ? 1000 : 0) + (b_cplx ? 100 : 0) + opa * 10 + opb;
     switch (code) {
-        case 100: return run<0, 1, 0, 0>(st, nchains, max_m, max_n, g);     // eK · G
-        case 110: return run<0, 1, 1, 0>(st, nchains, max_m, max_n, g);     // eKᵀ · X
-        case 1000: return run<1, 0, 0, 0>(st, nchains, max_m, max_n, g);    // T · eKinv
-        case 1100: return run<1, 1, 0, 0>(st, nchains, max_m, max_n, g);    // U_R · X
-        case 1120: return run<1, 1, 2, 0>(st, nchains, max_m, max_n, g);    // U_Lᴴᴴ · U_R
-        case 1102: return run<1, 1, 0, 2>(st, nchains, max_m, max_n, g);    // M⁻¹ · U_Lᴴᴴ
-        case 0: return run<0, 0, 0, 0>(st, nchains, max_m, max_n, g);
-        case 10: return run<0, 0, 1, 0>(st, nchains, max_m, max_n, g);
+        case 100: return run<0, 1, 0, 0>(st, nchains, max_m, max_n, max_k, g);     // eK · G
+        case 110: return run<0, 1, 1, 0>(st, nchains, max_m, max_n, max_k, g);     // eKᵀ · X
+        case 1000: return run<1, 0, 0, 0>(st, nchains, max_m, max_n, max_k, g);    // T · eKinv
+        case 1100: return run<1, 1, 0, 0>(st, nchains, max_m, max_n, max_k, g);    // U_R · X
+        case 1120: return run<1, 1, 2, 0>(st, nchains, max_m, max_n, max_k, g);    // U_Lᴴᴴ · U_R, Uᴴ U
+        case 1102: return run<1, 1, 0, 2>(st, nchains, max_m, max_n, max_k, g);
+        case 0: return run<0, 0, 0, 0>(st, nchains, max_m, max_n, max_k, g);
+        case 10: return run<0, 0, 1, 0>(st, nchains, max_m, max_n, max_k, g);
         default: return hipErrorInvalidValue;
     }
 }

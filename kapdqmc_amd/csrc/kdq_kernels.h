@@ -16,7 +16,7 @@ struct GemmArgs {
     DiagSpec rowS, colS, inS;
 };
 
-hipError_t launch_gemm(hipStream_t st, int nchains, int max_m, int max_n, bool a_cplx, bool b_cplx,
+hipError_t launch_gemm(hipStream_t st, int nchains, int max_m, int max_n, int max_k, bool a_cplx, bool b_cplx,
                        int opa, int opb, const GemmArgs &g);
 
 // Local update of one block of sites [s0, s0 + kb) at slice lt (t-U model).

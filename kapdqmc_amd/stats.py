@@ -35,6 +35,29 @@ def tau_int(x, c: float = 5.0) -> float:
     return max(tau, 0.5)
 
 
+def tau_int_chains(X, c: float = 5.0) -> float:
+    """τ_int of several independent chains of the same Markov process (rows of X, equal
+    lengths): the autocovariances of the chains (each about its own mean) are averaged,
+    normalised at lag 0, then windowed as in tau_int. Used where each chain alone is too
+    short to resolve its own autocorrelation."""
+    X = np.atleast_2d(np.asarray(X, dtype=np.float64))
+    nch, n = X.shape
+    if n < 4:
+        return 0.5
+    Y = X - X.mean(axis=1, keepdims=True)
+    f = np.fft.rfft(Y, 2 * n, axis=1)
+    acov = np.fft.irfft(f * np.conj(f), axis=1)[:, :n].mean(axis=0)
+    if acov[0] <= 0:
+        return 0.5
+    rho = acov / acov[0]
+    tau = 0.5
+    for M in range(1, n):
+        tau += rho[M]
+        if M >= c * tau:
+            break
+    return max(tau, 0.5)
+
+
 def binning_error(x, nbins: int = 16) -> float:
     """Standard error of the mean from nbins equal bins (cross-check of tau_int)."""
     x = np.asarray(x, dtype=np.float64)

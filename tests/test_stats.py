@@ -32,3 +32,13 @@ def test_binning_consistent_with_tau():
     naive = x.std(ddof=1) / np.sqrt(len(x))
     err = stats.binning_error(x, 50)
     assert abs((err / naive) ** 2 / (2 * stats.tau_int(x)) - 1) < 0.3
+
+
+def test_pooled_chains_match_closed_form():
+    """64 short AR(1) chains (400 steps each): the pooled estimate recovers the closed form
+    that a single 400-step chain resolves only loosely."""
+    phi = 0.6
+    X = np.stack([ar1(phi, 400, seed=100 + c) for c in range(64)])
+    ref = (1 + phi) / (2 * (1 - phi))
+    assert abs(stats.tau_int_chains(X) / ref - 1) < 0.12
+    assert abs(stats.tau_int_chains(ar1(0.0, 400, seed=5)[None, :]) - 0.5) < 0.2
