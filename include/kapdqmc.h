@@ -122,6 +122,43 @@ int kdq_measure(kdq_handle h, int32_t lt, double *out);
 /* Counters: out[0] = proposals, out[1] = accepted, per chain (nchains x 2). */
 int kdq_get_counters(kdq_handle h, double *out);
 
+/* log|det(U_L U_R)| of the overlap factorised by the last get_G! of every chain (nchains
+ * doubles). No counterpart in the reference, whose get_G! (src/public/Gupdate.jl:139-157)
+ * discards the LU diagonal; the weight of a configuration up to the chain normalisation. */
+int kdq_get_logdet(kdq_handle h, double *out);
+
+/* ---- swap-replica Rényi-2 entropy (SCEE) of the t-U model ----
+ * Regions A and B and the λ interpolation of ctrl_SCEEicr (src/tU/SCEE.jl:1):
+ * indexA / indexB are the reference's 1-based site lists (area_index), λ and Nλ its
+ * arguments. */
+typedef struct kdq_scee_desc {
+    int32_t nA;
+    const int64_t *indexA;
+    int32_t nB;
+    const int64_t *indexB;
+    double lambda;
+    int32_t nlam;
+    int32_t pad;
+} kdq_scee_desc;
+/* Creates `nchains` two-replica chains; run settings as in kdq_create (delay: sites per
+ * decision block, 1..16). Replaces the buffer set-up of ctrl_SCEEicr (src/tU/SCEE.jl:1-48). */
+int kdq_scee_create(const kdq_model_desc *models, const kdq_run_desc *runs, const kdq_scee_desc *regions,
+                    int32_t nchains, kdq_handle *out);
+/* Fields of replica 0 / 1 of chain c (ss[1], ss[2] of ctrl_SCEEicr), layout as kdq_set_fields. */
+int kdq_scee_set_fields(kdq_handle h, int32_t c, int32_t replica, const uint8_t *s);
+int kdq_scee_get_fields(kdq_handle h, int32_t c, int32_t replica, uint8_t *s);
+/* The chain set-up of ctrl_SCEEicr (src/tU/SCEE.jl:49-86): block products, both chains,
+ * get_ABGM! at the first node. */
+int kdq_scee_init(kdq_handle h);
+/* ctrl_SCEEicr sweeps (src/tU/SCEE.jl:87-251, set-up included): O receives nchains x sweeps
+ * values, per sweep the mean over slices of (det g_A / det g_B)^(1/Nλ), the quantity the
+ * reference writes per sweep. */
+int kdq_scee_sweeps(kdq_handle h, int32_t sweeps, double *O);
+/* G(τ), G(Θ), G(τ,Θ), G(Θ,τ) of replica `replica` of chain c (ns x ns ComplexF64 each). */
+int kdq_scee_get_G4(kdq_handle h, int32_t c, int32_t replica, double *Gt, double *G0, double *Gt0, double *G0t);
+/* |det g_A|² and |det g_B|² (nchains x 2; 1 for an empty region). */
+int kdq_scee_detg(kdq_handle h, double *out);
+
 /* ---- kernel timing (no counterpart in the reference; used by the benchmark) ----
  * Launches of the kernel families whose bit is set in `mask` are bracketed by HIP events
  * on the handle's stream; kdq_profile_read returns the summed device time (ms) and the
@@ -132,8 +169,8 @@ enum {
     KDQ_PROF_GETG = 2,       /* GEMMs of get_G! */
     KDQ_PROF_DECIDE = 3,     /* block Metropolis decisions */
     KDQ_PROF_FLUSH = 4,      /* rank-k flush of accepted flips */
-    KDQ_PROF_QR = 5,         /* Householder QR */
-    KDQ_PROF_INV = 6,        /* overlap inversion + log|det| */
+    KDQ_PROF_QR = 5,         /* orthonormalisation of U_R / U_L (Gram GEMM, Cholesky, solve) */
+    KDQ_PROF_INV = 6,        /* overlap LU + solve + log|det| */
     KDQ_PROF_MEAS_WRAP = 7,  /* wraps and HalfeK sandwich of phy_measure */
     KDQ_PROF_MEASURE = 8,    /* observable reduction */
     KDQ_PROF_GEMM_OTHER = 9,

@@ -20,7 +20,8 @@ PREC_FP64, PREC_MIXED = 0, 1
 EXPORTS = ("kdq_create", "kdq_destroy", "kdq_last_error", "kdq_num_chains", "kdq_num_obs", "kdq_set_fields",
            "kdq_get_fields", "kdq_initial_s", "kdq_phy_update", "kdq_get_G", "kdq_set_G", "kdq_wrap",
            "kdq_update_layer", "kdq_init_chain", "kdq_measure", "kdq_get_counters", "kdq_profile",
-           "kdq_profile_read")
+           "kdq_profile_read", "kdq_get_logdet", "kdq_scee_create", "kdq_scee_set_fields",
+           "kdq_scee_get_fields", "kdq_scee_init", "kdq_scee_sweeps", "kdq_scee_get_G4", "kdq_scee_detg")
 PROF_FAMILIES = ("wrap", "chain", "get_G", "decide", "flush", "qr", "inv", "meas_wrap", "measure",
                  "gemm_other")
 
@@ -45,6 +46,12 @@ class ModelDesc(C.Structure):
 class RunDesc(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("chain", C.c_uint32), ("sweep0", C.c_uint32),
                 ("precision", C.c_int32), ("delay", C.c_int32)]
+
+
+class SceeDesc(C.Structure):
+    _fields_ = [("nA", C.c_int32), ("indexA", C.POINTER(C.c_int64)), ("nB", C.c_int32),
+                ("indexB", C.POINTER(C.c_int64)), ("lam", C.c_double), ("nlam", C.c_int32),
+                ("pad", C.c_int32)]
 
 
 _lib = None
@@ -79,6 +86,14 @@ def load(path: str = LIB_PATH):
     L.kdq_get_counters.argtypes = [vp, dp]
     L.kdq_profile.argtypes = [vp, u32]
     L.kdq_profile_read.argtypes = [vp, dp, C.POINTER(C.c_int64)]
+    L.kdq_get_logdet.argtypes = [vp, dp]
+    L.kdq_scee_create.argtypes = [C.POINTER(ModelDesc), C.POINTER(RunDesc), C.POINTER(SceeDesc), i32, C.POINTER(vp)]
+    L.kdq_scee_set_fields.argtypes = [vp, i32, i32, C.POINTER(C.c_uint8)]
+    L.kdq_scee_get_fields.argtypes = [vp, i32, i32, C.POINTER(C.c_uint8)]
+    L.kdq_scee_init.argtypes = [vp]
+    L.kdq_scee_sweeps.argtypes = [vp, i32, dp]
+    L.kdq_scee_get_G4.argtypes = [vp, i32, i32, dp, dp, dp, dp]
+    L.kdq_scee_detg.argtypes = [vp, dp]
     for name in EXPORTS:
         if name != "kdq_last_error":
             getattr(L, name).restype = C.c_int

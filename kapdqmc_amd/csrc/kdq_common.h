@@ -34,7 +34,7 @@ struct ChainDev {
     int lattice, L1, L2, pad1;
     const double *K;       // ns x ns
     uint64_t seed;
-    uint32_t chain_id, pad2;
+    uint32_t chain_id, stream;   // Philox stream: 0 plain sweep, 1 + replica in SCEE
     // local-update scratch
     int *acc_cnt;          // [1]
     int *acc_idx;          // [64]
@@ -49,13 +49,16 @@ struct ChainDev {
     const int *ex, *ey;    // [n_elem] 0-based sites of element e (t-V, t-U-V)
     const int *inc;        // t-U-V: [ns * 6] incidence, entry = elem * 2 + (0: coefficient 1, 1: coefficient a)
     double a, U, V;        // t-U-V constants (a = V/g), couplings used by the observables
+    int nra, nrb;          // SCEE region sizes (0 outside SCEE)
 };
 
 #define KDQ_MEAS_SLOTS 16
 
-enum { DIM_NS = 0, DIM_NP = 1 };
+enum { DIM_NS = 0, DIM_NP = 1, DIM_NA = 2, DIM_NB = 3 };
 
-KDQ_DEV int dim_of(const ChainDev &c, int code) { return code == DIM_NS ? c.ns : c.np; }
+KDQ_DEV int dim_of(const ChainDev &c, int code) {
+    return code == DIM_NS ? c.ns : code == DIM_NP ? c.np : code == DIM_NA ? c.nra : c.nrb;
+}
 
 // Diagonal factor applied to GEMM rows, columns or the contraction index.
 // kind 0: none; kind 1: t-U phase cis(sign * alpha[lt-1] * eta[s[i, lt-1] - 1]);

@@ -57,6 +57,8 @@ struct kdq_handle_s {
     std::vector<ChainHost> ch;
     std::vector<kdq_model_desc> md;
     std::vector<kdq_run_desc> rd;
+    std::vector<ChainDev> cd_host;   // host copy of d_chains
+    struct SceeState *scee = nullptr;   // swap-replica state (kdq_scee_create handles only)
     hipStream_t stream = nullptr;
     ChainDev *d_chains = nullptr;
     // operand tables
@@ -304,7 +306,7 @@ int propagate_L(kdq_handle h, Opnd *src, Opnd *dst, int idx) {
 }
 
 // get_G!: G = I - U_R (U_L U_R)⁻¹ U_L into the current G buffer
-int get_G(kdq_handle h, int idx, int *bad = nullptr) {
+int get_G(kdq_handle h, int idx, Opnd *dst = nullptr) {
     Opnd *ULH = h->t_ULH[idx - 1], *UR = h->t_UR[idx - 1];
     GemmArgs g1 = gargs(h, ULH, UR, h->t_M, DIM_NP, DIM_NP, DIM_NS);
     if (int r = gemm(h, true, true, 2, 0, g1, KDQ_PROF_GETG)) return r;
@@ -315,9 +317,8 @@ int get_G(kdq_handle h, int idx, int *bad = nullptr) {
         LuSolveArgs sa{h->d_chains, h->t_M, ULH, h->t_X, h->d_piv, DIM_NP, DIM_NS, h->max_np, 0};
         HIPCHK(launch_lu_solve(h->stream, h->nch, h->max_np, h->max_ns, sa));
     }
-    GemmArgs g3 = gargs(h, UR, h->t_X, h->t_G[h->cur], DIM_NS, DIM_NS, DIM_NP);
+    GemmArgs g3 = gargs(h, UR, h->t_X, dst ? dst : h->t_G[h->cur], DIM_NS, DIM_NS, DIM_NP);
     g3.mode = 1;
-    (void)bad;
     return gemm(h, true, true, 0, 0, g3, KDQ_PROF_GETG);
 }
 
@@ -345,7 +346,7 @@ int update_family_r2(kdq_handle h, int lt, int dir, uint32_t sweep, int family) 
             ProfScope ps(h, KDQ_PROF_DECIDE);
             HIPCHK(launch_decide_r2(h->stream, h->nch, a));
         }
-        FlushArgs f{h->d_chains, h->t_G[h->cur], h->t_G[1 - h->cur], 64, 0};   // two slots per element
+        FlushArgs f{h->d_chains, h->t_G[h->cur], h->t_G[1 - h->cur], 64, 0, nullptr, nullptr};   // two slots per element
         {
             ProfScope ps(h, KDQ_PROF_FLUSH);
             HIPCHK(launch_flush(h->stream, h->nch, h->max_ns, f));
@@ -368,7 +369,7 @@ int update_layer(kdq_handle h, int lt, int dir, uint32_t sweep) {
             ProfScope ps(h, KDQ_PROF_DECIDE);
             HIPCHK(launch_decide_tu(h->stream, h->nch, a));
         }
-        FlushArgs f{h->d_chains, h->t_G[h->cur], h->t_G[1 - h->cur], kb, 0};
+        FlushArgs f{h->d_chains, h->t_G[h->cur], h->t_G[1 - h->cur], kb, 0, nullptr, nullptr};
         {
             ProfScope ps(h, KDQ_PROF_FLUSH);
             HIPCHK(launch_flush(h->stream, h->nch, h->max_ns, f));
@@ -529,6 +530,9 @@ int phy_update(kdq_handle h, int sweeps, int record, double *rows) {
     return check_health(h);
 }
 
+int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t nchains,
+                const uint32_t *streams, kdq_handle *out);
+void scee_free(kdq_handle h);
 }  // namespace
 
 // =================================================================== C ABI
@@ -541,6 +545,15 @@ const char *kdq_last_error(kdq_handle h) {
 }
 
 int kdq_create(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t nchains, kdq_handle *out) {
+    return create_impl(models, runs, nchains, nullptr, out);
+}
+}  // extern "C"
+
+namespace {
+// Builds a handle over `nchains` entries; streams[c] (null: all 0) is the Philox stream of
+// entry c (the SCEE handle holds replica r of chain c as entry c + r·nch with stream 1 + r).
+int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t nchains,
+                const uint32_t *streams, kdq_handle *out) {
     if (!models || !runs || !out || nchains < 1) return fail(nullptr, KDQ_EARG, "kdq_create: null argument or nchains < 1");
     *out = nullptr;
     for (int c = 0; c < nchains; ++c) {
@@ -730,6 +743,7 @@ int kdq_create(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t n
         d.K = C.K;
         d.seed = runs[c].seed;
         d.chain_id = runs[c].chain;
+        d.stream = streams ? streams[c] : 0;
         d.acc_cnt = C.acc_cnt; d.acc_idx = C.acc_idx; d.W = C.W;
         d.counters = C.counters; d.meas = C.meas;
         d.kind = m.kind;
@@ -763,6 +777,7 @@ int kdq_create(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t n
         h->dev_misc.push_back(q);
         hipMemcpy(q, cd.data(), sizeof(ChainDev) * nchains, hipMemcpyHostToDevice);
         h->d_chains = (ChainDev *)q;
+        h->cd_host = cd;
     }
     int r = 0;
     r |= make_table(h, &h->t_eK, v_eK); r |= make_table(h, &h->t_eKinv, v_eKinv);
@@ -820,8 +835,11 @@ int kdq_create(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t n
     *out = h;
     return KDQ_OK;
 }
+}  // namespace
 
+extern "C" {
 int kdq_destroy(kdq_handle h) {
+    if (h && h->scee) scee_free(h);
     delete h;
     return KDQ_OK;
 }
@@ -942,6 +960,13 @@ int kdq_profile_read(kdq_handle h, double *ms, int64_t *launches) {
     return KDQ_OK;
 }
 
+int kdq_get_logdet(kdq_handle h, double *out) {
+    if (!h || !out) return fail(h, KDQ_EARG, "kdq_get_logdet: bad argument");
+    HIPCHK(hipMemcpyAsync(out, h->d_logdet, sizeof(double) * h->nch, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return KDQ_OK;
+}
+
 int kdq_get_counters(kdq_handle h, double *out) {
     if (!h || !out) return fail(h, KDQ_EARG, "kdq_get_counters: bad argument");
     for (int c = 0; c < h->nch; ++c)
@@ -950,4 +975,500 @@ int kdq_get_counters(kdq_handle h, double *out) {
     return KDQ_OK;
 }
 
+}  // extern "C"
+
+// =================================================================== swap-replica Rényi-2
+// ctrl_SCEEicr of the t-U model (src/tU/SCEE.jl:1-262). The handle holds replica r of chain
+// c as entry c + r·nch (Philox stream 1 + r), so wraps, block products, chain
+// orthonormalisation and get_G! run over all 2·nch entries in the same launches as the
+// plain sweep. Per entry: G(τ) = Gt (the plain G buffers), G(Θ) = G0, G(τ,Θ) = Gt0,
+// G(Θ,τ) = G0t (ping-pong pairs), and the block products BM / BM⁻¹ of every node block.
+struct SceeState {
+    int nch = 0, kb = 8;
+    int na_max = 0, nb_max = 0, nx_max = 0, m2_max = 0;   // m2: largest nA² + nB²
+    Opnd *t_X[3][2] = {};          // G0, Gt0, G0t ping-pong tables over the 2·nch entries
+    int cur[3] = {0, 0, 0};
+    std::vector<Opnd *> t_BM, t_BMi;   // block idx (1-based) at idx - 1
+    Opnd *t_M[2][2] = {}, *t_C[2] = {}, *t_Inv[2] = {}, *t_IX[2] = {};   // region tables (nch)
+    SceeRegion *d_reg = nullptr;
+    double *d_ld = nullptr;        // [2][nch] region log|det|
+    int *d_info = nullptr, *d_piv = nullptr;
+    double *h_osum = nullptr;      // pinned [nch][2]
+    std::vector<SceeRegion> reg;   // host copy (device pointers)
+    std::vector<void *> allocs;
+};
+
+namespace {
+
+enum { SX_G0 = 0, SX_GT0 = 1, SX_G0T = 2 };
+
+Opnd *xcur(kdq_handle h, int k) { return h->scee->t_X[k][h->scee->cur[k]]; }
+Opnd *xspare(kdq_handle h, int k) { return h->scee->t_X[k][1 - h->scee->cur[k]]; }
+void xflip(kdq_handle h, int k) { h->scee->cur[k] ^= 1; }
+
+void scee_free(kdq_handle h) {
+    SceeState *S = h->scee;
+    if (h->stream) hipStreamSynchronize(h->stream);
+    for (void *p : S->allocs) hipFree(p);
+    if (S->h_osum) hipHostFree(S->h_osum);
+    delete S;
+    h->scee = nullptr;
+}
+
+// BM_F! / BMinv_F! (src/tU/GreenMatrix.jl:10-47) of block idx for every entry
+int scee_blocks(kdq_handle h, int idx) {
+    SceeState *S = h->scee;
+    const int lo = (int)h->nodes[idx - 1] + 1, hi = (int)h->nodes[idx], nb = hi - lo + 1;
+    Opnd *BM = S->t_BM[idx - 1], *BMi = S->t_BMi[idx - 1];
+    HIPCHK(launch_diag(h->stream, h->nch, h->max_ns, h->d_chains, h->t_Gm, DIM_NS, 0.0, 1));   // I
+    Opnd *src = h->t_Gm, *srci = h->t_Gm;
+    for (int s = 0; s < nb; ++s) {
+        const int lt = lo + s;
+        const bool last_even = ((nb - 1 - s) % 2) == 0;
+        Opnd *out = last_even ? BM : h->t_T;
+        GemmArgs g = gargs(h, h->t_eK, src, out, DIM_NS, DIM_NS, DIM_NS);   // D eK X
+        g.rowS = hs_diag(h, lt, +1);
+        if (int r = gemm(h, false, true, 0, 0, g, KDQ_PROF_CHAIN)) return r;
+        Opnd *outi = last_even ? BMi : h->t_Tm;
+        GemmArgs gi = gargs(h, srci, h->t_eKinv, outi, DIM_NS, DIM_NS, DIM_NS);   // X eK⁻¹ D⁻¹
+        gi.colS = hs_diag(h, lt, -1);
+        if (int r = gemm(h, true, false, 0, 0, gi, KDQ_PROF_CHAIN)) return r;
+        src = out;
+        srci = outi;
+    }
+    return KDQ_OK;
+}
+
+// BRMs[i] = QR(BMs[i-1] BRMs[i-1]) and BLMs[i] = RQ(BLMs[i+1] BMs[i]) (src/tU/SCEE.jl:60-86)
+int scee_update_BR(kdq_handle h, int i) {
+    GemmArgs g = gargs(h, h->scee->t_BM[i - 2], h->t_UR[i - 2], h->t_UR[i - 1], DIM_NS, DIM_NP, DIM_NS);
+    if (int r = gemm(h, true, true, 0, 0, g, KDQ_PROF_CHAIN)) return r;
+    return orthonormalise(h, h->t_UR[i - 1]);
+}
+int scee_update_BL(kdq_handle h, int i) {
+    GemmArgs g = gargs(h, h->scee->t_BM[i - 1], h->t_ULH[i], h->t_ULH[i - 1], DIM_NS, DIM_NP, DIM_NS);
+    if (int r = gemm(h, true, true, 2, 0, g, KDQ_PROF_CHAIN)) return r;   // BMᴴ U_Lᴴ
+    return orthonormalise(h, h->t_ULH[i - 1]);
+}
+
+// G4! (src/public/Gupdate.jl:159-212) for every entry
+int scee_G4(kdq_handle h, int idx, bool forward) {
+    const int thidx = h->nn / 2 + 1;
+    Opnd *Gt = h->t_G[h->cur];
+    if (int r = get_G(h, idx, Gt)) return r;
+    if (idx == thidx) {
+        for (int k = 0; k < 3; ++k)
+            HIPCHK(launch_copy_mat(h->stream, h->nch, h->max_ns, h->max_ns, h->d_chains, Gt, xcur(h, k), DIM_NS, DIM_NS, true));
+        HIPCHK(launch_diag(h->stream, h->nch, h->max_ns, h->d_chains, xcur(h, forward ? SX_G0T : SX_GT0), DIM_NS, -1.0, 0));
+        return KDQ_OK;
+    }
+    if (int r = get_G(h, thidx, xcur(h, SX_G0))) return r;
+    HIPCHK(launch_diag(h->stream, h->nch, h->max_ns, h->d_chains, xcur(h, SX_GT0), DIM_NS, 0.0, 1));
+    HIPCHK(launch_diag(h->stream, h->nch, h->max_ns, h->d_chains, xcur(h, SX_G0T), DIM_NS, 0.0, 1));
+    // idx < Θidx: G0t <- BM_j G_j G0t, Gt0 <- Gt0 (I - G_j) BM_j⁻¹, then Gt0 = -Gt0;
+    // idx > Θidx: the same with the roles of Gt0 and G0t exchanged.
+    const int j0 = std::min(idx, thidx), j1 = std::max(idx, thidx) - 1;
+    const int kp = idx < thidx ? SX_G0T : SX_GT0, km = idx < thidx ? SX_GT0 : SX_G0T;
+    for (int j = j0; j <= j1; ++j) {
+        Opnd *T2 = h->t_Tm;
+        if (j == j0) T2 = idx < thidx ? Gt : xcur(h, SX_G0);
+        else if (int r = get_G(h, j, h->t_Tm)) return r;
+        GemmArgs g1 = gargs(h, T2, xcur(h, kp), h->t_T, DIM_NS, DIM_NS, DIM_NS);
+        if (int r = gemm(h, true, true, 0, 0, g1, KDQ_PROF_GETG)) return r;
+        GemmArgs g2 = gargs(h, h->scee->t_BM[j - 1], h->t_T, xspare(h, kp), DIM_NS, DIM_NS, DIM_NS);
+        if (int r = gemm(h, true, true, 0, 0, g2, KDQ_PROF_GETG)) return r;
+        xflip(h, kp);
+        GemmArgs g3 = gargs(h, xcur(h, km), T2, h->t_Gm, DIM_NS, DIM_NS, DIM_NS);   // X - X G_j
+        g3.mode = 2;
+        g3.Cin = xcur(h, km);
+        if (int r = gemm(h, true, true, 0, 0, g3, KDQ_PROF_GETG)) return r;
+        GemmArgs g4 = gargs(h, h->t_Gm, h->scee->t_BMi[j - 1], xspare(h, km), DIM_NS, DIM_NS, DIM_NS);
+        if (j == j1) g4.mode = 3;   // the final sign flip
+        if (int r = gemm(h, true, true, 0, 0, g4, KDQ_PROF_GETG)) return r;
+        xflip(h, km);
+    }
+    return KDQ_OK;
+}
+
+SceeArgs scee_args(kdq_handle h) {
+    SceeState *S = h->scee;
+    SceeArgs a{};
+    a.chains = h->d_chains;
+    a.Gt = h->t_G[h->cur];
+    a.Gt0 = xcur(h, SX_GT0);
+    a.G0t = xcur(h, SX_G0T);
+    a.G0 = xcur(h, SX_G0);
+    a.reg = S->d_reg;
+    a.nch = S->nch;
+    a.nx_max = S->nx_max;
+    return a;
+}
+
+// GroverMatrix! of A and B, |det|² and the inverses (get_ABGM!, src/tU/SCEE.jl:264-276)
+int scee_grover(kdq_handle h) {
+    SceeState *S = h->scee;
+    if (S->nx_max == 0) return KDQ_OK;
+    SceeArgs a = scee_args(h);
+    HIPCHK(launch_scee_gather(h->stream, S->nch, S->nx_max, a));
+    const int dims[2] = {DIM_NA, DIM_NB}, nmax[2] = {S->na_max, S->nb_max};
+    for (int X = 0; X < 2; ++X) {
+        if (nmax[X] == 0) continue;
+        GemmArgs g{};
+        g.chains = h->d_chains;
+        g.A = S->t_M[0][X]; g.B = S->t_M[1][X]; g.C = S->t_C[X];
+        g.mdim = g.ndim = g.kdim = dims[X];
+        HIPCHK(launch_gemm(h->stream, S->nch, nmax[X], nmax[X], nmax[X], true, true, 0, 0, g));
+    }
+    HIPCHK(launch_scee_grover_form(h->stream, S->nch, S->m2_max, S->d_reg));
+    for (int X = 0; X < 2; ++X) {
+        if (nmax[X] == 0) continue;
+        LuArgs la{h->d_chains, S->t_C[X], dims[X], S->nx_max, S->d_piv, S->d_ld + X * S->nch, S->d_info};
+        HIPCHK(launch_lu(h->stream, S->nch, nmax[X], la));
+        LuSolveArgs sa{h->d_chains, S->t_C[X], S->t_IX[X], S->t_Inv[X], S->d_piv, dims[X], dims[X], S->nx_max, 0};
+        HIPCHK(launch_lu_solve(h->stream, S->nch, nmax[X], nmax[X], sa));
+    }
+    HIPCHK(launch_scee_detg(h->stream, S->nch, S->d_reg, S->d_ld, S->d_ld + S->nch));
+    return KDQ_OK;
+}
+
+int scee_get_ABGM(kdq_handle h, int idx, bool forward) {
+    if (int r = scee_G4(h, idx, forward)) return r;
+    return scee_grover(h);
+}
+
+// the four-matrix flush of one block (both replicas, every chain)
+int scee_flush(kdq_handle h) {
+    SceeState *S = h->scee;
+    const int kb = S->kb;
+    Opnd *Gt = h->t_G[h->cur];
+    FlushArgs f[4] = {
+        {h->d_chains, Gt, h->t_G[1 - h->cur], kb, 0, nullptr, nullptr},                    // Gt
+        {h->d_chains, xcur(h, SX_G0T), xspare(h, SX_G0T), kb, 0, nullptr, Gt},              // G0t
+        {h->d_chains, xcur(h, SX_GT0), xspare(h, SX_GT0), kb, 1, Gt, nullptr},              // Gt0
+        {h->d_chains, xcur(h, SX_G0), xspare(h, SX_G0), kb, 1, xcur(h, SX_G0T), xcur(h, SX_GT0)},   // G0
+    };
+    ProfScope ps(h, KDQ_PROF_FLUSH);
+    for (int k = 0; k < 4; ++k) HIPCHK(launch_flush(h->stream, h->nch, h->max_ns, f[k]));
+    h->cur ^= 1;
+    for (int k = 0; k < 3; ++k) xflip(h, k);
+    return KDQ_OK;
+}
+
+// UpdateSCEELayer! (src/tU/SCEE.jl:278-324) plus the per-slice Rényi accumulation
+int scee_layer(kdq_handle h, int lt, int dir, uint32_t sweep) {
+    SceeState *S = h->scee;
+    for (int s0 = 0; s0 < h->max_ns; s0 += S->kb) {
+        SceeArgs a = scee_args(h);
+        a.lt = lt; a.dir = dir; a.s0 = s0; a.kb = S->kb; a.sweep = sweep;
+        {
+            ProfScope ps(h, KDQ_PROF_DECIDE);
+            HIPCHK(launch_scee_decide(h->stream, a));
+        }
+        if (int r = scee_flush(h)) return r;
+    }
+    HIPCHK(launch_scee_accum(h->stream, S->nch, S->d_reg));
+    return KDQ_OK;
+}
+
+// the "L", "B", "R" wraps of Gt0, Gt and G0t (WrapKV!, src/tU/GreenMatrix.jl:56-86)
+int scee_wraps(kdq_handle h, int lt, bool forward) {
+    GemmArgs gl, gr;
+    if (forward) {   // Gt0 <- D eK Gt0, G0t <- G0t eK⁻¹ D⁻¹
+        gl = gargs(h, h->t_eK, xcur(h, SX_GT0), xspare(h, SX_GT0), DIM_NS, DIM_NS, DIM_NS);
+        gl.rowS = hs_diag(h, lt, +1);
+        gr = gargs(h, xcur(h, SX_G0T), h->t_eKinv, xspare(h, SX_G0T), DIM_NS, DIM_NS, DIM_NS);
+        gr.colS = hs_diag(h, lt, -1);
+    } else {         // Gt0 <- eK⁻¹ D⁻¹ Gt0, G0t <- G0t D eK
+        gl = gargs(h, h->t_eKinv, xcur(h, SX_GT0), xspare(h, SX_GT0), DIM_NS, DIM_NS, DIM_NS);
+        gl.inS = hs_diag(h, lt, -1);
+        gr = gargs(h, xcur(h, SX_G0T), h->t_eK, xspare(h, SX_G0T), DIM_NS, DIM_NS, DIM_NS);
+        gr.inS = hs_diag(h, lt, +1);
+    }
+    if (int r = gemm(h, false, true, 0, 0, gl, KDQ_PROF_WRAP)) return r;
+    xflip(h, SX_GT0);
+    if (int r = wrap_B(h, h->t_G[h->cur], h->t_T, lt, forward)) return r;
+    if (int r = gemm(h, true, false, 0, 0, gr, KDQ_PROF_WRAP)) return r;
+    xflip(h, SX_G0T);
+    return KDQ_OK;
+}
+
+// chain set-up of ctrl_SCEEicr (src/tU/SCEE.jl:49-86)
+int scee_init(kdq_handle h) {
+    const int NN = h->nn;
+    for (int idx = 1; idx <= NN - 1; ++idx)
+        if (int r = scee_blocks(h, idx)) return r;
+    HIPCHK(launch_real_to_cplx(h->stream, h->nch, h->max_ns, h->max_np, h->d_chains, h->t_Pt, h->t_UR[0], DIM_NS, DIM_NP));
+    HIPCHK(launch_real_to_cplx(h->stream, h->nch, h->max_ns, h->max_np, h->d_chains, h->t_Pt, h->t_ULH[NN - 1], DIM_NS, DIM_NP));
+    for (int i = 1; i <= NN - 1; ++i) {
+        if (int r = scee_update_BL(h, NN - i)) return r;
+        if (int r = scee_update_BR(h, i + 1)) return r;
+    }
+    h->idx = 1;
+    return scee_get_ABGM(h, 1, true);
+}
+
+int scee_sweeps(kdq_handle h, int sweeps, double *O) {
+    SceeState *S = h->scee;
+    const int NN = h->nn, nt = h->nt, thidx = NN / 2 + 1, nch = S->nch;
+    if (int r = scee_init(h)) return r;
+    for (int sw = 0; sw < sweeps; ++sw) {
+        const uint32_t sweep = h->sweep;
+        for (int c = 0; c < nch; ++c) HIPCHK(hipMemsetAsync(S->reg[c].osum, 0, 2 * sizeof(double), h->stream));
+        for (int lt = 1; lt <= nt; ++lt) {
+            if (int r = scee_wraps(h, lt, true)) return r;
+            if (int r = scee_layer(h, lt, 0, sweep)) return r;
+            if (is_node(h, lt)) {
+                const int idx = ++h->idx;
+                if (int r = scee_blocks(h, idx - 1)) return r;
+                for (int i = idx; i <= std::max(thidx, idx); ++i)
+                    if (int r = scee_update_BR(h, i)) return r;
+                for (int i = idx - 1; i >= std::min(thidx, idx); --i)
+                    if (int r = scee_update_BL(h, i)) return r;
+                if (int r = scee_get_ABGM(h, idx, true)) return r;
+            }
+        }
+        for (int lt = nt; lt >= 1; --lt) {
+            if (int r = scee_layer(h, lt, 1, sweep)) return r;
+            if (is_node(h, lt - 1)) {
+                const int idx = --h->idx;
+                if (int r = scee_blocks(h, idx)) return r;
+                for (int i = idx; i >= std::min(thidx, idx); --i)
+                    if (int r = scee_update_BL(h, i)) return r;
+                for (int i = idx + 1; i <= std::max(thidx, idx); ++i)
+                    if (int r = scee_update_BR(h, i)) return r;
+                if (int r = scee_get_ABGM(h, idx, false)) return r;
+            } else if (int r = scee_wraps(h, lt, false)) {
+                return r;
+            }
+        }
+        for (int c = 0; c < nch; ++c)
+            HIPCHK(hipMemcpyAsync(S->h_osum + 2 * c, S->reg[c].osum, 2 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        if (O)
+            for (int c = 0; c < nch; ++c) O[(size_t)c * sweeps + sw] = S->h_osum[2 * c] / S->h_osum[2 * c + 1];
+        h->sweep += 1;
+    }
+    return check_health(h);
+}
+
+template <class T>
+int salloc(kdq_handle h, T **p, size_t count) {
+    void *q = nullptr;
+    if (hipMalloc(&q, sizeof(T) * (count ? count : 1)) != hipSuccess) return 1;
+    h->scee->allocs.push_back(q);
+    hipMemset(q, 0, sizeof(T) * (count ? count : 1));
+    *p = (T *)q;
+    return 0;
+}
+
+int stable(kdq_handle h, Opnd **out, const std::vector<Opnd> &v) {
+    void *q = nullptr;
+    if (hipMalloc(&q, sizeof(Opnd) * v.size()) != hipSuccess) return 1;
+    h->scee->allocs.push_back(q);
+    hipMemcpy(q, v.data(), sizeof(Opnd) * v.size(), hipMemcpyHostToDevice);
+    *out = (Opnd *)q;
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+int kdq_scee_create(const kdq_model_desc *models, const kdq_run_desc *runs, const kdq_scee_desc *regions,
+                    int32_t nchains, kdq_handle *out) {
+    if (!models || !runs || !regions || !out || nchains < 1)
+        return fail(nullptr, KDQ_EARG, "kdq_scee_create: null argument or nchains < 1");
+    *out = nullptr;
+    int na_max = 0, nb_max = 0, m2_max = 0;
+    for (int c = 0; c < nchains; ++c) {
+        const kdq_scee_desc &g = regions[c];
+        m2_max = std::max(m2_max, (int)(g.nA * g.nA + g.nB * g.nB));
+        if (models[c].kind != KDQ_MODEL_TU)
+            return fail(nullptr, KDQ_EARG, "kdq_scee_create: the swap-replica sweep is built for the t-U model only");
+        if (runs[c].delay < 1 || runs[c].delay > 16)
+            return fail(nullptr, KDQ_EARG, "kdq_scee_create: delay must be in 1..16");
+        if (g.nA < 0 || g.nB < 0 || (g.nA && !g.indexA) || (g.nB && !g.indexB) || g.nlam < 1)
+            return fail(nullptr, KDQ_EARG, "kdq_scee_create: invalid region description");
+        for (int i = 0; i < g.nA; ++i)
+            if (g.indexA[i] < 1 || g.indexA[i] > models[c].ns) return fail(nullptr, KDQ_EARG, "kdq_scee_create: region A index out of range");
+        for (int i = 0; i < g.nB; ++i)
+            if (g.indexB[i] < 1 || g.indexB[i] > models[c].ns) return fail(nullptr, KDQ_EARG, "kdq_scee_create: region B index out of range");
+        na_max = std::max(na_max, (int)g.nA);
+        nb_max = std::max(nb_max, (int)g.nB);
+    }
+    const int ks = runs[0].delay <= 8 ? 8 : 16;
+    if (scee_lds_bytes(ks, na_max + nb_max) > 163840)
+        return fail(nullptr, KDQ_EARG, "kdq_scee_create: regions too large for the block size (reduce delay or |A|+|B|)");
+    std::vector<kdq_model_desc> md(2 * nchains);
+    std::vector<kdq_run_desc> rd(2 * nchains);
+    std::vector<uint32_t> streams(2 * nchains);
+    for (int r = 0; r < 2; ++r)
+        for (int c = 0; c < nchains; ++c) {
+            md[c + r * nchains] = models[c];
+            rd[c + r * nchains] = runs[c];
+            streams[c + r * nchains] = 1 + r;
+        }
+    kdq_handle h = nullptr;
+    if (int r = create_impl(md.data(), rd.data(), 2 * nchains, streams.data(), &h)) return r;
+    SceeState *S = new SceeState();
+    h->scee = S;
+    S->nch = nchains;
+    S->kb = runs[0].delay;
+    S->na_max = na_max;
+    S->nb_max = nb_max;
+    S->nx_max = na_max + nb_max;
+    S->m2_max = m2_max;
+    int bad = 0;
+    // four-matrix buffers and block products per entry
+    std::vector<Opnd> vx[3][2];
+    std::vector<std::vector<Opnd>> vbm(h->nn - 1), vbmi(h->nn - 1);
+    for (int e = 0; e < 2 * nchains; ++e) {
+        const int n = h->ch[e].ns;
+        const size_t nn = (size_t)n * n;
+        for (int k = 0; k < 3; ++k)
+            for (int b = 0; b < 2; ++b) {
+                double *p = nullptr;
+                bad |= salloc(h, &p, 2 * nn);
+                vx[k][b].push_back(cplx(p, n, nn));
+            }
+        for (int i = 0; i < h->nn - 1; ++i) {
+            double *p = nullptr, *q = nullptr;
+            bad |= salloc(h, &p, 2 * nn);
+            bad |= salloc(h, &q, 2 * nn);
+            vbm[i].push_back(cplx(p, n, nn));
+            vbmi[i].push_back(cplx(q, n, nn));
+        }
+    }
+    for (int k = 0; k < 3; ++k)
+        for (int b = 0; b < 2; ++b) bad |= stable(h, &S->t_X[k][b], vx[k][b]);
+    S->t_BM.resize(h->nn - 1);
+    S->t_BMi.resize(h->nn - 1);
+    for (int i = 0; i < h->nn - 1; ++i) {
+        bad |= stable(h, &S->t_BM[i], vbm[i]);
+        bad |= stable(h, &S->t_BMi[i], vbmi[i]);
+    }
+    // regions per chain
+    std::vector<Opnd> vm[2][2], vc[2], vinv[2], vix[2];
+    S->reg.resize(nchains);
+    std::vector<double> eye;
+    for (int c = 0; c < nchains; ++c) {
+        const kdq_scee_desc &g = regions[c];
+        SceeRegion &R = S->reg[c];
+        memset(&R, 0, sizeof(R));
+        R.nA = g.nA; R.nB = g.nB; R.nlam = g.nlam; R.lambda = g.lambda;
+        std::vector<int> idx;
+        for (int i = 0; i < g.nA; ++i) idx.push_back((int)g.indexA[i] - 1);
+        for (int i = 0; i < g.nB; ++i) idx.push_back((int)g.indexB[i] - 1);
+        int *didx = nullptr;
+        bad |= salloc(h, &didx, idx.size());
+        if (!idx.empty()) hipMemcpy(didx, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice);
+        R.idx = didx;
+        const int nx[2] = {g.nA, g.nB};
+        for (int X = 0; X < 2; ++X) {
+            const size_t m2 = (size_t)nx[X] * nx[X];
+            for (int r = 0; r < 2; ++r) {
+                bad |= salloc(h, &R.M[r][X], 2 * m2);
+                vm[r][X].push_back(cplx(R.M[r][X], std::max(nx[X], 1), m2));
+            }
+            bad |= salloc(h, &R.gmInv[X], 2 * m2);
+            bad |= salloc(h, &R.C[X], 2 * m2);
+            double *I = nullptr;
+            bad |= salloc(h, &I, 2 * m2);
+            eye.assign(2 * m2, 0.0);
+            for (int i = 0; i < nx[X]; ++i) eye[i + (size_t)i * nx[X]] = 1.0;
+            if (m2) hipMemcpy(I, eye.data(), sizeof(double) * 2 * m2, hipMemcpyHostToDevice);
+            vinv[X].push_back(cplx(R.gmInv[X], std::max(nx[X], 1), m2));
+            vc[X].push_back(cplx(R.C[X], std::max(nx[X], 1), m2));
+            vix[X].push_back(cplx(I, std::max(nx[X], 1), m2));
+        }
+        bad |= salloc(h, &R.detg, 2);
+        bad |= salloc(h, &R.osum, 2);
+        for (int r = 0; r < 2; ++r) {
+            h->cd_host[c + r * nchains].nra = g.nA;
+            h->cd_host[c + r * nchains].nrb = g.nB;
+        }
+    }
+    for (int X = 0; X < 2; ++X) {
+        for (int r = 0; r < 2; ++r) bad |= stable(h, &S->t_M[r][X], vm[r][X]);
+        bad |= stable(h, &S->t_C[X], vc[X]);
+        bad |= stable(h, &S->t_Inv[X], vinv[X]);
+        bad |= stable(h, &S->t_IX[X], vix[X]);
+    }
+    bad |= salloc(h, &S->d_ld, 2 * nchains);
+    bad |= salloc(h, &S->d_info, nchains);
+    bad |= salloc(h, &S->d_piv, (size_t)nchains * std::max(S->nx_max, 1));
+    {
+        void *q = nullptr;
+        if (hipMalloc(&q, sizeof(SceeRegion) * nchains) == hipSuccess) {
+            S->allocs.push_back(q);
+            hipMemcpy(q, S->reg.data(), sizeof(SceeRegion) * nchains, hipMemcpyHostToDevice);
+            S->d_reg = (SceeRegion *)q;
+        } else {
+            bad = 1;
+        }
+    }
+    if (hipHostMalloc((void **)&S->h_osum, sizeof(double) * 2 * nchains, hipHostMallocDefault) != hipSuccess) bad = 1;
+    hipMemcpy(h->d_chains, h->cd_host.data(), sizeof(ChainDev) * h->cd_host.size(), hipMemcpyHostToDevice);
+    if (bad || hipDeviceSynchronize() != hipSuccess) {
+        int r = fail(h, KDQ_EHIP, "kdq_scee_create: device allocation failed");
+        kdq_destroy(h);
+        return r;
+    }
+    *out = h;
+    return KDQ_OK;
+}
+
+int kdq_scee_set_fields(kdq_handle h, int32_t c, int32_t replica, const uint8_t *s) {
+    if (!h || !h->scee || replica < 0 || replica > 1 || c < 0 || c >= h->scee->nch)
+        return fail(h, KDQ_EARG, "kdq_scee_set_fields: bad argument");
+    return kdq_set_fields(h, c + replica * h->scee->nch, s);
+}
+
+int kdq_scee_get_fields(kdq_handle h, int32_t c, int32_t replica, uint8_t *s) {
+    if (!h || !h->scee || replica < 0 || replica > 1 || c < 0 || c >= h->scee->nch)
+        return fail(h, KDQ_EARG, "kdq_scee_get_fields: bad argument");
+    return kdq_get_fields(h, c + replica * h->scee->nch, s);
+}
+
+int kdq_scee_init(kdq_handle h) {
+    if (!h || !h->scee) return fail(h, KDQ_EARG, "kdq_scee_init: not a swap-replica handle");
+    if (int r = scee_init(h)) return r;
+    return check_health(h);
+}
+
+int kdq_scee_sweeps(kdq_handle h, int32_t sweeps, double *O) {
+    if (!h || !h->scee || sweeps < 0) return fail(h, KDQ_EARG, "kdq_scee_sweeps: bad argument");
+    return scee_sweeps(h, sweeps, O);
+}
+
+int kdq_scee_get_G4(kdq_handle h, int32_t c, int32_t replica, double *Gt, double *G0, double *Gt0, double *G0t) {
+    if (!h || !h->scee || replica < 0 || replica > 1 || c < 0 || c >= h->scee->nch || !Gt || !G0 || !Gt0 || !G0t)
+        return fail(h, KDQ_EARG, "kdq_scee_get_G4: bad argument");
+    const int e = c + replica * h->scee->nch, n = h->ch[e].ns;
+    const size_t nn = (size_t)n * n;
+    std::vector<Opnd> tab(2 * h->scee->nch);
+    double *outs[4] = {Gt, G0, Gt0, G0t};
+    const double *src[4];
+    src[0] = h->ch[e].G[h->cur];
+    for (int k = 0; k < 3; ++k) {
+        HIPCHK(hipMemcpy(tab.data(), xcur(h, k), sizeof(Opnd) * tab.size(), hipMemcpyDeviceToHost));
+        src[k + 1] = tab[e].re;
+    }
+    std::vector<double> tmp(2 * nn);
+    const int order[4] = {0, 1, 2, 3};   // Gt, G0, Gt0, G0t
+    for (int q = 0; q < 4; ++q) {
+        HIPCHK(hipMemcpy(tmp.data(), src[order[q]], 2 * nn * 8, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < nn; ++i) {
+            outs[q][2 * i] = tmp[i];
+            outs[q][2 * i + 1] = tmp[nn + i];
+        }
+    }
+    return KDQ_OK;
+}
+
+int kdq_scee_detg(kdq_handle h, double *out) {
+    if (!h || !h->scee || !out) return fail(h, KDQ_EARG, "kdq_scee_detg: bad argument");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    for (int c = 0; c < h->scee->nch; ++c) HIPCHK(hipMemcpy(out + 2 * c, h->scee->reg[c].detg, 2 * sizeof(double), hipMemcpyDeviceToHost));
+    return KDQ_OK;
+}
 }  // extern "C"

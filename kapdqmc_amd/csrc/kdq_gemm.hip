@@ -203,6 +203,9 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
                     const size_t oi = (size_t)i + (size_t)j * Ci.ld;
                     cr = Ci.re[oi] - cr;
                     if (OC) ci = Ci.im[oi] - ci;
+                } else if (g.mode == 3) {
+                    cr = -cr;
+                    ci = -ci;
                 }
                 Cm.re[off] = cr;
                 if (OC) Cm.im[off] = ci;

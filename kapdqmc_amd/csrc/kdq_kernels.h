@@ -3,7 +3,8 @@
 #include "kdq_common.h"
 
 // C = rowS ⊙ (op(A) · diag(inS) · op(B)) ⊙ colS, with
-//   mode 0: C = product, mode 1: C = I - product, mode 2: C = Cin - product.
+//   mode 0: C = product, mode 1: C = I - product, mode 2: C = Cin - product,
+//   mode 3: C = -product.
 // op codes: 0 = N, 1 = T, 2 = C (conjugate transpose). inS scales the complex operand
 // along the contraction index. Dimensions are per-chain codes (DIM_NS / DIM_NP).
 struct GemmArgs {
@@ -29,12 +30,17 @@ struct DecideArgs {
 };
 hipError_t launch_decide_tu(hipStream_t st, int nchains, const DecideArgs &a);
 
-// Gout = Gin - Gin[:,A] · W · (E_A - Gin[A,:]) with the block's accepted set A.
+// Delayed-update flush with the block's accepted set A and factor W (per chain):
+//   mode 0: Xout = Xin - Cin[:,A] · W · (E_A - Rin[A,:])
+//   mode 1: Xout = Xin + Cin[:,A] · W · Rin[A,:]
+// The plain sweep uses mode 0 with Cin = Rin = Xin = G. Null Cin / Rin mean Xin.
 struct FlushArgs {
     const ChainDev *chains;
-    const Opnd *Gin;
-    const Opnd *Gout;
-    int kmax, pad;   // largest accepted set the preceding decisions can produce (<= 64)
+    const Opnd *Gin;    // Xin
+    const Opnd *Gout;   // Xout
+    int kmax, mode;     // largest accepted set the preceding decisions can produce (<= 64)
+    const Opnd *Cin;
+    const Opnd *Rin;
 };
 hipError_t launch_flush(hipStream_t st, int nchains, int max_ns, const FlushArgs &f);
 
@@ -124,3 +130,38 @@ hipError_t launch_real_to_cplx(hipStream_t st, int nchains, int max_m, int max_n
                                const Opnd *src, const Opnd *dst, int mdim, int ndim);
 hipError_t launch_initial_s(hipStream_t st, int nchains, int max_elem, int nt, const ChainDev *chains);
 hipError_t launch_zero_meas(hipStream_t st, int nchains, const ChainDev *chains);
+
+// ---------------------------------------------------------------- swap-replica Rényi-2 (SCEE)
+// Per chain: regions A and B (0-based sites, A first), the replicas' region blocks of G(Θ)
+// M[r][X] = G0_r[X,X] (kept current inside a block of decisions), the inverse Grover
+// matrices gmInv[X], |det g_X| products detg[X] and the running Rényi accumulator.
+// All matrices planar complex, column-major, ld = region size.
+struct SceeRegion {
+    int nA, nB, nlam, pad;
+    double lambda;
+    const int *idx;        // [nA + nB]
+    double *M[2][2];       // [replica][region]
+    double *gmInv[2];      // [region]
+    double *C[2];          // [region] product, then Grover matrix
+    double *detg;          // [2]
+    double *osum;          // [2]: Σ (detg_A / detg_B)^(1/Nλ), count
+};
+// Entries: replica r of chain c is entry c + r·nch of the chain tables.
+struct SceeArgs {
+    const ChainDev *chains;
+    const Opnd *Gt, *Gt0, *G0t, *G0;   // current buffers, per entry
+    const SceeRegion *reg;             // [nch]
+    int nch, lt, dir, s0, kb, nx_max;
+    uint32_t sweep;
+    int pad;
+};
+hipError_t launch_scee_decide(hipStream_t st, const SceeArgs &a);
+size_t scee_lds_bytes(int ks, int nx);   // dynamic LDS of the decide kernel for block size ks
+hipError_t launch_scee_gather(hipStream_t st, int nch, int nx_max, const SceeArgs &a);
+hipError_t launch_scee_grover_form(hipStream_t st, int nch, int total, const SceeRegion *reg);
+hipError_t launch_scee_detg(hipStream_t st, int nch, const SceeRegion *reg, const double *logdet_a,
+                            const double *logdet_b);
+hipError_t launch_scee_accum(hipStream_t st, int nch, const SceeRegion *reg);
+// X[i][i] += v for the per-chain n x n matrices (dimension code ndim); set = 1 writes I.
+hipError_t launch_diag(hipStream_t st, int nchains, int max_n, const ChainDev *chains, const Opnd *X, int ndim,
+                       double v, int set);

@@ -1,0 +1,442 @@
+// Swap-replica Rényi-2 (SCEE) decisions of the t-U model.
+//
+// Reference: UpdateSCEELayer! (src/tU/SCEE.jl:278-324) visits the sites of slice lt in
+// order and, per site, replica 1 then replica 2. A proposal of replica r is weighed by
+// |1 + Δ(1 - Gt_ii)|² γ'/γ · |Tau_A|^{2λ} |Tau_B|^{2(1-λ)} with the Grover factors of
+// get_abTau1!/get_abTau2! (src/public/Gupdate.jl:91-132), which read the *other* replica's
+// G(Θ) on the region. An accepted flip updates detg, the inverse Grover matrices
+// (GMupdate!, Gupdate.jl:45-51) and the four matrices of the replica (G4update!,
+// Gupdate.jl:23-37).
+//
+// The four matrices are the blocks of Big = [[Gt, Gt0], [G0t, G0]], and G4update! is the
+// rank-1 Gupdate! of Big with the unit vector in the first block, so a block of accepted
+// sites A of one replica gives, with the same W = (diag(1/Δ_a) + I - Gt[A,A])⁻¹ as the plain
+// sweep, Gt -= Gt[:,A] W (E_A - Gt[A,:]), G0t -= G0t[:,A] W (E_A - Gt[A,:]),
+// Gt0 += Gt[:,A] W Gt0[A,:], G0 += G0t[:,A] W Gt0[A,:] (k_flush modes 0 and 1).
+// k_scee_decide therefore works on a block of KS sites of both replicas: Gt[S,S],
+// Gt0[S,X], G0t[X,S] (X = A ∪ B) in LDS, the region blocks M[r][X] of G0 and the inverse
+// Grover matrices in global memory, all kept current with the sequential rank-1 updates.
+#include "kdq_kernels.h"
+
+namespace {
+
+constexpr int ST = 512;   // threads of k_scee_decide
+
+KDQ_DEV void block_sum4(double v[4], double *red) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = wave_sum(v[k]);
+    if (lane == 0)
+        for (int k = 0; k < 4; ++k) red[wave * 4 + k] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        double s = 0.0;
+        for (int w = 0; w < ST / 64; ++w) s += red[w * 4 + k];
+        v[k] = s;
+    }
+    __syncthreads();
+}
+
+template <int KS>
+__global__ __launch_bounds__(ST) void k_scee_decide(SceeArgs a) {
+    const int c = blockIdx.x, nch = a.nch, tid = threadIdx.x;
+    const ChainDev &ch0 = a.chains[c];
+    const SceeRegion &R = a.reg[c];
+    const int n = ch0.ns, s0 = a.s0, lt = a.lt;
+    const int kb = min(a.kb, n - s0);
+    if (kb <= 0) {
+        if (tid < 2) *a.chains[c + tid * nch].acc_cnt = 0;
+        return;
+    }
+    const int nA = R.nA, nB = R.nB, nX = nA + nB, NX = a.nx_max;
+    extern __shared__ double lds[];
+    double *gtr = lds, *gti = gtr + 2 * KS * KS;               // Gt[S,S]   [r][i][j]
+    double *t0r = gti + 2 * KS * KS, *t0i = t0r + 2 * KS * NX;  // Gt0[S,X]  [r][i][x]
+    double *o0r = t0i + 2 * KS * NX, *o0i = o0r + 2 * KS * NX;  // G0t[X,S]  [r][x][i]
+    double *var_ = o0i + 2 * KS * NX, *vai = var_ + NX;         // a
+    double *vbr = vai + NX, *vbi = vbr + NX;                    // b
+    double *vzr = vbi + NX, *vzi = vzr + NX;                    // zN, then gmInv a
+    double *cxr = vzi + NX, *cxi = cxr + NX;                    // G0t[X, m]
+    double *rxr = cxi + NX, *rxi = rxr + NX;                    // Gt0[m, X]
+    double *csr = rxi + NX, *csi = csr + KS;                    // Gt[S, m]
+    double *rsr = csi + KS, *rsi = rsr + KS;                    // Gt[m, S]
+    double *Wr = rsi + KS, *Wi = Wr + 2 * KS * KS;              // W [r][t][u]
+    double *pd = Wi + 2 * KS * KS;                              // per site [6][2][KS]
+    double *red = pd + 12 * KS;                                 // ST/64 x 4
+    __shared__ int sv[2][KS], accl[2][KS];
+#define GT(r, i, j) ((r) * KS * KS + (i) * KS + (j))
+#define T0(r, i, x) ((r) * KS * NX + (i) * NX + (x))
+#define O0(r, x, i) ((r) * NX * KS + (x) * KS + (i))
+#define PD(k, r, q) pd[((k) * 2 + (r)) * KS + (q)]
+
+    const uint32_t tag = (uint32_t)lt | ((uint32_t)a.dir << 31);
+    if (tid < 2 * KS) {
+        const int r = tid / KS, q = tid % KS;
+        accl[r][q] = 0;
+        if (q < kb) {
+            const ChainDev &chr = a.chains[c + r * nch];
+            int chv;
+            double u;
+            proposal_draw(chr.seed, chr.chain_id, chr.stream, a.sweep, tag, (uint32_t)(s0 + q), chv, u);
+            const int si = chr.s[(size_t)(s0 + q) + (size_t)(lt - 1) * chr.n_elem];
+            const int sx = chv + 1 + ((chv + 1) >= si ? 1 : 0);
+            const double x = chr.alpha[lt - 1] * (chr.eta[sx - 1] - chr.eta[si - 1]);
+            double sn, cs;
+            sincos(x, &sn, &cs);
+            PD(0, r, q) = cs - 1.0;   // Δ
+            PD(1, r, q) = sn;
+            PD(2, r, q) = chr.gamma[sx - 1] / chr.gamma[si - 1];
+            PD(3, r, q) = 1.0 - cs;   // Δ/(1+Δ), 1+Δ = cis(x)
+            PD(4, r, q) = sn;
+            PD(5, r, q) = u;
+            sv[r][q] = si | (sx << 8);
+        }
+    }
+    for (int e = tid; e < 2 * KS * KS; e += ST) {
+        const int r = e / (KS * KS), i = (e / KS) % KS, j = e % KS;
+        double vr = 0.0, vi = 0.0;
+        if (i < kb && j < kb) {
+            const Opnd G = a.Gt[c + r * nch];
+            const size_t o = (size_t)(s0 + i) + (size_t)(s0 + j) * G.ld;
+            vr = G.re[o];
+            vi = G.im[o];
+        }
+        gtr[e] = vr;
+        gti[e] = vi;
+        Wr[e] = 0.0;
+        Wi[e] = 0.0;
+    }
+    for (int e = tid; e < 2 * KS * nX; e += ST) {
+        const int r = e / (KS * nX), rem = e % (KS * nX), x = rem / KS, i = rem % KS;
+        double ar = 0.0, ai = 0.0, br = 0.0, bi = 0.0;
+        if (i < kb) {
+            const Opnd T = a.Gt0[c + r * nch], O = a.G0t[c + r * nch];
+            const int sx = R.idx[x];
+            const size_t ot = (size_t)(s0 + i) + (size_t)sx * T.ld, oo = (size_t)sx + (size_t)(s0 + i) * O.ld;
+            ar = T.re[ot]; ai = T.im[ot];
+            br = O.re[oo]; bi = O.im[oo];
+        }
+        t0r[T0(r, i, x)] = ar; t0i[T0(r, i, x)] = ai;
+        o0r[O0(r, x, i)] = br; o0i[O0(r, x, i)] = bi;
+    }
+    // region blocks of G0 of both replicas
+    const int mA = nA * nA, mAB = mA + nB * nB;
+    for (int e = tid; e < 2 * mAB; e += ST) {
+        const int r = e / mAB, f = e % mAB;
+        const int X = f < mA ? 0 : 1, nr = X ? nB : nA, base = X ? nA : 0, g = X ? f - mA : f;
+        const int p = g % nr, q = g / nr;
+        const Opnd G0 = a.G0[c + r * nch];
+        const size_t o = (size_t)R.idx[base + p] + (size_t)R.idx[base + q] * G0.ld;
+        double *M = R.M[r][X];
+        M[p + (size_t)q * nr] = G0.re[o];
+        M[(size_t)nr * nr + p + (size_t)q * nr] = G0.im[o];
+    }
+    __syncthreads();
+
+    int na[2] = {0, 0};
+    for (int m = 0; m < kb; ++m) {
+        for (int r = 0; r < 2; ++r) {
+            const int o = 1 - r;
+            const double Dr = PD(0, r, m), Di = PD(1, r, m);
+            const double omr = 1.0 - gtr[GT(r, m, m)], omi = -gti[GT(r, m, m)];
+            double pr, pi;
+            cmul(Dr, Di, omr, omi, pr, pi);
+            pr += 1.0;
+            const double p2 = pr * pr + pi * pi, ip2 = 1.0 / p2;
+            double prob = p2 * PD(2, r, m);
+            const double rr = (Dr * pr + Di * pi) * ip2, ri = (Di * pr - Dr * pi) * ip2;
+            // Grover factors, phase 1: zN (replica 1) or a (replica 2)
+            for (int q = tid; q < nX; q += ST) {
+                const int X = q < nA ? 0 : 1, base = X ? nA : 0, nr = X ? nB : nA, ql = q - base;
+                const double *Mo = R.M[o][X], *Moi = Mo + (size_t)nr * nr;
+                double sr = 0.0, si = 0.0;
+                if (r == 0) {
+                    // zN_q = Σ_k Gt0[m, X_k] (2 Mo[k, q] - δ)
+                    for (int k = 0; k < nr; ++k) {
+                        const double mr = 2.0 * Mo[k + (size_t)ql * nr] - (k == ql ? 1.0 : 0.0);
+                        const double mi = 2.0 * Moi[k + (size_t)ql * nr];
+                        const double xr = t0r[T0(r, m, base + k)], xi = t0i[T0(r, m, base + k)];
+                        sr += xr * mr - xi * mi;
+                        si += xr * mi + xi * mr;
+                    }
+                    vzr[q] = sr; vzi[q] = si;
+                    cmul(o0r[O0(r, q, m)], o0i[O0(r, q, m)], rr, ri, var_[q], vai[q]);
+                } else {
+                    // a_q = Σ_k (2 Mo[q, k] - δ) G0t[X_k, m]
+                    for (int k = 0; k < nr; ++k) {
+                        const double mr = 2.0 * Mo[ql + (size_t)k * nr] - (k == ql ? 1.0 : 0.0);
+                        const double mi = 2.0 * Moi[ql + (size_t)k * nr];
+                        const double xr = o0r[O0(r, base + k, m)], xi = o0i[O0(r, base + k, m)];
+                        sr += mr * xr - mi * xi;
+                        si += mr * xi + mi * xr;
+                    }
+                    var_[q] = sr; vai[q] = si;
+                    cmul(rr, ri, t0r[T0(r, m, q)], t0i[T0(r, m, q)], vzr[q], vzi[q]);
+                }
+            }
+            __syncthreads();
+            // phase 2: b = zN gmInv, Tau = 1 + b a per region
+            double acc[4] = {0.0, 0.0, 0.0, 0.0};
+            for (int q = tid; q < nX; q += ST) {
+                const int X = q < nA ? 0 : 1, base = X ? nA : 0, nr = X ? nB : nA, ql = q - base;
+                const double *Gv = R.gmInv[X], *Gvi = Gv + (size_t)nr * nr;
+                double br = 0.0, bi = 0.0;
+                for (int k = 0; k < nr; ++k) {
+                    const double gr = Gv[k + (size_t)ql * nr], gi = Gvi[k + (size_t)ql * nr];
+                    const double zr = vzr[base + k], zi = vzi[base + k];
+                    br += zr * gr - zi * gi;
+                    bi += zr * gi + zi * gr;
+                }
+                vbr[q] = br; vbi[q] = bi;
+                double tr, ti;
+                cmul(br, bi, var_[q], vai[q], tr, ti);
+                acc[2 * X] += tr;
+                acc[2 * X + 1] += ti;
+            }
+            block_sum4(acc, red);
+            const double tAr = 1.0 + acc[0], tAi = acc[1], tBr = 1.0 + acc[2], tBi = acc[3];
+            const double dA = tAr * tAr + tAi * tAi, dB = tBr * tBr + tBi * tBi;
+            prob *= pow(dA, R.lambda) * pow(dB, 1.0 - R.lambda);
+            if (!(PD(5, r, m) < prob)) continue;
+            // ---- accepted: gmInv a, and publish the replica's rows / columns at m
+            for (int q = tid; q < nX; q += ST) {
+                const int X = q < nA ? 0 : 1, base = X ? nA : 0, nr = X ? nB : nA, ql = q - base;
+                const double *Gv = R.gmInv[X], *Gvi = Gv + (size_t)nr * nr;
+                double sr = 0.0, si = 0.0;
+                for (int k = 0; k < nr; ++k) {
+                    const double gr = Gv[ql + (size_t)k * nr], gi = Gvi[ql + (size_t)k * nr];
+                    const double xr = var_[base + k], xi = vai[base + k];
+                    sr += gr * xr - gi * xi;
+                    si += gr * xi + gi * xr;
+                }
+                vzr[q] = sr; vzi[q] = si;
+                cxr[q] = o0r[O0(r, q, m)]; cxi[q] = o0i[O0(r, q, m)];
+                rxr[q] = t0r[T0(r, m, q)]; rxi[q] = t0i[T0(r, m, q)];
+            }
+            if (tid < KS) {
+                csr[tid] = gtr[GT(r, tid, m)]; csi[tid] = gti[GT(r, tid, m)];
+                rsr[tid] = gtr[GT(r, m, tid)]; rsi[tid] = gti[GT(r, m, tid)];
+            }
+            if (tid == 0) {
+                R.detg[0] *= dA;
+                R.detg[1] *= dB;
+            }
+            __syncthreads();
+            // inverse Grover matrices and the replica's region blocks of G0
+            const double itA2 = 1.0 / dA, itB2 = 1.0 / dB;
+            for (int e = tid; e < mAB; e += ST) {
+                const int X = e < mA ? 0 : 1, nr = X ? nB : nA, base = X ? nA : 0, g = X ? e - mA : e;
+                const int p = g % nr, q = g / nr;
+                // 1/Tau
+                const double tr_ = X ? tBr : tAr, ti_ = X ? tBi : tAi, it2 = X ? itB2 : itA2;
+                double bt_r, bt_i;   // b_q / Tau
+                cmul(vbr[base + q], vbi[base + q], tr_ * it2, -ti_ * it2, bt_r, bt_i);
+                double ur, ui;
+                cmul(vzr[base + p], vzi[base + p], bt_r, bt_i, ur, ui);
+                double *Gv = R.gmInv[X];
+                Gv[p + (size_t)q * nr] -= ur;
+                Gv[(size_t)nr * nr + p + (size_t)q * nr] -= ui;
+                double cr_, ci_, vr_, vi_;
+                cmul(cxr[base + p], cxi[base + p], rr, ri, cr_, ci_);
+                cmul(cr_, ci_, rxr[base + q], rxi[base + q], vr_, vi_);
+                double *M = R.M[r][X];
+                M[p + (size_t)q * nr] += vr_;
+                M[(size_t)nr * nr + p + (size_t)q * nr] += vi_;
+            }
+            // working blocks of the replica
+            for (int e = tid; e < KS * KS; e += ST) {
+                const int i = e % KS, j = e / KS;
+                double fr, fi, tr, ti;
+                cmul(csr[i], csi[i], rr, ri, fr, fi);
+                cmul(fr, fi, (j == m ? 1.0 : 0.0) - rsr[j], -rsi[j], tr, ti);
+                gtr[GT(r, i, j)] -= tr;
+                gti[GT(r, i, j)] -= ti;
+            }
+            for (int e = tid; e < nX * KS; e += ST) {
+                const int x = e % nX, j = e / nX;
+                double fr, fi, tr, ti;
+                cmul(cxr[x], cxi[x], rr, ri, fr, fi);
+                cmul(fr, fi, (j == m ? 1.0 : 0.0) - rsr[j], -rsi[j], tr, ti);
+                o0r[O0(r, x, j)] -= tr;
+                o0i[O0(r, x, j)] -= ti;
+            }
+            for (int e = tid; e < KS * nX; e += ST) {
+                const int i = e % KS, x = e / KS;
+                double fr, fi, tr, ti;
+                cmul(csr[i], csi[i], rr, ri, fr, fi);
+                cmul(fr, fi, rxr[x], rxi[x], tr, ti);
+                t0r[T0(r, i, x)] += tr;
+                t0i[T0(r, i, x)] += ti;
+            }
+            // bordering of the replica's W (same factor for all four matrices)
+            if (tid < KS * KS) {
+                const int t = tid % KS, u = tid / KS, nr_ = na[r];
+                const int at = accl[r][t], au = accl[r][u];
+                double xr = 0.0, xi = 0.0;   // x_t = Δ_t/(1+Δ_t) Gt[a_t, m]
+                if (t < nr_) cmul(PD(3, r, at), PD(4, r, at), csr[at], csi[at], xr, xi);
+                double yr = 1.0, yi = 0.0;   // y_u = Δ_u Gt[m, a_u]
+                if (u < nr_) cmul(PD(0, r, au), PD(1, r, au), rsr[au], rsi[au], yr, yi);
+                const bool old_row = t < nr_, new_row = t == nr_;
+                double lr = rr, li = ri;
+                if (old_row) cmul(xr, xi, rr, ri, lr, li);
+                double vr_, vi_;
+                cmul(lr, li, yr, yi, vr_, vi_);
+                const int w = r * KS * KS + t * KS + u;
+                if (old_row && u < nr_) {
+                    Wr[w] += vr_;
+                    Wi[w] += vi_;
+                } else if ((old_row && u == nr_) || (new_row && u <= nr_)) {
+                    Wr[w] = vr_;
+                    Wi[w] = vi_;
+                }
+            }
+            if (tid == 0) {
+                accl[r][na[r]] = m;
+                sv[r][m] |= 1 << 16;
+            }
+            __syncthreads();
+            ++na[r];
+        }
+    }
+    for (int r = 0; r < 2; ++r) {
+        const ChainDev &chr = a.chains[c + r * nch];
+        if (tid == 0) {
+            *chr.acc_cnt = na[r];
+            chr.counters[0] += kb;
+            chr.counters[1] += na[r];
+        }
+        if (tid < na[r]) chr.acc_idx[tid] = s0 + accl[r][tid];
+        for (int e = tid; e < na[r] * na[r]; e += ST) {
+            const int t = e % na[r], u = e / na[r];
+            chr.W[t + u * 64] = Wr[r * KS * KS + t * KS + u];
+            chr.W[64 * 64 + t + u * 64] = Wi[r * KS * KS + t * KS + u];
+        }
+        if (tid < kb) {
+            const int v = sv[r][tid];
+            chr.s[(size_t)(s0 + tid) + (size_t)(lt - 1) * chr.n_elem] =
+                (uint8_t)((v >> 16) ? (v >> 8) & 0xff : v & 0xff);
+        }
+    }
+#undef GT
+#undef T0
+#undef O0
+#undef PD
+}
+
+// M[r][X] = G0_r[X, X] for both replicas and regions (grid: elements x chains)
+__global__ void k_scee_gather(SceeArgs a) {
+    const int c = blockIdx.y;
+    const SceeRegion &R = a.reg[c];
+    const int nA = R.nA, nB = R.nB, mA = nA * nA, mAB = mA + nB * nB;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= 2 * mAB) return;
+    const int r = e / mAB, f = e % mAB;
+    const int X = f < mA ? 0 : 1, nr = X ? nB : nA, base = X ? nA : 0, g = X ? f - mA : f;
+    const int p = g % nr, q = g / nr;
+    const Opnd G0 = a.G0[c + r * a.nch];
+    const size_t o = (size_t)R.idx[base + p] + (size_t)R.idx[base + q] * G0.ld;
+    double *M = R.M[r][X];
+    M[p + (size_t)q * nr] = G0.re[o];
+    M[(size_t)nr * nr + p + (size_t)q * nr] = G0.im[o];
+}
+
+// GroverMatrix! (src/public/GF.jl:101-109): C = 2 C - M1 - M2 + I with C = M1 M2 on entry
+__global__ void k_scee_grover_form(const SceeRegion *reg) {
+    const int c = blockIdx.y;
+    const SceeRegion &R = reg[c];
+    const int nA = R.nA, nB = R.nB, mA = nA * nA, mAB = mA + nB * nB;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= mAB) return;
+    const int X = e < mA ? 0 : 1, nr = X ? nB : nA, g = X ? e - mA : e;
+    const int p = g % nr, q = g / nr;
+    const size_t o = p + (size_t)q * nr, pl = (size_t)nr * nr;
+    double *C = R.C[X];
+    const double *M1 = R.M[0][X], *M2 = R.M[1][X];
+    C[o] = 2.0 * C[o] - M1[o] - M2[o] + (p == q ? 1.0 : 0.0);
+    C[pl + o] = 2.0 * C[pl + o] - M1[pl + o] - M2[pl + o];
+}
+
+__global__ void k_scee_detg(const SceeRegion *reg, int nch, const double *la, const double *lb) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nch) return;
+    const SceeRegion &R = reg[c];
+    R.detg[0] = R.nA ? exp(2.0 * la[c]) : 1.0;
+    R.detg[1] = R.nB ? exp(2.0 * lb[c]) : 1.0;
+}
+
+__global__ void k_scee_accum(const SceeRegion *reg, int nch) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nch) return;
+    const SceeRegion &R = reg[c];
+    R.osum[0] += pow(R.detg[0] / R.detg[1], 1.0 / R.nlam);
+    R.osum[1] += 1.0;
+}
+
+__global__ void k_diag(const ChainDev *chains, const Opnd *X, int ndim, double v, int set) {
+    const int c = blockIdx.z;
+    const int n = dim_of(chains[c], ndim);
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63), j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (i >= n || j >= n) return;
+    const Opnd A = X[c];
+    const size_t o = (size_t)i + (size_t)j * A.ld;
+    if (set) {
+        A.re[o] = i == j ? 1.0 : 0.0;
+        A.im[o] = 0.0;
+    } else if (i == j) {
+        A.re[o] += v;
+    }
+}
+
+}  // namespace
+
+size_t scee_lds_bytes(int ks, int nx) {
+    return (size_t)(4 * ks * ks + 8 * ks * nx + 10 * nx + 4 * ks + 4 * ks * ks + 12 * ks + 4 * (ST / 64)) * sizeof(double);
+}
+
+hipError_t launch_scee_decide(hipStream_t st, const SceeArgs &a) {
+    const int ks = a.kb <= 8 ? 8 : 16;
+    if (a.kb < 1 || a.kb > 16) return hipErrorInvalidValue;
+    const size_t b = scee_lds_bytes(ks, a.nx_max);
+    if (ks == 8) {
+        if (hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_scee_decide<8>),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)b))
+            return e;
+        hipLaunchKernelGGL(k_scee_decide<8>, dim3(a.nch), dim3(ST), b, st, a);
+    } else {
+        if (hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_scee_decide<16>),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)b))
+            return e;
+        hipLaunchKernelGGL(k_scee_decide<16>, dim3(a.nch), dim3(ST), b, st, a);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_scee_gather(hipStream_t st, int nch, int nx_max, const SceeArgs &a) {
+    const int tot = 2 * nx_max * nx_max;
+    hipLaunchKernelGGL(k_scee_gather, dim3((tot + 255) / 256, nch), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_scee_grover_form(hipStream_t st, int nch, int total, const SceeRegion *reg) {
+    const int tot = total;   // largest nA² + nB² over the chains
+    hipLaunchKernelGGL(k_scee_grover_form, dim3((tot + 255) / 256, nch), dim3(256), 0, st, reg);
+    return hipGetLastError();
+}
+
+hipError_t launch_scee_detg(hipStream_t st, int nch, const SceeRegion *reg, const double *la, const double *lb) {
+    hipLaunchKernelGGL(k_scee_detg, dim3((nch + 63) / 64), dim3(64), 0, st, reg, nch, la, lb);
+    return hipGetLastError();
+}
+
+hipError_t launch_scee_accum(hipStream_t st, int nch, const SceeRegion *reg) {
+    hipLaunchKernelGGL(k_scee_accum, dim3((nch + 63) / 64), dim3(64), 0, st, reg, nch);
+    return hipGetLastError();
+}
+
+hipError_t launch_diag(hipStream_t st, int nchains, int max_n, const ChainDev *chains, const Opnd *X, int ndim,
+                       double v, int set) {
+    dim3 grid((max_n + 63) / 64, (max_n + 3) / 4, nchains);
+    hipLaunchKernelGGL(k_diag, grid, dim3(256), 0, st, chains, X, ndim, v, set);
+    return hipGetLastError();
+}

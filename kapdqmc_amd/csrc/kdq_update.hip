@@ -200,7 +200,8 @@ __global__ __launch_bounds__(KBT >= 32 ? 1024 : KBT * KBT) void k_decide_tu(Deci
     }
 }
 
-// Gout[I,J] = Gin[I,J] - Gin[I,A] · (W · Y[:,J]),  Y[t, j] = δ(A_t, j) - Gin[A_t, j].
+// Xout[I,J] = Xin[I,J] - sgn · Cin[I,A] · (W · Y[:,J]),  Y[t, j] = edge · δ(A_t, j) - sgn · Rin[A_t, j]
+// (mode 0: sgn = edge = 1; mode 1: sgn = -1, edge = 0, so the product is added).
 // 32 x 32 tiles, 256 threads: thread (ii = tid & 31, jg = tid >> 5) owns row ii, columns
 // 4jg .. 4jg+3. LDS (dynamic, k = maximum block size): W (k x k), Y then Z (k x 32),
 // Gin[I,A] (32 x k).
@@ -212,6 +213,9 @@ __global__ __launch_bounds__(256) void k_flush(FlushArgs f) {
     const int i0 = blockIdx.y * FT, j0 = blockIdx.x * FT;
     if (i0 >= n || j0 >= n) return;
     const Opnd Gi = f.Gin[c], Go = f.Gout[c];
+    const Opnd Cs = f.Cin ? f.Cin[c] : Gi, Rs = f.Rin ? f.Rin[c] : Gi;
+    const double sgn = f.mode == 0 ? 1.0 : -1.0;   // Xout = Xin - sgn · Cin[:,A] W Y
+    const double edge = f.mode == 0 ? 1.0 : 0.0;   // Y = edge·E_A - sgn·Rin[A,:]
     const int tid = threadIdx.x;
     const int ii = tid & 31, jg = tid >> 5;
     const int i = i0 + ii;
@@ -255,9 +259,9 @@ __global__ __launch_bounds__(256) void k_flush(FlushArgs f) {
         const int j = j0 + jj;
         double yr = 0.0, yi = 0.0;
         if (j < n) {
-            const size_t off = (size_t)aidx[t] + (size_t)j * Gi.ld;
-            yr = (aidx[t] == j ? 1.0 : 0.0) - Gi.re[off];
-            yi = -Gi.im[off];
+            const size_t off = (size_t)aidx[t] + (size_t)j * Rs.ld;
+            yr = (aidx[t] == j ? edge : 0.0) - sgn * Rs.re[off];
+            yi = -sgn * Rs.im[off];
         }
         Pr[t * (FT + 1) + jj] = yr;
         Pi[t * (FT + 1) + jj] = yi;
@@ -267,9 +271,9 @@ __global__ __launch_bounds__(256) void k_flush(FlushArgs f) {
         const int ig = i0 + r;
         double vr = 0.0, vi = 0.0;
         if (ig < n) {
-            const size_t off = (size_t)ig + (size_t)aidx[t] * Gi.ld;
-            vr = Gi.re[off];
-            vi = Gi.im[off];
+            const size_t off = (size_t)ig + (size_t)aidx[t] * Cs.ld;
+            vr = sgn * Cs.re[off];
+            vi = sgn * Cs.im[off];
         }
         Br[r * KP + t] = vr;
         Bi[r * KP + t] = vi;

@@ -34,6 +34,46 @@ def field_shape(m):
     return (m.Ns * m.nnidx.shape[1], m.Nt)
 
 
+def _descriptors(models, seeds, chains, delay, sweep0):
+    """kdq_model_desc / kdq_run_desc arrays for a list of model structs (plus the numpy
+    arrays they point into, which must outlive the call)."""
+    n = len(models)
+    kind = getattr(models[0], "kind", "tU")
+    keep = []
+    descs = (L.ModelDesc * n)()
+    runs = (L.RunDesc * n)()
+    for c, m in enumerate(models):
+        if getattr(m, "kind", "tU") != kind:
+            raise ValueError("all chains of a batch must share the model kind")
+        f = lambda a: np.asfortranarray(np.asarray(a, dtype=np.float64))
+        arrs = dict(eta=f(m.η), gamma=f(m.γ), K=f(np.real(m.K)), eK=f(np.real(m.eK)),
+                    eKinv=f(np.real(m.eKinv)), halfeK=f(np.real(m.HalfeK)),
+                    halfeKinv=f(np.real(m.HalfeKinv)), Pt=f(np.real(m.Pt)))
+        if kind != "tUV":
+            arrs["alpha"] = f(m.α)
+        nodes = np.ascontiguousarray(m.nodes, dtype=np.int64)
+        keep += list(arrs.values()) + [nodes]
+        d = descs[c]
+        d.kind, d.ns, d.nt, d.n_nodes = KIND_CODE[kind], m.Ns, m.Nt, len(nodes)
+        if kind != "tU":
+            # model.nnidx in column-major (linear) order as 1-based (x, y) pairs
+            bonds = np.ascontiguousarray(np.transpose(m.nnidx, (1, 0, 2)).reshape(-1, 2), dtype=np.int32)
+            keep.append(bonds)
+            d.z = m.nnidx.shape[1]
+            d.bonds = bonds.ctypes.data_as(C.POINTER(C.c_int32))
+        if kind == "tUV":
+            d.a, d.is_complex, d.U, d.V = m.a, int(m.Type is complex), m.U, m.V
+        d.nodes = nodes.ctypes.data_as(C.POINTER(C.c_int64))
+        d.lattice = L.LATTICE_CODE[m.Lattice]
+        d.site[0], d.site[1] = m.site
+        d.ht = m.Ht
+        for k, v in arrs.items():
+            setattr(d, k, _dp(v))
+        r = runs[c]
+        r.seed, r.chain, r.sweep0, r.precision, r.delay = seeds[c], chains[c], sweep0, L.PREC_FP64, delay
+    return descs, runs, keep
+
+
 class DQMCBatch:
     """A batch of chains of one model kind (t-U, t-V or t-U-V) on the current GPU.
 
@@ -55,38 +95,7 @@ class DQMCBatch:
         self.models = list(models)
         self.kind = getattr(models[0], "kind", "tU")
         self.nobs = NOBS[self.kind]
-        self._keep = []
-        descs = (L.ModelDesc * n)()
-        runs = (L.RunDesc * n)()
-        for c, m in enumerate(models):
-            if getattr(m, "kind", "tU") != self.kind:
-                raise ValueError("all chains of a batch must share the model kind")
-            f = lambda a: np.asfortranarray(np.asarray(a, dtype=np.float64))
-            arrs = dict(eta=f(m.η), gamma=f(m.γ), K=f(np.real(m.K)), eK=f(np.real(m.eK)),
-                        eKinv=f(np.real(m.eKinv)), halfeK=f(np.real(m.HalfeK)),
-                        halfeKinv=f(np.real(m.HalfeKinv)), Pt=f(np.real(m.Pt)))
-            if self.kind != "tUV":
-                arrs["alpha"] = f(m.α)
-            nodes = np.ascontiguousarray(m.nodes, dtype=np.int64)
-            self._keep += list(arrs.values()) + [nodes]
-            d = descs[c]
-            d.kind, d.ns, d.nt, d.n_nodes = KIND_CODE[self.kind], m.Ns, m.Nt, len(nodes)
-            if self.kind != "tU":
-                # model.nnidx in column-major (linear) order as 1-based (x, y) pairs
-                bonds = np.ascontiguousarray(np.transpose(m.nnidx, (1, 0, 2)).reshape(-1, 2), dtype=np.int32)
-                self._keep.append(bonds)
-                d.z = m.nnidx.shape[1]
-                d.bonds = bonds.ctypes.data_as(C.POINTER(C.c_int32))
-            if self.kind == "tUV":
-                d.a, d.is_complex, d.U, d.V = m.a, int(m.Type is complex), m.U, m.V
-            d.nodes = nodes.ctypes.data_as(C.POINTER(C.c_int64))
-            d.lattice = L.LATTICE_CODE[m.Lattice]
-            d.site[0], d.site[1] = m.site
-            d.ht = m.Ht
-            for k, v in arrs.items():
-                setattr(d, k, _dp(v))
-            r = runs[c]
-            r.seed, r.chain, r.sweep0, r.precision, r.delay = seeds[c], chains[c], sweep0, L.PREC_FP64, delay
+        descs, runs, self._keep = _descriptors(self.models, seeds, chains, delay, sweep0)
         h = C.c_void_p()
         L.check(self.L.kdq_create(descs, runs, n, C.byref(h)))
         self.h = h
@@ -175,6 +184,12 @@ class DQMCBatch:
         self._chk(self.L.kdq_get_counters(self.h, _dp(out)))
         return out
 
+    def logdet(self) -> np.ndarray:
+        """log|det(U_L U_R)| of the overlap factorised by the last get_G! of each chain."""
+        out = np.zeros(self.n)
+        self._chk(self.L.kdq_get_logdet(self.h, _dp(out)))
+        return out
+
 
 def Initial_s(model, seed: int = 0, chain: int = 0) -> np.ndarray:
     """Initial_s (src/tU/GreenMatrix.jl:1-8, src/tV/GreenMatrix.jl:1-10,
@@ -227,3 +242,111 @@ def phy_update(path: str, model, s: np.ndarray, Sweeps: int, record: bool = Fals
             for r in rows:
                 fh.write(",".join(repr(float(x)) for x in r) + "\n")
     return s
+
+
+class SCEEBatch:
+    """Swap-replica Rényi-2 chains of the t-U model (ctrl_SCEEicr, src/tU/SCEE.jl:1-262).
+
+    indexA / indexB: the reference's 1-based site lists; lam, nlam: λ and Nλ. Each chain
+    holds two replica fields (ss[1], ss[2]); replica r draws from Philox stream 1 + r.
+    delay: sites per decision block (1..16).
+    """
+
+    def __init__(self, models, indexA, indexB, lam: float, nlam: int, seeds, chains=None,
+                 delay: int = 8, sweep0: int = 0):
+        self.L = L.load()
+        if not isinstance(models, (list, tuple)):
+            models = [models]
+        n = len(models)
+        seeds = list(seeds) if isinstance(seeds, (list, tuple)) else [seeds] * n
+        chains = list(range(n)) if chains is None else list(chains)
+        self.models = list(models)
+        descs, runs, self._keep = _descriptors(self.models, seeds, chains, delay, sweep0)
+        regs = (L.SceeDesc * n)()
+        for c in range(n):
+            ia = np.ascontiguousarray(indexA[c] if _is_nested(indexA) else indexA, dtype=np.int64)
+            ib = np.ascontiguousarray(indexB[c] if _is_nested(indexB) else indexB, dtype=np.int64)
+            self._keep += [ia, ib]
+            regs[c].nA, regs[c].nB = len(ia), len(ib)
+            regs[c].indexA = ia.ctypes.data_as(C.POINTER(C.c_int64))
+            regs[c].indexB = ib.ctypes.data_as(C.POINTER(C.c_int64))
+            regs[c].lam, regs[c].nlam = lam, nlam
+        h = C.c_void_p()
+        L.check(self.L.kdq_scee_create(descs, runs, regs, n, C.byref(h)))
+        self.h, self.n = h, n
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.kdq_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        L.check(rc, self.h)
+
+    def set_fields(self, c: int, replica: int, s):
+        a = np.asfortranarray(np.asarray(s, dtype=np.uint8))
+        if a.shape != field_shape(self.models[c]):
+            raise ValueError(f"s must have shape {field_shape(self.models[c])}")
+        self._chk(self.L.kdq_scee_set_fields(self.h, c, replica, a.ctypes.data_as(C.POINTER(C.c_uint8))))
+
+    def get_fields(self, c: int, replica: int) -> np.ndarray:
+        a = np.zeros(field_shape(self.models[c]), dtype=np.uint8, order="F")
+        self._chk(self.L.kdq_scee_get_fields(self.h, c, replica, a.ctypes.data_as(C.POINTER(C.c_uint8))))
+        return a
+
+    def init(self):
+        self._chk(self.L.kdq_scee_init(self.h))
+
+    def sweeps(self, n: int) -> np.ndarray:
+        """n sweeps of every chain (set-up included); returns [nchains, n] values of O."""
+        O = np.zeros((self.n, n))
+        self._chk(self.L.kdq_scee_sweeps(self.h, n, _dp(O)))
+        return O
+
+    def G4(self, c: int, replica: int):
+        ns = self.models[c].Ns
+        bufs = [np.zeros(2 * ns * ns) for _ in range(4)]   # interleaved, column-major
+        self._chk(self.L.kdq_scee_get_G4(self.h, c, replica, *[_dp(x) for x in bufs]))
+        return [(x[0::2] + 1j * x[1::2]).reshape((ns, ns), order="F") for x in bufs]   # Gt, G0, Gt0, G0t
+
+    def detg(self) -> np.ndarray:
+        out = np.zeros((self.n, 2))
+        self._chk(self.L.kdq_scee_detg(self.h, _dp(out)))
+        return out
+
+
+def _is_nested(x) -> bool:
+    return len(x) > 0 and hasattr(x[0], "__len__")
+
+
+def ctrl_SCEEicr(path: str, model, indexA, indexB, Sweeps: int, λ: float, Nλ: int, ss, record: bool = False,
+                 *, seed: int = 0, chain: int = 0, delay: int = 8, sweep0: int = 0):
+    """ctrl_SCEEicr on the GPU (t-U, src/tU/SCEE.jl:1-262): mutates the two replica fields
+    ss[0], ss[1] in place, returns ss, and appends the row [λ, O_1, …, O_Sweeps] to the
+    reference's CSV file name when ``record`` is true."""
+    b = SCEEBatch([model], indexA, indexB, λ, Nλ, [seed], [chain], delay=delay, sweep0=sweep0)
+    try:
+        b.set_fields(0, 0, ss[0])
+        b.set_fields(0, 1, ss[1])
+        O = b.sweeps(Sweeps)[0]
+        ss[0][...] = b.get_fields(0, 0)
+        ss[1][...] = b.get_fields(0, 1)
+    finally:
+        b.close()
+    if record:
+        name = _lattice_tag(model.Lattice)
+        if model.Θquench == 0.0:
+            fn = (f"tUSCEE{name}_t{model.Ht}U{model.Hu1}size{model.site}Δt{model.Δt}Θ{model.Θrelax}"
+                  f"N{Nλ}BS{model.BatchSize}.csv")
+        else:
+            fn = (f"tUSCEE{name}_t{model.Ht}U{model.Hu1}_{model.Hu2}size{model.site}Δt{model.Δt}"
+                  f"Θ{model.Θrelax}_{model.Θquench}N{Nλ}BS{model.BatchSize}.csv")
+        with open(os.path.join(path, fn), "a") as fh:
+            fh.write(",".join(repr(float(x)) for x in [λ, *O]) + "\n")
+    return ss
