@@ -54,6 +54,23 @@ def family_work(fam: str, N: int, Np: int):
     return "hbm", 16.0 * N * N, "byte"
 
 
+def measured_traffic(family: str, chains: int, delay: int):
+    """HBM bytes per launch of `family` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summaries under profiles/ (scaled by a copy of known size), for the same configuration;
+    None when no such measurement exists."""
+    import glob
+    import re
+    for fn in sorted(glob.glob(os.path.join(ROOT, "profiles", "*_pmc.md"))):
+        for blk in re.findall(r"```json\s*(\{.*?\})\s*```", open(fn, encoding="utf-8").read(), re.S):
+            try:
+                d = json.loads(blk)
+            except ValueError:
+                continue
+            if d.get("kernel_family") == family and d.get("chains") == chains and d.get("delay") == delay:
+                return float(d["traffic_bytes"])
+    return None
+
+
 def cpu_baseline(m, tau, nslices: int):
     """The oracle (CPU FP64 restatement of the reference algorithm, complex dense GEMMs and
     sequential Sherman-Morrison) on a bounded sample: chain initialisation + `nslices`
@@ -97,7 +114,9 @@ def run_batch(m, nch: int, seed0: int, delay: int, warmup: int, steps: int, prof
         b.set_fields(c, initial_fields(seed0 + c, c, N, m.Nt))
     fams_all = ["wrap", "chain", "get_G", "decide", "flush", "qr", "inv", "meas_wrap", "measure"]
     b.profile(fams_all)
-    b.phy_update(max(warmup, 1), True)
+    for w in range(max(warmup, 1)):
+        b.phy_update(1, True)
+        print(f"bench: {nch} chain(s), warm-up sweep {w + 1} done", file=sys.stderr, flush=True)
     prof_w = b.profile_read()
     dominant = max(prof_w, key=lambda f: prof_w[f][0])
     b.profile(list(prof_w) if profile_all else [dominant])
@@ -106,6 +125,7 @@ def run_batch(m, nch: int, seed0: int, delay: int, warmup: int, steps: int, prof
     rows = b.phy_update(steps, True)          # [nch, steps, nobs]
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    print(f"bench: {nch} chain(s), {steps} timed sweeps in {dt:.2f} s", file=sys.stderr, flush=True)
     prof = b.profile_read()
     cnt = b.counters()
     breakdown = {f: {"ms_per_sweep": prof_w[f][0] / max(warmup, 1),
@@ -127,6 +147,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-slices", type=int, default=20)
     ap.add_argument("--profile-all", action="store_true", help="time every kernel family (slower)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only copy a MIB-sized buffer 5 times (scale for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only (independent chains shard by chain "
@@ -139,6 +161,15 @@ def main():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
     torch.cuda.set_device(0)
+    if args.calibrate_copy:
+        n = args.calibrate_copy * (1 << 20) // 8
+        x = torch.rand(n, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 8 * n, "copy_bytes_written": 8 * n, "copies": 5}))
+        return
     from kapdqmc_amd import stats
 
     m = model_cfg2()
@@ -168,7 +199,8 @@ def main():
     else:
         achieved, peak, runit = work / avg_s / 1e9, PEAK_HBM_GBS, "GB/s"
     roofline = {"bound": bound, "achieved": achieved, "peak": peak, "unit": runit,
-                "frac": achieved / peak, "traffic": None, "kernel_family": dominant,
+                "frac": achieved / peak, "traffic": measured_traffic(dominant, C, args.delay),
+                "kernel_family": dominant,
                 "avg_launch_us": avg_s * 1e6, "launches": nl,
                 "work_per_launch": f"{work:.4g} {unit}"}
 
