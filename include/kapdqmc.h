@@ -122,6 +122,16 @@ int kdq_measure(kdq_handle h, int32_t lt, double *out);
 /* Counters: out[0] = proposals, out[1] = accepted, per chain (nchains x 2). */
 int kdq_get_counters(kdq_handle h, double *out);
 
+/* Run options (no counterpart in the reference):
+ *  KDQ_OPT_GRAPH      1 (default): kdq_phy_update captures one sweep into a HIP graph (on
+ *                     the first call, or when `record` changes) and replays it; 0: launches
+ *                     every kernel from the host each sweep. While kernel timing is on
+ *                     (kdq_profile mask non-zero) sweeps run with host launches.
+ *  KDQ_OPT_DELAY      decision block size k of t-U chains (1..64; 1..16 with replicas).
+ *  KDQ_OPT_GEMM_TILE  0 (default: by occupancy), 32 or 64: GEMM output tile edge. */
+enum { KDQ_OPT_GRAPH = 0, KDQ_OPT_DELAY = 1, KDQ_OPT_GEMM_TILE = 2 };
+int kdq_set_option(kdq_handle h, int32_t key, int32_t value);
+
 /* log|det(U_L U_R)| of the overlap factorised by the last get_G! of every chain (nchains
  * doubles). No counterpart in the reference, whose get_G! (src/public/Gupdate.jl:139-157)
  * discards the LU diagonal; the weight of a configuration up to the chain normalisation. */

@@ -81,9 +81,18 @@ struct kdq_handle_s {
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
     double prof_ms[KDQ_PROF_FAMILIES] = {};
     int64_t prof_n[KDQ_PROF_FAMILIES] = {};
+    // options (kdq_set_option) and the captured sweep
+    int use_graph = 1, gemm_tile = 0;
+    uint32_t *d_sweep = nullptr;              // device sweep counter read by graph replays
+    const uint32_t *sweep_dev = nullptr;      // non-null while a sweep is being captured
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int g_record = -1;
 
     ~kdq_handle_s() {
         if (stream) hipStreamSynchronize(stream);
+        if (gexec) hipGraphExecDestroy(gexec);
+        if (graph) hipGraphDestroy(graph);
         for (auto &c : ch)
             for (void *p : c.allocs) hipFree(p);
         for (void *p : dev_misc) hipFree(p);
@@ -143,6 +152,8 @@ void harvest(kdq_handle h) {
     h->ev_pending.clear();
 }
 
+// Brackets the launches of one kernel family with events on the handle's stream (eager
+// launches only: a sweep is not captured into a graph while a family is timed).
 struct ProfScope {
     kdq_handle h;
     int fam;
@@ -199,7 +210,8 @@ int maxdim(kdq_handle h, int code) { return code == DIM_NS ? h->max_ns : h->max_
 
 int gemm(kdq_handle h, bool ac, bool bc, int opa, int opb, const GemmArgs &g, int fam = KDQ_PROF_GEMM_OTHER) {
     ProfScope ps(h, fam);
-    HIPCHK(launch_gemm(h->stream, h->nch, maxdim(h, g.mdim), maxdim(h, g.ndim), maxdim(h, g.kdim), ac, bc, opa, opb, g));
+    HIPCHK(launch_gemm(h->stream, h->nch, maxdim(h, g.mdim), maxdim(h, g.ndim), maxdim(h, g.kdim), ac, bc, opa, opb, g,
+                       h->gemm_tile));
     return KDQ_OK;
 }
 
@@ -342,6 +354,7 @@ int update_family_r2(kdq_handle h, int lt, int dir, uint32_t sweep, int family) 
         a.block_offset = h->d_block_offset;
         a.nblocks = h->d_nblocks;
         a.lt = lt; a.dir = dir; a.family = family; a.block = b; a.sweep = sweep;
+        a.sweep_dev = h->sweep_dev;
         {
             ProfScope ps(h, KDQ_PROF_DECIDE);
             HIPCHK(launch_decide_r2(h->stream, h->nch, a));
@@ -364,6 +377,7 @@ int update_layer(kdq_handle h, int lt, int dir, uint32_t sweep) {
         a.chains = h->d_chains;
         a.G = h->t_G[h->cur];
         a.lt = lt; a.dir = dir; a.s0 = s0; a.kb = kb; a.sweep = sweep;
+        a.sweep_dev = h->sweep_dev;
         // t-U only; t-V layers go through slice_tv
         {
             ProfScope ps(h, KDQ_PROF_DECIDE);
@@ -480,46 +494,99 @@ bool in_window(kdq_handle h, int thidx, bool forward) {
     return d >= -1 && d <= 1;
 }
 
+// One sweep (forward and backward over all slices, stabilisation at the nodes, measurements
+// in the reference's windows) for every chain. Only launches: no host reads, no
+// synchronisation, no memset, so the sequence can be captured into a graph.
+int sweep_body(kdq_handle h, int record, uint32_t sweep) {
+    const int NN = h->nn, nt = h->nt, thidx = NN / 2 + 1;
+    const bool tv = h->kind == KDQ_MODEL_TV;
+    (void)NN;
+    if (record) HIPCHK(launch_zero_meas(h->stream, h->nch, h->d_chains));
+    for (int lt = 1; lt <= nt; ++lt) {
+        if (tv) {
+            if (int r = slice_tv_forward(h, h->t_G[h->cur], h->t_T, lt, true, sweep, KDQ_PROF_WRAP)) return r;
+        } else {
+            if (int r = wrap_B(h, h->t_G[h->cur], h->t_T, lt, true)) return r;
+            if (int r = update_layer(h, lt, 0, sweep)) return r;
+        }
+        if (record && in_window(h, thidx, true))
+            if (int r = measure(h, lt)) return r;
+        if (is_node(h, lt)) {
+            h->idx += 1;
+            if (int r = propagate_R(h, h->t_UR[h->idx - 2], h->t_UR[h->idx - 1], h->idx - 1)) return r;
+            if (int r = get_G(h, h->idx)) return r;
+        }
+    }
+    for (int lt = nt; lt >= 1; --lt) {
+        if (tv) {
+            if (int r = slice_tv_backward(h, h->t_G[h->cur], h->t_T, lt, true, sweep, KDQ_PROF_WRAP)) return r;
+            if (record && in_window(h, thidx, false))
+                if (int r = measure(h, lt - 1)) return r;
+        } else {
+            if (int r = update_layer(h, lt, 1, sweep)) return r;
+            if (record && in_window(h, thidx, false))
+                if (int r = measure(h, lt)) return r;
+            if (int r = wrap_B(h, h->t_G[h->cur], h->t_T, lt, false)) return r;
+        }
+        if (is_node(h, lt - 1)) {
+            h->idx -= 1;
+            if (int r = propagate_L(h, h->t_ULH[h->idx], h->t_ULH[h->idx - 1], h->idx)) return r;
+            if (int r = get_G(h, h->idx)) return r;
+        }
+    }
+    if (h->sweep_dev) HIPCHK(launch_bump(h->stream, h->d_sweep));
+    return KDQ_OK;
+}
+
+void drop_graph(kdq_handle h) {
+    if (h->gexec) hipGraphExecDestroy(h->gexec);
+    if (h->graph) hipGraphDestroy(h->graph);
+    h->gexec = nullptr;
+    h->graph = nullptr;
+    h->g_record = -1;
+}
+
+// Captures sweep_body into h->gexec (the sweep counter comes from h->d_sweep, which the
+// graph increments at its end). The host-side state a sweep mutates (current G buffer, node
+// index) must come back to its start value, or the graph would not be replayable.
+int capture_sweep(kdq_handle h, int record) {
+    drop_graph(h);
+    const int cur0 = h->cur, idx0 = h->idx;
+    HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed));
+    h->sweep_dev = h->d_sweep;
+    int r = sweep_body(h, record, 0);
+    h->sweep_dev = nullptr;
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(h->stream, &g);
+    if (r) {
+        if (g) hipGraphDestroy(g);
+        return r;
+    }
+    HIPCHK(e);
+    if (h->cur != cur0 || h->idx != idx0) {
+        hipGraphDestroy(g);
+        return fail(h, KDQ_ESTATE, "sweep is not replayable as a graph (state parity)");
+    }
+    h->graph = g;
+    HIPCHK(hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0));
+    h->g_record = record;
+    return KDQ_OK;
+}
+
 int phy_update(kdq_handle h, int sweeps, int record, double *rows) {
-    const int NN = h->nn, nt = h->nt, thidx = NN / 2 + 1, no = h->nobs;
+    const int no = h->nobs;
     if (int r = init_chain(h)) return r;
     std::vector<double> tmp(no * h->nch);
-    const bool tv = h->kind == KDQ_MODEL_TV;
+    const bool graph = h->use_graph && h->prof_mask == 0;
+    if (graph) {
+        HIPCHK(hipMemcpyAsync(h->d_sweep, &h->sweep, sizeof(uint32_t), hipMemcpyHostToDevice, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        if (!h->gexec || h->g_record != record)
+            if (int r = capture_sweep(h, record)) return r;
+    }
     for (int sw = 0; sw < sweeps; ++sw) {
-        const uint32_t sweep = h->sweep;
-        if (record) HIPCHK(launch_zero_meas(h->stream, h->nch, h->d_chains));
-        for (int lt = 1; lt <= nt; ++lt) {
-            if (tv) {
-                if (int r = slice_tv_forward(h, h->t_G[h->cur], h->t_T, lt, true, sweep, KDQ_PROF_WRAP)) return r;
-            } else {
-                if (int r = wrap_B(h, h->t_G[h->cur], h->t_T, lt, true)) return r;
-                if (int r = update_layer(h, lt, 0, sweep)) return r;
-            }
-            if (record && in_window(h, thidx, true))
-                if (int r = measure(h, lt)) return r;
-            if (is_node(h, lt)) {
-                h->idx += 1;
-                if (int r = propagate_R(h, h->t_UR[h->idx - 2], h->t_UR[h->idx - 1], h->idx - 1)) return r;
-                if (int r = get_G(h, h->idx)) return r;
-            }
-        }
-        for (int lt = nt; lt >= 1; --lt) {
-            if (tv) {
-                if (int r = slice_tv_backward(h, h->t_G[h->cur], h->t_T, lt, true, sweep, KDQ_PROF_WRAP)) return r;
-                if (record && in_window(h, thidx, false))
-                    if (int r = measure(h, lt - 1)) return r;
-            } else {
-                if (int r = update_layer(h, lt, 1, sweep)) return r;
-                if (record && in_window(h, thidx, false))
-                    if (int r = measure(h, lt)) return r;
-                if (int r = wrap_B(h, h->t_G[h->cur], h->t_T, lt, false)) return r;
-            }
-            if (is_node(h, lt - 1)) {
-                h->idx -= 1;
-                if (int r = propagate_L(h, h->t_ULH[h->idx], h->t_ULH[h->idx - 1], h->idx)) return r;
-                if (int r = get_G(h, h->idx)) return r;
-            }
-        }
+        if (graph) HIPCHK(hipGraphLaunch(h->gexec, h->stream));
+        else if (int r = sweep_body(h, record, h->sweep)) return r;
         if (record && rows) {
             if (int r = read_meas(h, tmp.data())) return r;
             for (int c = 0; c < h->nch; ++c)
@@ -533,6 +600,7 @@ int phy_update(kdq_handle h, int sweeps, int record, double *rows) {
 int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t nchains,
                 const uint32_t *streams, kdq_handle *out);
 void scee_free(kdq_handle h);
+void scee_set_kb(kdq_handle h, int kb);
 }  // namespace
 
 // =================================================================== C ABI
@@ -817,6 +885,9 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
         h->d_info = (int *)q2;
         h->d_qinfo = (int *)q3;
         h->d_piv = (int *)q4;
+        void *q5 = nullptr;
+        if (hipMalloc(&q5, sizeof(uint32_t)) == hipSuccess) h->dev_misc.push_back(q5); else r = 1;
+        h->d_sweep = (uint32_t *)q5;
         if (!r) {
             hipMemset(q2, 0, sizeof(int) * nchains);
             hipMemset(q3, 0, sizeof(int) * nchains);
@@ -960,6 +1031,31 @@ int kdq_profile_read(kdq_handle h, double *ms, int64_t *launches) {
     return KDQ_OK;
 }
 
+int kdq_set_option(kdq_handle h, int32_t key, int32_t value) {
+    if (!h) return fail(h, KDQ_EARG, "kdq_set_option: null handle");
+    switch (key) {
+        case KDQ_OPT_GRAPH:
+            h->use_graph = value ? 1 : 0;
+            break;
+        case KDQ_OPT_DELAY: {
+            const int kmax = h->scee ? 16 : 64;
+            if (h->kind != KDQ_MODEL_TU || value < 1 || value > kmax)
+                return fail(h, KDQ_EARG, "kdq_set_option: delay is adjustable for t-U chains only, 1..64 (1..16 with replicas)");
+            h->delay = value;
+            if (h->scee) scee_set_kb(h, value);
+            break;
+        }
+        case KDQ_OPT_GEMM_TILE:
+            if (value != 0 && value != 32 && value != 64) return fail(h, KDQ_EARG, "kdq_set_option: GEMM tile is 0, 32 or 64");
+            h->gemm_tile = value;
+            break;
+        default:
+            return fail(h, KDQ_EARG, "kdq_set_option: unknown option");
+    }
+    drop_graph(h);
+    return KDQ_OK;
+}
+
 int kdq_get_logdet(kdq_handle h, double *out) {
     if (!h || !out) return fail(h, KDQ_EARG, "kdq_get_logdet: bad argument");
     HIPCHK(hipMemcpyAsync(out, h->d_logdet, sizeof(double) * h->nch, hipMemcpyDeviceToHost, h->stream));
@@ -1005,6 +1101,8 @@ enum { SX_G0 = 0, SX_GT0 = 1, SX_G0T = 2 };
 Opnd *xcur(kdq_handle h, int k) { return h->scee->t_X[k][h->scee->cur[k]]; }
 Opnd *xspare(kdq_handle h, int k) { return h->scee->t_X[k][1 - h->scee->cur[k]]; }
 void xflip(kdq_handle h, int k) { h->scee->cur[k] ^= 1; }
+
+void scee_set_kb(kdq_handle h, int kb) { h->scee->kb = kb; }
 
 void scee_free(kdq_handle h) {
     SceeState *S = h->scee;

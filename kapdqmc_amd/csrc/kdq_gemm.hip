@@ -214,11 +214,11 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
 }
 
 template <int AC, int BC, int OPA, int OPB>
-hipError_t run(hipStream_t st, int nchains, int max_m, int max_n, int max_k, const GemmArgs &g) {
+hipError_t run(hipStream_t st, int nchains, int max_m, int max_n, int max_k, const GemmArgs &g, int tile) {
     if (g.inS.kind && max_k > KMAX) return hipErrorInvalidValue;
-    // 64 x 64 tiles unless that leaves most compute units idle
+    // 64 x 64 tiles unless that leaves most compute units idle (or as forced by `tile`)
     const long t64 = (long)((max_n + 63) / 64) * ((max_m + 63) / 64) * nchains;
-    if (t64 >= 192) {
+    if (tile == 64 || (tile == 0 && t64 >= 192)) {
         dim3 grid((max_n + 63) / 64, (max_m + 63) / 64, nchains);
         hipLaunchKernelGGL((k_gemm<AC, BC, OPA, OPB, 64, 64>), grid, dim3(256), 0, st, g);
     } else {
@@ -231,17 +231,17 @@ hipError_t run(hipStream_t st, int nchains, int max_m, int max_n, int max_k, con
 }  // namespace
 
 hipError_t launch_gemm(hipStream_t st, int nchains, int max_m, int max_n, int max_k, bool a_cplx, bool b_cplx,
-                       int opa, int opb, const GemmArgs &g) {
+                       int opa, int opb, const GemmArgs &g, int tile) {
     const int code = (a_cplx ? 1000 : 0) + (b_cplx ? 100 : 0) + opa * 10 + opb;
     switch (code) {
-        case 100: return run<0, 1, 0, 0>(st, nchains, max_m, max_n, max_k, g);     // eK · G
-        case 110: return run<0, 1, 1, 0>(st, nchains, max_m, max_n, max_k, g);     // eKᵀ · X
-        case 1000: return run<1, 0, 0, 0>(st, nchains, max_m, max_n, max_k, g);    // T · eKinv
-        case 1100: return run<1, 1, 0, 0>(st, nchains, max_m, max_n, max_k, g);    // U_R · X
-        case 1120: return run<1, 1, 2, 0>(st, nchains, max_m, max_n, max_k, g);    // U_Lᴴᴴ · U_R, Uᴴ U
-        case 1102: return run<1, 1, 0, 2>(st, nchains, max_m, max_n, max_k, g);
-        case 0: return run<0, 0, 0, 0>(st, nchains, max_m, max_n, max_k, g);
-        case 10: return run<0, 0, 1, 0>(st, nchains, max_m, max_n, max_k, g);
+        case 100: return run<0, 1, 0, 0>(st, nchains, max_m, max_n, max_k, g, tile);     // eK · G
+        case 110: return run<0, 1, 1, 0>(st, nchains, max_m, max_n, max_k, g, tile);     // eKᵀ · X
+        case 1000: return run<1, 0, 0, 0>(st, nchains, max_m, max_n, max_k, g, tile);    // T · eKinv
+        case 1100: return run<1, 1, 0, 0>(st, nchains, max_m, max_n, max_k, g, tile);    // U_R · X
+        case 1120: return run<1, 1, 2, 0>(st, nchains, max_m, max_n, max_k, g, tile);    // U_Lᴴᴴ · U_R, Uᴴ U
+        case 1102: return run<1, 1, 0, 2>(st, nchains, max_m, max_n, max_k, g, tile);
+        case 0: return run<0, 0, 0, 0>(st, nchains, max_m, max_n, max_k, g, tile);
+        case 10: return run<0, 0, 1, 0>(st, nchains, max_m, max_n, max_k, g, tile);
         default: return hipErrorInvalidValue;
     }
 }

@@ -18,7 +18,7 @@ struct GemmArgs {
 };
 
 hipError_t launch_gemm(hipStream_t st, int nchains, int max_m, int max_n, int max_k, bool a_cplx, bool b_cplx,
-                       int opa, int opb, const GemmArgs &g);
+                       int opa, int opb, const GemmArgs &g, int tile = 0);   // tile: 0 auto, 32, 64
 
 // Local update of one block of sites [s0, s0 + kb) at slice lt (t-U model).
 struct DecideArgs {
@@ -27,6 +27,7 @@ struct DecideArgs {
     int lt, dir, s0, kb;
     uint32_t sweep;
     int pad;
+    const uint32_t *sweep_dev;   // when set, the sweep counter is read here (graph replays)
 };
 hipError_t launch_decide_tu(hipStream_t st, int nchains, const DecideArgs &a);
 
@@ -110,6 +111,7 @@ struct DecideR2Args {
     int lt, dir, family, block;
     uint32_t sweep;
     int pad;
+    const uint32_t *sweep_dev;   // when set, the sweep counter is read here (graph replays)
 };
 hipError_t launch_decide_r2(hipStream_t st, int nchains, const DecideR2Args &a);
 
@@ -130,6 +132,7 @@ hipError_t launch_real_to_cplx(hipStream_t st, int nchains, int max_m, int max_n
                                const Opnd *src, const Opnd *dst, int mdim, int ndim);
 hipError_t launch_initial_s(hipStream_t st, int nchains, int max_elem, int nt, const ChainDev *chains);
 hipError_t launch_zero_meas(hipStream_t st, int nchains, const ChainDev *chains);
+hipError_t launch_bump(hipStream_t st, uint32_t *counter);   // *counter += 1 (in-graph sweep counter)
 
 // ---------------------------------------------------------------- swap-replica Rényi-2 (SCEE)
 // Per chain: regions A and B (0-based sites, A first), the replicas' region blocks of G(Θ)

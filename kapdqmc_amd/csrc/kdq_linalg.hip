@@ -28,6 +28,8 @@ __global__ void k_initial_s(const ChainDev *chains, int nt) {
     ch.s[e] = (uint8_t)init_field_value(ch.seed, ch.chain_id, 0, (uint32_t)lt, (uint32_t)i);
 }
 
+__global__ void k_bump(uint32_t *counter) { *counter += 1; }
+
 __global__ void k_zero_meas(const ChainDev *chains) {
     const ChainDev &ch = chains[blockIdx.x];
     if (threadIdx.x < KDQ_MEAS_SLOTS) ch.meas[threadIdx.x] = 0.0;
@@ -57,5 +59,10 @@ hipError_t launch_initial_s(hipStream_t st, int nchains, int max_elem, int nt, c
 
 hipError_t launch_zero_meas(hipStream_t st, int nchains, const ChainDev *chains) {
     hipLaunchKernelGGL(k_zero_meas, dim3(nchains), dim3(64), 0, st, chains);
+    return hipGetLastError();
+}
+
+hipError_t launch_bump(hipStream_t st, uint32_t *counter) {
+    hipLaunchKernelGGL(k_bump, dim3(1), dim3(1), 0, st, counter);
     return hipGetLastError();
 }

@@ -184,6 +184,12 @@ class DQMCBatch:
         self._chk(self.L.kdq_get_counters(self.h, _dp(out)))
         return out
 
+    def set_option(self, name: str, value: int):
+        """kdq_set_option: "graph" (0/1), "delay" (t-U block size), "gemm_tile" (0, 32, 64)."""
+        self._chk(self.L.kdq_set_option(self.h, L.OPTIONS[name], int(value)))
+        if name == "delay":
+            self.delay = int(value)
+
     def logdet(self) -> np.ndarray:
         """log|det(U_L U_R)| of the overlap factorised by the last get_G! of each chain."""
         out = np.zeros(self.n)
@@ -302,6 +308,9 @@ class SCEEBatch:
 
     def init(self):
         self._chk(self.L.kdq_scee_init(self.h))
+
+    def set_option(self, name: str, value: int):
+        self._chk(self.L.kdq_set_option(self.h, L.OPTIONS[name], int(value)))
 
     def sweeps(self, n: int) -> np.ndarray:
         """n sweeps of every chain (set-up included); returns [nchains, n] values of O."""
