@@ -101,7 +101,7 @@ def cpu_baseline(m, tau, nslices: int):
                        f"({sweeps_s:.4f} sweeps/s), ESS/s with the GPU chain's tau_int={tau:.2f}")}
 
 
-def run_batch(m, nch: int, seed0: int, delay: int, warmup: int, steps: int, profile_all: bool):
+def run_batch(m, nch: int, seed0: int, delay, warmup: int, steps: int, profile_all: bool):
     """W warm-up sweeps (the first with every kernel family timed by HIP events, host
     launches, to find the dominant family; the graph of a sweep is captured untimed after
     them), `steps` timed sweeps replayed from the graph, then one sweep with host launches
@@ -111,9 +111,15 @@ def run_batch(m, nch: int, seed0: int, delay: int, warmup: int, steps: int, prof
     from kapdqmc_amd.dqmc import DQMCBatch
     from kapdqmc_amd.rng import initial_fields
     N = m.Ns
-    b = DQMCBatch([m] * nch, [seed0 + c for c in range(nch)], list(range(nch)), delay=delay)
+    b = DQMCBatch([m] * nch, [seed0 + c for c in range(nch)], list(range(nch)), delay=delay or 32)
     for c in range(nch):
         b.set_fields(c, initial_fields(seed0 + c, c, N, m.Nt))
+    tuned = None
+    if not delay:   # autotuner: block size and GEMM tile from timed slices of this batch
+        from kapdqmc_amd import autotune
+        tuned = autotune.tune(b, delays=(16, 32, 64), tiles=(0, 32, 64), slices=4)
+        print(f"bench: {nch} chain(s), tuned delay {tuned['delay']} tile {tuned['gemm_tile']}",
+              file=sys.stderr, flush=True)
     fams_all = ["wrap", "chain", "get_G", "decide", "flush", "qr", "inv", "meas_wrap", "measure"]
     b.profile(fams_all)
     for w in range(max(warmup, 1)):
@@ -135,8 +141,9 @@ def run_batch(m, nch: int, seed0: int, delay: int, warmup: int, steps: int, prof
     prof = b.profile_read()
     cnt = b.counters()
     breakdown = {f: {"ms_per_sweep": prof_w[f][0], "launches_per_sweep": prof_w[f][1]} for f in prof_w}
-    return {"rows": np.asarray(rows), "dt": dt, "prof": prof, "dominant": dominant,
-            "breakdown": breakdown, "acceptance": float(cnt[:, 1].sum() / max(cnt[:, 0].sum(), 1))}
+    return {"rows": np.asarray(rows), "dt": dt, "prof": prof, "dominant": dominant, "delay": b.delay,
+            "tuned": tuned, "breakdown": breakdown,
+            "acceptance": float(cnt[:, 1].sum() / max(cnt[:, 0].sum(), 1))}
 
 
 def main():
@@ -145,7 +152,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--chains", type=int, default=64, help="independent chains batched per launch")
-    ap.add_argument("--delay", type=int, default=32)
+    ap.add_argument("--delay", type=int, default=0, help="block size k (0: autotune k and the GEMM tile)")
     ap.add_argument("--plain-steps", type=int, default=6, help="sweeps of the single-chain line (0: skip)")
     ap.add_argument("--plain-delay", type=int, default=16)
     ap.add_argument("--seed", type=int, default=1)
@@ -198,13 +205,14 @@ def main():
     ms, nl = bat["prof"][dominant]
     bound, work, unit = family_work(dominant, N, Np)
     work *= C                                        # one launch covers every chain
+    k_used = bat["delay"]
     avg_s = ms / 1e3 / max(nl, 1)
     if bound == "mfma":
         achieved, peak, runit = work / avg_s / 1e12, PEAK_FP64_TFLOPS, "TFLOP/s"
     else:
         achieved, peak, runit = work / avg_s / 1e9, PEAK_HBM_GBS, "GB/s"
     roofline = {"bound": bound, "achieved": achieved, "peak": peak, "unit": runit,
-                "frac": achieved / peak, "traffic": measured_traffic(dominant, C, args.delay),
+                "frac": achieved / peak, "traffic": measured_traffic(dominant, C, k_used),
                 "kernel_family": dominant,
                 "avg_launch_us": avg_s * 1e6, "launches": nl,
                 "work_per_launch": f"{work:.4g} {unit}"}
@@ -231,7 +239,8 @@ def main():
         "data": "synthetic (Philox-seeded HS fields; model constants from the reference formulas)",
         "config": {"workload": f"t-U honeycomb 12x12 N=288 Np=144 U=4 dtau=0.1 2Theta=20 Nt=200 BS=10, "
                                f"{C} independent chains batched per launch, FP64, record=true",
-                   "chains": C, "delay_k": args.delay,
+                   "chains": C, "delay_k": k_used,
+                   "autotuned": bat["tuned"] is not None,
                    "variant": "batched FP64 (mixed precision and SLMC moves not in this line)"},
         "sweeps_per_s": sweeps_s,
         "tau_int_double_occupancy": tau,
