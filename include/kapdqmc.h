@@ -65,7 +65,7 @@ typedef struct kdq_run_desc {
     uint64_t seed;
     uint32_t chain;          /* stream id (< 2^24) */
     uint32_t sweep0;         /* sweep counter of the first sweep */
-    int32_t precision;       /* KDQ_PREC_* */
+    int32_t precision;       /* KDQ_PREC_FP64, or KDQ_PREC_MIXED (t-U, see kdq_get_precision_error) */
     int32_t delay;           /* local-update block size k: sites (t-U, 1..64) or field elements
                                 (t-V / t-U-V, 1..32); 1 = one Sherman-Morrison update per flip */
 } kdq_run_desc;
@@ -127,10 +127,19 @@ int kdq_get_counters(kdq_handle h, double *out);
  *                     the first call, or when `record` changes) and replays it; 0: launches
  *                     every kernel from the host each sweep. While kernel timing is on
  *                     (kdq_profile mask non-zero) sweeps run with host launches.
+ *                     Sweeps of more than 20000 launches (small blocks on large lattices)
+ *                     also run with host launches.
  *  KDQ_OPT_DELAY      decision block size k of t-U chains (1..64; 1..16 with replicas).
  *  KDQ_OPT_GEMM_TILE  0 (default: by occupancy), 32 or 64: GEMM output tile edge. */
 enum { KDQ_OPT_GRAPH = 0, KDQ_OPT_DELAY = 1, KDQ_OPT_GEMM_TILE = 2 };
 int kdq_set_option(kdq_handle h, int32_t key, int32_t value);
+
+/* Mixed precision (kdq_run_desc.precision = KDQ_PREC_MIXED, t-U): G is carried in FP32
+ * between stabilisations (FP32 MFMA wraps, FP32 rank-k flushes; the Metropolis ratio, W,
+ * the chains, the orthonormalisation and the overlap solve stay FP64) and replaced by the
+ * FP64 recompute at every node. out[c] = max |G_FP32 - G_FP64| over the recomputes since the
+ * last call (0 for FP64 handles); the call resets it. No counterpart in the reference. */
+int kdq_get_precision_error(kdq_handle h, double *out);
 
 /* log|det(U_L U_R)| of the overlap factorised by the last get_G! of every chain (nchains
  * doubles). No counterpart in the reference, whose get_G! (src/public/Gupdate.jl:139-157)

@@ -21,7 +21,7 @@ OPTIONS = {"graph": 0, "delay": 1, "gemm_tile": 2}   # kdq_set_option keys
 EXPORTS = ("kdq_create", "kdq_destroy", "kdq_last_error", "kdq_num_chains", "kdq_num_obs", "kdq_set_fields",
            "kdq_get_fields", "kdq_initial_s", "kdq_phy_update", "kdq_get_G", "kdq_set_G", "kdq_wrap",
            "kdq_update_layer", "kdq_init_chain", "kdq_measure", "kdq_get_counters", "kdq_profile",
-           "kdq_profile_read", "kdq_get_logdet", "kdq_set_option", "kdq_scee_create", "kdq_scee_set_fields",
+           "kdq_profile_read", "kdq_get_logdet", "kdq_set_option", "kdq_get_precision_error", "kdq_scee_create", "kdq_scee_set_fields",
            "kdq_scee_get_fields", "kdq_scee_init", "kdq_scee_sweeps", "kdq_scee_get_G4", "kdq_scee_detg")
 PROF_FAMILIES = ("wrap", "chain", "get_G", "decide", "flush", "qr", "inv", "meas_wrap", "measure",
                  "gemm_other")
@@ -89,6 +89,7 @@ def load(path: str = LIB_PATH):
     L.kdq_profile_read.argtypes = [vp, dp, C.POINTER(C.c_int64)]
     L.kdq_get_logdet.argtypes = [vp, dp]
     L.kdq_set_option.argtypes = [vp, i32, i32]
+    L.kdq_get_precision_error.argtypes = [vp, dp]
     L.kdq_scee_create.argtypes = [C.POINTER(ModelDesc), C.POINTER(RunDesc), C.POINTER(SceeDesc), i32, C.POINTER(vp)]
     L.kdq_scee_set_fields.argtypes = [vp, i32, i32, C.POINTER(C.c_uint8)]
     L.kdq_scee_get_fields.argtypes = [vp, i32, i32, C.POINTER(C.c_uint8)]

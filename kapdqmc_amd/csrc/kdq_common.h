@@ -14,6 +14,7 @@
 #define KDQ_HD __host__ __device__ __forceinline__
 
 typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
 
 // One operand of a batched GEMM: pointers of chain c in a device table.
 struct Opnd {
@@ -22,6 +23,16 @@ struct Opnd {
     int ld;
     int pad;
 };
+// FP32 operand (mixed-precision G and kinetic factors); same layout rules as Opnd.
+struct OpndF {
+    float *re;
+    float *im;
+    int ld;
+    int pad;
+};
+template <class T> struct OpndOf;
+template <> struct OpndOf<double> { typedef Opnd type; };
+template <> struct OpndOf<float> { typedef OpndF type; };
 
 // Per-chain constants and state pointers (device table, one entry per chain).
 struct ChainDev {
@@ -100,6 +111,14 @@ KDQ_DEV void diag_value(const ChainDev &c, const DiagSpec &d, int i, double &re,
     if (d.conj) im = -im;
 }
 
+template <class T>
+KDQ_HD void cmulT(T ar, T ai, T br, T bi, T &cr, T &ci) {
+    T r = ar * br - ai * bi;
+    T i = ar * bi + ai * br;
+    cr = r;
+    ci = i;
+}
+
 KDQ_HD void cmul(double ar, double ai, double br, double bi, double &cr, double &ci) {
     double r = ar * br - ai * bi;
     double i = ar * bi + ai * br;
@@ -150,5 +169,11 @@ KDQ_HD int init_field_value(uint64_t seed, uint32_t chain, uint32_t stream, uint
 KDQ_DEV double wave_sum(double v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+KDQ_DEV double wave_max(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
     return v;
 }

@@ -33,6 +33,7 @@ struct ChainHost {
     std::vector<void *> allocs;
     double *eK, *eKinv, *hK, *hKinv, *K, *Pt, *alpha;
     double *G[2], *T, *Gm, *Tm, *P0, *P1, *P2, *M, *X, *S;
+    float *G32[2] = {nullptr, nullptr}, *T32 = nullptr, *eK32 = nullptr, *eKinv32 = nullptr;   // mixed precision
     int *ex = nullptr, *ey = nullptr, *inc = nullptr;
     std::vector<double *> UR, ULH;
     uint8_t *s;
@@ -81,6 +82,10 @@ struct kdq_handle_s {
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
     double prof_ms[KDQ_PROF_FAMILIES] = {};
     int64_t prof_n[KDQ_PROF_FAMILIES] = {};
+    // mixed precision: FP32 copy of G between stabilisations (t-U)
+    int mixed = 0, cur32 = 0;
+    OpndF *t_G32[2] = {nullptr, nullptr}, *t_T32 = nullptr, *t_eK32 = nullptr, *t_eKinv32 = nullptr;
+    double *d_maxerr = nullptr;   // [nch] max |G_FP32 - G_FP64| at the recomputes since the last read
     // options (kdq_set_option) and the captured sweep
     int use_graph = 1, gemm_tile = 0;
     uint32_t *d_sweep = nullptr;              // device sweep counter read by graph replays
@@ -185,6 +190,15 @@ hipError_t dalloc(ChainHost &c, T **p, size_t count) {
     return e;
 }
 
+int make_table_f(kdq_handle h, OpndF **out, const std::vector<OpndF> &v) {
+    void *q = nullptr;
+    HIPCHK(hipMalloc(&q, sizeof(OpndF) * v.size()));
+    h->dev_misc.push_back(q);
+    HIPCHK(hipMemcpy(q, v.data(), sizeof(OpndF) * v.size(), hipMemcpyHostToDevice));
+    *out = (OpndF *)q;
+    return KDQ_OK;
+}
+
 int make_table(kdq_handle h, Opnd **out, const std::vector<Opnd> &v) {
     void *q = nullptr;
     HIPCHK(hipMalloc(&q, sizeof(Opnd) * v.size()));
@@ -213,6 +227,22 @@ int gemm(kdq_handle h, bool ac, bool bc, int opa, int opb, const GemmArgs &g, in
     HIPCHK(launch_gemm(h->stream, h->nch, maxdim(h, g.mdim), maxdim(h, g.ndim), maxdim(h, g.kdim), ac, bc, opa, opb, g,
                        h->gemm_tile));
     return KDQ_OK;
+}
+
+int gemm_f(kdq_handle h, bool ac, bool bc, int opa, int opb, const GemmArgsF &g, int fam) {
+    ProfScope ps(h, fam);
+    HIPCHK(launch_gemm_f32(h->stream, h->nch, maxdim(h, g.mdim), maxdim(h, g.ndim), maxdim(h, g.kdim), ac, bc, opa, opb,
+                           g, h->gemm_tile));
+    return KDQ_OK;
+}
+
+GemmArgsF gargsf(kdq_handle h, const OpndF *A, const OpndF *B, const OpndF *C) {
+    GemmArgsF g{};
+    g.chains = h->d_chains;
+    g.A = A; g.B = B; g.C = C; g.Cin = nullptr;
+    g.mdim = g.ndim = g.kdim = DIM_NS;
+    g.mode = 0;
+    return g;
 }
 
 // HS diagonal of slice lt for the site-diagonal models (t-U: kind 1, t-U-V: kind 2)
@@ -252,6 +282,35 @@ int wrap_B(kdq_handle h, Opnd *Gt, Opnd *Tt, int lt, bool forward, int fam = KDQ
     if (int r = gemm(h, false, true, 0, 0, g1, fam)) return r;
     GemmArgs g2 = gargs(h, Tt, h->t_eK, Gt, DIM_NS, DIM_NS, DIM_NS);
     return gemm(h, true, false, 0, 0, g2, fam);
+}
+
+// WrapKV!(…, "B") on the FP32 copy of G (mixed precision): the same two GEMMs as wrap_B in
+// FP32 MFMA with FP32 accumulation, HS phases evaluated in FP64 and rounded.
+int wrap_B_f32(kdq_handle h, int lt, bool forward, int fam = KDQ_PROF_WRAP) {
+    OpndF *Gt = h->t_G32[h->cur32], *Tt = h->t_T32;
+    if (forward) {
+        GemmArgsF g1 = gargsf(h, h->t_eK32, Gt, Tt);
+        if (int r = gemm_f(h, false, true, 0, 0, g1, fam)) return r;
+        GemmArgsF g2 = gargsf(h, Tt, h->t_eKinv32, Gt);
+        g2.rowS = hs_diag(h, lt, +1);
+        g2.colS = hs_diag(h, lt, -1);
+        return gemm_f(h, true, false, 0, 0, g2, fam);
+    }
+    GemmArgsF g1 = gargsf(h, h->t_eKinv32, Gt, Tt);
+    g1.inS = hs_diag(h, lt, -1);
+    g1.colS = hs_diag(h, lt, +1);
+    if (int r = gemm_f(h, false, true, 0, 0, g1, fam)) return r;
+    GemmArgsF g2 = gargsf(h, Tt, h->t_eK32, Gt);
+    return gemm_f(h, true, false, 0, 0, g2, fam);
+}
+
+// Mixed precision: the FP64 G just recomputed from the chain replaces the FP32 copy; with
+// track set, max |G_FP32 - G_FP64| is recorded first.
+int refresh_g32(kdq_handle h, bool track) {
+    if (!h->mixed) return KDQ_OK;
+    HIPCHK(launch_to_f32(h->stream, h->nch, h->max_ns, h->d_chains, h->t_G[h->cur], h->t_G32[h->cur32], DIM_NS,
+                         track ? h->d_maxerr : nullptr));
+    return KDQ_OK;
 }
 
 // U <- orthonormal basis of span(U) for the N x Np propagator, in place: two Cholesky-QR
@@ -341,7 +400,8 @@ int init_chain(kdq_handle h) {
     for (int idx = NN - 1; idx >= 1; --idx)
         if (int r = propagate_L(h, h->t_ULH[idx], h->t_ULH[idx - 1], idx)) return r;
     h->idx = 1;
-    return get_G(h, 1);
+    if (int r = get_G(h, 1)) return r;
+    return refresh_g32(h, false);
 }
 
 // Rank-2 blocks of one family (t-V) or of the whole slice (t-U-V, family 0).
@@ -378,10 +438,18 @@ int update_layer(kdq_handle h, int lt, int dir, uint32_t sweep) {
         a.G = h->t_G[h->cur];
         a.lt = lt; a.dir = dir; a.s0 = s0; a.kb = kb; a.sweep = sweep;
         a.sweep_dev = h->sweep_dev;
+        if (h->mixed) a.Gf = h->t_G32[h->cur32];
         // t-U only; t-V layers go through slice_tv
         {
             ProfScope ps(h, KDQ_PROF_DECIDE);
             HIPCHK(launch_decide_tu(h->stream, h->nch, a));
+        }
+        if (h->mixed) {
+            FlushArgsF f{h->d_chains, h->t_G32[h->cur32], h->t_G32[1 - h->cur32], kb, 0, nullptr, nullptr};
+            ProfScope ps(h, KDQ_PROF_FLUSH);
+            HIPCHK(launch_flush_f32(h->stream, h->nch, h->max_ns, f));
+            h->cur32 ^= 1;
+            continue;
         }
         FlushArgs f{h->d_chains, h->t_G[h->cur], h->t_G[1 - h->cur], kb, 0, nullptr, nullptr};
         {
@@ -420,6 +488,7 @@ int slice_tv_backward(kdq_handle h, Opnd *Gt, Opnd *Tt, int lt, bool update, uin
 
 // One slice wrap of a G copy (no updates) in either direction, any model.
 int wrap_slice(kdq_handle h, Opnd *Gt, Opnd *Tt, int lt, bool forward, int fam) {
+    if (h->mixed && Gt == h->t_G[h->cur]) return wrap_B_f32(h, lt, forward, fam);   // the live G is the FP32 copy
     if (h->kind == KDQ_MODEL_TV)
         return forward ? slice_tv_forward(h, Gt, Tt, lt, false, 0, fam) : slice_tv_backward(h, Gt, Tt, lt, false, 0, fam);
     return wrap_B(h, Gt, Tt, lt, forward, fam);
@@ -427,7 +496,10 @@ int wrap_slice(kdq_handle h, Opnd *Gt, Opnd *Tt, int lt, bool forward, int fam) 
 
 // phy_measure: wrap a copy of G to τ = Nt/2, HalfeK sandwich, observables (accumulated).
 int measure(kdq_handle h, int lt) {
-    HIPCHK(launch_copy_mat(h->stream, h->nch, h->max_ns, h->max_ns, h->d_chains, h->t_G[h->cur], h->t_Gm, DIM_NS, DIM_NS, true));
+    if (h->mixed)
+        HIPCHK(launch_to_f64(h->stream, h->nch, h->max_ns, h->d_chains, h->t_G32[h->cur32], h->t_Gm, DIM_NS, 1));
+    else
+        HIPCHK(launch_copy_mat(h->stream, h->nch, h->max_ns, h->max_ns, h->d_chains, h->t_G[h->cur], h->t_Gm, DIM_NS, DIM_NS, true));
     const int nt = h->nt;
     if (lt > nt / 2.0) {
         for (int t = lt; t >= nt / 2 + 1; --t)
@@ -506,7 +578,7 @@ int sweep_body(kdq_handle h, int record, uint32_t sweep) {
         if (tv) {
             if (int r = slice_tv_forward(h, h->t_G[h->cur], h->t_T, lt, true, sweep, KDQ_PROF_WRAP)) return r;
         } else {
-            if (int r = wrap_B(h, h->t_G[h->cur], h->t_T, lt, true)) return r;
+            if (int r = h->mixed ? wrap_B_f32(h, lt, true) : wrap_B(h, h->t_G[h->cur], h->t_T, lt, true)) return r;
             if (int r = update_layer(h, lt, 0, sweep)) return r;
         }
         if (record && in_window(h, thidx, true))
@@ -515,6 +587,7 @@ int sweep_body(kdq_handle h, int record, uint32_t sweep) {
             h->idx += 1;
             if (int r = propagate_R(h, h->t_UR[h->idx - 2], h->t_UR[h->idx - 1], h->idx - 1)) return r;
             if (int r = get_G(h, h->idx)) return r;
+            if (int r = refresh_g32(h, true)) return r;
         }
     }
     for (int lt = nt; lt >= 1; --lt) {
@@ -526,12 +599,13 @@ int sweep_body(kdq_handle h, int record, uint32_t sweep) {
             if (int r = update_layer(h, lt, 1, sweep)) return r;
             if (record && in_window(h, thidx, false))
                 if (int r = measure(h, lt)) return r;
-            if (int r = wrap_B(h, h->t_G[h->cur], h->t_T, lt, false)) return r;
+            if (int r = h->mixed ? wrap_B_f32(h, lt, false) : wrap_B(h, h->t_G[h->cur], h->t_T, lt, false)) return r;
         }
         if (is_node(h, lt - 1)) {
             h->idx -= 1;
             if (int r = propagate_L(h, h->t_ULH[h->idx], h->t_ULH[h->idx - 1], h->idx)) return r;
             if (int r = get_G(h, h->idx)) return r;
+            if (int r = refresh_g32(h, true)) return r;
         }
     }
     if (h->sweep_dev) HIPCHK(launch_bump(h->stream, h->d_sweep));
@@ -551,7 +625,7 @@ void drop_graph(kdq_handle h) {
 // index) must come back to its start value, or the graph would not be replayable.
 int capture_sweep(kdq_handle h, int record) {
     drop_graph(h);
-    const int cur0 = h->cur, idx0 = h->idx;
+    const int cur0 = h->cur, idx0 = h->idx, cur32_0 = h->cur32;
     HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed));
     h->sweep_dev = h->d_sweep;
     int r = sweep_body(h, record, 0);
@@ -563,7 +637,7 @@ int capture_sweep(kdq_handle h, int record) {
         return r;
     }
     HIPCHK(e);
-    if (h->cur != cur0 || h->idx != idx0) {
+    if (h->cur != cur0 || h->idx != idx0 || h->cur32 != cur32_0) {
         hipGraphDestroy(g);
         return fail(h, KDQ_ESTATE, "sweep is not replayable as a graph (state parity)");
     }
@@ -573,11 +647,21 @@ int capture_sweep(kdq_handle h, int record) {
     return KDQ_OK;
 }
 
+// Upper estimate of the kernel launches of one sweep. A captured sweep is one linear chain
+// of graph nodes; past 20000 nodes (small decision blocks on large
+// lattices) the sweep runs with host launches instead of being captured.
+long sweep_launch_estimate(kdq_handle h) {
+    const long blocks = h->kind == KDQ_MODEL_TU ? (h->max_ns + h->delay - 1) / h->delay : 3L * h->max_blocks;
+    const long per_slice = 2 * blocks + 8 + (h->kind == KDQ_MODEL_TV ? 12 : 0);
+    return 2L * h->nt * per_slice + 40L * h->nn + 2L * h->nt;
+}
+constexpr long kMaxGraphLaunches = 20000;
+
 int phy_update(kdq_handle h, int sweeps, int record, double *rows) {
     const int no = h->nobs;
     if (int r = init_chain(h)) return r;
     std::vector<double> tmp(no * h->nch);
-    const bool graph = h->use_graph && h->prof_mask == 0;
+    const bool graph = h->use_graph && h->prof_mask == 0 && sweep_launch_estimate(h) <= kMaxGraphLaunches;
     if (graph) {
         HIPCHK(hipMemcpyAsync(h->d_sweep, &h->sweep, sizeof(uint32_t), hipMemcpyHostToDevice, h->stream));
         HIPCHK(hipStreamSynchronize(h->stream));
@@ -643,7 +727,11 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
         if (m.nodes[0] != 0 || m.nodes[m.n_nodes - 1] != m.nt) return fail(nullptr, KDQ_EARG, "kdq_create: nodes must start at 0 and end at nt");
         if (runs[c].delay < 1 || runs[c].delay > 64) return fail(nullptr, KDQ_EARG, "kdq_create: delay must be in 1..64");
         if (runs[c].delay != runs[0].delay) return fail(nullptr, KDQ_EARG, "kdq_create: delay must be equal across chains");
-        if (runs[c].precision != KDQ_PREC_FP64) return fail(nullptr, KDQ_EARG, "kdq_create: only FP64 precision is available in this build");
+        if (runs[c].precision != KDQ_PREC_FP64 && runs[c].precision != KDQ_PREC_MIXED)
+            return fail(nullptr, KDQ_EARG, "kdq_create: unknown precision");
+        if (runs[c].precision != runs[0].precision) return fail(nullptr, KDQ_EARG, "kdq_create: precision must be equal across chains");
+        if (runs[c].precision == KDQ_PREC_MIXED && (m.kind != KDQ_MODEL_TU || streams))
+            return fail(nullptr, KDQ_EARG, "kdq_create: mixed precision is built for the plain t-U sweep only");
         if (runs[c].chain >= (1u << 24)) return fail(nullptr, KDQ_EARG, "kdq_create: chain id must be < 2^24");
     }
     kdq_handle h = new kdq_handle_s();
@@ -653,6 +741,7 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
     h->nodes.assign(models[0].nodes, models[0].nodes + h->nn);
     h->delay = runs[0].delay;
     h->kind = models[0].kind;
+    h->mixed = runs[0].precision == KDQ_PREC_MIXED;
     h->nobs = h->kind == KDQ_MODEL_TU ? 6 : (h->kind == KDQ_MODEL_TV ? 10 : 7);
     h->md.assign(models, models + nchains);
     h->rd.assign(runs, runs + nchains);
@@ -666,6 +755,7 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
     }
     std::vector<ChainDev> cd(nchains);
     std::vector<Opnd> v_eK, v_eKinv, v_hK, v_hKinv, v_Pt, v_G0, v_G1, v_T, v_Gm, v_Tm, v_P0, v_P1, v_P2, v_M, v_X, v_S;
+    std::vector<OpndF> v_G32a, v_G32b, v_T32, v_eK32, v_eKinv32;
     std::vector<BlockDesc> blk_all;
     std::vector<int> blk_offset, blk_count;
     std::vector<std::vector<Opnd>> v_UR(h->nn), v_ULH(h->nn);
@@ -747,6 +837,13 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
         ok &= dalloc(C, &C.counters, 2) == hipSuccess;
         ok &= dalloc(C, &C.meas, KDQ_MEAS_SLOTS) == hipSuccess;
         ok &= dalloc(C, &C.logdet, 1) == hipSuccess;
+        if (h->mixed) {
+            ok &= dalloc(C, &C.G32[0], 2 * nn) == hipSuccess;
+            ok &= dalloc(C, &C.G32[1], 2 * nn) == hipSuccess;
+            ok &= dalloc(C, &C.T32, 2 * nn) == hipSuccess;
+            ok &= dalloc(C, &C.eK32, nn) == hipSuccess;
+            ok &= dalloc(C, &C.eKinv32, nn) == hipSuccess;
+        }
         if (!ok) {
             int r = fail(h, KDQ_EHIP, "kdq_create: device allocation failed");
             delete h;
@@ -754,6 +851,15 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
         }
         hipMemcpy(C.eK, m.eK, nn * 8, hipMemcpyHostToDevice);
         hipMemcpy(C.eKinv, m.eKinv, nn * 8, hipMemcpyHostToDevice);
+        if (h->mixed) {
+            std::vector<float> f1(nn), f2(nn);
+            for (size_t q = 0; q < nn; ++q) {
+                f1[q] = (float)m.eK[q];
+                f2[q] = (float)m.eKinv[q];
+            }
+            hipMemcpy(C.eK32, f1.data(), nn * 4, hipMemcpyHostToDevice);
+            hipMemcpy(C.eKinv32, f2.data(), nn * 4, hipMemcpyHostToDevice);
+        }
         hipMemcpy(C.hK, m.halfeK, nn * 8, hipMemcpyHostToDevice);
         hipMemcpy(C.hKinv, m.halfeKinv, nn * 8, hipMemcpyHostToDevice);
         hipMemcpy(C.K, m.K, nn * 8, hipMemcpyHostToDevice);
@@ -830,6 +936,13 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
         v_P0.push_back(cplx(C.P0, n, nP)); v_P1.push_back(cplx(C.P1, n, nP)); v_P2.push_back(cplx(C.P2, n, nP));
         v_M.push_back(cplx(C.M, np, pp)); v_X.push_back(cplx(C.X, np, nP));
         v_S.push_back(cplx(C.S, np, pp));
+        if (h->mixed) {
+            v_G32a.push_back(OpndF{C.G32[0], C.G32[0] + nn, n, 0});
+            v_G32b.push_back(OpndF{C.G32[1], C.G32[1] + nn, n, 0});
+            v_T32.push_back(OpndF{C.T32, C.T32 + nn, n, 0});
+            v_eK32.push_back(OpndF{C.eK32, nullptr, n, 0});
+            v_eKinv32.push_back(OpndF{C.eKinv32, nullptr, n, 0});
+        }
         for (int k = 0; k < h->nn; ++k) {
             v_UR[k].push_back(cplx(C.UR[k], n, nP));
             v_ULH[k].push_back(cplx(C.ULH[k], n, nP));
@@ -869,6 +982,19 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
         h->d_nblocks = (int *)q3;
     }
     r |= make_table(h, &h->t_M, v_M); r |= make_table(h, &h->t_X, v_X); r |= make_table(h, &h->t_S, v_S);
+    if (h->mixed) {
+        r |= make_table_f(h, &h->t_G32[0], v_G32a); r |= make_table_f(h, &h->t_G32[1], v_G32b);
+        r |= make_table_f(h, &h->t_T32, v_T32);
+        r |= make_table_f(h, &h->t_eK32, v_eK32); r |= make_table_f(h, &h->t_eKinv32, v_eKinv32);
+        void *q = nullptr;
+        if (hipMalloc(&q, sizeof(double) * nchains) == hipSuccess) {
+            h->dev_misc.push_back(q);
+            hipMemset(q, 0, sizeof(double) * nchains);
+            h->d_maxerr = (double *)q;
+        } else {
+            r = 1;
+        }
+    }
     h->t_UR.resize(h->nn);
     h->t_ULH.resize(h->nn);
     for (int k = 0; k < h->nn; ++k) {
@@ -954,8 +1080,15 @@ int kdq_get_G(kdq_handle h, int32_t c, double *G) {
     const int n = h->ch[c].ns;
     const size_t nn = (size_t)n * n;
     std::vector<double> tmp(2 * nn);
-    HIPCHK(hipMemcpyAsync(tmp.data(), h->ch[c].G[h->cur], 2 * nn * 8, hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(hipStreamSynchronize(h->stream));
+    if (h->mixed) {   // the live G is the FP32 copy
+        std::vector<float> f(2 * nn);
+        HIPCHK(hipMemcpyAsync(f.data(), h->ch[c].G32[h->cur32], 2 * nn * 4, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        for (size_t q = 0; q < 2 * nn; ++q) tmp[q] = f[q];
+    } else {
+        HIPCHK(hipMemcpyAsync(tmp.data(), h->ch[c].G[h->cur], 2 * nn * 8, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+    }
     for (size_t q = 0; q < nn; ++q) {
         G[2 * q] = tmp[q];
         G[2 * q + 1] = tmp[nn + q];
@@ -973,6 +1106,7 @@ int kdq_set_G(kdq_handle h, int32_t c, const double *G) {
         tmp[nn + q] = G[2 * q + 1];
     }
     HIPCHK(hipMemcpyAsync(h->ch[c].G[h->cur], tmp.data(), 2 * nn * 8, hipMemcpyHostToDevice, h->stream));
+    if (int r = refresh_g32(h, false)) return r;
     HIPCHK(hipStreamSynchronize(h->stream));
     return KDQ_OK;
 }
@@ -1053,6 +1187,18 @@ int kdq_set_option(kdq_handle h, int32_t key, int32_t value) {
             return fail(h, KDQ_EARG, "kdq_set_option: unknown option");
     }
     drop_graph(h);
+    return KDQ_OK;
+}
+
+int kdq_get_precision_error(kdq_handle h, double *out) {
+    if (!h || !out) return fail(h, KDQ_EARG, "kdq_get_precision_error: bad argument");
+    if (!h->mixed) {
+        for (int c = 0; c < h->nch; ++c) out[c] = 0.0;
+        return KDQ_OK;
+    }
+    HIPCHK(hipMemcpyAsync(out, h->d_maxerr, sizeof(double) * h->nch, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemsetAsync(h->d_maxerr, 0, sizeof(double) * h->nch, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
     return KDQ_OK;
 }
 

@@ -18,8 +18,20 @@ namespace {
 constexpr int BK = 16;
 constexpr int KMAX = 512;   // longest contraction with a diagonal factor (N <= 512)
 
-template <int AC, int BC, int OPA, int OPB, int BM, int BN>
-__global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
+KDQ_DEV v4d mma(double b, double a, v4d c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(b, a, c, 0, 0, 0); }
+KDQ_DEV v4f mma(float b, float a, v4f c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(b, a, c, 0, 0, 0); }
+template <class T> struct Acc;
+template <> struct Acc<double> { typedef v4d type; };
+template <> struct Acc<float> { typedef v4f type; };
+// column (within the 16x16 block) held in accumulator register v of lane l: the FP64 MFMA
+// spreads a row over l/16 + 4v, the FP32 one over 4(l/16) + v (both checked on gfx950)
+template <class T> KDQ_DEV int acc_col(int lane, int v);
+template <> KDQ_DEV int acc_col<double>(int lane, int v) { return (lane >> 4) + 4 * v; }
+template <> KDQ_DEV int acc_col<float>(int lane, int v) { return 4 * (lane >> 4) + v; }
+
+template <class T, int AC, int BC, int OPA, int OPB, int BM, int BN>
+__global__ __launch_bounds__(256) void k_gemm(GemmArgsT<T> g) {
+    typedef typename Acc<T>::type VA;
     constexpr bool OC = AC || BC;
     constexpr int WMB = BM / 32, WNB = BN / 32;   // 16x16 MFMA blocks per wave (each dim)
     constexpr int LPA = BM + 2, LPB = BN + 2;
@@ -29,40 +41,49 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
     const int M = dim_of(ch, g.mdim), N = dim_of(ch, g.ndim), K = dim_of(ch, g.kdim);
     const int i0 = blockIdx.y * BM, j0 = blockIdx.x * BN;
     if (i0 >= M || j0 >= N) return;
-    const Opnd A = g.A[c], B = g.B[c];
+    const auto A = g.A[c], B = g.B[c];
 
-    __shared__ double As[2][AC ? 2 : 1][BK][LPA];
-    __shared__ double Bs[2][BC ? 2 : 1][BK][LPB];
-    __shared__ double Sk[2][KMAX];
-    __shared__ double Sr[2][BM], Sc[2][BN];
+    __shared__ T As[2][AC ? 2 : 1][BK][LPA];
+    __shared__ T Bs[2][BC ? 2 : 1][BK][LPB];
+    __shared__ T Sk[2][KMAX];
+    __shared__ T Sr[2][BM], Sc[2][BN];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
     const bool scale_k = g.inS.kind != 0;
     // diagonal factors, once per workgroup
     if (scale_k)
-        for (int k = tid; k < K; k += 256) diag_value(ch, g.inS, k, Sk[0][k], Sk[1][k]);
+        for (int k = tid; k < K; k += 256) {
+            double dr, di;
+            diag_value(ch, g.inS, k, dr, di);
+            Sk[0][k] = (T)dr;
+            Sk[1][k] = (T)di;
+        }
     if (g.rowS.kind && tid < BM) {
-        if (i0 + tid < M) diag_value(ch, g.rowS, i0 + tid, Sr[0][tid], Sr[1][tid]);
-        else { Sr[0][tid] = 1.0; Sr[1][tid] = 0.0; }
+        double dr = 1.0, di = 0.0;
+        if (i0 + tid < M) diag_value(ch, g.rowS, i0 + tid, dr, di);
+        Sr[0][tid] = (T)dr;
+        Sr[1][tid] = (T)di;
     }
     if (g.colS.kind && tid >= 128 && tid < 128 + BN) {
         const int t = tid - 128;
-        if (j0 + t < N) diag_value(ch, g.colS, j0 + t, Sc[0][t], Sc[1][t]);
-        else { Sc[0][t] = 1.0; Sc[1][t] = 0.0; }
+        double dr = 1.0, di = 0.0;
+        if (j0 + t < N) diag_value(ch, g.colS, j0 + t, dr, di);
+        Sc[0][t] = (T)dr;
+        Sc[1][t] = (T)di;
     }
 
-    v4d accR[WMB][WNB], accI[WMB][WNB];
+    VA accR[WMB][WNB], accI[WMB][WNB];
 #pragma unroll
     for (int a = 0; a < WMB; ++a)
 #pragma unroll
         for (int b = 0; b < WNB; ++b) {
-            accR[a][b] = v4d{0, 0, 0, 0};
-            accI[a][b] = v4d{0, 0, 0, 0};
+            accR[a][b] = VA{0, 0, 0, 0};
+            accI[a][b] = VA{0, 0, 0, 0};
         }
 
     // register staging of one k-tile: element e = tid + 256 q of op(A)[BM x BK], op(B)[BK x BN]
-    double ra[EA], rai[EA], rb[EB], rbi[EB];
+    T ra[EA], rai[EA], rb[EB], rbi[EB];
     auto fetch = [&](int k0) {
 #pragma unroll
         for (int q = 0; q < EA; ++q) {
@@ -71,7 +92,7 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
             if (OPA == 0) { ii = e % BM; kk = e / BM; }
             else { kk = e % BK; ii = e / BK; }
             const int gi = i0 + ii, gk = k0 + kk;
-            double vr = 0.0, vi = 0.0;
+            T vr = 0, vi = 0;
             if (gi < M && gk < K) {
                 const size_t off = OPA == 0 ? (size_t)gi + (size_t)gk * A.ld : (size_t)gk + (size_t)gi * A.ld;
                 vr = A.re[off];
@@ -90,7 +111,7 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
             if (OPB == 0) { kk = e % BK; jj = e / BK; }
             else { jj = e % BN; kk = e / BN; }
             const int gj = j0 + jj, gk = k0 + kk;
-            double vr = 0.0, vi = 0.0;
+            T vr = 0, vi = 0;
             if (gj < N && gk < K) {
                 const size_t off = OPB == 0 ? (size_t)gk + (size_t)gj * B.ld : (size_t)gj + (size_t)gk * B.ld;
                 vr = B.re[off];
@@ -110,8 +131,8 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
             int ii, kk;
             if (OPA == 0) { ii = e % BM; kk = e / BM; }
             else { kk = e % BK; ii = e / BK; }
-            double vr = ra[q], vi = rai[q];
-            if (AC && scale_k && k0 + kk < K) cmul(vr, vi, Sk[0][k0 + kk], Sk[1][k0 + kk], vr, vi);
+            T vr = ra[q], vi = rai[q];
+            if (AC && scale_k && k0 + kk < K) cmulT(vr, vi, Sk[0][k0 + kk], Sk[1][k0 + kk], vr, vi);
             As[buf][0][kk][ii] = vr;
             if (AC) As[buf][1][kk][ii] = vi;
         }
@@ -121,8 +142,8 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
             int jj, kk;
             if (OPB == 0) { kk = e % BK; jj = e / BK; }
             else { jj = e % BN; kk = e / BN; }
-            double vr = rb[q], vi = rbi[q];
-            if (BC && !AC && scale_k && k0 + kk < K) cmul(vr, vi, Sk[0][k0 + kk], Sk[1][k0 + kk], vr, vi);
+            T vr = rb[q], vi = rbi[q];
+            if (BC && !AC && scale_k && k0 + kk < K) cmulT(vr, vi, Sk[0][k0 + kk], Sk[1][k0 + kk], vr, vi);
             Bs[buf][0][kk][jj] = vr;
             if (BC) Bs[buf][1][kk][jj] = vi;
         }
@@ -139,30 +160,30 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 4) {
             const int kr = kk + (lane >> 4), cc = lane & 15;
-            double ar[WMB], ai[WMB], br[WNB], bi[WNB];
+            T ar[WMB], ai[WMB], br[WNB], bi[WNB];
 #pragma unroll
             for (int t = 0; t < WMB; ++t) {
                 ar[t] = As[buf][0][kr][wm * (BM / 2) + t * 16 + cc];
-                ai[t] = AC ? As[buf][AC ? 1 : 0][kr][wm * (BM / 2) + t * 16 + cc] : 0.0;
+                ai[t] = AC ? As[buf][AC ? 1 : 0][kr][wm * (BM / 2) + t * 16 + cc] : T(0);
             }
 #pragma unroll
             for (int t = 0; t < WNB; ++t) {
                 br[t] = Bs[buf][0][kr][wn * (BN / 2) + t * 16 + cc];
-                bi[t] = BC ? Bs[buf][BC ? 1 : 0][kr][wn * (BN / 2) + t * 16 + cc] : 0.0;
+                bi[t] = BC ? Bs[buf][BC ? 1 : 0][kr][wn * (BN / 2) + t * 16 + cc] : T(0);
             }
 #pragma unroll
             for (int mi = 0; mi < WMB; ++mi)
 #pragma unroll
                 for (int ni = 0; ni < WNB; ++ni) {
-                    accR[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(br[ni], ar[mi], accR[mi][ni], 0, 0, 0);
+                    accR[mi][ni] = mma(br[ni], ar[mi], accR[mi][ni]);
                     if (AC && BC) {
-                        accR[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(-bi[ni], ai[mi], accR[mi][ni], 0, 0, 0);
-                        accI[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(bi[ni], ar[mi], accI[mi][ni], 0, 0, 0);
-                        accI[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(br[ni], ai[mi], accI[mi][ni], 0, 0, 0);
+                        accR[mi][ni] = mma(-bi[ni], ai[mi], accR[mi][ni]);
+                        accI[mi][ni] = mma(bi[ni], ar[mi], accI[mi][ni]);
+                        accI[mi][ni] = mma(br[ni], ai[mi], accI[mi][ni]);
                     } else if (BC) {
-                        accI[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(bi[ni], ar[mi], accI[mi][ni], 0, 0, 0);
+                        accI[mi][ni] = mma(bi[ni], ar[mi], accI[mi][ni]);
                     } else if (AC) {
-                        accI[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(br[ni], ai[mi], accI[mi][ni], 0, 0, 0);
+                        accI[mi][ni] = mma(br[ni], ai[mi], accI[mi][ni]);
                     }
                 }
         }
@@ -171,17 +192,17 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
         buf ^= 1;
     }
 
-    // ---- epilogue. For v_mfma_f64_16x16x4f64 lane l, register v holds D[(l/16) + 4v][l%16]
-    // of the 16x16 block (checked on gfx950); D = op(B)ᵀ op(A)ᵀ here, so the lane writes
-    // C[i][j] with i = l%16 and j = l/16 + 4v: 16 consecutive rows per store.
-    const Opnd Cm = g.C[c];
-    Opnd Ci;
+    // ---- epilogue. Lane l, register v holds D[acc_col(l, v)][l%16] of the 16x16 block;
+    // D = op(B)ᵀ op(A)ᵀ here, so the lane writes C[i][j] with i = l%16 and j = acc_col(l, v):
+    // 16 consecutive rows per store.
+    const auto Cm = g.C[c];
+    typename OpndOf<T>::type Ci{};
     if (g.mode == 2) Ci = g.Cin[c];
 #pragma unroll
     for (int mi = 0; mi < WMB; ++mi) {
         const int il = wm * (BM / 2) + mi * 16 + (lane & 15), i = i0 + il;
         if (i >= M) continue;
-        double rsr = 1.0, rsi = 0.0;
+        T rsr = 1, rsi = 0;
         if (g.rowS.kind) {
             rsr = Sr[0][il];
             rsi = Sr[1][il];
@@ -190,14 +211,14 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
         for (int ni = 0; ni < WNB; ++ni)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int jl = wn * (BN / 2) + ni * 16 + (lane >> 4) + 4 * v, j = j0 + jl;
+                const int jl = wn * (BN / 2) + ni * 16 + acc_col<T>(lane, v), j = j0 + jl;
                 if (j >= N) continue;
-                double cr = accR[mi][ni][v], ci = OC ? accI[mi][ni][v] : 0.0;
-                if (g.rowS.kind) cmul(cr, ci, rsr, rsi, cr, ci);
-                if (g.colS.kind) cmul(cr, ci, Sc[0][jl], Sc[1][jl], cr, ci);
+                T cr = accR[mi][ni][v], ci = OC ? accI[mi][ni][v] : T(0);
+                if (g.rowS.kind) cmulT(cr, ci, rsr, rsi, cr, ci);
+                if (g.colS.kind) cmulT(cr, ci, Sc[0][jl], Sc[1][jl], cr, ci);
                 const size_t off = (size_t)i + (size_t)j * Cm.ld;
                 if (g.mode == 1) {
-                    cr = (i == j ? 1.0 : 0.0) - cr;
+                    cr = (i == j ? T(1) : T(0)) - cr;
                     ci = -ci;
                 } else if (g.mode == 2) {
                     const size_t oi = (size_t)i + (size_t)j * Ci.ld;
@@ -213,17 +234,17 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
     }
 }
 
-template <int AC, int BC, int OPA, int OPB>
-hipError_t run(hipStream_t st, int nchains, int max_m, int max_n, int max_k, const GemmArgs &g, int tile) {
+template <class T, int AC, int BC, int OPA, int OPB>
+hipError_t run(hipStream_t st, int nchains, int max_m, int max_n, int max_k, const GemmArgsT<T> &g, int tile) {
     if (g.inS.kind && max_k > KMAX) return hipErrorInvalidValue;
     // 64 x 64 tiles unless that leaves most compute units idle (or as forced by `tile`)
     const long t64 = (long)((max_n + 63) / 64) * ((max_m + 63) / 64) * nchains;
     if (tile == 64 || (tile == 0 && t64 >= 192)) {
         dim3 grid((max_n + 63) / 64, (max_m + 63) / 64, nchains);
-        hipLaunchKernelGGL((k_gemm<AC, BC, OPA, OPB, 64, 64>), grid, dim3(256), 0, st, g);
+        hipLaunchKernelGGL((k_gemm<T, AC, BC, OPA, OPB, 64, 64>), grid, dim3(256), 0, st, g);
     } else {
         dim3 grid((max_n + 31) / 32, (max_m + 31) / 32, nchains);
-        hipLaunchKernelGGL((k_gemm<AC, BC, OPA, OPB, 32, 32>), grid, dim3(256), 0, st, g);
+        hipLaunchKernelGGL((k_gemm<T, AC, BC, OPA, OPB, 32, 32>), grid, dim3(256), 0, st, g);
     }
     return hipGetLastError();
 }
@@ -234,14 +255,24 @@ hipError_t launch_gemm(hipStream_t st, int nchains, int max_m, int max_n, int ma
                        int opa, int opb, const GemmArgs &g, int tile) {
     const int code = (a_cplx ? 1000 : 0) + (b_cplx ? 100 : 0) + opa * 10 + opb;
     switch (code) {
-        case 100: return run<0, 1, 0, 0>(st, nchains, max_m, max_n, max_k, g, tile);     // eK · G
-        case 110: return run<0, 1, 1, 0>(st, nchains, max_m, max_n, max_k, g, tile);     // eKᵀ · X
-        case 1000: return run<1, 0, 0, 0>(st, nchains, max_m, max_n, max_k, g, tile);    // T · eKinv
-        case 1100: return run<1, 1, 0, 0>(st, nchains, max_m, max_n, max_k, g, tile);    // U_R · X
-        case 1120: return run<1, 1, 2, 0>(st, nchains, max_m, max_n, max_k, g, tile);    // U_Lᴴᴴ · U_R, Uᴴ U
-        case 1102: return run<1, 1, 0, 2>(st, nchains, max_m, max_n, max_k, g, tile);
-        case 0: return run<0, 0, 0, 0>(st, nchains, max_m, max_n, max_k, g, tile);
-        case 10: return run<0, 0, 1, 0>(st, nchains, max_m, max_n, max_k, g, tile);
+        case 100: return run<double, 0, 1, 0, 0>(st, nchains, max_m, max_n, max_k, g, tile);     // eK · G
+        case 110: return run<double, 0, 1, 1, 0>(st, nchains, max_m, max_n, max_k, g, tile);     // eKᵀ · X
+        case 1000: return run<double, 1, 0, 0, 0>(st, nchains, max_m, max_n, max_k, g, tile);    // T · eKinv
+        case 1100: return run<double, 1, 1, 0, 0>(st, nchains, max_m, max_n, max_k, g, tile);    // U_R · X
+        case 1120: return run<double, 1, 1, 2, 0>(st, nchains, max_m, max_n, max_k, g, tile);    // U_Lᴴᴴ · U_R, Uᴴ U
+        case 1102: return run<double, 1, 1, 0, 2>(st, nchains, max_m, max_n, max_k, g, tile);
+        case 0: return run<double, 0, 0, 0, 0>(st, nchains, max_m, max_n, max_k, g, tile);
+        case 10: return run<double, 0, 0, 1, 0>(st, nchains, max_m, max_n, max_k, g, tile);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_gemm_f32(hipStream_t st, int nchains, int max_m, int max_n, int max_k, bool a_cplx, bool b_cplx,
+                           int opa, int opb, const GemmArgsF &g, int tile) {
+    const int code = (a_cplx ? 1000 : 0) + (b_cplx ? 100 : 0) + opa * 10 + opb;
+    switch (code) {
+        case 100: return run<float, 0, 1, 0, 0>(st, nchains, max_m, max_n, max_k, g, tile);    // eK · G
+        case 1000: return run<float, 1, 0, 0, 0>(st, nchains, max_m, max_n, max_k, g, tile);   // T · eKinv
         default: return hipErrorInvalidValue;
     }
 }

@@ -7,18 +7,24 @@
 //   mode 3: C = -product.
 // op codes: 0 = N, 1 = T, 2 = C (conjugate transpose). inS scales the complex operand
 // along the contraction index. Dimensions are per-chain codes (DIM_NS / DIM_NP).
-struct GemmArgs {
+template <class T>
+struct GemmArgsT {
     const ChainDev *chains;
-    const Opnd *A;
-    const Opnd *B;
-    const Opnd *C;
-    const Opnd *Cin;
+    const typename OpndOf<T>::type *A;
+    const typename OpndOf<T>::type *B;
+    const typename OpndOf<T>::type *C;
+    const typename OpndOf<T>::type *Cin;
     int mdim, ndim, kdim, mode;
     DiagSpec rowS, colS, inS;
 };
+typedef GemmArgsT<double> GemmArgs;
+typedef GemmArgsT<float> GemmArgsF;
 
 hipError_t launch_gemm(hipStream_t st, int nchains, int max_m, int max_n, int max_k, bool a_cplx, bool b_cplx,
                        int opa, int opb, const GemmArgs &g, int tile = 0);   // tile: 0 auto, 32, 64
+// FP32 variant (FP32 MFMA, FP32 accumulation; diagonal factors evaluated in FP64, rounded)
+hipError_t launch_gemm_f32(hipStream_t st, int nchains, int max_m, int max_n, int max_k, bool a_cplx, bool b_cplx,
+                           int opa, int opb, const GemmArgsF &g, int tile = 0);
 
 // Local update of one block of sites [s0, s0 + kb) at slice lt (t-U model).
 struct DecideArgs {
@@ -28,6 +34,7 @@ struct DecideArgs {
     uint32_t sweep;
     int pad;
     const uint32_t *sweep_dev;   // when set, the sweep counter is read here (graph replays)
+    const OpndF *Gf;             // when set, G[S,S] is read from this FP32 copy instead of G
 };
 hipError_t launch_decide_tu(hipStream_t st, int nchains, const DecideArgs &a);
 
@@ -35,15 +42,27 @@ hipError_t launch_decide_tu(hipStream_t st, int nchains, const DecideArgs &a);
 //   mode 0: Xout = Xin - Cin[:,A] · W · (E_A - Rin[A,:])
 //   mode 1: Xout = Xin + Cin[:,A] · W · Rin[A,:]
 // The plain sweep uses mode 0 with Cin = Rin = Xin = G. Null Cin / Rin mean Xin.
-struct FlushArgs {
+template <class T>
+struct FlushArgsT {
     const ChainDev *chains;
-    const Opnd *Gin;    // Xin
-    const Opnd *Gout;   // Xout
+    const typename OpndOf<T>::type *Gin;    // Xin
+    const typename OpndOf<T>::type *Gout;   // Xout
     int kmax, mode;     // largest accepted set the preceding decisions can produce (<= 64)
-    const Opnd *Cin;
-    const Opnd *Rin;
+    const typename OpndOf<T>::type *Cin;
+    const typename OpndOf<T>::type *Rin;
 };
+typedef FlushArgsT<double> FlushArgs;
+typedef FlushArgsT<float> FlushArgsF;
 hipError_t launch_flush(hipStream_t st, int nchains, int max_ns, const FlushArgs &f);
+hipError_t launch_flush_f32(hipStream_t st, int nchains, int max_ns, const FlushArgsF &f);   // FP32 G
+
+// Mixed precision: dst = FP32(src) for per-chain n x n complex matrices; when maxerr is set,
+// max |dst_before - src| is folded into maxerr[c] first (FP32 propagation error against the
+// FP64 recompute). to_f64 is the reverse copy.
+hipError_t launch_to_f32(hipStream_t st, int nchains, int max_n, const ChainDev *chains, const Opnd *src,
+                         const OpndF *dst, int ndim, double *maxerr);
+hipError_t launch_to_f64(hipStream_t st, int nchains, int max_n, const ChainDev *chains, const OpndF *src,
+                         const Opnd *dst, int ndim, int cplx);
 
 // Cholesky S + shift·trace(S)·I = L Lᴴ in place (lower triangle) of the per-chain n x n
 // Hermitian matrix; info (sticky, may be null) receives j+1 on a non-positive pivot.

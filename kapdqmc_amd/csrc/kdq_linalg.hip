@@ -30,6 +30,40 @@ __global__ void k_initial_s(const ChainDev *chains, int nt) {
 
 __global__ void k_bump(uint32_t *counter) { *counter += 1; }
 
+// FP64 -> FP32 copy of an n x n complex matrix; optionally max |dst_before - src| per chain
+// (non-negative doubles order like their bit patterns, so an unsigned 64-bit max works)
+__global__ void k_to_f32(const ChainDev *chains, const Opnd *src, const OpndF *dst, int ndim, double *maxerr) {
+    const int c = blockIdx.z;
+    const int n = dim_of(chains[c], ndim);
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63), j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    double err = 0.0;
+    if (i < n && j < n) {
+        const Opnd s = src[c];
+        const OpndF d = dst[c];
+        const size_t os = (size_t)i + (size_t)j * s.ld, od = (size_t)i + (size_t)j * d.ld;
+        const double sr = s.re[os], si = s.im[os];
+        if (maxerr) err = fmax(fabs((double)d.re[od] - sr), fabs((double)d.im[od] - si));
+        d.re[od] = (float)sr;
+        d.im[od] = (float)si;
+    }
+    if (!maxerr) return;
+    err = wave_max(err);
+    if ((threadIdx.x & 63) == 0 && err > 0.0)
+        atomicMax(reinterpret_cast<unsigned long long *>(maxerr + c), (unsigned long long)__double_as_longlong(err));
+}
+
+__global__ void k_to_f64(const ChainDev *chains, const OpndF *src, const Opnd *dst, int ndim, int cplx) {
+    const int c = blockIdx.z;
+    const int n = dim_of(chains[c], ndim);
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63), j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (i >= n || j >= n) return;
+    const OpndF s = src[c];
+    const Opnd d = dst[c];
+    const size_t os = (size_t)i + (size_t)j * s.ld, od = (size_t)i + (size_t)j * d.ld;
+    d.re[od] = s.re[os];
+    if (cplx) d.im[od] = s.im[os];
+}
+
 __global__ void k_zero_meas(const ChainDev *chains) {
     const ChainDev &ch = chains[blockIdx.x];
     if (threadIdx.x < KDQ_MEAS_SLOTS) ch.meas[threadIdx.x] = 0.0;
@@ -64,5 +98,19 @@ hipError_t launch_zero_meas(hipStream_t st, int nchains, const ChainDev *chains)
 
 hipError_t launch_bump(hipStream_t st, uint32_t *counter) {
     hipLaunchKernelGGL(k_bump, dim3(1), dim3(1), 0, st, counter);
+    return hipGetLastError();
+}
+
+hipError_t launch_to_f32(hipStream_t st, int nchains, int max_n, const ChainDev *chains, const Opnd *src,
+                         const OpndF *dst, int ndim, double *maxerr) {
+    dim3 grid((max_n + 63) / 64, (max_n + 3) / 4, nchains);
+    hipLaunchKernelGGL(k_to_f32, grid, dim3(256), 0, st, chains, src, dst, ndim, maxerr);
+    return hipGetLastError();
+}
+
+hipError_t launch_to_f64(hipStream_t st, int nchains, int max_n, const ChainDev *chains, const OpndF *src,
+                         const Opnd *dst, int ndim, int cplx) {
+    dim3 grid((max_n + 63) / 64, (max_n + 3) / 4, nchains);
+    hipLaunchKernelGGL(k_to_f64, grid, dim3(256), 0, st, chains, src, dst, ndim, cplx);
     return hipGetLastError();
 }

@@ -58,9 +58,16 @@ __global__ __launch_bounds__(KBT >= 32 ? 1024 : KBT * KBT) void k_decide_tu(Deci
     for (int q = 0; q < CPT; ++q) {
         const int cc = c0 + q;
         const bool ok = r < kb && cc < kb;
-        const size_t off = (size_t)(s0 + r) + (size_t)(s0 + cc) * G.ld;
-        wr[q] = ok ? G.re[off] : 0.0;
-        wi[q] = ok ? G.im[off] : 0.0;
+        if (a.Gf) {   // FP32 G: the decisions still run in FP64
+            const OpndF Gf = a.Gf[c];
+            const size_t off = (size_t)(s0 + r) + (size_t)(s0 + cc) * Gf.ld;
+            wr[q] = ok ? (double)Gf.re[off] : 0.0;
+            wi[q] = ok ? (double)Gf.im[off] : 0.0;
+        } else {
+            const size_t off = (size_t)(s0 + r) + (size_t)(s0 + cc) * G.ld;
+            wr[q] = ok ? G.re[off] : 0.0;
+            wi[q] = ok ? G.im[off] : 0.0;
+        }
         Wr[q] = 0.0;
         Wi[q] = 0.0;
     }
@@ -207,28 +214,29 @@ __global__ __launch_bounds__(KBT >= 32 ? 1024 : KBT * KBT) void k_decide_tu(Deci
 // 4jg .. 4jg+3. LDS (dynamic, k = maximum block size): W (k x k), Y then Z (k x 32),
 // Gin[I,A] (32 x k).
 constexpr int FT = 32;
-__global__ __launch_bounds__(256) void k_flush(FlushArgs f) {
+template <class T>
+__global__ __launch_bounds__(256) void k_flush(FlushArgsT<T> f) {
     const int c = blockIdx.z;
     const ChainDev &ch = f.chains[c];
     const int n = ch.ns;
     const int i0 = blockIdx.y * FT, j0 = blockIdx.x * FT;
     if (i0 >= n || j0 >= n) return;
-    const Opnd Gi = f.Gin[c], Go = f.Gout[c];
-    const Opnd Cs = f.Cin ? f.Cin[c] : Gi, Rs = f.Rin ? f.Rin[c] : Gi;
-    const double sgn = f.mode == 0 ? 1.0 : -1.0;   // Xout = Xin - sgn · Cin[:,A] W Y
-    const double edge = f.mode == 0 ? 1.0 : 0.0;   // Y = edge·E_A - sgn·Rin[A,:]
+    const auto Gi = f.Gin[c], Go = f.Gout[c];
+    const auto Cs = f.Cin ? f.Cin[c] : Gi, Rs = f.Rin ? f.Rin[c] : Gi;
+    const T sgn = f.mode == 0 ? T(1) : T(-1);   // Xout = Xin - sgn · Cin[:,A] W Y
+    const T edge = f.mode == 0 ? T(1) : T(0);   // Y = edge·E_A - sgn·Rin[A,:]
     const int tid = threadIdx.x;
     const int ii = tid & 31, jg = tid >> 5;
     const int i = i0 + ii;
     // this thread's 4 entries, fetched first so their latency overlaps the staging below
-    double gr[4], gim[4];
+    T gr[4], gim[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const int j = j0 + jg * 4 + q;
         const bool ok = i < n && j < n;
         const size_t oi = (size_t)i + (size_t)j * Gi.ld;
-        gr[q] = ok ? Gi.re[oi] : 0.0;
-        gim[q] = ok ? Gi.im[oi] : 0.0;
+        gr[q] = ok ? Gi.re[oi] : T(0);
+        gim[q] = ok ? Gi.im[oi] : T(0);
     }
     const int na = *ch.acc_cnt;
     if (na == 0) {
@@ -243,25 +251,26 @@ __global__ __launch_bounds__(256) void k_flush(FlushArgs f) {
         return;
     }
     const int K = f.kmax, KP = K + 1;
-    extern __shared__ double lds[];
-    double *Ar = lds, *Ai = Ar + (size_t)K * KP;             // W [t][u], stride KP
-    double *Pr = Ai + (size_t)K * KP, *Pi = Pr + K * (FT + 1);  // Y, then Z [t][jj]
-    double *Br = Pi + K * (FT + 1), *Bi = Br + FT * KP;        // Gin[I, A] [ii][t]
+    extern __shared__ __align__(8) unsigned char lds_raw[];
+    T *lds = reinterpret_cast<T *>(lds_raw);
+    T *Ar = lds, *Ai = Ar + (size_t)K * KP;             // W [t][u], stride KP
+    T *Pr = Ai + (size_t)K * KP, *Pi = Pr + K * (FT + 1);  // Y, then Z [t][jj]
+    T *Br = Pi + K * (FT + 1), *Bi = Br + FT * KP;        // Gin[I, A] [ii][t]
     __shared__ int aidx[KB];
     if (tid < na) aidx[tid] = ch.acc_idx[tid];
     for (int e = tid; e < na * na; e += 256) {
         const int t = e % na, u = e / na;
-        Ar[t * KP + u] = ch.W[t + u * KB];
-        Ai[t * KP + u] = ch.W[KB * KB + t + u * KB];
+        Ar[t * KP + u] = (T)ch.W[t + u * KB];
+        Ai[t * KP + u] = (T)ch.W[KB * KB + t + u * KB];
     }
     __syncthreads();
     for (int e = tid; e < na * FT; e += 256) {
         const int jj = e & (FT - 1), t = e / FT;
         const int j = j0 + jj;
-        double yr = 0.0, yi = 0.0;
+        T yr = 0, yi = 0;
         if (j < n) {
             const size_t off = (size_t)aidx[t] + (size_t)j * Rs.ld;
-            yr = (aidx[t] == j ? edge : 0.0) - sgn * Rs.re[off];
+            yr = (aidx[t] == j ? edge : T(0)) - sgn * Rs.re[off];
             yi = -sgn * Rs.im[off];
         }
         Pr[t * (FT + 1) + jj] = yr;
@@ -270,7 +279,7 @@ __global__ __launch_bounds__(256) void k_flush(FlushArgs f) {
     for (int e = tid; e < na * FT; e += 256) {
         const int r = e & (FT - 1), t = e / FT;
         const int ig = i0 + r;
-        double vr = 0.0, vi = 0.0;
+        T vr = 0, vi = 0;
         if (ig < n) {
             const size_t off = (size_t)ig + (size_t)aidx[t] * Cs.ld;
             vr = sgn * Cs.re[off];
@@ -283,16 +292,16 @@ __global__ __launch_bounds__(256) void k_flush(FlushArgs f) {
     // Z = W · Y: outputs (t, jj) spread over the threads, kept in registers until all
     // reads of Y are done
     constexpr int ZMAX = (KB * FT) / 256;
-    double zr[ZMAX], zi[ZMAX];
+    T zr[ZMAX], zi[ZMAX];
 #pragma unroll
     for (int q = 0; q < ZMAX; ++q) {
         const int e = tid + 256 * q;
         const int jj = e & (FT - 1), t = e / FT;
-        double sr = 0.0, sim = 0.0;
+        T sr = 0, sim = 0;
         if (t < na) {
             for (int u = 0; u < na; ++u) {
-                const double ar = Ar[t * KP + u], ai = Ai[t * KP + u];
-                const double br = Pr[u * (FT + 1) + jj], bi = Pi[u * (FT + 1) + jj];
+                const T ar = Ar[t * KP + u], ai = Ai[t * KP + u];
+                const T br = Pr[u * (FT + 1) + jj], bi = Pi[u * (FT + 1) + jj];
                 sr += ar * br - ai * bi;
                 sim += ar * bi + ai * br;
             }
@@ -315,10 +324,10 @@ __global__ __launch_bounds__(256) void k_flush(FlushArgs f) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const int jj = jg * 4 + q;
-        double sr = gr[q], sim = gim[q];
+        T sr = gr[q], sim = gim[q];
         for (int t = 0; t < na; ++t) {
-            const double ar = Br[ii * KP + t], ai = Bi[ii * KP + t];
-            const double pr = Pr[t * (FT + 1) + jj], pi = Pi[t * (FT + 1) + jj];
+            const T ar = Br[ii * KP + t], ai = Bi[ii * KP + t];
+            const T pr = Pr[t * (FT + 1) + jj], pi = Pi[t * (FT + 1) + jj];
             sr -= ar * pr - ai * pi;
             sim -= ar * pi + ai * pr;
         }
@@ -342,14 +351,23 @@ hipError_t launch_decide_tu(hipStream_t st, int nchains, const DecideArgs &a) {
     return hipGetLastError();
 }
 
-hipError_t launch_flush(hipStream_t st, int nchains, int max_ns, const FlushArgs &f) {
+template <class T>
+hipError_t flush_t(hipStream_t st, int nchains, int max_ns, const FlushArgsT<T> &f) {
     if (f.kmax < 1 || f.kmax > KB) return hipErrorInvalidValue;
     const int t = (max_ns + FT - 1) / FT;
     const size_t K = (size_t)f.kmax, KP = K + 1;
-    const size_t bytes = (2 * K * KP + 2 * K * (FT + 1) + 2 * FT * KP) * sizeof(double);
-    if (hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_flush),
+    const size_t bytes = (2 * K * KP + 2 * K * (FT + 1) + 2 * FT * KP) * sizeof(T);
+    if (hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_flush<T>),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes))
         return e;
-    hipLaunchKernelGGL(k_flush, dim3(t, t, nchains), dim3(256), bytes, st, f);
+    hipLaunchKernelGGL(k_flush<T>, dim3(t, t, nchains), dim3(256), bytes, st, f);
     return hipGetLastError();
+}
+
+hipError_t launch_flush(hipStream_t st, int nchains, int max_ns, const FlushArgs &f) {
+    return flush_t<double>(st, nchains, max_ns, f);
+}
+
+hipError_t launch_flush_f32(hipStream_t st, int nchains, int max_ns, const FlushArgsF &f) {
+    return flush_t<float>(st, nchains, max_ns, f);
 }

@@ -34,7 +34,7 @@ def field_shape(m):
     return (m.Ns * m.nnidx.shape[1], m.Nt)
 
 
-def _descriptors(models, seeds, chains, delay, sweep0):
+def _descriptors(models, seeds, chains, delay, sweep0, precision=L.PREC_FP64):
     """kdq_model_desc / kdq_run_desc arrays for a list of model structs (plus the numpy
     arrays they point into, which must outlive the call)."""
     n = len(models)
@@ -70,7 +70,7 @@ def _descriptors(models, seeds, chains, delay, sweep0):
         for k, v in arrs.items():
             setattr(d, k, _dp(v))
         r = runs[c]
-        r.seed, r.chain, r.sweep0, r.precision, r.delay = seeds[c], chains[c], sweep0, L.PREC_FP64, delay
+        r.seed, r.chain, r.sweep0, r.precision, r.delay = seeds[c], chains[c], sweep0, precision, delay
     return descs, runs, keep
 
 
@@ -90,12 +90,14 @@ class DQMCBatch:
         n = len(models)
         seeds = list(seeds) if isinstance(seeds, (list, tuple)) else [seeds] * n
         chains = list(range(n)) if chains is None else list(chains)
-        if precision != "fp64":
-            raise ValueError("only precision='fp64' is available")
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
         self.models = list(models)
         self.kind = getattr(models[0], "kind", "tU")
         self.nobs = NOBS[self.kind]
-        descs, runs, self._keep = _descriptors(self.models, seeds, chains, delay, sweep0)
+        self.precision = precision
+        descs, runs, self._keep = _descriptors(self.models, seeds, chains, delay, sweep0,
+                                               L.PREC_MIXED if precision == "mixed" else L.PREC_FP64)
         h = C.c_void_p()
         L.check(self.L.kdq_create(descs, runs, n, C.byref(h)))
         self.h = h
@@ -189,6 +191,13 @@ class DQMCBatch:
         self._chk(self.L.kdq_set_option(self.h, L.OPTIONS[name], int(value)))
         if name == "delay":
             self.delay = int(value)
+
+    def precision_error(self) -> np.ndarray:
+        """max |G_FP32 - G_FP64| per chain over the stabilisations since the last call
+        (mixed precision; zeros for FP64 batches)."""
+        out = np.zeros(self.n)
+        self._chk(self.L.kdq_get_precision_error(self.h, _dp(out)))
+        return out
 
     def logdet(self) -> np.ndarray:
         """log|det(U_L U_R)| of the overlap factorised by the last get_G! of each chain."""
