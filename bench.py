@@ -101,7 +101,8 @@ def cpu_baseline(m, tau, nslices: int):
                        f"({sweeps_s:.4f} sweeps/s), ESS/s with the GPU chain's tau_int={tau:.2f}")}
 
 
-def run_batch(m, nch: int, seed0: int, delay, warmup: int, steps: int, profile_all: bool):
+def run_batch(m, nch: int, seed0: int, delay, warmup: int, steps: int, profile_all: bool,
+              precision: str = "fp64"):
     """W warm-up sweeps (the first with every kernel family timed by HIP events, host
     launches, to find the dominant family; the graph of a sweep is captured untimed after
     them), `steps` timed sweeps replayed from the graph, then one sweep with host launches
@@ -111,7 +112,8 @@ def run_batch(m, nch: int, seed0: int, delay, warmup: int, steps: int, profile_a
     from kapdqmc_amd.dqmc import DQMCBatch
     from kapdqmc_amd.rng import initial_fields
     N = m.Ns
-    b = DQMCBatch([m] * nch, [seed0 + c for c in range(nch)], list(range(nch)), delay=delay or 32)
+    b = DQMCBatch([m] * nch, [seed0 + c for c in range(nch)], list(range(nch)), delay=delay or 32,
+                  precision=precision)
     for c in range(nch):
         b.set_fields(c, initial_fields(seed0 + c, c, N, m.Nt))
     tuned = None
@@ -136,12 +138,14 @@ def run_batch(m, nch: int, seed0: int, delay, warmup: int, steps: int, profile_a
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     print(f"bench: {nch} chain(s), {steps} timed sweeps in {dt:.2f} s", file=sys.stderr, flush=True)
+    err = b.precision_error()   # max |G_FP32 - G_FP64| over the timed sweeps (mixed)
     b.profile(list(prof_w) if profile_all else [dominant])
     b.phy_update(1, True)
     prof = b.profile_read()
     cnt = b.counters()
     breakdown = {f: {"ms_per_sweep": prof_w[f][0], "launches_per_sweep": prof_w[f][1]} for f in prof_w}
     return {"rows": np.asarray(rows), "dt": dt, "prof": prof, "dominant": dominant, "delay": b.delay,
+            "max_g_error": float(err.max()),
             "tuned": tuned, "breakdown": breakdown,
             "acceptance": float(cnt[:, 1].sum() / max(cnt[:, 0].sum(), 1))}
 
@@ -155,6 +159,8 @@ def main():
     ap.add_argument("--delay", type=int, default=0, help="block size k (0: autotune k and the GEMM tile)")
     ap.add_argument("--plain-steps", type=int, default=6, help="sweeps of the single-chain line (0: skip)")
     ap.add_argument("--plain-delay", type=int, default=16)
+    ap.add_argument("--mixed-steps", type=int, default=4,
+                    help="sweeps of the mixed-precision line on the same batch (0: skip)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-slices", type=int, default=20)
@@ -200,6 +206,24 @@ def main():
         plain = {"sweeps_per_s": p_sweeps, "ms_per_sweep": one["dt"] / args.plain_steps * 1e3,
                  "effective_samples_per_s": p_sweeps / (2 * tau), "delay_k": args.plain_delay,
                  "note": "same Markov chain, so the batched run's tau_int is used"}
+
+    mixed = None
+    if args.mixed_steps > 0:
+        mx = run_batch(m, C, args.seed, bat["delay"], args.warmup, args.mixed_steps, False, precision="mixed")
+        Dm = mx["rows"][:, :, 1] / N
+        m_sweeps = C * args.mixed_steps / mx["dt"]
+        # same sweeps of the same seeds; the chains part ways at the first differing decision,
+        # so the difference is judged against the chain-to-chain spread
+        d64, d32 = D[:, :args.mixed_steps].mean(axis=1), Dm.mean(axis=1)
+        sig = np.sqrt(d64.var(ddof=1) / C + d32.var(ddof=1) / C)
+        mixed = {"sweeps_per_s": m_sweeps, "ms_per_sweep": mx["dt"] / args.mixed_steps * 1e3,
+                 "effective_samples_per_s": m_sweeps / (2 * tau),
+                 "max_abs_G_fp32_minus_fp64": mx["max_g_error"],
+                 "double_occupancy_mean": float(Dm.mean()),
+                 "double_occupancy_fp64_minus_mixed": float(d64.mean() - d32.mean()),
+                 "difference_in_sigma": float((d64.mean() - d32.mean()) / sig),
+                 "note": "G carried in FP32 between stabilisations; error = max over the "
+                         "stabilisations of the timed sweeps; same seeds as the FP64 line"}
 
     dominant = bat["dominant"]
     ms, nl = bat["prof"][dominant]
@@ -248,6 +272,7 @@ def main():
         "acceptance": bat["acceptance"],
         "plain_fp64_1chain": plain,
         "ratio_vs_plain": (ess_s / plain["effective_samples_per_s"]) if plain else None,
+        "mixed_precision": mixed,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernel_breakdown_warmup": bat["breakdown"],
