@@ -141,6 +141,11 @@ int kdq_set_option(kdq_handle h, int32_t key, int32_t value);
  * last call (0 for FP64 handles); the call resets it. No counterpart in the reference. */
 int kdq_get_precision_error(kdq_handle h, double *out);
 
+/* log W(s) of every chain's current fields (t-U): 2 log|det(P† B(2Θ,0) P)| + Σ log γ(s),
+ * the weight whose ratios the sweep samples (nchains doubles). Used by the self-learning
+ * global moves; no counterpart in the reference. Leaves G to be rebuilt by the next sweep. */
+int kdq_log_weight(kdq_handle h, double *out);
+
 /* log|det(U_L U_R)| of the overlap factorised by the last get_G! of every chain (nchains
  * doubles). No counterpart in the reference, whose get_G! (src/public/Gupdate.jl:139-157)
  * discards the LU diagonal; the weight of a configuration up to the chain normalisation. */
@@ -197,6 +202,40 @@ enum {
 };
 int kdq_profile(kdq_handle h, uint32_t mask);
 int kdq_profile_read(kdq_handle h, double *ms, int64_t *launches);
+
+/* ---- self-learning global moves (t-U; no counterpart in the reference) ----
+ * A translation-equivariant convolutional effective model E(s) of the HS field on an
+ * L1 x L2 honeycomb lattice with nt slices (3 x 3 x 3 convolutions over (slice, y, x) with
+ * 8 -> C -> C -> 1 channels, tanh; periodic in x and y, zero-padded in the slice direction),
+ * trained on (field, log W) pairs so that E + log W is as constant as possible. */
+typedef struct kdq_net_s *kdq_net;
+int kdq_net_create(int32_t L1, int32_t L2, int32_t nt, int32_t channels, uint64_t seed, int32_t max_batch,
+                   kdq_net *out);
+int kdq_net_destroy(kdq_net n);
+const char *kdq_net_last_error(kdq_net n);
+int kdq_net_num_params(kdq_net n);
+int kdq_net_get_params(kdq_net n, float *out);      /* [W1 C x 8 x 27, b1 C, W2 C x C x 27, b2 C, W3 C x 27] */
+int kdq_net_set_params(kdq_net n, const float *in);
+/* E(s) of `batch` field arrays (batch x ns x nt, each as kdq_set_fields takes it). */
+int kdq_net_energy(kdq_net n, const uint8_t *fields, int32_t batch, double *E);
+/* loss = Var_b(E(s_b) + logw_b) and its gradient (grad may be NULL); batch >= 2. */
+int kdq_net_loss_grad(kdq_net n, const uint8_t *fields, const double *logw, int32_t batch, double *loss,
+                      float *grad);
+/* One Adam step (β = 0.9, 0.999) on that loss. */
+int kdq_net_train_step(kdq_net n, const uint8_t *fields, const double *logw, int32_t batch, float lr,
+                       double *loss);
+/* Inner chain alone: nsteps local steps on exp(-E) from each of the `batch` configurations in
+ * `fields` (changed in place; Philox stream 3, key = seed, chain = batch index). One step
+ * picks a slice class r in [0, 7) and proposes one single-field change (uniform site, uniform
+ * new value among the other three) in every slice t = r + 7 j. E receives E of the final
+ * fields, accepted the number of accepted changes (either may be NULL). */
+int kdq_net_sample(kdq_net n, uint8_t *fields, int32_t batch, int32_t nsteps, uint64_t seed, double *E,
+                   int32_t *accepted);
+/* One global move per chain: nsteps inner-chain steps on exp(-E) from the current fields,
+ * then acceptance with probability min(1, W(s')/W(s) e^{E(s')-E(s)}) using the exact FP64
+ * weights of kdq_log_weight (stream 4); accepted[c] = 1 or 0. The local sweep's G is rebuilt
+ * by the next kdq_phy_update. Chains must fit the network's batch and lattice. */
+int kdq_slmc_move(kdq_handle h, kdq_net net, int32_t nsteps, double *accepted);
 
 #ifdef __cplusplus
 }

@@ -21,8 +21,11 @@ OPTIONS = {"graph": 0, "delay": 1, "gemm_tile": 2}   # kdq_set_option keys
 EXPORTS = ("kdq_create", "kdq_destroy", "kdq_last_error", "kdq_num_chains", "kdq_num_obs", "kdq_set_fields",
            "kdq_get_fields", "kdq_initial_s", "kdq_phy_update", "kdq_get_G", "kdq_set_G", "kdq_wrap",
            "kdq_update_layer", "kdq_init_chain", "kdq_measure", "kdq_get_counters", "kdq_profile",
-           "kdq_profile_read", "kdq_get_logdet", "kdq_set_option", "kdq_get_precision_error", "kdq_scee_create", "kdq_scee_set_fields",
-           "kdq_scee_get_fields", "kdq_scee_init", "kdq_scee_sweeps", "kdq_scee_get_G4", "kdq_scee_detg")
+           "kdq_profile_read", "kdq_get_logdet", "kdq_set_option", "kdq_get_precision_error", "kdq_log_weight", "kdq_scee_create", "kdq_scee_set_fields",
+           "kdq_scee_get_fields", "kdq_scee_init", "kdq_scee_sweeps", "kdq_scee_get_G4", "kdq_scee_detg",
+           "kdq_net_create", "kdq_net_destroy", "kdq_net_last_error", "kdq_net_num_params", "kdq_net_get_params",
+           "kdq_net_set_params", "kdq_net_energy", "kdq_net_loss_grad", "kdq_net_train_step", "kdq_net_sample",
+           "kdq_slmc_move")
 PROF_FAMILIES = ("wrap", "chain", "get_G", "decide", "flush", "qr", "inv", "meas_wrap", "measure",
                  "gemm_other")
 
@@ -90,6 +93,7 @@ def load(path: str = LIB_PATH):
     L.kdq_get_logdet.argtypes = [vp, dp]
     L.kdq_set_option.argtypes = [vp, i32, i32]
     L.kdq_get_precision_error.argtypes = [vp, dp]
+    L.kdq_log_weight.argtypes = [vp, dp]
     L.kdq_scee_create.argtypes = [C.POINTER(ModelDesc), C.POINTER(RunDesc), C.POINTER(SceeDesc), i32, C.POINTER(vp)]
     L.kdq_scee_set_fields.argtypes = [vp, i32, i32, C.POINTER(C.c_uint8)]
     L.kdq_scee_get_fields.argtypes = [vp, i32, i32, C.POINTER(C.c_uint8)]
@@ -97,11 +101,31 @@ def load(path: str = LIB_PATH):
     L.kdq_scee_sweeps.argtypes = [vp, i32, dp]
     L.kdq_scee_get_G4.argtypes = [vp, i32, i32, dp, dp, dp, dp]
     L.kdq_scee_detg.argtypes = [vp, dp]
+    fp = C.POINTER(C.c_float)
+    u8p = C.POINTER(C.c_uint8)
+    L.kdq_net_create.argtypes = [i32, i32, i32, i32, C.c_uint64, i32, C.POINTER(vp)]
+    L.kdq_net_destroy.argtypes = [vp]
+    L.kdq_net_last_error.argtypes = [vp]
+    L.kdq_net_last_error.restype = C.c_char_p
+    L.kdq_net_num_params.argtypes = [vp]
+    L.kdq_net_get_params.argtypes = [vp, fp]
+    L.kdq_net_set_params.argtypes = [vp, fp]
+    L.kdq_net_energy.argtypes = [vp, u8p, i32, dp]
+    L.kdq_net_loss_grad.argtypes = [vp, u8p, dp, i32, dp, fp]
+    L.kdq_net_train_step.argtypes = [vp, u8p, dp, i32, C.c_float, dp]
+    L.kdq_net_sample.argtypes = [vp, u8p, i32, i32, C.c_uint64, dp, C.POINTER(C.c_int32)]
+    L.kdq_slmc_move.argtypes = [vp, vp, i32, dp]
     for name in EXPORTS:
-        if name != "kdq_last_error":
+        if name not in ("kdq_last_error", "kdq_net_last_error"):
             getattr(L, name).restype = C.c_int
     _lib = L
     return L
+
+
+def check_net(rc: int, net=None):
+    if rc != KDQ_OK:
+        msg = load().kdq_net_last_error(net).decode(errors="replace")
+        raise KdqError(rc, msg)
 
 
 def check(rc: int, handle=None):

@@ -67,6 +67,17 @@ struct kdq_handle_s {
     std::vector<Opnd *> t_UR, t_ULH;
     std::vector<void *> dev_misc;
     double *d_logdet = nullptr;   // [nch] log|det(U_L U_R)| of the last get_G
+    double *d_lw = nullptr;       // [nch] accumulator of log|det R| for kdq_log_weight
+    // self-learning global moves: trial / backup fields, device pointer table, RNG ids,
+    // inner-chain scratch, cached log W of the current fields
+    std::vector<uint8_t *> sl_trial, sl_backup;
+    uint8_t **d_sl_ptrs = nullptr;
+    uint64_t *d_sl_seed = nullptr;
+    uint32_t *d_sl_chain = nullptr;
+    double *d_sl_E = nullptr;
+    std::vector<double> sl_logw;
+    bool sl_logw_valid = false;
+    int sl_move = 0;
     int *d_info = nullptr;        // [nch] sticky: nonzero after a singular overlap
     int *d_qinfo = nullptr;       // [nch] sticky: nonzero after a rank-deficient propagator
     int *d_piv = nullptr;         // [nch][max_np] LU pivots of the overlap
@@ -315,7 +326,7 @@ int refresh_g32(kdq_handle h, bool track) {
 
 // U <- orthonormal basis of span(U) for the N x Np propagator, in place: two Cholesky-QR
 // passes (S = UᴴU, S + σI = L Lᴴ, U <- U L⁻ᴴ), the first shifted by σ = c·u·‖U‖_F².
-int orthonormalise(kdq_handle h, Opnd *U) {
+int orthonormalise(kdq_handle h, Opnd *U, double *logacc = nullptr) {
     const double u = 1.1102230246251565e-16;
     const double n = h->max_ns, k = h->max_np;
     const double shift = 11.0 * (n * k + k * (k + 1.0)) * u;
@@ -323,7 +334,7 @@ int orthonormalise(kdq_handle h, Opnd *U) {
         GemmArgs g = gargs(h, U, U, h->t_S, DIM_NP, DIM_NP, DIM_NS);
         if (int r = gemm(h, true, true, 2, 0, g, KDQ_PROF_QR)) return r;
         ProfScope ps(h, KDQ_PROF_QR);
-        CholArgs ca{h->d_chains, h->t_S, DIM_NP, 0, pass == 0 ? shift : 0.0, h->d_qinfo};
+        CholArgs ca{h->d_chains, h->t_S, DIM_NP, 0, pass == 0 ? shift : 0.0, h->d_qinfo, logacc};
         HIPCHK(launch_chol(h->stream, h->nch, h->max_np, ca));
         TrsmArgs ta{h->d_chains, U, h->t_S, DIM_NS, DIM_NP};
         HIPCHK(launch_trsm_rlh(h->stream, h->nch, h->max_ns, h->max_np, ta));
@@ -332,7 +343,7 @@ int orthonormalise(kdq_handle h, Opnd *U) {
 }
 
 // dst = QR_Q(B_block(idx) · src) for U_R: slices nodes[idx-1]+1 .. nodes[idx] (1-based idx)
-int propagate_R(kdq_handle h, Opnd *src, Opnd *dst, int idx) {
+int propagate_R(kdq_handle h, Opnd *src, Opnd *dst, int idx, double *logacc = nullptr) {
     const int lo = (int)h->nodes[idx - 1] + 1, hi = (int)h->nodes[idx];
     Opnd *cur = src;
     for (int lt = lo; lt <= hi; ++lt) {
@@ -346,7 +357,7 @@ int propagate_R(kdq_handle h, Opnd *src, Opnd *dst, int idx) {
         cur = out;
     }
     if (lo > hi) HIPCHK(launch_copy_mat(h->stream, h->nch, h->max_ns, h->max_np, h->d_chains, src, dst, DIM_NS, DIM_NP, true));
-    return orthonormalise(h, dst);
+    return orthonormalise(h, dst, logacc);
 }
 
 // dst = QR_Q((U_L · B_block(idx))ᴴ) with U_L stored as U_Lᴴ: U_Lᴴ <- eKᵀ (conj(D_l) ⊙ U_Lᴴ)
@@ -391,6 +402,37 @@ int get_G(kdq_handle h, int idx, Opnd *dst = nullptr) {
     GemmArgs g3 = gargs(h, UR, h->t_X, dst ? dst : h->t_G[h->cur], DIM_NS, DIM_NS, DIM_NP);
     g3.mode = 1;
     return gemm(h, true, true, 0, 0, g3, KDQ_PROF_GETG);
+}
+
+// log W(s) of every chain's current fields (t-U): 2 log|det(P† B(2Θ,0) P)| + Σ log γ(s).
+// U_R is rebuilt from P through every node block with the orthonormalisation's discarded
+// triangular factors summed (U = Q L₂ᴴ L₁ᴴ, so log|det R| = Σ log L₁,jj + Σ log L₂,jj), and
+// the last factor is the LU of P† U_R. The chain buffers are left for the next
+// phy_update, which rebuilds them. No counterpart in the reference (its sweep only needs
+// ratios).
+int log_weight(kdq_handle h, double *out) {
+    const int NN = h->nn;
+    HIPCHK(hipMemsetAsync(h->d_lw, 0, sizeof(double) * h->nch, h->stream));
+    HIPCHK(launch_real_to_cplx(h->stream, h->nch, h->max_ns, h->max_np, h->d_chains, h->t_Pt, h->t_UR[0], DIM_NS, DIM_NP));
+    for (int idx = 1; idx <= NN - 1; ++idx)
+        if (int r = propagate_R(h, h->t_UR[idx - 1], h->t_UR[idx], idx, h->d_lw)) return r;
+    GemmArgs g = gargs(h, h->t_Pt, h->t_UR[NN - 1], h->t_M, DIM_NP, DIM_NP, DIM_NS);   // Pᵀ U_R
+    if (int r = gemm(h, false, true, 1, 0, g, KDQ_PROF_GETG)) return r;
+    LuArgs la{h->d_chains, h->t_M, DIM_NP, h->max_np, h->d_piv, h->d_logdet, h->d_info};
+    HIPCHK(launch_lu(h->stream, h->nch, h->max_np, la));
+    std::vector<double> lr(h->nch), lm(h->nch);
+    HIPCHK(hipMemcpyAsync(lr.data(), h->d_lw, sizeof(double) * h->nch, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(lm.data(), h->d_logdet, sizeof(double) * h->nch, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    for (int c = 0; c < h->nch; ++c) {
+        const size_t ne = (size_t)h->ch[c].ns * h->nt;
+        std::vector<uint8_t> sv(ne);
+        HIPCHK(hipMemcpy(sv.data(), h->ch[c].s, ne, hipMemcpyDeviceToHost));
+        double lg = 0.0;
+        for (size_t q = 0; q < ne; ++q) lg += std::log(h->md[c].gamma[sv[q] - 1]);
+        out[c] = 2.0 * (lr[c] + lm[c]) + lg;
+    }
+    return KDQ_OK;
 }
 
 int init_chain(kdq_handle h) {
@@ -659,6 +701,7 @@ constexpr long kMaxGraphLaunches = 20000;
 
 int phy_update(kdq_handle h, int sweeps, int record, double *rows) {
     const int no = h->nobs;
+    h->sl_logw_valid = false;
     if (int r = init_chain(h)) return r;
     std::vector<double> tmp(no * h->nch);
     const bool graph = h->use_graph && h->prof_mask == 0 && sweep_launch_estimate(h) <= kMaxGraphLaunches;
@@ -1011,9 +1054,11 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
         h->d_info = (int *)q2;
         h->d_qinfo = (int *)q3;
         h->d_piv = (int *)q4;
-        void *q5 = nullptr;
+        void *q5 = nullptr, *q6 = nullptr;
         if (hipMalloc(&q5, sizeof(uint32_t)) == hipSuccess) h->dev_misc.push_back(q5); else r = 1;
+        if (hipMalloc(&q6, sizeof(double) * nchains) == hipSuccess) h->dev_misc.push_back(q6); else r = 1;
         h->d_sweep = (uint32_t *)q5;
+        h->d_lw = (double *)q6;
         if (!r) {
             hipMemset(q2, 0, sizeof(int) * nchains);
             hipMemset(q3, 0, sizeof(int) * nchains);
@@ -1052,6 +1097,7 @@ int kdq_set_fields(kdq_handle h, int32_t c, const uint8_t *s) {
         if (s[i] < 1 || s[i] > 4) return fail(h, KDQ_EARG, "kdq_set_fields: field values must be 1..4");
     HIPCHK(hipMemcpyAsync(h->ch[c].s, s, n, hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
+    h->sl_logw_valid = false;
     return KDQ_OK;
 }
 
@@ -1200,6 +1246,87 @@ int kdq_get_precision_error(kdq_handle h, double *out) {
     HIPCHK(hipMemsetAsync(h->d_maxerr, 0, sizeof(double) * h->nch, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     return KDQ_OK;
+}
+
+int kdq_slmc_move(kdq_handle h, kdq_net net, int32_t nsteps, double *accepted) {
+    if (!h || !net || !accepted || nsteps < 0 || h->kind != KDQ_MODEL_TU || h->scee)
+        return fail(h, KDQ_EARG, "kdq_slmc_move: t-U batches with a network only");
+    int nsn = 0, ntn = 0;
+    slmc_geometry(net, &nsn, &ntn);
+    for (int c = 0; c < h->nch; ++c)
+        if (h->ch[c].ns != nsn || h->nt != ntn) return fail(h, KDQ_EARG, "kdq_slmc_move: network lattice differs from a chain's");
+    const int nch = h->nch;
+    const size_t ne = (size_t)nsn * h->nt;
+    if (!h->d_sl_ptrs) {   // first move: scratch
+        h->sl_trial.resize(nch);
+        h->sl_backup.resize(nch);
+        int bad = 0;
+        for (int c = 0; c < nch; ++c) {
+            void *q1 = nullptr, *q2 = nullptr;
+            if (hipMalloc(&q1, ne) == hipSuccess) h->dev_misc.push_back(q1); else bad = 1;
+            if (hipMalloc(&q2, ne) == hipSuccess) h->dev_misc.push_back(q2); else bad = 1;
+            h->sl_trial[c] = (uint8_t *)q1;
+            h->sl_backup[c] = (uint8_t *)q2;
+        }
+        void *q[4] = {};
+        const size_t sz[4] = {sizeof(uint8_t *) * nch, sizeof(uint64_t) * nch, sizeof(uint32_t) * nch,
+                              sizeof(double) * nch};
+        for (int k = 0; k < 4; ++k) {
+            if (hipMalloc(&q[k], sz[k]) == hipSuccess) h->dev_misc.push_back(q[k]); else bad = 1;
+        }
+        if (bad) return fail(h, KDQ_EHIP, "kdq_slmc_move: device allocation failed");
+        h->d_sl_ptrs = (uint8_t **)q[0];
+        h->d_sl_seed = (uint64_t *)q[1];
+        h->d_sl_chain = (uint32_t *)q[2];
+        h->d_sl_E = (double *)q[3];
+        std::vector<uint64_t> sd(nch);
+        std::vector<uint32_t> cid(nch);
+        for (int c = 0; c < nch; ++c) {
+            sd[c] = h->rd[c].seed;
+            cid[c] = h->rd[c].chain;
+        }
+        HIPCHK(hipMemcpy(h->d_sl_ptrs, h->sl_trial.data(), sizeof(uint8_t *) * nch, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(h->d_sl_seed, sd.data(), sizeof(uint64_t) * nch, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(h->d_sl_chain, cid.data(), sizeof(uint32_t) * nch, hipMemcpyHostToDevice));
+    }
+    if (!h->sl_logw_valid) {
+        h->sl_logw.assign(nch, 0.0);
+        if (int r = log_weight(h, h->sl_logw.data())) return r;
+        h->sl_logw_valid = true;
+    }
+    for (int c = 0; c < nch; ++c) {
+        HIPCHK(hipMemcpyAsync(h->sl_trial[c], h->ch[c].s, ne, hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(hipMemcpyAsync(h->sl_backup[c], h->ch[c].s, ne, hipMemcpyDeviceToDevice, h->stream));
+    }
+    std::vector<double> E0(nch), E1(nch), lw1(nch);
+    if (int r = slmc_inner(net, h->stream, nch, h->d_sl_ptrs, h->d_sl_seed, h->d_sl_chain, h->d_sl_E, nsteps,
+                           h->sl_move, E0.data()))
+        return fail(h, r, std::string("kdq_slmc_move: ") + kdq_net_last_error(net));
+    HIPCHK(hipMemcpyAsync(E1.data(), h->d_sl_E, sizeof(double) * nch, hipMemcpyDeviceToHost, h->stream));
+    for (int c = 0; c < nch; ++c)
+        HIPCHK(hipMemcpyAsync(h->ch[c].s, h->sl_trial[c], ne, hipMemcpyDeviceToDevice, h->stream));
+    if (int r = log_weight(h, lw1.data())) return r;   // synchronises
+    // acceptance min(1, W(s')/W(s) · exp(E(s') - E(s))) with the exact FP64 weights
+    for (int c = 0; c < nch; ++c) {
+        uint32_t x[4];
+        philox4x32_10(0u, (uint32_t)h->sl_move, 0x51C2u, (4u << 24) | (h->rd[c].chain & 0xFFFFFFu), (uint32_t)h->rd[c].seed,
+                      (uint32_t)(h->rd[c].seed >> 32), x);
+        const double u = ((double)(x[1] >> 5) * 67108864.0 + (double)(x[2] >> 6)) * (1.0 / 9007199254740992.0);
+        const double la = lw1[c] - h->sl_logw[c] + (E1[c] - E0[c]);
+        const bool acc = std::isfinite(la) && (la >= 0.0 || u < std::exp(la));
+        accepted[c] = acc ? 1.0 : 0.0;
+        if (acc) h->sl_logw[c] = lw1[c];
+        else HIPCHK(hipMemcpyAsync(h->ch[c].s, h->sl_backup[c], ne, hipMemcpyDeviceToDevice, h->stream));
+    }
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->sl_move += 1;
+    return check_health(h);
+}
+
+int kdq_log_weight(kdq_handle h, double *out) {
+    if (!h || !out || h->kind != KDQ_MODEL_TU || h->scee) return fail(h, KDQ_EARG, "kdq_log_weight: t-U batches only");
+    if (int r = log_weight(h, out)) return r;
+    return check_health(h);
 }
 
 int kdq_get_logdet(kdq_handle h, double *out) {

@@ -115,6 +115,12 @@ __global__ __launch_bounds__(NT) void k_chol(CholArgs a) {
     }
     __syncthreads();
     if (tid == 0 && bad && a.info && !a.info[c]) a.info[c] = bad;
+    if (a.logdiag) {
+        double lsum = 0.0;
+        for (int i = tid; i < n; i += NT) lsum += log(Re[i + (size_t)i * ld]);
+        lsum = blk_sum(lsum, red);
+        if (tid == 0) a.logdiag[c] += lsum;
+    }
 }
 
 // X <- X L⁻ᴴ for the m x n matrix X (in place; L lower, real positive diagonal):

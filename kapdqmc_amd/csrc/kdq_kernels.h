@@ -2,6 +2,8 @@
 #pragma once
 #include "kdq_common.h"
 
+#include <vector>
+
 // C = rowS ⊙ (op(A) · diag(inS) · op(B)) ⊙ colS, with
 //   mode 0: C = product, mode 1: C = I - product, mode 2: C = Cin - product,
 //   mode 3: C = -product.
@@ -72,6 +74,7 @@ struct CholArgs {
     int ndim, pad;
     double shift;
     int *info;
+    double *logdiag;   // when set, logdiag[c] += Σ_j log L_jj (the log|det| of the factor)
 };
 hipError_t launch_chol(hipStream_t st, int nchains, int max_n, const CholArgs &a);
 
@@ -187,3 +190,12 @@ hipError_t launch_scee_accum(hipStream_t st, int nch, const SceeRegion *reg);
 // X[i][i] += v for the per-chain n x n matrices (dimension code ndim); set = 1 writes I.
 hipError_t launch_diag(hipStream_t st, int nchains, int max_n, const ChainDev *chains, const Opnd *X, int ndim,
                        double v, int set);
+
+// ---------------------------------------------------------------- self-learning moves
+typedef struct kdq_net_s *kdq_net;
+// Inner chain of a global move on exp(-E) of the network for nch chains whose trial fields
+// (device arrays; d_field_ptrs is the device table of them) change in place: nsteps local
+// steps; E0_host receives the starting energies, d_E1 the final ones.
+int slmc_inner(kdq_net n, hipStream_t st, int nch, uint8_t **d_field_ptrs, const uint64_t *d_seed,
+               const uint32_t *d_chain, double *d_E1, int nsteps, int move, double *E0_host);
+int slmc_geometry(kdq_net n, int *ns, int *nt);

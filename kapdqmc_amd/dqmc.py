@@ -199,10 +199,24 @@ class DQMCBatch:
         self._chk(self.L.kdq_get_precision_error(self.h, _dp(out)))
         return out
 
+    def log_weight(self) -> np.ndarray:
+        """log W(s) = 2 log|det(P† B(2Θ,0) P)| + Σ log γ(s) of every chain's current fields."""
+        out = np.zeros(self.n)
+        self._chk(self.L.kdq_log_weight(self.h, _dp(out)))
+        return out
+
     def logdet(self) -> np.ndarray:
         """log|det(U_L U_R)| of the overlap factorised by the last get_G! of each chain."""
         out = np.zeros(self.n)
         self._chk(self.L.kdq_get_logdet(self.h, _dp(out)))
+        return out
+
+    def slmc_move(self, net, nsteps: int) -> np.ndarray:
+        """One self-learning global move per chain (t-U): ``nsteps`` inner-chain steps of
+        ``net`` (a kapdqmc_amd.slmc.EffectiveModel) from the current fields, accepted with
+        min(1, W(s')/W(s) e^{E(s')-E(s)}). Returns 1.0 / 0.0 per chain."""
+        out = np.zeros(self.n)
+        self._chk(self.L.kdq_slmc_move(self.h, net.h, int(nsteps), _dp(out)))
         return out
 
 

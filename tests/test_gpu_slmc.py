@@ -1,0 +1,210 @@
+"""Self-learning Monte Carlo on the GPU, through the C ABI.
+
+* exact log W(s) = 2 log|det(P† B(2Θ,0) P)| + Σ log γ(s) vs a brute-force FP64 product of the
+  oracle's B blocks (tiny lattice, unstabilised);
+* network E and loss gradient vs the numpy restatement (FP32 vs FP64: 1e-4 relative on E,
+  2e-3 relative on the gradient norm);
+* the inner chain's tracked E equals a fresh forward pass of its final fields bit for bit
+  (with and without periodic wrap-around of the update neighbourhoods);
+* the inner chain samples exp(-E): element marginals and <E> vs exact enumeration of all
+  4^6 configurations of a 1 x 1 lattice with 3 slices (chi-square, p > 1e-4);
+* the global move's acceptance decision equals min(1, W(s')/W(s) e^{E(s')-E(s)}) replayed
+  from the exact weights, network energies and the Philox draw;
+* training lowers Var(E + log W) and raises the global-move acceptance over a zero network.
+"""
+import numpy as np
+import pytest
+
+from kapdqmc_amd.models import tU_Hubbard_Para
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def mods():
+    from kapdqmc_amd import dqmc, slmc
+    return dqmc, slmc
+
+
+def tu(L=3, U=4.0, theta=1.0, bs=5):
+    return tU_Hubbard_Para(Ht=1.0, Hu1=U, Hu2=U, Δt=0.1, Θrelax=theta, Θquench=0.0,
+                           Lattice="HoneyComb120", site=[L, L], BatchSize=bs)
+
+
+def brute_log_weight(oracle_mod, m, s):
+    o = oracle_mod.OracleTU(m, seed=0)
+    o.s = s
+    B = np.eye(m.Ns, dtype=np.complex128)
+    for idx in range(1, len(m.nodes)):
+        B = o.BM(idx) @ B
+    _, ld = np.linalg.slogdet(m.Pt.T @ B @ m.Pt)
+    return 2.0 * ld + np.log(m.γ[np.asarray(s, dtype=np.int64) - 1]).sum()
+
+
+def test_log_weight_brute_force(mods, oracle_mod):
+    dqmc, _ = mods
+    print("case: log_weight vs brute force, L=3 U=2 theta=1", flush=True)
+    m = tu(U=2.0)
+    b = dqmc.DQMCBatch([m, m, m], [3, 4, 5])
+    b.initial_s()
+    lw = b.log_weight()
+    for c in range(3):
+        ref = brute_log_weight(oracle_mod, m, b.get_fields(c))
+        assert abs(lw[c] - ref) < 1e-9 * max(1.0, abs(ref)), (c, lw[c], ref)
+    # one field change: the weight ratio follows
+    s = b.get_fields(0)
+    s[5, 7] = 1 + (s[5, 7] % 4)
+    b.set_fields(0, s)
+    lw2 = b.log_weight()
+    assert abs(lw2[0] - brute_log_weight(oracle_mod, m, s)) < 1e-9 * max(1.0, abs(lw2[0]))
+    b.close()
+
+
+def _net_setup(slmc, L1, L2, nt, C, B, scale, seed=7):
+    from oracle import slmc_net as sn
+    net = slmc.EffectiveModel(L1, L2, nt, channels=C, seed=seed, max_batch=max(B, 2))
+    rng = np.random.default_rng(seed)
+    p = (rng.normal(0, scale, net.num_params)).astype(np.float32)
+    net.set_params(p)
+    geo = sn.Geometry(L1, L2, nt)
+    f = rng.integers(1, 5, size=(B, geo.n, nt)).astype(np.uint8)
+    return net, p, geo, f, sn, rng
+
+
+def test_net_energy_and_grad_vs_oracle(mods):
+    _, slmc = mods
+    print("case: net E / grad vs numpy, 3x2 lattice nt=6 C=4", flush=True)
+    net, p, geo, f, sn, rng = _net_setup(slmc, 3, 2, 6, 4, 5, 0.3)
+    fields = [np.asfortranarray(x) for x in f]
+    E = net.energy(fields)
+    Eo = sn.energy(geo, p.astype(np.float64), 4, f)
+    assert np.abs(E - Eo).max() < 1e-4 * np.abs(Eo).max()
+    lw = rng.normal(0, 3, len(f))
+    loss, g = net.loss_grad(fields, lw)
+    lo, go = sn.loss_grad(geo, p.astype(np.float64), 4, f, lw)
+    assert abs(loss - lo) < 1e-4 * max(lo, 1.0)
+    assert np.linalg.norm(g - go) < 2e-3 * np.linalg.norm(go)
+    net.close()
+
+
+@pytest.mark.parametrize("L1,L2,nt", [(8, 9, 16), (3, 3, 10), (1, 2, 5)])
+def test_inner_chain_tracks_energy_exactly(mods, L1, L2, nt):
+    _, slmc = mods
+    print(f"case: inner chain E tracking {L1}x{L2} nt={nt}", flush=True)
+    net, p, geo, f, sn, rng = _net_setup(slmc, L1, L2, nt, 4, 6, 0.2)
+    fields = [np.asfortranarray(x) for x in f]
+    out, E, acc = net.sample(fields, 40, seed=11)
+    assert (acc > 0).all()
+    assert any((a != b).any() for a, b in zip(out, fields))
+    E2 = net.energy(out)
+    assert np.array_equal(E, E2), (E - E2)
+    # fields changed only where accepted changes can land, values stay in 1..4
+    assert all(((o >= 1) & (o <= 4)).all() for o in out)
+    net.close()
+
+
+def test_inner_chain_samples_exp_minus_E(mods):
+    _, slmc = mods
+    print("case: inner chain distribution vs enumeration, 1x1 lattice nt=3", flush=True)
+    nb = 512
+    net, p, geo, f, sn, rng = _net_setup(slmc, 1, 1, 3, 3, nb, 0.35, seed=3)
+    # exact distribution over all 4^6 configurations
+    allf = np.array(np.meshgrid(*[np.arange(1, 5)] * 6, indexing="ij")).reshape(6, -1).T
+    allf = allf.reshape(-1, 2, 3, order="F").astype(np.uint8)
+    E = np.concatenate([net.energy([np.asfortranarray(x) for x in allf[k:k + nb]])
+                        for k in range(0, len(allf), nb)])
+    w = np.exp(-(E - E.min()))
+    w /= w.sum()
+    flat = allf.reshape(len(allf), -1, order="F")
+    marg = np.stack([[w[flat[:, e] == v].sum() for v in range(1, 5)] for e in range(6)])
+    Emean = (w * E).sum()
+    Evar = (w * (E - Emean) ** 2).sum()
+    assert np.abs(marg - 0.25).max() > 0.05 and marg.min() > 0.002   # non-uniform, every bin populated
+    counts = np.zeros((6, 4))
+    Es = []
+    for call in range(8):
+        start = [np.asfortranarray(x) for x in rng.integers(1, 5, size=(nb, 2, 3)).astype(np.uint8)]
+        out, Ef, acc = net.sample(start, 200, seed=100 + call)
+        Es.append(Ef)
+        for o in out:
+            v = o.reshape(-1, order="F")
+            counts[np.arange(6), v - 1] += 1
+    n = counts.sum(axis=1)[0]
+    chi2 = (((counts - n * marg) ** 2) / (n * marg)).sum()
+    from scipy.stats import chi2 as chi2d
+    pval = chi2d.sf(chi2, 18)
+    Es = np.concatenate(Es)
+    z = (Es.mean() - Emean) / np.sqrt(Evar / len(Es))
+    print(f"chi2={chi2:.1f} p={pval:.3g} z(E)={z:.2f}", flush=True)
+    assert pval > 1e-4
+    assert abs(z) < 4.5
+    net.close()
+
+
+def test_global_move_acceptance_rule(mods):
+    dqmc, slmc = mods
+    from kapdqmc_amd.rng import philox4x32_10
+    print("case: global move acceptance replay, L=3 U=4 theta=1, 8 chains", flush=True)
+    m = tu()
+    n, seed, nsteps = 8, 1234, 2
+    b = dqmc.DQMCBatch([m] * n, [seed] * n, list(range(n)))
+    b.initial_s()
+    b.phy_update(2, False)
+    s0 = [b.get_fields(c) for c in range(n)]
+    lw0 = b.log_weight()
+    params = None
+    nets = []
+    for k in range(2):
+        net = slmc.EffectiveModel.for_model(m, channels=4, seed=9, max_batch=n)
+        if params is None:
+            params = net.get_params() * 0.2
+        net.set_params(params)
+        nets.append(net)
+    # the proposal, replayed on a twin network (same seeds, chain ids and move counter)
+    prop, E1, _ = nets[1].sample(s0, nsteps, seed=seed)
+    E0 = nets[1].energy(s0)
+    acc = b.slmc_move(nets[0], nsteps)
+    after = [b.get_fields(c) for c in range(n)]
+    t = dqmc.DQMCBatch([m] * n, [seed] * n, list(range(n)))
+    for c in range(n):
+        t.set_fields(c, prop[c])
+    lw1 = t.log_weight()
+    for c in range(n):
+        x = philox4x32_10(0, 0, 0x51C2, (4 << 24) | c, seed & 0xFFFFFFFF, seed >> 32)
+        u = ((int(x[1]) >> 5) * 67108864.0 + (int(x[2]) >> 6)) / 9007199254740992.0
+        la = lw1[c] - lw0[c] + (E1[c] - E0[c])
+        expect = la >= 0 or u < np.exp(la)
+        assert bool(acc[c]) == expect, (c, la, u)
+        assert np.array_equal(after[c], prop[c] if expect else s0[c])
+    print("accepted", acc, flush=True)
+    # the batch keeps sweeping from the new fields
+    b.phy_update(1, False)
+    for x in nets:
+        x.close()
+    b.close()
+    t.close()
+
+
+def test_training_raises_acceptance(mods):
+    dqmc, slmc = mods
+    print("case: training, L=3 U=4 theta=1, 32 chains", flush=True)
+    m = tu()
+    n = 32
+    b = dqmc.DQMCBatch([m] * n, list(range(n)), list(range(n)))
+    b.initial_s()
+    b.phy_update(3, False)
+    net = slmc.EffectiveModel.for_model(m, channels=4, seed=1, max_batch=n)
+    losses = slmc.train(b, net, rounds=30, sweeps_per_round=1, lr=3e-3, epochs=10)
+    print("loss first/last", losses[0], losses[-1], flush=True)
+    assert losses[-1] < 0.5 * losses[0]
+    zero = slmc.EffectiveModel.for_model(m, channels=4, seed=1, max_batch=n)
+    zero.set_params(np.zeros(zero.num_params, dtype=np.float32))
+    rate = {}
+    for name, nt in (("trained", net), ("zero", zero)):
+        a = [b.slmc_move(nt, 2).mean() for _ in range(6)]
+        rate[name] = float(np.mean(a))
+    print("acceptance", rate, flush=True)
+    assert rate["trained"] > rate["zero"] + 0.05
+    net.close()
+    zero.close()
+    b.close()
