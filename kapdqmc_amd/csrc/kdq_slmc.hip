@@ -74,19 +74,23 @@ KDQ_DEV float h1_eval(const Geo &g, const NetW &w, int co, int q, FV fv) {
     return tanhf(s);
 }
 
-// Σ_ci Σ_k W[co][ci][k] in(ci, q + off_k), started from s0, tap validity outermost
-template <class IV>
-KDQ_DEV float conv_eval(const Geo &g, const float *W, int cin, int co, int q, float s0, IV iv) {
-    int pos[27];
+// Σ_ci Σ_k W[co][ci][k] in(ci, q + off_k), started from s0: taps resolved once (tag(p)),
+// inputs read as iv(ci, tag); the fma sequence is the same whatever the accessors
+template <class TAG, class IV>
+KDQ_DEV float conv_eval(const Geo &g, const float *W, int cin, int co, int q, float s0, TAG tag, IV iv) {
+    int src[27];
     bool ok[27];
 #pragma unroll
-    for (int k = 0; k < 27; ++k) pos[k] = tap_pos(g, q, k, ok[k]);
+    for (int k = 0; k < 27; ++k) {
+        const int p = tap_pos(g, q, k, ok[k]);
+        src[k] = tag(p);
+    }
     float s = s0;
     for (int ci = 0; ci < cin; ++ci) {
         const float *wk = W + ((size_t)co * cin + ci) * 27;
 #pragma unroll
         for (int k = 0; k < 27; ++k)
-            if (ok[k]) s = fmaf(wk[k], iv(ci, pos[k]), s);
+            if (ok[k]) s = fmaf(wk[k], iv(ci, src[k]), s);
     }
     return s;
 }
@@ -107,7 +111,7 @@ __global__ void k_conv(const float *in, const float *W, const float *bias, float
     if (e >= (size_t)B * cout * g.P) return;
     const int q = e % g.P, co = (e / g.P) % cout, b = e / ((size_t)g.P * cout);
     const float *src = in + (size_t)b * cin * g.P;
-    const float s = conv_eval(g, W, cin, co, q, bias ? bias[co] : 0.f,
+    const float s = conv_eval(g, W, cin, co, q, bias ? bias[co] : 0.f, [](int p) { return p; },
                               [&](int ci, int p) { return src[(size_t)ci * g.P + p]; });
     out[e] = act ? tanhf(s) : s;
 }
@@ -305,9 +309,11 @@ __global__ __launch_bounds__(STEP_THREADS) void k_slmc_step(StepArgs a) {
     double *red = lds_d;
     const size_t cP = (size_t)C * g.P;
     const float *H1 = a.h1 + c * cP, *H2 = a.h2 + c * cP, *O = a.o + (size_t)c * g.P;
-    // new h1 around the change
+    // new h1 around the change; LDS is channel-major (nh1[co n1 + j]) and lanes run over
+    // positions, so weight reads are uniform across a wave. A tap's tag is -1 - j for
+    // position j of the inner box (new value in LDS) or the global position p >= 0.
     for (int e = tid; e < n1 * C; e += STEP_THREADS) {
-        const int jp = e / C, co = e % C;
+        const int co = e / n1, jp = e % n1;
         const int q = box_pos(g, B1, jp);
         nh1[e] = h1_eval(g, a.w, co, q, [&](int sb, int p) {
             return (p == q0 && sb == sub) ? nv : field_at(g, F, n, p, sb);
@@ -315,22 +321,28 @@ __global__ __launch_bounds__(STEP_THREADS) void k_slmc_step(StepArgs a) {
     }
     __syncthreads();
     for (int e = tid; e < n2 * C; e += STEP_THREADS) {
-        const int jp = e / C, co = e % C;
+        const int co = e / n2, jp = e % n2;
         const int q = box_pos(g, B2, jp);
-        const float s = conv_eval(g, a.w.W2, C, co, q, a.w.b2[co], [&](int ci, int p) {
-            const int jj = box_index(g, B1, p);
-            return jj >= 0 ? nh1[jj * C + ci] : H1[(size_t)ci * g.P + p];
-        });
+        const float s = conv_eval(
+            g, a.w.W2, C, co, q, a.w.b2[co],
+            [&](int p) {
+                const int jj = box_index(g, B1, p);
+                return jj >= 0 ? -1 - jj : p;
+            },
+            [&](int ci, int tg) { return tg < 0 ? nh1[ci * n1 - 1 - tg] : H1[(size_t)ci * g.P + tg]; });
         nh2[e] = tanhf(s);
     }
     __syncthreads();
     double d = 0.0;
     for (int e = tid; e < n3; e += STEP_THREADS) {
         const int q = box_pos(g, B3, e);
-        const float on = conv_eval(g, a.w.W3, C, 0, q, 0.f, [&](int ci, int p) {
-            const int jj = box_index(g, B2, p);
-            return jj >= 0 ? nh2[jj * C + ci] : H2[(size_t)ci * g.P + p];
-        });
+        const float on = conv_eval(
+            g, a.w.W3, C, 0, q, 0.f,
+            [&](int p) {
+                const int jj = box_index(g, B2, p);
+                return jj >= 0 ? -1 - jj : p;
+            },
+            [&](int ci, int tg) { return tg < 0 ? nh2[ci * n2 - 1 - tg] : H2[(size_t)ci * g.P + tg]; });
         no[e] = on;
         d += (double)on - (double)O[q];
     }
@@ -343,8 +355,8 @@ __global__ __launch_bounds__(STEP_THREADS) void k_slmc_step(StepArgs a) {
     const double dE = red[0];
     if (!(u < exp(-dE))) return;   // block-uniform
     float *H1w = a.h1 + c * cP, *H2w = a.h2 + c * cP, *Ow = a.o + (size_t)c * g.P;
-    for (int e = tid; e < n1 * C; e += STEP_THREADS) H1w[(size_t)(e % C) * g.P + box_pos(g, B1, e / C)] = nh1[e];
-    for (int e = tid; e < n2 * C; e += STEP_THREADS) H2w[(size_t)(e % C) * g.P + box_pos(g, B2, e / C)] = nh2[e];
+    for (int e = tid; e < n1 * C; e += STEP_THREADS) H1w[(size_t)(e / n1) * g.P + box_pos(g, B1, e % n1)] = nh1[e];
+    for (int e = tid; e < n2 * C; e += STEP_THREADS) H2w[(size_t)(e / n2) * g.P + box_pos(g, B2, e % n2)] = nh2[e];
     for (int e = tid; e < n3; e += STEP_THREADS) Ow[box_pos(g, B3, e)] = no[e];
     if (tid == 0) {
         F[site + (size_t)t * n] = (uint8_t)nv;
