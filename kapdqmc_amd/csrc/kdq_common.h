@@ -65,10 +65,12 @@ struct ChainDev {
 
 #define KDQ_MEAS_SLOTS 16
 
-enum { DIM_NS = 0, DIM_NP = 1, DIM_NA = 2, DIM_NB = 3 };
+// DIM_ACC: the accepted count of the current update block (device counter), the rank of a
+// delayed-update flush.
+enum { DIM_NS = 0, DIM_NP = 1, DIM_NA = 2, DIM_NB = 3, DIM_ACC = 4 };
 
 KDQ_DEV int dim_of(const ChainDev &c, int code) {
-    return code == DIM_NS ? c.ns : code == DIM_NP ? c.np : code == DIM_NA ? c.nra : c.nrb;
+    return code == DIM_NS ? c.ns : code == DIM_NP ? c.np : code == DIM_NA ? c.nra : code == DIM_NB ? c.nrb : *c.acc_cnt;
 }
 
 // Diagonal factor applied to GEMM rows, columns or the contraction index.

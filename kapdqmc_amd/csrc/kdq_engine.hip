@@ -34,6 +34,8 @@ struct ChainHost {
     double *eK, *eKinv, *hK, *hKinv, *K, *Pt, *alpha;
     double *G[2], *T, *Gm, *Tm, *P0, *P1, *P2, *M, *X, *S;
     float *G32[2] = {nullptr, nullptr}, *T32 = nullptr, *eK32 = nullptr, *eKinv32 = nullptr;   // mixed precision
+    double *FZ = nullptr, *FC = nullptr;     // flush scratch: W·Y (64 x N) and gathered columns (N x 64)
+    float *FZ32 = nullptr, *FC32 = nullptr;
     int *ex = nullptr, *ey = nullptr, *inc = nullptr;
     std::vector<double *> UR, ULH;
     uint8_t *s;
@@ -64,6 +66,8 @@ struct kdq_handle_s {
     ChainDev *d_chains = nullptr;
     // operand tables
     Opnd *t_eK, *t_eKinv, *t_hK, *t_hKinv, *t_Pt, *t_G[2], *t_T, *t_Gm, *t_Tm, *t_P0, *t_P1, *t_P2, *t_M, *t_X, *t_S;
+    Opnd *t_FZ = nullptr, *t_FC = nullptr;     // flush scratch operands
+    OpndF *t_FZ32 = nullptr, *t_FC32 = nullptr;
     std::vector<Opnd *> t_UR, t_ULH;
     std::vector<void *> dev_misc;
     double *d_logdet = nullptr;   // [nch] log|det(U_L U_R)| of the last get_G
@@ -461,7 +465,7 @@ int update_family_r2(kdq_handle h, int lt, int dir, uint32_t sweep, int family) 
             ProfScope ps(h, KDQ_PROF_DECIDE);
             HIPCHK(launch_decide_r2(h->stream, h->nch, a));
         }
-        FlushArgs f{h->d_chains, h->t_G[h->cur], h->t_G[1 - h->cur], 64, 0, nullptr, nullptr};   // two slots per element
+        FlushArgs f{h->d_chains, h->t_G[h->cur], h->t_G[1 - h->cur], 64, 0, nullptr, nullptr, h->t_FZ, h->t_FC};   // two slots per element
         {
             ProfScope ps(h, KDQ_PROF_FLUSH);
             HIPCHK(launch_flush(h->stream, h->nch, h->max_ns, f));
@@ -487,13 +491,13 @@ int update_layer(kdq_handle h, int lt, int dir, uint32_t sweep) {
             HIPCHK(launch_decide_tu(h->stream, h->nch, a));
         }
         if (h->mixed) {
-            FlushArgsF f{h->d_chains, h->t_G32[h->cur32], h->t_G32[1 - h->cur32], kb, 0, nullptr, nullptr};
+            FlushArgsF f{h->d_chains, h->t_G32[h->cur32], h->t_G32[1 - h->cur32], kb, 0, nullptr, nullptr, h->t_FZ32, h->t_FC32};
             ProfScope ps(h, KDQ_PROF_FLUSH);
             HIPCHK(launch_flush_f32(h->stream, h->nch, h->max_ns, f));
             h->cur32 ^= 1;
             continue;
         }
-        FlushArgs f{h->d_chains, h->t_G[h->cur], h->t_G[1 - h->cur], kb, 0, nullptr, nullptr};
+        FlushArgs f{h->d_chains, h->t_G[h->cur], h->t_G[1 - h->cur], kb, 0, nullptr, nullptr, h->t_FZ, h->t_FC};
         {
             ProfScope ps(h, KDQ_PROF_FLUSH);
             HIPCHK(launch_flush(h->stream, h->nch, h->max_ns, f));
@@ -798,7 +802,8 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
     }
     std::vector<ChainDev> cd(nchains);
     std::vector<Opnd> v_eK, v_eKinv, v_hK, v_hKinv, v_Pt, v_G0, v_G1, v_T, v_Gm, v_Tm, v_P0, v_P1, v_P2, v_M, v_X, v_S;
-    std::vector<OpndF> v_G32a, v_G32b, v_T32, v_eK32, v_eKinv32;
+    std::vector<OpndF> v_G32a, v_G32b, v_T32, v_eK32, v_eKinv32, v_FZ32, v_FC32;
+    std::vector<Opnd> v_FZ, v_FC;
     std::vector<BlockDesc> blk_all;
     std::vector<int> blk_offset, blk_count;
     std::vector<std::vector<Opnd>> v_UR(h->nn), v_ULH(h->nn);
@@ -877,6 +882,8 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
         ok &= dalloc(C, &C.acc_idx, 64) == hipSuccess;
         ok &= dalloc(C, &C.info, 1) == hipSuccess;
         ok &= dalloc(C, &C.W, 2 * 64 * 64) == hipSuccess;
+        ok &= dalloc(C, &C.FZ, 2 * 64 * (size_t)n) == hipSuccess;
+        ok &= dalloc(C, &C.FC, 2 * 64 * (size_t)n) == hipSuccess;
         ok &= dalloc(C, &C.counters, 2) == hipSuccess;
         ok &= dalloc(C, &C.meas, KDQ_MEAS_SLOTS) == hipSuccess;
         ok &= dalloc(C, &C.logdet, 1) == hipSuccess;
@@ -886,6 +893,8 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
             ok &= dalloc(C, &C.T32, 2 * nn) == hipSuccess;
             ok &= dalloc(C, &C.eK32, nn) == hipSuccess;
             ok &= dalloc(C, &C.eKinv32, nn) == hipSuccess;
+            ok &= dalloc(C, &C.FZ32, 2 * 64 * (size_t)n) == hipSuccess;
+            ok &= dalloc(C, &C.FC32, 2 * 64 * (size_t)n) == hipSuccess;
         }
         if (!ok) {
             int r = fail(h, KDQ_EHIP, "kdq_create: device allocation failed");
@@ -979,7 +988,11 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
         v_P0.push_back(cplx(C.P0, n, nP)); v_P1.push_back(cplx(C.P1, n, nP)); v_P2.push_back(cplx(C.P2, n, nP));
         v_M.push_back(cplx(C.M, np, pp)); v_X.push_back(cplx(C.X, np, nP));
         v_S.push_back(cplx(C.S, np, pp));
+        v_FZ.push_back(Opnd{C.FZ, C.FZ + 64 * (size_t)n, 64, 0});
+        v_FC.push_back(Opnd{C.FC, C.FC + 64 * (size_t)n, n, 0});
         if (h->mixed) {
+            v_FZ32.push_back(OpndF{C.FZ32, C.FZ32 + 64 * (size_t)n, 64, 0});
+            v_FC32.push_back(OpndF{C.FC32, C.FC32 + 64 * (size_t)n, n, 0});
             v_G32a.push_back(OpndF{C.G32[0], C.G32[0] + nn, n, 0});
             v_G32b.push_back(OpndF{C.G32[1], C.G32[1] + nn, n, 0});
             v_T32.push_back(OpndF{C.T32, C.T32 + nn, n, 0});
@@ -1025,10 +1038,12 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
         h->d_nblocks = (int *)q3;
     }
     r |= make_table(h, &h->t_M, v_M); r |= make_table(h, &h->t_X, v_X); r |= make_table(h, &h->t_S, v_S);
+    r |= make_table(h, &h->t_FZ, v_FZ); r |= make_table(h, &h->t_FC, v_FC);
     if (h->mixed) {
         r |= make_table_f(h, &h->t_G32[0], v_G32a); r |= make_table_f(h, &h->t_G32[1], v_G32b);
         r |= make_table_f(h, &h->t_T32, v_T32);
         r |= make_table_f(h, &h->t_eK32, v_eK32); r |= make_table_f(h, &h->t_eKinv32, v_eKinv32);
+        r |= make_table_f(h, &h->t_FZ32, v_FZ32); r |= make_table_f(h, &h->t_FC32, v_FC32);
         void *q = nullptr;
         if (hipMalloc(&q, sizeof(double) * nchains) == hipSuccess) {
             h->dev_misc.push_back(q);
@@ -1513,10 +1528,10 @@ int scee_flush(kdq_handle h) {
     const int kb = S->kb;
     Opnd *Gt = h->t_G[h->cur];
     FlushArgs f[4] = {
-        {h->d_chains, Gt, h->t_G[1 - h->cur], kb, 0, nullptr, nullptr},                    // Gt
-        {h->d_chains, xcur(h, SX_G0T), xspare(h, SX_G0T), kb, 0, nullptr, Gt},              // G0t
-        {h->d_chains, xcur(h, SX_GT0), xspare(h, SX_GT0), kb, 1, Gt, nullptr},              // Gt0
-        {h->d_chains, xcur(h, SX_G0), xspare(h, SX_G0), kb, 1, xcur(h, SX_G0T), xcur(h, SX_GT0)},   // G0
+        {h->d_chains, Gt, h->t_G[1 - h->cur], kb, 0, nullptr, nullptr, h->t_FZ, h->t_FC},                    // Gt
+        {h->d_chains, xcur(h, SX_G0T), xspare(h, SX_G0T), kb, 0, nullptr, Gt, h->t_FZ, h->t_FC},              // G0t
+        {h->d_chains, xcur(h, SX_GT0), xspare(h, SX_GT0), kb, 1, Gt, nullptr, h->t_FZ, h->t_FC},              // Gt0
+        {h->d_chains, xcur(h, SX_G0), xspare(h, SX_G0), kb, 1, xcur(h, SX_G0T), xcur(h, SX_GT0), h->t_FZ, h->t_FC},   // G0
     };
     ProfScope ps(h, KDQ_PROF_FLUSH);
     for (int k = 0; k < 4; ++k) HIPCHK(launch_flush(h->stream, h->nch, h->max_ns, f[k]));

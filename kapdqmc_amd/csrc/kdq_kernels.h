@@ -44,6 +44,9 @@ hipError_t launch_decide_tu(hipStream_t st, int nchains, const DecideArgs &a);
 //   mode 0: Xout = Xin - Cin[:,A] · W · (E_A - Rin[A,:])
 //   mode 1: Xout = Xin + Cin[:,A] · W · Rin[A,:]
 // The plain sweep uses mode 0 with Cin = Rin = Xin = G. Null Cin / Rin mean Xin.
+// Two launches: Z = W · Y (|A| x N) and the gathered columns ±Cin[:,A] (N x |A|) into the
+// scratch operands Zs (ld 64) and Cs, then Xout = Xin - Cs · Zs of rank |A| (the accepted
+// count, read on the device).
 template <class T>
 struct FlushArgsT {
     const ChainDev *chains;
@@ -52,6 +55,8 @@ struct FlushArgsT {
     int kmax, mode;     // largest accepted set the preceding decisions can produce (<= 64)
     const typename OpndOf<T>::type *Cin;
     const typename OpndOf<T>::type *Rin;
+    const typename OpndOf<T>::type *Zs;     // scratch: 64 x N per chain
+    const typename OpndOf<T>::type *Cs;     // scratch: N x 64 per chain
 };
 typedef FlushArgsT<double> FlushArgs;
 typedef FlushArgsT<float> FlushArgsF;

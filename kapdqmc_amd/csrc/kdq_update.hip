@@ -210,132 +210,141 @@ __global__ __launch_bounds__(KBT >= 32 ? 1024 : KBT * KBT) void k_decide_tu(Deci
 
 // Xout[I,J] = Xin[I,J] - sgn · Cin[I,A] · (W · Y[:,J]),  Y[t, j] = edge · δ(A_t, j) - sgn · Rin[A_t, j]
 // (mode 0: sgn = edge = 1; mode 1: sgn = -1, edge = 0, so the product is added).
-// 32 x 32 tiles, 256 threads: thread (ii = tid & 31, jg = tid >> 5) owns row ii, columns
-// 4jg .. 4jg+3. LDS (dynamic, k = maximum block size): W (k x k), Y then Z (k x 32),
-// Gin[I,A] (32 x k).
-constexpr int FT = 32;
+// Flush, first launch: per chain and chunk of PC columns, Z[:, J] = W · Y[:, J] with
+// Y = edge·E_A - sgn·Rin[A, :], and rows J of the gathered columns Cs = sgn·Cin[:, A]. The
+// accepted sites of a block are consecutive indices, so the row gather Rin[A, j] reads one or
+// two cache lines per column.
+constexpr int PC = 64;
 template <class T>
-__global__ __launch_bounds__(256) void k_flush(FlushArgsT<T> f) {
-    const int c = blockIdx.z;
+__global__ __launch_bounds__(256) void k_flush_prep(FlushArgsT<T> f) {
+    const int c = blockIdx.y;
     const ChainDev &ch = f.chains[c];
-    const int n = ch.ns;
-    const int i0 = blockIdx.y * FT, j0 = blockIdx.x * FT;
-    if (i0 >= n || j0 >= n) return;
-    const auto Gi = f.Gin[c], Go = f.Gout[c];
-    const auto Cs = f.Cin ? f.Cin[c] : Gi, Rs = f.Rin ? f.Rin[c] : Gi;
-    const T sgn = f.mode == 0 ? T(1) : T(-1);   // Xout = Xin - sgn · Cin[:,A] W Y
-    const T edge = f.mode == 0 ? T(1) : T(0);   // Y = edge·E_A - sgn·Rin[A,:]
-    const int tid = threadIdx.x;
-    const int ii = tid & 31, jg = tid >> 5;
-    const int i = i0 + ii;
-    // this thread's 4 entries, fetched first so their latency overlaps the staging below
-    T gr[4], gim[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int j = j0 + jg * 4 + q;
-        const bool ok = i < n && j < n;
-        const size_t oi = (size_t)i + (size_t)j * Gi.ld;
-        gr[q] = ok ? Gi.re[oi] : T(0);
-        gim[q] = ok ? Gi.im[oi] : T(0);
-    }
-    const int na = *ch.acc_cnt;
-    if (na == 0) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int j = j0 + jg * 4 + q;
-            if (i < n && j < n) {
-                Go.re[(size_t)i + (size_t)j * Go.ld] = gr[q];
-                Go.im[(size_t)i + (size_t)j * Go.ld] = gim[q];
-            }
-        }
-        return;
-    }
-    const int K = f.kmax, KP = K + 1;
+    const int n = ch.ns, na = *ch.acc_cnt;
+    const int j0 = blockIdx.x * PC;
+    if (na == 0 || j0 >= n) return;   // rank 0: the GEMM copies Xin
+    const auto Gi = f.Gin[c];
+    const auto Ci = f.Cin ? f.Cin[c] : Gi, Ri = f.Rin ? f.Rin[c] : Gi;
+    const auto Z = f.Zs[c], Cg = f.Cs[c];
+    const T sgn = f.mode == 0 ? T(1) : T(-1);
+    const T edge = f.mode == 0 ? T(1) : T(0);
+    const int K = f.kmax;
     extern __shared__ __align__(8) unsigned char lds_raw[];
-    T *lds = reinterpret_cast<T *>(lds_raw);
-    T *Ar = lds, *Ai = Ar + (size_t)K * KP;             // W [t][u], stride KP
-    T *Pr = Ai + (size_t)K * KP, *Pi = Pr + K * (FT + 1);  // Y, then Z [t][jj]
-    T *Br = Pi + K * (FT + 1), *Bi = Br + FT * KP;        // Gin[I, A] [ii][t]
+    T *Wr = reinterpret_cast<T *>(lds_raw), *Wi = Wr + K * K;   // W [t][u], stride K
+    T *Yr = Wi + K * K, *Yi = Yr + K * PC;                       // Y [u][jj]
     __shared__ int aidx[KB];
+    const int tid = threadIdx.x;
     if (tid < na) aidx[tid] = ch.acc_idx[tid];
     for (int e = tid; e < na * na; e += 256) {
         const int t = e % na, u = e / na;
-        Ar[t * KP + u] = (T)ch.W[t + u * KB];
-        Ai[t * KP + u] = (T)ch.W[KB * KB + t + u * KB];
+        Wr[t * K + u] = (T)ch.W[t + u * KB];
+        Wi[t * K + u] = (T)ch.W[KB * KB + t + u * KB];
     }
     __syncthreads();
-    for (int e = tid; e < na * FT; e += 256) {
-        const int jj = e & (FT - 1), t = e / FT;
-        const int j = j0 + jj;
+    for (int e = tid; e < na * PC; e += 256) {
+        const int jj = e % PC, u = e / PC, j = j0 + jj;
         T yr = 0, yi = 0;
         if (j < n) {
-            const size_t off = (size_t)aidx[t] + (size_t)j * Rs.ld;
-            yr = (aidx[t] == j ? edge : T(0)) - sgn * Rs.re[off];
-            yi = -sgn * Rs.im[off];
+            const size_t off = (size_t)aidx[u] + (size_t)j * Ri.ld;
+            yr = (aidx[u] == j ? edge : T(0)) - sgn * Ri.re[off];
+            yi = -sgn * Ri.im[off];
+            const size_t oc = (size_t)j + (size_t)aidx[u] * Ci.ld, og = (size_t)j + (size_t)u * Cg.ld;
+            Cg.re[og] = sgn * Ci.re[oc];
+            Cg.im[og] = sgn * Ci.im[oc];
         }
-        Pr[t * (FT + 1) + jj] = yr;
-        Pi[t * (FT + 1) + jj] = yi;
-    }
-    for (int e = tid; e < na * FT; e += 256) {
-        const int r = e & (FT - 1), t = e / FT;
-        const int ig = i0 + r;
-        T vr = 0, vi = 0;
-        if (ig < n) {
-            const size_t off = (size_t)ig + (size_t)aidx[t] * Cs.ld;
-            vr = sgn * Cs.re[off];
-            vi = sgn * Cs.im[off];
-        }
-        Br[r * KP + t] = vr;
-        Bi[r * KP + t] = vi;
+        Yr[u * PC + jj] = yr;
+        Yi[u * PC + jj] = yi;
     }
     __syncthreads();
-    // Z = W · Y: outputs (t, jj) spread over the threads, kept in registers until all
-    // reads of Y are done
-    constexpr int ZMAX = (KB * FT) / 256;
-    T zr[ZMAX], zi[ZMAX];
+    for (int e = tid; e < na * PC; e += 256) {
+        const int jj = e % PC, t = e / PC, j = j0 + jj;
+        if (j >= n) continue;
+        T sr = 0, si = 0;
+        for (int u = 0; u < na; ++u) {
+            const T ar = Wr[t * K + u], ai = Wi[t * K + u];
+            const T br = Yr[u * PC + jj], bi = Yi[u * PC + jj];
+            sr += ar * br - ai * bi;
+            si += ar * bi + ai * br;
+        }
+        const size_t oz = (size_t)t + (size_t)j * Z.ld;
+        Z.re[oz] = sr;
+        Z.im[oz] = si;
+    }
+}
+
+// Flush, second launch: Xout = Xin - Cs · Zs of rank |A|. Tile 32 rows x 64 columns; the
+// thread's Xin entries (row ii, columns 8 jg .. 8 jg + 7) are loaded first, together with the
+// first chunk of Cs[I, t] and Zs[t, J] (16 ranks per chunk, staged in LDS); the rank loop then
+// reads Cs rows (consecutive across the wave) and Zs columns (one address per half-wave) from LDS.
+constexpr int AM = 32, AN = 64, AQ = 8, AK = 16;
+template <class T>
+__global__ __launch_bounds__(256) void k_flush_apply(FlushArgsT<T> f) {
+    const int c = blockIdx.z;
+    const ChainDev &ch = f.chains[c];
+    const int n = ch.ns;
+    const int i0 = blockIdx.y * AM, j0 = blockIdx.x * AN;
+    if (i0 >= n || j0 >= n) return;
+    const int tid = threadIdx.x, ii = tid & (AM - 1), jg = tid >> 5;
+    const int i = i0 + ii;
+    const auto Gi = f.Gin[c], Go = f.Gout[c], Z = f.Zs[c], Cg = f.Cs[c];
+    __shared__ T Csr[AK][AM], Csi[AK][AM];
+    __shared__ T Zsr[AK][AN + 1], Zsi[AK][AN + 1];
+    T xr[AQ], xi[AQ], ar[AQ], ai[AQ];
 #pragma unroll
-    for (int q = 0; q < ZMAX; ++q) {
-        const int e = tid + 256 * q;
-        const int jj = e & (FT - 1), t = e / FT;
-        T sr = 0, sim = 0;
-        if (t < na) {
-            for (int u = 0; u < na; ++u) {
-                const T ar = Ar[t * KP + u], ai = Ai[t * KP + u];
-                const T br = Pr[u * (FT + 1) + jj], bi = Pi[u * (FT + 1) + jj];
-                sr += ar * br - ai * bi;
-                sim += ar * bi + ai * br;
+    for (int q = 0; q < AQ; ++q) {
+        const int j = j0 + jg * AQ + q;
+        const bool ok = i < n && j < n;
+        const size_t oi = (size_t)i + (size_t)j * Gi.ld;
+        xr[q] = ok ? Gi.re[oi] : T(0);
+        xi[q] = ok ? Gi.im[oi] : T(0);
+        ar[q] = T(0);
+        ai[q] = T(0);
+    }
+    const int na = *ch.acc_cnt;
+    for (int t0 = 0; t0 < na; t0 += AK) {
+        const int tk = min(AK, na - t0);
+        // stage: Cs[I, t0 + t] (2 x 16 x 32) and Zs[t0 + t, J] (2 x 16 x 64)
+        for (int e = tid; e < AK * AM; e += 256) {
+            const int r = e & (AM - 1), t = e / AM;
+            T vr = 0, vi = 0;
+            if (t < tk && i0 + r < n) {
+                const size_t oc = (size_t)(i0 + r) + (size_t)(t0 + t) * Cg.ld;
+                vr = Cg.re[oc];
+                vi = Cg.im[oc];
+            }
+            Csr[t][r] = vr;
+            Csi[t][r] = vi;
+        }
+        for (int e = tid; e < AK * AN; e += 256) {
+            const int t = e % AK, jj = e / AK;
+            T vr = 0, vi = 0;
+            if (t < tk && j0 + jj < n) {
+                const size_t oz = (size_t)(t0 + t) + (size_t)(j0 + jj) * Z.ld;
+                vr = Z.re[oz];
+                vi = Z.im[oz];
+            }
+            Zsr[t][jj] = vr;
+            Zsi[t][jj] = vi;
+        }
+        __syncthreads();
+        for (int t = 0; t < tk; ++t) {
+            const T cr = Csr[t][ii], ci = Csi[t][ii];
+#pragma unroll
+            for (int q = 0; q < AQ; ++q) {
+                const T zr = Zsr[t][jg * AQ + q], zi = Zsi[t][jg * AQ + q];
+                ar[q] = fma(cr, zr, fma(-ci, zi, ar[q]));
+                ai[q] = fma(cr, zi, fma(ci, zr, ai[q]));
             }
         }
-        zr[q] = sr;
-        zi[q] = sim;
+        __syncthreads();
     }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < ZMAX; ++q) {
-        const int e = tid + 256 * q;
-        const int jj = e & (FT - 1), t = e / FT;
-        if (t < na) {
-            Pr[t * (FT + 1) + jj] = zr[q];
-            Pi[t * (FT + 1) + jj] = zi[q];
-        }
-    }
-    __syncthreads();
     if (i >= n) return;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int jj = jg * 4 + q;
-        T sr = gr[q], sim = gim[q];
-        for (int t = 0; t < na; ++t) {
-            const T ar = Br[ii * KP + t], ai = Bi[ii * KP + t];
-            const T pr = Pr[t * (FT + 1) + jj], pi = Pi[t * (FT + 1) + jj];
-            sr -= ar * pr - ai * pi;
-            sim -= ar * pi + ai * pr;
-        }
-        const int j = j0 + jj;
+    for (int q = 0; q < AQ; ++q) {
+        const int j = j0 + jg * AQ + q;
         if (j < n) {
             const size_t oo = (size_t)i + (size_t)j * Go.ld;
-            Go.re[oo] = sr;
-            Go.im[oo] = sim;
+            Go.re[oo] = xr[q] - ar[q];
+            Go.im[oo] = xi[q] - ai[q];
         }
     }
 }
@@ -353,14 +362,16 @@ hipError_t launch_decide_tu(hipStream_t st, int nchains, const DecideArgs &a) {
 
 template <class T>
 hipError_t flush_t(hipStream_t st, int nchains, int max_ns, const FlushArgsT<T> &f) {
-    if (f.kmax < 1 || f.kmax > KB) return hipErrorInvalidValue;
-    const int t = (max_ns + FT - 1) / FT;
-    const size_t K = (size_t)f.kmax, KP = K + 1;
-    const size_t bytes = (2 * K * KP + 2 * K * (FT + 1) + 2 * FT * KP) * sizeof(T);
-    if (hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_flush<T>),
+    if (f.kmax < 1 || f.kmax > KB || !f.Zs || !f.Cs) return hipErrorInvalidValue;
+    const size_t K = (size_t)f.kmax;
+    const size_t bytes = (2 * K * K + 2 * K * PC) * sizeof(T);
+    if (hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_flush_prep<T>),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes))
         return e;
-    hipLaunchKernelGGL(k_flush<T>, dim3(t, t, nchains), dim3(256), bytes, st, f);
+    hipLaunchKernelGGL(k_flush_prep<T>, dim3((max_ns + PC - 1) / PC, nchains), dim3(256), bytes, st, f);
+    if (hipError_t e = hipGetLastError()) return e;
+    hipLaunchKernelGGL(k_flush_apply<T>, dim3((max_ns + AN - 1) / AN, (max_ns + AM - 1) / AM, nchains), dim3(256), 0,
+                       st, f);
     return hipGetLastError();
 }
 
