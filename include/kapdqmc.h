@@ -130,8 +130,14 @@ int kdq_get_counters(kdq_handle h, double *out);
  *                     Sweeps of more than 20000 launches (small blocks on large lattices)
  *                     also run with host launches.
  *  KDQ_OPT_DELAY      decision block size k of t-U chains (1..64; 1..16 with replicas).
- *  KDQ_OPT_GEMM_TILE  0 (default: by occupancy), 32 or 64: GEMM output tile edge. */
-enum { KDQ_OPT_GRAPH = 0, KDQ_OPT_DELAY = 1, KDQ_OPT_GEMM_TILE = 2 };
+ *  KDQ_OPT_GEMM_TILE  0 (default: by occupancy), 32 or 64: GEMM output tile edge.
+ *  KDQ_OPT_DEFER      t-U chains. 0 (default): every block of k sites rewrites G (rank-|A|
+ *                     flush). 1: a slice's local updates keep G0 and low-rank factors U, V
+ *                     (the current G is G0 - U V); each block reads its corrected k x k
+ *                     submatrix and appends its rank-|A| update to U and V, and G is rewritten
+ *                     once per slice by one GEMM of the accumulated rank. Same Markov chain;
+ *                     G agrees to rounding. */
+enum { KDQ_OPT_GRAPH = 0, KDQ_OPT_DELAY = 1, KDQ_OPT_GEMM_TILE = 2, KDQ_OPT_DEFER = 3 };
 int kdq_set_option(kdq_handle h, int32_t key, int32_t value);
 
 /* Mixed precision (kdq_run_desc.precision = KDQ_PREC_MIXED, t-U): G is carried in FP32

@@ -61,16 +61,28 @@ struct ChainDev {
     const int *inc;        // t-U-V: [ns * 6] incidence, entry = elem * 2 + (0: coefficient 1, 1: coefficient a)
     double a, U, V;        // t-U-V constants (a = V/g), couplings used by the observables
     int nra, nrb;          // SCEE region sizes (0 outside SCEE)
+    // deferred flush of a t-U slice: rk[b] = rank of the factors U, V before block b (rk[0]
+    // is never read: 0), *rank = rank after the latest block
+    int *rk;
+    int *rank;
 };
 
 #define KDQ_MEAS_SLOTS 16
 
 // DIM_ACC: the accepted count of the current update block (device counter), the rank of a
 // delayed-update flush.
-enum { DIM_NS = 0, DIM_NP = 1, DIM_NA = 2, DIM_NB = 3, DIM_ACC = 4 };
+// DIM_RANK: the rank of the deferred factors after the latest block.
+enum { DIM_NS = 0, DIM_NP = 1, DIM_NA = 2, DIM_NB = 3, DIM_ACC = 4, DIM_RANK = 5 };
 
 KDQ_DEV int dim_of(const ChainDev &c, int code) {
-    return code == DIM_NS ? c.ns : code == DIM_NP ? c.np : code == DIM_NA ? c.nra : code == DIM_NB ? c.nrb : *c.acc_cnt;
+    switch (code) {
+        case DIM_NS: return c.ns;
+        case DIM_NP: return c.np;
+        case DIM_NA: return c.nra;
+        case DIM_NB: return c.nrb;
+        case DIM_ACC: return *c.acc_cnt;
+        default: return *c.rank;
+    }
 }
 
 // Diagonal factor applied to GEMM rows, columns or the contraction index.

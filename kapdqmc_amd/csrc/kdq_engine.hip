@@ -36,6 +36,9 @@ struct ChainHost {
     float *G32[2] = {nullptr, nullptr}, *T32 = nullptr, *eK32 = nullptr, *eKinv32 = nullptr;   // mixed precision
     double *FZ = nullptr, *FC = nullptr;     // flush scratch: W·Y (64 x N) and gathered columns (N x 64)
     float *FZ32 = nullptr, *FC32 = nullptr;
+    double *DU = nullptr, *DV = nullptr;     // deferred-flush factors of a t-U slice (N x N each)
+    float *DU32 = nullptr, *DV32 = nullptr;
+    int *rk = nullptr, *rank = nullptr;
     int *ex = nullptr, *ey = nullptr, *inc = nullptr;
     std::vector<double *> UR, ULH;
     uint8_t *s;
@@ -68,6 +71,9 @@ struct kdq_handle_s {
     Opnd *t_eK, *t_eKinv, *t_hK, *t_hKinv, *t_Pt, *t_G[2], *t_T, *t_Gm, *t_Tm, *t_P0, *t_P1, *t_P2, *t_M, *t_X, *t_S;
     Opnd *t_FZ = nullptr, *t_FC = nullptr;     // flush scratch operands
     OpndF *t_FZ32 = nullptr, *t_FC32 = nullptr;
+    Opnd *t_DU = nullptr, *t_DV = nullptr;     // deferred-flush factors (t-U)
+    OpndF *t_DU32 = nullptr, *t_DV32 = nullptr;
+    bool defer = false;                        // t-U slices flush once (U, V factors)
     std::vector<Opnd *> t_UR, t_ULH;
     std::vector<void *> dev_misc;
     double *d_logdet = nullptr;   // [nch] log|det(U_L U_R)| of the last get_G
@@ -475,20 +481,61 @@ int update_family_r2(kdq_handle h, int lt, int dir, uint32_t sweep, int family) 
     return KDQ_OK;
 }
 
+// The rank-R flush at the end of a deferred slice: G_new = G0 - U[:, :R] V[:R, :] (V stored
+// transposed, so the GEMM reads it as Vᵀ).
+int defer_flush(kdq_handle h) {
+    ProfScope ps(h, KDQ_PROF_FLUSH);
+    if (h->mixed) {
+        GemmArgsF g{};
+        g.chains = h->d_chains;
+        g.A = h->t_DU32; g.B = h->t_DV32; g.C = h->t_G32[1 - h->cur32]; g.Cin = h->t_G32[h->cur32];
+        g.mdim = DIM_NS; g.ndim = DIM_NS; g.kdim = DIM_RANK; g.mode = 2;
+        HIPCHK(launch_gemm_f32(h->stream, h->nch, h->max_ns, h->max_ns, h->max_ns, true, true, 0, 1, g, h->gemm_tile));
+        h->cur32 ^= 1;
+    } else {
+        GemmArgs g{};
+        g.chains = h->d_chains;
+        g.A = h->t_DU; g.B = h->t_DV; g.C = h->t_G[1 - h->cur]; g.Cin = h->t_G[h->cur];
+        g.mdim = DIM_NS; g.ndim = DIM_NS; g.kdim = DIM_RANK; g.mode = 2;
+        HIPCHK(launch_gemm(h->stream, h->nch, h->max_ns, h->max_ns, h->max_ns, true, true, 0, 1, g, h->gemm_tile));
+        h->cur ^= 1;
+    }
+    return KDQ_OK;
+}
+
 int update_layer(kdq_handle h, int lt, int dir, uint32_t sweep) {
     if (h->kind == KDQ_MODEL_TUV) return update_family_r2(h, lt, dir, sweep, 0);
     const int kb = h->delay;
-    for (int s0 = 0; s0 < h->max_ns; s0 += kb) {
+    const bool defer = h->defer && !h->scee;
+    int blk = 0;
+    for (int s0 = 0; s0 < h->max_ns; s0 += kb, ++blk) {
         DecideArgs a{};
         a.chains = h->d_chains;
         a.G = h->t_G[h->cur];
         a.lt = lt; a.dir = dir; a.s0 = s0; a.kb = kb; a.sweep = sweep;
         a.sweep_dev = h->sweep_dev;
         if (h->mixed) a.Gf = h->t_G32[h->cur32];
+        if (defer) {
+            a.deferred = 1;
+            a.blk = blk;
+            a.U = h->t_DU; a.V = h->t_DV;
+            if (h->mixed) { a.Uf = h->t_DU32; a.Vf = h->t_DV32; }
+        }
         // t-U only; t-V layers go through slice_tv
         {
             ProfScope ps(h, KDQ_PROF_DECIDE);
             HIPCHK(launch_decide_tu(h->stream, h->nch, a));
+        }
+        if (defer) {
+            ProfScope ps(h, KDQ_PROF_FLUSH);
+            if (h->mixed) {
+                DeferArgsT<float> d{h->d_chains, h->t_G32[h->cur32], h->t_DU32, h->t_DV32, blk, kb};
+                HIPCHK(launch_defer_xy(h->stream, h->nch, h->max_ns, d));
+            } else {
+                DeferArgsT<double> d{h->d_chains, h->t_G[h->cur], h->t_DU, h->t_DV, blk, kb};
+                HIPCHK(launch_defer_xy(h->stream, h->nch, h->max_ns, d));
+            }
+            continue;
         }
         if (h->mixed) {
             FlushArgsF f{h->d_chains, h->t_G32[h->cur32], h->t_G32[1 - h->cur32], kb, 0, nullptr, nullptr, h->t_FZ32, h->t_FC32};
@@ -504,6 +551,7 @@ int update_layer(kdq_handle h, int lt, int dir, uint32_t sweep) {
         }
         h->cur ^= 1;
     }
+    if (defer) return defer_flush(h);
     return KDQ_OK;
 }
 
@@ -697,8 +745,11 @@ int capture_sweep(kdq_handle h, int record) {
 // of graph nodes; past 20000 nodes (small decision blocks on large
 // lattices) the sweep runs with host launches instead of being captured.
 long sweep_launch_estimate(kdq_handle h) {
+    // t-U: decide + factor append per block and one flush GEMM (deferred), or decide + two
+    // flush launches per block; rank-2 models: decide + two flush launches per block
     const long blocks = h->kind == KDQ_MODEL_TU ? (h->max_ns + h->delay - 1) / h->delay : 3L * h->max_blocks;
-    const long per_slice = 2 * blocks + 8 + (h->kind == KDQ_MODEL_TV ? 12 : 0);
+    const bool defer = h->kind == KDQ_MODEL_TU && h->defer && !h->scee;
+    const long per_slice = (defer ? 2 * blocks + 1 : 3 * blocks) + 8 + (h->kind == KDQ_MODEL_TV ? 12 : 0);
     return 2L * h->nt * per_slice + 40L * h->nn + 2L * h->nt;
 }
 constexpr long kMaxGraphLaunches = 20000;
@@ -755,6 +806,8 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
                 const uint32_t *streams, kdq_handle *out) {
     if (!models || !runs || !out || nchains < 1) return fail(nullptr, KDQ_EARG, "kdq_create: null argument or nchains < 1");
     *out = nullptr;
+    int ns_all = 0;
+    for (int c = 0; c < nchains; ++c) ns_all = std::max(ns_all, (int)models[c].ns);
     for (int c = 0; c < nchains; ++c) {
         const kdq_model_desc &m = models[c];
         if (m.kind < KDQ_MODEL_TU || m.kind > KDQ_MODEL_TUV) return fail(nullptr, KDQ_EARG, "kdq_create: unknown model kind");
@@ -789,6 +842,7 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
     h->delay = runs[0].delay;
     h->kind = models[0].kind;
     h->mixed = runs[0].precision == KDQ_PREC_MIXED;
+    h->defer = false;   // per-block flush by default (faster on the measured workloads)
     h->nobs = h->kind == KDQ_MODEL_TU ? 6 : (h->kind == KDQ_MODEL_TV ? 10 : 7);
     h->md.assign(models, models + nchains);
     h->rd.assign(runs, runs + nchains);
@@ -803,7 +857,8 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
     std::vector<ChainDev> cd(nchains);
     std::vector<Opnd> v_eK, v_eKinv, v_hK, v_hKinv, v_Pt, v_G0, v_G1, v_T, v_Gm, v_Tm, v_P0, v_P1, v_P2, v_M, v_X, v_S;
     std::vector<OpndF> v_G32a, v_G32b, v_T32, v_eK32, v_eKinv32, v_FZ32, v_FC32;
-    std::vector<Opnd> v_FZ, v_FC;
+    std::vector<Opnd> v_FZ, v_FC, v_DU, v_DV;
+    std::vector<OpndF> v_DU32, v_DV32;
     std::vector<BlockDesc> blk_all;
     std::vector<int> blk_offset, blk_count;
     std::vector<std::vector<Opnd>> v_UR(h->nn), v_ULH(h->nn);
@@ -884,6 +939,12 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
         ok &= dalloc(C, &C.W, 2 * 64 * 64) == hipSuccess;
         ok &= dalloc(C, &C.FZ, 2 * 64 * (size_t)n) == hipSuccess;
         ok &= dalloc(C, &C.FC, 2 * 64 * (size_t)n) == hipSuccess;
+        ok &= dalloc(C, &C.rk, (size_t)ns_all + 2) == hipSuccess;   // blocks run over the largest chain
+        ok &= dalloc(C, &C.rank, 1) == hipSuccess;
+        if (m.kind == KDQ_MODEL_TU && !h->mixed) {
+            ok &= dalloc(C, &C.DU, 2 * nn) == hipSuccess;
+            ok &= dalloc(C, &C.DV, 2 * nn) == hipSuccess;
+        }
         ok &= dalloc(C, &C.counters, 2) == hipSuccess;
         ok &= dalloc(C, &C.meas, KDQ_MEAS_SLOTS) == hipSuccess;
         ok &= dalloc(C, &C.logdet, 1) == hipSuccess;
@@ -895,6 +956,10 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
             ok &= dalloc(C, &C.eKinv32, nn) == hipSuccess;
             ok &= dalloc(C, &C.FZ32, 2 * 64 * (size_t)n) == hipSuccess;
             ok &= dalloc(C, &C.FC32, 2 * 64 * (size_t)n) == hipSuccess;
+            if (m.kind == KDQ_MODEL_TU) {
+                ok &= dalloc(C, &C.DU32, 2 * nn) == hipSuccess;
+                ok &= dalloc(C, &C.DV32, 2 * nn) == hipSuccess;
+            }
         }
         if (!ok) {
             int r = fail(h, KDQ_EHIP, "kdq_create: device allocation failed");
@@ -971,6 +1036,7 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
         d.chain_id = runs[c].chain;
         d.stream = streams ? streams[c] : 0;
         d.acc_cnt = C.acc_cnt; d.acc_idx = C.acc_idx; d.W = C.W;
+        d.rk = C.rk; d.rank = C.rank;
         d.counters = C.counters; d.meas = C.meas;
         d.kind = m.kind;
         d.n_elem = n_elem;
@@ -989,6 +1055,10 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
         v_M.push_back(cplx(C.M, np, pp)); v_X.push_back(cplx(C.X, np, nP));
         v_S.push_back(cplx(C.S, np, pp));
         v_FZ.push_back(Opnd{C.FZ, C.FZ + 64 * (size_t)n, 64, 0});
+        v_DU.push_back(Opnd{C.DU, C.DU ? C.DU + nn : nullptr, n, 0});
+        v_DV.push_back(Opnd{C.DV, C.DV ? C.DV + nn : nullptr, n, 0});
+        v_DU32.push_back(OpndF{C.DU32, C.DU32 ? C.DU32 + nn : nullptr, n, 0});
+        v_DV32.push_back(OpndF{C.DV32, C.DV32 ? C.DV32 + nn : nullptr, n, 0});
         v_FC.push_back(Opnd{C.FC, C.FC + 64 * (size_t)n, n, 0});
         if (h->mixed) {
             v_FZ32.push_back(OpndF{C.FZ32, C.FZ32 + 64 * (size_t)n, 64, 0});
@@ -1039,6 +1109,8 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
     }
     r |= make_table(h, &h->t_M, v_M); r |= make_table(h, &h->t_X, v_X); r |= make_table(h, &h->t_S, v_S);
     r |= make_table(h, &h->t_FZ, v_FZ); r |= make_table(h, &h->t_FC, v_FC);
+    r |= make_table(h, &h->t_DU, v_DU); r |= make_table(h, &h->t_DV, v_DV);
+    r |= make_table_f(h, &h->t_DU32, v_DU32); r |= make_table_f(h, &h->t_DV32, v_DV32);
     if (h->mixed) {
         r |= make_table_f(h, &h->t_G32[0], v_G32a); r |= make_table_f(h, &h->t_G32[1], v_G32b);
         r |= make_table_f(h, &h->t_T32, v_T32);
@@ -1243,6 +1315,10 @@ int kdq_set_option(kdq_handle h, int32_t key, int32_t value) {
         case KDQ_OPT_GEMM_TILE:
             if (value != 0 && value != 32 && value != 64) return fail(h, KDQ_EARG, "kdq_set_option: GEMM tile is 0, 32 or 64");
             h->gemm_tile = value;
+            break;
+        case KDQ_OPT_DEFER:
+            if (h->kind != KDQ_MODEL_TU) return fail(h, KDQ_EARG, "kdq_set_option: the deferred flush is for t-U chains");
+            h->defer = value != 0;
             break;
         default:
             return fail(h, KDQ_EARG, "kdq_set_option: unknown option");

@@ -29,7 +29,9 @@ template <class T> KDQ_DEV int acc_col(int lane, int v);
 template <> KDQ_DEV int acc_col<double>(int lane, int v) { return (lane >> 4) + 4 * v; }
 template <> KDQ_DEV int acc_col<float>(int lane, int v) { return 4 * (lane >> 4) + v; }
 
-template <class T, int AC, int BC, int OPA, int OPB, int BM, int BN>
+// PF: mode 2 with its Cin entries loaded before the k loop (all at once, overlapping the first
+// k-tile) instead of one by one between the epilogue's stores.
+template <class T, int AC, int BC, int OPA, int OPB, int BM, int BN, bool PF>
 __global__ __launch_bounds__(256) void k_gemm(GemmArgsT<T> g) {
     typedef typename Acc<T>::type VA;
     constexpr bool OC = AC || BC;
@@ -149,6 +151,23 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgsT<T> g) {
         }
     };
 
+    T pcr[PF ? WMB : 1][PF ? WNB : 1][4], pci[PF ? WMB : 1][PF ? WNB : 1][4];
+    if constexpr (PF) {
+        const auto Ci = g.Cin[c];
+#pragma unroll
+        for (int mi = 0; mi < WMB; ++mi)
+#pragma unroll
+            for (int ni = 0; ni < WNB; ++ni)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int i = i0 + wm * (BM / 2) + mi * 16 + (lane & 15);
+                    const int j = j0 + wn * (BN / 2) + ni * 16 + acc_col<T>(lane, v);
+                    const bool ok = i < M && j < N;
+                    const size_t oi = (size_t)i + (size_t)j * Ci.ld;
+                    pcr[mi][ni][v] = ok ? Ci.re[oi] : T(0);
+                    pci[mi][ni][v] = (ok && OC) ? Ci.im[oi] : T(0);
+                }
+    }
     fetch(0);
     __syncthreads();   // diagonal factors visible
     stash(0, 0);
@@ -221,9 +240,14 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgsT<T> g) {
                     cr = (i == j ? T(1) : T(0)) - cr;
                     ci = -ci;
                 } else if (g.mode == 2) {
-                    const size_t oi = (size_t)i + (size_t)j * Ci.ld;
-                    cr = Ci.re[oi] - cr;
-                    if (OC) ci = Ci.im[oi] - ci;
+                    if constexpr (PF) {
+                        cr = pcr[mi][ni][v] - cr;
+                        if (OC) ci = pci[mi][ni][v] - ci;
+                    } else {
+                        const size_t oi = (size_t)i + (size_t)j * Ci.ld;
+                        cr = Ci.re[oi] - cr;
+                        if (OC) ci = Ci.im[oi] - ci;
+                    }
                 } else if (g.mode == 3) {
                     cr = -cr;
                     ci = -ci;
@@ -234,17 +258,17 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgsT<T> g) {
     }
 }
 
-template <class T, int AC, int BC, int OPA, int OPB>
+template <class T, int AC, int BC, int OPA, int OPB, bool PF = false>
 hipError_t run(hipStream_t st, int nchains, int max_m, int max_n, int max_k, const GemmArgsT<T> &g, int tile) {
     if (g.inS.kind && max_k > KMAX) return hipErrorInvalidValue;
     // 64 x 64 tiles unless that leaves most compute units idle (or as forced by `tile`)
     const long t64 = (long)((max_n + 63) / 64) * ((max_m + 63) / 64) * nchains;
     if (tile == 64 || (tile == 0 && t64 >= 192)) {
         dim3 grid((max_n + 63) / 64, (max_m + 63) / 64, nchains);
-        hipLaunchKernelGGL((k_gemm<T, AC, BC, OPA, OPB, 64, 64>), grid, dim3(256), 0, st, g);
+        hipLaunchKernelGGL((k_gemm<T, AC, BC, OPA, OPB, 64, 64, PF>), grid, dim3(256), 0, st, g);
     } else {
         dim3 grid((max_n + 31) / 32, (max_m + 31) / 32, nchains);
-        hipLaunchKernelGGL((k_gemm<T, AC, BC, OPA, OPB, 32, 32>), grid, dim3(256), 0, st, g);
+        hipLaunchKernelGGL((k_gemm<T, AC, BC, OPA, OPB, 32, 32, PF>), grid, dim3(256), 0, st, g);
     }
     return hipGetLastError();
 }
@@ -261,6 +285,7 @@ hipError_t launch_gemm(hipStream_t st, int nchains, int max_m, int max_n, int ma
         case 1100: return run<double, 1, 1, 0, 0>(st, nchains, max_m, max_n, max_k, g, tile);    // U_R · X
         case 1120: return run<double, 1, 1, 2, 0>(st, nchains, max_m, max_n, max_k, g, tile);    // U_Lᴴᴴ · U_R, Uᴴ U
         case 1102: return run<double, 1, 1, 0, 2>(st, nchains, max_m, max_n, max_k, g, tile);
+        case 1101: return run<double, 1, 1, 0, 1, true>(st, nchains, max_m, max_n, max_k, g, tile);   // G0 - U Vᵀ (mode 2)
         case 0: return run<double, 0, 0, 0, 0>(st, nchains, max_m, max_n, max_k, g, tile);
         case 10: return run<double, 0, 0, 1, 0>(st, nchains, max_m, max_n, max_k, g, tile);
         default: return hipErrorInvalidValue;
@@ -273,6 +298,7 @@ hipError_t launch_gemm_f32(hipStream_t st, int nchains, int max_m, int max_n, in
     switch (code) {
         case 100: return run<float, 0, 1, 0, 0>(st, nchains, max_m, max_n, max_k, g, tile);    // eK · G
         case 1000: return run<float, 1, 0, 0, 0>(st, nchains, max_m, max_n, max_k, g, tile);   // T · eKinv
+        case 1101: return run<float, 1, 1, 0, 1, true>(st, nchains, max_m, max_n, max_k, g, tile);   // G0 - U Vᵀ (mode 2)
         default: return hipErrorInvalidValue;
     }
 }

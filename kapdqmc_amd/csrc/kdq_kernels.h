@@ -37,8 +37,26 @@ struct DecideArgs {
     int pad;
     const uint32_t *sweep_dev;   // when set, the sweep counter is read here (graph replays)
     const OpndF *Gf;             // when set, G[S,S] is read from this FP32 copy instead of G
+    // deferred flush: G is the slice's G0 and the current G is G0 - U V with the factors'
+    // rank rk[blk]; the block reads G0[S,S] - U[S,:] V[:,S] and records rk[blk + 1]
+    int deferred, blk;
+    const Opnd *U, *V;           // FP64 factors (N x N each; V stored transposed)
+    const OpndF *Uf, *Vf;        // FP32 factors (mixed precision)
 };
 hipError_t launch_decide_tu(hipStream_t st, int nchains, const DecideArgs &a);
+
+// Deferred flush, per block after its decisions: with r = rk[blk], A the accepted set and W
+// its factor, append X = G[:,A] (columns r.. of U) and Y = W (E_A - G[A,:]) (rows r.. of V),
+// where G = G0 - U[:, :r] V[:r, :] is the current matrix. V is stored transposed (N x rank,
+// V[t, j] at j + t·ld) so that its rows are contiguous. T = double or float.
+template <class T>
+struct DeferArgsT {
+    const ChainDev *chains;
+    const typename OpndOf<T>::type *G0, *U, *V;
+    int blk, kmax;
+};
+hipError_t launch_defer_xy(hipStream_t st, int nchains, int max_ns, const DeferArgsT<double> &a);
+hipError_t launch_defer_xy(hipStream_t st, int nchains, int max_ns, const DeferArgsT<float> &a);
 
 // Delayed-update flush with the block's accepted set A and factor W (per chain):
 //   mode 0: Xout = Xin - Cin[:,A] · W · (E_A - Rin[A,:])

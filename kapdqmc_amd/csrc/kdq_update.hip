@@ -41,7 +41,14 @@ __global__ __launch_bounds__(KBT >= 32 ? 1024 : KBT * KBT) void k_decide_tu(Deci
     const int kb = min(a.kb, n - s0);
     const int tid = threadIdx.x;
     if (kb <= 0) {
-        if (tid == 0) *ch.acc_cnt = 0;
+        if (tid == 0) {
+            *ch.acc_cnt = 0;
+            if (a.deferred) {
+                const int rk = a.blk > 0 ? ch.rk[a.blk] : 0;
+                ch.rk[a.blk + 1] = rk;
+                *ch.rank = rk;
+            }
+        }
         return;
     }
     __shared__ double colr[KBT], coli[KBT], rowr[KBT], rowi[KBT];   // published column / row
@@ -70,6 +77,66 @@ __global__ __launch_bounds__(KBT >= 32 ? 1024 : KBT * KBT) void k_decide_tu(Deci
         }
         Wr[q] = 0.0;
         Wi[q] = 0.0;
+    }
+    // deferred flush: subtract U[S, :r] V[:r, S] (chunks of DCH ranks staged in LDS)
+    if (a.deferred && a.blk > 0) {
+        constexpr int DCH = 16;
+        __shared__ double Usr[KBT][DCH + 1], Usi[KBT][DCH + 1], Vsr[DCH][KBT + 1], Vsi[DCH][KBT + 1];
+        const int rk = ch.rk[a.blk];
+        for (int t0 = 0; t0 < rk; t0 += DCH) {
+            const int tc = min(DCH, rk - t0);
+            for (int e = tid; e < KBT * DCH; e += NTH) {
+                const int i = e % KBT, t = e / KBT;
+                double ur = 0.0, ui = 0.0;
+                if (i < kb && t < tc) {
+                    if (a.Uf) {
+                        const OpndF U = a.Uf[c];
+                        const size_t o = (size_t)(s0 + i) + (size_t)(t0 + t) * U.ld;
+                        ur = U.re[o];
+                        ui = U.im[o];
+                    } else {
+                        const Opnd U = a.U[c];
+                        const size_t o = (size_t)(s0 + i) + (size_t)(t0 + t) * U.ld;
+                        ur = U.re[o];
+                        ui = U.im[o];
+                    }
+                }
+                Usr[i][t] = ur;
+                Usi[i][t] = ui;
+            }
+            for (int e = tid; e < DCH * KBT; e += NTH) {
+                const int j = e % KBT, t = e / KBT;
+                double vr = 0.0, vi = 0.0;
+                if (j < kb && t < tc) {   // V is stored transposed: V[t, j] at j + t·ld
+                    if (a.Vf) {
+                        const OpndF V = a.Vf[c];
+                        const size_t o = (size_t)(s0 + j) + (size_t)(t0 + t) * V.ld;
+                        vr = V.re[o];
+                        vi = V.im[o];
+                    } else {
+                        const Opnd V = a.V[c];
+                        const size_t o = (size_t)(s0 + j) + (size_t)(t0 + t) * V.ld;
+                        vr = V.re[o];
+                        vi = V.im[o];
+                    }
+                }
+                Vsr[t][j] = vr;
+                Vsi[t][j] = vi;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < CPT; ++q) {
+                const int cc = c0 + q;
+                double sr = 0.0, si = 0.0;
+                for (int t = 0; t < tc; ++t) {
+                    sr += Usr[r][t] * Vsr[t][cc] - Usi[r][t] * Vsi[t][cc];
+                    si += Usr[r][t] * Vsi[t][cc] + Usi[r][t] * Vsr[t][cc];
+                }
+                wr[q] -= sr;
+                wi[q] -= si;
+            }
+            __syncthreads();
+        }
     }
     const uint32_t tag = (uint32_t)lt | ((uint32_t)a.dir << 31);
     if (tid < KBT) {
@@ -190,6 +257,11 @@ __global__ __launch_bounds__(KBT >= 32 ? 1024 : KBT * KBT) void k_decide_tu(Deci
         *ch.acc_cnt = na;
         ch.counters[0] += kb;
         ch.counters[1] += na;
+        if (a.deferred) {
+            const int rk = (a.blk > 0 ? ch.rk[a.blk] : 0) + na;
+            ch.rk[a.blk + 1] = rk;
+            *ch.rank = rk;
+        }
     }
     if (tid < na) ch.acc_idx[tid] = s0 + accl[tid];
     if (r < na) {
@@ -271,82 +343,274 @@ __global__ __launch_bounds__(256) void k_flush_prep(FlushArgsT<T> f) {
     }
 }
 
-// Flush, second launch: Xout = Xin - Cs · Zs of rank |A|. Tile 32 rows x 64 columns; the
-// thread's Xin entries (row ii, columns 8 jg .. 8 jg + 7) are loaded first, together with the
-// first chunk of Cs[I, t] and Zs[t, J] (16 ranks per chunk, staged in LDS); the rank loop then
-// reads Cs rows (consecutive across the wave) and Zs columns (one address per half-wave) from LDS.
-constexpr int AM = 32, AN = 64, AQ = 8, AK = 16;
+// Flush, second launch: Xout = Xin - Cs · Zs of rank |A|. Tile 32 rows x 64 columns; thread
+// (ri = tid % 16, cg = tid / 16) owns rows ri and ri + 16 and columns cg + 16 b (b < 4), so one
+// load instruction covers 16 consecutive rows of four columns. Its Xin entries are loaded
+// first; Cs[I, t] and Zs[t, J] (16 ranks per chunk) are staged in LDS as interleaved complex
+// pairs, so each rank step reads 2 + 4 complex values (16-byte LDS reads) for 8 complex MACs.
+// Workgroups are dealt to the 8 XCDs round-robin by linear id; with xcd_map set (batches of at
+// least 8 chains) every tile of chain c runs on XCD c % 8, so each chain's Cs / Zs operands are
+// fetched into one L2 only.
+constexpr int AM = 32, AN = 64, AK = 16;
+constexpr int NXCD = 8;
+template <class T> struct Cplx2;
+template <> struct Cplx2<double> { typedef double2 type; };
+template <> struct Cplx2<float> { typedef float2 type; };
 template <class T>
-__global__ __launch_bounds__(256) void k_flush_apply(FlushArgsT<T> f) {
-    const int c = blockIdx.z;
+__global__ __launch_bounds__(256) void k_flush_apply(FlushArgsT<T> f, int nch, int tx, int ty, int xcd_map) {
+    typedef typename Cplx2<T>::type C2;
+    int c, bx, by;
+    if (xcd_map) {
+        const int L = blockIdx.x, x = L % NXCD, k = L / NXCD, tiles = tx * ty;
+        c = x + NXCD * (k / tiles);
+        bx = (k % tiles) % tx;
+        by = (k % tiles) / tx;
+        if (c >= nch) return;
+    } else {
+        c = blockIdx.z;
+        bx = blockIdx.x;
+        by = blockIdx.y;
+    }
     const ChainDev &ch = f.chains[c];
     const int n = ch.ns;
-    const int i0 = blockIdx.y * AM, j0 = blockIdx.x * AN;
+    const int i0 = by * AM, j0 = bx * AN;
     if (i0 >= n || j0 >= n) return;
-    const int tid = threadIdx.x, ii = tid & (AM - 1), jg = tid >> 5;
-    const int i = i0 + ii;
+    const int tid = threadIdx.x, ri = tid & 15, cg = tid >> 4;
     const auto Gi = f.Gin[c], Go = f.Gout[c], Z = f.Zs[c], Cg = f.Cs[c];
-    __shared__ T Csr[AK][AM], Csi[AK][AM];
-    __shared__ T Zsr[AK][AN + 1], Zsi[AK][AN + 1];
-    T xr[AQ], xi[AQ], ar[AQ], ai[AQ];
+    __shared__ C2 Cs2[AK][AM];
+    __shared__ C2 Zs2[AK][AN];
+    T xr[2][4], xi[2][4], ar[2][4], ai[2][4];
 #pragma unroll
-    for (int q = 0; q < AQ; ++q) {
-        const int j = j0 + jg * AQ + q;
-        const bool ok = i < n && j < n;
-        const size_t oi = (size_t)i + (size_t)j * Gi.ld;
-        xr[q] = ok ? Gi.re[oi] : T(0);
-        xi[q] = ok ? Gi.im[oi] : T(0);
-        ar[q] = T(0);
-        ai[q] = T(0);
-    }
-    const int na = *ch.acc_cnt;
-    for (int t0 = 0; t0 < na; t0 += AK) {
-        const int tk = min(AK, na - t0);
-        // stage: Cs[I, t0 + t] (2 x 16 x 32) and Zs[t0 + t, J] (2 x 16 x 64)
-        for (int e = tid; e < AK * AM; e += 256) {
-            const int r = e & (AM - 1), t = e / AM;
-            T vr = 0, vi = 0;
-            if (t < tk && i0 + r < n) {
-                const size_t oc = (size_t)(i0 + r) + (size_t)(t0 + t) * Cg.ld;
-                vr = Cg.re[oc];
-                vi = Cg.im[oc];
-            }
-            Csr[t][r] = vr;
-            Csi[t][r] = vi;
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int i = i0 + ri + 16 * a, j = j0 + cg + 16 * b;
+            const bool ok = i < n && j < n;
+            const size_t oi = (size_t)i + (size_t)j * Gi.ld;
+            xr[a][b] = ok ? Gi.re[oi] : T(0);
+            xi[a][b] = ok ? Gi.im[oi] : T(0);
+            ar[a][b] = T(0);
+            ai[a][b] = T(0);
         }
-        for (int e = tid; e < AK * AN; e += 256) {
-            const int t = e % AK, jj = e / AK;
-            T vr = 0, vi = 0;
-            if (t < tk && j0 + jj < n) {
-                const size_t oz = (size_t)(t0 + t) + (size_t)(j0 + jj) * Z.ld;
-                vr = Z.re[oz];
-                vi = Z.im[oz];
-            }
-            Zsr[t][jj] = vr;
-            Zsi[t][jj] = vi;
+    const int na = *ch.acc_cnt;
+    for (int t0 = 0; t0 < na || t0 == 0; t0 += AK) {
+        // the loads do not wait for the count: ranks t0 .. t0 + 15 are inside the 64-rank
+        // scratch whatever it is, and entries at or beyond the count are zeroed by a select
+        C2 cst[(AK * AM) / 256], zst[(AK * AN) / 256];
+#pragma unroll
+        for (int q = 0; q < (AK * AM) / 256; ++q) {
+            const int e = tid + 256 * q, r = e & (AM - 1), t = e / AM;
+            const int ir = min(i0 + r, n - 1);
+            const size_t oc = (size_t)ir + (size_t)(t0 + t) * Cg.ld;
+            cst[q].x = Cg.re[oc];
+            cst[q].y = Cg.im[oc];
+        }
+#pragma unroll
+        for (int q = 0; q < (AK * AN) / 256; ++q) {
+            const int e = tid + 256 * q, t = e % AK, jj = e / AK;
+            const int jc = min(j0 + jj, n - 1);
+            const size_t oz = (size_t)(t0 + t) + (size_t)jc * Z.ld;
+            zst[q].x = Z.re[oz];
+            zst[q].y = Z.im[oz];
+        }
+        const int tk = min(AK, na - t0);
+#pragma unroll
+        for (int q = 0; q < (AK * AM) / 256; ++q) {
+            const int e = tid + 256 * q, r = e & (AM - 1), t = e / AM;
+            const bool ok = t < tk && i0 + r < n;
+            C2 v;
+            v.x = ok ? cst[q].x : T(0);
+            v.y = ok ? cst[q].y : T(0);
+            Cs2[t][r] = v;
+        }
+#pragma unroll
+        for (int q = 0; q < (AK * AN) / 256; ++q) {
+            const int e = tid + 256 * q, t = e % AK, jj = e / AK;
+            const bool ok = t < tk && j0 + jj < n;
+            C2 v;
+            v.x = ok ? zst[q].x : T(0);
+            v.y = ok ? zst[q].y : T(0);
+            Zs2[t][jj] = v;
         }
         __syncthreads();
         for (int t = 0; t < tk; ++t) {
-            const T cr = Csr[t][ii], ci = Csi[t][ii];
+            C2 cv[2], zv[4];
 #pragma unroll
-            for (int q = 0; q < AQ; ++q) {
-                const T zr = Zsr[t][jg * AQ + q], zi = Zsi[t][jg * AQ + q];
-                ar[q] = fma(cr, zr, fma(-ci, zi, ar[q]));
-                ai[q] = fma(cr, zi, fma(ci, zr, ai[q]));
-            }
+            for (int a = 0; a < 2; ++a) cv[a] = Cs2[t][ri + 16 * a];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) zv[b] = Zs2[t][cg + 16 * b];
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    ar[a][b] = fma(cv[a].x, zv[b].x, fma(-cv[a].y, zv[b].y, ar[a][b]));
+                    ai[a][b] = fma(cv[a].x, zv[b].y, fma(cv[a].y, zv[b].x, ai[a][b]));
+                }
         }
         __syncthreads();
     }
-    if (i >= n) return;
 #pragma unroll
-    for (int q = 0; q < AQ; ++q) {
-        const int j = j0 + jg * AQ + q;
-        if (j < n) {
-            const size_t oo = (size_t)i + (size_t)j * Go.ld;
-            Go.re[oo] = xr[q] - ar[q];
-            Go.im[oo] = xi[q] - ai[q];
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int i = i0 + ri + 16 * a, j = j0 + cg + 16 * b;
+            if (i < n && j < n) {
+                const size_t oo = (size_t)i + (size_t)j * Go.ld;
+                Go.re[oo] = xr[a][b] - ar[a][b];
+                Go.im[oo] = xi[a][b] - ai[a][b];
+            }
         }
+}
+
+// Deferred flush, per block: chunk of XR rows (for X) and XR columns (for T = G[A, J]) of one
+// chain per workgroup; the rank sum over the factors runs in chunks of XCH staged in LDS. After
+// it, T and W share the staging memory and Y = W (E_A - T) is formed for the same columns.
+constexpr int XR = 32, XCH = 16;
+template <class T, int KBT>
+size_t defer_lds_elems() {
+    const size_t stage = 2 * ((size_t)XR * (XCH + 1) + XCH * (KBT + 1) + KBT * (XCH + 1) + XCH * (XR + 1));
+    const size_t post = 2 * ((size_t)KBT * (XR + 1) + KBT * (KBT + 1));
+    return stage > post ? stage : post;
+}
+template <class T, int KBT>
+__global__ __launch_bounds__(256) void k_defer_xy(DeferArgsT<T> a) {
+    constexpr int NQ = XR * KBT / 256;   // X (and T) entries per thread
+    const int c = blockIdx.y;
+    const ChainDev &ch = a.chains[c];
+    const int n = ch.ns, i0 = blockIdx.x * XR;
+    if (i0 >= n) return;
+    const int na = *ch.acc_cnt;
+    if (na == 0) return;
+    const int rk = a.blk > 0 ? ch.rk[a.blk] : 0;
+    const auto G = a.G0[c], U = a.U[c], V = a.V[c];
+    extern __shared__ __align__(8) unsigned char lds_raw[];
+    T *sm = reinterpret_cast<T *>(lds_raw);
+    T *Uir = sm, *Uii = Uir + XR * (XCH + 1);                   // U[I, t]      [XR][XCH+1]
+    T *VAr = Uii + XR * (XCH + 1), *VAi = VAr + XCH * (KBT + 1);  // V[t, A]      [XCH][KBT+1]
+    T *UAr = VAi + XCH * (KBT + 1), *UAi = UAr + KBT * (XCH + 1); // U[A, t]      [KBT][XCH+1]
+    T *Vjr = UAi + KBT * (XCH + 1), *Vji = Vjr + XCH * (XR + 1);  // V[t, J]      [XCH][XR+1]
+    T *Tsr = sm, *Tsi = Tsr + KBT * (XR + 1);                     // T[A, J]      [KBT][XR+1]
+    T *Wsr = Tsi + KBT * (XR + 1), *Wsi = Wsr + KBT * (KBT + 1);  // W            [KBT][KBT+1]
+    __shared__ int aidx[KBT];
+    const int tid = threadIdx.x;
+    if (tid < na) aidx[tid] = ch.acc_idx[tid];
+    __syncthreads();
+    T xr[NQ], xi[NQ], tr[NQ], ti[NQ];   // X[i0 + ii][aa] and T[aa][i0 + ii] of entry e = tid + 256 q
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int e = tid + 256 * q, ii = e % XR, aa = e / XR, i = i0 + ii;
+        const bool ok = aa < na && i < n;
+        const int ac = ok ? aidx[aa] : 0;
+        xr[q] = ok ? G.re[(size_t)i + (size_t)ac * G.ld] : T(0);
+        xi[q] = ok ? G.im[(size_t)i + (size_t)ac * G.ld] : T(0);
+        tr[q] = ok ? G.re[(size_t)ac + (size_t)i * G.ld] : T(0);
+        ti[q] = ok ? G.im[(size_t)ac + (size_t)i * G.ld] : T(0);
     }
+    for (int t0 = 0; t0 < rk; t0 += XCH) {
+        const int tc = min(XCH, rk - t0);
+        for (int e = tid; e < XR * XCH; e += 256) {
+            const int ii = e % XR, t = e / XR;
+            const bool ok = i0 + ii < n && t < tc;
+            const size_t o = (size_t)(i0 + ii) + (size_t)(t0 + t) * U.ld;
+            Uir[ii * (XCH + 1) + t] = ok ? U.re[o] : T(0);
+            Uii[ii * (XCH + 1) + t] = ok ? U.im[o] : T(0);
+        }
+        for (int e = tid; e < XCH * KBT; e += 256) {
+            const int aa = e % KBT, t = e / KBT;
+            const bool ok = aa < na && t < tc;
+            const size_t o = (size_t)(ok ? aidx[aa] : 0) + (size_t)(t0 + t) * V.ld;
+            VAr[t * (KBT + 1) + aa] = ok ? V.re[o] : T(0);
+            VAi[t * (KBT + 1) + aa] = ok ? V.im[o] : T(0);
+        }
+        for (int e = tid; e < KBT * XCH; e += 256) {
+            const int aa = e % KBT, t = e / KBT;
+            const bool ok = aa < na && t < tc;
+            const size_t o = (size_t)(ok ? aidx[aa] : 0) + (size_t)(t0 + t) * U.ld;
+            UAr[aa * (XCH + 1) + t] = ok ? U.re[o] : T(0);
+            UAi[aa * (XCH + 1) + t] = ok ? U.im[o] : T(0);
+        }
+        for (int e = tid; e < XCH * XR; e += 256) {
+            const int jj = e % XR, t = e / XR;
+            const bool ok = i0 + jj < n && t < tc;
+            const size_t o = (size_t)(i0 + jj) + (size_t)(t0 + t) * V.ld;
+            Vjr[t * (XR + 1) + jj] = ok ? V.re[o] : T(0);
+            Vji[t * (XR + 1) + jj] = ok ? V.im[o] : T(0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int e = tid + 256 * q, ii = e % XR, aa = e / XR;
+            T sxr = 0, sxi = 0, str = 0, sti = 0;
+            for (int t = 0; t < tc; ++t) {
+                const T ur = Uir[ii * (XCH + 1) + t], ui = Uii[ii * (XCH + 1) + t];
+                const T vr = VAr[t * (KBT + 1) + aa], vi = VAi[t * (KBT + 1) + aa];
+                sxr += ur * vr - ui * vi;
+                sxi += ur * vi + ui * vr;
+                const T pr = UAr[aa * (XCH + 1) + t], pi = UAi[aa * (XCH + 1) + t];
+                const T wr = Vjr[t * (XR + 1) + ii], wi = Vji[t * (XR + 1) + ii];
+                str += pr * wr - pi * wi;
+                sti += pr * wi + pi * wr;
+            }
+            xr[q] -= sxr;
+            xi[q] -= sxi;
+            tr[q] -= str;
+            ti[q] -= sti;
+        }
+        __syncthreads();
+    }
+    // X -> columns rk + aa of U; T and W -> LDS
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int e = tid + 256 * q, ii = e % XR, aa = e / XR, i = i0 + ii;
+        if (aa < na && i < n) {
+            const size_t o = (size_t)i + (size_t)(rk + aa) * U.ld;
+            U.re[o] = xr[q];
+            U.im[o] = xi[q];
+        }
+        Tsr[aa * (XR + 1) + ii] = tr[q];
+        Tsi[aa * (XR + 1) + ii] = ti[q];
+    }
+    for (int e = tid; e < na * na; e += 256) {
+        const int u = e % na, v = e / na;
+        Wsr[u * (KBT + 1) + v] = (T)ch.W[u + v * KB];
+        Wsi[u * (KBT + 1) + v] = (T)ch.W[KB * KB + u + v * KB];
+    }
+    __syncthreads();
+    // Y[aa, j] = Σ_b W[aa][b] (δ(A_b, j) - T[b, j]) -> rows rk + aa of V
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int e = tid + 256 * q, jj = e % XR, aa = e / XR, j = i0 + jj;
+        if (aa >= na || j >= n) continue;
+        T sr = 0, si = 0;
+        for (int b = 0; b < na; ++b) {
+            const T wr = Wsr[aa * (KBT + 1) + b], wi = Wsi[aa * (KBT + 1) + b];
+            const T er = (aidx[b] == j ? T(1) : T(0)) - Tsr[b * (XR + 1) + jj], ei = -Tsi[b * (XR + 1) + jj];
+            sr += wr * er - wi * ei;
+            si += wr * ei + wi * er;
+        }
+        const size_t o = (size_t)j + (size_t)(rk + aa) * V.ld;
+        V.re[o] = sr;
+        V.im[o] = si;
+    }
+}
+
+template <class T, int KBT>
+hipError_t defer_launch(hipStream_t st, int nchains, int max_ns, const DeferArgsT<T> &a) {
+    const size_t bytes = defer_lds_elems<T, KBT>() * sizeof(T);
+    if (hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_defer_xy<T, KBT>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes))
+        return e;
+    hipLaunchKernelGGL((k_defer_xy<T, KBT>), dim3((max_ns + XR - 1) / XR, nchains), dim3(256), bytes, st, a);
+    return hipGetLastError();
+}
+
+template <class T>
+hipError_t defer_t(hipStream_t st, int nchains, int max_ns, const DeferArgsT<T> &a) {
+    if (a.kmax < 1 || a.kmax > KB) return hipErrorInvalidValue;
+    if (a.kmax <= 8) return defer_launch<T, 8>(st, nchains, max_ns, a);
+    if (a.kmax <= 16) return defer_launch<T, 16>(st, nchains, max_ns, a);
+    if (a.kmax <= 32) return defer_launch<T, 32>(st, nchains, max_ns, a);
+    return defer_launch<T, 64>(st, nchains, max_ns, a);
 }
 
 }  // namespace
@@ -370,8 +634,13 @@ hipError_t flush_t(hipStream_t st, int nchains, int max_ns, const FlushArgsT<T> 
         return e;
     hipLaunchKernelGGL(k_flush_prep<T>, dim3((max_ns + PC - 1) / PC, nchains), dim3(256), bytes, st, f);
     if (hipError_t e = hipGetLastError()) return e;
-    hipLaunchKernelGGL(k_flush_apply<T>, dim3((max_ns + AN - 1) / AN, (max_ns + AM - 1) / AM, nchains), dim3(256), 0,
-                       st, f);
+    const int tx = (max_ns + AN - 1) / AN, ty = (max_ns + AM - 1) / AM;
+    if (nchains >= NXCD) {
+        const int groups = (nchains + NXCD - 1) / NXCD;
+        hipLaunchKernelGGL(k_flush_apply<T>, dim3(NXCD * groups * tx * ty), dim3(256), 0, st, f, nchains, tx, ty, 1);
+    } else {
+        hipLaunchKernelGGL(k_flush_apply<T>, dim3(tx, ty, nchains), dim3(256), 0, st, f, nchains, tx, ty, 0);
+    }
     return hipGetLastError();
 }
 
@@ -381,4 +650,12 @@ hipError_t launch_flush(hipStream_t st, int nchains, int max_ns, const FlushArgs
 
 hipError_t launch_flush_f32(hipStream_t st, int nchains, int max_ns, const FlushArgsF &f) {
     return flush_t<float>(st, nchains, max_ns, f);
+}
+
+hipError_t launch_defer_xy(hipStream_t st, int nchains, int max_ns, const DeferArgsT<double> &a) {
+    return defer_t<double>(st, nchains, max_ns, a);
+}
+
+hipError_t launch_defer_xy(hipStream_t st, int nchains, int max_ns, const DeferArgsT<float> &a) {
+    return defer_t<float>(st, nchains, max_ns, a);
 }
