@@ -102,7 +102,7 @@ def cpu_baseline(m, tau, nslices: int):
 
 
 def run_batch(m, nch: int, seed0: int, delay, warmup: int, steps: int, profile_all: bool,
-              precision: str = "fp64"):
+              precision: str = "fp64", graph: bool = True):
     """W warm-up sweeps (the first with every kernel family timed by HIP events, host
     launches, to find the dominant family; the graph of a sweep is captured untimed after
     them), `steps` timed sweeps replayed from the graph, then one sweep with host launches
@@ -114,6 +114,8 @@ def run_batch(m, nch: int, seed0: int, delay, warmup: int, steps: int, profile_a
     N = m.Ns
     b = DQMCBatch([m] * nch, [seed0 + c for c in range(nch)], list(range(nch)), delay=delay or 32,
                   precision=precision)
+    if not graph:
+        b.set_option("graph", 0)
     for c in range(nch):
         b.set_fields(c, initial_fields(seed0 + c, c, N, m.Nt))
     tuned = None
@@ -134,7 +136,14 @@ def run_batch(m, nch: int, seed0: int, delay, warmup: int, steps: int, profile_a
     b.phy_update(0, True)   # captures the sweep graph (untimed)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    rows = b.phy_update(steps, True)          # [nch, steps, nobs]
+    if graph:
+        rows = b.phy_update(steps, True)          # [nch, steps, nobs]
+    else:   # host launches, one progress line per sweep
+        rows = []
+        for k in range(steps):
+            rows.append(b.phy_update(1, True))
+            print(f"bench: {nch} chain(s), timed sweep {k + 1} done", file=sys.stderr, flush=True)
+        rows = np.concatenate(rows, axis=1)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     print(f"bench: {nch} chain(s), {steps} timed sweeps in {dt:.2f} s", file=sys.stderr, flush=True)
@@ -163,6 +172,11 @@ def main():
                     help="sweeps of the mixed-precision line on the same batch (0: skip)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-graph", action="store_true", help="timed sweeps with host launches")
+    ap.add_argument("--counter-sample", type=int, default=0, metavar="SLICES",
+                    help="for counter runs (which serialise every dispatch): the same batch from its "
+                         "initial fields, chain set-up, then SLICES forward slices (wrap + local "
+                         "updates) with host launches; no JSON line")
     ap.add_argument("--cpu-slices", type=int, default=20)
     ap.add_argument("--profile-all", action="store_true", help="time every kernel family (slower)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -192,8 +206,25 @@ def main():
 
     m = model_cfg2()
     N, Np = m.Ns, m.Ns // 2
+    if args.counter_sample:
+        from kapdqmc_amd.dqmc import DQMCBatch
+        from kapdqmc_amd.rng import initial_fields
+        C = args.chains
+        b = DQMCBatch([m] * C, [args.seed + c for c in range(C)], list(range(C)), delay=args.delay or 32)
+        b.set_option("graph", 0)
+        for c in range(C):
+            b.set_fields(c, initial_fields(args.seed + c, c, N, m.Nt))
+        b.init_chain()
+        print(f"bench: {C} chain(s), chain set-up done", file=sys.stderr, flush=True)
+        for lt in range(1, args.counter_sample + 1):
+            b.wrap(lt, 1)
+            b.update_layer(lt, 1, 1)
+        torch.cuda.synchronize()
+        print(f"bench: {C} chain(s), {args.counter_sample} slices (delay {b.delay}) done", file=sys.stderr, flush=True)
+        return
     C = args.chains
-    bat = run_batch(m, C, args.seed, args.delay, args.warmup, args.steps, args.profile_all)
+    bat = run_batch(m, C, args.seed, args.delay, args.warmup, args.steps, args.profile_all,
+                    graph=not args.no_graph)
     D = bat["rows"][:, :, 1] / N                     # [C, steps]
     tau = stats.tau_int_chains(D)
     sweeps_s = C * args.steps / bat["dt"]            # chain-sweeps per second
