@@ -108,6 +108,17 @@ def _declare(L):
     L.orc_scee_get_G4.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp]
     L.orc_scee_detg.argtypes = [C.c_void_p, C.c_int]
     L.orc_scee_detg.restype = C.c_double
+    L.orc_scee2_new.argtypes = [C.c_int, C.c_void_p, C.c_int, _ip, C.c_int, _ip, C.c_double, C.c_int,
+                                C.POINTER(C.c_uint8), C.POINTER(C.c_uint8)]
+    L.orc_scee2_new.restype = C.c_void_p
+    L.orc_scee2_free.argtypes = [C.c_void_p]
+    L.orc_scee2_init.argtypes = [C.c_void_p]
+    L.orc_scee2_sweeps.argtypes = [C.c_void_p, C.c_int, _dp]
+    L.orc_scee2_forward_slice.argtypes = [C.c_void_p, C.c_int, C.c_uint32]
+    L.orc_scee2_get_fields.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_uint8)]
+    L.orc_scee2_get_G4.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp]
+    L.orc_scee2_detg.argtypes = [C.c_void_p, C.c_int]
+    L.orc_scee2_detg.restype = C.c_double
 
 
 def _p(a, t=C.c_double):
@@ -546,3 +557,55 @@ class OracleSCEE:
 
     def detg(self, which: int):
         return lib().orc_scee_detg(self.h, which)
+
+
+class OracleSCEE2:
+    """ctrl_SCEEicr of the t-V (src/tV/SCEE.jl) or t-U-V (src/tUV/SCEE.jl) model on two replica
+    fields in the model's layout; indexA / indexB are 1-based site lists."""
+
+    def __init__(self, model, indexA, indexB, lam: float, nlam: int, s1, s2, seed: int = 0, chain: int = 0):
+        kind = getattr(model, "kind", "tU")
+        if kind == "tV":
+            self.m = OracleTV(model, seed=seed, chain=chain)
+            code, shape = 1, (model.Ns // 2, model.nnidx.shape[1], model.Nt)
+        elif kind == "tUV":
+            self.m = OracleTUV(model, seed=seed, chain=chain)
+            code, shape = 2, (model.Ns * model.nnidx.shape[1], model.Nt)
+        else:
+            raise ValueError("OracleSCEE2 is for t-V / t-U-V models")
+        self.ns, self.nt, self.shape = model.Ns, model.Nt, shape
+        self._keep = [np.ascontiguousarray(indexA, dtype=np.int32), np.ascontiguousarray(indexB, dtype=np.int32),
+                      np.asfortranarray(s1, dtype=np.uint8), np.asfortranarray(s2, dtype=np.uint8)]
+        iA, iB, a1, a2 = self._keep
+        self.h = lib().orc_scee2_new(code, self.m.h, len(iA), _p(iA, C.c_int), len(iB), _p(iB, C.c_int), lam, nlam,
+                                     _p(a1, C.c_uint8), _p(a2, C.c_uint8))
+
+    def __del__(self):
+        try:
+            lib().orc_scee2_free(self.h)
+        except Exception:
+            pass
+
+    def init(self):
+        lib().orc_scee2_init(self.h)
+
+    def sweeps(self, n: int):
+        O = np.zeros(n)
+        lib().orc_scee2_sweeps(self.h, n, _p(O))
+        return O
+
+    def forward_slice(self, lt: int, sweep: int = 0):
+        lib().orc_scee2_forward_slice(self.h, lt, sweep)
+
+    def fields(self, r: int):
+        out = np.zeros(self.shape, dtype=np.uint8, order="F")
+        lib().orc_scee2_get_fields(self.h, r, _p(out, C.c_uint8))
+        return out
+
+    def G4(self, r: int):
+        mats = [np.zeros((self.ns, self.ns), dtype=np.complex128, order="F") for _ in range(4)]
+        lib().orc_scee2_get_G4(self.h, r, *[_p(x) for x in mats])
+        return mats   # Gt, G0, Gt0, G0t
+
+    def detg(self, which: int):
+        return lib().orc_scee2_detg(self.h, which)

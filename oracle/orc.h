@@ -167,4 +167,19 @@ void orc_scee_get_fields(orc_scee *e, int r, uint8_t *s);
 void orc_scee_get_G4(orc_scee *e, int r, zdouble *Gt, zdouble *G0, zdouble *Gt0, zdouble *G0t);
 double orc_scee_detg(orc_scee *e, int which);
 
+/* ---------------- t-V / t-U-V swap-replica Rényi-2 (src/tV/SCEE.jl, src/tUV/SCEE.jl) ---
+ * kind 1: t-V (model = orc_tv *), kind 2: t-U-V (model = orc_tuv *); fields in the model's
+ * layout ([nb, 3, nt] or [nbd, nt]). Parity: G4 and the Grover determinants against
+ * brute-force products; every decision of a forward slice replayed with the full weight. */
+typedef struct orc_scee2 orc_scee2;
+orc_scee2 *orc_scee2_new(int kind, void *model, int nA, const int *indexA, int nB, const int *indexB,
+                         double lambda, int nlam, const uint8_t *s1, const uint8_t *s2);
+void orc_scee2_free(orc_scee2 *e);
+void orc_scee2_init(orc_scee2 *e);
+void orc_scee2_sweeps(orc_scee2 *e, int sweeps, double *O);
+void orc_scee2_forward_slice(orc_scee2 *e, int lt, uint32_t sweep);
+void orc_scee2_get_fields(orc_scee2 *e, int r, uint8_t *s);
+void orc_scee2_get_G4(orc_scee2 *e, int r, zdouble *Gt, zdouble *G0, zdouble *Gt0, zdouble *G0t);
+double orc_scee2_detg(orc_scee2 *e, int which);
+
 #endif
