@@ -1452,6 +1452,7 @@ struct SceeState {
     Opnd *t_M[2][2] = {}, *t_C[2] = {}, *t_Inv[2] = {}, *t_IX[2] = {};   // region tables (nch)
     SceeRegion *d_reg = nullptr;
     double *d_ld = nullptr;        // [2][nch] region log|det|
+    double *d_ph = nullptr;        // [2][nch][2] region det phase (t-V keeps the signed det)
     int *d_info = nullptr, *d_piv = nullptr;
     double *h_osum = nullptr;      // pinned [nch][2]
     std::vector<SceeRegion> reg;   // host copy (device pointers)
@@ -1482,6 +1483,25 @@ int scee_blocks(kdq_handle h, int idx) {
     SceeState *S = h->scee;
     const int lo = (int)h->nodes[idx - 1] + 1, hi = (int)h->nodes[idx], nb = hi - lo + 1;
     Opnd *BM = S->t_BM[idx - 1], *BMi = S->t_BMi[idx - 1];
+    if (h->kind == KDQ_MODEL_TV) {
+        // src/tV/GreenMatrix.jl:12-67: per slice BM <- Π_{j=3..1}(UV_j D_j UV_j) eK BM and
+        // BMinv <- BMinv eKinv Π_{j=3..1}(UV_j D_j⁻¹ UV_j)
+        HIPCHK(launch_diag(h->stream, h->nch, h->max_ns, h->d_chains, BM, DIM_NS, 0.0, 1));
+        HIPCHK(launch_diag(h->stream, h->nch, h->max_ns, h->d_chains, BMi, DIM_NS, 0.0, 1));
+        for (int lt = lo; lt <= hi; ++lt) {
+            GemmArgs g = gargs(h, h->t_eK, BM, h->t_T, DIM_NS, DIM_NS, DIM_NS);
+            if (int r = gemm(h, false, true, 0, 0, g, KDQ_PROF_CHAIN)) return r;
+            HIPCHK(launch_copy_mat(h->stream, h->nch, h->max_ns, h->max_ns, h->d_chains, h->t_T, BM, DIM_NS, DIM_NS, true));
+            GemmArgs gi = gargs(h, BMi, h->t_eKinv, h->t_Tm, DIM_NS, DIM_NS, DIM_NS);
+            if (int r = gemm(h, true, false, 0, 0, gi, KDQ_PROF_CHAIN)) return r;
+            HIPCHK(launch_copy_mat(h->stream, h->nch, h->max_ns, h->max_ns, h->d_chains, h->t_Tm, BMi, DIM_NS, DIM_NS, true));
+            for (int j = 2; j >= 0; --j) {
+                if (int r = bondrot(h, BM, j, lt, +1, 1, DIM_NS, KDQ_PROF_CHAIN)) return r;
+                if (int r = bondrot(h, BMi, j, lt, +1, 2, DIM_NS, KDQ_PROF_CHAIN)) return r;
+            }
+        }
+        return KDQ_OK;
+    }
     HIPCHK(launch_diag(h->stream, h->nch, h->max_ns, h->d_chains, h->t_Gm, DIM_NS, 0.0, 1));   // I
     Opnd *src = h->t_Gm, *srci = h->t_Gm;
     for (int s = 0; s < nb; ++s) {
@@ -1584,12 +1604,17 @@ int scee_grover(kdq_handle h) {
     HIPCHK(launch_scee_grover_form(h->stream, S->nch, S->m2_max, S->d_reg));
     for (int X = 0; X < 2; ++X) {
         if (nmax[X] == 0) continue;
-        LuArgs la{h->d_chains, S->t_C[X], dims[X], S->nx_max, S->d_piv, S->d_ld + X * S->nch, S->d_info};
+        LuArgs la{h->d_chains, S->t_C[X], dims[X], S->nx_max, S->d_piv, S->d_ld + X * S->nch, S->d_info,
+                  S->d_ph + 2 * X * S->nch};
         HIPCHK(launch_lu(h->stream, S->nch, nmax[X], la));
         LuSolveArgs sa{h->d_chains, S->t_C[X], S->t_IX[X], S->t_Inv[X], S->d_piv, dims[X], dims[X], S->nx_max, 0};
         HIPCHK(launch_lu_solve(h->stream, S->nch, nmax[X], nmax[X], sa));
     }
-    HIPCHK(launch_scee_detg(h->stream, S->nch, S->d_reg, S->d_ld, S->d_ld + S->nch));
+    if (h->kind == KDQ_MODEL_TV)   // src/tV/SCEE.jl:436: det g itself (real, signed)
+        HIPCHK(launch_scee_detg_signed(h->stream, S->nch, S->d_reg, S->d_ld, S->d_ld + S->nch, S->d_ph,
+                                       S->d_ph + 2 * S->nch));
+    else
+        HIPCHK(launch_scee_detg(h->stream, S->nch, S->d_reg, S->d_ld, S->d_ld + S->nch));
     return KDQ_OK;
 }
 
@@ -1630,6 +1655,50 @@ int scee_layer(kdq_handle h, int lt, int dir, uint32_t sweep) {
     }
     HIPCHK(launch_scee_accum(h->stream, S->nch, S->d_reg));
     return KDQ_OK;
+}
+
+// UpdateSCEELayer! of the rank-2 models: t-V bond family `family`, or every element (-1);
+// no flush: the decision kernel updates the four matrices in place
+int scee_layer_r2(kdq_handle h, int lt, int dir, uint32_t sweep, int family) {
+    SceeState *S = h->scee;
+    SceeR2Args a{};
+    a.chains = h->d_chains;
+    a.Gt = h->t_G[h->cur];
+    a.Gt0 = xcur(h, SX_GT0);
+    a.G0t = xcur(h, SX_G0T);
+    a.G0 = xcur(h, SX_G0);
+    a.reg = S->d_reg;
+    a.nch = S->nch;
+    a.lt = lt; a.dir = dir; a.family = family;
+    a.kind = h->kind == KDQ_MODEL_TV ? 1 : 2;
+    a.sweep = sweep;
+    a.nx_max = S->nx_max;
+    a.max_ns = h->max_ns;
+    ProfScope ps(h, KDQ_PROF_DECIDE);
+    HIPCHK(launch_scee_r2(h->stream, a));
+    return KDQ_OK;
+}
+
+// t-V: WrapK! (src/tV/SCEE.jl:509-517) of the four matrices: Gt <- eK Gt eKinv, Gt0 <- eK Gt0,
+// G0t <- G0t eKinv (forward), eK and eKinv exchanged backward
+int scee_wrapK_tv(kdq_handle h, bool forward) {
+    if (int r = wrap_K(h, h->t_G[h->cur], h->t_T, forward, KDQ_PROF_WRAP)) return r;
+    GemmArgs gl = gargs(h, forward ? h->t_eK : h->t_eKinv, xcur(h, SX_GT0), xspare(h, SX_GT0), DIM_NS, DIM_NS, DIM_NS);
+    if (int r = gemm(h, false, true, 0, 0, gl, KDQ_PROF_WRAP)) return r;
+    xflip(h, SX_GT0);
+    GemmArgs gr = gargs(h, xcur(h, SX_G0T), forward ? h->t_eKinv : h->t_eK, xspare(h, SX_G0T), DIM_NS, DIM_NS, DIM_NS);
+    if (int r = gemm(h, true, false, 0, 0, gr, KDQ_PROF_WRAP)) return r;
+    xflip(h, SX_G0T);
+    return KDQ_OK;
+}
+
+// t-V WrapV! of family j (src/tV/SCEE.jl:118-123, 205-210): "L" on Gt0, "B" on Gt, "R" on G0t
+// with D (forward) or D⁻¹ (backward)
+int scee_wrapV_tv(kdq_handle h, int lt, bool forward, int j) {
+    const int sg = forward ? +1 : -1;
+    if (int r = bondrot(h, xcur(h, SX_GT0), j, lt, sg, 1, DIM_NS, KDQ_PROF_WRAP)) return r;
+    if (int r = bondrot(h, h->t_G[h->cur], j, lt, sg, 0, DIM_NS, KDQ_PROF_WRAP)) return r;
+    return bondrot(h, xcur(h, SX_G0T), j, lt, sg, 2, DIM_NS, KDQ_PROF_WRAP);
 }
 
 // the "L", "B", "R" wraps of Gt0, Gt and G0t (WrapKV!, src/tU/GreenMatrix.jl:56-86)
@@ -1677,8 +1746,21 @@ int scee_sweeps(kdq_handle h, int sweeps, double *O) {
         const uint32_t sweep = h->sweep;
         for (int c = 0; c < nch; ++c) HIPCHK(hipMemsetAsync(S->reg[c].osum, 0, 2 * sizeof(double), h->stream));
         for (int lt = 1; lt <= nt; ++lt) {
-            if (int r = scee_wraps(h, lt, true)) return r;
-            if (int r = scee_layer(h, lt, 0, sweep)) return r;
+            if (h->kind == KDQ_MODEL_TV) {   // src/tV/SCEE.jl:103-138
+                if (int r = scee_wrapK_tv(h, true)) return r;
+                for (int j = 2; j >= 0; --j) {
+                    if (int r = scee_wrapV_tv(h, lt, true, j)) return r;
+                    if (int r = scee_layer_r2(h, lt, 0, sweep, j)) return r;
+                }
+                HIPCHK(launch_scee_accum(h->stream, S->nch, S->d_reg));
+            } else if (h->kind == KDQ_MODEL_TUV) {   // src/tUV/SCEE.jl:90-122
+                if (int r = scee_wraps(h, lt, true)) return r;
+                if (int r = scee_layer_r2(h, lt, 0, sweep, -1)) return r;
+                HIPCHK(launch_scee_accum(h->stream, S->nch, S->d_reg));
+            } else {
+                if (int r = scee_wraps(h, lt, true)) return r;
+                if (int r = scee_layer(h, lt, 0, sweep)) return r;
+            }
             if (is_node(h, lt)) {
                 const int idx = ++h->idx;
                 if (int r = scee_blocks(h, idx - 1)) return r;
@@ -1690,7 +1772,25 @@ int scee_sweeps(kdq_handle h, int sweeps, double *O) {
             }
         }
         for (int lt = nt; lt >= 1; --lt) {
-            if (int r = scee_layer(h, lt, 1, sweep)) return r;
+            // rank-2 models: the updates and the inverse wraps; before a node, where G4!
+            // recomputes the four matrices, the wraps after the last update are skipped
+            // (src/tV/SCEE.jl:177-214, src/tUV/SCEE.jl:156-183)
+            if (h->kind == KDQ_MODEL_TV) {
+                const bool node = is_node(h, lt - 1);
+                for (int j = 0; j < 3; ++j) {
+                    if (int r = scee_layer_r2(h, lt, 1, sweep, j)) return r;
+                    if (!node || j < 2)
+                        if (int r = scee_wrapV_tv(h, lt, false, j)) return r;
+                }
+                if (!node)
+                    if (int r = scee_wrapK_tv(h, false)) return r;
+                HIPCHK(launch_scee_accum(h->stream, S->nch, S->d_reg));
+            } else if (h->kind == KDQ_MODEL_TUV) {
+                if (int r = scee_layer_r2(h, lt, 1, sweep, -1)) return r;
+                HIPCHK(launch_scee_accum(h->stream, S->nch, S->d_reg));
+            } else if (int r = scee_layer(h, lt, 1, sweep)) {
+                return r;
+            }
             if (is_node(h, lt - 1)) {
                 const int idx = --h->idx;
                 if (int r = scee_blocks(h, idx)) return r;
@@ -1699,8 +1799,8 @@ int scee_sweeps(kdq_handle h, int sweeps, double *O) {
                 for (int i = idx + 1; i <= std::max(thidx, idx); ++i)
                     if (int r = scee_update_BR(h, i)) return r;
                 if (int r = scee_get_ABGM(h, idx, false)) return r;
-            } else if (int r = scee_wraps(h, lt, false)) {
-                return r;
+            } else if (h->kind != KDQ_MODEL_TV) {
+                if (int r = scee_wraps(h, lt, false)) return r;
             }
         }
         for (int c = 0; c < nch; ++c)
@@ -1744,9 +1844,9 @@ int kdq_scee_create(const kdq_model_desc *models, const kdq_run_desc *runs, cons
     for (int c = 0; c < nchains; ++c) {
         const kdq_scee_desc &g = regions[c];
         m2_max = std::max(m2_max, (int)(g.nA * g.nA + g.nB * g.nB));
-        if (models[c].kind != KDQ_MODEL_TU)
-            return fail(nullptr, KDQ_EARG, "kdq_scee_create: the swap-replica sweep is built for the t-U model only");
-        if (runs[c].delay < 1 || runs[c].delay > 16)
+        if (models[c].kind != models[0].kind)
+            return fail(nullptr, KDQ_EARG, "kdq_scee_create: all chains of a batch must share the model kind");
+        if (models[c].kind == KDQ_MODEL_TU && (runs[c].delay < 1 || runs[c].delay > 16))
             return fail(nullptr, KDQ_EARG, "kdq_scee_create: delay must be in 1..16");
         if (g.nA < 0 || g.nB < 0 || (g.nA && !g.indexA) || (g.nB && !g.indexB) || g.nlam < 1)
             return fail(nullptr, KDQ_EARG, "kdq_scee_create: invalid region description");
@@ -1757,9 +1857,16 @@ int kdq_scee_create(const kdq_model_desc *models, const kdq_run_desc *runs, cons
         na_max = std::max(na_max, (int)g.nA);
         nb_max = std::max(nb_max, (int)g.nB);
     }
-    const int ks = runs[0].delay <= 8 ? 8 : 16;
-    if (scee_lds_bytes(ks, na_max + nb_max) > 163840)
-        return fail(nullptr, KDQ_EARG, "kdq_scee_create: regions too large for the block size (reduce delay or |A|+|B|)");
+    if (models[0].kind == KDQ_MODEL_TU) {
+        const int ks = runs[0].delay <= 8 ? 8 : 16;
+        if (scee_lds_bytes(ks, na_max + nb_max) > 163840)
+            return fail(nullptr, KDQ_EARG, "kdq_scee_create: regions too large for the block size (reduce delay or |A|+|B|)");
+    } else {
+        int ns_max = 0;
+        for (int c = 0; c < nchains; ++c) ns_max = std::max(ns_max, (int)models[c].ns);
+        if (scee_r2_lds_bytes(na_max + nb_max, ns_max) > 163840)
+            return fail(nullptr, KDQ_EARG, "kdq_scee_create: lattice and regions too large for the rank-2 decision kernel");
+    }
     std::vector<kdq_model_desc> md(2 * nchains);
     std::vector<kdq_run_desc> rd(2 * nchains);
     std::vector<uint32_t> streams(2 * nchains);
@@ -1856,6 +1963,7 @@ int kdq_scee_create(const kdq_model_desc *models, const kdq_run_desc *runs, cons
         bad |= stable(h, &S->t_IX[X], vix[X]);
     }
     bad |= salloc(h, &S->d_ld, 2 * nchains);
+    bad |= salloc(h, &S->d_ph, 4 * nchains);
     bad |= salloc(h, &S->d_info, nchains);
     bad |= salloc(h, &S->d_piv, (size_t)nchains * std::max(S->nx_max, 1));
     {

@@ -216,11 +216,13 @@ __global__ __launch_bounds__(NT) void k_lu(LuArgs a) {
     double *Ur = lds + 2 * (size_t)n * PS, *Ui = Ur + (size_t)n * PS;  // U12 [col][j]
     __shared__ double red_v[NT / 64];
     __shared__ int red_i[NT / 64], lpiv[NB], bad;
-    __shared__ double s_logdet;
+    __shared__ double s_logdet, s_phr, s_phi;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (tid == 0) {
         bad = 0;
         s_logdet = 0.0;
+        s_phr = 1.0;
+        s_phi = 0.0;
     }
     for (int k0 = 0; k0 < n; k0 += NB) {
         const int kb = min(NB, n - k0), m = n - k0;
@@ -310,10 +312,19 @@ __global__ __launch_bounds__(NT) void k_lu(LuArgs a) {
             Im[o] = Pi[r * PS + q];
         }
         if (tid == 0) {
-            double s = 0.0;
-            for (int j = 0; j < kb; ++j)
-                s += 0.5 * log(Pr[j * PS + j] * Pr[j * PS + j] + Pi[j * PS + j] * Pi[j * PS + j]);
+            double s = 0.0, phr = s_phr, phi = s_phi;
+            for (int j = 0; j < kb; ++j) {
+                const double ur = Pr[j * PS + j], ui = Pi[j * PS + j], m2 = ur * ur + ui * ui;
+                s += 0.5 * log(m2);
+                // phase of det: the pivots' phases, a sign per row interchange
+                const double im = m2 > 0.0 ? 1.0 / sqrt(m2) : 0.0;
+                const double nr = phr * ur * im - phi * ui * im, ni = phr * ui * im + phi * ur * im;
+                phr = lpiv[j] != j ? -nr : nr;
+                phi = lpiv[j] != j ? -ni : ni;
+            }
             s_logdet += s;
+            s_phr = phr;
+            s_phi = phi;
         }
         // the panel's interchanges on every column outside it (one thread per column)
         for (int col = tid; col < n; col += NT) {
@@ -369,6 +380,10 @@ __global__ __launch_bounds__(NT) void k_lu(LuArgs a) {
     __syncthreads();
     if (tid == 0) {
         if (a.logdet) a.logdet[c] = s_logdet;
+        if (a.phase) {
+            a.phase[2 * c] = s_phr;
+            a.phase[2 * c + 1] = s_phi;
+        }
         if (bad && a.info && !a.info[c]) a.info[c] = bad;
     }
 }

@@ -119,6 +119,7 @@ struct LuArgs {
     int *piv;
     double *logdet;
     int *info;
+    double *phase;   // [2 per chain] det A / |det A| (re, im), or null
 };
 hipError_t launch_lu(hipStream_t st, int nchains, int max_n, const LuArgs &a);
 
@@ -164,7 +165,7 @@ hipError_t launch_decide_r2(hipStream_t st, int nchains, const DecideR2Args &a);
 struct BondRotArgs {
     const ChainDev *chains;
     const Opnd *X;
-    int family, lt, sign, mode, ncol_dim, pad;
+    int family, lt, sign, mode, ncol_dim, pad;   // mode 2: ncol_dim gives the row count
 };
 hipError_t launch_bondrot(hipStream_t st, int nchains, int max_nb, int max_cols, const BondRotArgs &a);
 // Observables of t-V (kind 1) and t-U-V (kind 2).
@@ -210,6 +211,24 @@ hipError_t launch_scee_grover_form(hipStream_t st, int nch, int total, const Sce
 hipError_t launch_scee_detg(hipStream_t st, int nch, const SceeRegion *reg, const double *logdet_a,
                             const double *logdet_b);
 hipError_t launch_scee_accum(hipStream_t st, int nch, const SceeRegion *reg);
+// det g of the regions from the LU log|det| and phase: |det|² (t-U, t-U-V) or the signed
+// real det (t-V, signed = 1)
+hipError_t launch_scee_detg_signed(hipStream_t st, int nch, const SceeRegion *reg, const double *logdet_a,
+                                   const double *logdet_b, const double *phase_a, const double *phase_b);
+// Decisions of the rank-2 models (t-V bonds of one family, t-U-V directed bonds) of slice lt,
+// replica 1 then replica 2 per element, every accepted change applied at once to the
+// replica's four matrices and to both regions.
+struct SceeR2Args {
+    const ChainDev *chains;
+    const Opnd *Gt, *Gt0, *G0t, *G0;   // per entry (replica r of chain c at c + r·nch)
+    const SceeRegion *reg;
+    int nch, lt, dir, family, kind;    // family: t-V bond family (elements family·nb ..), -1: every element; kind 1 t-V, 2 t-U-V
+    uint32_t sweep;
+    int nx_max, max_ns, pad;
+    const uint32_t *sweep_dev;
+};
+hipError_t launch_scee_r2(hipStream_t st, const SceeR2Args &a);
+size_t scee_r2_lds_bytes(int nx_max, int max_ns);
 // X[i][i] += v for the per-chain n x n matrices (dimension code ndim); set = 1 writes I.
 hipError_t launch_diag(hipStream_t st, int nchains, int max_n, const ChainDev *chains, const Opnd *X, int ndim,
                        double v, int set);

@@ -321,7 +321,8 @@ __global__ __launch_bounds__(256) void k_decide_r2(DecideR2Args a) {
 
 // t-V bond rotations of family j at slice lt: M = blockdiag over bonds (x, y) of
 // m(v) = uv diag(e^{v}, e^{-v}) uv, v = sign α[lt] η[s]; m(v)⁻¹ = m(-v).
-// mode 0: X <- M X M⁻¹ (both sides, N x N), mode 1: X <- M X (rows only, N x ncols).
+// mode 0: X <- M X M⁻¹ (both sides, N x N), mode 1: X <- M X (rows only, N x ncols),
+// mode 2: X <- X M⁻¹ (columns only, nrows x N).
 KDQ_DEV void bond_m(const ChainDev &ch, int e, int lt, int sign, double m[2][2]) {
     const double h = 0.7071067811865476;
     const double v = sign * ch.alpha[lt - 1] * ch.eta[ch.s[(size_t)e + (size_t)(lt - 1) * ch.n_elem] - 1];
@@ -346,6 +347,17 @@ __global__ void k_bondrot(BondRotArgs a) {
     const int x = ch.ex[e], y = ch.ey[e];
     double ml[2][2];
     bond_m(ch, e, a.lt, a.sign, ml);
+    if (a.mode == 2) {   // row cj, columns x and y: [X_x X_y] <- [X_x X_y] m(-v)
+        double mr[2][2];
+        bond_m(ch, e, a.lt, -a.sign, mr);
+        const size_t o0 = (size_t)cj + (size_t)x * X.ld, o1 = (size_t)cj + (size_t)y * X.ld;
+        const double ar = X.re[o0], ai = X.im[o0], br = X.re[o1], bi2 = X.im[o1];
+        X.re[o0] = ar * mr[0][0] + br * mr[1][0];
+        X.im[o0] = ai * mr[0][0] + bi2 * mr[1][0];
+        X.re[o1] = ar * mr[0][1] + br * mr[1][1];
+        X.im[o1] = ai * mr[0][1] + bi2 * mr[1][1];
+        return;
+    }
     if (a.mode == 1) {
         const size_t o0 = (size_t)x + (size_t)cj * X.ld, o1 = (size_t)y + (size_t)cj * X.ld;
         const double ar = X.re[o0], ai = X.im[o0], br = X.re[o1], bi2 = X.im[o1];

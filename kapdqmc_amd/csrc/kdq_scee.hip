@@ -429,6 +429,402 @@ hipError_t launch_scee_detg(hipStream_t st, int nch, const SceeRegion *reg, cons
     return hipGetLastError();
 }
 
+namespace {
+
+// ------------------------------------------------------------ rank-2 models (t-V, t-U-V)
+// UpdateSCEELayer! of t-V (src/tV/SCEE.jl:447-494, the bonds of one family) and t-U-V
+// (src/tUV/SCEE.jl:355-402, every directed bond): one workgroup per chain walks the elements
+// in order, replica 1 then replica 2 per element. The proposal weight is get_r!'s det r (t-V,
+// src/tV/GreenMatrix.jl:76-89) or |det r|² (t-U-V, src/tUV/GreenMatrix.jl:73-89), times γ'/γ
+// and the Grover factors det Tau (t-V) or |det Tau|² (t-U-V) of get_abTau1!/2!
+// (src/public/Gupdate.jl:91-132) to the powers λ and 1 - λ. An accepted change is applied at
+// once: det g and GMupdate! (Gupdate.jl:45-51) of both regions, G4update! (Gupdate.jl:23-37)
+// on the replica's four matrices. The other replica's G0 on a region is read from its matrix.
+constexpr int RT = 512;
+
+KDQ_DEV void c22_inv(const double ar[2][2], const double ai[2][2], double br[2][2], double bi[2][2], double &dr,
+                     double &di) {
+    double t1r, t1i, t2r, t2i;
+    cmul(ar[0][0], ai[0][0], ar[1][1], ai[1][1], t1r, t1i);
+    cmul(ar[0][1], ai[0][1], ar[1][0], ai[1][0], t2r, t2i);
+    dr = t1r - t2r;
+    di = t1i - t2i;
+    const double n2 = dr * dr + di * di, qr = dr / n2, qi = -di / n2;   // 1 / det
+    cmul(ar[1][1], ai[1][1], qr, qi, br[0][0], bi[0][0]);
+    cmul(-ar[0][1], -ai[0][1], qr, qi, br[0][1], bi[0][1]);
+    cmul(-ar[1][0], -ai[1][0], qr, qi, br[1][0], bi[1][0]);
+    cmul(ar[0][0], ai[0][0], qr, qi, br[1][1], bi[1][1]);
+}
+
+// sum of 8 doubles over the block (RT threads)
+KDQ_DEV void block_sum8(double v[8], double *red) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = wave_sum(v[k]);
+    __syncthreads();
+    if (lane == 0)
+        for (int k = 0; k < 8; ++k) red[wave * 8 + k] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        double s = 0.0;
+        for (int w = 0; w < RT / 64; ++w) s += red[w * 8 + k];
+        v[k] = s;
+    }
+}
+
+__global__ __launch_bounds__(RT) void k_scee_r2(SceeR2Args a) {
+    const int c = blockIdx.x, nch = a.nch, tid = threadIdx.x;
+    const SceeRegion &R = a.reg[c];
+    const int nA = R.nA, nB = R.nB, nX = nA + nB, NXM = max(a.nx_max, 1), NM = a.max_ns;
+    const int n = a.chains[c].ns, lt = a.lt;
+    extern __shared__ double lds[];
+    // region work vectors over the concatenated index q in [0, nX): region A then B
+    double *zr = lds, *zi = zr + 2 * NXM;            // zN [t][q]
+    double *ar_ = zi + 2 * NXM, *ai_ = ar_ + 2 * NXM;  // a  [u][q]
+    double *br_ = ai_ + 2 * NXM, *bi_ = br_ + 2 * NXM; // b  [t][q]
+    double *nr_ = bi_ + 2 * NXM, *ni_ = nr_ + 2 * NXM; // gmInv a [u][q]
+    double *wr_ = ni_ + 2 * NXM, *wi_ = wr_ + 2 * NXM; // Tau⁻¹ b [t][q]
+    double *c0r = wi_ + 2 * NXM, *c0i = c0r + 2 * NM;  // G0t[:, sub] [u][i]
+    double *c1r = c0i + 2 * NM, *c1i = c1r + 2 * NM;   // Gt[:, sub]  [u][i]
+    double *z1r = c1i + 2 * NM, *z1i = z1r + 2 * NM;   // r Gt0[sub, :]          [t][j]
+    double *z2r = z1i + 2 * NM, *z2i = z2r + 2 * NM;   // r (E_sub - Gt[sub, :]) [t][j]
+    double *red = z2i + 2 * NM;                        // RT/64 x 8
+    __shared__ double rmr[2][2], rmi[2][2], tinv[2][2][2][2], dreg[2], prob_s;   // tinv [X][re/im][t][u]
+    __shared__ int acc_s, sx_s;
+    const uint32_t tag = (uint32_t)lt | ((uint32_t)a.dir << 31);
+    const uint32_t sweep = a.sweep_dev ? *a.sweep_dev : a.sweep;
+    const int e0 = a.family >= 0 ? a.family * a.chains[c].nb : 0;
+    const int ne = a.family >= 0 ? a.chains[c].nb : a.chains[c].n_elem;
+    for (int el = e0; el < e0 + ne; ++el) {
+        const int sub0 = a.chains[c].ex[el], sub1 = a.chains[c].ey[el];
+        auto sub = [&](int k) { return k ? sub1 : sub0; };
+        for (int r = 0; r < 2; ++r) {
+            const int e = c + r * nch, o = c + (1 - r) * nch;
+            const ChainDev &ch = a.chains[e];
+            const Opnd Gt = a.Gt[e], Gt0 = a.Gt0[e], G0t = a.G0t[e], G0 = a.G0[e], G0o = a.G0[o];
+            const size_t sidx = (size_t)el + (size_t)(lt - 1) * ch.n_elem;
+            if (tid == 0) {
+                int chv;
+                double u;
+                proposal_draw(ch.seed, ch.chain_id, ch.stream, sweep, tag, (uint32_t)el, chv, u);
+                const int si = ch.s[sidx];
+                const int sx = chv + 1 + ((chv + 1) >= si ? 1 : 0);
+                double Dr[2][2] = {{0.0, 0.0}, {0.0, 0.0}}, Di[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+                if (a.kind == 1) {
+                    const double h = 0.7071067811865476, uv[2][2] = {{-h, -h}, {-h, h}};
+                    const double ds = ch.alpha[lt - 1] * (ch.eta[sx - 1] - ch.eta[si - 1]);
+                    const double t2[2] = {exp(ds) - 1.0, exp(-ds) - 1.0};
+#pragma unroll
+                    for (int p = 0; p < 2; ++p)
+#pragma unroll
+                        for (int q = 0; q < 2; ++q) Dr[p][q] = uv[p][0] * t2[0] * uv[0][q] + uv[p][1] * t2[1] * uv[1][q];
+                } else {
+                    const double ds = ch.eta[sx - 1] - ch.eta[si - 1], v[2] = {ds, ch.a * ds};
+#pragma unroll
+                    for (int q = 0; q < 2; ++q) {
+                        if (ch.is_complex) {
+                            double sn, cs;
+                            sincos(v[q], &sn, &cs);
+                            Dr[q][q] = cs - 1.0;
+                            Di[q][q] = sn;
+                        } else {
+                            Dr[q][q] = exp(v[q]) - 1.0;
+                        }
+                    }
+                }
+                // rr = I + Δ (I - Gt[sub, sub])
+                double rr[2][2], ri[2][2];
+#pragma unroll
+                for (int p = 0; p < 2; ++p)
+#pragma unroll
+                    for (int q = 0; q < 2; ++q) {
+                        double sr = Dr[p][q] + (p == q ? 1.0 : 0.0), sim = Di[p][q];
+#pragma unroll
+                        for (int k = 0; k < 2; ++k) {
+                            const size_t og = (size_t)sub(k) + (size_t)sub(q) * Gt.ld;
+                            double tr, ti;
+                            cmul(Dr[p][k], Di[p][k], Gt.re[og], Gt.im[og], tr, ti);
+                            sr -= tr;
+                            sim -= ti;
+                        }
+                        rr[p][q] = sr;
+                        ri[p][q] = sim;
+                    }
+                double ivr[2][2], ivi[2][2], dr, di;
+                c22_inv(rr, ri, ivr, ivi, dr, di);
+#pragma unroll
+                for (int p = 0; p < 2; ++p)
+#pragma unroll
+                    for (int q = 0; q < 2; ++q) {
+                        double sr = 0.0, sim = 0.0;
+#pragma unroll
+                        for (int k = 0; k < 2; ++k) {
+                            double tr, ti;
+                            cmul(ivr[p][k], ivi[p][k], Dr[k][q], Di[k][q], tr, ti);
+                            sr += tr;
+                            sim += ti;
+                        }
+                        rmr[p][q] = sr;
+                        rmi[p][q] = sim;
+                    }
+                const double pw = a.kind == 1 ? dr : dr * dr + di * di;
+                prob_s = pw * (ch.gamma[sx - 1] / ch.gamma[si - 1]);
+                sx_s = sx | (si << 8);
+                acc_s = 0;
+                // u is kept in the prob comparison below
+                dreg[0] = u;
+            }
+            __syncthreads();
+            const double u = dreg[0];
+            // ---- Grover factors of both regions
+            // which = 1: zN[t][q] = Σ_k Gt0[sub_t, X_k] (2 G0o[X_k, X_q] - δ); a[u][q] = Σ_t G0t[X_q, sub_t] r[t][u]
+            // which = 2: a[u][q] = Σ_k (2 G0o[X_q, X_k] - δ) G0t[X_k, sub_u]; zN[t][q] = Σ_t' r[t][t'] Gt0[sub_t', X_q]
+            for (int it = tid; it < 2 * nX; it += RT) {
+                const int t = it / nX, q = it % nX;
+                const int k0 = q < nA ? 0 : nA, k1 = q < nA ? nA : nX;
+                const int xq = R.idx[q];
+                double sr = 0.0, sim = 0.0, pr = 0.0, pi = 0.0;
+                if (r == 0) {
+                    for (int k = k0; k < k1; ++k) {
+                        const int xk = R.idx[k];
+                        const size_t og = (size_t)sub(t) + (size_t)xk * Gt0.ld, oo = (size_t)xk + (size_t)xq * G0o.ld;
+                        double mr = 2.0 * G0o.re[oo] - (k == q ? 1.0 : 0.0), mi = 2.0 * G0o.im[oo], tr, ti;
+                        cmul(Gt0.re[og], Gt0.im[og], mr, mi, tr, ti);
+                        sr += tr;
+                        sim += ti;
+                    }
+                    // a[u = t][q]
+#pragma unroll
+                    for (int k = 0; k < 2; ++k) {
+                        const size_t oc = (size_t)xq + (size_t)sub(k) * G0t.ld;
+                        double tr, ti;
+                        cmul(G0t.re[oc], G0t.im[oc], rmr[k][t], rmi[k][t], tr, ti);
+                        pr += tr;
+                        pi += ti;
+                    }
+                } else {
+                    for (int k = k0; k < k1; ++k) {
+                        const int xk = R.idx[k];
+                        const size_t oo = (size_t)xq + (size_t)xk * G0o.ld, oc = (size_t)xk + (size_t)sub(t) * G0t.ld;
+                        double mr = 2.0 * G0o.re[oo] - (k == q ? 1.0 : 0.0), mi = 2.0 * G0o.im[oo], tr, ti;
+                        cmul(mr, mi, G0t.re[oc], G0t.im[oc], tr, ti);
+                        pr += tr;
+                        pi += ti;
+                    }
+#pragma unroll
+                    for (int k = 0; k < 2; ++k) {
+                        const size_t og = (size_t)sub(k) + (size_t)xq * Gt0.ld;
+                        double tr, ti;
+                        cmul(rmr[t][k], rmi[t][k], Gt0.re[og], Gt0.im[og], tr, ti);
+                        sr += tr;
+                        sim += ti;
+                    }
+                }
+                zr[t * NXM + q] = sr;
+                zi[t * NXM + q] = sim;
+                ar_[t * NXM + q] = pr;
+                ai_[t * NXM + q] = pi;
+            }
+            __syncthreads();
+            // b[t][q] = Σ_k zN[t][k] gmInv_X[k][q] (k over q's region)
+            for (int it = tid; it < 2 * nX; it += RT) {
+                const int t = it / nX, q = it % nX;
+                const int X = q < nA ? 0 : 1, off = X ? nA : 0, m = X ? nB : nA;
+                const double *gr = R.gmInv[X], *gi = gr + (size_t)m * m;
+                double sr = 0.0, sim = 0.0;
+                for (int k = 0; k < m; ++k) {
+                    const size_t og = (size_t)k + (size_t)(q - off) * m;
+                    double tr, ti;
+                    cmul(zr[t * NXM + off + k], zi[t * NXM + off + k], gr[og], gi[og], tr, ti);
+                    sr += tr;
+                    sim += ti;
+                }
+                br_[t * NXM + q] = sr;
+                bi_[t * NXM + q] = sim;
+            }
+            __syncthreads();
+            // Tau_X[t][u] = δ + Σ_{q in X} b[t][q] a[u][q]: 2 regions x 4 complex sums
+            double v[8] = {0, 0, 0, 0, 0, 0, 0, 0}, w8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            for (int q = tid; q < nX; q += RT) {
+                const bool inA = q < nA;
+#pragma unroll
+                for (int t = 0; t < 2; ++t)
+#pragma unroll
+                    for (int uu = 0; uu < 2; ++uu) {
+                        double tr, ti;
+                        cmul(br_[t * NXM + q], bi_[t * NXM + q], ar_[uu * NXM + q], ai_[uu * NXM + q], tr, ti);
+                        const int k = 2 * (t * 2 + uu);
+                        v[k] += inA ? tr : 0.0;
+                        v[k + 1] += inA ? ti : 0.0;
+                        w8[k] += inA ? 0.0 : tr;
+                        w8[k + 1] += inA ? 0.0 : ti;
+                    }
+            }
+            block_sum8(v, red);
+            block_sum8(w8, red + 8 * (RT / 64));
+            if (tid == 0) {
+                const int m[2] = {nA, nB};
+                double dX[2];
+#pragma unroll
+                for (int X = 0; X < 2; ++X) {
+                    double tr[2][2], ti[2][2], ir[2][2], ii[2][2], dr = 1.0, di = 0.0;
+#pragma unroll
+                    for (int t = 0; t < 2; ++t)
+#pragma unroll
+                        for (int uu = 0; uu < 2; ++uu) {
+                            const int k = 2 * (t * 2 + uu);
+                            tr[t][uu] = (X ? w8[k] : v[k]) + (t == uu ? 1.0 : 0.0);
+                            ti[t][uu] = X ? w8[k + 1] : v[k + 1];
+                        }
+                    if (m[X]) c22_inv(tr, ti, ir, ii, dr, di);
+                    else {
+                        ir[0][0] = ir[1][1] = 1.0;
+                        ir[0][1] = ir[1][0] = 0.0;
+                        ii[0][0] = ii[0][1] = ii[1][0] = ii[1][1] = 0.0;
+                    }
+#pragma unroll
+                    for (int t = 0; t < 2; ++t)
+#pragma unroll
+                        for (int uu = 0; uu < 2; ++uu) {
+                            tinv[X][0][t][uu] = ir[t][uu];
+                            tinv[X][1][t][uu] = ii[t][uu];
+                        }
+                    dX[X] = m[X] ? (a.kind == 1 ? dr : dr * dr + di * di) : 1.0;
+                }
+                const double p = prob_s * pow(dX[0], R.lambda) * pow(dX[1], 1.0 - R.lambda);
+                const int ok = u < p;
+                acc_s = ok;
+                dreg[0] = dX[0];
+                dreg[1] = dX[1];
+                if (ok) {
+                    R.detg[0] *= dX[0];
+                    R.detg[1] *= dX[1];
+                }
+            }
+            __syncthreads();
+            if (!acc_s) continue;   // block-uniform
+            // ---- GMupdate!: gmInv_X -= (gmInv a)(Tau⁻¹ b)
+            for (int it = tid; it < 2 * nX; it += RT) {
+                const int uu = it / nX, q = it % nX;
+                const int X = q < nA ? 0 : 1, off = X ? nA : 0, m = X ? nB : nA;
+                const double *gr = R.gmInv[X], *gi = gr + (size_t)m * m;
+                double sr = 0.0, sim = 0.0;
+                for (int k = 0; k < m; ++k) {
+                    const size_t og = (size_t)(q - off) + (size_t)k * m;
+                    double tr, ti;
+                    cmul(gr[og], gi[og], ar_[uu * NXM + off + k], ai_[uu * NXM + off + k], tr, ti);
+                    sr += tr;
+                    sim += ti;
+                }
+                nr_[uu * NXM + q] = sr;
+                ni_[uu * NXM + q] = sim;
+                // Tau⁻¹ b [t = uu][q]
+                double wr = 0.0, wi = 0.0;
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    double tr, ti;
+                    cmul(tinv[X][0][uu][k], tinv[X][1][uu][k], br_[k * NXM + q], bi_[k * NXM + q], tr, ti);
+                    wr += tr;
+                    wi += ti;
+                }
+                wr_[uu * NXM + q] = wr;
+                wi_[uu * NXM + q] = wi;
+            }
+            // ---- G4update! operands from the current matrices
+            for (int it = tid; it < 2 * n; it += RT) {
+                const int uu = it / n, i = it % n;
+                const size_t o0 = (size_t)i + (size_t)sub(uu) * G0t.ld, o1 = (size_t)i + (size_t)sub(uu) * Gt.ld;
+                c0r[uu * NM + i] = G0t.re[o0];
+                c0i[uu * NM + i] = G0t.im[o0];
+                c1r[uu * NM + i] = Gt.re[o1];
+                c1i[uu * NM + i] = Gt.im[o1];
+                // column i of r Gt0[sub, :] and of r (E_sub - Gt[sub, :]) for row t = uu
+                double s1r = 0.0, s1i = 0.0, s2r = 0.0, s2i = 0.0;
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    const size_t og = (size_t)sub(k) + (size_t)i * Gt0.ld, ot = (size_t)sub(k) + (size_t)i * Gt.ld;
+                    double tr, ti;
+                    cmul(rmr[uu][k], rmi[uu][k], Gt0.re[og], Gt0.im[og], tr, ti);
+                    s1r += tr;
+                    s1i += ti;
+                    const double er = (i == sub(k) ? 1.0 : 0.0) - Gt.re[ot], ei = -Gt.im[ot];
+                    cmul(rmr[uu][k], rmi[uu][k], er, ei, tr, ti);
+                    s2r += tr;
+                    s2i += ti;
+                }
+                z1r[uu * NM + i] = s1r;
+                z1i[uu * NM + i] = s1i;
+                z2r[uu * NM + i] = s2r;
+                z2i[uu * NM + i] = s2i;
+            }
+            __syncthreads();
+            for (int it = tid; it < nX * nX; it += RT) {   // both regions' nX_A² + nX_B² entries
+                const int q = it % nX, cc = it / nX;
+                const bool inA = q < nA && cc < nA, inB = q >= nA && cc >= nA;
+                if (!inA && !inB) continue;
+                const int X = inA ? 0 : 1, off = X ? nA : 0, m = X ? nB : nA;
+                double *gr = R.gmInv[X], *gi = gr + (size_t)m * m;
+                const size_t og = (size_t)(q - off) + (size_t)(cc - off) * m;
+                double sr = 0.0, sim = 0.0;
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    double tr, ti;
+                    cmul(nr_[k * NXM + q], ni_[k * NXM + q], wr_[k * NXM + cc], wi_[k * NXM + cc], tr, ti);
+                    sr += tr;
+                    sim += ti;
+                }
+                gr[og] -= sr;
+                gi[og] -= sim;
+            }
+            // G0 += G0t[:, sub] z1; Gt0 += Gt[:, sub] z1; G0t -= G0t[:, sub] z2; Gt -= Gt[:, sub] z2
+            for (int it = tid; it < n * n; it += RT) {
+                const int i = it % n, j = it / n;
+                double p0r = 0.0, p0i = 0.0, p1r = 0.0, p1i = 0.0, q0r = 0.0, q0i = 0.0, q1r = 0.0, q1i = 0.0;
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    double tr, ti;
+                    cmul(c0r[k * NM + i], c0i[k * NM + i], z1r[k * NM + j], z1i[k * NM + j], tr, ti);
+                    p0r += tr; p0i += ti;
+                    cmul(c1r[k * NM + i], c1i[k * NM + i], z1r[k * NM + j], z1i[k * NM + j], tr, ti);
+                    p1r += tr; p1i += ti;
+                    cmul(c0r[k * NM + i], c0i[k * NM + i], z2r[k * NM + j], z2i[k * NM + j], tr, ti);
+                    q0r += tr; q0i += ti;
+                    cmul(c1r[k * NM + i], c1i[k * NM + i], z2r[k * NM + j], z2i[k * NM + j], tr, ti);
+                    q1r += tr; q1i += ti;
+                }
+                const size_t oa = (size_t)i + (size_t)j * G0.ld, ob = (size_t)i + (size_t)j * Gt0.ld;
+                const size_t oc = (size_t)i + (size_t)j * G0t.ld, od = (size_t)i + (size_t)j * Gt.ld;
+                G0.re[oa] += p0r; G0.im[oa] += p0i;
+                Gt0.re[ob] += p1r; Gt0.im[ob] += p1i;
+                G0t.re[oc] -= q0r; G0t.im[oc] -= q0i;
+                Gt.re[od] -= q1r; Gt.im[od] -= q1i;
+            }
+            if (tid == 0) {
+                ch.s[sidx] = (uint8_t)(sx_s & 0xff);
+                ch.counters[1] += 1.0;
+            }
+            __syncthreads();
+        }
+        if (tid == 0) {
+            a.chains[c].counters[0] += 1.0;
+            a.chains[c + nch].counters[0] += 1.0;
+        }
+    }
+}
+
+
+__global__ void k_scee_detg_signed(const SceeRegion *reg, int nch, const double *la, const double *lb,
+                                   const double *pa, const double *pb) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nch) return;
+    const SceeRegion &R = reg[c];
+    R.detg[0] = R.nA ? exp(la[c]) * pa[2 * c] : 1.0;
+    R.detg[1] = R.nB ? exp(lb[c]) * pb[2 * c] : 1.0;
+}
+
+}  // namespace
+
 hipError_t launch_scee_accum(hipStream_t st, int nch, const SceeRegion *reg) {
     hipLaunchKernelGGL(k_scee_accum, dim3((nch + 63) / 64), dim3(64), 0, st, reg, nch);
     return hipGetLastError();
@@ -438,5 +834,27 @@ hipError_t launch_diag(hipStream_t st, int nchains, int max_n, const ChainDev *c
                        double v, int set) {
     dim3 grid((max_n + 63) / 64, (max_n + 3) / 4, nchains);
     hipLaunchKernelGGL(k_diag, grid, dim3(256), 0, st, chains, X, ndim, v, set);
+    return hipGetLastError();
+}
+
+size_t scee_r2_lds_bytes(int nx_max, int max_ns) {
+    const size_t nxm = nx_max > 1 ? nx_max : 1;
+    return (20 * nxm + 16 * (size_t)max_ns + 8 * 2 * (RT / 64)) * sizeof(double);
+}
+
+hipError_t launch_scee_detg_signed(hipStream_t st, int nch, const SceeRegion *reg, const double *logdet_a,
+                                   const double *logdet_b, const double *phase_a, const double *phase_b) {
+    hipLaunchKernelGGL(k_scee_detg_signed, dim3((nch + 63) / 64), dim3(64), 0, st, reg, nch, logdet_a, logdet_b,
+                       phase_a, phase_b);
+    return hipGetLastError();
+}
+
+hipError_t launch_scee_r2(hipStream_t st, const SceeR2Args &a) {
+    const size_t bytes = scee_r2_lds_bytes(a.nx_max, a.max_ns);
+    if (bytes > 163840) return hipErrorInvalidValue;
+    if (hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_scee_r2),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes))
+        return e;
+    hipLaunchKernelGGL(k_scee_r2, dim3(a.nch), dim3(RT), bytes, st, a);
     return hipGetLastError();
 }

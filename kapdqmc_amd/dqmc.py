@@ -274,11 +274,13 @@ def phy_update(path: str, model, s: np.ndarray, Sweeps: int, record: bool = Fals
 
 
 class SCEEBatch:
-    """Swap-replica Rényi-2 chains of the t-U model (ctrl_SCEEicr, src/tU/SCEE.jl:1-262).
+    """Swap-replica Rényi-2 chains (ctrl_SCEEicr) of the t-U (src/tU/SCEE.jl:1-262), t-V
+    (src/tV/SCEE.jl:1-431) or t-U-V (src/tUV/SCEE.jl:1-339) model.
 
     indexA / indexB: the reference's 1-based site lists; lam, nlam: λ and Nλ. Each chain
-    holds two replica fields (ss[1], ss[2]); replica r draws from Philox stream 1 + r.
-    delay: sites per decision block (1..16).
+    holds two replica fields (ss[1], ss[2]) in the model's field layout; replica r draws from
+    Philox stream 1 + r. delay: t-U sites per decision block (1..16); the rank-2 models decide
+    element by element.
     """
 
     def __init__(self, models, indexA, indexB, lam: float, nlam: int, seeds, chains=None,
@@ -359,9 +361,10 @@ def _is_nested(x) -> bool:
 
 def ctrl_SCEEicr(path: str, model, indexA, indexB, Sweeps: int, λ: float, Nλ: int, ss, record: bool = False,
                  *, seed: int = 0, chain: int = 0, delay: int = 8, sweep0: int = 0):
-    """ctrl_SCEEicr on the GPU (t-U, src/tU/SCEE.jl:1-262): mutates the two replica fields
-    ss[0], ss[1] in place, returns ss, and appends the row [λ, O_1, …, O_Sweeps] to the
-    reference's CSV file name when ``record`` is true."""
+    """ctrl_SCEEicr on the GPU (t-U src/tU/SCEE.jl:1-262, t-V src/tV/SCEE.jl:1-431, t-U-V
+    src/tUV/SCEE.jl:1-339): mutates the two replica fields ss[0], ss[1] in place, returns ss,
+    and appends the row [λ, O_1, …, O_Sweeps] to the reference's CSV file name when
+    ``record`` is true."""
     b = SCEEBatch([model], indexA, indexB, λ, Nλ, [seed], [chain], delay=delay, sweep0=sweep0)
     try:
         b.set_fields(0, 0, ss[0])
@@ -372,13 +375,23 @@ def ctrl_SCEEicr(path: str, model, indexA, indexB, Sweeps: int, λ: float, Nλ: 
     finally:
         b.close()
     if record:
-        name = _lattice_tag(model.Lattice)
-        if model.Θquench == 0.0:
+        kind = getattr(model, "kind", "tU")
+        name = _lattice_tag(model.Lattice, kind)
+        if kind == "tU" and model.Θquench == 0.0:
             fn = (f"tUSCEE{name}_t{model.Ht}U{model.Hu1}size{model.site}Δt{model.Δt}Θ{model.Θrelax}"
                   f"N{Nλ}BS{model.BatchSize}.csv")
-        else:
+        elif kind == "tU":
             fn = (f"tUSCEE{name}_t{model.Ht}U{model.Hu1}_{model.Hu2}size{model.site}Δt{model.Δt}"
                   f"Θ{model.Θrelax}_{model.Θquench}N{Nλ}BS{model.BatchSize}.csv")
+        elif kind == "tV" and len(np.unique(model.α)) == 1:   # src/tV/SCEE.jl:25-29
+            fn = (f"tVSCEE{name}_t{model.Ht}V{model.Hv1}size{model.site}Δt{model.Δt}Θ{model.Θrelax}"
+                  f"N{Nλ}BS{model.BatchSize}.csv")
+        elif kind == "tV":
+            fn = (f"tVSCEE{name}_t{model.Ht}V{model.Hv1}_{model.Hv2}size{model.site}Δt{model.Δt}"
+                  f"Θ{model.Θrelax}_{model.Θquench}N{Nλ}BS{model.BatchSize}.csv")
+        else:   # src/tUV/SCEE.jl:25
+            fn = (f"tUVSCEE{name}_t{model.Ht}U{model.U}V{model.V}size{model.site}Δt{model.Δt}Θ{model.Θ}"
+                  f"N{Nλ}BS{model.BatchSize}.csv")
         with open(os.path.join(path, fn), "a") as fh:
             fh.write(",".join(repr(float(x)) for x in [λ, *O]) + "\n")
     return ss
