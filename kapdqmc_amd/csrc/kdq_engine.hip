@@ -37,6 +37,7 @@ struct ChainHost {
     double *FZ = nullptr, *FC = nullptr;     // flush scratch: W·Y (64 x N) and gathered columns (N x 64)
     float *FZ32 = nullptr, *FC32 = nullptr;
     double *DU = nullptr, *DV = nullptr;     // deferred-flush factors of a t-U slice (N x N each)
+    double *DI = nullptr;                    // inverses of the 16 x 16 diagonal blocks (Np x 32)
     float *DU32 = nullptr, *DV32 = nullptr;
     int *rk = nullptr, *rank = nullptr;
     int *ex = nullptr, *ey = nullptr, *inc = nullptr;
@@ -72,6 +73,7 @@ struct kdq_handle_s {
     Opnd *t_FZ = nullptr, *t_FC = nullptr;     // flush scratch operands
     OpndF *t_FZ32 = nullptr, *t_FC32 = nullptr;
     Opnd *t_DU = nullptr, *t_DV = nullptr;     // deferred-flush factors (t-U)
+    Opnd *t_DI = nullptr;                      // diagonal-block inverses of the Np x Np factors
     OpndF *t_DU32 = nullptr, *t_DV32 = nullptr;
     bool defer = false;                        // t-U slices flush once (U, V factors)
     std::vector<Opnd *> t_UR, t_ULH;
@@ -344,9 +346,9 @@ int orthonormalise(kdq_handle h, Opnd *U, double *logacc = nullptr) {
         GemmArgs g = gargs(h, U, U, h->t_S, DIM_NP, DIM_NP, DIM_NS);
         if (int r = gemm(h, true, true, 2, 0, g, KDQ_PROF_QR)) return r;
         ProfScope ps(h, KDQ_PROF_QR);
-        CholArgs ca{h->d_chains, h->t_S, DIM_NP, 0, pass == 0 ? shift : 0.0, h->d_qinfo, logacc};
+        CholArgs ca{h->d_chains, h->t_S, DIM_NP, 0, pass == 0 ? shift : 0.0, h->d_qinfo, logacc, h->t_DI};
         HIPCHK(launch_chol(h->stream, h->nch, h->max_np, ca));
-        TrsmArgs ta{h->d_chains, U, h->t_S, DIM_NS, DIM_NP};
+        TrsmArgs ta{h->d_chains, U, h->t_S, DIM_NS, DIM_NP, h->t_DI};
         HIPCHK(launch_trsm_rlh(h->stream, h->nch, h->max_ns, h->max_np, ta));
     }
     return KDQ_OK;
@@ -404,9 +406,9 @@ int get_G(kdq_handle h, int idx, Opnd *dst = nullptr) {
     if (int r = gemm(h, true, true, 2, 0, g1, KDQ_PROF_GETG)) return r;
     {
         ProfScope ps(h, KDQ_PROF_INV);
-        LuArgs la{h->d_chains, h->t_M, DIM_NP, h->max_np, h->d_piv, h->d_logdet, h->d_info};
+        LuArgs la{h->d_chains, h->t_M, DIM_NP, h->max_np, h->d_piv, h->d_logdet, h->d_info, nullptr, h->t_DI};
         HIPCHK(launch_lu(h->stream, h->nch, h->max_np, la));
-        LuSolveArgs sa{h->d_chains, h->t_M, ULH, h->t_X, h->d_piv, DIM_NP, DIM_NS, h->max_np, 0};
+        LuSolveArgs sa{h->d_chains, h->t_M, ULH, h->t_X, h->d_piv, DIM_NP, DIM_NS, h->max_np, 0, h->t_DI};
         HIPCHK(launch_lu_solve(h->stream, h->nch, h->max_np, h->max_ns, sa));
     }
     GemmArgs g3 = gargs(h, UR, h->t_X, dst ? dst : h->t_G[h->cur], DIM_NS, DIM_NS, DIM_NP);
@@ -857,7 +859,7 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
     std::vector<ChainDev> cd(nchains);
     std::vector<Opnd> v_eK, v_eKinv, v_hK, v_hKinv, v_Pt, v_G0, v_G1, v_T, v_Gm, v_Tm, v_P0, v_P1, v_P2, v_M, v_X, v_S;
     std::vector<OpndF> v_G32a, v_G32b, v_T32, v_eK32, v_eKinv32, v_FZ32, v_FC32;
-    std::vector<Opnd> v_FZ, v_FC, v_DU, v_DV;
+    std::vector<Opnd> v_FZ, v_FC, v_DU, v_DV, v_DI;
     std::vector<OpndF> v_DU32, v_DV32;
     std::vector<BlockDesc> blk_all;
     std::vector<int> blk_offset, blk_count;
@@ -939,7 +941,8 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
         ok &= dalloc(C, &C.W, 2 * 64 * 64) == hipSuccess;
         ok &= dalloc(C, &C.FZ, 2 * 64 * (size_t)n) == hipSuccess;
         ok &= dalloc(C, &C.FC, 2 * 64 * (size_t)n) == hipSuccess;
-        ok &= dalloc(C, &C.rk, (size_t)ns_all + 2) == hipSuccess;   // blocks run over the largest chain
+        ok &= dalloc(C, &C.rk, (size_t)ns_all + 2) == hipSuccess;
+        ok &= dalloc(C, &C.DI, 2 * 32 * (size_t)np) == hipSuccess;   // blocks run over the largest chain
         ok &= dalloc(C, &C.rank, 1) == hipSuccess;
         if (m.kind == KDQ_MODEL_TU && !h->mixed) {
             ok &= dalloc(C, &C.DU, 2 * nn) == hipSuccess;
@@ -1055,6 +1058,7 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
         v_M.push_back(cplx(C.M, np, pp)); v_X.push_back(cplx(C.X, np, nP));
         v_S.push_back(cplx(C.S, np, pp));
         v_FZ.push_back(Opnd{C.FZ, C.FZ + 64 * (size_t)n, 64, 0});
+        v_DI.push_back(Opnd{C.DI, C.DI + 32 * (size_t)np, np, 0});
         v_DU.push_back(Opnd{C.DU, C.DU ? C.DU + nn : nullptr, n, 0});
         v_DV.push_back(Opnd{C.DV, C.DV ? C.DV + nn : nullptr, n, 0});
         v_DU32.push_back(OpndF{C.DU32, C.DU32 ? C.DU32 + nn : nullptr, n, 0});
@@ -1110,6 +1114,7 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
     r |= make_table(h, &h->t_M, v_M); r |= make_table(h, &h->t_X, v_X); r |= make_table(h, &h->t_S, v_S);
     r |= make_table(h, &h->t_FZ, v_FZ); r |= make_table(h, &h->t_FC, v_FC);
     r |= make_table(h, &h->t_DU, v_DU); r |= make_table(h, &h->t_DV, v_DV);
+    r |= make_table(h, &h->t_DI, v_DI);
     r |= make_table_f(h, &h->t_DU32, v_DU32); r |= make_table_f(h, &h->t_DV32, v_DV32);
     if (h->mixed) {
         r |= make_table_f(h, &h->t_G32[0], v_G32a); r |= make_table_f(h, &h->t_G32[1], v_G32b);

@@ -17,11 +17,15 @@
 
 namespace {
 
-constexpr int NT = 256;
+constexpr int NT = 256;     // threads of the triangular solves
+constexpr int NF = 1024;    // threads of the one-workgroup Cholesky (4 waves per SIMD hide the
+                            // FP64 dependency latency that one wave per SIMD exposes)
 constexpr int NB = 16;      // panel / slab width
 constexpr int PS = NB + 1;  // padded LDS row stride of a panel (doubles)
 constexpr int RB = 16;      // rows (right-hand sides) per workgroup of the solves
+constexpr int TB = 8;       // trailing-update entries per thread whose loads are issued together
 
+template <int T>
 KDQ_DEV double blk_sum(double v, double *red) {
     v = wave_sum(v);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -30,13 +34,86 @@ KDQ_DEV double blk_sum(double v, double *red) {
     __syncthreads();
     double s = 0.0;
 #pragma unroll
-    for (int w = 0; w < NT / 64; ++w) s += red[w];
+    for (int w = 0; w < T / 64; ++w) s += red[w];
     return s;
+}
+
+// Inverse of the panel's kb x kb diagonal block (LDS rows [r * PS + q]) by row sweeps: thread
+// (r, q) = (tid / NB, tid % NB), tid < NB², keeps X[r][q] in registers; row i is final after
+// step i and is published in LDS (X: real [i * NB + q], then imaginary) to the rows it feeds.
+// LU = false: the lower Cholesky factor (real positive diagonal), to columns 0..NB-1 of D.
+// LU = true: the unit-lower factor (forward, X, columns 0..NB-1) and the upper factor
+// (backward in the same steps, X + 2 NB², columns NB..2NB-1). Entries outside the triangle
+// come out as exact zeros. Every thread of the block calls it (one barrier per step).
+template <bool LU>
+KDQ_DEV void diag_inv(const double *Pr, const double *Pi, double *X, int kb, int k0, const Opnd &D) {
+    const int tid = threadIdx.x, r = tid / NB, q = tid % NB;
+    const bool act = r < kb && q < kb;
+    double *Xi = X + NB * NB, *Yr = X + 2 * NB * NB, *Yi = X + 3 * NB * NB;
+    double lr = r == q ? 1.0 : 0.0, li = 0.0, ur = lr, ui = 0.0;
+    for (int s = 0; s < kb; ++s) {
+        const int i = kb - 1 - s;
+        if (act && r == s) {
+            double xr = lr, xi = li;
+            if (!LU) {
+                const double id = 1.0 / Pr[s * PS + s];
+                xr *= id;
+                xi *= id;
+            }
+            X[s * NB + q] = xr;
+            Xi[s * NB + q] = xi;
+            const size_t o = (size_t)(k0 + s) + (size_t)q * D.ld;
+            D.re[o] = xr;
+            D.im[o] = xi;
+        }
+        if (LU && act && r == i) {
+            const double dr = Pr[i * PS + i], di = Pi[i * PS + i], d2 = dr * dr + di * di;
+            double xr, xi;
+            cmul(ur, ui, dr / d2, -di / d2, xr, xi);
+            Yr[i * NB + q] = xr;
+            Yi[i * NB + q] = xi;
+            const size_t o = (size_t)(k0 + i) + (size_t)(NB + q) * D.ld;
+            D.re[o] = xr;
+            D.im[o] = xi;
+        }
+        __syncthreads();
+        double tr_, ti_;
+        if (act && r > s) {
+            cmul(Pr[r * PS + s], Pi[r * PS + s], X[s * NB + q], Xi[s * NB + q], tr_, ti_);
+            lr -= tr_;
+            li -= ti_;
+        }
+        if (LU && act && r < i) {
+            cmul(Pr[r * PS + i], Pi[r * PS + i], Yr[i * NB + q], Yi[i * NB + q], tr_, ti_);
+            ur -= tr_;
+            ui -= ti_;
+        }
+    }
+}
+
+// Inverses of the 16 x 16 diagonal blocks of a factored n x n matrix (lower Cholesky factor,
+// or the unit-lower and upper LU factors) into D: one workgroup per block and chain.
+template <bool LU>
+__global__ __launch_bounds__(NT) void k_diag_inv(const ChainDev *chains, const Opnd *A_, int ndim, const Opnd *D_) {
+    const int c = blockIdx.y, k0 = blockIdx.x * NB;
+    const int n = dim_of(chains[c], ndim);
+    if (k0 >= n) return;   // uniform per workgroup
+    const int kb = min(NB, n - k0);
+    const Opnd A = A_[c];
+    __shared__ double Pr[NB * PS], Pi[NB * PS], X[4 * NB * NB];
+    const int tid = threadIdx.x, r = tid % NB, q = tid / NB;
+    if (r < kb && q < kb) {
+        const size_t o = (size_t)(k0 + r) + (size_t)(k0 + q) * A.ld;
+        Pr[r * PS + q] = A.re[o];
+        Pi[r * PS + q] = A.im[o];
+    }
+    __syncthreads();
+    diag_inv<LU>(Pr, Pi, X, kb, k0, D_[c]);
 }
 
 // In-place Cholesky S + σI = L Lᴴ of the Hermitian n x n matrix (lower triangle read and
 // written), σ = shift · trace(S). info (sticky) receives j+1 for a non-positive pivot j.
-__global__ __launch_bounds__(NT) void k_chol(CholArgs a) {
+__global__ __launch_bounds__(NF) void k_chol(CholArgs a) {
     const int c = blockIdx.x;
     const ChainDev &ch = a.chains[c];
     const int n = dim_of(ch, a.ndim);
@@ -45,17 +122,17 @@ __global__ __launch_bounds__(NT) void k_chol(CholArgs a) {
     double *Re = A.re, *Im = A.im;
     extern __shared__ double lds[];
     double *Pr = lds, *Pi = lds + (size_t)n * PS;  // panel rows k0.., [r][q]
-    __shared__ double red[NT / 64];
+    __shared__ double red[NF / 64];
     __shared__ int bad;
     const int tid = threadIdx.x;
     if (tid == 0) bad = 0;
     double tr = 0.0;
-    for (int i = tid; i < n; i += NT) tr += Re[i + (size_t)i * ld];
-    const double sigma = a.shift * blk_sum(tr, red);
+    for (int i = tid; i < n; i += NF) tr += Re[i + (size_t)i * ld];
+    const double sigma = a.shift * blk_sum<NF>(tr, red);
     for (int k0 = 0; k0 < n; k0 += NB) {
         const int kb = min(NB, n - k0), m = n - k0;
         __syncthreads();
-        for (int e = tid; e < m * kb; e += NT) {
+        for (int e = tid; e < m * kb; e += NF) {
             const int r = e % m, q = e / m;
             const size_t o = (k0 + r) + (size_t)(k0 + q) * ld;
             Pr[r * PS + q] = Re[o] + (r == q ? sigma : 0.0);
@@ -67,7 +144,7 @@ __global__ __launch_bounds__(NT) void k_chol(CholArgs a) {
             __syncthreads();
             if (!(djj > 0.0) && tid == 0 && !bad) bad = k0 + j + 1;
             const double d = sqrt(fmax(djj, 1e-300)), id = 1.0 / d;
-            for (int r = j + tid; r < m; r += NT) {
+            for (int r = j + tid; r < m; r += NF) {
                 if (r == j) {
                     Pr[j * PS + j] = d;
                     Pi[j * PS + j] = 0.0;
@@ -78,7 +155,7 @@ __global__ __launch_bounds__(NT) void k_chol(CholArgs a) {
             }
             __syncthreads();
             const int nq = kb - j - 1;
-            for (int e = tid; e < (m - j - 1) * nq; e += NT) {
+            for (int e = tid; e < (m - j - 1) * nq; e += NF) {
                 const int r = j + 1 + e / nq, q = j + 1 + e % nq;
                 if (r < q) continue;
                 double tr_, ti_;
@@ -88,37 +165,58 @@ __global__ __launch_bounds__(NT) void k_chol(CholArgs a) {
             }
             __syncthreads();
         }
-        for (int e = tid; e < m * kb; e += NT) {
+        for (int e = tid; e < m * kb; e += NF) {
             const int r = e % m, q = e / m;
             if (r < q) continue;
             const size_t o = (k0 + r) + (size_t)(k0 + q) * ld;
             Re[o] = Pr[r * PS + q];
             Im[o] = Pi[r * PS + q];
         }
-        // trailing Hermitian update (lower): S[i][j] -= Σ_q P[i][q] conj(P[j][q]), i >= j
+        // trailing Hermitian update (lower): S[i][j] -= Σ_q P[i][q] conj(P[j][q]), i >= j.
+        // TB entries per thread at a time, their loads issued before any store (the stores
+        // may alias later loads as far as the compiler knows, which would serialise them)
         const int mt = m - kb;
-        for (int e = tid; e < mt * mt; e += NT) {
-            const int i = kb + e % mt, j = kb + e / mt;
-            if (i < j) continue;
-            double sr = 0.0, si = 0.0;
-#pragma unroll 4
-            for (int q = 0; q < kb; ++q) {
-                const double ar = Pr[i * PS + q], ai = Pi[i * PS + q];
-                const double br = Pr[j * PS + q], bi = -Pi[j * PS + q];
-                sr += ar * br - ai * bi;
-                si += ar * bi + ai * br;
+        for (int e0 = tid; e0 < mt * mt; e0 += NF * TB) {
+            double gr[TB], gi[TB];
+            size_t off[TB];
+            bool ok[TB];
+#pragma unroll
+            for (int b = 0; b < TB; ++b) {
+                const int e = e0 + b * NF, i = kb + e % mt, j = kb + e / mt;
+                ok[b] = e < mt * mt && i >= j;
+                off[b] = ok[b] ? (k0 + i) + (size_t)(k0 + j) * ld : 0;
+                gr[b] = ok[b] ? Re[off[b]] : 0.0;
+                gi[b] = ok[b] ? Im[off[b]] : 0.0;
             }
-            const size_t o = (k0 + i) + (size_t)(k0 + j) * ld;
-            Re[o] -= sr;
-            Im[o] -= si;
+#pragma unroll
+            for (int b = 0; b < TB; ++b) {
+                if (!ok[b]) continue;
+                const int e = e0 + b * NF, i = kb + e % mt, j = kb + e / mt;
+                double sr = 0.0, si = 0.0;
+#pragma unroll 4
+                for (int q = 0; q < kb; ++q) {
+                    const double ar = Pr[i * PS + q], ai = Pi[i * PS + q];
+                    const double br = Pr[j * PS + q], bi = -Pi[j * PS + q];
+                    sr += ar * br - ai * bi;
+                    si += ar * bi + ai * br;
+                }
+                gr[b] -= sr;
+                gi[b] -= si;
+            }
+#pragma unroll
+            for (int b = 0; b < TB; ++b)
+                if (ok[b]) {
+                    Re[off[b]] = gr[b];
+                    Im[off[b]] = gi[b];
+                }
         }
     }
     __syncthreads();
     if (tid == 0 && bad && a.info && !a.info[c]) a.info[c] = bad;
     if (a.logdiag) {
         double lsum = 0.0;
-        for (int i = tid; i < n; i += NT) lsum += log(Re[i + (size_t)i * ld]);
-        lsum = blk_sum(lsum, red);
+        for (int i = tid; i < n; i += NF) lsum += log(Re[i + (size_t)i * ld]);
+        lsum = blk_sum<NF>(lsum, red);
         if (tid == 0) a.logdiag[c] += lsum;
     }
 }
@@ -137,6 +235,7 @@ __global__ __launch_bounds__(NT) void k_trsm_rlh(TrsmArgs t) {
     extern __shared__ double lds[];
     double *Xr = lds, *Xi = lds + (size_t)n * RB;                      // [j][r]
     double *Lr = lds + 2 * (size_t)n * RB, *Li = Lr + (size_t)n * PS;  // [jj][i]
+    double *Dr_ = Li + (size_t)n * PS, *Di_ = Dr_ + NB * PS;           // block of L⁻¹ [jj][i]
     const int tid = threadIdx.x;
     for (int e = tid; e < n * RB; e += NT) {
         const int r = e % RB, j = e / RB;
@@ -158,8 +257,32 @@ __global__ __launch_bounds__(NT) void k_trsm_rlh(TrsmArgs t) {
             Lr[jj * PS + i] = L.re[o];
             Li[jj * PS + i] = L.im[o];
         }
+        if (t.Dinv)   // the slab's block of L⁻¹ (lower, written by k_chol)
+            for (int e = tid; e < bw * bw; e += NT) {
+                const int jj = e % bw, i = e / bw;
+                const Opnd D = t.Dinv[c];
+                const size_t o = (size_t)(b0 + jj) + (size_t)i * D.ld;
+                Dr_[jj * PS + i] = D.re[o];
+                Di_[jj * PS + i] = D.im[o];
+            }
         __syncthreads();
-        if (tid < RB) {
+        if (t.Dinv) {
+            // X_slab <- X_slab (L_ss⁻¹)ᴴ: X[r][j] = Σ_{i <= j} X[r][i] conj(L⁻¹[j][i]), all (r, j) at once
+            const int r = tid % RB, j = tid / RB;
+            double sr = 0.0, si = 0.0;
+            if (j < bw)
+                for (int i = 0; i <= j; ++i) {
+                    const double xr = Xr[(b0 + i) * RB + r], xi = Xi[(b0 + i) * RB + r];
+                    const double lr = Dr_[j * PS + i], li = -Di_[j * PS + i];
+                    sr += xr * lr - xi * li;
+                    si += xr * li + xi * lr;
+                }
+            __syncthreads();
+            if (j < bw) {
+                Xr[(b0 + j) * RB + r] = sr;
+                Xi[(b0 + j) * RB + r] = si;
+            }
+        } else if (tid < RB) {
             const int r = tid;
             for (int j = 0; j < bw; ++j) {
                 double sr = Xr[(b0 + j) * RB + r], si = Xi[(b0 + j) * RB + r];
@@ -403,6 +526,7 @@ __global__ __launch_bounds__(NT) void k_lu_solve(LuSolveArgs s) {
     extern __shared__ double lds[];
     double *Br = lds, *Bi = lds + (size_t)n * RB;                      // [i][q]
     double *Fr = lds + 2 * (size_t)n * RB, *Fi = Fr + (size_t)n * PS;  // factor slab [row][i]
+    double *Dr_ = Fi + (size_t)n * PS, *Di_ = Dr_ + NB * PS;           // block inverse [jj][i]
     const int tid = threadIdx.x;
     const size_t ld = F.ld;
     for (int e = tid; e < n * RB; e += NT) {
@@ -440,8 +564,31 @@ __global__ __launch_bounds__(NT) void k_lu_solve(LuSolveArgs s) {
             Fr[jj * PS + i] = F.re[o];
             Fi[jj * PS + i] = F.im[o];
         }
+        if (s.Dinv)   // L11⁻¹ of this slab (unit lower, written by k_lu)
+            for (int e = tid; e < bw * bw; e += NT) {
+                const int jj = e % bw, i = e / bw;
+                const Opnd D = s.Dinv[c];
+                const size_t o = (size_t)(b0 + jj) + (size_t)i * D.ld;
+                Dr_[jj * PS + i] = D.re[o];
+                Di_[jj * PS + i] = D.im[o];
+            }
         __syncthreads();
-        if (tid < RB) {
+        if (s.Dinv) {   // B_slab <- L11⁻¹ B_slab, every (j, q) at once
+            const int q = tid % RB, j = tid / RB;
+            double sr = 0.0, si = 0.0;
+            if (j < bw)
+                for (int i = 0; i <= j; ++i) {
+                    double tr_, ti_;
+                    cmul(Dr_[j * PS + i], Di_[j * PS + i], Br[(b0 + i) * RB + q], Bi[(b0 + i) * RB + q], tr_, ti_);
+                    sr += tr_;
+                    si += ti_;
+                }
+            __syncthreads();
+            if (j < bw) {
+                Br[(b0 + j) * RB + q] = sr;
+                Bi[(b0 + j) * RB + q] = si;
+            }
+        } else if (tid < RB) {
             const int q = tid;
             for (int j = 1; j < bw; ++j) {
                 double sr = Br[(b0 + j) * RB + q], si = Bi[(b0 + j) * RB + q];
@@ -481,8 +628,31 @@ __global__ __launch_bounds__(NT) void k_lu_solve(LuSolveArgs s) {
             Fr[jj * PS + i] = F.re[o];
             Fi[jj * PS + i] = F.im[o];
         }
+        if (s.Dinv)   // U11⁻¹ of this slab (columns 16..31 of the inverse blocks)
+            for (int e = tid; e < bw * bw; e += NT) {
+                const int jj = e % bw, i = e / bw;
+                const Opnd D = s.Dinv[c];
+                const size_t o = (size_t)(b0 + jj) + (size_t)(NB + i) * D.ld;
+                Dr_[jj * PS + i] = D.re[o];
+                Di_[jj * PS + i] = D.im[o];
+            }
         __syncthreads();
-        if (tid < RB) {
+        if (s.Dinv) {   // B_slab <- U11⁻¹ B_slab
+            const int q = tid % RB, j = tid / RB;
+            double sr = 0.0, si = 0.0;
+            if (j < bw)
+                for (int i = j; i < bw; ++i) {
+                    double tr_, ti_;
+                    cmul(Dr_[j * PS + i], Di_[j * PS + i], Br[(b0 + i) * RB + q], Bi[(b0 + i) * RB + q], tr_, ti_);
+                    sr += tr_;
+                    si += ti_;
+                }
+            __syncthreads();
+            if (j < bw) {
+                Br[(b0 + j) * RB + q] = sr;
+                Bi[(b0 + j) * RB + q] = si;
+            }
+        } else if (tid < RB) {
             const int q = tid;
             for (int j = bw - 1; j >= 0; --j) {
                 double sr = Br[(b0 + j) * RB + q], si = Bi[(b0 + j) * RB + q];
@@ -538,7 +708,7 @@ size_t factor_lds_bytes(int which, int max_n) {
     const size_t n = (size_t)max_n;
     switch (which) {
         case 0: return 2 * n * PS * sizeof(double);                 // k_chol
-        case 1: return (2 * n * RB + 2 * n * PS) * sizeof(double);  // k_trsm_rlh, k_lu_solve
+        case 1: return (2 * n * RB + 2 * n * PS + 4 * NB * PS) * sizeof(double);  // k_trsm_rlh, k_lu_solve
         case 2: return 4 * n * PS * sizeof(double);                 // k_lu
         default: return 0;
     }
@@ -547,7 +717,9 @@ size_t factor_lds_bytes(int which, int max_n) {
 hipError_t launch_chol(hipStream_t st, int nchains, int max_n, const CholArgs &a) {
     const size_t b = factor_lds_bytes(0, max_n);
     if (hipError_t e = allow_lds(k_chol, b)) return e;
-    hipLaunchKernelGGL(k_chol, dim3(nchains), dim3(NT), b, st, a);
+    hipLaunchKernelGGL(k_chol, dim3(nchains), dim3(NF), b, st, a);
+    if (a.Dinv) hipLaunchKernelGGL(k_diag_inv<false>, dim3((max_n + NB - 1) / NB, nchains), dim3(NT), 0, st,
+                                   a.chains, a.A, a.ndim, a.Dinv);
     return hipGetLastError();
 }
 
@@ -562,6 +734,8 @@ hipError_t launch_lu(hipStream_t st, int nchains, int max_n, const LuArgs &a) {
     const size_t b = factor_lds_bytes(2, max_n);
     if (hipError_t e = allow_lds(k_lu, b)) return e;
     hipLaunchKernelGGL(k_lu, dim3(nchains), dim3(NT), b, st, a);
+    if (a.Dinv) hipLaunchKernelGGL(k_diag_inv<true>, dim3((max_n + NB - 1) / NB, nchains), dim3(NT), 0, st,
+                                   a.chains, a.A, a.ndim, a.Dinv);
     return hipGetLastError();
 }
 

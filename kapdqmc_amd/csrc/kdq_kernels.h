@@ -98,6 +98,8 @@ struct CholArgs {
     double shift;
     int *info;
     double *logdiag;   // when set, logdiag[c] += Σ_j log L_jj (the log|det| of the factor)
+    const Opnd *Dinv;  // when set: the inverses of the 16 x 16 diagonal blocks of L, block b at
+                       // rows 16b.., columns 0..15
 };
 hipError_t launch_chol(hipStream_t st, int nchains, int max_n, const CholArgs &a);
 
@@ -107,6 +109,7 @@ struct TrsmArgs {
     const Opnd *X;
     const Opnd *L;
     int mdim, ndim;
+    const Opnd *Dinv;  // when set: the diagonal-block inverses written by k_chol
 };
 hipError_t launch_trsm_rlh(hipStream_t st, int nchains, int max_m, int max_n, const TrsmArgs &t);
 
@@ -120,6 +123,8 @@ struct LuArgs {
     double *logdet;
     int *info;
     double *phase;   // [2 per chain] det A / |det A| (re, im), or null
+    const Opnd *Dinv;  // when set: inverses of the 16 x 16 diagonal blocks, unit lower L at
+                       // columns 0..15 and upper U at columns 16..31 (block b at rows 16b..)
 };
 hipError_t launch_lu(hipStream_t st, int nchains, int max_n, const LuArgs &a);
 
@@ -131,6 +136,7 @@ struct LuSolveArgs {
     const Opnd *X;
     const int *piv;
     int ndim, rdim, piv_ld, pad;
+    const Opnd *Dinv;  // when set: the diagonal-block inverses written by k_lu
 };
 hipError_t launch_lu_solve(hipStream_t st, int nchains, int max_n, int max_r, const LuSolveArgs &s);
 size_t factor_lds_bytes(int which, int max_n);
