@@ -31,23 +31,34 @@ template <> KDQ_DEV int acc_col<float>(int lane, int v) { return 4 * (lane >> 4)
 
 // PF: mode 2 with its Cin entries loaded before the k loop (all at once, overlapping the first
 // k-tile) instead of one by one between the epilogue's stores.
+// Workgroups are dealt to the 8 XCDs round-robin by linear id. The launch is one-dimensional
+// and linear id L is given tile g of the chain-major (chain, row tile, column tile) order, with
+// each XCD taking one contiguous run of g: every XCD works through its own chains, whose
+// operands are then fetched into that XCD's L2 only (a bijection for any tile count).
+constexpr int NXCD = 8;
+KDQ_DEV int xcd_tile(int L, int total) {
+    const int x = L % NXCD, k = L / NXCD, q = total / NXCD, r = total % NXCD;
+    return x * q + min(x, r) + k;
+}
+
 template <class T, int AC, int BC, int OPA, int OPB, int BM, int BN, bool PF>
-__global__ __launch_bounds__(256) void k_gemm(GemmArgsT<T> g) {
+__global__ __launch_bounds__(256) void k_gemm(GemmArgsT<T> g, int tx, int ty) {
     typedef typename Acc<T>::type VA;
     constexpr bool OC = AC || BC;
     constexpr int WMB = BM / 32, WNB = BN / 32;   // 16x16 MFMA blocks per wave (each dim)
     constexpr int LPA = BM + 2, LPB = BN + 2;
     constexpr int EA = (BM * BK) / 256, EB = (BN * BK) / 256;   // staged elements per thread
-    const int c = blockIdx.z;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x), c = tile / (tx * ty);
     const ChainDev &ch = g.chains[c];
     const int M = dim_of(ch, g.mdim), N = dim_of(ch, g.ndim), K = dim_of(ch, g.kdim);
-    const int i0 = blockIdx.y * BM, j0 = blockIdx.x * BN;
+    const int i0 = (tile % (tx * ty)) / tx * BM, j0 = tile % tx * BN;
     if (i0 >= M || j0 >= N) return;
     const auto A = g.A[c], B = g.B[c];
 
     __shared__ T As[2][AC ? 2 : 1][BK][LPA];
     __shared__ T Bs[2][BC ? 2 : 1][BK][LPB];
-    __shared__ T Sk[2][KMAX];
+    extern __shared__ __align__(16) char gemm_dyn[];   // k-index factors (only when scaled)
+    T *const Sk0 = reinterpret_cast<T *>(gemm_dyn), *const Sk1 = Sk0 + K;
     __shared__ T Sr[2][BM], Sc[2][BN];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -58,8 +69,8 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgsT<T> g) {
         for (int k = tid; k < K; k += 256) {
             double dr, di;
             diag_value(ch, g.inS, k, dr, di);
-            Sk[0][k] = (T)dr;
-            Sk[1][k] = (T)di;
+            Sk0[k] = (T)dr;
+            Sk1[k] = (T)di;
         }
     if (g.rowS.kind && tid < BM) {
         double dr = 1.0, di = 0.0;
@@ -134,7 +145,7 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgsT<T> g) {
             if (OPA == 0) { ii = e % BM; kk = e / BM; }
             else { kk = e % BK; ii = e / BK; }
             T vr = ra[q], vi = rai[q];
-            if (AC && scale_k && k0 + kk < K) cmulT(vr, vi, Sk[0][k0 + kk], Sk[1][k0 + kk], vr, vi);
+            if (AC && scale_k && k0 + kk < K) cmulT(vr, vi, Sk0[k0 + kk], Sk1[k0 + kk], vr, vi);
             As[buf][0][kk][ii] = vr;
             if (AC) As[buf][1][kk][ii] = vi;
         }
@@ -145,7 +156,7 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgsT<T> g) {
             if (OPB == 0) { kk = e % BK; jj = e / BK; }
             else { jj = e % BN; kk = e / BN; }
             T vr = rb[q], vi = rbi[q];
-            if (BC && !AC && scale_k && k0 + kk < K) cmulT(vr, vi, Sk[0][k0 + kk], Sk[1][k0 + kk], vr, vi);
+            if (BC && !AC && scale_k && k0 + kk < K) cmulT(vr, vi, Sk0[k0 + kk], Sk1[k0 + kk], vr, vi);
             Bs[buf][0][kk][jj] = vr;
             if (BC) Bs[buf][1][kk][jj] = vi;
         }
@@ -261,16 +272,17 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgsT<T> g) {
 template <class T, int AC, int BC, int OPA, int OPB, bool PF = false>
 hipError_t run(hipStream_t st, int nchains, int max_m, int max_n, int max_k, const GemmArgsT<T> &g, int tile) {
     if (g.inS.kind && max_k > KMAX) return hipErrorInvalidValue;
+    const size_t dyn = g.inS.kind ? 2 * sizeof(T) * (size_t)max_k : 0;
     // 32 x 32 tiles (more workgroups in flight hide the operand latency better: 64 batched
     // N = 288 wraps run in ~90 µs against ~180 µs with 64 x 64) unless the 32 x 32 grid is
     // very large, or as forced by `tile`
     const long t32 = (long)((max_n + 31) / 32) * ((max_m + 31) / 32) * nchains;
     if (tile == 64 || (tile == 0 && t32 > 32768)) {
-        dim3 grid((max_n + 63) / 64, (max_m + 63) / 64, nchains);
-        hipLaunchKernelGGL((k_gemm<T, AC, BC, OPA, OPB, 64, 64, PF>), grid, dim3(256), 0, st, g);
+        const int tx = (max_n + 63) / 64, ty = (max_m + 63) / 64;
+        hipLaunchKernelGGL((k_gemm<T, AC, BC, OPA, OPB, 64, 64, PF>), dim3(tx * ty * nchains), dim3(256), dyn, st, g, tx, ty);
     } else {
-        dim3 grid((max_n + 31) / 32, (max_m + 31) / 32, nchains);
-        hipLaunchKernelGGL((k_gemm<T, AC, BC, OPA, OPB, 32, 32, PF>), grid, dim3(256), 0, st, g);
+        const int tx = (max_n + 31) / 32, ty = (max_m + 31) / 32;
+        hipLaunchKernelGGL((k_gemm<T, AC, BC, OPA, OPB, 32, 32, PF>), dim3(tx * ty * nchains), dim3(256), dyn, st, g, tx, ty);
     }
     return hipGetLastError();
 }
