@@ -136,8 +136,16 @@ int kdq_get_counters(kdq_handle h, double *out);
  *                     (the current G is G0 - U V); each block reads its corrected k x k
  *                     submatrix and appends its rank-|A| update to U and V, and G is rewritten
  *                     once per slice by one GEMM of the accumulated rank. Same Markov chain;
- *                     G agrees to rounding. */
-enum { KDQ_OPT_GRAPH = 0, KDQ_OPT_DELAY = 1, KDQ_OPT_GEMM_TILE = 2, KDQ_OPT_DEFER = 3 };
+ *                     G agrees to rounding.
+ *  KDQ_OPT_PIPE       t-U chains with the per-block flush, in captured sweeps. 1 (default):
+ *                     the decisions of block b+1 are issued on a second stream as soon as
+ *                     block b's flush operands exist; they read their k x k submatrix from
+ *                     the G before that flush minus its rank-|A| product (restricted to the
+ *                     block), so they do not wait for the flush to rewrite G. 0: decisions
+ *                     and flushes alternate on one stream. Same Markov chain; G agrees to
+ *                     rounding. Sweeps with host launches (graph option 0, kernel timing on,
+ *                     kdq_update_layer) always use the one-stream order. */
+enum { KDQ_OPT_GRAPH = 0, KDQ_OPT_DELAY = 1, KDQ_OPT_GEMM_TILE = 2, KDQ_OPT_DEFER = 3, KDQ_OPT_PIPE = 4 };
 int kdq_set_option(kdq_handle h, int32_t key, int32_t value);
 
 /* Mixed precision (kdq_run_desc.precision = KDQ_PREC_MIXED, t-U): G is carried in FP32

@@ -47,7 +47,7 @@ struct ChainDev {
     uint64_t seed;
     uint32_t chain_id, stream;   // Philox stream: 0 plain sweep, 1 + replica in SCEE
     // local-update scratch
-    int *acc_cnt;          // [1]
+    int *acc_cnt;          // [2]: accepted count of a block, slot by block parity (t-U sweep)
     int *acc_idx;          // [64]
     double *W;             // 2 planes of 64 x 64 (re, im), ld 64
     double *counters;      // [2]: proposals, accepted

@@ -42,7 +42,7 @@ __global__ __launch_bounds__(KBT >= 32 ? 1024 : KBT * KBT) void k_decide_tu(Deci
     const int tid = threadIdx.x;
     if (kb <= 0) {
         if (tid == 0) {
-            *ch.acc_cnt = 0;
+            ch.acc_cnt[a.slot] = 0;
             if (a.deferred) {
                 const int rk = a.blk > 0 ? ch.rk[a.blk] : 0;
                 ch.rk[a.blk + 1] = rk;
@@ -78,24 +78,28 @@ __global__ __launch_bounds__(KBT >= 32 ? 1024 : KBT * KBT) void k_decide_tu(Deci
         Wr[q] = 0.0;
         Wi[q] = 0.0;
     }
-    // deferred flush: subtract U[S, :r] V[:r, S] (chunks of DCH ranks staged in LDS)
-    if (a.deferred && a.blk > 0) {
+    // a rank-rk product not yet applied to G: subtract U[S, :rk] V[:rk, S] (chunks of DCH ranks
+    // staged in LDS). Deferred flush: the slice's factors (V transposed, V[t, j] at j + t·ld);
+    // pipelined sweep: the previous block's Cs and Zs (Zs rank-major, Zs[t, j] at t + j·ld)
+    if ((a.deferred && a.blk > 0) || a.pre) {
         constexpr int DCH = 16;
         __shared__ double Usr[KBT][DCH + 1], Usi[KBT][DCH + 1], Vsr[DCH][KBT + 1], Vsi[DCH][KBT + 1];
-        const int rk = ch.rk[a.blk];
+        const int rk = a.pre ? ch.acc_cnt[1 - a.slot] : ch.rk[a.blk];
+        const OpndF *Uf_ = a.pre ? a.Pcf : a.Uf, *Vf_ = a.pre ? a.Pzf : a.Vf;
+        const Opnd *U_ = a.pre ? a.Pc : a.U, *V_ = a.pre ? a.Pz : a.V;
         for (int t0 = 0; t0 < rk; t0 += DCH) {
             const int tc = min(DCH, rk - t0);
             for (int e = tid; e < KBT * DCH; e += NTH) {
                 const int i = e % KBT, t = e / KBT;
                 double ur = 0.0, ui = 0.0;
                 if (i < kb && t < tc) {
-                    if (a.Uf) {
-                        const OpndF U = a.Uf[c];
+                    if (Uf_) {
+                        const OpndF U = Uf_[c];
                         const size_t o = (size_t)(s0 + i) + (size_t)(t0 + t) * U.ld;
                         ur = U.re[o];
                         ui = U.im[o];
                     } else {
-                        const Opnd U = a.U[c];
+                        const Opnd U = U_[c];
                         const size_t o = (size_t)(s0 + i) + (size_t)(t0 + t) * U.ld;
                         ur = U.re[o];
                         ui = U.im[o];
@@ -107,15 +111,17 @@ __global__ __launch_bounds__(KBT >= 32 ? 1024 : KBT * KBT) void k_decide_tu(Deci
             for (int e = tid; e < DCH * KBT; e += NTH) {
                 const int j = e % KBT, t = e / KBT;
                 double vr = 0.0, vi = 0.0;
-                if (j < kb && t < tc) {   // V is stored transposed: V[t, j] at j + t·ld
-                    if (a.Vf) {
-                        const OpndF V = a.Vf[c];
-                        const size_t o = (size_t)(s0 + j) + (size_t)(t0 + t) * V.ld;
+                if (j < kb && t < tc) {
+                    if (Vf_) {
+                        const OpndF V = Vf_[c];
+                        const size_t o = a.pre ? (size_t)(t0 + t) + (size_t)(s0 + j) * V.ld
+                                               : (size_t)(s0 + j) + (size_t)(t0 + t) * V.ld;
                         vr = V.re[o];
                         vi = V.im[o];
                     } else {
-                        const Opnd V = a.V[c];
-                        const size_t o = (size_t)(s0 + j) + (size_t)(t0 + t) * V.ld;
+                        const Opnd V = V_[c];
+                        const size_t o = a.pre ? (size_t)(t0 + t) + (size_t)(s0 + j) * V.ld
+                                               : (size_t)(s0 + j) + (size_t)(t0 + t) * V.ld;
                         vr = V.re[o];
                         vi = V.im[o];
                     }
@@ -254,7 +260,7 @@ __global__ __launch_bounds__(KBT >= 32 ? 1024 : KBT * KBT) void k_decide_tu(Deci
     }
     // outputs
     if (tid == 0) {
-        *ch.acc_cnt = na;
+        ch.acc_cnt[a.slot] = na;
         ch.counters[0] += kb;
         ch.counters[1] += na;
         if (a.deferred) {
@@ -291,7 +297,7 @@ template <class T>
 __global__ __launch_bounds__(256) void k_flush_prep(FlushArgsT<T> f) {
     const int c = blockIdx.y;
     const ChainDev &ch = f.chains[c];
-    const int n = ch.ns, na = *ch.acc_cnt;
+    const int n = ch.ns, na = ch.acc_cnt[f.slot];
     const int j0 = blockIdx.x * PC;
     if (na == 0 || j0 >= n) return;   // rank 0: the GEMM copies Xin
     const auto Gi = f.Gin[c];
@@ -392,7 +398,7 @@ __global__ __launch_bounds__(256) void k_flush_apply(FlushArgsT<T> f, int nch, i
             ar[a][b] = T(0);
             ai[a][b] = T(0);
         }
-    const int na = *ch.acc_cnt;
+    const int na = ch.acc_cnt[f.slot];
     for (int t0 = 0; t0 < na || t0 == 0; t0 += AK) {
         // the loads do not wait for the count: ranks t0 .. t0 + 15 are inside the 64-rank
         // scratch whatever it is, and entries at or beyond the count are zeroed by a select
@@ -616,7 +622,8 @@ hipError_t defer_t(hipStream_t st, int nchains, int max_ns, const DeferArgsT<T> 
 }  // namespace
 
 hipError_t launch_decide_tu(hipStream_t st, int nchains, const DecideArgs &a) {
-    if (a.kb < 1 || a.kb > KB) return hipErrorInvalidValue;
+    if (a.kb < 1 || a.kb > KB || a.slot < 0 || a.slot > 1) return hipErrorInvalidValue;
+    if (a.pre && (a.Gf ? !a.Pcf || !a.Pzf : !a.Pc || !a.Pz)) return hipErrorInvalidValue;
     if (a.kb <= 8) hipLaunchKernelGGL(k_decide_tu<8>, dim3(nchains), dim3(64), 0, st, a);
     else if (a.kb <= 16) hipLaunchKernelGGL(k_decide_tu<16>, dim3(nchains), dim3(256), 0, st, a);
     else if (a.kb <= 32) hipLaunchKernelGGL(k_decide_tu<32>, dim3(nchains), dim3(1024), 0, st, a);
@@ -625,15 +632,18 @@ hipError_t launch_decide_tu(hipStream_t st, int nchains, const DecideArgs &a) {
 }
 
 template <class T>
-hipError_t flush_t(hipStream_t st, int nchains, int max_ns, const FlushArgsT<T> &f) {
-    if (f.kmax < 1 || f.kmax > KB || !f.Zs || !f.Cs) return hipErrorInvalidValue;
-    const size_t K = (size_t)f.kmax;
-    const size_t bytes = (2 * K * K + 2 * K * PC) * sizeof(T);
-    if (hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_flush_prep<T>),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes))
-        return e;
-    hipLaunchKernelGGL(k_flush_prep<T>, dim3((max_ns + PC - 1) / PC, nchains), dim3(256), bytes, st, f);
-    if (hipError_t e = hipGetLastError()) return e;
+hipError_t flush_t(hipStream_t st, int nchains, int max_ns, const FlushArgsT<T> &f, int part = 3) {
+    if (f.kmax < 1 || f.kmax > KB || !f.Zs || !f.Cs || f.slot < 0 || f.slot > 1) return hipErrorInvalidValue;
+    if (part & 1) {
+        const size_t K = (size_t)f.kmax;
+        const size_t bytes = (2 * K * K + 2 * K * PC) * sizeof(T);
+        if (hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_flush_prep<T>),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes))
+            return e;
+        hipLaunchKernelGGL(k_flush_prep<T>, dim3((max_ns + PC - 1) / PC, nchains), dim3(256), bytes, st, f);
+        if (hipError_t e = hipGetLastError()) return e;
+    }
+    if (!(part & 2)) return hipSuccess;
     const int tx = (max_ns + AN - 1) / AN, ty = (max_ns + AM - 1) / AM;
     if (nchains >= NXCD) {
         const int groups = (nchains + NXCD - 1) / NXCD;
@@ -650,6 +660,14 @@ hipError_t launch_flush(hipStream_t st, int nchains, int max_ns, const FlushArgs
 
 hipError_t launch_flush_f32(hipStream_t st, int nchains, int max_ns, const FlushArgsF &f) {
     return flush_t<float>(st, nchains, max_ns, f);
+}
+
+hipError_t launch_flush_part(hipStream_t st, int nchains, int max_ns, const FlushArgs &f, int part) {
+    return flush_t<double>(st, nchains, max_ns, f, part);
+}
+
+hipError_t launch_flush_part(hipStream_t st, int nchains, int max_ns, const FlushArgsF &f, int part) {
+    return flush_t<float>(st, nchains, max_ns, f, part);
 }
 
 hipError_t launch_defer_xy(hipStream_t st, int nchains, int max_ns, const DeferArgsT<double> &a) {

@@ -2,7 +2,9 @@
 
 Every replay is checked: several sweeps in one call (host reads of the measurement rows
 between replays), then more calls with eager work (G read-back, chain re-initialisation)
-between them; t-U, t-V and t-U-V, plus a batch of uneven chains."""
+between them; t-U, t-V and t-U-V, plus a batch of uneven chains. Bit for bit in the
+one-stream block order; with the pipelined t-U blocks (graphs only) the fields bit for bit
+and G / the rows to rounding."""
 import numpy as np
 import pytest
 
@@ -33,9 +35,10 @@ def tuv(L=3):
                             BatchSize=5)
 
 
-def run(dq, models, graph: bool, delay=8):
+def run(dq, models, graph: bool, delay=8, pipe=0):
     b = dq.DQMCBatch(models, [9 + c for c in range(len(models))], list(range(len(models))), delay=delay)
     b.set_option("graph", int(graph))
+    b.set_option("pipe", pipe)
     b.initial_s()
     out = [b.phy_update(3, True)]
     for _ in range(2):
@@ -58,3 +61,16 @@ def test_graph_replays_equal_eager(dq, kind):
         assert np.array_equal(a, b)
     for a, b in zip(g1, g0):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("kind", ["tU", "tU_batch"])
+def test_pipelined_graph_replays_against_eager(dq, kind):
+    print(f"case: pipelined graph vs eager {kind}", flush=True)
+    models = {"tU": [tu()], "tU_batch": [tu(3), tu(4, 3.0), tu(6, 5.0)]}[kind]
+    r1, f1, g1 = run(dq, models, True, pipe=1)
+    r0, f0, g0 = run(dq, models, False, pipe=1)
+    assert np.allclose(r1, r0, rtol=1e-9, atol=1e-11)
+    for a, b in zip(f1, f0):
+        assert np.array_equal(a, b)
+    for a, b in zip(g1, g0):
+        assert np.abs(a - b).max() <= 1e-10 * np.abs(b).max()

@@ -68,6 +68,7 @@ def test_mixed_graph_equals_eager(dq):
     for graph in (1, 0):
         b = dq.DQMCBatch([m, model(3, 3.0)], [5, 6], [0, 1], delay=16, precision="mixed")
         b.set_option("graph", graph)
+        b.set_option("pipe", 0)   # bit-identical order; the pipelined graph: test_gpu_pipe.py
         b.initial_s()
         rows = np.concatenate([b.phy_update(2, True), b.phy_update(1, True)], axis=1)
         out.append((rows, [b.get_fields(c) for c in range(2)], [b.get_G(c) for c in range(2)]))

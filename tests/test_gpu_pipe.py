@@ -1,0 +1,85 @@
+"""Pipelined t-U blocks (the decisions of block b+1 on a second stream beside block b's
+flush, reading their submatrix as G minus that flush's product) against the one-stream
+order on the same batch: identical Philox draws, so the fields must agree bit for bit and
+G / the measurement rows to rounding, for block sizes 1..64, FP64 and mixed precision, a
+batch of uneven lattices, and graph replays (where the pipelining applies) against host
+launches (which keep the one-stream order). The oracle comparison of
+the default (pipelined) path is in test_gpu_tu.py."""
+import numpy as np
+import pytest
+
+from kapdqmc_amd.models import tU_Hubbard_Para
+
+pytestmark = pytest.mark.gpu
+
+
+def tu(L, U=4.0, theta=1.0, bs=5):
+    return tU_Hubbard_Para(Ht=1.0, Hu1=U, Hu2=U, Δt=0.1, Θrelax=theta, Θquench=0.0,
+                           Lattice="HoneyComb120", site=[L, L], BatchSize=bs)
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def run(dqmc, models, delay, pipe, sweeps=2, graph=1, precision="fp64"):
+    b = dqmc.DQMCBatch(models, list(range(7, 7 + len(models))), list(range(len(models))), delay=delay,
+                       precision=precision)
+    b.set_option("pipe", pipe)
+    b.set_option("graph", graph)
+    b.initial_s()
+    rows = b.phy_update(sweeps, True)
+    out = ([b.get_fields(c) for c in range(b.n)], [b.get_G(c) for c in range(b.n)], rows)
+    b.close()
+    return out
+
+
+@pytest.mark.parametrize("delay", [1, 5, 16, 32, 64])
+def test_pipe_equals_one_stream(delay):
+    from kapdqmc_amd import dqmc
+    print(f"case: pipelined vs one-stream blocks, delay={delay}", flush=True)
+    models = [tu(3), tu(4, U=2.0), tu(6, U=6.0)]
+    f1, g1, r1 = run(dqmc, models, delay, 1)
+    f0, g0, r0 = run(dqmc, models, delay, 0)
+    for c in range(len(models)):
+        assert np.array_equal(f1[c], f0[c]), c
+        assert rel(g1[c], g0[c]) < 1e-10, (c, rel(g1[c], g0[c]))
+    assert np.allclose(r1, r0, rtol=1e-9, atol=1e-11)
+
+
+def test_pipe_mixed_precision():
+    from kapdqmc_amd import dqmc
+    print("case: pipelined vs one-stream blocks, mixed precision", flush=True)
+    models = [tu(4), tu(6, U=2.0)]
+    f1, g1, r1 = run(dqmc, models, 16, 1, precision="mixed")
+    f0, g0, r0 = run(dqmc, models, 16, 0, precision="mixed")
+    for c in range(len(models)):
+        assert np.array_equal(f1[c], f0[c]), c
+        assert rel(g1[c], g0[c]) < 1e-5, (c, rel(g1[c], g0[c]))
+    assert np.allclose(r1, r0, rtol=1e-4, atol=1e-6)
+
+
+def test_pipe_graph_against_host_launches():
+    # host-launched sweeps always take the one-stream order
+    from kapdqmc_amd import dqmc
+    print("case: pipelined graph vs host launches", flush=True)
+    models = [tu(4), tu(3)]
+    f1, g1, r1 = run(dqmc, models, 16, 1, sweeps=3, graph=1)
+    f0, g0, r0 = run(dqmc, models, 16, 1, sweeps=3, graph=0)
+    for c in range(len(models)):
+        assert np.array_equal(f1[c], f0[c])
+        assert rel(g1[c], g0[c]) < 1e-10, (c, rel(g1[c], g0[c]))
+    assert np.allclose(r1, r0, rtol=1e-9, atol=1e-11)
+
+
+def test_pipe_graph_replays_repeat():
+    # two batches with the same seeds, captured and replayed: bit-identical
+    from kapdqmc_amd import dqmc
+    print("case: pipelined graph replays, two identical batches", flush=True)
+    models = [tu(4), tu(6, U=2.0)]
+    f1, g1, r1 = run(dqmc, models, 32, 1, sweeps=3)
+    f0, g0, r0 = run(dqmc, models, 32, 1, sweeps=3)
+    for c in range(len(models)):
+        assert np.array_equal(f1[c], f0[c])
+        assert np.array_equal(g1[c], g0[c])
+    assert np.array_equal(r1, r0)
