@@ -46,7 +46,10 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgsT<T> g, int tx, int ty) {
     typedef typename Acc<T>::type VA;
     constexpr bool OC = AC || BC;
     constexpr int WMB = BM / 32, WNB = BN / 32;   // 16x16 MFMA blocks per wave (each dim)
-    constexpr int LPA = BM + 2, LPB = BN + 2;
+    // LDS row strides: FP64 odd, so that the transposed staging stores (16 lanes of 8 bytes
+    // down one column) hit distinct banks; FP32 2 mod 4 (32 lanes over two columns)
+    constexpr int PAD = sizeof(T) == 8 ? 1 : 2;
+    constexpr int LPA = BM + PAD, LPB = BN + PAD;
     constexpr int EA = (BM * BK) / 256, EB = (BN * BK) / 256;   // staged elements per thread
     const int tile = xcd_tile(blockIdx.x, gridDim.x), c = tile / (tx * ty);
     const ChainDev &ch = g.chains[c];
