@@ -197,6 +197,16 @@ int kdq_scee_get_G4(kdq_handle h, int32_t c, int32_t replica, double *Gt, double
 /* |det g_A|² and |det g_B|² (nchains x 2; 1 for an empty region). */
 int kdq_scee_detg(kdq_handle h, double *out);
 
+/* Disorder operator of the t-U model (ctrl_SCDOPicr, src/tU/disorder_operate.jl:1): `nchains`
+ * one-replica chains weighted by W · |det X_A|^{2λ} |det X_B|^{2(1-λ)} with
+ * X = e^{iα} I + (1 - e^{iα}) G(Θ)[region] (get_ABX!, :202-223; DOPBuffer(alpha, index) of
+ * both regions). regions: as kdq_scee_create (λ, Nλ, 1-based site lists). The kdq_scee_*
+ * calls apply with replica 0 only: kdq_scee_sweeps returns per sweep the mean over slices of
+ * (det X_A / det X_B)^(1/Nλ) (:96-98), kdq_scee_detg |det X_A|², |det X_B|². The field draws
+ * from Philox stream 1. Replaces the buffer set-up of ctrl_SCDOPicr (:1-48). */
+int kdq_dop_create(const kdq_model_desc *models, const kdq_run_desc *runs, const kdq_scee_desc *regions,
+                   double alpha, int32_t nchains, kdq_handle *out);
+
 /* ---- kernel timing (no counterpart in the reference; used by the benchmark) ----
  * Launches of the kernel families whose bit is set in `mask` are bracketed by HIP events
  * on the handle's stream; kdq_profile_read returns the summed device time (ms) and the

@@ -1534,6 +1534,8 @@ int kdq_get_counters(kdq_handle h, double *out) {
 // G(Θ,τ) = G0t (ping-pong pairs), and the block products BM / BM⁻¹ of every node block.
 struct SceeState {
     int nch = 0, kb = 8;
+    int nrep = 2;                  // replicas per chain: 2 (Rényi-2) or 1 (disorder operator)
+    bool dop = false;
     int na_max = 0, nb_max = 0, nx_max = 0, m2_max = 0;   // m2: largest nA² + nB²
     Opnd *t_X[3][2] = {};          // G0, Gt0, G0t ping-pong tables over the 2·nch entries
     int cur[3] = {0, 0, 0};
@@ -1675,14 +1677,16 @@ SceeArgs scee_args(kdq_handle h) {
     return a;
 }
 
-// GroverMatrix! of A and B, |det|² and the inverses (get_ABGM!, src/tU/SCEE.jl:264-276)
+// GroverMatrix! of A and B, |det|² and the inverses (get_ABGM!, src/tU/SCEE.jl:264-276), or
+// X = e^{iα}I + (1 - e^{iα})G0[X,X], |det X|² and X⁻¹ of the disorder operator (get_ABX!,
+// src/tU/disorder_operate.jl:202-223)
 int scee_grover(kdq_handle h) {
     SceeState *S = h->scee;
     if (S->nx_max == 0) return KDQ_OK;
     SceeArgs a = scee_args(h);
     HIPCHK(launch_scee_gather(h->stream, S->nch, S->nx_max, a));
     const int dims[2] = {DIM_NA, DIM_NB}, nmax[2] = {S->na_max, S->nb_max};
-    for (int X = 0; X < 2; ++X) {
+    for (int X = 0; X < 2 && !S->dop; ++X) {
         if (nmax[X] == 0) continue;
         GemmArgs g{};
         g.chains = h->d_chains;
@@ -1690,7 +1694,10 @@ int scee_grover(kdq_handle h) {
         g.mdim = g.ndim = g.kdim = dims[X];
         HIPCHK(launch_gemm(h->stream, S->nch, nmax[X], nmax[X], nmax[X], true, true, 0, 0, g));
     }
-    HIPCHK(launch_scee_grover_form(h->stream, S->nch, S->m2_max, S->d_reg));
+    if (S->dop)
+        HIPCHK(launch_dop_form(h->stream, S->nch, S->m2_max, S->d_reg));
+    else
+        HIPCHK(launch_scee_grover_form(h->stream, S->nch, S->m2_max, S->d_reg));
     for (int X = 0; X < 2; ++X) {
         if (nmax[X] == 0) continue;
         LuArgs la{h->d_chains, S->t_C[X], dims[X], S->nx_max, S->d_piv, S->d_ld + X * S->nch, S->d_info,
@@ -1923,9 +1930,10 @@ int stable(kdq_handle h, Opnd **out, const std::vector<Opnd> &v) {
 
 }  // namespace
 
-extern "C" {
-int kdq_scee_create(const kdq_model_desc *models, const kdq_run_desc *runs, const kdq_scee_desc *regions,
-                    int32_t nchains, kdq_handle *out) {
+// The swap-replica handle (nrep = 2) or the disorder-operator handle (nrep = 1, field
+// potential α of X = e^{iα}I + (1 - e^{iα})G0).
+static int scee_create(const kdq_model_desc *models, const kdq_run_desc *runs, const kdq_scee_desc *regions,
+                int32_t nchains, int nrep, double alpha, kdq_handle *out) {
     if (!models || !runs || !regions || !out || nchains < 1)
         return fail(nullptr, KDQ_EARG, "kdq_scee_create: null argument or nchains < 1");
     *out = nullptr;
@@ -1956,20 +1964,22 @@ int kdq_scee_create(const kdq_model_desc *models, const kdq_run_desc *runs, cons
         if (scee_r2_lds_bytes(na_max + nb_max, ns_max) > 163840)
             return fail(nullptr, KDQ_EARG, "kdq_scee_create: lattice and regions too large for the rank-2 decision kernel");
     }
-    std::vector<kdq_model_desc> md(2 * nchains);
-    std::vector<kdq_run_desc> rd(2 * nchains);
-    std::vector<uint32_t> streams(2 * nchains);
-    for (int r = 0; r < 2; ++r)
+    std::vector<kdq_model_desc> md(nrep * nchains);
+    std::vector<kdq_run_desc> rd(nrep * nchains);
+    std::vector<uint32_t> streams(nrep * nchains);
+    for (int r = 0; r < nrep; ++r)
         for (int c = 0; c < nchains; ++c) {
             md[c + r * nchains] = models[c];
             rd[c + r * nchains] = runs[c];
             streams[c + r * nchains] = 1 + r;
         }
     kdq_handle h = nullptr;
-    if (int r = create_impl(md.data(), rd.data(), 2 * nchains, streams.data(), &h)) return r;
+    if (int r = create_impl(md.data(), rd.data(), nrep * nchains, streams.data(), &h)) return r;
     SceeState *S = new SceeState();
     h->scee = S;
     S->nch = nchains;
+    S->nrep = nrep;
+    S->dop = nrep == 1;
     S->kb = runs[0].delay;
     S->na_max = na_max;
     S->nb_max = nb_max;
@@ -1979,7 +1989,7 @@ int kdq_scee_create(const kdq_model_desc *models, const kdq_run_desc *runs, cons
     // four-matrix buffers and block products per entry
     std::vector<Opnd> vx[3][2];
     std::vector<std::vector<Opnd>> vbm(h->nn - 1), vbmi(h->nn - 1);
-    for (int e = 0; e < 2 * nchains; ++e) {
+    for (int e = 0; e < nrep * nchains; ++e) {
         const int n = h->ch[e].ns;
         const size_t nn = (size_t)n * n;
         for (int k = 0; k < 3; ++k)
@@ -2013,6 +2023,9 @@ int kdq_scee_create(const kdq_model_desc *models, const kdq_run_desc *runs, cons
         SceeRegion &R = S->reg[c];
         memset(&R, 0, sizeof(R));
         R.nA = g.nA; R.nB = g.nB; R.nlam = g.nlam; R.lambda = g.lambda;
+        R.dop = nrep == 1;
+        R.ea[0] = std::cos(alpha);
+        R.ea[1] = std::sin(alpha);
         std::vector<int> idx;
         for (int i = 0; i < g.nA; ++i) idx.push_back((int)g.indexA[i] - 1);
         for (int i = 0; i < g.nB; ++i) idx.push_back((int)g.indexB[i] - 1);
@@ -2040,7 +2053,7 @@ int kdq_scee_create(const kdq_model_desc *models, const kdq_run_desc *runs, cons
         }
         bad |= salloc(h, &R.detg, 2);
         bad |= salloc(h, &R.osum, 2);
-        for (int r = 0; r < 2; ++r) {
+        for (int r = 0; r < nrep; ++r) {
             h->cd_host[c + r * nchains].nra = g.nA;
             h->cd_host[c + r * nchains].nrb = g.nB;
         }
@@ -2076,14 +2089,28 @@ int kdq_scee_create(const kdq_model_desc *models, const kdq_run_desc *runs, cons
     return KDQ_OK;
 }
 
+extern "C" {
+int kdq_scee_create(const kdq_model_desc *models, const kdq_run_desc *runs, const kdq_scee_desc *regions,
+                    int32_t nchains, kdq_handle *out) {
+    return scee_create(models, runs, regions, nchains, 2, 0.0, out);
+}
+
+int kdq_dop_create(const kdq_model_desc *models, const kdq_run_desc *runs, const kdq_scee_desc *regions,
+                   double alpha, int32_t nchains, kdq_handle *out) {
+    if (!models || nchains < 1) return fail(nullptr, KDQ_EARG, "kdq_dop_create: null argument or nchains < 1");
+    for (int c = 0; c < nchains; ++c)
+        if (models[c].kind != KDQ_MODEL_TU) return fail(nullptr, KDQ_EARG, "kdq_dop_create: the disorder operator is for t-U chains");
+    return scee_create(models, runs, regions, nchains, 1, alpha, out);
+}
+
 int kdq_scee_set_fields(kdq_handle h, int32_t c, int32_t replica, const uint8_t *s) {
-    if (!h || !h->scee || replica < 0 || replica > 1 || c < 0 || c >= h->scee->nch)
+    if (!h || !h->scee || replica < 0 || replica >= h->scee->nrep || c < 0 || c >= h->scee->nch)
         return fail(h, KDQ_EARG, "kdq_scee_set_fields: bad argument");
     return kdq_set_fields(h, c + replica * h->scee->nch, s);
 }
 
 int kdq_scee_get_fields(kdq_handle h, int32_t c, int32_t replica, uint8_t *s) {
-    if (!h || !h->scee || replica < 0 || replica > 1 || c < 0 || c >= h->scee->nch)
+    if (!h || !h->scee || replica < 0 || replica >= h->scee->nrep || c < 0 || c >= h->scee->nch)
         return fail(h, KDQ_EARG, "kdq_scee_get_fields: bad argument");
     return kdq_get_fields(h, c + replica * h->scee->nch, s);
 }
@@ -2100,11 +2127,12 @@ int kdq_scee_sweeps(kdq_handle h, int32_t sweeps, double *O) {
 }
 
 int kdq_scee_get_G4(kdq_handle h, int32_t c, int32_t replica, double *Gt, double *G0, double *Gt0, double *G0t) {
-    if (!h || !h->scee || replica < 0 || replica > 1 || c < 0 || c >= h->scee->nch || !Gt || !G0 || !Gt0 || !G0t)
+    if (!h || !h->scee || replica < 0 || replica >= h->scee->nrep || c < 0 || c >= h->scee->nch || !Gt || !G0 ||
+        !Gt0 || !G0t)
         return fail(h, KDQ_EARG, "kdq_scee_get_G4: bad argument");
     const int e = c + replica * h->scee->nch, n = h->ch[e].ns;
     const size_t nn = (size_t)n * n;
-    std::vector<Opnd> tab(2 * h->scee->nch);
+    std::vector<Opnd> tab(h->nch);
     double *outs[4] = {Gt, G0, Gt0, G0t};
     const double *src[4];
     src[0] = h->ch[e].G[h->cur];

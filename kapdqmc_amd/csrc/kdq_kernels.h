@@ -203,7 +203,8 @@ hipError_t launch_bump(hipStream_t st, uint32_t *counter);   // *counter += 1 (i
 // matrices gmInv[X], |det g_X| products detg[X] and the running Rényi accumulator.
 // All matrices planar complex, column-major, ld = region size.
 struct SceeRegion {
-    int nA, nB, nlam, pad;
+    int nA, nB, nlam;
+    int dop;               // 1: disorder operator (one replica, X = e^{iα}I + (1-e^{iα})G0[X,X])
     double lambda;
     const int *idx;        // [nA + nB]
     double *M[2][2];       // [replica][region]
@@ -211,6 +212,7 @@ struct SceeRegion {
     double *C[2];          // [region] product, then Grover matrix
     double *detg;          // [2]
     double *osum;          // [2]: Σ (detg_A / detg_B)^(1/Nλ), count
+    double ea[2];          // e^{iα} of the disorder operator
 };
 // Entries: replica r of chain c is entry c + r·nch of the chain tables.
 struct SceeArgs {
@@ -225,6 +227,8 @@ hipError_t launch_scee_decide(hipStream_t st, const SceeArgs &a);
 size_t scee_lds_bytes(int ks, int nx);   // dynamic LDS of the decide kernel for block size ks
 hipError_t launch_scee_gather(hipStream_t st, int nch, int nx_max, const SceeArgs &a);
 hipError_t launch_scee_grover_form(hipStream_t st, int nch, int total, const SceeRegion *reg);
+// Disorder operator (get_ABX!, src/tU/disorder_operate.jl:202-223): C = e^{iα}I + (1-e^{iα})M[0]
+hipError_t launch_dop_form(hipStream_t st, int nch, int total, const SceeRegion *reg);
 hipError_t launch_scee_detg(hipStream_t st, int nch, const SceeRegion *reg, const double *logdet_a,
                             const double *logdet_b);
 hipError_t launch_scee_accum(hipStream_t st, int nch, const SceeRegion *reg);

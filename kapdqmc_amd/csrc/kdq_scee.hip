@@ -16,6 +16,12 @@
 // k_scee_decide therefore works on a block of KS sites of both replicas: Gt[S,S],
 // Gt0[S,X], G0t[X,S] (X = A ∪ B) in LDS, the region blocks M[r][X] of G0 and the inverse
 // Grover matrices in global memory, all kept current with the sequential rank-1 updates.
+//
+// The disorder operator (ctrl_SCDOPicr, src/tU/disorder_operate.jl:1-247) runs the same
+// kernel on one replica: UpdateSCDOPLayer! (:225-247) weighs a proposal with |Tau_A|^{2λ}
+// |Tau_B|^{2(1-λ)} of get_abTau! for DOPBuffer_ (src/public/Gupdate.jl:70-82),
+// a = (1 - e^{iα}) G0t[X, m] r, b = Gt0[m, X] X⁻¹, Tau = 1 + b a, and GMupdate! (:52-58)
+// updates X⁻¹ the same way as the inverse Grover matrix; no G0 region blocks are needed.
 #include "kdq_kernels.h"
 
 namespace {
@@ -45,8 +51,9 @@ __global__ __launch_bounds__(ST) void k_scee_decide(SceeArgs a) {
     const SceeRegion &R = a.reg[c];
     const int n = ch0.ns, s0 = a.s0, lt = a.lt;
     const int kb = min(a.kb, n - s0);
+    const int NR = R.dop ? 1 : 2;   // replicas
     if (kb <= 0) {
-        if (tid < 2) *a.chains[c + tid * nch].acc_cnt = 0;
+        if (tid < NR) *a.chains[c + tid * nch].acc_cnt = 0;
         return;
     }
     const int nA = R.nA, nB = R.nB, nX = nA + nB, NX = a.nx_max;
@@ -71,7 +78,7 @@ __global__ __launch_bounds__(ST) void k_scee_decide(SceeArgs a) {
 #define PD(k, r, q) pd[((k) * 2 + (r)) * KS + (q)]
 
     const uint32_t tag = (uint32_t)lt | ((uint32_t)a.dir << 31);
-    if (tid < 2 * KS) {
+    if (tid < NR * KS) {
         const int r = tid / KS, q = tid % KS;
         accl[r][q] = 0;
         if (q < kb) {
@@ -96,7 +103,7 @@ __global__ __launch_bounds__(ST) void k_scee_decide(SceeArgs a) {
     for (int e = tid; e < 2 * KS * KS; e += ST) {
         const int r = e / (KS * KS), i = (e / KS) % KS, j = e % KS;
         double vr = 0.0, vi = 0.0;
-        if (i < kb && j < kb) {
+        if (i < kb && j < kb && r < NR) {
             const Opnd G = a.Gt[c + r * nch];
             const size_t o = (size_t)(s0 + i) + (size_t)(s0 + j) * G.ld;
             vr = G.re[o];
@@ -110,7 +117,7 @@ __global__ __launch_bounds__(ST) void k_scee_decide(SceeArgs a) {
     for (int e = tid; e < 2 * KS * nX; e += ST) {
         const int r = e / (KS * nX), rem = e % (KS * nX), x = rem / KS, i = rem % KS;
         double ar = 0.0, ai = 0.0, br = 0.0, bi = 0.0;
-        if (i < kb) {
+        if (i < kb && r < NR) {
             const Opnd T = a.Gt0[c + r * nch], O = a.G0t[c + r * nch];
             const int sx = R.idx[x];
             const size_t ot = (size_t)(s0 + i) + (size_t)sx * T.ld, oo = (size_t)sx + (size_t)(s0 + i) * O.ld;
@@ -120,9 +127,9 @@ __global__ __launch_bounds__(ST) void k_scee_decide(SceeArgs a) {
         t0r[T0(r, i, x)] = ar; t0i[T0(r, i, x)] = ai;
         o0r[O0(r, x, i)] = br; o0i[O0(r, x, i)] = bi;
     }
-    // region blocks of G0 of both replicas
+    // region blocks of G0 of both replicas (the Grover factors read the other one's)
     const int mA = nA * nA, mAB = mA + nB * nB;
-    for (int e = tid; e < 2 * mAB; e += ST) {
+    for (int e = tid; e < (R.dop ? 0 : 2 * mAB); e += ST) {
         const int r = e / mAB, f = e % mAB;
         const int X = f < mA ? 0 : 1, nr = X ? nB : nA, base = X ? nA : 0, g = X ? f - mA : f;
         const int p = g % nr, q = g / nr;
@@ -135,8 +142,9 @@ __global__ __launch_bounds__(ST) void k_scee_decide(SceeArgs a) {
     __syncthreads();
 
     int na[2] = {0, 0};
+    const double car = 1.0 - R.ea[0], cai = -R.ea[1];   // 1 - e^{iα} (disorder operator)
     for (int m = 0; m < kb; ++m) {
-        for (int r = 0; r < 2; ++r) {
+        for (int r = 0; r < NR; ++r) {
             const int o = 1 - r;
             const double Dr = PD(0, r, m), Di = PD(1, r, m);
             const double omr = 1.0 - gtr[GT(r, m, m)], omi = -gti[GT(r, m, m)];
@@ -151,7 +159,13 @@ __global__ __launch_bounds__(ST) void k_scee_decide(SceeArgs a) {
                 const int X = q < nA ? 0 : 1, base = X ? nA : 0, nr = X ? nB : nA, ql = q - base;
                 const double *Mo = R.M[o][X], *Moi = Mo + (size_t)nr * nr;
                 double sr = 0.0, si = 0.0;
-                if (r == 0) {
+                if (R.dop) {
+                    // zN = Gt0[m, X], a_q = (1 - e^{iα}) (G0t[X_q, m] r)
+                    vzr[q] = t0r[T0(r, m, q)]; vzi[q] = t0i[T0(r, m, q)];
+                    double tr_, ti_;
+                    cmul(o0r[O0(r, q, m)], o0i[O0(r, q, m)], rr, ri, tr_, ti_);
+                    cmul(car, cai, tr_, ti_, var_[q], vai[q]);
+                } else if (r == 0) {
                     // zN_q = Σ_k Gt0[m, X_k] (2 Mo[k, q] - δ)
                     for (int k = 0; k < nr; ++k) {
                         const double mr = 2.0 * Mo[k + (size_t)ql * nr] - (k == ql ? 1.0 : 0.0);
@@ -237,6 +251,7 @@ __global__ __launch_bounds__(ST) void k_scee_decide(SceeArgs a) {
                 double *Gv = R.gmInv[X];
                 Gv[p + (size_t)q * nr] -= ur;
                 Gv[(size_t)nr * nr + p + (size_t)q * nr] -= ui;
+                if (R.dop) continue;   // no region blocks of G0 in the disorder operator
                 double cr_, ci_, vr_, vi_;
                 cmul(cxr[base + p], cxi[base + p], rr, ri, cr_, ci_);
                 cmul(cr_, ci_, rxr[base + q], rxi[base + q], vr_, vi_);
@@ -299,7 +314,7 @@ __global__ __launch_bounds__(ST) void k_scee_decide(SceeArgs a) {
             ++na[r];
         }
     }
-    for (int r = 0; r < 2; ++r) {
+    for (int r = 0; r < NR; ++r) {
         const ChainDev &chr = a.chains[c + r * nch];
         if (tid == 0) {
             *chr.acc_cnt = na[r];
@@ -324,13 +339,14 @@ __global__ __launch_bounds__(ST) void k_scee_decide(SceeArgs a) {
 #undef PD
 }
 
-// M[r][X] = G0_r[X, X] for both replicas and regions (grid: elements x chains)
+// M[r][X] = G0_r[X, X] for both replicas (one with the disorder operator) and regions
+// (grid: elements x chains)
 __global__ void k_scee_gather(SceeArgs a) {
     const int c = blockIdx.y;
     const SceeRegion &R = a.reg[c];
     const int nA = R.nA, nB = R.nB, mA = nA * nA, mAB = mA + nB * nB;
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= 2 * mAB) return;
+    if (e >= (R.dop ? 1 : 2) * mAB) return;
     const int r = e / mAB, f = e % mAB;
     const int X = f < mA ? 0 : 1, nr = X ? nB : nA, base = X ? nA : 0, g = X ? f - mA : f;
     const int p = g % nr, q = g / nr;
@@ -355,6 +371,23 @@ __global__ void k_scee_grover_form(const SceeRegion *reg) {
     const double *M1 = R.M[0][X], *M2 = R.M[1][X];
     C[o] = 2.0 * C[o] - M1[o] - M2[o] + (p == q ? 1.0 : 0.0);
     C[pl + o] = 2.0 * C[pl + o] - M1[pl + o] - M2[pl + o];
+}
+
+// get_ABX! (src/tU/disorder_operate.jl:202-223): C = e^{iα} I + (1 - e^{iα}) G0[X, X]
+__global__ void k_dop_form(const SceeRegion *reg) {
+    const int c = blockIdx.y;
+    const SceeRegion &R = reg[c];
+    const int nA = R.nA, nB = R.nB, mA = nA * nA, mAB = mA + nB * nB;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= mAB) return;
+    const int X = e < mA ? 0 : 1, nr = X ? nB : nA, g = X ? e - mA : e;
+    const int p = g % nr, q = g / nr;
+    const size_t o = p + (size_t)q * nr, pl = (size_t)nr * nr;
+    const double *M = R.M[0][X];
+    double vr, vi;
+    cmul(1.0 - R.ea[0], -R.ea[1], M[o], M[pl + o], vr, vi);
+    R.C[X][o] = vr + (p == q ? R.ea[0] : 0.0);
+    R.C[X][pl + o] = vi + (p == q ? R.ea[1] : 0.0);
 }
 
 __global__ void k_scee_detg(const SceeRegion *reg, int nch, const double *la, const double *lb) {
@@ -421,6 +454,11 @@ hipError_t launch_scee_gather(hipStream_t st, int nch, int nx_max, const SceeArg
 hipError_t launch_scee_grover_form(hipStream_t st, int nch, int total, const SceeRegion *reg) {
     const int tot = total;   // largest nA² + nB² over the chains
     hipLaunchKernelGGL(k_scee_grover_form, dim3((tot + 255) / 256, nch), dim3(256), 0, st, reg);
+    return hipGetLastError();
+}
+
+hipError_t launch_dop_form(hipStream_t st, int nch, int total, const SceeRegion *reg) {
+    hipLaunchKernelGGL(k_dop_form, dim3((total + 255) / 256, nch), dim3(256), 0, st, reg);
     return hipGetLastError();
 }
 

@@ -284,7 +284,7 @@ class SCEEBatch:
     """
 
     def __init__(self, models, indexA, indexB, lam: float, nlam: int, seeds, chains=None,
-                 delay: int = 8, sweep0: int = 0):
+                 delay: int = 8, sweep0: int = 0, _dop_alpha=None):
         self.L = L.load()
         if not isinstance(models, (list, tuple)):
             models = [models]
@@ -303,7 +303,10 @@ class SCEEBatch:
             regs[c].indexB = ib.ctypes.data_as(C.POINTER(C.c_int64))
             regs[c].lam, regs[c].nlam = lam, nlam
         h = C.c_void_p()
-        L.check(self.L.kdq_scee_create(descs, runs, regs, n, C.byref(h)))
+        if _dop_alpha is None:
+            L.check(self.L.kdq_scee_create(descs, runs, regs, n, C.byref(h)))
+        else:
+            L.check(self.L.kdq_dop_create(descs, runs, regs, float(_dop_alpha), n, C.byref(h)))
         self.h, self.n = h, n
 
     def close(self):
@@ -353,6 +356,53 @@ class SCEEBatch:
         out = np.zeros((self.n, 2))
         self._chk(self.L.kdq_scee_detg(self.h, _dp(out)))
         return out
+
+
+class DOPBatch(SCEEBatch):
+    """Disorder operator of the t-U model (ctrl_SCDOPicr, src/tU/disorder_operate.jl:1-247)
+    for a batch of chains: one field per chain, sampled with W · |det X_A|^{2λ} |det X_B|^{2(1-λ)},
+    X = e^{iα}I + (1 - e^{iα})G(Θ)[region] (DOPBuffer(alpha, index), get_ABX!). sweeps()
+    returns per sweep the mean over slices of (det X_A / det X_B)^(1/Nλ); detg() gives
+    |det X_A|², |det X_B|². The field draws from Philox stream 1."""
+
+    def __init__(self, models, alpha: float, indexA, indexB, lam: float, nlam: int, seeds, chains=None,
+                 delay: int = 8, sweep0: int = 0):
+        super().__init__(models, indexA, indexB, lam, nlam, seeds, chains, delay, sweep0, _dop_alpha=alpha)
+        self.alpha = alpha
+
+    def set_fields(self, c: int, s):
+        super().set_fields(c, 0, s)
+
+    def get_fields(self, c: int) -> np.ndarray:
+        return super().get_fields(c, 0)
+
+    def G4(self, c: int):
+        return super().G4(c, 0)
+
+
+def ctrl_SCDOPicr(path: str, model, alpha: float, indexA, indexB, Sweeps: int, λ: float, Nλ: int, s,
+                  record: bool = False, *, seed: int = 0, chain: int = 0, delay: int = 8, sweep0: int = 0):
+    """ctrl_SCDOPicr on the GPU (src/tU/disorder_operate.jl:1-200): mutates the field s in
+    place, returns it, and appends the row [λ, O_1, …, O_Sweeps] to the reference's CSV file
+    name (:26-30) when ``record`` is true."""
+    b = DOPBatch([model], alpha, indexA, indexB, λ, Nλ, [seed], [chain], delay=delay, sweep0=sweep0)
+    try:
+        b.set_fields(0, s)
+        O = b.sweeps(Sweeps)[0]
+        s[...] = b.get_fields(0)
+    finally:
+        b.close()
+    if record:
+        name = _lattice_tag(model.Lattice, "tU")
+        if len(np.unique(model.α)) == 1:
+            fn = (f"tUSCDOP{name}_t{model.Ht}U{model.Hu1}size{model.site}Δt{model.Δt}Θ{model.Θrelax}"
+                  f"N{Nλ}BS{model.BatchSize}.csv")
+        else:
+            fn = (f"tUSCDOP{name}_t{model.Ht}U{model.Hu1}_{model.Hu2}size{model.site}Δt{model.Δt}"
+                  f"Θ{model.Θrelax}_{model.Θquench}N{Nλ}BS{model.BatchSize}.csv")
+        with open(os.path.join(path, fn), "a") as fh:
+            fh.write(",".join(repr(float(x)) for x in [λ, *O]) + "\n")
+    return s
 
 
 def _is_nested(x) -> bool:

@@ -108,6 +108,9 @@ def _declare(L):
     L.orc_scee_get_G4.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp]
     L.orc_scee_detg.argtypes = [C.c_void_p, C.c_int]
     L.orc_scee_detg.restype = C.c_double
+    L.orc_dop_new.argtypes = [C.c_void_p, C.c_double, C.c_int, _ip, C.c_int, _ip, C.c_double, C.c_int,
+                              C.POINTER(C.c_uint8)]
+    L.orc_dop_new.restype = C.c_void_p
     L.orc_scee2_new.argtypes = [C.c_int, C.c_void_p, C.c_int, _ip, C.c_int, _ip, C.c_double, C.c_int,
                                 C.POINTER(C.c_uint8), C.POINTER(C.c_uint8)]
     L.orc_scee2_new.restype = C.c_void_p
@@ -557,6 +560,22 @@ class OracleSCEE:
 
     def detg(self, which: int):
         return lib().orc_scee_detg(self.h, which)
+
+
+class OracleDOP(OracleSCEE):
+    """ctrl_SCDOPicr of the t-U model (src/tU/disorder_operate.jl): one field, regions A and B
+    with X = e^{iα}I + (1 - e^{iα})G0[region]; detg(which) returns |det X|². The field draws
+    from Philox stream 1 (as replica 0 of ctrl_SCEEicr)."""
+
+    def __init__(self, model, alpha: float, indexA, indexB, lam: float, nlam: int, s, seed: int = 0,
+                 chain: int = 0):
+        self.tu = OracleTU(model, seed=seed, chain=chain)
+        self.ns, self.nt = model.Ns, model.Nt
+        self._keep = [np.ascontiguousarray(indexA, dtype=np.int32), np.ascontiguousarray(indexB, dtype=np.int32),
+                      np.asfortranarray(s, dtype=np.uint8)]
+        iA, iB, a1 = self._keep
+        self.h = lib().orc_dop_new(self.tu.h, alpha, len(iA), _p(iA, C.c_int), len(iB), _p(iB, C.c_int), lam,
+                                   nlam, _p(a1, C.c_uint8))
 
 
 class OracleSCEE2:

@@ -166,6 +166,11 @@ void orc_scee_forward_slice(orc_scee *e, int lt, uint32_t sweep);
 void orc_scee_get_fields(orc_scee *e, int r, uint8_t *s);
 void orc_scee_get_G4(orc_scee *e, int r, zdouble *Gt, zdouble *G0, zdouble *Gt0, zdouble *G0t);
 double orc_scee_detg(orc_scee *e, int which);
+/* Disorder operator ctrl_SCDOPicr of the t-U model (src/tU/disorder_operate.jl:1-247): the
+ * orc_scee driver on one replica with X_A = e^{iα}I + (1-e^{iα})G0[A,A]; the functions above
+ * apply (replica 0; orc_scee_detg returns |det X_A|², |det X_B|²). */
+orc_scee *orc_dop_new(orc_tu *m, double alpha, int nA, const int *indexA, int nB, const int *indexB,
+                      double lambda, int nlam, const uint8_t *s);
 
 /* ---------------- t-V / t-U-V swap-replica Rényi-2 (src/tV/SCEE.jl, src/tUV/SCEE.jl) ---
  * kind 1: t-V (model = orc_tv *), kind 2: t-U-V (model = orc_tuv *); fields in the model's

@@ -7,6 +7,10 @@
  * Two replicas (fields ss[1], ss[2]) share the orc_tu model constants; each replica holds
  * G(τ), G(Θ), G(τ,Θ), G(Θ,τ) and its block products, and the Grover matrices of regions A
  * and B are tracked through their inverses and |det|.
+ * The disorder operator of the t-U model (orc_dop_new) runs the same driver on one replica:
+ *   src/tU/disorder_operate.jl:1-247 (ctrl_SCDOPicr, get_ABX!, UpdateSCDOPLayer!)
+ *   src/public/Gupdate.jl:52-82     (GMupdate! and get_abTau! of DOPBuffer_)
+ * with X_A = e^{iα} I + (1 - e^{iα}) G0[A,A] in place of the Grover matrix.
  */
 #include "orc.h"
 #include <math.h>
@@ -34,6 +38,8 @@ typedef struct area {
 
 struct orc_scee {
     orc_tu *m;         /* model constants and scratch; its s/G are not used */
+    int nrep, dop;     /* 2 replicas (Rényi-2) or 1 (disorder operator) */
+    zdouble ea;        /* e^{iα} of the disorder operator */
     g4rep R[2];
     area A, B;
     double lambda;
@@ -61,6 +67,7 @@ orc_scee *orc_scee_new(orc_tu *m, int nA, const int *indexA, int nB, const int *
                        int nlam, const uint8_t *s1, const uint8_t *s2) {
     orc_scee *e = (orc_scee *)calloc(1, sizeof(orc_scee));
     e->m = m;
+    e->nrep = 2;
     alloc_rep(m, &e->R[0]);
     alloc_rep(m, &e->R[1]);
     memcpy(e->R[0].s, s1, (size_t)m->ns * m->nt);
@@ -71,6 +78,17 @@ orc_scee *orc_scee_new(orc_tu *m, int nA, const int *indexA, int nB, const int *
     int n = m->ns, np = m->np;
     e->tNN = zz((size_t)n * n); e->tNN2 = zz((size_t)n * n); e->tNn = zz((size_t)n * np);
     e->tnN = zz((size_t)np * n); e->tN = zz(n);
+    return e;
+}
+
+/* ctrl_SCDOPicr (src/tU/disorder_operate.jl:1): one field s, DOPBuffer(alpha, index) of both
+ * regions */
+orc_scee *orc_dop_new(orc_tu *m, double alpha, int nA, const int *indexA, int nB, const int *indexB,
+                      double lambda, int nlam, const uint8_t *s) {
+    orc_scee *e = orc_scee_new(m, nA, indexA, nB, indexB, lambda, nlam, s, s);
+    e->nrep = 1;
+    e->dop = 1;
+    e->ea = cos(alpha) + I * sin(alpha);
     return e;
 }
 
@@ -205,12 +223,48 @@ static void grover(orc_scee *e, area *X) {
     orc_zinv(nA, GM, nA);
 }
 
-/* get_ABGM! (src/tU/SCEE.jl:264-276) */
+/* get_ABX! (src/tU/disorder_operate.jl:202-223): X = e^{iα} I + (1 - e^{iα}) G0[A,A],
+ * detX = |det X|², and X⁻¹ (kept in gmInv) */
+static void xform(orc_scee *e, area *X) {
+    int n = e->m->ns, nA = X->nA;
+    if (nA == 0) { X->detg = 1.0; return; }
+    zdouble *G0 = e->R[0].G0, *XM = X->gmInv;
+    for (int j = 0; j < nA; ++j)
+        for (int i = 0; i < nA; ++i)
+            XM[IX(i, j, nA)] = (1.0 - e->ea) * G0[IX(X->index[i], X->index[j], n)] + (i == j ? e->ea : 0.0);
+    zdouble d = orc_zdet(nA, XM, nA);
+    X->detg = creal(d * conj(d));
+    orc_zinv(nA, XM, nA);
+}
+
+/* get_ABGM! (src/tU/SCEE.jl:264-276), or get_ABX! of the disorder operator */
 static void get_ABGM(orc_scee *e, int idx, int forward) {
-    G4(e, 0, idx, forward);
-    G4(e, 1, idx, forward);
+    for (int r = 0; r < e->nrep; ++r) G4(e, r, idx, forward);
+    if (e->dop) {
+        xform(e, &e->A);
+        xform(e, &e->B);
+        return;
+    }
     grover(e, &e->A);
     grover(e, &e->B);
+}
+
+/* get_abTau! of DOPBuffer_ (Gupdate.jl:70-82): a = (1 - e^{iα}) G0t[A, i] r,
+ * b = Gt0[i, A] X⁻¹, Tau = b a + 1 */
+static zdouble abtau_dop(orc_scee *e, area *X, int i, zdouble r, const zdouble *Gt0, const zdouble *G0t) {
+    int n = e->m->ns, nA = X->nA;
+    if (nA == 0) { X->Tau = 1.0; return 1.0; }
+    const int *ix = X->index;
+    for (int q = 0; q < nA; ++q) X->a[q] = (1.0 - e->ea) * (G0t[IX(ix[q], i, n)] * r);
+    for (int q = 0; q < nA; ++q) {
+        zdouble s = 0;
+        for (int k = 0; k < nA; ++k) s += Gt0[IX(i, ix[k], n)] * X->gmInv[IX(k, q, nA)];
+        X->b[q] = s;
+    }
+    zdouble tau = 1.0;
+    for (int q = 0; q < nA; ++q) tau += X->b[q] * X->a[q];
+    X->Tau = tau;
+    return tau;
 }
 
 /* get_abTau1! / get_abTau2! (Gupdate.jl:91-132); returns Tau (1 x 1 for t-U) */
@@ -291,13 +345,14 @@ static void g4_update(orc_scee *e, g4rep *p, int i, zdouble r) {
         }
 }
 
-/* UpdateSCEELayer! (src/tU/SCEE.jl:278-324). Replica r draws from Philox stream 1 + r. */
+/* UpdateSCEELayer! (src/tU/SCEE.jl:278-324). Replica r draws from Philox stream 1 + r.
+ * Disorder operator: UpdateSCDOPLayer! (src/tU/disorder_operate.jl:225-247), one replica. */
 static void update_layer(orc_scee *e, int lt, int dir, uint32_t sweep) {
     orc_tu *m = e->m;
     int n = m->ns;
     uint32_t tag = (uint32_t)lt | ((uint32_t)dir << 31);
     for (int i = 0; i < n; ++i) {
-        for (int r = 0; r < 2; ++r) {
+        for (int r = 0; r < e->nrep; ++r) {
             g4rep *p = &e->R[r], *o = &e->R[1 - r];
             uint8_t *sv = p->s + IX(i, lt - 1, n);
             int si = *sv, choice;
@@ -310,8 +365,10 @@ static void update_layer(orc_scee *e, int lt, int dir, uint32_t sweep) {
             zdouble pc = 1.0 + Delta * (1.0 - p->Gt[IX(i, i, n)]);
             zdouble rr = Delta / pc;
             double prob = creal(pc * conj(pc)) * (m->gamma[sx - 1] / m->gamma[si - 1]);
-            zdouble tA = abtau(e, &e->A, r + 1, i, rr, o->G0, p->Gt0, p->G0t);
-            zdouble tB = abtau(e, &e->B, r + 1, i, rr, o->G0, p->Gt0, p->G0t);
+            zdouble tA = e->dop ? abtau_dop(e, &e->A, i, rr, p->Gt0, p->G0t)
+                                : abtau(e, &e->A, r + 1, i, rr, o->G0, p->Gt0, p->G0t);
+            zdouble tB = e->dop ? abtau_dop(e, &e->B, i, rr, p->Gt0, p->G0t)
+                                : abtau(e, &e->B, r + 1, i, rr, o->G0, p->Gt0, p->G0t);
             double dA = creal(tA * conj(tA)), dB = creal(tB * conj(tB));
             prob *= pow(dA, e->lambda) * pow(dB, 1.0 - e->lambda);
             m->n_propose++;
@@ -334,14 +391,14 @@ void orc_scee_forward_slice(orc_scee *e, int lt, uint32_t sweep) {
     orc_tu *m = e->m;
     int n = m->ns;
     zdouble *D[2] = {zz(n), zz(n)}, *Di[2] = {zz(n), zz(n)};
-    for (int r = 0; r < 2; ++r)
+    for (int r = 0; r < e->nrep; ++r)
         for (int i = 0; i < n; ++i) {
             D[r][i] = cisd(m->alpha[lt - 1] * m->eta[e->R[r].s[IX(i, lt - 1, n)] - 1]);
             Di[r][i] = 1.0 / D[r][i];
         }
-    for (int r = 0; r < 2; ++r) orc_tu_WrapKV(m, e->R[r].Gt0, D[r], 1, 'L');
-    for (int r = 0; r < 2; ++r) orc_tu_WrapKV(m, e->R[r].Gt, D[r], 1, 'B');
-    for (int r = 0; r < 2; ++r) orc_tu_WrapKV(m, e->R[r].G0t, Di[r], 1, 'R');
+    for (int r = 0; r < e->nrep; ++r) orc_tu_WrapKV(m, e->R[r].Gt0, D[r], 1, 'L');
+    for (int r = 0; r < e->nrep; ++r) orc_tu_WrapKV(m, e->R[r].Gt, D[r], 1, 'B');
+    for (int r = 0; r < e->nrep; ++r) orc_tu_WrapKV(m, e->R[r].G0t, Di[r], 1, 'R');
     update_layer(e, lt, 0, sweep);
     for (int r = 0; r < 2; ++r) { free(D[r]); free(Di[r]); }
 }
@@ -350,15 +407,16 @@ void orc_scee_forward_slice(orc_scee *e, int lt, uint32_t sweep) {
 void orc_scee_init(orc_scee *e) {
     orc_tu *m = e->m;
     int n = m->ns, np = m->np, NN = m->nn_nodes;
-    for (int idx = 1; idx <= NN - 1; ++idx) { blocks(e, 0, idx); blocks(e, 1, idx); }
-    for (int r = 0; r < 2; ++r) {
+    for (int idx = 1; idx <= NN - 1; ++idx)
+        for (int r = 0; r < e->nrep; ++r) blocks(e, r, idx);
+    for (int r = 0; r < e->nrep; ++r) {
         g4rep *p = &e->R[r];
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < np; ++i) RBL(p, NN)[IX(i, j, np)] = conj(m->Pt[IX(j, i, n)]);
         memcpy(RBR(p, 1), m->Pt, sizeof(zdouble) * (size_t)n * np);
     }
     for (int i = 1; i <= NN - 1; ++i)
-        for (int r = 0; r < 2; ++r) {
+        for (int r = 0; r < e->nrep; ++r) {
             update_BL(e, r, NN - i);
             update_BR(e, r, i + 1);
         }
@@ -366,30 +424,35 @@ void orc_scee_init(orc_scee *e) {
     get_ABGM(e, 1, 1);
 }
 
-/* ctrl_SCEEicr sweeps (src/tU/SCEE.jl:87-251). O receives one value per sweep. */
+/* ctrl_SCEEicr sweeps (src/tU/SCEE.jl:87-251), or ctrl_SCDOPicr sweeps
+ * (src/tU/disorder_operate.jl:63-188). O receives one value per sweep. */
+static void wraps(orc_scee *e, int lt, int forward, zdouble *D, zdouble *Di) {
+    orc_tu *m = e->m;
+    int n = m->ns;
+    const double sg = forward ? 1.0 : -1.0;
+    /* "L" with D, "B" with D (which the reference then mutates to 1/D), "R" with 1/D, each
+     * matrix for every replica before the next */
+    for (int r = 0; r < e->nrep; ++r)
+        for (int i = 0; i < n; ++i) {
+            D[r * n + i] = cisd(sg * m->alpha[lt - 1] * m->eta[e->R[r].s[IX(i, lt - 1, n)] - 1]);
+            Di[r * n + i] = 1.0 / D[r * n + i];
+        }
+    for (int r = 0; r < e->nrep; ++r) orc_tu_WrapKV(m, e->R[r].Gt0, D + r * n, forward, 'L');
+    for (int r = 0; r < e->nrep; ++r) orc_tu_WrapKV(m, e->R[r].Gt, D + r * n, forward, 'B');
+    for (int r = 0; r < e->nrep; ++r) orc_tu_WrapKV(m, e->R[r].G0t, Di + r * n, forward, 'R');
+}
+
 void orc_scee_sweeps(orc_scee *e, int sweeps, double *O) {
     orc_tu *m = e->m;
     int n = m->ns, nt = m->nt, NN = m->nn_nodes, thidx = NN / 2 + 1;
-    zdouble *D1 = zz(n), *D2 = zz(n), *D1i = zz(n), *D2i = zz(n);
+    zdouble *D = zz(2 * (size_t)n), *Di = zz(2 * (size_t)n);
     orc_scee_init(e);
     for (int sw = 0; sw < sweeps; ++sw) {
         uint32_t sweep = m->sweep;
         double tmpO = 0.0;
         int counter = 0;
         for (int lt = 1; lt <= nt; ++lt) {
-            for (int i = 0; i < n; ++i) {
-                D1[i] = cisd(m->alpha[lt - 1] * m->eta[e->R[0].s[IX(i, lt - 1, n)] - 1]);
-                D2[i] = cisd(m->alpha[lt - 1] * m->eta[e->R[1].s[IX(i, lt - 1, n)] - 1]);
-                D1i[i] = 1.0 / D1[i];
-                D2i[i] = 1.0 / D2[i];
-            }
-            /* "L" with D, "B" with D (which the reference then mutates to 1/D), "R" with 1/D */
-            orc_tu_WrapKV(m, e->R[0].Gt0, D1, 1, 'L');
-            orc_tu_WrapKV(m, e->R[1].Gt0, D2, 1, 'L');
-            orc_tu_WrapKV(m, e->R[0].Gt, D1, 1, 'B');
-            orc_tu_WrapKV(m, e->R[1].Gt, D2, 1, 'B');
-            orc_tu_WrapKV(m, e->R[0].G0t, D1i, 1, 'R');
-            orc_tu_WrapKV(m, e->R[1].G0t, D2i, 1, 'R');
+            wraps(e, lt, 1, D, Di);
             update_layer(e, lt, 0, sweep);
             tmpO += pow(e->A.detg / e->B.detg, 1.0 / e->nlam);
             counter++;
@@ -398,10 +461,11 @@ void orc_scee_sweeps(orc_scee *e, int sweeps, double *O) {
             if (node) {
                 e->idx += 1;
                 int idx = e->idx;
-                blocks(e, 0, idx - 1);
-                blocks(e, 1, idx - 1);
-                for (int i = idx; i <= (thidx > idx ? thidx : idx); ++i) { update_BR(e, 0, i); update_BR(e, 1, i); }
-                for (int i = idx - 1; i >= (thidx < idx ? thidx : idx); --i) { update_BL(e, 0, i); update_BL(e, 1, i); }
+                for (int r = 0; r < e->nrep; ++r) blocks(e, r, idx - 1);
+                for (int i = idx; i <= (thidx > idx ? thidx : idx); ++i)
+                    for (int r = 0; r < e->nrep; ++r) update_BR(e, r, i);
+                for (int i = idx - 1; i >= (thidx < idx ? thidx : idx); --i)
+                    for (int r = 0; r < e->nrep; ++r) update_BL(e, r, i);
                 get_ABGM(e, idx, 1);
             }
         }
@@ -414,28 +478,18 @@ void orc_scee_sweeps(orc_scee *e, int sweeps, double *O) {
             if (node) {
                 e->idx -= 1;
                 int idx = e->idx;
-                blocks(e, 0, idx);
-                blocks(e, 1, idx);
-                for (int i = idx; i >= (thidx < idx ? thidx : idx); --i) { update_BL(e, 0, i); update_BL(e, 1, i); }
-                for (int i = idx + 1; i <= (thidx > idx ? thidx : idx); ++i) { update_BR(e, 0, i); update_BR(e, 1, i); }
+                for (int r = 0; r < e->nrep; ++r) blocks(e, r, idx);
+                for (int i = idx; i >= (thidx < idx ? thidx : idx); --i)
+                    for (int r = 0; r < e->nrep; ++r) update_BL(e, r, i);
+                for (int i = idx + 1; i <= (thidx > idx ? thidx : idx); ++i)
+                    for (int r = 0; r < e->nrep; ++r) update_BR(e, r, i);
                 get_ABGM(e, idx, 0);
             } else {
-                for (int i = 0; i < n; ++i) {
-                    D1[i] = cisd(-m->alpha[lt - 1] * m->eta[e->R[0].s[IX(i, lt - 1, n)] - 1]);
-                    D2[i] = cisd(-m->alpha[lt - 1] * m->eta[e->R[1].s[IX(i, lt - 1, n)] - 1]);
-                    D1i[i] = 1.0 / D1[i];
-                    D2i[i] = 1.0 / D2[i];
-                }
-                orc_tu_WrapKV(m, e->R[0].Gt0, D1, 0, 'L');
-                orc_tu_WrapKV(m, e->R[1].Gt0, D2, 0, 'L');
-                orc_tu_WrapKV(m, e->R[0].Gt, D1, 0, 'B');
-                orc_tu_WrapKV(m, e->R[1].Gt, D2, 0, 'B');
-                orc_tu_WrapKV(m, e->R[0].G0t, D1i, 0, 'R');
-                orc_tu_WrapKV(m, e->R[1].G0t, D2i, 0, 'R');
+                wraps(e, lt, 0, D, Di);
             }
         }
         O[sw] = tmpO / counter;
         m->sweep += 1;
     }
-    free(D1); free(D2); free(D1i); free(D2i);
+    free(D); free(Di);
 }
