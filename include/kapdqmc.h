@@ -138,13 +138,13 @@ int kdq_get_counters(kdq_handle h, double *out);
  *                     once per slice by one GEMM of the accumulated rank. Same Markov chain;
  *                     G agrees to rounding.
  *  KDQ_OPT_PIPE       t-U chains with the per-block flush, in captured sweeps. 1 (default):
- *                     the decisions of block b+1 are issued on a second stream as soon as
- *                     block b's flush operands exist; they read their k x k submatrix from
- *                     the G before that flush minus its rank-|A| product (restricted to the
- *                     block), so they do not wait for the flush to rewrite G. 0: decisions
- *                     and flushes alternate on one stream. Same Markov chain; G agrees to
- *                     rounding. Sweeps with host launches (graph option 0, kernel timing on,
- *                     kdq_update_layer) always use the one-stream order. */
+ *                     the decisions of block b+1 run on a second stream beside block b's
+ *                     flush; they read their k x k submatrix of the updated G from the G
+ *                     before that flush, block b's accepted sites and its W, so they do not
+ *                     wait for the flush to rewrite G. 0: decisions and flushes alternate on
+ *                     one stream. Same Markov chain; G agrees to rounding. Sweeps with host
+ *                     launches (graph option 0, kernel timing on, kdq_update_layer) always
+ *                     use the one-stream order. */
 enum { KDQ_OPT_GRAPH = 0, KDQ_OPT_DELAY = 1, KDQ_OPT_GEMM_TILE = 2, KDQ_OPT_DEFER = 3, KDQ_OPT_PIPE = 4 };
 int kdq_set_option(kdq_handle h, int32_t key, int32_t value);
 

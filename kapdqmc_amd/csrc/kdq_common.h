@@ -48,8 +48,8 @@ struct ChainDev {
     uint32_t chain_id, stream;   // Philox stream: 0 plain sweep, 1 + replica in SCEE
     // local-update scratch
     int *acc_cnt;          // [2]: accepted count of a block, slot by block parity (t-U sweep)
-    int *acc_idx;          // [64]
-    double *W;             // 2 planes of 64 x 64 (re, im), ld 64
+    int *acc_idx;          // [2][64]: accepted sites of a block, by slot
+    double *W;             // [2 slots][2 planes (re, im)] of 64 x 64, ld 64
     double *counters;      // [2]: proposals, accepted
     double *meas;          // [KDQ_MEAS_SLOTS]: accumulated observables, count in the last slot
     // model kind and field layout

@@ -34,7 +34,7 @@ struct ChainHost {
     double *eK, *eKinv, *hK, *hKinv, *K, *Pt, *alpha;
     double *G[2], *T, *Gm, *Tm, *P0, *P1, *P2, *M, *X, *S;
     float *G32[2] = {nullptr, nullptr}, *T32 = nullptr, *eK32 = nullptr, *eKinv32 = nullptr;   // mixed precision
-    double *FZ = nullptr, *FC = nullptr;     // flush scratch, two slots: W·Y (64 x N) and gathered columns (N x 64)
+    double *FZ = nullptr, *FC = nullptr;     // flush scratch: W·Y (64 x N) and gathered columns (N x 64)
     float *FZ32 = nullptr, *FC32 = nullptr;
     double *DU = nullptr, *DV = nullptr;     // deferred-flush factors of a t-U slice (N x N each)
     double *DI = nullptr;                    // inverses of the 16 x 16 diagonal blocks (Np x 32)
@@ -70,10 +70,8 @@ struct kdq_handle_s {
     ChainDev *d_chains = nullptr;
     // operand tables
     Opnd *t_eK, *t_eKinv, *t_hK, *t_hKinv, *t_Pt, *t_G[2], *t_T, *t_Gm, *t_Tm, *t_P0, *t_P1, *t_P2, *t_M, *t_X, *t_S;
-    Opnd *t_FZ = nullptr, *t_FC = nullptr;     // flush scratch operands (slot 0)
+    Opnd *t_FZ = nullptr, *t_FC = nullptr;     // flush scratch operands
     OpndF *t_FZ32 = nullptr, *t_FC32 = nullptr;
-    Opnd *t_FZb = nullptr, *t_FCb = nullptr;   // slot 1 (pipelined t-U blocks)
-    OpndF *t_FZ32b = nullptr, *t_FC32b = nullptr;
     Opnd *t_DU = nullptr, *t_DV = nullptr;     // deferred-flush factors (t-U)
     Opnd *t_DI = nullptr;                      // diagonal-block inverses of the Np x Np factors
     OpndF *t_DU32 = nullptr, *t_DV32 = nullptr;
@@ -115,7 +113,7 @@ struct kdq_handle_s {
     int use_graph = 1, gemm_tile = 0;
     bool pipe = true;                         // t-U decisions of block b+1 beside block b's flush
     hipStream_t side = nullptr;               // their stream
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_app = nullptr, ev_dec[2] = {nullptr, nullptr};   // their dependencies
     uint32_t *d_sweep = nullptr;              // device sweep counter read by graph replays
     const uint32_t *sweep_dev = nullptr;      // non-null while a sweep is being captured
     hipGraph_t graph = nullptr;
@@ -136,8 +134,9 @@ struct kdq_handle_s {
         }
         for (hipEvent_t e : ev_pool) hipEventDestroy(e);
         if (side) hipStreamSynchronize(side);
-        if (ev_fork) hipEventDestroy(ev_fork);
-        if (ev_join) hipEventDestroy(ev_join);
+        if (ev_app) hipEventDestroy(ev_app);
+        for (hipEvent_t e : ev_dec)
+            if (e) hipEventDestroy(e);
         if (side) hipStreamDestroy(side);
         if (stream) hipStreamDestroy(stream);
     }
@@ -514,60 +513,58 @@ int defer_flush(kdq_handle h) {
     return KDQ_OK;
 }
 
-// Pipelined t-U layer. Block b's flush runs on the handle's stream as its two launches; once
-// the first has written Cs, Zs (slot b % 2) the decisions of block b+1 start on the side stream
-// from the same G, reading G[S,S] - Cs[S,:] Zs[:,S], while the second rewrites G; the first
-// launch of block b+1's flush waits for them. Decisions write their accepted count, W and the
-// accepted sites, which only the next first launch reads; the two slots keep block b's Cs, Zs
-// and count intact for its second launch.
+// Pipelined t-U layer (captured sweeps). The decisions of block b+1 need only their k x k
+// submatrix of G_{b+1} = G_b - G_b[:,A_b] W_b (E_A - G_b[A_b,:]); they read it from G_b and
+// block b's accepted sites and W, so they depend on the flush of block b-1 (which wrote G_b)
+// and on the decisions of block b, not on block b's flush. They run on the side stream, the
+// flushes (prep, apply) on the handle's stream:
+//   side:   decide_0  decide_1  decide_2 ...      decide_{b+1} waits for apply_{b-1}
+//   main:       prep_0 apply_0 prep_1 apply_1 ... prep_b waits for decide_b
+// so decide_{b+1} is issued while prep_b runs and shares the device with apply_b. Accepted
+// counts, sites and W alternate between two slots by block parity.
 int update_layer_pipe(kdq_handle h, int lt, int dir, uint32_t sweep) {
     const int kb = h->delay;
     const bool mx = h->mixed;
-    auto decide = [&](int s0, int slot, bool pre, hipStream_t st) -> hipError_t {
+    auto decide = [&](int s0, int slot, bool pre, const Opnd *Gb, const OpndF *Gbf) -> hipError_t {
         DecideArgs a{};
         a.chains = h->d_chains;
-        a.G = h->t_G[h->cur];
+        a.G = Gb;
+        a.Gf = Gbf;
         a.lt = lt; a.dir = dir; a.s0 = s0; a.kb = kb; a.sweep = sweep;
         a.sweep_dev = h->sweep_dev;
-        if (mx) a.Gf = h->t_G32[h->cur32];
         a.slot = slot;
         a.pre = pre ? 1 : 0;
-        a.Pc = slot ? h->t_FC : h->t_FCb;   // the previous block's operands: the other slot
-        a.Pz = slot ? h->t_FZ : h->t_FZb;
-        if (mx) {
-            a.Pcf = slot ? h->t_FC32 : h->t_FC32b;
-            a.Pzf = slot ? h->t_FZ32 : h->t_FZ32b;
-        }
-        return launch_decide_tu(st, h->nch, a);
+        return launch_decide_tu(h->side, h->nch, a);
     };
-    HIPCHK(decide(0, 0, false, h->stream));
-    int slot = 0;
-    for (int s0 = 0; s0 < h->max_ns; s0 += kb, slot ^= 1) {
-        const bool next = s0 + kb < h->max_ns;
-        auto fork = [&]() -> hipError_t {
-            if (!next) return hipSuccess;
-            hipError_t e = hipEventRecord(h->ev_fork, h->stream);
-            if (e == hipSuccess) e = hipStreamWaitEvent(h->side, h->ev_fork, 0);
-            if (e == hipSuccess) e = decide(s0 + kb, slot ^ 1, true, h->side);
-            if (e == hipSuccess) e = hipEventRecord(h->ev_join, h->side);
-            return e;
-        };
+    const int nb = (h->max_ns + kb - 1) / kb;
+    // G_0 (this layer's G after the wrap) is ready
+    HIPCHK(hipEventRecord(h->ev_app, h->stream));
+    HIPCHK(hipStreamWaitEvent(h->side, h->ev_app, 0));
+    HIPCHK(decide(0, 0, false, h->t_G[h->cur], mx ? h->t_G32[h->cur32] : nullptr));
+    HIPCHK(hipEventRecord(h->ev_dec[0], h->side));
+    for (int b = 0; b < nb; ++b) {
+        const int slot = b & 1;
+        // G_b, the input of this block's flush
+        const Opnd *Gb = h->t_G[h->cur];
+        const OpndF *Gbf = mx ? h->t_G32[h->cur32] : nullptr;
+        if (b + 1 < nb) {
+            if (b >= 1) HIPCHK(hipStreamWaitEvent(h->side, h->ev_app, 0));   // apply_{b-1} wrote G_b
+            HIPCHK(decide((b + 1) * kb, slot ^ 1, true, Gb, Gbf));
+            HIPCHK(hipEventRecord(h->ev_dec[slot ^ 1], h->side));
+        }
+        HIPCHK(hipStreamWaitEvent(h->stream, h->ev_dec[slot], 0));
         if (mx) {
             FlushArgsF f{h->d_chains, h->t_G32[h->cur32], h->t_G32[1 - h->cur32], kb, 0, nullptr, nullptr,
-                         slot ? h->t_FZ32b : h->t_FZ32, slot ? h->t_FC32b : h->t_FC32, slot};
-            HIPCHK(launch_flush_part(h->stream, h->nch, h->max_ns, f, 1));
-            HIPCHK(fork());
-            HIPCHK(launch_flush_part(h->stream, h->nch, h->max_ns, f, 2));
+                         h->t_FZ32, h->t_FC32, slot};
+            HIPCHK(launch_flush_f32(h->stream, h->nch, h->max_ns, f));
             h->cur32 ^= 1;
         } else {
-            FlushArgs f{h->d_chains, h->t_G[h->cur], h->t_G[1 - h->cur], kb, 0, nullptr, nullptr,
-                        slot ? h->t_FZb : h->t_FZ, slot ? h->t_FCb : h->t_FC, slot};
-            HIPCHK(launch_flush_part(h->stream, h->nch, h->max_ns, f, 1));
-            HIPCHK(fork());
-            HIPCHK(launch_flush_part(h->stream, h->nch, h->max_ns, f, 2));
+            FlushArgs f{h->d_chains, h->t_G[h->cur], h->t_G[1 - h->cur], kb, 0, nullptr, nullptr, h->t_FZ, h->t_FC,
+                        slot};
+            HIPCHK(launch_flush(h->stream, h->nch, h->max_ns, f));
             h->cur ^= 1;
         }
-        if (next) HIPCHK(hipStreamWaitEvent(h->stream, h->ev_join, 0));
+        if (b + 2 < nb) HIPCHK(hipEventRecord(h->ev_app, h->stream));
     }
     return KDQ_OK;
 }
@@ -923,8 +920,8 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
     h->ch.resize(nchains);
     hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_app, hipEventDisableTiming);
+    for (int k = 0; k < 2 && e == hipSuccess; ++k) e = hipEventCreateWithFlags(&h->ev_dec[k], hipEventDisableTiming);
     if (e != hipSuccess) {
         int r = fail(h, KDQ_EHIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
         delete h;
@@ -933,8 +930,7 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
     std::vector<ChainDev> cd(nchains);
     std::vector<Opnd> v_eK, v_eKinv, v_hK, v_hKinv, v_Pt, v_G0, v_G1, v_T, v_Gm, v_Tm, v_P0, v_P1, v_P2, v_M, v_X, v_S;
     std::vector<OpndF> v_G32a, v_G32b, v_T32, v_eK32, v_eKinv32, v_FZ32, v_FC32;
-    std::vector<Opnd> v_FZ, v_FC, v_DU, v_DV, v_DI, v_FZb, v_FCb;
-    std::vector<OpndF> v_FZ32b, v_FC32b;
+    std::vector<Opnd> v_FZ, v_FC, v_DU, v_DV, v_DI;
     std::vector<OpndF> v_DU32, v_DV32;
     std::vector<BlockDesc> blk_all;
     std::vector<int> blk_offset, blk_count;
@@ -1011,11 +1007,11 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
             }
         }
         ok &= dalloc(C, &C.acc_cnt, 2) == hipSuccess;
-        ok &= dalloc(C, &C.acc_idx, 64) == hipSuccess;
+        ok &= dalloc(C, &C.acc_idx, 2 * 64) == hipSuccess;
         ok &= dalloc(C, &C.info, 1) == hipSuccess;
-        ok &= dalloc(C, &C.W, 2 * 64 * 64) == hipSuccess;
-        ok &= dalloc(C, &C.FZ, 4 * 64 * (size_t)n) == hipSuccess;
-        ok &= dalloc(C, &C.FC, 4 * 64 * (size_t)n) == hipSuccess;
+        ok &= dalloc(C, &C.W, 2 * 2 * 64 * 64) == hipSuccess;
+        ok &= dalloc(C, &C.FZ, 2 * 64 * (size_t)n) == hipSuccess;
+        ok &= dalloc(C, &C.FC, 2 * 64 * (size_t)n) == hipSuccess;
         ok &= dalloc(C, &C.rk, (size_t)ns_all + 2) == hipSuccess;
         ok &= dalloc(C, &C.DI, 2 * 32 * (size_t)np) == hipSuccess;   // blocks run over the largest chain
         ok &= dalloc(C, &C.rank, 1) == hipSuccess;
@@ -1032,8 +1028,8 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
             ok &= dalloc(C, &C.T32, 2 * nn) == hipSuccess;
             ok &= dalloc(C, &C.eK32, nn) == hipSuccess;
             ok &= dalloc(C, &C.eKinv32, nn) == hipSuccess;
-            ok &= dalloc(C, &C.FZ32, 4 * 64 * (size_t)n) == hipSuccess;
-            ok &= dalloc(C, &C.FC32, 4 * 64 * (size_t)n) == hipSuccess;
+            ok &= dalloc(C, &C.FZ32, 2 * 64 * (size_t)n) == hipSuccess;
+            ok &= dalloc(C, &C.FC32, 2 * 64 * (size_t)n) == hipSuccess;
             if (m.kind == KDQ_MODEL_TU) {
                 ok &= dalloc(C, &C.DU32, 2 * nn) == hipSuccess;
                 ok &= dalloc(C, &C.DV32, 2 * nn) == hipSuccess;
@@ -1133,8 +1129,6 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
         v_M.push_back(cplx(C.M, np, pp)); v_X.push_back(cplx(C.X, np, nP));
         v_S.push_back(cplx(C.S, np, pp));
         v_FZ.push_back(Opnd{C.FZ, C.FZ + 64 * (size_t)n, 64, 0});
-        v_FZb.push_back(Opnd{C.FZ + 128 * (size_t)n, C.FZ + 192 * (size_t)n, 64, 0});
-        v_FCb.push_back(Opnd{C.FC + 128 * (size_t)n, C.FC + 192 * (size_t)n, n, 0});
         v_DI.push_back(Opnd{C.DI, C.DI + 32 * (size_t)np, np, 0});
         v_DU.push_back(Opnd{C.DU, C.DU ? C.DU + nn : nullptr, n, 0});
         v_DV.push_back(Opnd{C.DV, C.DV ? C.DV + nn : nullptr, n, 0});
@@ -1144,8 +1138,6 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
         if (h->mixed) {
             v_FZ32.push_back(OpndF{C.FZ32, C.FZ32 + 64 * (size_t)n, 64, 0});
             v_FC32.push_back(OpndF{C.FC32, C.FC32 + 64 * (size_t)n, n, 0});
-            v_FZ32b.push_back(OpndF{C.FZ32 + 128 * (size_t)n, C.FZ32 + 192 * (size_t)n, 64, 0});
-            v_FC32b.push_back(OpndF{C.FC32 + 128 * (size_t)n, C.FC32 + 192 * (size_t)n, n, 0});
             v_G32a.push_back(OpndF{C.G32[0], C.G32[0] + nn, n, 0});
             v_G32b.push_back(OpndF{C.G32[1], C.G32[1] + nn, n, 0});
             v_T32.push_back(OpndF{C.T32, C.T32 + nn, n, 0});
@@ -1192,7 +1184,6 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
     }
     r |= make_table(h, &h->t_M, v_M); r |= make_table(h, &h->t_X, v_X); r |= make_table(h, &h->t_S, v_S);
     r |= make_table(h, &h->t_FZ, v_FZ); r |= make_table(h, &h->t_FC, v_FC);
-    r |= make_table(h, &h->t_FZb, v_FZb); r |= make_table(h, &h->t_FCb, v_FCb);
     r |= make_table(h, &h->t_DU, v_DU); r |= make_table(h, &h->t_DV, v_DV);
     r |= make_table(h, &h->t_DI, v_DI);
     r |= make_table_f(h, &h->t_DU32, v_DU32); r |= make_table_f(h, &h->t_DV32, v_DV32);
@@ -1201,7 +1192,6 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
         r |= make_table_f(h, &h->t_T32, v_T32);
         r |= make_table_f(h, &h->t_eK32, v_eK32); r |= make_table_f(h, &h->t_eKinv32, v_eKinv32);
         r |= make_table_f(h, &h->t_FZ32, v_FZ32); r |= make_table_f(h, &h->t_FC32, v_FC32);
-        r |= make_table_f(h, &h->t_FZ32b, v_FZ32b); r |= make_table_f(h, &h->t_FC32b, v_FC32b);
         void *q = nullptr;
         if (hipMalloc(&q, sizeof(double) * nchains) == hipSuccess) {
             h->dev_misc.push_back(q);

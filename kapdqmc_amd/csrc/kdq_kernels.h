@@ -42,12 +42,11 @@ struct DecideArgs {
     int deferred, blk;
     const Opnd *U, *V;           // FP64 factors (N x N each; V stored transposed)
     const OpndF *Uf, *Vf;        // FP32 factors (mixed precision)
-    // slot: the accepted count goes to acc_cnt[slot]. pre: the previous block's flush
-    // G_new = G - Pc[:, :m] Pz[:m, :] (m = acc_cnt[1 - slot]) has not been applied to G yet (it
-    // runs concurrently); the block reads G[S,S] - Pc[S,:] Pz[:,S] (Pz rank-major, ld 64)
+    // slot: the block's accepted count, sites and W go to slot `slot` of acc_cnt, acc_idx and W.
+    // pre: the previous block's update G - G[:,A] W (E_A - G[A,:]) (A, W, count in slot
+    // 1 - slot) has not been applied to G yet (its flush runs concurrently); the block reads
+    // G[S,S] + G[S,A] W G[A,S]
     int slot, pre;
-    const Opnd *Pc, *Pz;
-    const OpndF *Pcf, *Pzf;      // the same in FP32 (mixed precision)
 };
 hipError_t launch_decide_tu(hipStream_t st, int nchains, const DecideArgs &a);
 
@@ -81,16 +80,12 @@ struct FlushArgsT {
     const typename OpndOf<T>::type *Rin;
     const typename OpndOf<T>::type *Zs;     // scratch: 64 x N per chain
     const typename OpndOf<T>::type *Cs;     // scratch: N x 64 per chain
-    int slot;           // the rank is acc_cnt[slot]
+    int slot;           // the rank, accepted sites and W of slot `slot`
 };
 typedef FlushArgsT<double> FlushArgs;
 typedef FlushArgsT<float> FlushArgsF;
 hipError_t launch_flush(hipStream_t st, int nchains, int max_ns, const FlushArgs &f);
 hipError_t launch_flush_f32(hipStream_t st, int nchains, int max_ns, const FlushArgsF &f);   // FP32 G
-// The two launches of a flush separately (part 1: Zs, Cs; part 2: Xout), for callers that
-// run other work between them.
-hipError_t launch_flush_part(hipStream_t st, int nchains, int max_ns, const FlushArgs &f, int part);
-hipError_t launch_flush_part(hipStream_t st, int nchains, int max_ns, const FlushArgsF &f, int part);
 
 // Mixed precision: dst = FP32(src) for per-chain n x n complex matrices; when maxerr is set,
 // max |dst_before - src| is folded into maxerr[c] first (FP32 propagation error against the

@@ -79,27 +79,43 @@ __global__ __launch_bounds__(KBT >= 32 ? 1024 : KBT * KBT) void k_decide_tu(Deci
         Wi[q] = 0.0;
     }
     // a rank-rk product not yet applied to G: subtract U[S, :rk] V[:rk, S] (chunks of DCH ranks
-    // staged in LDS). Deferred flush: the slice's factors (V transposed, V[t, j] at j + t·ld);
-    // pipelined sweep: the previous block's Cs and Zs (Zs rank-major, Zs[t, j] at t + j·ld)
+    // staged in LDS). Deferred flush: the slice's factors (V transposed, V[t, j] at j + t·ld).
+    // Pipelined sweep: the previous block's update G - G[:,A] W (E_A - G[A,:]) with its accepted
+    // sites A and W (slot 1 - slot), so U = G[S, A] and V = -W G[A, S] (E_A vanishes on S).
     if ((a.deferred && a.blk > 0) || a.pre) {
         constexpr int DCH = 16;
         __shared__ double Usr[KBT][DCH + 1], Usi[KBT][DCH + 1], Vsr[DCH][KBT + 1], Vsi[DCH][KBT + 1];
-        const int rk = a.pre ? ch.acc_cnt[1 - a.slot] : ch.rk[a.blk];
-        const OpndF *Uf_ = a.pre ? a.Pcf : a.Uf, *Vf_ = a.pre ? a.Pzf : a.Vf;
-        const Opnd *U_ = a.pre ? a.Pc : a.U, *V_ = a.pre ? a.Pz : a.V;
+        const int ps = 1 - a.slot;
+        const int rk = a.pre ? ch.acc_cnt[ps] : ch.rk[a.blk];
+        const int *Ap = ch.acc_idx + ps * KB;
+        const double *Wp = ch.W + (size_t)ps * 2 * KB * KB;
+        auto g_at = [&](int i, int j, double &vr, double &vi) {   // the current G, either precision
+            if (a.Gf) {
+                const OpndF Gf = a.Gf[c];
+                const size_t o = (size_t)i + (size_t)j * Gf.ld;
+                vr = Gf.re[o];
+                vi = Gf.im[o];
+            } else {
+                const size_t o = (size_t)i + (size_t)j * G.ld;
+                vr = G.re[o];
+                vi = G.im[o];
+            }
+        };
         for (int t0 = 0; t0 < rk; t0 += DCH) {
             const int tc = min(DCH, rk - t0);
             for (int e = tid; e < KBT * DCH; e += NTH) {
                 const int i = e % KBT, t = e / KBT;
                 double ur = 0.0, ui = 0.0;
                 if (i < kb && t < tc) {
-                    if (Uf_) {
-                        const OpndF U = Uf_[c];
+                    if (a.pre) {
+                        g_at(s0 + i, Ap[t0 + t], ur, ui);
+                    } else if (a.Uf) {
+                        const OpndF U = a.Uf[c];
                         const size_t o = (size_t)(s0 + i) + (size_t)(t0 + t) * U.ld;
                         ur = U.re[o];
                         ui = U.im[o];
                     } else {
-                        const Opnd U = U_[c];
+                        const Opnd U = a.U[c];
                         const size_t o = (size_t)(s0 + i) + (size_t)(t0 + t) * U.ld;
                         ur = U.re[o];
                         ui = U.im[o];
@@ -112,16 +128,22 @@ __global__ __launch_bounds__(KBT >= 32 ? 1024 : KBT * KBT) void k_decide_tu(Deci
                 const int j = e % KBT, t = e / KBT;
                 double vr = 0.0, vi = 0.0;
                 if (j < kb && t < tc) {
-                    if (Vf_) {
-                        const OpndF V = Vf_[c];
-                        const size_t o = a.pre ? (size_t)(t0 + t) + (size_t)(s0 + j) * V.ld
-                                               : (size_t)(s0 + j) + (size_t)(t0 + t) * V.ld;
+                    if (a.pre) {   // -Σ_u W[t, u] G[A_u, j]
+                        for (int u = 0; u < rk; ++u) {
+                            double gr, gi;
+                            g_at(Ap[u], s0 + j, gr, gi);
+                            const double wr_ = Wp[(t0 + t) + u * KB], wi_ = Wp[KB * KB + (t0 + t) + u * KB];
+                            vr -= wr_ * gr - wi_ * gi;
+                            vi -= wr_ * gi + wi_ * gr;
+                        }
+                    } else if (a.Vf) {
+                        const OpndF V = a.Vf[c];
+                        const size_t o = (size_t)(s0 + j) + (size_t)(t0 + t) * V.ld;
                         vr = V.re[o];
                         vi = V.im[o];
                     } else {
-                        const Opnd V = V_[c];
-                        const size_t o = a.pre ? (size_t)(t0 + t) + (size_t)(s0 + j) * V.ld
-                                               : (size_t)(s0 + j) + (size_t)(t0 + t) * V.ld;
+                        const Opnd V = a.V[c];
+                        const size_t o = (size_t)(s0 + j) + (size_t)(t0 + t) * V.ld;
                         vr = V.re[o];
                         vi = V.im[o];
                     }
@@ -269,14 +291,15 @@ __global__ __launch_bounds__(KBT >= 32 ? 1024 : KBT * KBT) void k_decide_tu(Deci
             *ch.rank = rk;
         }
     }
-    if (tid < na) ch.acc_idx[tid] = s0 + accl[tid];
+    if (tid < na) ch.acc_idx[a.slot * KB + tid] = s0 + accl[tid];
     if (r < na) {
+        double *Wo = ch.W + (size_t)a.slot * 2 * KB * KB;
 #pragma unroll
         for (int q = 0; q < CPT; ++q) {
             const int u = c0 + q;
             if (u < na) {
-                ch.W[r + u * KB] = Wr[q];
-                ch.W[KB * KB + r + u * KB] = Wi[q];
+                Wo[r + u * KB] = Wr[q];
+                Wo[KB * KB + r + u * KB] = Wi[q];
             }
         }
     }
@@ -311,11 +334,12 @@ __global__ __launch_bounds__(256) void k_flush_prep(FlushArgsT<T> f) {
     T *Yr = Wi + K * K, *Yi = Yr + K * PC;                       // Y [u][jj]
     __shared__ int aidx[KB];
     const int tid = threadIdx.x;
-    if (tid < na) aidx[tid] = ch.acc_idx[tid];
+    const double *Wg = ch.W + (size_t)f.slot * 2 * KB * KB;   // the block's W and accepted sites
+    if (tid < na) aidx[tid] = ch.acc_idx[f.slot * KB + tid];
     for (int e = tid; e < na * na; e += 256) {
         const int t = e % na, u = e / na;
-        Wr[t * K + u] = (T)ch.W[t + u * KB];
-        Wi[t * K + u] = (T)ch.W[KB * KB + t + u * KB];
+        Wr[t * K + u] = (T)Wg[t + u * KB];
+        Wi[t * K + u] = (T)Wg[KB * KB + t + u * KB];
     }
     __syncthreads();
     for (int e = tid; e < na * PC; e += 256) {
@@ -623,7 +647,6 @@ hipError_t defer_t(hipStream_t st, int nchains, int max_ns, const DeferArgsT<T> 
 
 hipError_t launch_decide_tu(hipStream_t st, int nchains, const DecideArgs &a) {
     if (a.kb < 1 || a.kb > KB || a.slot < 0 || a.slot > 1) return hipErrorInvalidValue;
-    if (a.pre && (a.Gf ? !a.Pcf || !a.Pzf : !a.Pc || !a.Pz)) return hipErrorInvalidValue;
     if (a.kb <= 8) hipLaunchKernelGGL(k_decide_tu<8>, dim3(nchains), dim3(64), 0, st, a);
     else if (a.kb <= 16) hipLaunchKernelGGL(k_decide_tu<16>, dim3(nchains), dim3(256), 0, st, a);
     else if (a.kb <= 32) hipLaunchKernelGGL(k_decide_tu<32>, dim3(nchains), dim3(1024), 0, st, a);
@@ -632,18 +655,15 @@ hipError_t launch_decide_tu(hipStream_t st, int nchains, const DecideArgs &a) {
 }
 
 template <class T>
-hipError_t flush_t(hipStream_t st, int nchains, int max_ns, const FlushArgsT<T> &f, int part = 3) {
+hipError_t flush_t(hipStream_t st, int nchains, int max_ns, const FlushArgsT<T> &f) {
     if (f.kmax < 1 || f.kmax > KB || !f.Zs || !f.Cs || f.slot < 0 || f.slot > 1) return hipErrorInvalidValue;
-    if (part & 1) {
-        const size_t K = (size_t)f.kmax;
-        const size_t bytes = (2 * K * K + 2 * K * PC) * sizeof(T);
-        if (hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_flush_prep<T>),
-                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes))
-            return e;
-        hipLaunchKernelGGL(k_flush_prep<T>, dim3((max_ns + PC - 1) / PC, nchains), dim3(256), bytes, st, f);
-        if (hipError_t e = hipGetLastError()) return e;
-    }
-    if (!(part & 2)) return hipSuccess;
+    const size_t K = (size_t)f.kmax;
+    const size_t bytes = (2 * K * K + 2 * K * PC) * sizeof(T);
+    if (hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_flush_prep<T>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes))
+        return e;
+    hipLaunchKernelGGL(k_flush_prep<T>, dim3((max_ns + PC - 1) / PC, nchains), dim3(256), bytes, st, f);
+    if (hipError_t e = hipGetLastError()) return e;
     const int tx = (max_ns + AN - 1) / AN, ty = (max_ns + AM - 1) / AM;
     if (nchains >= NXCD) {
         const int groups = (nchains + NXCD - 1) / NXCD;
@@ -660,14 +680,6 @@ hipError_t launch_flush(hipStream_t st, int nchains, int max_ns, const FlushArgs
 
 hipError_t launch_flush_f32(hipStream_t st, int nchains, int max_ns, const FlushArgsF &f) {
     return flush_t<float>(st, nchains, max_ns, f);
-}
-
-hipError_t launch_flush_part(hipStream_t st, int nchains, int max_ns, const FlushArgs &f, int part) {
-    return flush_t<double>(st, nchains, max_ns, f, part);
-}
-
-hipError_t launch_flush_part(hipStream_t st, int nchains, int max_ns, const FlushArgsF &f, int part) {
-    return flush_t<float>(st, nchains, max_ns, f, part);
 }
 
 hipError_t launch_defer_xy(hipStream_t st, int nchains, int max_ns, const DeferArgsT<double> &a) {

@@ -1,10 +1,10 @@
 """Pipelined t-U blocks (the decisions of block b+1 on a second stream beside block b's
-flush, reading their submatrix as G minus that flush's product) against the one-stream
-order on the same batch: identical Philox draws, so the fields must agree bit for bit and
-G / the measurement rows to rounding, for block sizes 1..64, FP64 and mixed precision, a
-batch of uneven lattices, and graph replays (where the pipelining applies) against host
-launches (which keep the one-stream order). The oracle comparison of
-the default (pipelined) path is in test_gpu_tu.py."""
+flush, reading their submatrix of the updated G from the G before that flush and block b's
+accepted sites and W) against the one-stream order on the same batch: identical Philox
+draws, so the fields must agree bit for bit and G / the measurement rows to rounding, for
+block sizes 1..64, FP64 and mixed precision, a batch of uneven lattices, and graph replays
+(where the pipelining applies) against host launches (which keep the one-stream order). The
+oracle comparison of the default (pipelined) path is in test_gpu_tu.py."""
 import numpy as np
 import pytest
 
