@@ -78,10 +78,8 @@ __global__ __launch_bounds__(KBT >= 32 ? 1024 : KBT * KBT) void k_decide_tu(Deci
         Wr[q] = 0.0;
         Wi[q] = 0.0;
     }
-    // a rank-rk product not yet applied to G: subtract U[S, :rk] V[:rk, S] (chunks of DCH ranks
-    // staged in LDS). Deferred flush: the slice's factors (V transposed, V[t, j] at j + t·ld).
-    // Pipelined sweep: the previous block's update G - G[:,A] W (E_A - G[A,:]) with its accepted
-    // sites A and W (slot 1 - slot), so U = G[S, A] and V = -W G[A, S] (E_A vanishes on S).
+    // a rank-rk product not yet applied to G (deferred flush, or the previous block's flush in
+    // the pipelined sweep): the block works on G[S,S] - U[S,:] V[:,S]
     if ((a.deferred && a.blk > 0) || a.pre) {
         constexpr int DCH = 16;
         __shared__ double Usr[KBT][DCH + 1], Usi[KBT][DCH + 1], Vsr[DCH][KBT + 1], Vsi[DCH][KBT + 1];
@@ -101,6 +99,46 @@ __global__ __launch_bounds__(KBT >= 32 ? 1024 : KBT * KBT) void k_decide_tu(Deci
                 vi = G.im[o];
             }
         };
+        // pipelined sweep, phase 1: V = -W Y (rk x kb) with Y = G[A, S], in registers of the
+        // entry layout (thread (r, c0 + q) holds V[r][c0 + q]), over chunks of DCH of the
+        // contraction index staged in LDS (W columns in Usr, Y rows in Vsr)
+        double pvr[CPT], pvi[CPT];
+#pragma unroll
+        for (int q = 0; q < CPT; ++q) {
+            pvr[q] = 0.0;
+            pvi[q] = 0.0;
+        }
+        if (a.pre) {
+            for (int u0 = 0; u0 < rk; u0 += DCH) {
+                const int uc = min(DCH, rk - u0);
+                for (int e = tid; e < KBT * DCH; e += NTH) {
+                    const int t = e % KBT, u = e / KBT;
+                    const bool ok = t < rk && u < uc;
+                    Usr[t][u] = ok ? Wp[t + (u0 + u) * KB] : 0.0;
+                    Usi[t][u] = ok ? Wp[KB * KB + t + (u0 + u) * KB] : 0.0;
+                }
+                for (int e = tid; e < DCH * KBT; e += NTH) {
+                    const int j = e % KBT, u = e / KBT;
+                    double yr = 0.0, yi = 0.0;
+                    if (j < kb && u < uc) g_at(Ap[u0 + u], s0 + j, yr, yi);
+                    Vsr[u][j] = yr;
+                    Vsi[u][j] = yi;
+                }
+                __syncthreads();
+#pragma unroll
+                for (int q = 0; q < CPT; ++q)
+                    for (int u = 0; u < uc; ++u) {
+                        const double wr_ = Usr[r][u], wi_ = Usi[r][u], yr = Vsr[u][c0 + q], yi = Vsi[u][c0 + q];
+                        pvr[q] -= wr_ * yr - wi_ * yi;
+                        pvi[q] -= wr_ * yi + wi_ * yr;
+                    }
+                __syncthreads();
+            }
+        }
+        // subtract U[S, :rk] V[:rk, S] over chunks of DCH ranks staged in LDS. Deferred flush:
+        // the slice's factors (V transposed, V[t, j] at j + t·ld). Pipelined sweep: the previous
+        // block's update G - G[:,A] W (E_A - G[A,:]) with its accepted sites A and W (slot
+        // 1 - slot), so U = G[S, A] and V = -W G[A, S] from phase 1 (E_A vanishes on S)
         for (int t0 = 0; t0 < rk; t0 += DCH) {
             const int tc = min(DCH, rk - t0);
             for (int e = tid; e < KBT * DCH; e += NTH) {
@@ -124,32 +162,34 @@ __global__ __launch_bounds__(KBT >= 32 ? 1024 : KBT * KBT) void k_decide_tu(Deci
                 Usr[i][t] = ur;
                 Usi[i][t] = ui;
             }
-            for (int e = tid; e < DCH * KBT; e += NTH) {
-                const int j = e % KBT, t = e / KBT;
-                double vr = 0.0, vi = 0.0;
-                if (j < kb && t < tc) {
-                    if (a.pre) {   // -Σ_u W[t, u] G[A_u, j]
-                        for (int u = 0; u < rk; ++u) {
-                            double gr, gi;
-                            g_at(Ap[u], s0 + j, gr, gi);
-                            const double wr_ = Wp[(t0 + t) + u * KB], wi_ = Wp[KB * KB + (t0 + t) + u * KB];
-                            vr -= wr_ * gr - wi_ * gi;
-                            vi -= wr_ * gi + wi_ * gr;
-                        }
-                    } else if (a.Vf) {
-                        const OpndF V = a.Vf[c];
-                        const size_t o = (size_t)(s0 + j) + (size_t)(t0 + t) * V.ld;
-                        vr = V.re[o];
-                        vi = V.im[o];
-                    } else {
-                        const Opnd V = a.V[c];
-                        const size_t o = (size_t)(s0 + j) + (size_t)(t0 + t) * V.ld;
-                        vr = V.re[o];
-                        vi = V.im[o];
+            if (a.pre) {   // rows t0 .. t0 + tc of V from the threads that hold them
+                if (r >= t0 && r < t0 + DCH)
+#pragma unroll
+                    for (int q = 0; q < CPT; ++q) {
+                        const bool ok = r < t0 + tc && c0 + q < kb;
+                        Vsr[r - t0][c0 + q] = ok ? pvr[q] : 0.0;
+                        Vsi[r - t0][c0 + q] = ok ? pvi[q] : 0.0;
                     }
+            } else {
+                for (int e = tid; e < DCH * KBT; e += NTH) {
+                    const int j = e % KBT, t = e / KBT;
+                    double vr = 0.0, vi = 0.0;
+                    if (j < kb && t < tc) {
+                        if (a.Vf) {
+                            const OpndF V = a.Vf[c];
+                            const size_t o = (size_t)(s0 + j) + (size_t)(t0 + t) * V.ld;
+                            vr = V.re[o];
+                            vi = V.im[o];
+                        } else {
+                            const Opnd V = a.V[c];
+                            const size_t o = (size_t)(s0 + j) + (size_t)(t0 + t) * V.ld;
+                            vr = V.re[o];
+                            vi = V.im[o];
+                        }
+                    }
+                    Vsr[t][j] = vr;
+                    Vsi[t][j] = vi;
                 }
-                Vsr[t][j] = vr;
-                Vsi[t][j] = vi;
             }
             __syncthreads();
 #pragma unroll
