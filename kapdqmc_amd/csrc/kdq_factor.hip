@@ -24,6 +24,8 @@ constexpr int NB = 16;      // panel / slab width
 constexpr int PS = NB + 1;  // padded LDS row stride of a panel (doubles)
 constexpr int RB = 16;      // rows (right-hand sides) per workgroup of the solves
 constexpr int TB = 8;       // trailing-update entries per thread whose loads are issued together
+constexpr int NL = 1024;    // threads of the one-workgroup LU
+constexpr int LB = 4;       // its trailing-update entries per thread per batch
 
 template <int T>
 KDQ_DEV double blk_sum(double v, double *red) {
@@ -326,7 +328,7 @@ __global__ __launch_bounds__(NT) void k_trsm_rlh(TrsmArgs t) {
 // In-place LU with partial pivoting of the n x n matrix, P A = L U (getrf semantics: the
 // interchanges are applied to whole rows; pivots stored 0-based, row k swapped with piv[k]).
 // logdet receives log|det A|; info (sticky) receives j+1 for an exactly zero pivot j.
-__global__ __launch_bounds__(NT) void k_lu(LuArgs a) {
+__global__ __launch_bounds__(NL) void k_lu(LuArgs a) {
     const int c = blockIdx.x;
     const ChainDev &ch = a.chains[c];
     const int n = dim_of(ch, a.ndim);
@@ -337,8 +339,7 @@ __global__ __launch_bounds__(NT) void k_lu(LuArgs a) {
     extern __shared__ double lds[];
     double *Pr = lds, *Pi = lds + (size_t)n * PS;                      // panel [r][q]
     double *Ur = lds + 2 * (size_t)n * PS, *Ui = Ur + (size_t)n * PS;  // U12 [col][j]
-    __shared__ double red_v[NT / 64];
-    __shared__ int red_i[NT / 64], lpiv[NB], bad;
+    __shared__ int lpiv[NB], bad;
     __shared__ double s_logdet, s_phr, s_phi;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (tid == 0) {
@@ -350,85 +351,77 @@ __global__ __launch_bounds__(NT) void k_lu(LuArgs a) {
     for (int k0 = 0; k0 < n; k0 += NB) {
         const int kb = min(NB, n - k0), m = n - k0;
         __syncthreads();
-        for (int e = tid; e < m * kb; e += NT) {
+        for (int e = tid; e < m * kb; e += NL) {
             const int r = e % m, q = e / m;
             const size_t o = (k0 + r) + (size_t)(k0 + q) * ld;
             Pr[r * PS + q] = Re[o];
             Pi[r * PS + q] = Im[o];
         }
         __syncthreads();
-        for (int j = 0; j < kb; ++j) {
-            // pivot: first index of max |P[r][j]|², r >= j
-            double best = -1.0;
-            int bi = m;
-            for (int r = j + tid; r < m; r += NT) {
-                const double v = Pr[r * PS + j] * Pr[r * PS + j] + Pi[r * PS + j] * Pi[r * PS + j];
-                if (v > best) {
-                    best = v;
-                    bi = r;
-                }
-            }
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) {
-                const double ob = __shfl_xor(best, off, 64);
-                const int oi = __shfl_xor(bi, off, 64);
-                if (ob > best || (ob == best && oi < bi)) {
-                    best = ob;
-                    bi = oi;
-                }
-            }
-            if (lane == 0) {
-                red_v[wave] = best;
-                red_i[wave] = bi;
-            }
-            __syncthreads();
-            if (tid == 0) {
-                double b = red_v[0];
-                int p = red_i[0];
-                for (int w = 1; w < NT / 64; ++w)
-                    if (red_v[w] > b || (red_v[w] == b && red_i[w] < p)) {
-                        b = red_v[w];
-                        p = red_i[w];
+        // panel factorisation by wave 0 alone (wave-synchronous: no workgroup barriers in the
+        // column loop; LDS accesses of one wave execute in program order)
+        if (wave == 0) {
+            for (int j = 0; j < kb; ++j) {
+                // pivot: first index of max |P[r][j]|², r >= j
+                double best = -1.0;
+                int bi = m;
+                for (int r = j + lane; r < m; r += 64) {
+                    const double v = Pr[r * PS + j] * Pr[r * PS + j] + Pi[r * PS + j] * Pi[r * PS + j];
+                    if (v > best) {
+                        best = v;
+                        bi = r;
                     }
-                lpiv[j] = p;
-                piv[k0 + j] = k0 + p;
+                }
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) {
+                    const double ob = __shfl_xor(best, off, 64);
+                    const int oi = __shfl_xor(bi, off, 64);
+                    if (ob > best || (ob == best && oi < bi)) {
+                        best = ob;
+                        bi = oi;
+                    }
+                }
+                const int p = bi;   // the same in every lane after the butterfly
+                if (lane == 0) {
+                    lpiv[j] = p;
+                    piv[k0 + j] = k0 + p;
+                }
+                if (p != j && lane < kb) {
+                    const double ar = Pr[j * PS + lane], ai = Pi[j * PS + lane];
+                    Pr[j * PS + lane] = Pr[p * PS + lane];
+                    Pi[j * PS + lane] = Pi[p * PS + lane];
+                    Pr[p * PS + lane] = ar;
+                    Pi[p * PS + lane] = ai;
+                }
+                __builtin_amdgcn_wave_barrier();
+                const double pr = Pr[j * PS + j], pim = Pi[j * PS + j];
+                const double p2 = pr * pr + pim * pim;
+                if (p2 == 0.0) {   // singular: leave the column, flag (uniform branch)
+                    if (lane == 0 && !bad) bad = k0 + j + 1;
+                    continue;
+                }
+                const double ir = pr / p2, ii = -pim / p2;
+                for (int r = j + 1 + lane; r < m; r += 64) {
+                    double tr_, ti_;
+                    cmul(Pr[r * PS + j], Pi[r * PS + j], ir, ii, tr_, ti_);
+                    Pr[r * PS + j] = tr_;
+                    Pi[r * PS + j] = ti_;
+                }
+                __builtin_amdgcn_wave_barrier();
+                const int nq = kb - j - 1;
+                for (int e = lane; e < (m - j - 1) * nq; e += 64) {
+                    const int r = j + 1 + e / nq, q = j + 1 + e % nq;
+                    double tr_, ti_;
+                    cmul(Pr[r * PS + j], Pi[r * PS + j], Pr[j * PS + q], Pi[j * PS + q], tr_, ti_);
+                    Pr[r * PS + q] -= tr_;
+                    Pi[r * PS + q] -= ti_;
+                }
+                __builtin_amdgcn_wave_barrier();
             }
-            __syncthreads();
-            const int p = lpiv[j];
-            if (p != j && tid < kb) {
-                const double ar = Pr[j * PS + tid], ai = Pi[j * PS + tid];
-                Pr[j * PS + tid] = Pr[p * PS + tid];
-                Pi[j * PS + tid] = Pi[p * PS + tid];
-                Pr[p * PS + tid] = ar;
-                Pi[p * PS + tid] = ai;
-            }
-            __syncthreads();
-            const double pr = Pr[j * PS + j], pim = Pi[j * PS + j];
-            const double p2 = pr * pr + pim * pim;
-            if (p2 == 0.0) {   // singular: leave the column, flag (uniform branch)
-                if (tid == 0 && !bad) bad = k0 + j + 1;
-                continue;
-            }
-            const double ir = pr / p2, ii = -pim / p2;
-            for (int r = j + 1 + tid; r < m; r += NT) {
-                double tr_, ti_;
-                cmul(Pr[r * PS + j], Pi[r * PS + j], ir, ii, tr_, ti_);
-                Pr[r * PS + j] = tr_;
-                Pi[r * PS + j] = ti_;
-            }
-            __syncthreads();
-            const int nq = kb - j - 1;
-            for (int e = tid; e < (m - j - 1) * nq; e += NT) {
-                const int r = j + 1 + e / nq, q = j + 1 + e % nq;
-                double tr_, ti_;
-                cmul(Pr[r * PS + j], Pi[r * PS + j], Pr[j * PS + q], Pi[j * PS + q], tr_, ti_);
-                Pr[r * PS + q] -= tr_;
-                Pi[r * PS + q] -= ti_;
-            }
-            __syncthreads();
         }
+        __syncthreads();
         // panel (L21, U11 and L11) back to memory; log|det| of this panel's pivots
-        for (int e = tid; e < m * kb; e += NT) {
+        for (int e = tid; e < m * kb; e += NL) {
             const int r = e % m, q = e / m;
             const size_t o = (k0 + r) + (size_t)(k0 + q) * ld;
             Re[o] = Pr[r * PS + q];
@@ -450,7 +443,7 @@ __global__ __launch_bounds__(NT) void k_lu(LuArgs a) {
             s_phi = phi;
         }
         // the panel's interchanges on every column outside it (one thread per column)
-        for (int col = tid; col < n; col += NT) {
+        for (int col = tid; col < n; col += NL) {
             if (col >= k0 && col < k0 + kb) continue;
             for (int j = 0; j < kb; ++j) {
                 const int p = lpiv[j];
@@ -466,7 +459,7 @@ __global__ __launch_bounds__(NT) void k_lu(LuArgs a) {
         __syncthreads();
         // U12 = L11⁻¹ A12 (unit lower), one thread per column, kept in LDS and stored
         const int ncol = n - k0 - kb;
-        for (int q = tid; q < ncol; q += NT) {
+        for (int q = tid; q < ncol; q += NL) {
             const size_t col = (size_t)(k0 + kb + q) * ld;
             for (int j = 0; j < kb; ++j) {
                 double sr = Re[k0 + j + col], si = Im[k0 + j + col];
@@ -483,21 +476,35 @@ __global__ __launch_bounds__(NT) void k_lu(LuArgs a) {
             }
         }
         __syncthreads();
-        // trailing update A22 -= L21 U12
+        // trailing update A22 -= L21 U12, LB entries per thread at a time with their loads issued
+        // before any store (the stores may alias later loads as far as the compiler knows)
         const int mt = m - kb;
-        for (int e = tid; e < mt * ncol; e += NT) {
-            const int i = kb + e % mt, q = e / mt;
-            double sr = 0.0, si = 0.0;
-#pragma unroll 4
-            for (int j = 0; j < kb; ++j) {
-                const double ar = Pr[i * PS + j], ai = Pi[i * PS + j];
-                const double br = Ur[q * PS + j], bi = Ui[q * PS + j];
-                sr += ar * br - ai * bi;
-                si += ar * bi + ai * br;
+        for (int e0 = tid; e0 < mt * ncol; e0 += NL * LB) {
+            double gr[LB], gi[LB];
+            size_t off[LB];
+#pragma unroll
+            for (int b = 0; b < LB; ++b) {
+                const int e = e0 + b * NL, i = kb + e % mt, q = e / mt;
+                const bool ok = e < mt * ncol;
+                off[b] = ok ? (k0 + i) + (size_t)(k0 + kb + q) * ld : 0;
+                gr[b] = ok ? Re[off[b]] : 0.0;
+                gi[b] = ok ? Im[off[b]] : 0.0;
             }
-            const size_t o = (k0 + i) + (size_t)(k0 + kb + q) * ld;
-            Re[o] -= sr;
-            Im[o] -= si;
+#pragma unroll
+            for (int b = 0; b < LB; ++b) {
+                const int e = e0 + b * NL, i = kb + e % mt, q = e / mt;
+                if (e >= mt * ncol) continue;
+                double sr = 0.0, si = 0.0;
+#pragma unroll 4
+                for (int j = 0; j < kb; ++j) {
+                    const double ar = Pr[i * PS + j], ai = Pi[i * PS + j];
+                    const double br = Ur[q * PS + j], bi = Ui[q * PS + j];
+                    sr += ar * br - ai * bi;
+                    si += ar * bi + ai * br;
+                }
+                Re[off[b]] = gr[b] - sr;
+                Im[off[b]] = gi[b] - si;
+            }
         }
     }
     __syncthreads();
@@ -733,7 +740,7 @@ hipError_t launch_trsm_rlh(hipStream_t st, int nchains, int max_m, int max_n, co
 hipError_t launch_lu(hipStream_t st, int nchains, int max_n, const LuArgs &a) {
     const size_t b = factor_lds_bytes(2, max_n);
     if (hipError_t e = allow_lds(k_lu, b)) return e;
-    hipLaunchKernelGGL(k_lu, dim3(nchains), dim3(NT), b, st, a);
+    hipLaunchKernelGGL(k_lu, dim3(nchains), dim3(NL), b, st, a);
     if (a.Dinv) hipLaunchKernelGGL(k_diag_inv<true>, dim3((max_n + NB - 1) / NB, nchains), dim3(NT), 0, st,
                                    a.chains, a.A, a.ndim, a.Dinv);
     return hipGetLastError();
