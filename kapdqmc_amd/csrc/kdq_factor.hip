@@ -715,7 +715,7 @@ size_t factor_lds_bytes(int which, int max_n) {
     const size_t n = (size_t)max_n;
     switch (which) {
         case 0: return 2 * n * PS * sizeof(double);                 // k_chol
-        case 1: return (2 * n * RB + 2 * n * PS + 4 * NB * PS) * sizeof(double);  // k_trsm_rlh, k_lu_solve
+        case 1: return (2 * n * RB + 2 * n * PS + 2 * NB * PS) * sizeof(double);  // k_trsm_rlh, k_lu_solve
         case 2: return 4 * n * PS * sizeof(double);                 // k_lu
         default: return 0;
     }
