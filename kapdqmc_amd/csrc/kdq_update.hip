@@ -448,7 +448,7 @@ __global__ __launch_bounds__(256) void k_flush_apply(FlushArgsT<T> f, int nch, i
     const int tid = threadIdx.x, ri = tid & 15, cg = tid >> 4;
     const auto Gi = f.Gin[c], Go = f.Gout[c], Z = f.Zs[c], Cg = f.Cs[c];
     __shared__ C2 Cs2[AK][AM];
-    __shared__ C2 Zs2[AK][AN];
+    __shared__ C2 Zs2[AK][AN + 1];   // padded: the staging stores run down a column (t)
     T xr[2][4], xi[2][4], ar[2][4], ai[2][4];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
