@@ -119,6 +119,14 @@ int kdq_init_chain(kdq_handle h);
  * nchains x kdq_num_obs doubles. */
 int kdq_measure(kdq_handle h, int32_t lt, double *out);
 
+/* Time-displaced Green's functions of chain c from its current fields (t-U), replacing
+ * G4(model, s, τ1, τ2) (src/tU/GreenMatrix.jl:126-226): G1 = G(τ1), G2 = G(τ2),
+ * G12 = G(τ1, τ2), G21 = G(τ2, τ1), each ns x ns ComplexF64 (interleaved, column-major),
+ * 0 <= τ1, τ2 <= Nt; τ1 == τ2 gives G, G, -(I - G), G as the reference does. Blocks of
+ * BatchSize slices between the two times with the equal-time G at every block boundary,
+ * orthonormalised chains; the sweep state (chains, G) is not changed. */
+int kdq_G4(kdq_handle h, int32_t c, int32_t tau1, int32_t tau2, double *G1, double *G2, double *G12, double *G21);
+
 /* Counters: out[0] = proposals, out[1] = accepted, per chain (nchains x 2). */
 int kdq_get_counters(kdq_handle h, double *out);
 

@@ -23,7 +23,7 @@ EXPORTS = ("kdq_create", "kdq_destroy", "kdq_last_error", "kdq_num_chains", "kdq
            "kdq_update_layer", "kdq_init_chain", "kdq_measure", "kdq_get_counters", "kdq_profile",
            "kdq_profile_read", "kdq_get_logdet", "kdq_set_option", "kdq_get_precision_error", "kdq_log_weight", "kdq_scee_create", "kdq_scee_set_fields",
            "kdq_scee_get_fields", "kdq_scee_init", "kdq_scee_sweeps", "kdq_scee_get_G4", "kdq_scee_detg",
-           "kdq_dop_create", "kdq_net_create", "kdq_net_destroy", "kdq_net_last_error", "kdq_net_num_params", "kdq_net_get_params",
+           "kdq_dop_create", "kdq_G4", "kdq_net_create", "kdq_net_destroy", "kdq_net_last_error", "kdq_net_num_params", "kdq_net_get_params",
            "kdq_net_set_params", "kdq_net_energy", "kdq_net_loss_grad", "kdq_net_train_step", "kdq_net_sample",
            "kdq_slmc_move")
 PROF_FAMILIES = ("wrap", "chain", "get_G", "decide", "flush", "qr", "inv", "meas_wrap", "measure",
@@ -101,6 +101,7 @@ def load(path: str = LIB_PATH):
     L.kdq_scee_sweeps.argtypes = [vp, i32, dp]
     L.kdq_scee_get_G4.argtypes = [vp, i32, i32, dp, dp, dp, dp]
     L.kdq_scee_detg.argtypes = [vp, dp]
+    L.kdq_G4.argtypes = [vp, i32, i32, i32, dp, dp, dp, dp]
     L.kdq_dop_create.argtypes = [C.POINTER(ModelDesc), C.POINTER(RunDesc), C.POINTER(SceeDesc), C.c_double, i32,
                                  C.POINTER(vp)]
     fp = C.POINTER(C.c_float)

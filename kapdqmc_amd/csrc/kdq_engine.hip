@@ -112,6 +112,10 @@ struct kdq_handle_s {
     // options (kdq_set_option) and the captured sweep
     int use_graph = 1, gemm_tile = 0;
     bool pipe = true;                         // t-U decisions of block b+1 beside block b's flush
+    // time-displaced G (kdq_G4) scratch, allocated on first use: left chain per block
+    // boundary, and N x N work matrices
+    std::vector<Opnd *> g4_ul;
+    Opnd *g4_m[6] = {};
     hipStream_t side = nullptr;               // their stream
     hipEvent_t ev_app = nullptr, ev_dec[2] = {nullptr, nullptr};   // their dependencies
     uint32_t *d_sweep = nullptr;              // device sweep counter read by graph replays
@@ -362,6 +366,35 @@ int orthonormalise(kdq_handle h, Opnd *U, double *logacc = nullptr) {
     return KDQ_OK;
 }
 
+// dst = B(hi) ... B(lo) · src for U_R (N x Np), not orthonormalised (t-U); src may be dst
+int prop_R_range(kdq_handle h, Opnd *src, Opnd *dst, int lo, int hi) {
+    Opnd *cur = src;
+    for (int lt = lo; lt <= hi; ++lt) {
+        Opnd *out = (lt == hi && cur != dst) ? dst : ((cur == h->t_P0) ? h->t_P1 : h->t_P0);
+        GemmArgs g = gargs(h, h->t_eK, cur, out, DIM_NS, DIM_NP, DIM_NS);
+        g.rowS = hs_diag(h, lt, +1);
+        if (int r = gemm(h, false, true, 0, 0, g, KDQ_PROF_CHAIN)) return r;
+        cur = out;
+    }
+    if (cur != dst) HIPCHK(launch_copy_mat(h->stream, h->nch, h->max_ns, h->max_np, h->d_chains, cur, dst, DIM_NS, DIM_NP, true));
+    return KDQ_OK;
+}
+
+// U_Lᴴ <- (U_L · B(hi) ... B(lo))ᴴ with U_L stored as U_Lᴴ (N x Np), not orthonormalised (t-U);
+// src may be dst
+int prop_L_range(kdq_handle h, Opnd *src, Opnd *dst, int lo, int hi) {
+    Opnd *cur = src;
+    for (int lt = hi; lt >= lo; --lt) {
+        Opnd *out = (lt == lo && cur != dst) ? dst : ((cur == h->t_P0) ? h->t_P1 : h->t_P0);
+        GemmArgs g = gargs(h, h->t_eK, cur, out, DIM_NS, DIM_NP, DIM_NS);
+        g.inS = hs_diag(h, lt, +1, 1);
+        if (int r = gemm(h, false, true, 1, 0, g, KDQ_PROF_CHAIN)) return r;
+        cur = out;
+    }
+    if (cur != dst) HIPCHK(launch_copy_mat(h->stream, h->nch, h->max_ns, h->max_np, h->d_chains, cur, dst, DIM_NS, DIM_NP, true));
+    return KDQ_OK;
+}
+
 // dst = QR_Q(B_block(idx) · src) for U_R: slices nodes[idx-1]+1 .. nodes[idx] (1-based idx)
 int propagate_R(kdq_handle h, Opnd *src, Opnd *dst, int idx, double *logacc = nullptr) {
     const int lo = (int)h->nodes[idx - 1] + 1, hi = (int)h->nodes[idx];
@@ -408,6 +441,19 @@ int propagate_L(kdq_handle h, Opnd *src, Opnd *dst, int idx) {
 }
 
 // get_G!: G = I - U_R (U_L U_R)⁻¹ U_L into the current G buffer
+// G = I - U_R (U_L U_R)⁻¹ U_L for explicit chain operands (log|det| not recorded)
+int get_G_from(kdq_handle h, Opnd *UR, Opnd *ULH, Opnd *dst) {
+    GemmArgs g1 = gargs(h, ULH, UR, h->t_M, DIM_NP, DIM_NP, DIM_NS);
+    if (int r = gemm(h, true, true, 2, 0, g1, KDQ_PROF_GETG)) return r;
+    LuArgs la{h->d_chains, h->t_M, DIM_NP, h->max_np, h->d_piv, nullptr, h->d_info, nullptr, h->t_DI};
+    HIPCHK(launch_lu(h->stream, h->nch, h->max_np, la));
+    LuSolveArgs sa{h->d_chains, h->t_M, ULH, h->t_X, h->d_piv, DIM_NP, DIM_NS, h->max_np, 0, h->t_DI};
+    HIPCHK(launch_lu_solve(h->stream, h->nch, h->max_np, h->max_ns, sa));
+    GemmArgs g3 = gargs(h, UR, h->t_X, dst, DIM_NS, DIM_NS, DIM_NP);
+    g3.mode = 1;
+    return gemm(h, true, true, 0, 0, g3, KDQ_PROF_GETG);
+}
+
 int get_G(kdq_handle h, int idx, Opnd *dst = nullptr) {
     Opnd *ULH = h->t_ULH[idx - 1], *UR = h->t_UR[idx - 1];
     GemmArgs g1 = gargs(h, ULH, UR, h->t_M, DIM_NP, DIM_NP, DIM_NS);
@@ -452,6 +498,130 @@ int log_weight(kdq_handle h, double *out) {
         for (size_t q = 0; q < ne; ++q) lg += std::log(h->md[c].gamma[sv[q] - 1]);
         out[c] = 2.0 * (lr[c] + lm[c]) + lg;
     }
+    return KDQ_OK;
+}
+
+// ---- time-displaced Green's functions: G4(model, s, τ1, τ2) of src/tU/GreenMatrix.jl:126-226
+// for every chain from its current fields (t-U). Blocks of BatchSize slices between τ2 and τ1
+// (the last one shorter); U_R from P through slices 1..τ2 and U_L from Pᴴ through Nt..τ1+1,
+// orthonormalised after every block; the left chain at each block boundary is stored, the
+// right chain is advanced block by block, and G at each boundary gives
+//   G(τ1,τ2) = BB[nb] G[nb] ... BB[1] G[1],  G(τ2,τ1) = -(I - G[1]) BB[1]⁻¹ ... (I - G[nb]) BB[nb]⁻¹
+// (G[1] = G(τ2), G[nb+1] = G(τ1)). Works on its own scratch; the chain state and G of the
+// sweep are not touched.
+int g4_alloc(kdq_handle h) {
+    if (!h->g4_ul.empty()) return KDQ_OK;
+    std::vector<std::vector<Opnd>> vul(h->nn + 1), vm(6);   // NN left chains and the right one
+    for (int c = 0; c < h->nch; ++c) {
+        ChainHost &C = h->ch[c];
+        const int n = C.ns, np = C.np;
+        for (int k = 0; k <= h->nn; ++k) {
+            double *p = nullptr;
+            if (dalloc(C, &p, 2 * (size_t)n * np) != hipSuccess) return fail(h, KDQ_EHIP, "kdq_G4: device allocation failed");
+            vul[k].push_back(cplx(p, n, (size_t)n * np));
+        }
+        for (int k = 0; k < 6; ++k) {
+            double *p = nullptr;
+            if (dalloc(C, &p, 2 * (size_t)n * n) != hipSuccess) return fail(h, KDQ_EHIP, "kdq_G4: device allocation failed");
+            vm[k].push_back(cplx(p, n, (size_t)n * n));
+        }
+    }
+    h->g4_ul.resize(h->nn + 1);
+    for (int k = 0; k <= h->nn; ++k)
+        if (int r = make_table(h, &h->g4_ul[k], vul[k])) return r;
+    for (int k = 0; k < 6; ++k)
+        if (int r = make_table(h, &h->g4_m[k], vm[k])) return r;
+    return KDQ_OK;
+}
+
+// X <- B(hi) ... B(lo) X (N x N), through tmp
+int g4_left(kdq_handle h, Opnd *X, Opnd *tmp, int lo, int hi) {
+    Opnd *cur = X, *oth = tmp;
+    for (int lt = lo; lt <= hi; ++lt) {
+        GemmArgs g = gargs(h, h->t_eK, cur, oth, DIM_NS, DIM_NS, DIM_NS);   // D eK X
+        g.rowS = hs_diag(h, lt, +1);
+        if (int r = gemm(h, false, true, 0, 0, g)) return r;
+        std::swap(cur, oth);
+    }
+    if (cur != X) HIPCHK(launch_copy_mat(h->stream, h->nch, h->max_ns, h->max_ns, h->d_chains, cur, X, DIM_NS, DIM_NS, true));
+    return KDQ_OK;
+}
+
+// Y <- Y B(lo)⁻¹ ... B(hi)⁻¹ (N x N), B⁻¹ = eK⁻¹ D⁻¹, through tmp
+int g4_right_inv(kdq_handle h, Opnd *Y, Opnd *tmp, int lo, int hi) {
+    Opnd *cur = Y, *oth = tmp;
+    for (int lt = lo; lt <= hi; ++lt) {
+        GemmArgs g = gargs(h, cur, h->t_eKinv, oth, DIM_NS, DIM_NS, DIM_NS);   // Y eK⁻¹ D⁻¹
+        g.colS = hs_diag(h, lt, -1);
+        if (int r = gemm(h, true, false, 0, 0, g)) return r;
+        std::swap(cur, oth);
+    }
+    if (cur != Y) HIPCHK(launch_copy_mat(h->stream, h->nch, h->max_ns, h->max_ns, h->d_chains, cur, Y, DIM_NS, DIM_NS, true));
+    return KDQ_OK;
+}
+
+// out[0..3] = G(τ1), G(τ2), G(τ1,τ2), G(τ2,τ1) (tables of the scratch matrices), τ1 > τ2
+int g4_compute(kdq_handle h, int t1, int t2, Opnd **out) {
+    if (int r = g4_alloc(h)) return r;
+    int bs = 1;   // BatchSize: the largest gap between stabilisation nodes
+    for (size_t k = 1; k < h->nodes.size(); ++k) bs = std::max(bs, (int)(h->nodes[k] - h->nodes[k - 1]));
+    const int nb = (t1 - t2 + bs - 1) / bs;
+    if (nb + 1 > h->nn) return fail(h, KDQ_EARG, "kdq_G4: more blocks between the slices than the chain holds");
+    Opnd **UL = h->g4_ul.data(), **M = h->g4_m;
+    Opnd *Gi = M[0], *X12 = M[1], *Y21 = M[2], *T = M[3], *T2 = M[4], *G2 = M[5];
+    Opnd *UR = UL[h->nn];   // the right chain (N x Np), advanced block by block
+    // U_L at τ1: from Pᴴ through slices Nt .. τ1+1 (orthonormalised every BatchSize slices)
+    HIPCHK(launch_real_to_cplx(h->stream, h->nch, h->max_ns, h->max_np, h->d_chains, h->t_Pt, UL[nb], DIM_NS, DIM_NP));
+    for (int hi = h->nt; hi > t1; hi -= bs) {
+        const int lo = std::max(t1 + 1, hi - bs + 1);
+        if (int r = prop_L_range(h, UL[nb], UL[nb], lo, hi)) return r;
+        if (int r = orthonormalise(h, UL[nb])) return r;
+    }
+    // U_L at the block boundaries τ2 + i·BatchSize, i = nb-1 .. 0
+    for (int i = nb - 1; i >= 0; --i) {
+        const int lo = t2 + i * bs + 1, hi = std::min(t2 + (i + 1) * bs, t1);
+        if (int r = prop_L_range(h, UL[i + 1], UL[i], lo, hi)) return r;
+        if (int r = orthonormalise(h, UL[i])) return r;
+    }
+    // U_R at τ2: from P through slices 1 .. τ2
+    HIPCHK(launch_real_to_cplx(h->stream, h->nch, h->max_ns, h->max_np, h->d_chains, h->t_Pt, UR, DIM_NS, DIM_NP));
+    for (int lo = 1; lo <= t2; lo += bs) {
+        const int hi = std::min(t2, lo + bs - 1);
+        if (int r = prop_R_range(h, UR, UR, lo, hi)) return r;
+        if (int r = orthonormalise(h, UR)) return r;
+    }
+    for (int i = 0; i < nb; ++i) {
+        const int lo = t2 + i * bs + 1, hi = std::min(t2 + (i + 1) * bs, t1);
+        if (int r = get_G_from(h, UR, UL[i], Gi)) return r;   // G at the block's lower edge
+        if (i == 0) {
+            HIPCHK(launch_copy_mat(h->stream, h->nch, h->max_ns, h->max_ns, h->d_chains, Gi, G2, DIM_NS, DIM_NS, true));
+            HIPCHK(launch_copy_mat(h->stream, h->nch, h->max_ns, h->max_ns, h->d_chains, Gi, X12, DIM_NS, DIM_NS, true));
+            HIPCHK(launch_copy_mat(h->stream, h->nch, h->max_ns, h->max_ns, h->d_chains, Gi, Y21, DIM_NS, DIM_NS, true));
+            HIPCHK(launch_diag(h->stream, h->nch, h->max_ns, h->d_chains, Y21, DIM_NS, -1.0, 0));   // G - I
+        } else {
+            GemmArgs g = gargs(h, Gi, X12, T, DIM_NS, DIM_NS, DIM_NS);   // G[i] X12
+            if (int r = gemm(h, true, true, 0, 0, g)) return r;
+            std::swap(h->g4_m[1], h->g4_m[3]);
+            X12 = h->g4_m[1];
+            T = h->g4_m[3];
+            GemmArgs gy = gargs(h, Y21, Gi, T, DIM_NS, DIM_NS, DIM_NS);   // Y21 - Y21 G[i]
+            gy.mode = 2;
+            gy.Cin = Y21;
+            if (int r = gemm(h, true, true, 0, 0, gy)) return r;
+            std::swap(h->g4_m[2], h->g4_m[3]);
+            Y21 = h->g4_m[2];
+            T = h->g4_m[3];
+        }
+        if (int r = g4_left(h, X12, T2, lo, hi)) return r;          // BB[i] ...
+        if (int r = g4_right_inv(h, Y21, T2, lo, hi)) return r;     // ... BB[i]⁻¹
+        if (int r = prop_R_range(h, UR, UR, lo, hi)) return r;
+        if (int r = orthonormalise(h, UR)) return r;
+    }
+    if (int r = get_G_from(h, UR, UL[nb], Gi)) return r;            // G(τ1)
+    out[0] = Gi;
+    out[1] = G2;
+    out[2] = X12;
+    out[3] = Y21;
     return KDQ_OK;
 }
 
@@ -1337,6 +1507,42 @@ int kdq_update_layer(kdq_handle h, int32_t lt, int32_t dir, uint32_t sweep) {
     }
     HIPCHK(hipStreamSynchronize(h->stream));
     return KDQ_OK;
+}
+
+int kdq_G4(kdq_handle h, int32_t c, int32_t tau1, int32_t tau2, double *G1, double *G2, double *G12,
+           double *G21) {
+    if (!h || h->scee || c < 0 || c >= h->nch || !G1 || !G2 || !G12 || !G21)
+        return fail(h, KDQ_EARG, "kdq_G4: bad argument");
+    if (h->kind != KDQ_MODEL_TU) return fail(h, KDQ_EARG, "kdq_G4: time-displaced G is for t-U chains");
+    if (tau1 < 0 || tau1 > h->nt || tau2 < 0 || tau2 > h->nt) return fail(h, KDQ_EARG, "kdq_G4: slice out of range");
+    const bool swap = tau1 < tau2;
+    Opnd *res[4] = {};
+    if (int r = g4_compute(h, swap ? tau2 : tau1, swap ? tau1 : tau2, res)) return r;
+    if (int r = check_health(h)) return r;
+    const int n = h->ch[c].ns;
+    const size_t nn = (size_t)n * n;
+    std::vector<double> tmp(2 * nn);
+    auto fetch = [&](Opnd *tab, double *dst, bool minus_identity) -> int {
+        Opnd o;
+        HIPCHK(hipMemcpy(&o, tab + c, sizeof(Opnd), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(tmp.data(), o.re, 2 * nn * sizeof(double), hipMemcpyDeviceToHost));
+        for (size_t q = 0; q < nn; ++q) {
+            dst[2 * q] = tmp[q] - (minus_identity && q % (n + 1) == 0 ? 1.0 : 0.0);
+            dst[2 * q + 1] = tmp[nn + q];
+        }
+        return KDQ_OK;
+    };
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (tau1 == tau2) {   // G, G, -(I - G), G
+        if (int r = fetch(res[0], G1, false)) return r;
+        if (int r = fetch(res[0], G2, false)) return r;
+        if (int r = fetch(res[0], G12, true)) return r;
+        return fetch(res[0], G21, false);
+    }
+    if (int r = fetch(res[swap ? 1 : 0], G1, false)) return r;
+    if (int r = fetch(res[swap ? 0 : 1], G2, false)) return r;
+    if (int r = fetch(res[swap ? 3 : 2], G12, false)) return r;
+    return fetch(res[swap ? 2 : 3], G21, false);
 }
 
 int kdq_init_chain(kdq_handle h) {

@@ -148,6 +148,14 @@ class DQMCBatch:
         self._chk(self.L.kdq_get_G(self.h, c, _dp(buf)))
         return buf.view(np.complex128).reshape(n, n).T.copy()
 
+    def G4(self, c: int, tau1: int, tau2: int):
+        """G(τ1), G(τ2), G(τ1,τ2), G(τ2,τ1) of chain c from its current fields, as
+        G4(model, s, τ1, τ2) returns them (src/tU/GreenMatrix.jl:126-226; t-U)."""
+        n = self.models[c].Ns
+        bufs = [np.zeros(2 * n * n) for _ in range(4)]
+        self._chk(self.L.kdq_G4(self.h, c, int(tau1), int(tau2), *[_dp(b) for b in bufs]))
+        return [b.view(np.complex128).reshape(n, n).T.copy() for b in bufs]
+
     def set_G(self, c: int, G):
         n = self.models[c].Ns
         buf = np.ascontiguousarray(np.asarray(G, dtype=np.complex128).T).reshape(-1).view(np.float64).copy()

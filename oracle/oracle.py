@@ -207,6 +207,87 @@ def get_G(BL, BR):
     return G, ld.value
 
 
+# ------------------------------------------------ t-U time-displaced G (numpy restatement)
+def _tu_D(model, s, i, sign=1.0):
+    """diag(cis(±α_i η_s)) of slice i (1-based)."""
+    return np.exp(sign * 1j * model.α[i - 1] * np.asarray(model.η)[np.asarray(s)[:, i - 1].astype(int) - 1])
+
+
+def tu_Gtau(model, s, tau):
+    """Gτ(model, s, τ) (src/tU/GreenMatrix.jl:90-123): equal-time G at slice τ with a QR every
+    BatchSize slices and a final one, then I - BR (BL BR)⁻¹ BL."""
+    BL = np.asarray(model.Pt, dtype=complex).conj().T
+    BR = np.asarray(model.Pt, dtype=complex)
+    counter = 0
+    for i in range(model.Nt, tau, -1):
+        BL = (BL * _tu_D(model, s, i)[None, :]) @ model.eK
+        counter += 1
+        if counter == model.BatchSize:
+            counter = 0
+            BL = qr_q(BL.conj().T)[0].conj().T
+    counter = 0
+    for i in range(1, tau + 1):
+        BR = _tu_D(model, s, i)[:, None] * (model.eK @ BR)
+        counter += 1
+        if counter == model.BatchSize:
+            counter = 0
+            BR = qr_q(BR)[0]
+    BL = qr_q(BL.conj().T)[0].conj().T
+    BR = qr_q(BR)[0]
+    return np.eye(model.Ns) - BR @ np.linalg.solve(BL @ BR, BL)
+
+
+def tu_G4(model, s, tau1, tau2):
+    """G4(model, s, τ1, τ2) (src/tU/GreenMatrix.jl:125-226): G(τ1), G(τ2), G(τ1,τ2), G(τ2,τ1)
+    with the block products BBs of BatchSize slices between τ2 and τ1 and the equal-time G at
+    every block boundary."""
+    if tau1 < tau2:
+        G2, G1, G21, G12 = tu_G4(model, s, tau2, tau1)
+        return G1, G2, G12, G21
+    if tau1 == tau2:
+        G = tu_Gtau(model, s, tau1)
+        return G, G, -(np.eye(model.Ns) - G), G
+    bs, ns, np_ = model.BatchSize, model.Ns, model.Ns // 2
+    nb = -(-(tau1 - tau2) // bs)
+    BBs = [np.eye(ns, dtype=complex) for _ in range(nb)]
+    BBsInv = [np.eye(ns, dtype=complex) for _ in range(nb)]
+    UL = [None] * (nb + 1)
+    UR = [None] * (nb + 1)
+    UR0 = np.asarray(model.Pt, dtype=complex)
+    counter = 0
+    for i in range(1, tau2 + 1):
+        UR0 = _tu_D(model, s, i)[:, None] * (model.eK @ UR0)
+        counter += 1
+        if counter == bs:
+            counter = 0
+            UR0 = qr_q(UR0)[0]
+    UR[0] = qr_q(UR0)[0]
+    ULe = np.asarray(model.Pt, dtype=complex).conj().T
+    counter = 0
+    for i in range(model.Nt, tau1, -1):
+        ULe = (ULe * _tu_D(model, s, i)[None, :]) @ model.eK
+        counter += 1
+        if counter == bs:
+            counter = 0
+            ULe = qr_q(ULe.conj().T)[0].conj().T
+    UL[nb] = qr_q(ULe.conj().T)[0].conj().T
+    for b in range(nb):
+        last = tau1 if b == nb - 1 else tau2 + (b + 1) * bs
+        for j in range(tau2 + b * bs + 1, last + 1):
+            BBs[b] = _tu_D(model, s, j)[:, None] * (model.eK @ BBs[b])
+            BBsInv[b] = (BBsInv[b] @ model.eKinv) * _tu_D(model, s, j, -1.0)[None, :]
+    for i in range(1, nb + 1):
+        UL[nb - i] = qr_q((UL[nb - i + 1] @ BBs[nb - i]).conj().T)[0].conj().T
+        UR[i] = qr_q(BBs[i - 1] @ UR[i - 1])[0]
+    G = [np.eye(ns) - UR[i] @ np.linalg.solve(UL[i] @ UR[i], UL[i]) for i in range(nb + 1)]
+    G12 = np.eye(ns, dtype=complex)
+    G21 = -np.eye(ns, dtype=complex)
+    for i in range(1, nb + 1):
+        G12 = G12 @ BBs[nb - i] @ G[nb - i]
+        G21 = G21 @ (np.eye(ns) - G[i - 1]) @ BBsInv[i - 1]
+    return G[nb], G[0], G12, G21
+
+
 # -------------------------------------------------------------------------- t-U
 class OracleTU:
     """Oracle state for one t-U chain built from a host model (kapdqmc_amd.models)."""
