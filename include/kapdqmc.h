@@ -58,6 +58,35 @@ typedef struct kdq_model_desc {
     double U, V;             /* t-U-V: model.U, model.V (observable prefactors) */
 } kdq_model_desc;
 
+/* ---- host model construction (no GPU work) ----
+ * The keyword constructors tU_Hubbard_Para (src/tU/model.jl:39-96), tV_Hubbard_Para
+ * (src/tV/model.jl:30-122) and tUV_Hubbard_Para (src/tUV/model.jl:30-118) with the lattice
+ * geometry they call (nn2idx, nnK_Matrix, nnidx_F: src/public/Geometry.jl:8-207), so that a
+ * C or C++ caller fills the kdq_model_desc of kdq_create without Julia. flux = 0. */
+enum { KDQ_INITIAL_H0 = 0, KDQ_INITIAL_V = 1 };
+typedef struct kdq_model_params {
+    int32_t kind;            /* KDQ_MODEL_* */
+    int32_t lattice;         /* KDQ_LATTICE_* (t-V, t-U-V: honeycomb only) */
+    int32_t site[2];         /* site */
+    int32_t batch_size;      /* BatchSize */
+    int32_t initial;         /* Initial: KDQ_INITIAL_H0 ("H0") or KDQ_INITIAL_V ("V") */
+    double ht;               /* Ht */
+    double c1, c2;           /* t-U: Hu1, Hu2; t-V: Hv1, Hv2; t-U-V: Hu, Hv */
+    double dt;               /* Δt */
+    double theta_relax;      /* t-U, t-V: Θrelax; t-U-V: Θ */
+    double theta_quench;     /* t-U, t-V: Θquench; t-U-V: ignored */
+} kdq_model_params;
+typedef struct kdq_model_s *kdq_model;
+/* Builds the constants on the host and fills *desc with pointers into storage that *out
+ * owns until kdq_model_free. The two LAPACK.syevd! calls of each constructor are one
+ * symmetric eigensolver here (Householder tridiagonalisation, implicit-shift QR);
+ * eigenvector signs and bases inside degenerate levels may differ from LAPACK's, which
+ * changes no matrix function of K and no projector onto the trial state's span. The
+ * reference's @asserts (equal couplings without a quench, symmetric coupling profile,
+ * eK·eKinv = I, U² > 4z²V²) return KDQ_EARG with their message in kdq_last_error(NULL). */
+int kdq_model_build(const kdq_model_params *p, kdq_model *out, kdq_model_desc *desc);
+int kdq_model_free(kdq_model m);
+
 /* Per-chain run settings. The random stream is Philox4x32-10 keyed by `seed`, with the
  * chain id and the sweep counter in the counter words, so a run is bit-reproducible
  * and resumable from (fields, sweep counter). */

@@ -23,7 +23,7 @@ EXPORTS = ("kdq_create", "kdq_destroy", "kdq_last_error", "kdq_num_chains", "kdq
            "kdq_update_layer", "kdq_init_chain", "kdq_measure", "kdq_get_counters", "kdq_profile",
            "kdq_profile_read", "kdq_get_logdet", "kdq_set_option", "kdq_get_precision_error", "kdq_log_weight", "kdq_scee_create", "kdq_scee_set_fields",
            "kdq_scee_get_fields", "kdq_scee_init", "kdq_scee_sweeps", "kdq_scee_get_G4", "kdq_scee_detg",
-           "kdq_dop_create", "kdq_G4", "kdq_net_create", "kdq_net_destroy", "kdq_net_last_error", "kdq_net_num_params", "kdq_net_get_params",
+           "kdq_dop_create", "kdq_G4", "kdq_model_build", "kdq_model_free", "kdq_net_create", "kdq_net_destroy", "kdq_net_last_error", "kdq_net_num_params", "kdq_net_get_params",
            "kdq_net_set_params", "kdq_net_energy", "kdq_net_loss_grad", "kdq_net_train_step", "kdq_net_sample",
            "kdq_slmc_move")
 PROF_FAMILIES = ("wrap", "chain", "get_G", "decide", "flush", "qr", "inv", "meas_wrap", "measure",
@@ -45,6 +45,15 @@ class ModelDesc(C.Structure):
                 ("halfeK", C.POINTER(C.c_double)), ("halfeKinv", C.POINTER(C.c_double)),
                 ("Pt", C.POINTER(C.c_double)), ("z", C.c_int32), ("bonds", C.POINTER(C.c_int32)),
                 ("a", C.c_double), ("is_complex", C.c_int32), ("U", C.c_double), ("V", C.c_double)]
+
+
+class ModelParams(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("lattice", C.c_int32), ("site", C.c_int32 * 2), ("batch_size", C.c_int32),
+                ("initial", C.c_int32), ("ht", C.c_double), ("c1", C.c_double), ("c2", C.c_double),
+                ("dt", C.c_double), ("theta_relax", C.c_double), ("theta_quench", C.c_double)]
+
+
+INITIAL_CODE = {"H0": 0, "V": 1}
 
 
 class RunDesc(C.Structure):
@@ -104,6 +113,8 @@ def load(path: str = LIB_PATH):
     L.kdq_G4.argtypes = [vp, i32, i32, i32, dp, dp, dp, dp]
     L.kdq_dop_create.argtypes = [C.POINTER(ModelDesc), C.POINTER(RunDesc), C.POINTER(SceeDesc), C.c_double, i32,
                                  C.POINTER(vp)]
+    L.kdq_model_build.argtypes = [C.POINTER(ModelParams), C.POINTER(vp), C.POINTER(ModelDesc)]
+    L.kdq_model_free.argtypes = [vp]
     fp = C.POINTER(C.c_float)
     u8p = C.POINTER(C.c_uint8)
     L.kdq_net_create.argtypes = [i32, i32, i32, i32, C.c_uint64, i32, C.POINTER(vp)]

@@ -13,7 +13,8 @@ HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",
 
 
 def sources():
-    return sorted(glob.glob(os.path.join(HERE, "csrc", "*.hip")))
+    # *.hip: device kernels and the handle API; *.cpp: host-only parts (model construction)
+    return sorted(glob.glob(os.path.join(HERE, "csrc", "*.hip")) + glob.glob(os.path.join(HERE, "csrc", "*.cpp")))
 
 
 def build(verbose: bool = False) -> str:

@@ -53,6 +53,9 @@ struct Table {  // one logical operand across chains
 
 }  // namespace
 
+// The host model builder (kdq_model.cpp) reports through the same global message.
+void kdq_internal_set_error(const std::string &msg) { set_global_error(msg); }
+
 struct kdq_handle_s {
     int nch = 0, nt = 0, nn = 0, max_ns = 0, max_np = 0, delay = 16;
     int kind = KDQ_MODEL_TU, nobs = 6, max_elem = 0, max_nb = 0;

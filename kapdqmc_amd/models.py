@@ -252,3 +252,60 @@ def tUV_Hubbard_Para(*, Ht, Hu, Hv, Lattice: str, site, Δt, Θ, BatchSize,
     return tUV_Hubbard_Para_(Lattice, float(Ht), float(Hu), float(Hv), a, Type, site, float(Θ),
                              Ns, int(Nt), K, int(BatchSize), float(Δt), GAMMA.copy(), eta, Pt,
                              HalfeK, eK, HalfeKinv, eKinv, nnidx, _nodes(int(Nt), BatchSize))
+
+
+def native_model(kind: str, **kw):
+    """The same constructors run by the library's C++ builder (``kdq_model_build``): the
+    keyword arguments of ``tU_Hubbard_Para`` / ``tV_Hubbard_Para`` / ``tUV_Hubbard_Para``,
+    the same struct back. What a C or C++ caller without Python gets; Python callers may use
+    either. The eigenvectors come from the library's own eigensolver, so ``Pt`` equals the
+    numpy-built one up to a rotation inside its span (the same projector)."""
+    import ctypes as C
+
+    from . import _lib as L
+
+    lib = L.load()
+    p = L.ModelParams()
+    p.kind = {"tU": L.MODEL_TU, "tV": L.MODEL_TV, "tUV": L.MODEL_TUV}[kind]
+    p.lattice = L.LATTICE_CODE[kw["Lattice"]]
+    p.site[0], p.site[1] = kw["site"]
+    p.batch_size = kw["BatchSize"]
+    p.initial = L.INITIAL_CODE[kw.get("Initial", "H0")]
+    p.ht, p.dt = kw["Ht"], kw["Δt"]
+    if kind == "tUV":
+        p.c1, p.c2, p.theta_relax = kw["Hu"], kw["Hv"], kw["Θ"]
+    else:
+        c = "Hu" if kind == "tU" else "Hv"
+        p.c1, p.c2 = kw[c + "1"], kw[c + "2"]
+        p.theta_relax, p.theta_quench = kw["Θrelax"], kw["Θquench"]
+    h = C.c_void_p()
+    d = L.ModelDesc()
+    L.check(lib.kdq_model_build(C.byref(p), C.byref(h), C.byref(d)))
+    try:
+        ns, nt = d.ns, d.nt
+        mat = lambda ptr, cols=None: np.ctypeslib.as_array(ptr, shape=((cols or ns) * ns,)).reshape(
+            (cols or ns, ns)).T.copy()
+        vec = lambda ptr, n: np.ctypeslib.as_array(ptr, shape=(n,)).copy()
+        K, eK, eKinv = mat(d.K), mat(d.eK), mat(d.eKinv)
+        hK, hKinv, Pt = mat(d.halfeK), mat(d.halfeKinv), mat(d.Pt, ns // 2)
+        nodes = vec(d.nodes, d.n_nodes)
+        eta, gamma = vec(d.eta, 4), vec(d.gamma, 4)
+        site = list(kw["site"])
+        if kind == "tU":
+            return tU_Hubbard_Para_(kw["Lattice"], float(p.ht), float(p.c1), float(p.c2), site,
+                                    float(p.theta_relax), float(p.theta_quench), ns, nt, K, int(p.batch_size),
+                                    float(p.dt), vec(d.alpha, nt), gamma, eta, Pt, hK, eK, hKinv, eKinv, nodes)
+        bonds = vec(d.bonds, 2 * 3 * (ns // 2 if kind == "tV" else ns)).astype(np.int64)
+        if kind == "tV":
+            nnidx = bonds.reshape(3, ns // 2, 2).transpose(1, 0, 2).copy()
+            return tV_Hubbard_Para_(kw["Lattice"], float(p.ht), float(p.c1), float(p.c2), site,
+                                    float(p.theta_relax), float(p.theta_quench), ns, nt, K, int(p.batch_size),
+                                    float(p.dt), vec(d.alpha, nt), gamma, eta, Pt, hK, eK, hKinv, eKinv, nnidx,
+                                    nodes)
+        nnidx = bonds.reshape(3, ns, 2).transpose(1, 0, 2).copy()
+        return tUV_Hubbard_Para_(kw["Lattice"], float(p.ht), float(p.c1), float(p.c2), d.a,
+                                 complex if d.is_complex else float, site, float(p.theta_relax), ns, nt, K,
+                                 int(p.batch_size), float(p.dt), gamma, eta, Pt, hK, eK, hKinv, eKinv, nnidx,
+                                 nodes)
+    finally:
+        lib.kdq_model_free(h)
