@@ -156,7 +156,8 @@ def run_batch(m, nch: int, seed0: int, delay, warmup: int, steps: int, profile_a
     return {"rows": np.asarray(rows), "dt": dt, "prof": prof, "dominant": dominant, "delay": b.delay,
             "max_g_error": float(err.max()),
             "tuned": tuned, "breakdown": breakdown,
-            "acceptance": float(cnt[:, 1].sum() / max(cnt[:, 0].sum(), 1))}
+            "acceptance": float(cnt[:, 1].sum() / max(cnt[:, 0].sum(), 1)),
+            "decisions": int(cnt[:, 0].sum()), "near_ties": int(b.near_ties().sum())}
 
 
 def main():
@@ -301,6 +302,10 @@ def main():
         "tau_int_double_occupancy": tau,
         "double_occupancy_mean": float(D.mean()),
         "acceptance": bat["acceptance"],
+        "near_ties": {"count": bat["near_ties"], "decisions": bat["decisions"],
+                      "note": "Metropolis decisions of the whole run with |uniform draw - acceptance "
+                              "probability| < 1e-12, the only ones a rounding difference to the "
+                              "reference may flip"},
         "plain_fp64_1chain": plain,
         "ratio_vs_plain": (ess_s / plain["effective_samples_per_s"]) if plain else None,
         "mixed_precision": mixed,

@@ -159,6 +159,15 @@ int kdq_G4(kdq_handle h, int32_t c, int32_t tau1, int32_t tau2, double *G1, doub
 /* Counters: out[0] = proposals, out[1] = accepted, per chain (nchains x 2). */
 int kdq_get_counters(kdq_handle h, double *out);
 
+/* Near ties, per chain (nchains doubles; swap-replica handles: one per replica entry, chain c
+ * of replica r at c + r * nchains): Metropolis decisions whose uniform draw u and acceptance
+ * probability p differ by less than 1e-12. Those are the decisions that rounding differences
+ * between two correct FP64 implementations (this library, the Julia reference on OpenBLAS)
+ * may flip; all other decisions are compared one by one. Reference: the accept test
+ * `rand(rng) < p` of UpdatePhyLayer! (src/tU/phy_update.jl:262, src/tV/phy_update.jl:200,
+ * src/tUV/phy_update.jl:198) and UpdateSCEELayer! (src/tU/SCEE.jl:292,313). */
+int kdq_get_near_ties(kdq_handle h, double *out);
+
 /* Run options (no counterpart in the reference):
  *  KDQ_OPT_GRAPH      1 (default): kdq_phy_update captures one sweep into a HIP graph (on
  *                     the first call, or when `record` changes) and replays it; 0: launches

@@ -50,7 +50,7 @@ struct ChainDev {
     int *acc_cnt;          // [2]: accepted count of a block, slot by block parity (t-U sweep)
     int *acc_idx;          // [2][64]: accepted sites of a block, by slot
     double *W;             // [2 slots][2 planes (re, im)] of 64 x 64, ld 64
-    double *counters;      // [2]: proposals, accepted
+    double *counters;      // [3]: proposals, accepted, near ties (|u - p| < KDQ_NEAR_TIE)
     double *meas;          // [KDQ_MEAS_SLOTS]: accumulated observables, count in the last slot
     // model kind and field layout
     int kind;              // 0 t-U, 1 t-V, 2 t-U-V
@@ -68,6 +68,9 @@ struct ChainDev {
 };
 
 #define KDQ_MEAS_SLOTS 16
+// A decision whose uniform draw u and acceptance probability p differ by less than this is a
+// near tie: rounding differences between two correct implementations can flip it.
+#define KDQ_NEAR_TIE 1e-12
 
 // DIM_ACC: the accepted count of the current update block (device counter), the rank of a
 // delayed-update flush.

@@ -1192,7 +1192,7 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
             ok &= dalloc(C, &C.DU, 2 * nn) == hipSuccess;
             ok &= dalloc(C, &C.DV, 2 * nn) == hipSuccess;
         }
-        ok &= dalloc(C, &C.counters, 2) == hipSuccess;
+        ok &= dalloc(C, &C.counters, 3) == hipSuccess;
         ok &= dalloc(C, &C.meas, KDQ_MEAS_SLOTS) == hipSuccess;
         ok &= dalloc(C, &C.logdet, 1) == hipSuccess;
         if (h->mixed) {
@@ -1719,6 +1719,14 @@ int kdq_get_counters(kdq_handle h, double *out) {
     if (!h || !out) return fail(h, KDQ_EARG, "kdq_get_counters: bad argument");
     for (int c = 0; c < h->nch; ++c)
         HIPCHK(hipMemcpyAsync(out + 2 * c, h->ch[c].counters, 2 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return KDQ_OK;
+}
+
+int kdq_get_near_ties(kdq_handle h, double *out) {
+    if (!h || !out) return fail(h, KDQ_EARG, "kdq_get_near_ties: bad argument");
+    for (int c = 0; c < h->nch; ++c)
+        HIPCHK(hipMemcpyAsync(out + c, h->ch[c].counters + 2, sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     return KDQ_OK;
 }

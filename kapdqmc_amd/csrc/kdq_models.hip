@@ -118,7 +118,7 @@ __global__ __launch_bounds__(256) void k_decide_r2(DecideR2Args a) {
     }
     __syncthreads();
 
-    int na = 0;
+    int na = 0, nties = 0;
     for (int m = 0; m < ne; ++m) {
         const int si = sv[m];
         const int cidx = chs[m];
@@ -151,6 +151,7 @@ __global__ __launch_bounds__(256) void k_decide_r2(DecideR2Args a) {
         deti -= ti;
         double prob = (kind == 1) ? detr : (detr * detr + deti * deti);
         prob *= ftab[k];
+        nties += fabs(us[m] - prob) < KDQ_NEAR_TIE;
         if (!(us[m] < prob)) continue;
         // w = r⁻¹ Δ
         const double d2 = detr * detr + deti * deti;
@@ -309,6 +310,7 @@ __global__ __launch_bounds__(256) void k_decide_r2(DecideR2Args a) {
         *ch.acc_cnt = na;
         ch.counters[0] += ne;
         ch.counters[1] += na / 2;
+        ch.counters[2] += nties;
     }
     if (tid < na) ch.acc_idx[tid] = gsite[acc[tid]];
     for (int e = tid; e < na * na; e += 256) {

@@ -141,7 +141,7 @@ __global__ __launch_bounds__(ST) void k_scee_decide(SceeArgs a) {
     }
     __syncthreads();
 
-    int na[2] = {0, 0};
+    int na[2] = {0, 0}, nties[2] = {0, 0};
     const double car = 1.0 - R.ea[0], cai = -R.ea[1];   // 1 - e^{iα} (disorder operator)
     for (int m = 0; m < kb; ++m) {
         for (int r = 0; r < NR; ++r) {
@@ -212,6 +212,7 @@ __global__ __launch_bounds__(ST) void k_scee_decide(SceeArgs a) {
             const double tAr = 1.0 + acc[0], tAi = acc[1], tBr = 1.0 + acc[2], tBi = acc[3];
             const double dA = tAr * tAr + tAi * tAi, dB = tBr * tBr + tBi * tBi;
             prob *= pow(dA, R.lambda) * pow(dB, 1.0 - R.lambda);
+            nties[r] += fabs(PD(5, r, m) - prob) < KDQ_NEAR_TIE;
             if (!(PD(5, r, m) < prob)) continue;
             // ---- accepted: gmInv a, and publish the replica's rows / columns at m
             for (int q = tid; q < nX; q += ST) {
@@ -320,6 +321,7 @@ __global__ __launch_bounds__(ST) void k_scee_decide(SceeArgs a) {
             *chr.acc_cnt = na[r];
             chr.counters[0] += kb;
             chr.counters[1] += na[r];
+            chr.counters[2] += nties[r];
         }
         if (tid < na[r]) chr.acc_idx[tid] = s0 + accl[r][tid];
         for (int e = tid; e < na[r] * na[r]; e += ST) {
@@ -732,6 +734,7 @@ __global__ __launch_bounds__(RT) void k_scee_r2(SceeR2Args a) {
                 }
                 const double p = prob_s * pow(dX[0], R.lambda) * pow(dX[1], 1.0 - R.lambda);
                 const int ok = u < p;
+                if (fabs(u - p) < KDQ_NEAR_TIE) ch.counters[2] += 1.0;
                 acc_s = ok;
                 dreg[0] = dX[0];
                 dreg[1] = dX[1];

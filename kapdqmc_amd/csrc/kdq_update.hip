@@ -237,7 +237,7 @@ __global__ __launch_bounds__(KBT >= 32 ? 1024 : KBT * KBT) void k_decide_tu(Deci
         }
     __syncthreads();
 
-    int na = 0;
+    int na = 0, nties = 0;
     for (int m = 0; m < kb; ++m) {
         const double Dr = dR[m], Di = dI[m];
         // p = 1 + Δ (1 - G_mm)
@@ -247,6 +247,7 @@ __global__ __launch_bounds__(KBT >= 32 ? 1024 : KBT * KBT) void k_decide_tu(Deci
         pr += 1.0;
         const double p2 = pr * pr + pi * pi;
         const double prob = p2 * pg[m];
+        nties += fabs(us[m] - prob) < KDQ_NEAR_TIE;
         if (!(us[m] < prob)) continue;
         // r = Δ / p
         const double ip2 = 1.0 / p2;
@@ -325,6 +326,7 @@ __global__ __launch_bounds__(KBT >= 32 ? 1024 : KBT * KBT) void k_decide_tu(Deci
         ch.acc_cnt[a.slot] = na;
         ch.counters[0] += kb;
         ch.counters[1] += na;
+        ch.counters[2] += nties;
         if (a.deferred) {
             const int rk = (a.blk > 0 ? ch.rk[a.blk] : 0) + na;
             ch.rk[a.blk + 1] = rk;

@@ -194,6 +194,13 @@ class DQMCBatch:
         self._chk(self.L.kdq_get_counters(self.h, _dp(out)))
         return out
 
+    def near_ties(self) -> np.ndarray:
+        """Per chain: decisions so far with |uniform draw − acceptance probability| < 1e-12,
+        the only ones a rounding difference between two FP64 implementations may flip."""
+        out = np.zeros(self.n)
+        self._chk(self.L.kdq_get_near_ties(self.h, _dp(out)))
+        return out
+
     def set_option(self, name: str, value: int):
         """kdq_set_option: "graph" (0/1), "delay" (t-U block size), "gemm_tile" (0, 32, 64)."""
         self._chk(self.L.kdq_set_option(self.h, L.OPTIONS[name], int(value)))
@@ -359,6 +366,14 @@ class SCEEBatch:
         bufs = [np.zeros(2 * ns * ns) for _ in range(4)]   # interleaved, column-major
         self._chk(self.L.kdq_scee_get_G4(self.h, c, replica, *[_dp(x) for x in bufs]))
         return [(x[0::2] + 1j * x[1::2]).reshape((ns, ns), order="F") for x in bufs]   # Gt, G0, Gt0, G0t
+
+    def near_ties(self) -> np.ndarray:
+        """[chains, replicas]: decisions so far with |uniform draw − acceptance probability|
+        < 1e-12."""
+        nrep = 1 if isinstance(self, DOPBatch) else 2
+        out = np.zeros(nrep * self.n)
+        self._chk(self.L.kdq_get_near_ties(self.h, _dp(out)))
+        return out.reshape(nrep, self.n).T.copy()
 
     def detg(self) -> np.ndarray:
         out = np.zeros((self.n, 2))

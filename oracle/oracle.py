@@ -323,7 +323,8 @@ class OracleTU:
                     ("Pt", _dp), ("s", C.POINTER(C.c_uint8)), ("G", _dp), ("BLs", _dp), ("BRs", _dp),
                     ("tmpNN", _dp), ("tmpNN2", _dp), ("BM", _dp), ("tmpNn", _dp), ("tmpnN", _dp),
                     ("tmpN", _dp), ("idx", C.c_int), ("seed", C.c_uint64), ("chain", C.c_uint32),
-                    ("sweep", C.c_uint32), ("n_accept", C.c_longlong), ("n_propose", C.c_longlong)]
+                    ("sweep", C.c_uint32), ("n_accept", C.c_longlong), ("n_propose", C.c_longlong),
+                    ("n_ties", C.c_longlong)]
 
     @property
     def _st(self):
@@ -367,6 +368,11 @@ class OracleTU:
     def acceptance(self):
         st = self._st
         return st.n_accept / max(st.n_propose, 1)
+
+    @property
+    def near_ties(self):
+        """Decisions so far whose draw and acceptance probability differ by < 1e-12."""
+        return self._st.n_ties
 
     @property
     def idx(self):
@@ -423,7 +429,7 @@ class OracleTV:
                     ("BRs", _dp), ("T", _dp), ("T2", _dp), ("BM", _dp), ("tNn", _dp), ("tnN", _dp),
                     ("tN", _dp), ("idx", C.c_int), ("seed", C.c_uint64), ("chain", C.c_uint32),
                     ("sweep", C.c_uint32), ("n_accept", C.c_longlong), ("n_propose", C.c_longlong),
-                    ("n_negative", C.c_longlong)]
+                    ("n_negative", C.c_longlong), ("n_ties", C.c_longlong)]
 
     def __init__(self, model, seed: int = 0, chain: int = 0):
         self.model = model
@@ -482,6 +488,11 @@ class OracleTV:
         st = self._st
         return st.n_accept / max(st.n_propose, 1)
 
+    @property
+    def near_ties(self):
+        """Decisions so far whose draw and acceptance probability differ by < 1e-12."""
+        return self._st.n_ties
+
     def init_chain(self):
         lib().orc_tv_init_chain(self.h)
 
@@ -519,7 +530,7 @@ class OracleTUV:
                     ("s", C.POINTER(C.c_uint8)), ("G", _dp), ("BLs", _dp), ("BRs", _dp), ("T", _dp),
                     ("T2", _dp), ("BM", _dp), ("tNn", _dp), ("tnN", _dp), ("tN", _dp), ("idx", C.c_int),
                     ("seed", C.c_uint64), ("chain", C.c_uint32), ("sweep", C.c_uint32),
-                    ("n_accept", C.c_longlong), ("n_propose", C.c_longlong)]
+                    ("n_accept", C.c_longlong), ("n_propose", C.c_longlong), ("n_ties", C.c_longlong)]
 
     def __init__(self, model, seed: int = 0, chain: int = 0):
         self.model = model
@@ -579,6 +590,11 @@ class OracleTUV:
     def acceptance(self):
         st = self._st
         return st.n_accept / max(st.n_propose, 1)
+
+    @property
+    def near_ties(self):
+        """Decisions so far whose draw and acceptance probability differ by < 1e-12."""
+        return self._st.n_ties
 
     def init_chain(self):
         lib().orc_tuv_init_chain(self.h)

@@ -29,6 +29,10 @@
 #include <complex.h>
 #include <stdint.h>
 
+/* A decision whose uniform draw u and acceptance probability p differ by less than this is a
+ * near tie: rounding differences between two correct implementations can flip it. */
+#define ORC_NEAR_TIE 1e-12
+
 typedef double complex zdouble;
 
 /* ---------------- random numbers (shared specification with the HIP library) ------ */
@@ -75,6 +79,7 @@ typedef struct orc_tu {
     uint64_t seed;
     uint32_t chain, sweep;  /* RNG stream position */
     long long n_accept, n_propose;
+    long long n_ties;       /* decisions with |u - p| < ORC_NEAR_TIE */
 } orc_tu;
 
 orc_tu *orc_tu_new(int ns, int nt, int bs, int nn_nodes, const int64_t *nodes, int lattice,
@@ -109,6 +114,7 @@ typedef struct orc_tv {
     uint64_t seed;
     uint32_t chain, sweep;
     long long n_accept, n_propose, n_negative;
+    long long n_ties;
 } orc_tv;
 
 orc_tv *orc_tv_new(int ns, int nt, int bs, int nn_nodes, const int64_t *nodes, int lattice,
@@ -140,6 +146,7 @@ typedef struct orc_tuv {
     uint64_t seed;
     uint32_t chain, sweep;
     long long n_accept, n_propose;
+    long long n_ties;
 } orc_tuv;
 
 orc_tuv *orc_tuv_new(int ns, int nt, int bs, int nn_nodes, const int64_t *nodes, int lattice,

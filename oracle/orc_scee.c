@@ -372,6 +372,7 @@ static void update_layer(orc_scee *e, int lt, int dir, uint32_t sweep) {
             double dA = creal(tA * conj(tA)), dB = creal(tB * conj(tB));
             prob *= pow(dA, e->lambda) * pow(dB, 1.0 - e->lambda);
             m->n_propose++;
+            if (fabs(u - prob) < ORC_NEAR_TIE) m->n_ties++;
             if (u < prob) {
                 m->n_accept++;
                 e->A.detg *= dA;

@@ -551,6 +551,7 @@ static void update_elems(orc_scee2 *e, int lt, int dir, uint32_t sweep, int e0, 
             const double dA = e->kind == 1 ? creal(tA) : creal(tA * conj(tA));
             const double dB = e->kind == 1 ? creal(tB) : creal(tB * conj(tB));
             prob *= pow(dA, e->lambda) * pow(dB, 1.0 - e->lambda);
+            if (fabs(u - prob) < ORC_NEAR_TIE) ++*(e->kind == 1 ? &e->tv->n_ties : &e->tuv->n_ties);
             if (u < prob) {
                 e->A.detg *= dA;
                 e->B.detg *= dB;

@@ -163,6 +163,7 @@ void orc_tu_update_layer(orc_tu *m, int lt, int dir) {
         zdouble r = Delta / p;
         double prob = creal(p * conj(p)) * (m->gamma[sx - 1] / m->gamma[si - 1]);
         m->n_propose++;
+        if (fabs(u - prob) < ORC_NEAR_TIE) m->n_ties++;
         if (u < prob) {
             m->n_accept++;
             /* zN = r * (e_i - G[i,:]); G -= G[:,i] * zN */

@@ -132,6 +132,7 @@ void orc_tuv_update_layer(orc_tuv *m, int lt, int dir) {
         double prob = creal(det * conj(det)) * (m->gamma[sx - 1] / m->gamma[si - 1]);
         if (!m->is_complex) prob *= exp((m->eta[si - 1] - m->eta[sx - 1]) * (m->a + 1));
         m->n_propose++;
+        if (fabs(u - prob) < ORC_NEAR_TIE) m->n_ties++;
         if (u < prob) {
             m->n_accept++;
             /* w = r^{-1} Δ  (inv22!) */

@@ -210,6 +210,7 @@ void orc_tv_update_family(orc_tv *m, int lt, int dir, int j) {
         double prob = p * (m->gamma[sx - 1] / m->gamma[si - 1]);
         if (prob < -1e-3) m->n_negative++;
         m->n_propose++;
+        if (fabs(u - prob) < ORC_NEAR_TIE) m->n_ties++;
         if (u < prob) {
             m->n_accept++;
             /* r <- inv(r) Δ */

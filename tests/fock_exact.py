@@ -1,0 +1,127 @@
+"""Exact finite-Δτ expectation values of the t-U projector DQMC on a tiny lattice, computed
+in Fock space (test infrastructure: an independent check of the sampled distribution, not a
+restatement of the reference's algorithm).
+
+The t-U sweep (`src/tU/phy_update.jl:3-152`) samples HS fields s with weight
+|det(Pᵀ B(2Θ,0) P)|² Π γ_s, B_l = D_l eK, D_l = diag(e^{iα η_s}) (`src/tU/GreenMatrix.jl:10-27`,
+`get_r!` `:49-54`). |det|² is the weight of the particle-hole transformed problem: spin up with
+phases e^{+iαη n↑}, spin down (c↓ → ±c↓† on the two sublattices, hopping unchanged on the
+bipartite honeycomb) with e^{-iαη ñ↓}, both with the trial Slater determinant of P. Summing the
+fields site by site turns the sampled ensemble into the Fock-space transfer operator
+
+    T = V_HS · e^{-Δτ H_kin},   V_HS = Π_i ¼ Σ_l γ_l cos(α η_l (ñ↑_i − ñ↓_i)),
+
+with no Trotter or HS approximation left: the sampled average of any per-field estimator
+equals ⟨ψ_T| T^{Nt−M} e^{+ΔτK/2} Ô e^{−ΔτK/2} T^{M} |ψ_T⟩ / ⟨ψ_T| T^{Nt} |ψ_T⟩ with M = Nt/2
+and the HalfeK sandwich of `phy_measure` (`src/tU/phy_update.jl:181-182`). The observables are
+the kinetic energy Ek = ⟨Ht Σ_σ Σ_ij K_ij c†_iσ c_jσ⟩ (`:184`) and Eu = Σ_i ⟨n_i↑ n_i↓⟩
+(`:185-190`), in the original basis n_i↓ = 1 − ñ_i↓.
+
+The many-body state is a matrix ψ[a, b] over (spin-up configuration a) x (spin-down
+configuration b), each a set of Np occupied sites in ascending (Jordan-Wigner) order; a
+one-species operator X acts as X ψ on the up side and ψ Xᵀ on the down side.
+"""
+import itertools
+
+import numpy as np
+import scipy.linalg as la
+
+
+def _configurations(N: int, Np: int):
+    confs = list(itertools.combinations(range(N), Np))
+    occ = np.zeros((len(confs), N))
+    for a, c in enumerate(confs):
+        occ[a, list(c)] = 1.0
+    return confs, occ
+
+
+def hopping_fock(K: np.ndarray, Ht: float, Np: int) -> np.ndarray:
+    """Ht Σ_ij K_ij c†_i c_j on the Np-particle space of one species (JW signs)."""
+    N = K.shape[0]
+    confs, _ = _configurations(N, Np)
+    index = {c: a for a, c in enumerate(confs)}
+    T = np.zeros((len(confs), len(confs)))
+    for a, c in enumerate(confs):
+        occ = set(c)
+        for i in range(N):
+            for j in range(N):
+                if i == j or K[i, j] == 0 or j not in occ or i in occ:
+                    continue
+                rest = sorted(occ - {j})
+                sign = (-1) ** (sum(x < j for x in rest) + sum(x < i for x in rest))
+                T[index[tuple(sorted(rest + [i]))], a] += Ht * K[i, j] * sign
+    return T
+
+
+def slater(P: np.ndarray) -> np.ndarray:
+    """Amplitudes ⟨a|ψ_P⟩ = det P[a, :] of the Slater determinant of the columns of P."""
+    confs, _ = _configurations(P.shape[0], P.shape[1])
+    return np.array([np.linalg.det(P[list(c), :]) for c in confs])
+
+
+def ground_state(K: np.ndarray, Ht: float, U: float):
+    """Lowest eigenpair of Ht K (both spins) + U/2 Σ_i (n_i − 1)² at half filling (exact
+    diagonalisation); returns (E0, Ek, Eu) with Eu = Σ_i ⟨n_i↑ n_i↓⟩."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as sla
+    N = K.shape[0]
+    Np = N // 2
+    _, occ = _configurations(N, Np)
+    T = sp.csr_matrix(hopping_fock(K, Ht, Np))
+    one = sp.identity(T.shape[0], format="csr")
+    Hk = sp.kron(T, one) + sp.kron(one, T)
+    nup = np.repeat(occ, len(occ), axis=0)
+    ndn = np.tile(occ, (len(occ), 1))
+    H = Hk + sp.diags(0.5 * U * ((nup + ndn - 1.0) ** 2).sum(1))
+    w, v = sla.eigsh(H, k=1, which="SA")
+    g = v[:, 0]
+    return float(w[0]), float(g @ (Hk @ g)), float((g * g) @ (nup * ndn).sum(1))
+
+
+def transfer_diagonal(occ: np.ndarray, alpha: float, eta, gamma) -> np.ndarray:
+    """V_HS[a, b] = Π_i ¼ Σ_l γ_l cos(α η_l (ñ↑_i − ñ↓_i)) over up/down configurations."""
+    x = occ[:, None, :] - occ[None, :, :]
+    return sum(0.25 * g * np.cos(alpha * e * x) for g, e in zip(gamma, eta)).prod(axis=2)
+
+
+def field_weight(m, s: np.ndarray) -> float:
+    """⟨ψ_T| Π_l D↑_l(s) E ⊗ D↓_l(s) E |ψ_T⟩ for one field s (no sum over fields): equals
+    |det(Pᵀ B(2Θ,0) P)|² of the DQMC weight, which pins the particle-hole picture above."""
+    N, Np = m.Ns, m.Ns // 2
+    _, occ = _configurations(N, Np)
+    E = la.expm(-m.Δt * hopping_fock(m.K, m.Ht, Np))
+    amp = slater(m.Pt)
+    psi = np.outer(amp, amp).astype(np.complex128)
+    for lt in range(m.Nt):
+        ph = m.α[lt] * np.asarray(m.η)[s[:, lt] - 1]           # per-site phase α η_s
+        up = np.exp(1j * occ @ ph)
+        dn = np.exp(-1j * occ @ ph)
+        psi = up[:, None] * (E @ psi @ E.T) * dn[None, :]
+    return float(np.real((np.outer(amp, amp) * psi).sum()))
+
+
+def expectation(m):
+    """Exact (Ek, Eu) of the distribution the t-U sweep samples, measured at slice Nt/2."""
+    N, Np = m.Ns, m.Ns // 2
+    if not np.allclose(m.α, m.α[0]):
+        raise ValueError("constant coupling profiles only")
+    _, occ = _configurations(N, Np)
+    T = hopping_fock(m.K, m.Ht, Np)
+    E = la.expm(-m.Δt * T)
+    Eh, Ehi = la.expm(-0.5 * m.Δt * T), la.expm(0.5 * m.Δt * T)
+    V = transfer_diagonal(occ, m.α[0], m.η, m.γ)
+    amp = slater(m.Pt)
+    R = np.outer(amp, amp)
+    Lv = R.copy()
+    M = m.Nt // 2
+    for _ in range(M):
+        R = V * (E @ R @ E.T)
+    for _ in range(m.Nt - M):
+        Lv = E.T @ (V * Lv) @ E
+    R = Eh @ R @ Eh.T
+    Lv = Ehi.T @ Lv @ Ehi
+    norm = (Lv * R).sum()
+    Ek = (Lv * (T @ R + R @ T.T)).sum() / norm
+    dop = (occ[:, None, :] * (1.0 - occ[None, :, :])).sum(2)       # n↑ (1 − ñ↓) summed over sites
+    Eu = (Lv * R * dop).sum() / norm
+    return float(Ek), float(Eu)

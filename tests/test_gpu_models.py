@@ -43,6 +43,7 @@ def test_tv_phy_update_parity(dq, oracle_mod, L, delay):
     ro = o.phy_update(2, True)
     rb = b.phy_update(2, True)[0]
     assert np.array_equal(b.get_fields(0), o.s)
+    assert b.near_ties()[0] == o.near_ties
     G = b.get_G(0)
     assert np.abs(G.imag).max() < 1e-12
     assert rel(G.real, o.G) < G_TOL
@@ -60,6 +61,7 @@ def test_tuv_phy_update_parity(dq, oracle_mod, U, delay):
     ro = o.phy_update(2, True)
     rb = b.phy_update(2, True)[0]
     assert np.array_equal(b.get_fields(0), o.s)
+    assert b.near_ties()[0] == o.near_ties
     assert rel(b.get_G(0), o.G) < G_TOL
     assert np.allclose(rb, ro, rtol=1e-9, atol=1e-11)
 

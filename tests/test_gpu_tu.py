@@ -1,7 +1,8 @@
 """GPU parity of the t-U projector-DQMC path against the FP64 oracle, through the C ABI.
 
-Bars: fields (accept/reject decisions with identical Philox draws) bit-exact; G within
-1e-10 relative (max |ΔG| / max |G|); measurement rows within 1e-9 relative.
+Bars: fields (accept/reject decisions with identical Philox draws) bit-exact, near-tie counts
+(|u − p| < 1e-12) equal; G within 1e-10 relative (max |ΔG| / max |G|); measurement rows
+within 1e-9 relative.
 """
 import numpy as np
 import pytest
@@ -103,6 +104,7 @@ def test_phy_update_parity(dq, oracle_mod, L, delay):
     ro = o.phy_update(2, True)
     rb = b.phy_update(2, True)[0]
     assert np.array_equal(b.get_fields(0), o.s)
+    assert b.near_ties()[0] == o.near_ties
     assert rel(b.get_G(0), o.G) < G_TOL
     assert np.allclose(rb, ro, rtol=1e-9, atol=1e-11)
 
@@ -123,6 +125,7 @@ def test_batched_uneven_chains(dq, oracle_mod):
     for c, o in enumerate(os_):
         ro = o.phy_update(1, True)
         assert np.array_equal(b.get_fields(c), o.s)
+        assert b.near_ties()[c] == o.near_ties
         assert rel(b.get_G(c), o.G) < G_TOL
         assert np.allclose(rb[c], ro, rtol=1e-9, atol=1e-11)
 
