@@ -277,7 +277,10 @@ int kdq_profile_read(kdq_handle h, double *ms, int64_t *launches);
  * A translation-equivariant convolutional effective model E(s) of the HS field on an
  * L1 x L2 honeycomb lattice with nt slices (3 x 3 x 3 convolutions over (slice, y, x) with
  * 8 -> C -> C -> 1 channels, tanh; periodic in x and y, zero-padded in the slice direction),
- * trained on (field, log W) pairs so that E + log W is as constant as possible. */
+ * trained on (field, log W) pairs so that E + log W is as constant as possible.
+ * Precision: the parameters (kdq_net_get_params / set_params, Adam) are FP32 masters; the
+ * kernels read the weights W1, W2, W3 rounded to BF16 (nearest even) and accumulate in FP32
+ * with FP32 biases, so E and its gradient are those of the network with the rounded weights. */
 typedef struct kdq_net_s *kdq_net;
 int kdq_net_create(int32_t L1, int32_t L2, int32_t nt, int32_t channels, uint64_t seed, int32_t max_batch,
                    kdq_net *out);

@@ -4,13 +4,17 @@
 //
 // Field tensor: site i of the honeycomb lattice is unit cell i/2 = x + L1 y, sublattice i%2;
 // channel 4·sublattice + (s - 1) is the one-hot value of s at (slice t, y, x). Layout
-// [b][channel][t][y][x], x fastest. Network (FP32):
+// [b][channel][t][y][x], x fastest. Network (BF16 weights, FP32 biases, activations and
+// accumulation):
 //   h1 = tanh(conv(X, W1) + b1),  h2 = tanh(conv(h1, W2) + b2),  o = conv(h2, W3),
 //   E(s) = Σ_q o_q,
 // 3 x 3 x 3 kernels, periodic in x and y (lattice translations), zero padding in t (the
 // projection has open ends in imaginary time). Training minimises the batch variance of
 // E + log W (only ratios enter the moves). All reductions run in a fixed order, so E is a
 // deterministic function of the field, which the cumulative global move requires.
+// The optimiser keeps FP32 master parameters; the kernels read a BF16 copy of the weights
+// (round to nearest even), refreshed whenever the parameters change, so E is the network of
+// the rounded weights and the weight gradient is taken at them.
 #include "../../include/kapdqmc.h"
 #include "kdq_kernels.h"
 
@@ -50,9 +54,18 @@ __global__ void k_embed(const uint8_t *fields, float *X0, Geo g, int n, int B) {
 // ---- per-position evaluation, shared by the full forward pass and the local step so that
 // both produce bit-identical activations (fixed tap order, explicit fma, same tanhf)
 struct NetW {
-    const float *W1, *b1, *W2, *b2, *W3;
+    const uint16_t *W1, *W2, *W3;   // BF16 bit patterns
+    const float *b1, *b2;
     int C;
 };
+
+KDQ_DEV float bf16f(uint16_t u) { return __uint_as_float((uint32_t)u << 16); }
+
+// FP32 -> BF16, round to nearest even (finite inputs)
+KDQ_DEV uint16_t f2bf16(float x) {
+    const uint32_t u = __float_as_uint(x);
+    return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
 
 // value (1..4) of the field at position p, sublattice sub, of configuration F (ns x nt,
 // column-major: s[i + t ns])
@@ -68,8 +81,8 @@ KDQ_DEV float h1_eval(const Geo &g, const NetW &w, int co, int q, FV fv) {
         bool ok;
         const int p = tap_pos(g, q, k, ok);
         if (!ok) continue;
-        s += w.W1[((size_t)co * CIN + fv(0, p) - 1) * 27 + k];
-        s += w.W1[((size_t)co * CIN + 4 + fv(1, p) - 1) * 27 + k];
+        s += bf16f(w.W1[((size_t)co * CIN + fv(0, p) - 1) * 27 + k]);
+        s += bf16f(w.W1[((size_t)co * CIN + 4 + fv(1, p) - 1) * 27 + k]);
     }
     return tanhf(s);
 }
@@ -77,7 +90,7 @@ KDQ_DEV float h1_eval(const Geo &g, const NetW &w, int co, int q, FV fv) {
 // Σ_ci Σ_k W[co][ci][k] in(ci, q + off_k), started from s0: taps resolved once (tag(p)),
 // inputs read as iv(ci, tag); the fma sequence is the same whatever the accessors
 template <class TAG, class IV>
-KDQ_DEV float conv_eval(const Geo &g, const float *W, int cin, int co, int q, float s0, TAG tag, IV iv) {
+KDQ_DEV float conv_eval(const Geo &g, const uint16_t *W, int cin, int co, int q, float s0, TAG tag, IV iv) {
     int src[27];
     bool ok[27];
 #pragma unroll
@@ -87,10 +100,10 @@ KDQ_DEV float conv_eval(const Geo &g, const float *W, int cin, int co, int q, fl
     }
     float s = s0;
     for (int ci = 0; ci < cin; ++ci) {
-        const float *wk = W + ((size_t)co * cin + ci) * 27;
+        const uint16_t *wk = W + ((size_t)co * cin + ci) * 27;
 #pragma unroll
         for (int k = 0; k < 27; ++k)
-            if (ok[k]) s = fmaf(wk[k], iv(ci, src[k]), s);
+            if (ok[k]) s = fmaf(bf16f(wk[k]), iv(ci, src[k]), s);
     }
     return s;
 }
@@ -105,7 +118,7 @@ __global__ void k_layer1(uint8_t *const *fp, NetW w, float *h1, Geo g, int n, in
 }
 
 // out[b][co][q] = act(bias[co] + conv(in)[co][q])
-__global__ void k_conv(const float *in, const float *W, const float *bias, float *out, Geo g, int B, int cin,
+__global__ void k_conv(const float *in, const uint16_t *W, const float *bias, float *out, Geo g, int B, int cin,
                        int cout, int act) {
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (size_t)B * cout * g.P) return;
@@ -141,7 +154,7 @@ __global__ void k_final(const double *part, double *E, int nblk, int B) {
 
 // din[b][ci][q] = (Σ_co Σ_k w[co][ci][k] dout[b][co][q - off_k]) · (1 - h[b][ci][q]²) (h: the
 // tanh output that `in` was; null: no derivative)
-__global__ void k_conv_bwd_data(const float *dout, const float *w, const float *h, float *din, Geo g, int B,
+__global__ void k_conv_bwd_data(const float *dout, const uint16_t *w, const float *h, float *din, Geo g, int B,
                                 int cin, int cout) {
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (size_t)B * cin * g.P) return;
@@ -153,10 +166,10 @@ __global__ void k_conv_bwd_data(const float *dout, const float *w, const float *
     float s = 0.f;
     for (int co = 0; co < cout; ++co) {
         const float *src = dout + ((size_t)b * cout + co) * g.P;
-        const float *wk = w + ((size_t)co * cin + ci) * 27;
+        const uint16_t *wk = w + ((size_t)co * cin + ci) * 27;
 #pragma unroll
         for (int k = 0; k < 27; ++k)
-            if (ok[k]) s += wk[k] * src[pos[k]];
+            if (ok[k]) s += bf16f(wk[k]) * src[pos[k]];
     }
     if (h) {
         const float hv = h[e];
@@ -212,15 +225,23 @@ __global__ void k_fill_i(int *x, int n) {
     if (e < n) x[e] = 0;
 }
 
-__global__ void k_adam(float *p, const float *g, float *m, float *v, int n, float lr, float b1, float b2, float eps,
-                       float c1, float c2) {
+// Adam on the FP32 master parameters; the BF16 copy follows
+__global__ void k_adam(float *p, uint16_t *wb, const float *g, float *m, float *v, int n, float lr, float b1, float b2,
+                       float eps, float c1, float c2) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n) return;
     const float gv = g[e];
     const float mv = b1 * m[e] + (1.f - b1) * gv, vv = b2 * v[e] + (1.f - b2) * gv * gv;
     m[e] = mv;
     v[e] = vv;
-    p[e] -= lr * (mv / c1) / (sqrtf(vv / c2) + eps);
+    const float pv = p[e] - lr * (mv / c1) / (sqrtf(vv / c2) + eps);
+    p[e] = pv;
+    wb[e] = f2bf16(pv);
+}
+
+__global__ void k_to_bf16(const float *p, uint16_t *wb, int n) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) wb[e] = f2bf16(p[e]);
 }
 
 // ---- inner chain of a global move: local Metropolis steps on exp(-E)
@@ -374,6 +395,7 @@ struct kdq_net_s {
     int n = 0, C = 0, maxB = 0, nblk = 0;
     int np = 0;                  // parameter count
     float *d_p = nullptr, *d_g = nullptr, *d_m = nullptr, *d_v = nullptr;
+    uint16_t *d_wb = nullptr;        // [np] BF16 copy of d_p (the kernels read its weight entries)
     float *d_X0 = nullptr, *d_h1 = nullptr, *d_h2 = nullptr, *d_o = nullptr;
     float *d_d2 = nullptr, *d_d1 = nullptr, *d_do = nullptr;
     double *d_part = nullptr, *d_E = nullptr, *d_gb = nullptr;
@@ -428,7 +450,16 @@ int nalloc(kdq_net n, T **p, size_t count) {
 
 NetW net_weights(kdq_net n) {
     const float *p = n->d_p;
-    return NetW{p + n->oW1, p + n->ob1, p + n->oW2, p + n->ob2, p + n->oW3, n->C};
+    const uint16_t *wb = n->d_wb;
+    return NetW{wb + n->oW1, wb + n->oW2, wb + n->oW3, p + n->ob1, p + n->ob2, n->C};
+}
+
+// refresh the BF16 weight copy after the master parameters changed
+int net_sync_bf16(kdq_net n) {
+    hipLaunchKernelGGL(k_to_bf16, dim3(nb(n->np, 256)), dim3(256), 0, n->stream, n->d_p, n->d_wb, n->np);
+    NCHK(hipGetLastError());
+    NCHK(hipStreamSynchronize(n->stream));
+    return KDQ_OK;
 }
 
 // E[b] = Σ_q o[b][q] in a fixed order
@@ -517,7 +548,7 @@ int net_grad(kdq_net n, const uint8_t *fields, const double *logw, int B, double
     *loss = L / B;
     const Geo g = n->g;
     const int C = n->C;
-    float *p = n->d_p, *gr = n->d_g;
+    float *gr = n->d_g;
     NCHK(hipMemcpyAsync(n->d_gb, gb.data(), sizeof(double) * B, hipMemcpyHostToDevice, n->stream));
     const size_t no = (size_t)B * g.P;
     hipLaunchKernelGGL(k_fill, dim3(nb(no, 256)), dim3(256), 0, n->stream, n->d_do, no, n->d_gb, g.P);
@@ -525,12 +556,12 @@ int net_grad(kdq_net n, const uint8_t *fields, const double *logw, int B, double
     hipLaunchKernelGGL(k_conv_bwd_w, dim3(C * 27, 1), dim3(RT), 0, n->stream, n->d_do, n->d_h2, gr + n->oW3,
                        (float *)nullptr, g, B, C, 1);
     hipLaunchKernelGGL(k_conv_bwd_data, dim3(nb((size_t)B * C * g.P, 256)), dim3(256), 0, n->stream, n->d_do,
-                       p + n->oW3, n->d_h2, n->d_d2, g, B, C, 1);
+                       n->d_wb + n->oW3, n->d_h2, n->d_d2, g, B, C, 1);
     // layer 2
     hipLaunchKernelGGL(k_conv_bwd_w, dim3(C * 27 + 1, C), dim3(RT), 0, n->stream, n->d_d2, n->d_h1, gr + n->oW2,
                        gr + n->ob2, g, B, C, C);
     hipLaunchKernelGGL(k_conv_bwd_data, dim3(nb((size_t)B * C * g.P, 256)), dim3(256), 0, n->stream, n->d_d2,
-                       p + n->oW2, n->d_h1, n->d_d1, g, B, C, C);
+                       n->d_wb + n->oW2, n->d_h1, n->d_d1, g, B, C, C);
     // layer 1
     hipLaunchKernelGGL(k_conv_bwd_w, dim3(CIN * 27 + 1, C), dim3(RT), 0, n->stream, n->d_d1, n->d_X0, gr + n->oW1,
                        gr + n->ob1, g, B, CIN, C);
@@ -564,7 +595,7 @@ int kdq_net_create(int32_t L1, int32_t L2, int32_t nt, int32_t channels, uint64_
     const size_t P = n->g.P, B = max_batch;
     int r = 0;
     if (hipStreamCreateWithFlags(&n->stream, hipStreamNonBlocking) != hipSuccess) r = KDQ_EHIP;
-    r |= nalloc(n, &n->d_p, n->np); r |= nalloc(n, &n->d_g, n->np);
+    r |= nalloc(n, &n->d_p, n->np); r |= nalloc(n, &n->d_g, n->np); r |= nalloc(n, &n->d_wb, n->np);
     r |= nalloc(n, &n->d_m, n->np); r |= nalloc(n, &n->d_v, n->np);
     r |= nalloc(n, &n->d_X0, B * CIN * P); r |= nalloc(n, &n->d_h1, B * C * P); r |= nalloc(n, &n->d_h2, B * C * P);
     r |= nalloc(n, &n->d_o, B * P); r |= nalloc(n, &n->d_d2, B * C * P); r |= nalloc(n, &n->d_d1, B * C * P);
@@ -602,7 +633,8 @@ int kdq_net_create(int32_t L1, int32_t L2, int32_t nt, int32_t channels, uint64_
         hipMemcpy(n->d_fptr, fpt.data(), sizeof(uint8_t *) * B, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(n->d_chain, cid.data(), sizeof(uint32_t) * B, hipMemcpyHostToDevice) != hipSuccess ||
         hipFuncSetAttribute((const void *)k_slmc_step, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)step_lds_bytes(C)) != hipSuccess) {
+                            (int)step_lds_bytes(C)) != hipSuccess ||
+        net_sync_bf16(n) != KDQ_OK) {
         delete n;
         nfail(nullptr, "kdq_net_create: parameter upload failed");
         return KDQ_EHIP;
@@ -627,7 +659,7 @@ int kdq_net_get_params(kdq_net n, float *out) {
 int kdq_net_set_params(kdq_net n, const float *in) {
     if (!n || !in) return nfail(n, "kdq_net_set_params: bad argument");
     NCHK(hipMemcpy(n->d_p, in, sizeof(float) * n->np, hipMemcpyHostToDevice));
-    return KDQ_OK;
+    return net_sync_bf16(n);
 }
 
 int kdq_net_energy(kdq_net n, const uint8_t *fields, int32_t batch, double *E) {
@@ -671,8 +703,8 @@ int kdq_net_train_step(kdq_net n, const uint8_t *fields, const double *logw, int
     n->step += 1;
     const float b1 = 0.9f, b2 = 0.999f;
     const float c1 = 1.f - std::pow(b1, (float)n->step), c2 = 1.f - std::pow(b2, (float)n->step);
-    hipLaunchKernelGGL(k_adam, dim3(nb(n->np, 256)), dim3(256), 0, n->stream, n->d_p, n->d_g, n->d_m, n->d_v, n->np,
-                       lr, b1, b2, 1e-8f, c1, c2);
+    hipLaunchKernelGGL(k_adam, dim3(nb(n->np, 256)), dim3(256), 0, n->stream, n->d_p, n->d_wb, n->d_g, n->d_m, n->d_v,
+                       n->np, lr, b1, b2, 1e-8f, c1, c2);
     NCHK(hipGetLastError());
     NCHK(hipStreamSynchronize(n->stream));
     return KDQ_OK;

@@ -26,7 +26,9 @@ class EffectiveModel:
     """Translation-equivariant conv net E(s) on an L1 x L2 honeycomb lattice with nt slices.
 
     channels: hidden channels C; max_batch: the most configurations (or chains) one call takes.
-    Parameters are FP32, laid out as [W1 (C, 8, 27), b1 (C), W2 (C, C, 27), b2 (C), W3 (C, 27)].
+    Parameters are FP32 masters, laid out as [W1 (C, 8, 27), b1 (C), W2 (C, C, 27), b2 (C),
+    W3 (C, 27)]; the GPU kernels evaluate the network with W1, W2, W3 rounded to BF16 (nearest
+    even) and FP32 biases, activations and accumulation.
     """
 
     def __init__(self, L1: int, L2: int, nt: int, channels: int = 8, seed: int = 0, max_batch: int = 64):

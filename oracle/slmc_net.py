@@ -53,6 +53,19 @@ def unpack(p, C: int):
             g("W3").reshape(1, C, 27))
 
 
+def bf16_weights(p, C: int):
+    """The parameters the GPU kernels evaluate: W1, W2, W3 rounded to BF16 (round to nearest
+    even of their FP32 values), biases as FP32; returned as float64."""
+    q = np.asarray(p, dtype=np.float32).copy()
+    lay, _ = param_layout(C)
+    for name in ("W1", "W2", "W3"):
+        off, n = lay[name]
+        u = q[off:off + n].view(np.uint32).astype(np.uint64)
+        u = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+        q[off:off + n] = u.astype(np.uint32).view(np.float32)
+    return q.astype(np.float64)
+
+
 def embed(geo: Geometry, fields):
     """fields: [B, ns, nt] values 1..4 -> X [B, 8, P]."""
     fields = np.asarray(fields)

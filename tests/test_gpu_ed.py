@@ -1,7 +1,7 @@
 """The GPU t-U sweep's observables against the exact value of the distribution it samples
 (2x2 honeycomb, N = 8, U = 4, Δτ = 0.1, 2Θ = 8; `tests/fock_exact.py`), through the C ABI.
 
-512 chains batched in one handle, 600 sweeps each, give a chain-to-chain error of ≈ 0.006 on
+512 chains batched in one handle, 600 sweeps each, give a chain-to-chain error of ≈ 0.0043 on
 Eu (256 x 300 gave 0.011: Eu decorrelates slowly on this lattice), so a dropped HalfeK
 sandwich in the measurement (Eu − 0.037) or a wrong HS constant (≥ 0.19) fails; the bar is 3σ
 for Ek and Eu in FP64 and in mixed precision (FP32 G between stabilisations).

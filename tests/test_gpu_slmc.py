@@ -77,11 +77,13 @@ def test_net_energy_and_grad_vs_oracle(mods):
     net, p, geo, f, sn, rng = _net_setup(slmc, 3, 2, 6, 4, 5, 0.3)
     fields = [np.asfortranarray(x) for x in f]
     E = net.energy(fields)
-    Eo = sn.energy(geo, p.astype(np.float64), 4, f)
+    pb = sn.bf16_weights(p, 4)          # the kernels read BF16 weights
+    assert np.abs(pb - p).max() > 0    # (so the rounding is exercised)
+    Eo = sn.energy(geo, pb, 4, f)
     assert np.abs(E - Eo).max() < 1e-4 * np.abs(Eo).max()
     lw = rng.normal(0, 3, len(f))
     loss, g = net.loss_grad(fields, lw)
-    lo, go = sn.loss_grad(geo, p.astype(np.float64), 4, f, lw)
+    lo, go = sn.loss_grad(geo, pb, 4, f, lw)
     assert abs(loss - lo) < 1e-4 * max(lo, 1.0)
     assert np.linalg.norm(g - go) < 2e-3 * np.linalg.norm(go)
     net.close()

@@ -70,3 +70,24 @@ def test_energy_translation_invariant():
         j = 2 * (((x + 1) % X) + X * ((y + 2) % Y)) + sub
         g[:, j, :] = f[:, i, :]
     assert np.allclose(sn.energy(geo, p, C, g), E, rtol=1e-12)
+
+
+def test_bf16_rounding_matches_torch():
+    """bf16_weights rounds W1, W2, W3 as torch's bfloat16 conversion does (nearest even) and
+    leaves the biases alone."""
+    import torch
+    C = 3
+    _, n = sn.param_layout(C)
+    rng = np.random.default_rng(11)
+    p = rng.normal(0, 0.5, n).astype(np.float32)
+    p[:5] = [1 + 2 ** -8, 1 + 3 * 2 ** -8, -(1 + 2 ** -8), 2 ** -8 + 2 ** -16, 0.0]   # ties and small values
+    q = sn.bf16_weights(p, C)
+    ref = torch.from_numpy(p).to(torch.bfloat16).float().double().numpy()
+    lay, _ = sn.param_layout(C)
+    for name in ("W1", "W2", "W3"):
+        off, k = lay[name]
+        assert np.array_equal(q[off:off + k], ref[off:off + k])
+    for name in ("b1", "b2"):
+        off, k = lay[name]
+        assert np.array_equal(q[off:off + k], p[off:off + k].astype(np.float64))
+    assert q[0] == 1.0 and q[1] == 1 + 2 ** -6   # ties to even
