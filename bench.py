@@ -160,12 +160,118 @@ def run_batch(m, nch: int, seed0: int, delay, warmup: int, steps: int, profile_a
             "decisions": int(cnt[:, 0].sum()), "near_ties": int(b.near_ties().sum())}
 
 
+def bench_scee(args):
+    """Swap-replica Rényi-2 workload (ctrl_SCEEicr, src/tV/SCEE.jl, src/tUV/SCEE.jl): 9 x 9
+    honeycomb (N = 162, Np = 81), Δτ = 0.05, 2Θ = 20 (Nt = 400), stabilisation every 10
+    slices; t-V with V/t = 0.5, 1.0, 1.5 over the chains, or t-U-V with U/t = 4, V/t = 0.5;
+    two replicas per chain, subsystem A = the 72 sites of cells x = 1..4 plus the 9 A-sublattice
+    sites at x = 5 (81 sites), B empty, λ = 0.5, Nλ = 2; --chains chains in one handle."""
+    import numpy as np
+    import torch
+    from kapdqmc_amd import dqmc
+    from kapdqmc_amd.geometry import area_index
+    from kapdqmc_amd.models import tUV_Hubbard_Para, tV_Hubbard_Para
+    C = args.chains
+    if args.workload == "scee-tv":
+        ms = [tV_Hubbard_Para(Ht=1.0, Hv1=v, Hv2=v, Δt=0.05, Θrelax=10.0, Θquench=0.0, Lattice="HoneyComb120",
+                              site=[9, 9], BatchSize=10) for v in (0.5, 1.0, 1.5)]
+        ms = [ms[c % 3] for c in range(C)]
+        desc = "t-V honeycomb 9x9 N=162 V=0.5/1.0/1.5 dtau=0.05 2Theta=20 Nt=400 BS=10"
+    else:
+        m = tUV_Hubbard_Para(Ht=1.0, Hu=4.0, Hv=0.5, Lattice="HoneyComb120", site=[9, 9], Δt=0.05, Θ=10.0,
+                             BatchSize=10)
+        ms = [m] * C
+        desc = "t-U-V honeycomb 9x9 N=162 U=4 V=0.5 dtau=0.05 2Theta=20 Nt=400 BS=10"
+    m0 = ms[0]
+    iA = list(area_index(m0.Lattice, m0.site, ([1, 1], [4, 9]))) + [2 * (4 + 9 * y) + 1 for y in range(9)]
+    iA = np.asarray(iA, dtype=np.int64)
+    assert len(iA) == 81 and len(set(iA.tolist())) == 81
+    iB = np.zeros(0, dtype=np.int64)
+    b = dqmc.SCEEBatch(ms, iA, iB, 0.5, 2, [args.seed + c for c in range(C)], list(range(C)))
+    for c in range(C):
+        for r in range(2):
+            b.set_fields(c, r, dqmc.Initial_s(ms[c], args.seed + c + 1000 * r, c))
+    t0 = time.perf_counter()
+    b.sweeps(1)   # set-up + one sweep (untimed)
+    torch.cuda.synchronize()
+    print(f"bench: SCEE {C} chain(s), set-up + warm-up sweep in {time.perf_counter() - t0:.1f} s",
+          file=sys.stderr, flush=True)
+    O = []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        O.append(b.sweeps(1))
+        print(f"bench: SCEE timed sweep {k + 1} done", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    O = np.concatenate(O, axis=1)
+    b.profile(["wrap", "chain", "get_G", "decide", "flush", "qr", "inv", "gemm_other"])
+    b.sweeps(1)   # one more sweep with every kernel family timed by HIP events (host launches)
+    prof = {f: {"ms_per_sweep": v[0], "launches_per_sweep": v[1]} for f, v in b.profile_read().items() if v[1]}
+    print(json.dumps({
+        "metric": "swap-replica Renyi-2 chain-sweeps/s (both replicas)", "value": C * args.steps / dt,
+        "unit": "chain-sweeps/s", "n_gpus": 1, "steps": args.steps, "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True, "dtype": "f64", "data": "synthetic (Philox-seeded HS fields)",
+        "config": {"workload": f"{desc}, 2 replicas, |A| = 81, B empty, lambda = 0.5, N_lambda = 2, "
+                               f"{C} chains per handle", "chains": C},
+        "O_mean": float(np.nanmean(O)), "near_ties": int(b.near_ties().sum()),
+        "kernel_breakdown": prof}))
+
+
+def bench_slmc(args):
+    """Self-learning global moves on the t-U model: 15 x 15 honeycomb (N = 450, Np = 225),
+    U/t = 5, Δτ = 0.1, 2Θ = 24 (Nt = 240), --chains chains (16), effective model with 32
+    channels (BF16 weights, FP32 accumulation). Times the local FP64 sweep of every chain, the
+    exact log W of every chain, one Adam step on the chains' (field, log W) pairs, and a global
+    move of --slmc-inner inner steps with its exact FP64 acceptance."""
+    import numpy as np
+    import torch
+    from kapdqmc_amd import slmc
+    from kapdqmc_amd.dqmc import DQMCBatch
+    from kapdqmc_amd.models import tU_Hubbard_Para
+    C = args.chains
+    m = tU_Hubbard_Para(Ht=1.0, Hu1=5.0, Hu2=5.0, Δt=0.1, Θrelax=12.0, Θquench=0.0, Lattice="HoneyComb120",
+                        site=[15, 15], BatchSize=10)
+    b = DQMCBatch([m] * C, [args.seed + c for c in range(C)], list(range(C)), delay=32)
+    b.initial_s()
+    net = slmc.EffectiveModel.for_model(m, channels=32, seed=1, max_batch=C)
+
+    def timed(f, reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            out = f()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps, out
+
+    b.phy_update(1, False)   # warm-up (graph capture)
+    t_sweep, _ = timed(lambda: b.phy_update(1, False), args.steps)
+    print(f"bench: SLMC local sweep {t_sweep:.3f} s", file=sys.stderr, flush=True)
+    t_lw, lw = timed(b.log_weight, 2)
+    fields = [b.get_fields(c) for c in range(C)]
+    net.train_step(fields, lw, 1e-3)   # warm-up
+    t_train, _ = timed(lambda: net.train_step(fields, lw, 1e-3), 3)
+    t_E, _ = timed(lambda: net.energy(fields), 3)
+    b.slmc_move(net, args.slmc_inner)   # warm-up
+    t_move, acc = timed(lambda: b.slmc_move(net, args.slmc_inner), 2)
+    print(json.dumps({
+        "metric": "SLMC global-move costs (t-U 15x15 N=450)", "value": C / t_sweep, "unit": "chain-sweeps/s",
+        "n_gpus": 1, "steps": args.steps, "ms_per_step": t_sweep * 1e3, "higher_is_better": True,
+        "dtype": "f64 sweep / bf16-weight fp32 network", "data": "synthetic (Philox-seeded HS fields)",
+        "config": {"workload": f"t-U honeycomb 15x15 N=450 Np=225 U=5 dtau=0.1 2Theta=24 Nt=240 BS=10, {C} chains, "
+                               f"conv net 3 layers x 32 channels", "chains": C},
+        "ms_local_sweep_all_chains": t_sweep * 1e3, "ms_log_weight_all_chains": t_lw * 1e3,
+        "ms_net_forward_all_chains": t_E * 1e3, "ms_train_step_batch": t_train * 1e3,
+        "ms_global_move_all_chains": t_move * 1e3, "inner_steps_per_move": args.slmc_inner,
+        "global_move_acceptance_untrained": float(np.mean(acc))}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--chains", type=int, default=64, help="independent chains batched per launch")
+    ap.add_argument("--chains", type=int, default=None,
+                    help="independent chains batched per launch (default 64; scee 32, slmc 16)")
     ap.add_argument("--delay", type=int, default=0, help="block size k (0: autotune k and the GEMM tile)")
     ap.add_argument("--plain-steps", type=int, default=6, help="sweeps of the single-chain line (0: skip)")
     ap.add_argument("--plain-delay", type=int, default=16)
@@ -182,6 +288,10 @@ def main():
     ap.add_argument("--profile-all", action="store_true", help="time every kernel family (slower)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-sized buffer 5 times (scale for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--workload", choices=("tu", "scee-tv", "scee-tuv", "slmc"), default="tu",
+                    help="tu: the headline t-U sweep (default); scee-tv / scee-tuv: swap-replica Renyi-2 "
+                         "on 9x9 t-V / t-U-V; slmc: self-learning global moves on 15x15 t-U")
+    ap.add_argument("--slmc-inner", type=int, default=3, help="inner steps per global move (slmc)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only (independent chains shard by chain "
@@ -194,6 +304,12 @@ def main():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
     torch.cuda.set_device(0)
+    if args.chains is None:
+        args.chains = {"tu": 64, "slmc": 16}.get(args.workload, 32)
+    if args.workload in ("scee-tv", "scee-tuv"):
+        return bench_scee(args)
+    if args.workload == "slmc":
+        return bench_slmc(args)
     if args.calibrate_copy:
         n = args.calibrate_copy * (1 << 20) // 8
         x = torch.rand(n, dtype=torch.float64, device="cuda")

@@ -1590,9 +1590,11 @@ int kdq_set_option(kdq_handle h, int32_t key, int32_t value) {
             h->use_graph = value ? 1 : 0;
             break;
         case KDQ_OPT_DELAY: {
-            const int kmax = h->scee ? 16 : 64;
-            if (h->kind != KDQ_MODEL_TU || value < 1 || value > kmax)
-                return fail(h, KDQ_EARG, "kdq_set_option: delay is adjustable for t-U chains only, 1..64 (1..16 with replicas)");
+            const bool tv_scee = h->scee && h->kind == KDQ_MODEL_TV;
+            const int kmax = h->scee ? (tv_scee ? 8 : 16) : 64;
+            if ((h->kind != KDQ_MODEL_TU && !tv_scee) || value < 1 || value > kmax)
+                return fail(h, KDQ_EARG, "kdq_set_option: delay is adjustable for t-U chains (1..64, 1..16 with "
+                                         "replicas) and t-V replica chains (1..8 bonds)");
             h->delay = value;
             if (h->scee) scee_set_kb(h, value);
             break;
@@ -1926,10 +1928,11 @@ int scee_get_ABGM(kdq_handle h, int idx, bool forward) {
     return scee_grover(h);
 }
 
-// the four-matrix flush of one block (both replicas, every chain)
-int scee_flush(kdq_handle h) {
+// the four-matrix flush of one block (both replicas, every chain); kmax: the largest
+// accepted set of the block (0: the t-U block size)
+int scee_flush(kdq_handle h, int kmax = 0) {
     SceeState *S = h->scee;
-    const int kb = S->kb;
+    const int kb = kmax ? kmax : S->kb;
     Opnd *Gt = h->t_G[h->cur];
     FlushArgs f[4] = {
         {h->d_chains, Gt, h->t_G[1 - h->cur], kb, 0, nullptr, nullptr, h->t_FZ, h->t_FC},                    // Gt
@@ -1960,8 +1963,19 @@ int scee_layer(kdq_handle h, int lt, int dir, uint32_t sweep) {
     return KDQ_OK;
 }
 
-// UpdateSCEELayer! of the rank-2 models: t-V bond family `family`, or every element (-1);
-// no flush: the decision kernel updates the four matrices in place
+// bonds per delayed block of the t-V replica decisions (0: element by element): 8, or 4,
+// as the handle's delay allows and the block's LDS fits
+int scee_r2_block(kdq_handle h) {
+    const SceeState *S = h->scee;
+    if (h->kind != KDQ_MODEL_TV) return 0;
+    for (int kb : {8, 4})
+        if (S->kb >= kb && scee_r2_block_lds_bytes(kb, S->nx_max) <= 150 * 1024) return kb;
+    return 0;
+}
+
+// UpdateSCEELayer! of the rank-2 models: t-V bond family `family`, or every element (-1).
+// t-V: delayed blocks of bonds, each followed by the four-matrix flush; otherwise (t-U-V, or
+// delay below 4) the element kernel updates the four matrices in place
 int scee_layer_r2(kdq_handle h, int lt, int dir, uint32_t sweep, int family) {
     SceeState *S = h->scee;
     SceeR2Args a{};
@@ -1977,6 +1991,22 @@ int scee_layer_r2(kdq_handle h, int lt, int dir, uint32_t sweep, int family) {
     a.sweep = sweep;
     a.nx_max = S->nx_max;
     a.max_ns = h->max_ns;
+    const int kb = family >= 0 ? scee_r2_block(h) : 0;
+    if (kb) {
+        for (int s0 = 0; s0 < h->max_ns / 2; s0 += kb) {
+            a.s0 = s0;
+            a.Gt = h->t_G[h->cur];
+            a.Gt0 = xcur(h, SX_GT0);
+            a.G0t = xcur(h, SX_G0T);
+            a.G0 = xcur(h, SX_G0);
+            {
+                ProfScope ps(h, KDQ_PROF_DECIDE);
+                HIPCHK(launch_scee_r2_block(h->stream, a, kb));
+            }
+            if (int r = scee_flush(h, 2 * kb)) return r;
+        }
+        return KDQ_OK;
+    }
     ProfScope ps(h, KDQ_PROF_DECIDE);
     HIPCHK(launch_scee_r2(h->stream, a));
     return KDQ_OK;

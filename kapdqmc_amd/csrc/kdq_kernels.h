@@ -240,11 +240,16 @@ struct SceeR2Args {
     const SceeRegion *reg;
     int nch, lt, dir, family, kind;    // family: t-V bond family (elements family·nb ..), -1: every element; kind 1 t-V, 2 t-U-V
     uint32_t sweep;
-    int nx_max, max_ns, pad;
+    int nx_max, max_ns, s0;            // s0: first bond of the block within the family (block kernel)
     const uint32_t *sweep_dev;
 };
 hipError_t launch_scee_r2(hipStream_t st, const SceeR2Args &a);
 size_t scee_r2_lds_bytes(int nx_max, int max_ns);
+// t-V delayed blocks of kb (4 or 8) bonds of one family: decisions with LDS working sets, W and
+// the accepted sites for the four-matrix flush; returns hipErrorInvalidValue if the block's
+// LDS does not fit (then the element kernel is used)
+hipError_t launch_scee_r2_block(hipStream_t st, const SceeR2Args &a, int kb);
+size_t scee_r2_block_lds_bytes(int kb, int nx_max);
 // X[i][i] += v for the per-chain n x n matrices (dimension code ndim); set = 1 writes I.
 hipError_t launch_diag(hipStream_t st, int nchains, int max_n, const ChainDev *chains, const Opnd *X, int ndim,
                        double v, int set);

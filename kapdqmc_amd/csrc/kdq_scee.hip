@@ -481,6 +481,10 @@ namespace {
 // once: det g and GMupdate! (Gupdate.jl:45-51) of both regions, G4update! (Gupdate.jl:23-37)
 // on the replica's four matrices. The other replica's G0 on a region is read from its matrix.
 constexpr int RT = 512;
+// threads of the delayed-block rank-2 decision kernel: one wave per SIMD leaves 512 registers
+// per lane, which its 2x2 algebra needs without spilling (the element kernel keeps RT: its
+// full-matrix updates need the threads more than it loses to spills)
+constexpr int BT = 256;
 
 KDQ_DEV void c22_inv(const double ar[2][2], const double ai[2][2], double br[2][2], double bi[2][2], double &dr,
                      double &di) {
@@ -496,7 +500,8 @@ KDQ_DEV void c22_inv(const double ar[2][2], const double ai[2][2], double br[2][
     cmul(ar[0][0], ai[0][0], qr, qi, br[1][1], bi[1][1]);
 }
 
-// sum of 8 doubles over the block (RT threads)
+// sum of 8 doubles over the block (NT threads)
+template <int NT = RT>
 KDQ_DEV void block_sum8(double v[8], double *red) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
@@ -508,7 +513,7 @@ KDQ_DEV void block_sum8(double v[8], double *red) {
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         double s = 0.0;
-        for (int w = 0; w < RT / 64; ++w) s += red[w * 8 + k];
+        for (int w = 0; w < NT / 64; ++w) s += red[w * 8 + k];
         v[k] = s;
     }
 }
@@ -529,7 +534,7 @@ __global__ __launch_bounds__(RT) void k_scee_r2(SceeR2Args a) {
     double *c1r = c0i + 2 * NM, *c1i = c1r + 2 * NM;   // Gt[:, sub]  [u][i]
     double *z1r = c1i + 2 * NM, *z1i = z1r + 2 * NM;   // r Gt0[sub, :]          [t][j]
     double *z2r = z1i + 2 * NM, *z2i = z2r + 2 * NM;   // r (E_sub - Gt[sub, :]) [t][j]
-    double *red = z2i + 2 * NM;                        // RT/64 x 8
+    double *red = z2i + 2 * NM;                        // RT/64 x 16
     __shared__ double rmr[2][2], rmi[2][2], tinv[2][2][2][2], dreg[2], prob_s;   // tinv [X][re/im][t][u]
     __shared__ int acc_s, sx_s;
     const uint32_t tag = (uint32_t)lt | ((uint32_t)a.dir << 31);
@@ -855,6 +860,568 @@ __global__ __launch_bounds__(RT) void k_scee_r2(SceeR2Args a) {
 }
 
 
+// ---- delayed blocks of the t-V rank-2 decisions (UpdateSCEELayer!, src/tV/SCEE.jl:447-494)
+// A block holds KB consecutive bonds of one family. The bonds of a family are disjoint, so the
+// block's 2·KB sites S are distinct. Gt[S,S], Gt0[S,X] and G0t[X,S] of both replicas live in
+// LDS; they, the region blocks M[r][X] = G0_r[X,X] and gmInv are kept current through the
+// sequential rank-2 G4update! / GMupdate! (src/public/Gupdate.jl:23-51), and the decisions
+// are those of k_scee_r2, element by element, replica 1 then 2. The four full matrices are
+// left to the four-matrix flush: the sequential updates of a block equal one Woodbury update
+// with the accepted sites A (in acceptance order) and W = (I + Δ_A (I - Gt[A,A]))⁻¹ Δ_A,
+// Δ_A block-diagonal. W grows by bordering with 2x2 blocks, from the working matrix Gc before
+// each accepted bond s with w = r⁻¹Δ: x = Q_A Gc[A,s] with Q_a = (I + Δ_a)⁻¹ Δ_a,
+// y = Gc[s,A] Δ_A, W' = [[W + x w y, x w], [w y, w]].
+template <int KB>
+__global__ __launch_bounds__(BT) void k_scee_r2_block(SceeR2Args a) {
+    constexpr int KS = 2 * KB;
+    const int c = blockIdx.x, nch = a.nch, tid = threadIdx.x;
+    const SceeRegion &R = a.reg[c];
+    const ChainDev &ch0 = a.chains[c];
+    const int nA = R.nA, nB = R.nB, nX = nA + nB, NX = max(a.nx_max, 1);
+    const int lt = a.lt, mA = nA * nA, mAB = mA + nB * nB;
+    const int e0 = a.family * ch0.nb + a.s0;
+    const int kb = min(KB, ch0.nb - a.s0);
+    if (kb <= 0) {
+        if (tid < 2) *a.chains[c + tid * nch].acc_cnt = 0;
+        return;
+    }
+    extern __shared__ double lds[];
+    double *gtr = lds, *gti = gtr + 2 * KS * KS;                 // Gt[S,S]  [r][i][j]
+    double *t0r = gti + 2 * KS * KS, *t0i = t0r + 2 * KS * NX;  // Gt0[S,X] [r][i][x]
+    double *o0r = t0i + 2 * KS * NX, *o0i = o0r + 2 * KS * NX;  // G0t[X,S] [r][x][i]
+    double *zr = o0i + 2 * KS * NX, *zi = zr + 2 * NX;          // zN [t][q]
+    double *ar_ = zi + 2 * NX, *ai_ = ar_ + 2 * NX;             // a [u][q]
+    double *br_ = ai_ + 2 * NX, *bi_ = br_ + 2 * NX;            // b [t][q]
+    double *nr_ = bi_ + 2 * NX, *ni_ = nr_ + 2 * NX;            // gmInv a [u][q]
+    double *wr_ = ni_ + 2 * NX, *wi_ = wr_ + 2 * NX;            // Tau⁻¹ b [t][q]
+    double *cxr = wi_ + 2 * NX, *cxi = cxr + 2 * NX;            // G0t[X, s_t] [t][x]
+    double *rxr = cxi + 2 * NX, *rxi = rxr + 2 * NX;            // Gt0[s_t, X] [t][x]
+    double *csr = rxi + 2 * NX, *csi = csr + 2 * KS;            // Gt[S, s_t]  [t][i]
+    double *rsr = csi + 2 * KS, *rsi = rsr + 2 * KS;            // Gt[s_t, S]  [t][j]
+    double *Wr = rsi + 2 * KS, *Wi = Wr + 2 * KS * KS;          // W [r][t][u]
+    double *red = Wi + 2 * KS * KS;                             // BT/64 x 16
+    __shared__ double dlr[2][KB][2][2], dli[2][KB][2][2];       // Δ of each bond's proposal
+    __shared__ double qlr[2][KB][2][2], qli[2][KB][2][2];       // (I + Δ)⁻¹ Δ
+    __shared__ double pgam[2][KB], pu[2][KB];                   // γ'/γ, uniform draw
+    __shared__ int sv[2][KB], site[KS], accl[2][KS], accb[2][KB];
+    __shared__ double rmr[2][2], rmi[2][2], tinv[2][2][2][2], prob_s;
+    __shared__ int acc_s;
+#define GT(r, i, j) ((r) * KS * KS + (i) * KS + (j))
+#define T0(r, i, x) ((r) * KS * NX + (i) * NX + (x))
+#define O0(r, x, i) ((r) * NX * KS + (x) * KS + (i))
+    const uint32_t tag = (uint32_t)lt | ((uint32_t)a.dir << 31);
+    const uint32_t sweep = a.sweep_dev ? *a.sweep_dev : a.sweep;
+    if (tid < 2 * KB) {
+        const int r = tid / KB, q = tid % KB;
+        if (r == 0) {
+            site[2 * q] = q < kb ? ch0.ex[e0 + q] : 0;
+            site[2 * q + 1] = q < kb ? ch0.ey[e0 + q] : 0;
+        }
+        if (q < kb) {
+            const ChainDev &chr = a.chains[c + r * nch];
+            const int el = e0 + q;
+            int chv;
+            double u;
+            proposal_draw(chr.seed, chr.chain_id, chr.stream, sweep, tag, (uint32_t)el, chv, u);
+            const int si = chr.s[(size_t)el + (size_t)(lt - 1) * chr.n_elem];
+            const int sx = chv + 1 + ((chv + 1) >= si ? 1 : 0);
+            // Δ = uv diag(e^{ds} - 1, e^{-ds} - 1) uv (src/tV/GreenMatrix.jl:76-89), and
+            // (I + Δ)⁻¹ Δ = uv diag(1 - e^{-ds}, 1 - e^{ds}) uv since uv² = I
+            const double h = 0.7071067811865476, uv[2][2] = {{-h, -h}, {-h, h}};
+            const double ds = chr.alpha[lt - 1] * (chr.eta[sx - 1] - chr.eta[si - 1]);
+            const double t2[2] = {exp(ds) - 1.0, exp(-ds) - 1.0};
+            const double q2[2] = {1.0 - exp(-ds), 1.0 - exp(ds)};
+#pragma unroll
+            for (int p = 0; p < 2; ++p)
+#pragma unroll
+                for (int s = 0; s < 2; ++s) {
+                    dlr[r][q][p][s] = uv[p][0] * t2[0] * uv[0][s] + uv[p][1] * t2[1] * uv[1][s];
+                    dli[r][q][p][s] = 0.0;
+                    qlr[r][q][p][s] = uv[p][0] * q2[0] * uv[0][s] + uv[p][1] * q2[1] * uv[1][s];
+                    qli[r][q][p][s] = 0.0;
+                }
+            pgam[r][q] = chr.gamma[sx - 1] / chr.gamma[si - 1];
+            pu[r][q] = u;
+            sv[r][q] = si | (sx << 8);
+        }
+    }
+    __syncthreads();
+    // working blocks of both replicas and the region blocks of G0 (M[r][X] = G0_r[X,X])
+    for (int e = tid; e < 2 * KS * KS; e += BT) {
+        const int r = e / (KS * KS), i = (e / KS) % KS, j = e % KS;
+        double vr = 0.0, vi = 0.0;
+        if (i < 2 * kb && j < 2 * kb) {
+            const Opnd G = a.Gt[c + r * nch];
+            const size_t o = (size_t)site[i] + (size_t)site[j] * G.ld;
+            vr = G.re[o];
+            vi = G.im[o];
+        }
+        gtr[e] = vr;
+        gti[e] = vi;
+        Wr[e] = 0.0;
+        Wi[e] = 0.0;
+    }
+    for (int e = tid; e < 2 * KS * nX; e += BT) {
+        const int r = e / (KS * nX), rem = e % (KS * nX), x = rem / KS, i = rem % KS;
+        double tr = 0.0, ti = 0.0, orr = 0.0, oi = 0.0;
+        if (i < 2 * kb) {
+            const Opnd T = a.Gt0[c + r * nch], O = a.G0t[c + r * nch];
+            const int sx = R.idx[x];
+            const size_t ot = (size_t)site[i] + (size_t)sx * T.ld, oo = (size_t)sx + (size_t)site[i] * O.ld;
+            tr = T.re[ot]; ti = T.im[ot];
+            orr = O.re[oo]; oi = O.im[oo];
+        }
+        t0r[T0(r, i, x)] = tr; t0i[T0(r, i, x)] = ti;
+        o0r[O0(r, x, i)] = orr; o0i[O0(r, x, i)] = oi;
+    }
+    for (int e = tid; e < 2 * mAB; e += BT) {
+        const int r = e / mAB, f = e % mAB;
+        const int X = f < mA ? 0 : 1, nr = X ? nB : nA, base = X ? nA : 0, g = X ? f - mA : f;
+        const int p = g % nr, q = g / nr;
+        const Opnd G0 = a.G0[c + r * nch];
+        const size_t o = (size_t)R.idx[base + p] + (size_t)R.idx[base + q] * G0.ld;
+        double *M = R.M[r][X];
+        M[p + (size_t)q * nr] = G0.re[o];
+        M[(size_t)nr * nr + p + (size_t)q * nr] = G0.im[o];
+    }
+    __syncthreads();
+
+    int na[2] = {0, 0}, nties[2] = {0, 0};
+    for (int qb = 0; qb < kb; ++qb) {
+        const int s0l = 2 * qb;   // local indices of the bond's sites: s0l, s0l + 1
+        for (int r = 0; r < 2; ++r) {
+            const int o = 1 - r;
+            if (tid == 0) {
+                // rr = I + Δ (I - Gt[sub, sub]); w = rr⁻¹ Δ
+                double rr[2][2], ri[2][2];
+#pragma unroll
+                for (int p = 0; p < 2; ++p)
+#pragma unroll
+                    for (int q = 0; q < 2; ++q) {
+                        double sr = dlr[r][qb][p][q] + (p == q ? 1.0 : 0.0), sim = dli[r][qb][p][q];
+#pragma unroll
+                        for (int k = 0; k < 2; ++k) {
+                            double tr, ti;
+                            cmul(dlr[r][qb][p][k], dli[r][qb][p][k], gtr[GT(r, s0l + k, s0l + q)],
+                                 gti[GT(r, s0l + k, s0l + q)], tr, ti);
+                            sr -= tr;
+                            sim -= ti;
+                        }
+                        rr[p][q] = sr;
+                        ri[p][q] = sim;
+                    }
+                double ivr[2][2], ivi[2][2], dr, di;
+                c22_inv(rr, ri, ivr, ivi, dr, di);
+#pragma unroll
+                for (int p = 0; p < 2; ++p)
+#pragma unroll
+                    for (int q = 0; q < 2; ++q) {
+                        double sr = 0.0, sim = 0.0;
+#pragma unroll
+                        for (int k = 0; k < 2; ++k) {
+                            double tr, ti;
+                            cmul(ivr[p][k], ivi[p][k], dlr[r][qb][k][q], dli[r][qb][k][q], tr, ti);
+                            sr += tr;
+                            sim += ti;
+                        }
+                        rmr[p][q] = sr;
+                        rmi[p][q] = sim;
+                    }
+                const double pw = a.kind == 1 ? dr : dr * dr + di * di;
+                prob_s = pw * pgam[r][qb];
+                acc_s = 0;
+            }
+            __syncthreads();
+            // ---- Grover factors of both regions (as k_scee_r2, from the working blocks)
+            for (int it = tid; it < 2 * nX; it += BT) {
+                const int t = it / nX, q = it % nX;
+                const int X = q < nA ? 0 : 1, k0 = X ? nA : 0, k1 = X ? nX : nA, nr = X ? nB : nA;
+                const double *Mo = R.M[o][X], *Moi = Mo + (size_t)nr * nr;
+                const int ql = q - k0;
+                double sr = 0.0, sim = 0.0, pr = 0.0, pi = 0.0;
+                if (r == 0) {
+                    // zN[t][q] = Σ_k Gt0[s_t, X_k] (2 G0o[X_k, X_q] - δ); a[u = t][q] = Σ_k G0t[X_q, s_k] w[k][t]
+#pragma unroll 8
+                    for (int k = k0; k < k1; ++k) {
+                        const int kl = k - k0;
+                        const double mr = 2.0 * Mo[kl + (size_t)ql * nr] - (k == q ? 1.0 : 0.0);
+                        const double mi = 2.0 * Moi[kl + (size_t)ql * nr];
+                        double tr, ti;
+                        cmul(t0r[T0(r, s0l + t, k)], t0i[T0(r, s0l + t, k)], mr, mi, tr, ti);
+                        sr += tr;
+                        sim += ti;
+                    }
+#pragma unroll
+                    for (int k = 0; k < 2; ++k) {
+                        double tr, ti;
+                        cmul(o0r[O0(r, q, s0l + k)], o0i[O0(r, q, s0l + k)], rmr[k][t], rmi[k][t], tr, ti);
+                        pr += tr;
+                        pi += ti;
+                    }
+                } else {
+                    // a[u = t][q] = Σ_k (2 G0o[X_q, X_k] - δ) G0t[X_k, s_t]; zN[t][q] = Σ_k w[t][k] Gt0[s_k, X_q]
+#pragma unroll 8
+                    for (int k = k0; k < k1; ++k) {
+                        const int kl = k - k0;
+                        const double mr = 2.0 * Mo[ql + (size_t)kl * nr] - (k == q ? 1.0 : 0.0);
+                        const double mi = 2.0 * Moi[ql + (size_t)kl * nr];
+                        double tr, ti;
+                        cmul(mr, mi, o0r[O0(r, k, s0l + t)], o0i[O0(r, k, s0l + t)], tr, ti);
+                        pr += tr;
+                        pi += ti;
+                    }
+#pragma unroll
+                    for (int k = 0; k < 2; ++k) {
+                        double tr, ti;
+                        cmul(rmr[t][k], rmi[t][k], t0r[T0(r, s0l + k, q)], t0i[T0(r, s0l + k, q)], tr, ti);
+                        sr += tr;
+                        sim += ti;
+                    }
+                }
+                zr[t * NX + q] = sr;
+                zi[t * NX + q] = sim;
+                ar_[t * NX + q] = pr;
+                ai_[t * NX + q] = pi;
+            }
+            __syncthreads();
+            // b[t][q] = Σ_k zN[t][k] gmInv_X[k][q] (k over q's region)
+            for (int it = tid; it < 2 * nX; it += BT) {
+                const int t = it / nX, q = it % nX;
+                const int X = q < nA ? 0 : 1, off = X ? nA : 0, m = X ? nB : nA;
+                const double *gr = R.gmInv[X], *gi = gr + (size_t)m * m;
+                double sr = 0.0, sim = 0.0;
+#pragma unroll 8
+                for (int k = 0; k < m; ++k) {
+                    const size_t og = (size_t)k + (size_t)(q - off) * m;
+                    double tr, ti;
+                    cmul(zr[t * NX + off + k], zi[t * NX + off + k], gr[og], gi[og], tr, ti);
+                    sr += tr;
+                    sim += ti;
+                }
+                br_[t * NX + q] = sr;
+                bi_[t * NX + q] = sim;
+            }
+            __syncthreads();
+            // Tau_X[t][u] = δ + Σ_{q in X} b[t][q] a[u][q]
+            double v[8] = {0, 0, 0, 0, 0, 0, 0, 0}, w8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            for (int q = tid; q < nX; q += BT) {
+                const bool inA = q < nA;
+#pragma unroll
+                for (int t = 0; t < 2; ++t)
+#pragma unroll
+                    for (int uu = 0; uu < 2; ++uu) {
+                        double tr, ti;
+                        cmul(br_[t * NX + q], bi_[t * NX + q], ar_[uu * NX + q], ai_[uu * NX + q], tr, ti);
+                        const int k = 2 * (t * 2 + uu);
+                        v[k] += inA ? tr : 0.0;
+                        v[k + 1] += inA ? ti : 0.0;
+                        w8[k] += inA ? 0.0 : tr;
+                        w8[k + 1] += inA ? 0.0 : ti;
+                    }
+            }
+            block_sum8<BT>(v, red);
+            block_sum8<BT>(w8, red + 8 * (BT / 64));
+            if (tid == 0) {
+                const int m[2] = {nA, nB};
+                double dX[2];
+#pragma unroll
+                for (int X = 0; X < 2; ++X) {
+                    double tr[2][2], ti[2][2], ir[2][2], ii[2][2], dr = 1.0, di = 0.0;
+#pragma unroll
+                    for (int t = 0; t < 2; ++t)
+#pragma unroll
+                        for (int uu = 0; uu < 2; ++uu) {
+                            const int k = 2 * (t * 2 + uu);
+                            tr[t][uu] = (X ? w8[k] : v[k]) + (t == uu ? 1.0 : 0.0);
+                            ti[t][uu] = X ? w8[k + 1] : v[k + 1];
+                        }
+                    if (m[X]) c22_inv(tr, ti, ir, ii, dr, di);
+                    else {
+                        ir[0][0] = ir[1][1] = 1.0;
+                        ir[0][1] = ir[1][0] = 0.0;
+                        ii[0][0] = ii[0][1] = ii[1][0] = ii[1][1] = 0.0;
+                    }
+#pragma unroll
+                    for (int t = 0; t < 2; ++t)
+#pragma unroll
+                        for (int uu = 0; uu < 2; ++uu) {
+                            tinv[X][0][t][uu] = ir[t][uu];
+                            tinv[X][1][t][uu] = ii[t][uu];
+                        }
+                    dX[X] = m[X] ? (a.kind == 1 ? dr : dr * dr + di * di) : 1.0;
+                }
+                const double u = pu[r][qb];
+                const double p = prob_s * pow(dX[0], R.lambda) * pow(dX[1], 1.0 - R.lambda);
+                const int ok = u < p;
+                if (fabs(u - p) < KDQ_NEAR_TIE) ++nties[r];
+                acc_s = ok;
+                if (ok) {
+                    R.detg[0] *= dX[0];
+                    R.detg[1] *= dX[1];
+                }
+            }
+            __syncthreads();
+            if (!acc_s) continue;   // block-uniform
+            // ---- accepted: GMupdate! operands, and the working rows / columns at s (before the update)
+            for (int it = tid; it < 2 * nX; it += BT) {
+                const int uu = it / nX, q = it % nX;
+                const int X = q < nA ? 0 : 1, off = X ? nA : 0, m = X ? nB : nA;
+                const double *gr = R.gmInv[X], *gi = gr + (size_t)m * m;
+                double sr = 0.0, sim = 0.0;
+#pragma unroll 8
+                for (int k = 0; k < m; ++k) {
+                    const size_t og = (size_t)(q - off) + (size_t)k * m;
+                    double tr, ti;
+                    cmul(gr[og], gi[og], ar_[uu * NX + off + k], ai_[uu * NX + off + k], tr, ti);
+                    sr += tr;
+                    sim += ti;
+                }
+                nr_[uu * NX + q] = sr;
+                ni_[uu * NX + q] = sim;
+                double wr = 0.0, wi = 0.0;
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    double tr, ti;
+                    cmul(tinv[X][0][uu][k], tinv[X][1][uu][k], br_[k * NX + q], bi_[k * NX + q], tr, ti);
+                    wr += tr;
+                    wi += ti;
+                }
+                wr_[uu * NX + q] = wr;
+                wi_[uu * NX + q] = wi;
+                cxr[uu * NX + q] = o0r[O0(r, q, s0l + uu)];
+                cxi[uu * NX + q] = o0i[O0(r, q, s0l + uu)];
+                rxr[uu * NX + q] = t0r[T0(r, s0l + uu, q)];
+                rxi[uu * NX + q] = t0i[T0(r, s0l + uu, q)];
+            }
+            if (tid < 2 * KS) {
+                const int t = tid / KS, i = tid % KS;
+                csr[t * KS + i] = gtr[GT(r, i, s0l + t)];
+                csi[t * KS + i] = gti[GT(r, i, s0l + t)];
+                rsr[t * KS + i] = gtr[GT(r, s0l + t, i)];
+                rsi[t * KS + i] = gti[GT(r, s0l + t, i)];
+            }
+            __syncthreads();
+            // inverse Grover matrices: gmInv_X -= (gmInv a)(Tau⁻¹ b); the replica's region blocks
+            // of G0: M_r += G0t[X, s] w Gt0[s, X]
+            for (int it = tid; it < mAB; it += BT) {
+                const int X = it < mA ? 0 : 1, nr = X ? nB : nA, base = X ? nA : 0, g = X ? it - mA : it;
+                const int p = g % nr, q = g / nr;
+                double sr = 0.0, sim = 0.0, mr = 0.0, mi = 0.0;
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    double tr, ti;
+                    cmul(nr_[k * NX + base + p], ni_[k * NX + base + p], wr_[k * NX + base + q],
+                         wi_[k * NX + base + q], tr, ti);
+                    sr += tr;
+                    sim += ti;
+                    // (w Gt0[s, X_q])_k
+                    double zr_ = 0.0, zi_ = 0.0;
+#pragma unroll
+                    for (int k2 = 0; k2 < 2; ++k2) {
+                        double ur, ui;
+                        cmul(rmr[k][k2], rmi[k][k2], rxr[k2 * NX + base + q], rxi[k2 * NX + base + q], ur, ui);
+                        zr_ += ur;
+                        zi_ += ui;
+                    }
+                    cmul(cxr[k * NX + base + p], cxi[k * NX + base + p], zr_, zi_, tr, ti);
+                    mr += tr;
+                    mi += ti;
+                }
+                double *gv = R.gmInv[X];
+                gv[p + (size_t)q * nr] -= sr;
+                gv[(size_t)nr * nr + p + (size_t)q * nr] -= sim;
+                double *M = R.M[r][X];
+                M[p + (size_t)q * nr] += mr;
+                M[(size_t)nr * nr + p + (size_t)q * nr] += mi;
+            }
+            // working blocks: with f = w (E_s - Gt[s, S]) (2 x KS) and g = w Gt0[s, X] (2 x nX),
+            // Gt[S,S] -= Gt[S,s] f, G0t[X,S] -= G0t[X,s] f, Gt0[S,X] += Gt[S,s] g
+            for (int e = tid; e < KS * KS; e += BT) {
+                const int i = e / KS, j = e % KS;
+                double sr = 0.0, sim = 0.0;
+#pragma unroll
+                for (int t = 0; t < 2; ++t) {
+                    double fr = 0.0, fi = 0.0;
+#pragma unroll
+                    for (int k = 0; k < 2; ++k) {
+                        double tr, ti;
+                        cmul(rmr[t][k], rmi[t][k], (j == s0l + k ? 1.0 : 0.0) - rsr[k * KS + j], -rsi[k * KS + j],
+                             tr, ti);
+                        fr += tr;
+                        fi += ti;
+                    }
+                    double tr, ti;
+                    cmul(csr[t * KS + i], csi[t * KS + i], fr, fi, tr, ti);
+                    sr += tr;
+                    sim += ti;
+                }
+                gtr[GT(r, i, j)] -= sr;
+                gti[GT(r, i, j)] -= sim;
+            }
+            for (int e = tid; e < nX * KS; e += BT) {
+                const int x = e / KS, j = e % KS;
+                double sr = 0.0, sim = 0.0;
+#pragma unroll
+                for (int t = 0; t < 2; ++t) {
+                    double fr = 0.0, fi = 0.0;
+#pragma unroll
+                    for (int k = 0; k < 2; ++k) {
+                        double tr, ti;
+                        cmul(rmr[t][k], rmi[t][k], (j == s0l + k ? 1.0 : 0.0) - rsr[k * KS + j], -rsi[k * KS + j],
+                             tr, ti);
+                        fr += tr;
+                        fi += ti;
+                    }
+                    double tr, ti;
+                    cmul(cxr[t * NX + x], cxi[t * NX + x], fr, fi, tr, ti);
+                    sr += tr;
+                    sim += ti;
+                }
+                o0r[O0(r, x, j)] -= sr;
+                o0i[O0(r, x, j)] -= sim;
+            }
+            for (int e = tid; e < KS * nX; e += BT) {
+                const int i = e / nX, x = e % nX;
+                double sr = 0.0, sim = 0.0;
+#pragma unroll
+                for (int t = 0; t < 2; ++t) {
+                    double gr = 0.0, gi = 0.0;
+#pragma unroll
+                    for (int k = 0; k < 2; ++k) {
+                        double tr, ti;
+                        cmul(rmr[t][k], rmi[t][k], rxr[k * NX + x], rxi[k * NX + x], tr, ti);
+                        gr += tr;
+                        gi += ti;
+                    }
+                    double tr, ti;
+                    cmul(csr[t * KS + i], csi[t * KS + i], gr, gi, tr, ti);
+                    sr += tr;
+                    sim += ti;
+                }
+                t0r[T0(r, i, x)] += sr;
+                t0i[T0(r, i, x)] += sim;
+            }
+            // bordering of W with the 2x2 block of this bond (rows / columns nA_ and nA_ + 1)
+            const int n0 = na[r];
+            for (int e = tid; e < (n0 + 2) * (n0 + 2); e += BT) {
+                const int t = e % (n0 + 2), u = e / (n0 + 2);
+                // left factor L[t][k] (k = 0, 1): x w for old rows, w for new rows
+                // right factor Rr[k][u]: y for old columns, δ for new columns
+                double lr[2], li[2], rr_[2], ri_[2];
+                if (t < n0) {
+                    // x[t][k2] = Σ_{t'} Q[t][t'] Gc[A_t', s_k2] over t's 2x2 block
+                    const int pb = t >> 1, tl = t & 1, qq = accb[r][pb];
+                    double xr[2], xi[2];
+#pragma unroll
+                    for (int k2 = 0; k2 < 2; ++k2) {
+                        double sr = 0.0, sim = 0.0;
+#pragma unroll
+                        for (int tp = 0; tp < 2; ++tp) {
+                            double tr, ti;
+                            const int ia = accl[r][2 * pb + tp];
+                            cmul(qlr[r][qq][tl][tp], qli[r][qq][tl][tp], csr[k2 * KS + ia], csi[k2 * KS + ia], tr, ti);
+                            sr += tr;
+                            sim += ti;
+                        }
+                        xr[k2] = sr;
+                        xi[k2] = sim;
+                    }
+#pragma unroll
+                    for (int k = 0; k < 2; ++k) {
+                        double sr = 0.0, sim = 0.0;
+#pragma unroll
+                        for (int k2 = 0; k2 < 2; ++k2) {
+                            double tr, ti;
+                            cmul(xr[k2], xi[k2], rmr[k2][k], rmi[k2][k], tr, ti);
+                            sr += tr;
+                            sim += ti;
+                        }
+                        lr[k] = sr;
+                        li[k] = sim;
+                    }
+                } else {
+#pragma unroll
+                    for (int k = 0; k < 2; ++k) {
+                        lr[k] = rmr[t - n0][k];
+                        li[k] = rmi[t - n0][k];
+                    }
+                }
+                if (u < n0) {
+                    // y[k][u] = Σ_{u'} Gc[s_k, A_u'] Δ[u'][u] over u's 2x2 block
+                    const int pb = u >> 1, ul = u & 1, qq = accb[r][pb];
+#pragma unroll
+                    for (int k = 0; k < 2; ++k) {
+                        double sr = 0.0, sim = 0.0;
+#pragma unroll
+                        for (int up = 0; up < 2; ++up) {
+                            double tr, ti;
+                            const int ia = accl[r][2 * pb + up];
+                            cmul(rsr[k * KS + ia], rsi[k * KS + ia], dlr[r][qq][up][ul], dli[r][qq][up][ul], tr, ti);
+                            sr += tr;
+                            sim += ti;
+                        }
+                        rr_[k] = sr;
+                        ri_[k] = sim;
+                    }
+                } else {
+#pragma unroll
+                    for (int k = 0; k < 2; ++k) {
+                        rr_[k] = (k == u - n0) ? 1.0 : 0.0;
+                        ri_[k] = 0.0;
+                    }
+                }
+                double vr = 0.0, vi = 0.0;
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    double tr, ti;
+                    cmul(lr[k], li[k], rr_[k], ri_[k], tr, ti);
+                    vr += tr;
+                    vi += ti;
+                }
+                const int w = r * KS * KS + t * KS + u;
+                if (t < n0 && u < n0) {
+                    Wr[w] += vr;
+                    Wi[w] += vi;
+                } else {
+                    Wr[w] = vr;
+                    Wi[w] = vi;
+                }
+            }
+            if (tid == 0) {
+                accl[r][n0] = s0l;
+                accl[r][n0 + 1] = s0l + 1;
+                accb[r][n0 >> 1] = qb;
+                sv[r][qb] |= 1 << 16;
+            }
+            __syncthreads();
+            na[r] += 2;
+        }
+    }
+    for (int r = 0; r < 2; ++r) {
+        const ChainDev &chr = a.chains[c + r * nch];
+        if (tid == 0) {
+            *chr.acc_cnt = na[r];
+            chr.counters[0] += kb;
+            chr.counters[1] += na[r] / 2;
+            chr.counters[2] += nties[r];
+        }
+        if (tid < na[r]) chr.acc_idx[tid] = site[accl[r][tid]];
+        for (int e = tid; e < na[r] * na[r]; e += BT) {
+            const int t = e % na[r], u = e / na[r];
+            chr.W[t + u * 64] = Wr[r * KS * KS + t * KS + u];
+            chr.W[64 * 64 + t + u * 64] = Wi[r * KS * KS + t * KS + u];
+        }
+        if (tid < kb) {
+            const int v = sv[r][tid];
+            chr.s[(size_t)(e0 + tid) + (size_t)(lt - 1) * chr.n_elem] =
+                (uint8_t)((v >> 16) ? (v >> 8) & 0xff : v & 0xff);
+        }
+    }
+#undef GT
+#undef T0
+#undef O0
+}
+
 __global__ void k_scee_detg_signed(const SceeRegion *reg, int nch, const double *la, const double *lb,
                                    const double *pa, const double *pb) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -887,6 +1454,23 @@ hipError_t launch_scee_detg_signed(hipStream_t st, int nch, const SceeRegion *re
                                    const double *logdet_b, const double *phase_a, const double *phase_b) {
     hipLaunchKernelGGL(k_scee_detg_signed, dim3((nch + 63) / 64), dim3(64), 0, st, reg, nch, logdet_a, logdet_b,
                        phase_a, phase_b);
+    return hipGetLastError();
+}
+
+size_t scee_r2_block_lds_bytes(int kb, int nx_max) {
+    const size_t ks = 2 * (size_t)kb, nx = nx_max > 1 ? nx_max : 1;
+    return (8 * ks * ks + 8 * ks * nx + 28 * nx + 8 * ks + 16 * (BT / 64)) * sizeof(double);
+}
+
+hipError_t launch_scee_r2_block(hipStream_t st, const SceeR2Args &a, int kb) {
+    if (kb != 4 && kb != 8) return hipErrorInvalidValue;
+    const size_t bytes = scee_r2_block_lds_bytes(kb, a.nx_max);
+    if (bytes > 150 * 1024) return hipErrorInvalidValue;
+    const void *f = kb == 8 ? reinterpret_cast<const void *>(k_scee_r2_block<8>)
+                            : reinterpret_cast<const void *>(k_scee_r2_block<4>);
+    if (hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes)) return e;
+    if (kb == 8) hipLaunchKernelGGL(k_scee_r2_block<8>, dim3(a.nch), dim3(BT), bytes, st, a);
+    else hipLaunchKernelGGL(k_scee_r2_block<4>, dim3(a.nch), dim3(BT), bytes, st, a);
     return hipGetLastError();
 }
 

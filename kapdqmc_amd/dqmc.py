@@ -294,8 +294,9 @@ class SCEEBatch:
 
     indexA / indexB: the reference's 1-based site lists; lam, nlam: λ and Nλ. Each chain
     holds two replica fields (ss[1], ss[2]) in the model's field layout; replica r draws from
-    Philox stream 1 + r. delay: t-U sites per decision block (1..16); the rank-2 models decide
-    element by element.
+    Philox stream 1 + r. delay: t-U sites per decision block (1..16); t-V bonds per delayed
+    block (8, or 4; below 4 element by element with in-place updates); t-U-V decides element
+    by element.
     """
 
     def __init__(self, models, indexA, indexB, lam: float, nlam: int, seeds, chains=None,
@@ -366,6 +367,9 @@ class SCEEBatch:
         bufs = [np.zeros(2 * ns * ns) for _ in range(4)]   # interleaved, column-major
         self._chk(self.L.kdq_scee_get_G4(self.h, c, replica, *[_dp(x) for x in bufs]))
         return [(x[0::2] + 1j * x[1::2]).reshape((ns, ns), order="F") for x in bufs]   # Gt, G0, Gt0, G0t
+
+    profile = DQMCBatch.profile
+    profile_read = DQMCBatch.profile_read
 
     def near_ties(self) -> np.ndarray:
         """[chains, replicas]: decisions so far with |uniform draw − acceptance probability|

@@ -109,3 +109,33 @@ def test_scee_r2_uneven_batch_and_empty_b(dq, oracle_mod):
         assert np.allclose(Og[c], Oo, rtol=1e-8, atol=0)
         assert b.detg()[c][1] == 1.0
     b.close()
+
+
+@pytest.mark.parametrize("delay", [1, 4, 8])
+def test_tv_scee_delayed_blocks(dq, oracle_mod, delay):
+    """t-V decisions in delayed blocks of `delay` bonds (the four-matrix flush after each
+    block; 1: element by element with in-place updates) against the oracle's sequential
+    G4update! on a 4x4 lattice (16 bonds per family: whole and partial blocks). O is the
+    running product of Grover-determinant ratios (signed for t-V); at V = 1.5 its rounding is
+    amplified where det g is small (the element path differs from the oracle by 4e-7 after
+    two sweeps with G4 equal to 1e-9), so O is held to 1e-6 here."""
+    print(f"case: SCEE t-V L=4 V=1.5 delay={delay}", flush=True)
+    m = tv(4, V=1.5)
+    iA, iB = regions(m)
+    seed = 31
+    s1, s2 = dq.Initial_s(m, seed), dq.Initial_s(m, seed + 100)
+    o = oracle_mod.OracleSCEE2(m, iA.astype(np.int32), iB.astype(np.int32), 0.5, 2, s1, s2, seed=seed)
+    b = dq.SCEEBatch([m], iA, iB, 0.5, 2, [seed], [0], delay=delay)
+    b.set_fields(0, 0, s1)
+    b.set_fields(0, 1, s2)
+    Oo = o.sweeps(2)
+    Og = b.sweeps(2)[0]
+    for r in (0, 1):
+        assert np.array_equal(b.get_fields(0, r), o.fields(r)), r
+        for nm, x, y in zip(("Gt", "G0", "Gt0", "G0t"), b.G4(0, r), o.G4(r)):
+            assert rel(x, y) < 1e-9, (r, nm, rel(x, y))
+    print("max rel |dO|", np.abs(Og / Oo - 1).max(), flush=True)
+    assert np.allclose(Og, Oo, rtol=1e-6, atol=0), (Og, Oo)
+    assert not np.array_equal(b.get_fields(0, 0), s1)
+    assert b.near_ties().sum() == 0
+    b.close()
