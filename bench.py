@@ -265,6 +265,77 @@ def bench_slmc(args):
         "global_move_acceptance_untrained": float(np.mean(acc))}))
 
 
+def bench_uneven(args):
+    """Uneven batch: 64 t-U chains, chain c on an L x L honeycomb with L = (6, 9, 12)[c % 3]
+    (N = 72 / 162 / 288), U/t = (2, 4, 6)[(c // 3) % 3], seed c, Δτ = 0.1, 2Θ = 2L (Nt = 120 /
+    180 / 240), stabilisation every 10 slices. Chains of one Nt share a handle (one stream);
+    the three handles run concurrently from three host threads. Mixed precision is timed and
+    compared with FP64 on the same seeds: max |G_FP32 - G_FP64| per chain at the
+    stabilisations, and the double occupancy per group in units of the chain spread."""
+    import threading
+    import numpy as np
+    import torch
+    from kapdqmc_amd.dqmc import DQMCBatch
+    from kapdqmc_amd.models import tU_Hubbard_Para
+    C = args.chains
+    groups = {}
+    for c in range(C):
+        L, U = (6, 9, 12)[c % 3], (2.0, 4.0, 6.0)[(c // 3) % 3]
+        m = tU_Hubbard_Para(Ht=1.0, Hu1=U, Hu2=U, Δt=0.1, Θrelax=float(L), Θquench=0.0, Lattice="HoneyComb120",
+                            site=[L, L], BatchSize=10)
+        groups.setdefault(L, []).append((c, m))
+
+    def run(precision):
+        hs = {L: DQMCBatch([m for _, m in g], [c for c, _ in g], [c for c, _ in g], delay=32, precision=precision)
+              for L, g in groups.items()}
+        for b in hs.values():
+            b.initial_s()
+            b.phy_update(args.warmup, True)
+            b.phy_update(0, True)   # graph capture
+            b.precision_error()     # reset the error record
+        torch.cuda.synchronize()
+        rows, errs = {}, {}
+
+        def go(L):
+            rows[L] = hs[L].phy_update(args.steps, True)
+            errs[L] = hs[L].precision_error()
+
+        t0 = time.perf_counter()
+        th = [threading.Thread(target=go, args=(L,)) for L in hs]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        for b in hs.values():
+            b.close()
+        return dt, rows, errs
+
+    dt64, rows64, _ = run("fp64")
+    print(f"bench: uneven FP64 {args.steps} sweeps in {dt64:.2f} s", file=sys.stderr, flush=True)
+    dtm, rowsm, errs = run("mixed")
+    print(f"bench: uneven mixed {args.steps} sweeps in {dtm:.2f} s", file=sys.stderr, flush=True)
+    per_group = {}
+    for L, g in groups.items():
+        N = 2 * L * L
+        d64, dm = rows64[L][:, :, 1].mean(axis=1) / N, rowsm[L][:, :, 1].mean(axis=1) / N
+        sig = np.sqrt(d64.var(ddof=1) / len(g) + dm.var(ddof=1) / len(g))
+        per_group[f"L={L}"] = {"chains": len(g), "N": N, "Nt": 20 * L,
+                               "max_abs_G_fp32_minus_fp64_per_chain": [float(x) for x in errs[L]],
+                               "double_occupancy_fp64": float(d64.mean()), "double_occupancy_mixed": float(dm.mean()),
+                               "difference_in_sigma": float((d64.mean() - dm.mean()) / sig)}
+    print(json.dumps({
+        "metric": "uneven batch chain-sweeps/s (t-U, L = 6/9/12, 2Theta = 2L)", "value": C * args.steps / dtm,
+        "unit": "chain-sweeps/s", "n_gpus": 1, "steps": args.steps, "ms_per_step": dtm / args.steps * 1e3,
+        "higher_is_better": True, "dtype": "mixed (f32 G between stabilisations, f64 decisions / QR / LU)",
+        "data": "synthetic (Philox-seeded HS fields)",
+        "config": {"workload": f"{C} t-U chains, L = (6, 9, 12)[c % 3], U = (2, 4, 6)[(c // 3) % 3], seed c, "
+                               "dtau = 0.1, 2Theta = 2L, BS = 10; one handle per Nt, three concurrent streams",
+                   "chains": C},
+        "fp64_chain_sweeps_per_s": C * args.steps / dt64, "groups": per_group}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -288,9 +359,10 @@ def main():
     ap.add_argument("--profile-all", action="store_true", help="time every kernel family (slower)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-sized buffer 5 times (scale for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
-    ap.add_argument("--workload", choices=("tu", "scee-tv", "scee-tuv", "slmc"), default="tu",
-                    help="tu: the headline t-U sweep (default); scee-tv / scee-tuv: swap-replica Renyi-2 "
-                         "on 9x9 t-V / t-U-V; slmc: self-learning global moves on 15x15 t-U")
+    ap.add_argument("--workload", choices=("tu", "uneven", "scee-tv", "scee-tuv", "slmc"), default="tu",
+                    help="tu: the headline t-U sweep (default); uneven: 64 t-U chains of L = 6/9/12 with "
+                         "2Theta = 2L, mixed vs FP64; scee-tv / scee-tuv: swap-replica Renyi-2 on 9x9 "
+                         "t-V / t-U-V; slmc: self-learning global moves on 15x15 t-U")
     ap.add_argument("--slmc-inner", type=int, default=3, help="inner steps per global move (slmc)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -305,11 +377,13 @@ def main():
         sys.exit(1)
     torch.cuda.set_device(0)
     if args.chains is None:
-        args.chains = {"tu": 64, "slmc": 16}.get(args.workload, 32)
+        args.chains = {"tu": 64, "uneven": 64, "slmc": 16}.get(args.workload, 32)
     if args.workload in ("scee-tv", "scee-tuv"):
         return bench_scee(args)
     if args.workload == "slmc":
         return bench_slmc(args)
+    if args.workload == "uneven":
+        return bench_uneven(args)
     if args.calibrate_copy:
         n = args.calibrate_copy * (1 << 20) // 8
         x = torch.rand(n, dtype=torch.float64, device="cuda")
