@@ -20,6 +20,10 @@ the kinetic energy Ek = ⟨Ht Σ_σ Σ_ij K_ij c†_iσ c_jσ⟩ (`:184`) and Eu
 The many-body state is a matrix ψ[a, b] over (spin-up configuration a) x (spin-down
 configuration b), each a set of Np occupied sites in ascending (Jordan-Wigner) order; a
 one-species operator X acts as X ψ on the up side and ψ Xᵀ on the down side.
+
+The Rényi-2 entropy S2(A) = −log tr ρ_A² of the exact ground state checks the swap-replica
+estimator (`ctrl_SCEEicr`, `src/tU/SCEE.jl:1-262`); the particle-hole transformation is a
+product of single-site unitaries and leaves it unchanged.
 """
 import itertools
 
@@ -125,3 +129,80 @@ def expectation(m):
     dop = (occ[:, None, :] * (1.0 - occ[None, :, :])).sum(2)       # n↑ (1 − ñ↓) summed over sites
     Eu = (Lv * R * dop).sum() / norm
     return float(Ek), float(Eu)
+
+
+def ground_state_vector(K: np.ndarray, Ht: float, U: float):
+    """(configurations, ψ[up configuration, down configuration]) of the half-filled ground
+    state of Ht K + U/2 Σ_i (n_i − 1)² (exact diagonalisation)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as sla
+    N = K.shape[0]
+    Np = N // 2
+    confs, occ = _configurations(N, Np)
+    T = sp.csr_matrix(hopping_fock(K, Ht, Np))
+    one = sp.identity(T.shape[0], format="csr")
+    nup = np.repeat(occ, len(occ), axis=0)
+    ndn = np.tile(occ, (len(occ), 1))
+    H = sp.kron(T, one) + sp.kron(one, T) + sp.diags(0.5 * U * ((nup + ndn - 1.0) ** 2).sum(1))
+    _, v = sla.eigsh(H, k=1, which="SA")
+    return confs, v[:, 0].reshape(len(confs), len(confs))
+
+
+def _bipartite(confs, psi: np.ndarray, A: set) -> dict:
+    """ψ as a map ((A↑, A↓) part, (B↑, B↓) part) -> amplitude. Modes are ordered (all ↑
+    sites, all ↓ sites); bringing A's modes to the front, (A↑, A↓, B↑, B↓), costs the parity
+    of the occupied (B before A) pairs within each spin block and of moving the occupied A↓
+    modes past the occupied B↑ modes."""
+    M = {}
+    for a, cu in enumerate(confs):
+        for b, cd in enumerate(confs):
+            v = psi[a, b]
+            if v == 0.0:
+                continue
+            parts = []
+            for c in (cu, cd):
+                inA = tuple(x for x in c if x in A)
+                inB = tuple(x for x in c if x not in A)
+                parts.append((inA, inB, sum(1 for i in inB for j in inA if i < j)))
+            (aU, bU, iu), (aD, bD, idn) = parts
+            key = ((aU, aD), (bU, bD))
+            M[key] = M.get(key, 0.0) + (-1) ** (iu + idn + len(bU) * len(aD)) * v
+    return M
+
+
+def renyi2(confs, psi: np.ndarray, sites, left: np.ndarray = None) -> float:
+    """S2(A) = −log tr ρ_A² for the set A of (0-based) sites, both spins: ρ = |ψ⟩⟨ψ| (ψ
+    normalised), or ρ = |ψ⟩⟨left| / ⟨left|ψ⟩ when a left vector is given."""
+    A = set(int(x) for x in sites)
+    MR = _bipartite(confs, psi, A)
+    ML = MR if left is None else _bipartite(confs, left, A)
+    keys = set(MR) | set(ML)
+    rows = {k: i for i, k in enumerate(sorted({k[0] for k in keys}))}
+    cols = {k: i for i, k in enumerate(sorted({k[1] for k in keys}))}
+    XR, XL = np.zeros((len(rows), len(cols))), np.zeros((len(rows), len(cols)))
+    for X, M in ((XR, MR), (XL, ML)):
+        for (r, c), v in M.items():
+            X[rows[r], cols[c]] = v
+    norm = (psi * psi).sum() if left is None else (psi * left).sum()
+    rho = XR @ XL.T / norm
+    return float(-np.log(np.trace(rho @ rho)))
+
+
+def renyi2_sampled(m, sites) -> float:
+    """Exact S2(A) of what the swap-replica estimator averages at finite Δτ: ρ = |R⟩⟨L| /
+    ⟨L|R⟩ with R = T^{Nt/2} |ψ_T⟩ and ⟨L| = ⟨ψ_T| T^{Nt/2}, T = V_HS e^{-Δτ H_kin}, at the
+    node Nt/2 where G(Θ) enters the Grover matrices (no HalfeK sandwich)."""
+    N, Np = m.Ns, m.Ns // 2
+    if not np.allclose(m.α, m.α[0]):
+        raise ValueError("constant coupling profiles only")
+    confs, occ = _configurations(N, Np)
+    E = la.expm(-m.Δt * hopping_fock(m.K, m.Ht, Np))
+    V = transfer_diagonal(occ, m.α[0], m.η, m.γ)
+    amp = slater(m.Pt)
+    R = np.outer(amp, amp)
+    L = R.copy()
+    for _ in range(m.Nt // 2):
+        R = V * (E @ R @ E.T)
+    for _ in range(m.Nt - m.Nt // 2):
+        L = E.T @ (V * L) @ E
+    return renyi2(confs, R, sites, left=L)
