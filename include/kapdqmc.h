@@ -175,7 +175,8 @@ int kdq_get_near_ties(kdq_handle h, double *out);
  *                     (kdq_profile mask non-zero) sweeps run with host launches.
  *                     Sweeps of more than 20000 launches (small blocks on large lattices)
  *                     also run with host launches.
- *  KDQ_OPT_DELAY      decision block size k of t-U chains (1..64; 1..16 with replicas).
+ *  KDQ_OPT_DELAY      decision block size k of t-U chains (1..64; 1..16 with replicas); t-V
+ *                     replica chains: bonds per delayed block (8 or 4, 1..3: element by element).
  *  KDQ_OPT_GEMM_TILE  0 (default: by occupancy), 32 or 64: GEMM output tile edge.
  *  KDQ_OPT_DEFER      t-U chains. 0 (default): every block of k sites rewrites G (rank-|A|
  *                     flush). 1: a slice's local updates keep G0 and low-rank factors U, V
