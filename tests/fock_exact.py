@@ -104,8 +104,44 @@ def field_weight(m, s: np.ndarray) -> float:
     return float(np.real((np.outer(amp, amp) * psi).sum()))
 
 
-def expectation(m):
-    """Exact (Ek, Eu) of the distribution the t-U sweep samples, measured at slice Nt/2."""
+def staggered_correlations(m, occ: np.ndarray):
+    """Diagonal operators (over up configuration a, down configuration b of the
+    particle-hole picture) of the four structure factors of `phy_measure`
+    (`src/tU/phy_update.jl:194-235`) as they accumulate there: with the staggered density
+    m_σ(c) = n_σ(c, A) − n_σ(c, B) of cell c (site 2c of the cell is A, 2c+1 is B,
+    c = x + L1 y), the per-field Wick estimators sum to
+
+        tmp1(r) = Σ_c m↑(c) m↑(c + r),   tmp2(r) = Σ_c m↑(c) m↓(c + r)
+
+    (the `+adjoint(` continuation lines that would add the ↓↓ and ↓↑ halves are separate,
+    discarded statements); CDW0 = Σ_r (tmp1 + tmp2) / N², CDW1 = Σ_r cos(q·r)(tmp1 + tmp2) / N²
+    with q·r = 2π rx / L1 + 2π ry / L2, SDW0 and SDW1 the same with tmp1 − tmp2.
+    Returns an array [4, n_up, n_down]."""
+    L1, L2 = m.site
+    N = m.Ns
+    cells = [(x, y) for y in range(L2) for x in range(L1)]
+    mu = np.stack([occ[:, 2 * k] - occ[:, 2 * k + 1] for k in range(len(cells))], axis=1)
+    md = -mu                                              # n↓ = 1 − ñ↓
+    out = np.zeros((4,) + (len(occ), len(occ)))
+    for rx in range(1, L1 + 1):
+        for ry in range(1, L2 + 1):
+            t1 = np.zeros(len(occ))
+            t2 = np.zeros((len(occ), len(occ)))
+            for k, (x, y) in enumerate(cells):
+                kk = ((x + rx) % L1) + L1 * ((y + ry) % L2)
+                t1 += mu[:, k] * mu[:, kk]
+                t2 += np.outer(mu[:, k], md[:, kk])
+            cq = np.cos(2 * np.pi / L1 * rx + 2 * np.pi / L2 * ry)
+            out[0] += t1[:, None] + t2
+            out[1] += cq * (t1[:, None] + t2)
+            out[2] += t1[:, None] - t2
+            out[3] += cq * (t1[:, None] - t2)
+    return out / float(N) ** 2
+
+
+def expectation(m, structure_factors: bool = False):
+    """Exact (Ek, Eu) of the distribution the t-U sweep samples, measured at slice Nt/2;
+    with structure_factors, (Ek, Eu, CDW0, CDW1, SDW0, SDW1)."""
     N, Np = m.Ns, m.Ns // 2
     if not np.allclose(m.α, m.α[0]):
         raise ValueError("constant coupling profiles only")
@@ -128,7 +164,10 @@ def expectation(m):
     Ek = (Lv * (T @ R + R @ T.T)).sum() / norm
     dop = (occ[:, None, :] * (1.0 - occ[None, :, :])).sum(2)       # n↑ (1 − ñ↓) summed over sites
     Eu = (Lv * R * dop).sum() / norm
-    return float(Ek), float(Eu)
+    if not structure_factors:
+        return float(Ek), float(Eu)
+    sf = [(Lv * R * op).sum() / norm for op in staggered_correlations(m, occ)]
+    return (float(Ek), float(Eu)) + tuple(float(x) for x in sf)
 
 
 def ground_state_vector(K: np.ndarray, Ht: float, U: float):

@@ -3,9 +3,12 @@ the Fock-space value of the distribution the sweep samples (`tests/fock_exact.py
 finite Δτ) and, as Δτ → 0, exact diagonalisation of the Hubbard Hamiltonian.
 
 Bars: the Fock machinery is pinned by closed forms (U = 0 rows, |det|² of one field, Richardson
-limit = ED ground state); the sampled averages of Ek and Eu agree with the exact values within
-3σ of the chain-to-chain spread. A dropped HalfeK sandwich moves Eu by 0.037 (≈ 3σ here and
-≈ 6σ in the GPU test), a swapped γ or a 10 % error in α by ≥ 0.19.
+limit = ED ground state); the sampled averages of all six observables of the row (Ek, Eu and the
+structure factors CDW0, CDW1, SDW0, SDW1 of `src/tU/phy_update.jl:194-235`) agree with the exact
+values jointly at the 95 % level (Hotelling T², `stat_bars.py`) and each within 3σ. A dropped
+HalfeK sandwich moves Eu by 0.037 (≈ 4σ here), a swapped γ or a 10 % error in α by ≥ 0.19, a
+q-phase off by one cell flips the sign of CDW1 and SDW1 (≥ 180σ), keeping the discarded
+`+adjoint(` halves doubles the structure factors (≥ 40σ).
 """
 from concurrent.futures import ThreadPoolExecutor
 
@@ -15,6 +18,9 @@ import pytest
 from kapdqmc_amd.models import tU_Hubbard_Para
 
 import fock_exact as F
+from stat_bars import assert_agrees
+
+ROW = ["Ek", "Eu", "CDW0", "CDW1", "SDW0", "SDW1"]
 
 
 def tiny(U=4.0, dt=0.1, theta=4.0, bs=10):
@@ -23,13 +29,15 @@ def tiny(U=4.0, dt=0.1, theta=4.0, bs=10):
 
 
 def test_fock_free_limit_equals_oracle_rows(oracle_mod):
-    """U = 0: every field gives the same G, so one measured row is exact."""
+    """U = 0: every field gives the same G, so one measured row is exact, structure factors
+    included (their Fock operators are written from the staggered densities, not from G)."""
     m = tiny(U=0.0, theta=2.0)
     o = oracle_mod.OracleTU(m, seed=3)
     o.initial_s()
     row = o.phy_update(1, True)[0]
-    Ek, Eu = F.expectation(m)
-    assert abs(row[0] - Ek) < 1e-9 and abs(row[1] - Eu) < 1e-9
+    ex = F.expectation(m, structure_factors=True)
+    assert np.allclose(row, ex, rtol=1e-9, atol=1e-11), (row, ex)
+    Ek, Eu = ex[:2]
     eps = np.sort(np.linalg.eigvalsh(m.Ht * m.K))[: m.Ns // 2]
     assert abs(Ek - 2 * eps.sum()) < 1e-6 and abs(Eu - m.Ns / 4) < 1e-6
 
@@ -60,23 +68,22 @@ def test_fock_dtau_limit_is_ground_state():
 
 def test_oracle_samples_exact_distribution(oracle_mod):
     m = tiny()
-    Ek_x, Eu_x = F.expectation(m)
+    exact = np.array(F.expectation(m, structure_factors=True))
     nch, nsw = 32, 800
 
     def chain(c):
         o = oracle_mod.OracleTU(m, seed=500 + c, chain=c)
         o.initial_s()
         o.phy_update(20, False)
-        return o.phy_update(nsw, True)[:, :2].mean(axis=0)
+        return o.phy_update(nsw, True).mean(axis=0)
 
     with ThreadPoolExecutor(8) as ex:
         per = np.array(list(ex.map(chain, range(nch))))
-    mean = per.mean(axis=0)
     err = per.std(axis=0, ddof=1) / np.sqrt(nch)
-    print(f"Ek {mean[0]:.4f} ± {err[0]:.4f} (exact {Ek_x:.4f}); "
-          f"Eu {mean[1]:.4f} ± {err[1]:.4f} (exact {Eu_x:.4f})")
-    assert abs(mean[0] - Ek_x) < 3 * err[0]
-    assert abs(mean[1] - Eu_x) < 3 * err[1]
+    assert_agrees(per, exact, ROW)
+    # the slips named above would be far outside these bars
+    assert abs(2 * exact[3]) > 50 * err[3] and abs(2 * exact[5]) > 50 * err[5]
+    assert abs(exact[2]) > 20 * err[2] and abs(exact[4]) > 20 * err[4]
 
 
 # ----------------------------------------------------------------- Rényi-2 entropy
