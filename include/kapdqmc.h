@@ -147,6 +147,16 @@ int kdq_init_chain(kdq_handle h);
  * src/tV/phy_update.jl:207-314, src/tUV/phy_update.jl:205-355); out receives
  * nchains x kdq_num_obs doubles. */
 int kdq_measure(kdq_handle h, int32_t lt, double *out);
+/* kdq_measure without any device allocation. No counterpart in the reference: for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). The per-chain working matrices of the measurement (the
+ * copy of G wrapped to Nt/2 and its wrap temporary, 4 ns² doubles per chain), their operand
+ * tables and the accumulators live in a static device array of the library, one per GPU, of
+ * at most 32 MiB; a batch that needs more returns KDQ_EARG (use kdq_measure). The handle's
+ * device must be current. Every call on one GPU shares that GPU's array (calls from several
+ * threads or handles that overlap on one GPU race); each call clears the accumulators it
+ * reads. Same observables and layout as kdq_measure (sweep handles only). */
+int kdq_measure_noalloc(kdq_handle h, int32_t lt, double *out);
 
 /* Time-displaced Green's functions of chain c from its current fields (t-U), replacing
  * G4(model, s, τ1, τ2) (src/tU/GreenMatrix.jl:126-226): G1 = G(τ1), G2 = G(τ2),

@@ -832,29 +832,32 @@ int wrap_slice(kdq_handle h, Opnd *Gt, Opnd *Tt, int lt, bool forward, int fam) 
 }
 
 // phy_measure: wrap a copy of G to τ = Nt/2, HalfeK sandwich, observables (accumulated).
-int measure(kdq_handle h, int lt) {
+// Gm, Tm: per-chain N x N complex working matrices; meas: accumulators (nullptr: the chains').
+int measure_into(kdq_handle h, int lt, Opnd *Gm, Opnd *Tm, double *meas) {
     if (h->mixed)
-        HIPCHK(launch_to_f64(h->stream, h->nch, h->max_ns, h->d_chains, h->t_G32[h->cur32], h->t_Gm, DIM_NS, 1));
+        HIPCHK(launch_to_f64(h->stream, h->nch, h->max_ns, h->d_chains, h->t_G32[h->cur32], Gm, DIM_NS, 1));
     else
-        HIPCHK(launch_copy_mat(h->stream, h->nch, h->max_ns, h->max_ns, h->d_chains, h->t_G[h->cur], h->t_Gm, DIM_NS, DIM_NS, true));
+        HIPCHK(launch_copy_mat(h->stream, h->nch, h->max_ns, h->max_ns, h->d_chains, h->t_G[h->cur], Gm, DIM_NS, DIM_NS, true));
     const int nt = h->nt;
     if (lt > nt / 2.0) {
         for (int t = lt; t >= nt / 2 + 1; --t)
-            if (int r = wrap_slice(h, h->t_Gm, h->t_Tm, t, false, KDQ_PROF_MEAS_WRAP)) return r;
+            if (int r = wrap_slice(h, Gm, Tm, t, false, KDQ_PROF_MEAS_WRAP)) return r;
     } else {
         for (int t = lt + 1; t <= nt / 2; ++t)
-            if (int r = wrap_slice(h, h->t_Gm, h->t_Tm, t, true, KDQ_PROF_MEAS_WRAP)) return r;
+            if (int r = wrap_slice(h, Gm, Tm, t, true, KDQ_PROF_MEAS_WRAP)) return r;
     }
-    GemmArgs g1 = gargs(h, h->t_hK, h->t_Gm, h->t_Tm, DIM_NS, DIM_NS, DIM_NS);
+    GemmArgs g1 = gargs(h, h->t_hK, Gm, Tm, DIM_NS, DIM_NS, DIM_NS);
     if (int r = gemm(h, false, true, 0, 0, g1, KDQ_PROF_MEAS_WRAP)) return r;
-    GemmArgs g2 = gargs(h, h->t_Tm, h->t_hKinv, h->t_Gm, DIM_NS, DIM_NS, DIM_NS);
+    GemmArgs g2 = gargs(h, Tm, h->t_hKinv, Gm, DIM_NS, DIM_NS, DIM_NS);
     if (int r = gemm(h, true, false, 0, 0, g2, KDQ_PROF_MEAS_WRAP)) return r;
-    MeasArgs m{h->d_chains, h->t_Gm};
+    MeasArgs m{h->d_chains, Gm, meas};
     ProfScope ps(h, KDQ_PROF_MEASURE);
     if (h->kind == KDQ_MODEL_TU) HIPCHK(launch_measure_tu(h->stream, h->nch, m));
     else HIPCHK(launch_measure_model(h->stream, h->nch, h->kind, m));
     return KDQ_OK;
 }
+
+int measure(kdq_handle h, int lt) { return measure_into(h, lt, h->t_Gm, h->t_Tm, nullptr); }
 
 int elems(kdq_handle h, int c) {
     const int n = h->ch[c].ns;
@@ -1559,6 +1562,111 @@ int kdq_measure(kdq_handle h, int32_t lt, double *out) {
     HIPCHK(launch_zero_meas(h->stream, h->nch, h->d_chains));
     if (int r = measure(h, lt)) return r;
     return read_meas(h, out);
+}
+
+}  // extern "C"
+
+// ---- measurement without device allocation -------------------------------------------------
+// Working memory of kdq_measure_noalloc: a file-scope device array, one copy per GPU, held
+// until the process ends. Host code never passes `scratch` itself to a kernel or a copy: the
+// symbol's name is a host-side stand-in; the device address comes from hipGetSymbolAddress,
+// looked up once per device and kept in g_scratch_ptr.
+namespace {
+
+constexpr size_t KDQ_NOALLOC_SCRATCH_BYTES = size_t(32) << 20;
+static __device__ __align__(256) unsigned char scratch[KDQ_NOALLOC_SCRATCH_BYTES];
+
+constexpr int KDQ_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch_ptr[KDQ_MAX_DEVICES] = {};
+
+size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+// Device address of this device's scratch array (dev is the current device).
+int scratch_base(kdq_handle h, int dev, unsigned char **out) {
+    if (dev < 0 || dev >= KDQ_MAX_DEVICES) return fail(h, KDQ_EARG, "kdq_measure_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(h, KDQ_EHIP, std::string("kdq_measure_noalloc: hipGetSymbolAddress: ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = (unsigned char *)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return KDQ_OK;
+}
+
+// Scratch layout: [Opnd tables Gm[nch], Tm[nch]] [accumulators nch x KDQ_MEAS_SLOTS]
+// [per chain: Gm re, Gm im, Tm re, Tm im, each ns x ns doubles padded to 256 bytes].
+struct NoallocLayout {
+    size_t off_meas, off_mats, total;
+};
+
+NoallocLayout noalloc_layout(kdq_handle h) {
+    NoallocLayout L{};
+    L.off_meas = align256(2 * sizeof(Opnd) * (size_t)h->nch);
+    L.off_mats = L.off_meas + align256(sizeof(double) * KDQ_MEAS_SLOTS * (size_t)h->nch);
+    size_t t = L.off_mats;
+    for (int c = 0; c < h->nch; ++c) t += 4 * align256(sizeof(double) * (size_t)h->ch[c].ns * h->ch[c].ns);
+    L.total = t;
+    return L;
+}
+
+// Fills the operand tables that point each chain's Gm / Tm into the scratch array (the same
+// prefix sums as noalloc_layout, from the chains' ns) and clears the accumulators.
+__global__ void k_noalloc_layout(const ChainDev *chains, int nch, unsigned char *base, size_t off_meas,
+                                 size_t off_mats) {
+    Opnd *tGm = (Opnd *)base, *tTm = tGm + nch;
+    double *meas = (double *)(base + off_meas);
+    for (int q = threadIdx.x; q < nch * KDQ_MEAS_SLOTS; q += blockDim.x) meas[q] = 0.0;
+    for (int c = threadIdx.x; c < nch; c += blockDim.x) {
+        size_t off = off_mats;
+        for (int k = 0; k < c; ++k) off += 4 * (((size_t)chains[k].ns * chains[k].ns * sizeof(double) + 255) & ~size_t(255));
+        const int n = chains[c].ns;
+        const size_t plane = ((size_t)n * n * sizeof(double) + 255) & ~size_t(255);
+        double *g = (double *)(base + off);
+        tGm[c] = Opnd{g, (double *)(base + off + plane), n, 0};
+        tTm[c] = Opnd{(double *)(base + off + 2 * plane), (double *)(base + off + 3 * plane), n, 0};
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int kdq_measure_noalloc(kdq_handle h, int32_t lt, double *out) {
+    if (!h || !out || lt < 1 || lt > h->nt) return fail(h, KDQ_EARG, "kdq_measure_noalloc: bad argument");
+    if (h->scee) return fail(h, KDQ_EARG, "kdq_measure_noalloc: sweep handles only (not swap-replica handles)");
+    int dev = -1, sdev = -1;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipStreamGetDevice(h->stream, &sdev));
+    if (dev != sdev) return fail(h, KDQ_EARG, "kdq_measure_noalloc: the handle's device must be the current device");
+    const NoallocLayout L = noalloc_layout(h);
+    if (L.total > KDQ_NOALLOC_SCRATCH_BYTES)
+        return fail(h, KDQ_EARG, "kdq_measure_noalloc: the batch needs " + std::to_string(L.total) +
+                                     " bytes of working memory; the limit is " +
+                                     std::to_string(KDQ_NOALLOC_SCRATCH_BYTES) + " (use kdq_measure)");
+    unsigned char *base = nullptr;
+    if (int r = scratch_base(h, dev, &base)) return r;
+    hipLaunchKernelGGL(k_noalloc_layout, dim3(1), dim3(256), 0, h->stream, h->d_chains, h->nch, base, L.off_meas,
+                       L.off_mats);
+    HIPCHK(hipGetLastError());
+    Opnd *tGm = (Opnd *)base, *tTm = tGm + h->nch;
+    double *meas = (double *)(base + L.off_meas);
+    const uint32_t mask = h->prof_mask;   // kernel timing would create events: off for this call
+    h->prof_mask = 0;
+    const int r = measure_into(h, lt, tGm, tTm, meas);
+    h->prof_mask = mask;
+    if (r) return r;
+    const int S = KDQ_MEAS_SLOTS, no = h->nobs;
+    HIPCHK(hipMemcpyAsync(h->h_meas, meas, sizeof(double) * S * (size_t)h->nch, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    for (int c = 0; c < h->nch; ++c) {
+        const double cnt = h->h_meas[S * c + S - 1];
+        for (int k = 0; k < no; ++k) out[no * c + k] = cnt > 0 ? h->h_meas[S * c + k] / cnt : 0.0;
+    }
+    return KDQ_OK;
 }
 
 int kdq_profile(kdq_handle h, uint32_t mask) {

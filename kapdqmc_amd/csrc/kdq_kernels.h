@@ -151,6 +151,7 @@ size_t factor_lds_bytes(int which, int max_n);
 struct MeasArgs {
     const ChainDev *chains;
     const Opnd *G;
+    double *meas;   // nullptr: accumulate into chains[c].meas; else into meas + c * KDQ_MEAS_SLOTS
 };
 hipError_t launch_measure_tu(hipStream_t st, int nchains, const MeasArgs &m);
 

@@ -101,13 +101,14 @@ __global__ __launch_bounds__(NT) void k_measure_tu(MeasArgs a) {
             out[k] = s;
         }
         const double n2 = (double)n * (double)n;
-        ch.meas[0] += -2.0 * ch.ht * out[0];
-        ch.meas[1] += out[1];
-        ch.meas[2] += out[2] / n2;
-        ch.meas[3] += out[3] / n2;
-        ch.meas[4] += out[4] / n2;
-        ch.meas[5] += out[5] / n2;
-        ch.meas[KDQ_MEAS_SLOTS - 1] += 1.0;
+        double *acc = a.meas ? a.meas + (size_t)c * KDQ_MEAS_SLOTS : ch.meas;
+        acc[0] += -2.0 * ch.ht * out[0];
+        acc[1] += out[1];
+        acc[2] += out[2] / n2;
+        acc[3] += out[3] / n2;
+        acc[4] += out[4] / n2;
+        acc[5] += out[5] / n2;
+        acc[KDQ_MEAS_SLOTS - 1] += 1.0;
     }
 }
 

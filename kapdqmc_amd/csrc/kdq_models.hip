@@ -461,7 +461,7 @@ __global__ __launch_bounds__(NT) void k_measure_tv(MeasArgs a) {
     v[0] *= ch.ht;
     for (int k = 2; k < 10; ++k) v[k] *= f;
     __shared__ double red[10 * (NT / 64)];
-    block_reduce_add(v, 10, red, ch.meas, 1.0);
+    block_reduce_add(v, 10, red, a.meas ? a.meas + (size_t)blockIdx.x * KDQ_MEAS_SLOTS : ch.meas, 1.0);
 }
 
 // phy_measure of t-U-V (src/tUV/phy_update.jl:205-355): Ek, Eu, Ev, CDW0/1, SDW0/1.
@@ -546,7 +546,7 @@ __global__ __launch_bounds__(NT) void k_measure_tuv(MeasArgs a) {
     v[2] *= ch.V;
     for (int k = 3; k < 7; ++k) v[k] /= n2;
     __shared__ double red[7 * (NT / 64)];
-    block_reduce_add(v, 7, red, ch.meas, 1.0);
+    block_reduce_add(v, 7, red, a.meas ? a.meas + (size_t)blockIdx.x * KDQ_MEAS_SLOTS : ch.meas, 1.0);
 }
 
 }  // namespace

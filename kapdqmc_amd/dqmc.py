@@ -175,6 +175,13 @@ class DQMCBatch:
         self._chk(self.L.kdq_measure(self.h, lt, _dp(out)))
         return out
 
+    def measure_noalloc(self, lt: int) -> np.ndarray:
+        """measure() through kdq_measure_noalloc: the working matrices live in the library's
+        per-device static array (at most 32 MiB for the whole batch), nothing is allocated."""
+        out = np.zeros((self.n, self.nobs))
+        self._chk(self.L.kdq_measure_noalloc(self.h, lt, _dp(out)))
+        return out
+
     def profile(self, families=()):
         """Enable HIP-event timing of the named kernel families (see _lib.PROF_FAMILIES)."""
         mask = 0
