@@ -7,12 +7,18 @@ slices, FP64, local updates as delayed rank-k blocks, record = true (40 measurem
 sweep as in the reference). A step is one sweep (forward + backward) of every chain.
 
 Measured configuration (value): --chains independent Markov chains (seeds 1..C) advanced
-together, every kernel launch covering all chains. Reported beside it: the plain
-single-chain FP64 sweep (the drop-in path one reference call corresponds to).
+together in FP64, every kernel launch covering all chains. Reported beside it: the plain
+single-chain FP64 sweep (the drop-in path one reference call corresponds to) with
+Sherman-Morrison (k = 1) and delayed rank-k updates (k = 8..64); the batch at each k; the
+mixed-precision batch (own seeds) with its max |G_FP32 - G_FP64|; and the mixed + batched +
+self-learning leg (a network trained on the chains' own (field, log W) pairs, one exact global
+move per chain after every sweep) with its own τ_int, acceptance and error.
 
 value = effective independent samples per second of the double occupancy D = Eu/N,
 summed over chains: C · (sweeps/s) / (2 τ_int), τ_int from the autocovariance of all
-chains pooled (Sokal windowing).
+chains pooled (Sokal windowing) over --tau-sweeps recorded sweeps after --warmup warm-up
+sweeps (the timed ones first), cross-checked by binning, with the drift of the block means
+reported as an equilibration check.
 """
 from __future__ import annotations
 
@@ -102,11 +108,12 @@ def cpu_baseline(m, tau, nslices: int):
 
 
 def run_batch(m, nch: int, seed0: int, delay, warmup: int, steps: int, profile_all: bool,
-              precision: str = "fp64", graph: bool = True):
+              precision: str = "fp64", graph: bool = True, extra: int = 0, keep: bool = False):
     """W warm-up sweeps (the first with every kernel family timed by HIP events, host
     launches, to find the dominant family; the graph of a sweep is captured untimed after
-    them), `steps` timed sweeps replayed from the graph, then one sweep with host launches
-    and the dominant family (or every family) timed by HIP events on the handle's stream."""
+    them), `steps` timed sweeps replayed from the graph, `extra` more recorded sweeps (untimed,
+    for the autocorrelation estimate), then one sweep with host launches and the dominant
+    family (or every family) timed by HIP events on the handle's stream."""
     import numpy as np
     import torch
     from kapdqmc_amd.dqmc import DQMCBatch
@@ -121,7 +128,7 @@ def run_batch(m, nch: int, seed0: int, delay, warmup: int, steps: int, profile_a
     tuned = None
     if not delay:   # autotuner: block size and GEMM tile from timed slices of this batch
         from kapdqmc_amd import autotune
-        tuned = autotune.tune(b, delays=(16, 32, 64), tiles=(0, 32, 64), slices=4)
+        tuned = autotune.tune(b, delays=(8, 16, 32, 64), tiles=(0, 32, 64), slices=4)
         print(f"bench: {nch} chain(s), tuned delay {tuned['delay']} tile {tuned['gemm_tile']}",
               file=sys.stderr, flush=True)
     fams_all = ["wrap", "chain", "get_G", "decide", "flush", "qr", "inv", "meas_wrap", "measure"]
@@ -131,9 +138,11 @@ def run_batch(m, nch: int, seed0: int, delay, warmup: int, steps: int, profile_a
         if w == 0:
             prof_w = b.profile_read()
             b.profile([])
-        print(f"bench: {nch} chain(s), warm-up sweep {w + 1} done", file=sys.stderr, flush=True)
+        if w == 0 or (w + 1) % 10 == 0 or w + 1 == warmup:
+            print(f"bench: {nch} chain(s) {precision}, warm-up sweep {w + 1} done", file=sys.stderr, flush=True)
     dominant = max(prof_w, key=lambda f: prof_w[f][0])
     b.phy_update(0, True)   # captures the sweep graph (untimed)
+    b.precision_error()     # resets the FP32-vs-FP64 record (mixed)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     if graph:
@@ -146,18 +155,104 @@ def run_batch(m, nch: int, seed0: int, delay, warmup: int, steps: int, profile_a
         rows = np.concatenate(rows, axis=1)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    print(f"bench: {nch} chain(s), {steps} timed sweeps in {dt:.2f} s", file=sys.stderr, flush=True)
+    print(f"bench: {nch} chain(s) {precision}, {steps} timed sweeps in {dt:.2f} s", file=sys.stderr, flush=True)
     err = b.precision_error()   # max |G_FP32 - G_FP64| over the timed sweeps (mixed)
+    more = []
+    for k in range(0, extra, 20):
+        more.append(b.phy_update(min(20, extra - k), True))
+        print(f"bench: {nch} chain(s) {precision}, {min(k + 20, extra)} of {extra} further sweeps done",
+              file=sys.stderr, flush=True)
+    rows_all = np.concatenate([np.asarray(rows)] + more, axis=1) if more else np.asarray(rows)
     b.profile(list(prof_w) if profile_all else [dominant])
     b.phy_update(1, True)
     prof = b.profile_read()
+    b.profile([])
     cnt = b.counters()
     breakdown = {f: {"ms_per_sweep": prof_w[f][0], "launches_per_sweep": prof_w[f][1]} for f in prof_w}
-    return {"rows": np.asarray(rows), "dt": dt, "prof": prof, "dominant": dominant, "delay": b.delay,
-            "max_g_error": float(err.max()),
-            "tuned": tuned, "breakdown": breakdown,
-            "acceptance": float(cnt[:, 1].sum() / max(cnt[:, 0].sum(), 1)),
-            "decisions": int(cnt[:, 0].sum()), "near_ties": int(b.near_ties().sum())}
+    out = {"rows": np.asarray(rows), "rows_all": rows_all, "dt": dt, "prof": prof, "dominant": dominant,
+           "delay": b.delay, "max_g_error": float(err.max()), "g_error_per_chain": err,
+           "tuned": tuned, "breakdown": breakdown,
+           "acceptance": float(cnt[:, 1].sum() / max(cnt[:, 0].sum(), 1)),
+           "decisions": int(cnt[:, 0].sum()), "near_ties": int(b.near_ties().sum())}
+    if keep:
+        out["batch"] = b
+    else:
+        b.close()
+    return out
+
+
+def tau_report(D):
+    """τ_int of the per-sweep double occupancy D[chain, sweep] (pooled autocovariance, Sokal
+    windowing), cross-checked by binning (10 bins per chain: τ = ½ σ²_bins / σ²_naive, averaged
+    over chains), and an equilibration check: the slope of the means of four consecutive blocks
+    (one slope per chain, error from their spread)."""
+    import numpy as np
+    from kapdqmc_amd import stats
+    C, n = D.shape
+    tau = stats.tau_int_chains(D)
+    ratios = []
+    for c in range(C):
+        naive = D[c].std(ddof=1) / np.sqrt(n)
+        if naive > 0:
+            ratios.append((stats.binning_error(D[c], nbins=10) / naive) ** 2)
+    tau_bin = 0.5 * float(np.mean(ratios)) if ratios else float("nan")
+    nb = n // 4
+    blocks = D[:, :4 * nb].reshape(C, 4, nb).mean(axis=2)
+    x = np.arange(4) - 1.5
+    slopes = blocks @ x / (x @ x)
+    se = slopes.std(ddof=1) / np.sqrt(C) if C > 1 else float("nan")
+    return tau, {"tau_int": tau, "tau_int_binning": tau_bin, "sweeps_per_chain": n, "chains": C,
+                 "block_means": [float(v) for v in blocks.mean(axis=0)],
+                 "block_slope_in_sigma": float(slopes.mean() / se) if se > 0 else 0.0}
+
+
+def slmc_leg(m, C, seed0, delay, warmup, sweeps, inner, rounds, channels):
+    """Mixed precision + batched + self-learning global moves: C chains (own seeds) in FP32-carried
+    mode, an effective model trained on the chains' own (field, exact log W) pairs, then
+    `sweeps` recorded local sweeps each followed by one global move per chain (inner chain of
+    `inner` steps on exp(-E), exact FP64 acceptance). ESS/s from its own τ_int of D."""
+    import numpy as np
+    import torch
+    from kapdqmc_amd import slmc
+    from kapdqmc_amd.dqmc import DQMCBatch
+    from kapdqmc_amd.rng import initial_fields
+    N = m.Ns
+    b = DQMCBatch([m] * C, [seed0 + c for c in range(C)], list(range(C)), delay=delay, precision="mixed")
+    for c in range(C):
+        b.set_fields(c, initial_fields(seed0 + c, c, N, m.Nt))
+    b.phy_update(warmup, False)
+    net = slmc.EffectiveModel.for_model(m, channels=channels, seed=1, max_batch=C)
+    t0 = time.perf_counter()
+    losses = slmc.train(b, net, rounds=rounds, sweeps_per_round=1, lr=3e-3, epochs=4)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    print(f"bench: SLMC leg trained in {t_train:.1f} s, loss {losses[0]:.3g} -> {losses[-1]:.3g}",
+          file=sys.stderr, flush=True)
+    b.phy_update(0, True)   # graph capture
+    b.precision_error()
+    rows, acc = [], []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(sweeps):
+        rows.append(b.phy_update(1, True))
+        acc.append(b.slmc_move(net, inner))
+        if (k + 1) % 20 == 0:
+            print(f"bench: SLMC leg {k + 1} of {sweeps} sweeps + moves done", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    err = b.precision_error()
+    rows = np.concatenate(rows, axis=1)
+    b.close()
+    net.close()
+    D = rows[:, :, 1] / N
+    tau, rep = tau_report(D)
+    sweeps_s = C * sweeps / dt
+    return D, {"effective_samples_per_s": sweeps_s / (2 * tau), "chain_sweeps_per_s": sweeps_s,
+               "ms_per_sweep_with_move": dt / sweeps * 1e3, "global_move_acceptance": float(np.mean(acc)),
+               "inner_steps_per_move": inner, "network_channels": channels, "training_rounds": rounds,
+               "training_s": t_train, "training_loss_first_last": [losses[0], losses[-1]],
+               "max_abs_G_fp32_minus_fp64": float(err.max()), "autocorrelation": rep,
+               "double_occupancy_mean": float(D.mean())}
 
 
 def bench_scee(args):
@@ -339,15 +434,23 @@ def bench_uneven(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--tau-sweeps", type=int, default=200,
+                    help="recorded sweeps per chain (timed ones included) for tau_int of D")
     ap.add_argument("--chains", type=int, default=None,
                     help="independent chains batched per launch (default 64; scee 32, slmc 16)")
     ap.add_argument("--delay", type=int, default=0, help="block size k (0: autotune k and the GEMM tile)")
     ap.add_argument("--plain-steps", type=int, default=6, help="sweeps of the single-chain line (0: skip)")
     ap.add_argument("--plain-delay", type=int, default=16)
-    ap.add_argument("--mixed-steps", type=int, default=4,
-                    help="sweeps of the mixed-precision line on the same batch (0: skip)")
+    ap.add_argument("--mixed-steps", type=int, default=40,
+                    help="sweeps of the mixed-precision line (own seeds) (0: skip)")
+    ap.add_argument("--scan", type=int, default=1, help="1: per-k timings of the batch and the single chain")
+    ap.add_argument("--slmc-sweeps", type=int, default=60,
+                    help="sweeps of the mixed+batched+SLMC leg, each followed by a global move (0: skip)")
+    ap.add_argument("--slmc-rounds", type=int, default=8, help="training rounds of the SLMC leg's network")
+    ap.add_argument("--slmc-channels", type=int, default=8, help="hidden channels of the SLMC leg's network")
+    ap.add_argument("--slmc-leg-inner", type=int, default=1, help="inner steps per global move (SLMC leg)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="timed sweeps with host launches")
@@ -414,12 +517,28 @@ def main():
         print(f"bench: {C} chain(s), {args.counter_sample} slices (delay {b.delay}) done", file=sys.stderr, flush=True)
         return
     C = args.chains
+    extra = max(0, args.tau_sweeps - args.steps)
     bat = run_batch(m, C, args.seed, args.delay, args.warmup, args.steps, args.profile_all,
-                    graph=not args.no_graph)
-    D = bat["rows"][:, :, 1] / N                     # [C, steps]
-    tau = stats.tau_int_chains(D)
+                    graph=not args.no_graph, extra=extra, keep=True)
+    D_all = bat["rows_all"][:, :, 1] / N             # [C, tau_sweeps]
+    tau, tau_rep = tau_report(D_all)
     sweeps_s = C * args.steps / bat["dt"]            # chain-sweeps per second
     ess_s = sweeps_s / (2 * tau)
+
+    scan_batch = None
+    if args.scan:   # the same batch at each block size k (graph replays, 2 sweeps each)
+        b = bat["batch"]
+        scan_batch = {}
+        for k in (8, 16, 32, 64):
+            b.set_option("delay", k)
+            b.phy_update(1, False)                   # re-capture at this k (untimed)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            b.phy_update(2, False)
+            torch.cuda.synchronize()
+            scan_batch[str(k)] = {"ms_per_sweep": (time.perf_counter() - t0) / 2 * 1e3}
+        print(f"bench: batch k scan {scan_batch}", file=sys.stderr, flush=True)
+    bat["batch"].close()
 
     plain = None
     if args.plain_steps > 0:
@@ -427,25 +546,48 @@ def main():
         p_sweeps = args.plain_steps / one["dt"]
         plain = {"sweeps_per_s": p_sweeps, "ms_per_sweep": one["dt"] / args.plain_steps * 1e3,
                  "effective_samples_per_s": p_sweeps / (2 * tau), "delay_k": args.plain_delay,
-                 "note": "same Markov chain, so the batched run's tau_int is used"}
+                 "note": "one chain per handle, the drop-in path of one reference call; same Markov chain "
+                         "as the batch, so the batch's tau_int is used"}
+        if args.scan:   # Sherman-Morrison (k = 1) and delayed rank-k blocks, single chain
+            plain["by_delay_k"] = {}
+            for k in (1, 8, 16, 32, 64):
+                r = run_batch(m, 1, args.seed, k, 1, 2, False)
+                plain["by_delay_k"][str(k)] = {
+                    "ms_per_sweep": r["dt"] / 2 * 1e3,
+                    "decide_ms_per_sweep": r["breakdown"]["decide"]["ms_per_sweep"],
+                    "flush_ms_per_sweep": r["breakdown"]["flush"]["ms_per_sweep"],
+                    "launches_per_sweep": sum(v["launches_per_sweep"] for v in r["breakdown"].values())}
+            print(f"bench: single-chain k scan {plain['by_delay_k']}", file=sys.stderr, flush=True)
 
     mixed = None
     if args.mixed_steps > 0:
-        mx = run_batch(m, C, args.seed, bat["delay"], args.warmup, args.mixed_steps, False, precision="mixed")
+        mx = run_batch(m, C, args.seed + 1000, bat["delay"], args.warmup, args.mixed_steps, False,
+                       precision="mixed")
         Dm = mx["rows"][:, :, 1] / N
         m_sweeps = C * args.mixed_steps / mx["dt"]
-        # same sweeps of the same seeds; the chains part ways at the first differing decision,
-        # so the difference is judged against the chain-to-chain spread
-        d64, d32 = D[:, :args.mixed_steps].mean(axis=1), Dm.mean(axis=1)
+        # independent seeds: group means of the per-chain means, each with its own error
+        d64, d32 = D_all.mean(axis=1), Dm.mean(axis=1)
         sig = np.sqrt(d64.var(ddof=1) / C + d32.var(ddof=1) / C)
         mixed = {"sweeps_per_s": m_sweeps, "ms_per_sweep": mx["dt"] / args.mixed_steps * 1e3,
-                 "effective_samples_per_s": m_sweeps / (2 * tau),
                  "max_abs_G_fp32_minus_fp64": mx["max_g_error"],
+                 "median_chain_max_abs_G_fp32_minus_fp64": float(np.median(mx["g_error_per_chain"])),
                  "double_occupancy_mean": float(Dm.mean()),
                  "double_occupancy_fp64_minus_mixed": float(d64.mean() - d32.mean()),
                  "difference_in_sigma": float((d64.mean() - d32.mean()) / sig),
-                 "note": "G carried in FP32 between stabilisations; error = max over the "
-                         "stabilisations of the timed sweeps; same seeds as the FP64 line"}
+                 "note": "G carried in FP32 between stabilisations; error = max over the stabilisations "
+                         "of the timed sweeps vs the FP64 recompute; seeds independent of the FP64 line "
+                         f"({args.seed + 1000}..), D compared as group means with their own errors"}
+
+    slmc_l = None
+    if args.slmc_sweeps > 0:
+        Ds, slmc_l = slmc_leg(m, C, args.seed + 2000, bat["delay"], args.warmup, args.slmc_sweeps,
+                              args.slmc_leg_inner, args.slmc_rounds, args.slmc_channels)
+        d64, ds = D_all.mean(axis=1), Ds.mean(axis=1)
+        sig = np.sqrt(d64.var(ddof=1) / C + ds.var(ddof=1) / C)
+        slmc_l["double_occupancy_fp64_minus_slmc_in_sigma"] = float((d64.mean() - ds.mean()) / sig)
+        if plain:
+            slmc_l["ratio_vs_plain_fp64"] = slmc_l["effective_samples_per_s"] / plain["effective_samples_per_s"]
+        slmc_l["meets_error_bound_1e-6"] = slmc_l["max_abs_G_fp32_minus_fp64"] <= 1e-6
 
     dominant = bat["dominant"]
     ms, nl = bat["prof"][dominant]
@@ -487,10 +629,12 @@ def main():
                                f"{C} independent chains batched per launch, FP64, record=true",
                    "chains": C, "delay_k": k_used,
                    "autotuned": bat["tuned"] is not None,
-                   "variant": "batched FP64 (mixed precision and SLMC moves not in this line)"},
+                   "variant": "batched FP64: the equal-error form (the mixed+batched+SLMC leg is measured "
+                              "below and misses the 1e-6 bound on max|dG|)"},
         "sweeps_per_s": sweeps_s,
         "tau_int_double_occupancy": tau,
-        "double_occupancy_mean": float(D.mean()),
+        "autocorrelation": tau_rep,
+        "double_occupancy_mean": float(D_all.mean()),
         "acceptance": bat["acceptance"],
         "near_ties": {"count": bat["near_ties"], "decisions": bat["decisions"],
                       "note": "Metropolis decisions of the whole run with |uniform draw - acceptance "
@@ -499,6 +643,8 @@ def main():
         "plain_fp64_1chain": plain,
         "ratio_vs_plain": (ess_s / plain["effective_samples_per_s"]) if plain else None,
         "mixed_precision": mixed,
+        "mixed_batched_slmc": slmc_l,
+        "batch_by_delay_k": scan_batch,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernel_breakdown_warmup": bat["breakdown"],
