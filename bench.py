@@ -360,6 +360,37 @@ def bench_slmc(args):
         "global_move_acceptance_untrained": float(np.mean(acc))}))
 
 
+def bench_mixed_curve(args):
+    """Mixed-precision error against the recompute interval: the N = 288 t-U model with
+    stabilisation (FP64 recompute of G) every BS = 1, 2, 5, 10 slices, --chains chains (16),
+    FP64 and mixed: ms per sweep, and max / median over chains of max |G_FP32 - G_FP64| at the
+    recomputes of the timed sweeps."""
+    import numpy as np
+    import torch
+    from kapdqmc_amd.models import tU_Hubbard_Para
+    C = args.chains
+    res = {}
+    for bs in (1, 2, 5, 10):
+        m = tU_Hubbard_Para(Ht=1.0, Hu1=4.0, Hu2=4.0, Δt=0.1, Θrelax=10.0, Θquench=0.0,
+                            Lattice="HoneyComb120", site=[12, 12], BatchSize=bs)
+        row = {}
+        for prec in ("fp64", "mixed"):
+            r = run_batch(m, C, args.seed, 32, max(args.warmup, 1), args.steps, False, precision=prec)
+            row[prec + "_ms_per_sweep"] = r["dt"] / args.steps * 1e3
+            if prec == "mixed":
+                row["max_abs_dG"] = r["max_g_error"]
+                row["median_chain_max_abs_dG"] = float(np.median(r["g_error_per_chain"]))
+        res[str(bs)] = row
+        print(f"bench: mixed curve BS={bs}: {row}", file=sys.stderr, flush=True)
+    print(json.dumps({
+        "metric": "mixed-precision max|G_FP32 - G_FP64| vs recompute interval (t-U 12x12 N=288)",
+        "value": res["10"]["max_abs_dG"], "unit": "max abs dG at BS=10", "n_gpus": 1, "steps": args.steps,
+        "higher_is_better": False, "dtype": "mixed vs f64", "data": "synthetic (Philox-seeded HS fields)",
+        "config": {"workload": f"t-U honeycomb 12x12 N=288 U=4 dtau=0.1 2Theta=20, {C} chains, k=32, "
+                               "BatchSize (recompute interval) 1/2/5/10", "chains": C},
+        "by_recompute_interval": res}))
+
+
 def bench_uneven(args):
     """Uneven batch: 64 t-U chains, chain c on an L x L honeycomb with L = (6, 9, 12)[c % 3]
     (N = 72 / 162 / 288), U/t = (2, 4, 6)[(c // 3) % 3], seed c, Δτ = 0.1, 2Θ = 2L (Nt = 120 /
@@ -462,10 +493,12 @@ def main():
     ap.add_argument("--profile-all", action="store_true", help="time every kernel family (slower)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-sized buffer 5 times (scale for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
-    ap.add_argument("--workload", choices=("tu", "uneven", "scee-tv", "scee-tuv", "slmc"), default="tu",
+    ap.add_argument("--workload", choices=("tu", "uneven", "scee-tv", "scee-tuv", "slmc", "mixed-curve"),
+                    default="tu",
                     help="tu: the headline t-U sweep (default); uneven: 64 t-U chains of L = 6/9/12 with "
                          "2Theta = 2L, mixed vs FP64; scee-tv / scee-tuv: swap-replica Renyi-2 on 9x9 "
-                         "t-V / t-U-V; slmc: self-learning global moves on 15x15 t-U")
+                         "t-V / t-U-V; slmc: self-learning global moves on 15x15 t-U; mixed-curve: "
+                         "mixed-precision error vs recompute interval")
     ap.add_argument("--slmc-inner", type=int, default=3, help="inner steps per global move (slmc)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -480,13 +513,15 @@ def main():
         sys.exit(1)
     torch.cuda.set_device(0)
     if args.chains is None:
-        args.chains = {"tu": 64, "uneven": 64, "slmc": 16}.get(args.workload, 32)
+        args.chains = {"tu": 64, "uneven": 64, "slmc": 16, "mixed-curve": 16}.get(args.workload, 32)
     if args.workload in ("scee-tv", "scee-tuv"):
         return bench_scee(args)
     if args.workload == "slmc":
         return bench_slmc(args)
     if args.workload == "uneven":
         return bench_uneven(args)
+    if args.workload == "mixed-curve":
+        return bench_mixed_curve(args)
     if args.calibrate_copy:
         n = args.calibrate_copy * (1 << 20) // 8
         x = torch.rand(n, dtype=torch.float64, device="cuda")

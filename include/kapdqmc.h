@@ -43,12 +43,12 @@ typedef struct kdq_model_desc {
     const double *alpha;     /* model.α, length nt */
     const double *eta;       /* model.η, length 4 */
     const double *gamma;     /* model.γ, length 4 */
-    const double *K;         /* model.K (real part; flux = 0) */
+    const double *K;         /* model.K (real part) */
     const double *eK;        /* model.eK */
     const double *eKinv;     /* model.eKinv */
     const double *halfeK;    /* model.HalfeK */
     const double *halfeKinv; /* model.HalfeKinv */
-    const double *Pt;        /* model.Pt (real) */
+    const double *Pt;        /* model.Pt (real part) */
     /* t-V and t-U-V only (ignored for t-U) */
     int32_t z;               /* coordination number (3 on the honeycomb lattice) */
     const int32_t *bonds;    /* model.nnidx as 1-based (x, y) pairs in column-major order:
@@ -56,6 +56,12 @@ typedef struct kdq_model_desc {
     double a;                /* t-U-V: model.a */
     int32_t is_complex;      /* t-U-V: model.Type == ComplexF64 */
     double U, V;             /* t-U-V: model.U, model.V (observable prefactors) */
+    /* t-U with complex hopping (flux != 0, src/public/Geometry.jl:178-198, stored ComplexF64 in
+     * tU_Hubbard_Para_, src/tU/model.jl:24-34): imaginary parts of K, eK, eKinv, HalfeK,
+     * HalfeKinv (ns x ns) and Pt (ns x ns/2), the real parts above. All six NULL: real
+     * matrices (flux = 0). All chains of a batch are real or all complex; complex matrices
+     * need FP64 precision. */
+    const double *K_im, *eK_im, *eKinv_im, *halfeK_im, *halfeKinv_im, *Pt_im;
 } kdq_model_desc;
 
 /* ---- host model construction (no GPU work) ----
@@ -120,6 +126,23 @@ int kdq_set_fields(kdq_handle h, int32_t c, const uint8_t *s);
 int kdq_get_fields(kdq_handle h, int32_t c, uint8_t *s);
 /* Initial_s (src/tU/GreenMatrix.jl:1-8) from the chain's Philox stream. */
 int kdq_initial_s(kdq_handle h);
+
+/* Replayed random stream of chain c: the local-update draws of the next `nsweeps` sweeps
+ * (from the handle's current sweep counter) come from the caller instead of Philox, so that
+ * the reference's own generator (the MersenneTwister of phy_update, src/tU/phy_update.jl:31)
+ * can drive the decisions. One draw per proposal, as UpdatePhyLayer! takes them
+ * (src/tU/phy_update.jl:259-262, src/tV/phy_update.jl:194-200, src/tUV/phy_update.jl:192-198):
+ * choice = the index (0, 1, 2) of `rand(rng, model.samplers_dict[s[i]])` among the other three
+ * field values in increasing order, u = the `rand(rng)` the acceptance compares with.
+ * Layout: entry ((s * 2 + pass) * nt + lt - 1) * n_elem + e, s = sweep since this call,
+ * pass 0 forward / 1 backward, e = the field element's linear index within the slice (the
+ * kdq_set_fields layout), n_elem = ns (t-U), 3 ns/2 (t-V), ns z (t-U-V). The reference visits
+ * the forward pass lt = 1..nt and the backward pass lt = nt..1, elements in order, except t-V
+ * (families 3, 2, 1 forward and 1, 2, 3 backward). The arrays are copied. nsweeps = 0 returns
+ * the chain to its Philox stream. kdq_phy_update / kdq_update_layer calls that would reach
+ * beyond the replayed sweeps return KDQ_EARG; choices above 2 are KDQ_EARG. Sweep handles
+ * only. */
+int kdq_set_replay(kdq_handle h, int32_t c, const uint8_t *choice, const double *u, int64_t nsweeps);
 
 /* phy_update(path, model, s, Sweeps, record) for every chain (t-U: src/tU/phy_update.jl:3-152,
  * t-V: src/tV/phy_update.jl:1-188, t-U-V: src/tUV/phy_update.jl:1-186).

@@ -20,7 +20,7 @@ OPTIONS = {"graph": 0, "delay": 1, "gemm_tile": 2, "defer": 3, "pipe": 4}   # kd
 # Every symbol include/kapdqmc.h declares (checked by tests/test_capi.py).
 EXPORTS = ("kdq_create", "kdq_destroy", "kdq_last_error", "kdq_num_chains", "kdq_num_obs", "kdq_set_fields",
            "kdq_get_fields", "kdq_initial_s", "kdq_phy_update", "kdq_get_G", "kdq_set_G", "kdq_wrap",
-           "kdq_update_layer", "kdq_init_chain", "kdq_measure", "kdq_measure_noalloc", "kdq_get_counters", "kdq_get_near_ties", "kdq_profile",
+           "kdq_update_layer", "kdq_init_chain", "kdq_measure", "kdq_measure_noalloc", "kdq_set_replay", "kdq_get_counters", "kdq_get_near_ties", "kdq_profile",
            "kdq_profile_read", "kdq_get_logdet", "kdq_set_option", "kdq_get_precision_error", "kdq_log_weight", "kdq_scee_create", "kdq_scee_set_fields",
            "kdq_scee_get_fields", "kdq_scee_init", "kdq_scee_sweeps", "kdq_scee_get_G4", "kdq_scee_detg",
            "kdq_dop_create", "kdq_G4", "kdq_model_build", "kdq_model_free", "kdq_net_create", "kdq_net_destroy", "kdq_net_last_error", "kdq_net_num_params", "kdq_net_get_params",
@@ -44,7 +44,10 @@ class ModelDesc(C.Structure):
                 ("eK", C.POINTER(C.c_double)), ("eKinv", C.POINTER(C.c_double)),
                 ("halfeK", C.POINTER(C.c_double)), ("halfeKinv", C.POINTER(C.c_double)),
                 ("Pt", C.POINTER(C.c_double)), ("z", C.c_int32), ("bonds", C.POINTER(C.c_int32)),
-                ("a", C.c_double), ("is_complex", C.c_int32), ("U", C.c_double), ("V", C.c_double)]
+                ("a", C.c_double), ("is_complex", C.c_int32), ("U", C.c_double), ("V", C.c_double),
+                ("K_im", C.POINTER(C.c_double)), ("eK_im", C.POINTER(C.c_double)),
+                ("eKinv_im", C.POINTER(C.c_double)), ("halfeK_im", C.POINTER(C.c_double)),
+                ("halfeKinv_im", C.POINTER(C.c_double)), ("Pt_im", C.POINTER(C.c_double))]
 
 
 class ModelParams(C.Structure):
@@ -87,6 +90,7 @@ def load(path: str = LIB_PATH):
     L.kdq_num_chains.argtypes = [vp]
     L.kdq_num_obs.argtypes = [vp]
     L.kdq_set_fields.argtypes = [vp, i32, C.POINTER(C.c_uint8)]
+    L.kdq_set_replay.argtypes = [vp, i32, C.POINTER(C.c_uint8), dp, C.c_int64]
     L.kdq_get_fields.argtypes = [vp, i32, C.POINTER(C.c_uint8)]
     L.kdq_initial_s.argtypes = [vp]
     L.kdq_phy_update.argtypes = [vp, i32, i32, dp]

@@ -65,6 +65,13 @@ struct ChainDev {
     // is never read: 0), *rank = rank after the latest block
     int *rk;
     int *rank;
+    // replayed draws (kdq_set_replay): nullptr = the Philox stream. Entry
+    // ((s * 2 + dir) * nt + lt - 1) * n_elem + e is the proposal of element e at slice lt of
+    // pass dir of sweep rp_sweep0 + s.
+    const uint8_t *rp_choice;
+    const double *rp_u;
+    uint32_t rp_sweep0, rp_nsweeps;
+    const double *Kim;     // imaginary part of K (complex hopping), nullptr when K is real
 };
 
 #define KDQ_MEAS_SLOTS 16
@@ -172,6 +179,25 @@ KDQ_HD void proposal_draw(uint64_t seed, uint32_t chain, uint32_t stream, uint32
                   (uint32_t)(seed >> 32), x);
     choice = (int)(((uint64_t)x[0] * 3u) >> 32);
     u = ((double)(x[1] >> 5) * 67108864.0 + (double)(x[2] >> 6)) * (1.0 / 9007199254740992.0);
+}
+
+// Draw of a local-update proposal of the plain sweep (Philox stream 0), or the caller's replayed
+// draw when the chain has a replay buffer. A sweep outside the buffer reads as a rejected
+// proposal (u = +inf); the host refuses calls that would reach one.
+KDQ_HD void sweep_draw(const ChainDev &ch, uint32_t sweep, int lt, int dir, uint32_t elem, int &choice, double &u) {
+    if (ch.rp_u) {
+        const uint32_t s = sweep - ch.rp_sweep0;
+        if (s < ch.rp_nsweeps) {
+            const size_t q = (((size_t)s * 2 + (size_t)dir) * (size_t)ch.nt + (size_t)(lt - 1)) * (size_t)ch.n_elem + elem;
+            choice = ch.rp_choice[q] % 3;
+            u = ch.rp_u[q];
+        } else {
+            choice = 0;
+            u = __builtin_huge_val();
+        }
+        return;
+    }
+    proposal_draw(ch.seed, ch.chain_id, 0, sweep, (uint32_t)lt | ((uint32_t)dir << 31), elem, choice, u);
 }
 
 KDQ_HD int init_field_value(uint64_t seed, uint32_t chain, uint32_t stream, uint32_t slice,

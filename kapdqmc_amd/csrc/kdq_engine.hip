@@ -59,6 +59,7 @@ void kdq_internal_set_error(const std::string &msg) { set_global_error(msg); }
 struct kdq_handle_s {
     int nch = 0, nt = 0, nn = 0, max_ns = 0, max_np = 0, delay = 16;
     int kind = KDQ_MODEL_TU, nobs = 6, max_elem = 0, max_nb = 0;
+    bool cK = false;   // complex model matrices (t-U with flux): eK, eK⁻¹, HalfeK, HalfeK⁻¹, K, P
     // rank-2 block tables (t-V: per family, t-U-V: one group)
     BlockDesc *d_blocks = nullptr;
     int *d_block_offset = nullptr, *d_nblocks = nullptr;
@@ -126,6 +127,7 @@ struct kdq_handle_s {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int g_record = -1;
+    std::vector<void *> rp_buf;               // [nch] replayed draws (kdq_set_replay), nullptr: Philox
 
     ~kdq_handle_s() {
         if (stream) hipStreamSynchronize(stream);
@@ -134,6 +136,8 @@ struct kdq_handle_s {
         for (auto &c : ch)
             for (void *p : c.allocs) hipFree(p);
         for (void *p : dev_misc) hipFree(p);
+        for (void *p : rp_buf)
+            if (p) hipFree(p);
         if (h_meas) hipHostFree(h_meas);
         for (auto &p : ev_pending) {
             hipEventDestroy(p.second.first);
@@ -267,6 +271,28 @@ int gemm(kdq_handle h, bool ac, bool bc, int opa, int opb, const GemmArgs &g, in
     return KDQ_OK;
 }
 
+// dst (N x Np, complex) <- the trial state P (real, or complex for complex hopping)
+hipError_t load_P(kdq_handle h, Opnd *dst) {
+    if (h->cK) return launch_copy_mat(h->stream, h->nch, h->max_ns, h->max_np, h->d_chains, h->t_Pt, dst, DIM_NS, DIM_NP, true);
+    return launch_real_to_cplx(h->stream, h->nch, h->max_ns, h->max_np, h->d_chains, h->t_Pt, dst, DIM_NS, DIM_NP);
+}
+
+// A GEMM with a model constant (eK, eK⁻¹, HalfeK, HalfeK⁻¹, P) as operand A (amodel) or B
+// (bmodel): real for flux = 0, complex for complex hopping (the kinetic factors are then
+// Hermitian), where a transposed model operand becomes its conjugate transpose.
+int gemm_m(kdq_handle h, bool amodel, bool ac, bool bmodel, bool bc, int opa, int opb, const GemmArgs &g,
+           int fam = KDQ_PROF_GEMM_OTHER) {
+    if (amodel && h->cK) {
+        ac = true;
+        if (opa == 1) opa = 2;
+    }
+    if (bmodel && h->cK) {
+        bc = true;
+        if (opb == 1) opb = 2;
+    }
+    return gemm(h, ac, bc, opa, opb, g, fam);
+}
+
 int gemm_f(kdq_handle h, bool ac, bool bc, int opa, int opb, const GemmArgsF &g, int fam) {
     ProfScope ps(h, fam);
     HIPCHK(launch_gemm_f32(h->stream, h->nch, maxdim(h, g.mdim), maxdim(h, g.ndim), maxdim(h, g.kdim), ac, bc, opa, opb,
@@ -298,28 +324,28 @@ int bondrot(kdq_handle h, Opnd *X, int family, int lt, int sign, int mode, int n
 // t-V kinetic wrap: G <- eK G eKinv (forward) or eKinv G eK (backward)
 int wrap_K(kdq_handle h, Opnd *Gt, Opnd *Tt, bool forward, int fam) {
     GemmArgs g1 = gargs(h, forward ? h->t_eK : h->t_eKinv, Gt, Tt, DIM_NS, DIM_NS, DIM_NS);
-    if (int r = gemm(h, false, true, 0, 0, g1, fam)) return r;
+    if (int r = gemm_m(h, true, false, false, true, 0, 0, g1, fam)) return r;
     GemmArgs g2 = gargs(h, Tt, forward ? h->t_eKinv : h->t_eK, Gt, DIM_NS, DIM_NS, DIM_NS);
-    return gemm(h, true, false, 0, 0, g2, fam);
+    return gemm_m(h, false, true, true, false, 0, 0, g2, fam);
 }
 
 // G <- D eK G eKinv D⁻¹ (forward) or eKinv D⁻¹ G D eK (backward), D = cis(α η_s) at slice lt.
 int wrap_B(kdq_handle h, Opnd *Gt, Opnd *Tt, int lt, bool forward, int fam = KDQ_PROF_WRAP) {
     if (forward) {
         GemmArgs g1 = gargs(h, h->t_eK, Gt, Tt, DIM_NS, DIM_NS, DIM_NS);
-        if (int r = gemm(h, false, true, 0, 0, g1, fam)) return r;
+        if (int r = gemm_m(h, true, false, false, true, 0, 0, g1, fam)) return r;
         GemmArgs g2 = gargs(h, Tt, h->t_eKinv, Gt, DIM_NS, DIM_NS, DIM_NS);
         g2.rowS = hs_diag(h, lt, +1);
         g2.colS = hs_diag(h, lt, -1);
-        return gemm(h, true, false, 0, 0, g2, fam);
+        return gemm_m(h, false, true, true, false, 0, 0, g2, fam);
     }
     // D' = D(-): T = eKinv · (D' G) · D'⁻¹, then G = T eK
     GemmArgs g1 = gargs(h, h->t_eKinv, Gt, Tt, DIM_NS, DIM_NS, DIM_NS);
     g1.inS = hs_diag(h, lt, -1);
     g1.colS = hs_diag(h, lt, +1);
-    if (int r = gemm(h, false, true, 0, 0, g1, fam)) return r;
+    if (int r = gemm_m(h, true, false, false, true, 0, 0, g1, fam)) return r;
     GemmArgs g2 = gargs(h, Tt, h->t_eK, Gt, DIM_NS, DIM_NS, DIM_NS);
-    return gemm(h, true, false, 0, 0, g2, fam);
+    return gemm_m(h, false, true, true, false, 0, 0, g2, fam);
 }
 
 // WrapKV!(…, "B") on the FP32 copy of G (mixed precision): the same two GEMMs as wrap_B in
@@ -376,7 +402,7 @@ int prop_R_range(kdq_handle h, Opnd *src, Opnd *dst, int lo, int hi) {
         Opnd *out = (lt == hi && cur != dst) ? dst : ((cur == h->t_P0) ? h->t_P1 : h->t_P0);
         GemmArgs g = gargs(h, h->t_eK, cur, out, DIM_NS, DIM_NP, DIM_NS);
         g.rowS = hs_diag(h, lt, +1);
-        if (int r = gemm(h, false, true, 0, 0, g, KDQ_PROF_CHAIN)) return r;
+        if (int r = gemm_m(h, true, false, false, true, 0, 0, g, KDQ_PROF_CHAIN)) return r;
         cur = out;
     }
     if (cur != dst) HIPCHK(launch_copy_mat(h->stream, h->nch, h->max_ns, h->max_np, h->d_chains, cur, dst, DIM_NS, DIM_NP, true));
@@ -391,7 +417,7 @@ int prop_L_range(kdq_handle h, Opnd *src, Opnd *dst, int lo, int hi) {
         Opnd *out = (lt == lo && cur != dst) ? dst : ((cur == h->t_P0) ? h->t_P1 : h->t_P0);
         GemmArgs g = gargs(h, h->t_eK, cur, out, DIM_NS, DIM_NP, DIM_NS);
         g.inS = hs_diag(h, lt, +1, 1);
-        if (int r = gemm(h, false, true, 1, 0, g, KDQ_PROF_CHAIN)) return r;
+        if (int r = gemm_m(h, true, false, false, true, 1, 0, g, KDQ_PROF_CHAIN)) return r;
         cur = out;
     }
     if (cur != dst) HIPCHK(launch_copy_mat(h->stream, h->nch, h->max_ns, h->max_np, h->d_chains, cur, dst, DIM_NS, DIM_NP, true));
@@ -406,7 +432,7 @@ int propagate_R(kdq_handle h, Opnd *src, Opnd *dst, int idx, double *logacc = nu
         Opnd *out = (lt == hi) ? dst : ((cur == h->t_P0) ? h->t_P1 : h->t_P0);
         GemmArgs g = gargs(h, h->t_eK, cur, out, DIM_NS, DIM_NP, DIM_NS);
         if (h->kind != KDQ_MODEL_TV) g.rowS = hs_diag(h, lt, +1);
-        if (int r = gemm(h, false, true, 0, 0, g, KDQ_PROF_CHAIN)) return r;
+        if (int r = gemm_m(h, true, false, false, true, 0, 0, g, KDQ_PROF_CHAIN)) return r;
         if (h->kind == KDQ_MODEL_TV)   // B(lt) = V1 V2 V3 eK: V3 first
             for (int j = 2; j >= 0; --j)
                 if (int r = bondrot(h, out, j, lt, +1, 1, DIM_NP, KDQ_PROF_CHAIN)) return r;
@@ -431,11 +457,11 @@ int propagate_L(kdq_handle h, Opnd *src, Opnd *dst, int idx) {
             for (int j = 0; j < 3; ++j)
                 if (int r = bondrot(h, tmp, j, lt, +1, 1, DIM_NP, KDQ_PROF_CHAIN)) return r;
             GemmArgs g = gargs(h, h->t_eK, tmp, out, DIM_NS, DIM_NP, DIM_NS);
-            if (int r = gemm(h, false, true, 1, 0, g, KDQ_PROF_CHAIN)) return r;
+            if (int r = gemm_m(h, true, false, false, true, 1, 0, g, KDQ_PROF_CHAIN)) return r;
         } else {
             GemmArgs g = gargs(h, h->t_eK, cur, out, DIM_NS, DIM_NP, DIM_NS);
             g.inS = hs_diag(h, lt, +1, 1);   // conj(D): D⁻¹ for unimodular phases, D for real ones
-            if (int r = gemm(h, false, true, 1, 0, g, KDQ_PROF_CHAIN)) return r;
+            if (int r = gemm_m(h, true, false, false, true, 1, 0, g, KDQ_PROF_CHAIN)) return r;
         }
         cur = out;
     }
@@ -482,11 +508,11 @@ int get_G(kdq_handle h, int idx, Opnd *dst = nullptr) {
 int log_weight(kdq_handle h, double *out) {
     const int NN = h->nn;
     HIPCHK(hipMemsetAsync(h->d_lw, 0, sizeof(double) * h->nch, h->stream));
-    HIPCHK(launch_real_to_cplx(h->stream, h->nch, h->max_ns, h->max_np, h->d_chains, h->t_Pt, h->t_UR[0], DIM_NS, DIM_NP));
+    HIPCHK(load_P(h, h->t_UR[0]));
     for (int idx = 1; idx <= NN - 1; ++idx)
         if (int r = propagate_R(h, h->t_UR[idx - 1], h->t_UR[idx], idx, h->d_lw)) return r;
     GemmArgs g = gargs(h, h->t_Pt, h->t_UR[NN - 1], h->t_M, DIM_NP, DIM_NP, DIM_NS);   // Pᵀ U_R
-    if (int r = gemm(h, false, true, 1, 0, g, KDQ_PROF_GETG)) return r;
+    if (int r = gemm_m(h, true, false, false, true, 1, 0, g, KDQ_PROF_GETG)) return r;
     LuArgs la{h->d_chains, h->t_M, DIM_NP, h->max_np, h->d_piv, h->d_logdet, h->d_info};
     HIPCHK(launch_lu(h->stream, h->nch, h->max_np, la));
     std::vector<double> lr(h->nch), lm(h->nch);
@@ -543,7 +569,7 @@ int g4_left(kdq_handle h, Opnd *X, Opnd *tmp, int lo, int hi) {
     for (int lt = lo; lt <= hi; ++lt) {
         GemmArgs g = gargs(h, h->t_eK, cur, oth, DIM_NS, DIM_NS, DIM_NS);   // D eK X
         g.rowS = hs_diag(h, lt, +1);
-        if (int r = gemm(h, false, true, 0, 0, g)) return r;
+        if (int r = gemm_m(h, true, false, false, true, 0, 0, g)) return r;
         std::swap(cur, oth);
     }
     if (cur != X) HIPCHK(launch_copy_mat(h->stream, h->nch, h->max_ns, h->max_ns, h->d_chains, cur, X, DIM_NS, DIM_NS, true));
@@ -556,7 +582,7 @@ int g4_right_inv(kdq_handle h, Opnd *Y, Opnd *tmp, int lo, int hi) {
     for (int lt = lo; lt <= hi; ++lt) {
         GemmArgs g = gargs(h, cur, h->t_eKinv, oth, DIM_NS, DIM_NS, DIM_NS);   // Y eK⁻¹ D⁻¹
         g.colS = hs_diag(h, lt, -1);
-        if (int r = gemm(h, true, false, 0, 0, g)) return r;
+        if (int r = gemm_m(h, false, true, true, false, 0, 0, g)) return r;
         std::swap(cur, oth);
     }
     if (cur != Y) HIPCHK(launch_copy_mat(h->stream, h->nch, h->max_ns, h->max_ns, h->d_chains, cur, Y, DIM_NS, DIM_NS, true));
@@ -574,7 +600,7 @@ int g4_compute(kdq_handle h, int t1, int t2, Opnd **out) {
     Opnd *Gi = M[0], *X12 = M[1], *Y21 = M[2], *T = M[3], *T2 = M[4], *G2 = M[5];
     Opnd *UR = UL[h->nn];   // the right chain (N x Np), advanced block by block
     // U_L at τ1: from Pᴴ through slices Nt .. τ1+1 (orthonormalised every BatchSize slices)
-    HIPCHK(launch_real_to_cplx(h->stream, h->nch, h->max_ns, h->max_np, h->d_chains, h->t_Pt, UL[nb], DIM_NS, DIM_NP));
+    HIPCHK(load_P(h, UL[nb]));
     for (int hi = h->nt; hi > t1; hi -= bs) {
         const int lo = std::max(t1 + 1, hi - bs + 1);
         if (int r = prop_L_range(h, UL[nb], UL[nb], lo, hi)) return r;
@@ -587,7 +613,7 @@ int g4_compute(kdq_handle h, int t1, int t2, Opnd **out) {
         if (int r = orthonormalise(h, UL[i])) return r;
     }
     // U_R at τ2: from P through slices 1 .. τ2
-    HIPCHK(launch_real_to_cplx(h->stream, h->nch, h->max_ns, h->max_np, h->d_chains, h->t_Pt, UR, DIM_NS, DIM_NP));
+    HIPCHK(load_P(h, UR));
     for (int lo = 1; lo <= t2; lo += bs) {
         const int hi = std::min(t2, lo + bs - 1);
         if (int r = prop_R_range(h, UR, UR, lo, hi)) return r;
@@ -630,8 +656,8 @@ int g4_compute(kdq_handle h, int t1, int t2, Opnd **out) {
 
 int init_chain(kdq_handle h) {
     const int NN = h->nn;
-    HIPCHK(launch_real_to_cplx(h->stream, h->nch, h->max_ns, h->max_np, h->d_chains, h->t_Pt, h->t_UR[0], DIM_NS, DIM_NP));
-    HIPCHK(launch_real_to_cplx(h->stream, h->nch, h->max_ns, h->max_np, h->d_chains, h->t_Pt, h->t_ULH[NN - 1], DIM_NS, DIM_NP));
+    HIPCHK(load_P(h, h->t_UR[0]));
+    HIPCHK(load_P(h, h->t_ULH[NN - 1]));
     for (int idx = NN - 1; idx >= 1; --idx)
         if (int r = propagate_L(h, h->t_ULH[idx], h->t_ULH[idx - 1], idx)) return r;
     h->idx = 1;
@@ -847,9 +873,9 @@ int measure_into(kdq_handle h, int lt, Opnd *Gm, Opnd *Tm, double *meas) {
             if (int r = wrap_slice(h, Gm, Tm, t, true, KDQ_PROF_MEAS_WRAP)) return r;
     }
     GemmArgs g1 = gargs(h, h->t_hK, Gm, Tm, DIM_NS, DIM_NS, DIM_NS);
-    if (int r = gemm(h, false, true, 0, 0, g1, KDQ_PROF_MEAS_WRAP)) return r;
+    if (int r = gemm_m(h, true, false, false, true, 0, 0, g1, KDQ_PROF_MEAS_WRAP)) return r;
     GemmArgs g2 = gargs(h, Tm, h->t_hKinv, Gm, DIM_NS, DIM_NS, DIM_NS);
-    if (int r = gemm(h, true, false, 0, 0, g2, KDQ_PROF_MEAS_WRAP)) return r;
+    if (int r = gemm_m(h, false, true, true, false, 0, 0, g2, KDQ_PROF_MEAS_WRAP)) return r;
     MeasArgs m{h->d_chains, Gm, meas};
     ProfScope ps(h, KDQ_PROF_MEASURE);
     if (h->kind == KDQ_MODEL_TU) HIPCHK(launch_measure_tu(h->stream, h->nch, m));
@@ -1079,6 +1105,16 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
         if (runs[c].precision == KDQ_PREC_MIXED && (m.kind != KDQ_MODEL_TU || streams))
             return fail(nullptr, KDQ_EARG, "kdq_create: mixed precision is built for the plain t-U sweep only");
         if (runs[c].chain >= (1u << 24)) return fail(nullptr, KDQ_EARG, "kdq_create: chain id must be < 2^24");
+        const int nim = (m.K_im != nullptr) + (m.eK_im != nullptr) + (m.eKinv_im != nullptr) +
+                        (m.halfeK_im != nullptr) + (m.halfeKinv_im != nullptr) + (m.Pt_im != nullptr);
+        if (nim != 0 && nim != 6)
+            return fail(nullptr, KDQ_EARG, "kdq_create: give all six imaginary parts of the model matrices or none");
+        if (nim && m.kind != KDQ_MODEL_TU)
+            return fail(nullptr, KDQ_EARG, "kdq_create: complex model matrices are for the t-U model only");
+        if ((nim != 0) != (models[0].K_im != nullptr))
+            return fail(nullptr, KDQ_EARG, "kdq_create: the chains of a batch must all be real or all complex");
+        if (nim && runs[c].precision != KDQ_PREC_FP64)
+            return fail(nullptr, KDQ_EARG, "kdq_create: complex model matrices need FP64 precision");
     }
     kdq_handle h = new kdq_handle_s();
     h->nch = nchains;
@@ -1088,6 +1124,7 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
     h->delay = runs[0].delay;
     h->kind = models[0].kind;
     h->mixed = runs[0].precision == KDQ_PREC_MIXED;
+    h->cK = models[0].K_im != nullptr;
     h->defer = false;   // per-block flush by default (faster on the measured workloads)
     h->nobs = h->kind == KDQ_MODEL_TU ? 6 : (h->kind == KDQ_MODEL_TV ? 10 : 7);
     h->md.assign(models, models + nchains);
@@ -1120,12 +1157,13 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
         h->max_np = std::max(h->max_np, np);
         const size_t nn = (size_t)n * n, nP = (size_t)n * np, pp = (size_t)np * np;
         bool ok = true;
-        ok &= dalloc(C, &C.eK, nn) == hipSuccess;
-        ok &= dalloc(C, &C.eKinv, nn) == hipSuccess;
-        ok &= dalloc(C, &C.hK, nn) == hipSuccess;
-        ok &= dalloc(C, &C.hKinv, nn) == hipSuccess;
-        ok &= dalloc(C, &C.K, nn) == hipSuccess;
-        ok &= dalloc(C, &C.Pt, nP) == hipSuccess;
+        const size_t planes = h->cK ? 2 : 1;   // complex model matrices: real plane, then imaginary
+        ok &= dalloc(C, &C.eK, planes * nn) == hipSuccess;
+        ok &= dalloc(C, &C.eKinv, planes * nn) == hipSuccess;
+        ok &= dalloc(C, &C.hK, planes * nn) == hipSuccess;
+        ok &= dalloc(C, &C.hKinv, planes * nn) == hipSuccess;
+        ok &= dalloc(C, &C.K, planes * nn) == hipSuccess;
+        ok &= dalloc(C, &C.Pt, planes * nP) == hipSuccess;
         ok &= dalloc(C, &C.alpha, (size_t)m.nt) == hipSuccess;
         ok &= dalloc(C, &C.G[0], 2 * nn) == hipSuccess;
         ok &= dalloc(C, &C.G[1], 2 * nn) == hipSuccess;
@@ -1231,6 +1269,14 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
         hipMemcpy(C.hKinv, m.halfeKinv, nn * 8, hipMemcpyHostToDevice);
         hipMemcpy(C.K, m.K, nn * 8, hipMemcpyHostToDevice);
         hipMemcpy(C.Pt, m.Pt, nP * 8, hipMemcpyHostToDevice);
+        if (h->cK) {
+            hipMemcpy(C.eK + nn, m.eK_im, nn * 8, hipMemcpyHostToDevice);
+            hipMemcpy(C.eKinv + nn, m.eKinv_im, nn * 8, hipMemcpyHostToDevice);
+            hipMemcpy(C.hK + nn, m.halfeK_im, nn * 8, hipMemcpyHostToDevice);
+            hipMemcpy(C.hKinv + nn, m.halfeKinv_im, nn * 8, hipMemcpyHostToDevice);
+            hipMemcpy(C.K + nn, m.K_im, nn * 8, hipMemcpyHostToDevice);
+            hipMemcpy(C.Pt + nP, m.Pt_im, nP * 8, hipMemcpyHostToDevice);
+        }
         if (m.alpha) hipMemcpy(C.alpha, m.alpha, (size_t)m.nt * 8, hipMemcpyHostToDevice);
         else {
             std::vector<double> ones((size_t)m.nt, 1.0);
@@ -1282,6 +1328,7 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
         d.ht = m.ht;
         d.lattice = m.lattice; d.L1 = m.site[0]; d.L2 = m.site[1];
         d.K = C.K;
+        d.Kim = h->cK ? C.K + nn : nullptr;
         d.seed = runs[c].seed;
         d.chain_id = runs[c].chain;
         d.stream = streams ? streams[c] : 0;
@@ -1296,9 +1343,15 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
         d.a = m.kind == KDQ_MODEL_TUV ? m.a : 0.0;
         d.U = m.U; d.V = m.V;
 
-        v_eK.push_back(real(C.eK, n)); v_eKinv.push_back(real(C.eKinv, n));
-        v_hK.push_back(real(C.hK, n)); v_hKinv.push_back(real(C.hKinv, n));
-        v_Pt.push_back(real(C.Pt, n));
+        if (h->cK) {
+            v_eK.push_back(cplx(C.eK, n, nn)); v_eKinv.push_back(cplx(C.eKinv, n, nn));
+            v_hK.push_back(cplx(C.hK, n, nn)); v_hKinv.push_back(cplx(C.hKinv, n, nn));
+            v_Pt.push_back(cplx(C.Pt, n, nP));
+        } else {
+            v_eK.push_back(real(C.eK, n)); v_eKinv.push_back(real(C.eKinv, n));
+            v_hK.push_back(real(C.hK, n)); v_hKinv.push_back(real(C.hKinv, n));
+            v_Pt.push_back(real(C.Pt, n));
+        }
         v_G0.push_back(cplx(C.G[0], n, nn)); v_G1.push_back(cplx(C.G[1], n, nn));
         v_T.push_back(cplx(C.T, n, nn)); v_Gm.push_back(cplx(C.Gm, n, nn)); v_Tm.push_back(cplx(C.Tm, n, nn));
         v_P0.push_back(cplx(C.P0, n, nP)); v_P1.push_back(cplx(C.P1, n, nP)); v_P2.push_back(cplx(C.P2, n, nP));
@@ -1454,10 +1507,60 @@ int kdq_initial_s(kdq_handle h) {
     return KDQ_OK;
 }
 
+// A chain with replayed draws must hold the sweeps [first, first + count) of the call.
+static int check_replay(kdq_handle h, uint32_t first, uint32_t count, const char *who) {
+    for (int c = 0; c < (int)h->rp_buf.size(); ++c) {
+        const ChainDev &d = h->cd_host[c];
+        if (!d.rp_u || count == 0) continue;
+        const uint32_t s = first - d.rp_sweep0;
+        if (s >= d.rp_nsweeps || count > d.rp_nsweeps - s)
+            return fail(h, KDQ_EARG, std::string(who) + ": the replayed draws of chain " + std::to_string(c) +
+                                         " cover sweeps " + std::to_string(d.rp_sweep0) + ".." +
+                                         std::to_string(d.rp_sweep0 + d.rp_nsweeps - 1) + ", the call needs " +
+                                         std::to_string(first) + ".." + std::to_string(first + count - 1));
+    }
+    return KDQ_OK;
+}
+
 int kdq_phy_update(kdq_handle h, int32_t sweeps, int32_t record, double *rows) {
     if (!h || sweeps < 0) return fail(h, KDQ_EARG, "kdq_phy_update: bad argument");
     if (record && !rows) return fail(h, KDQ_EARG, "kdq_phy_update: rows must not be NULL when record != 0");
+    if (int r = check_replay(h, h->sweep, (uint32_t)sweeps, "kdq_phy_update")) return r;
     return phy_update(h, sweeps, record, rows);
+}
+
+int kdq_set_replay(kdq_handle h, int32_t c, const uint8_t *choice, const double *u, int64_t nsweeps) {
+    if (!h || c < 0 || c >= h->nch || nsweeps < 0 || (nsweeps > 0 && (!choice || !u)))
+        return fail(h, KDQ_EARG, "kdq_set_replay: bad argument");
+    if (h->scee) return fail(h, KDQ_EARG, "kdq_set_replay: sweep handles only (not swap-replica handles)");
+    if (nsweeps > 0xFFFFFFF) return fail(h, KDQ_EARG, "kdq_set_replay: too many sweeps");
+    ChainDev &d = h->cd_host[c];
+    const size_t count = (size_t)nsweeps * 2 * (size_t)h->nt * (size_t)d.n_elem;
+    for (size_t q = 0; q < count; ++q)
+        if (choice[q] > 2) return fail(h, KDQ_EARG, "kdq_set_replay: choice values must be 0, 1 or 2");
+    if (h->rp_buf.empty()) h->rp_buf.assign(h->nch, nullptr);
+    HIPCHK(hipStreamSynchronize(h->stream));   // no kernel may still read the old buffer
+    if (h->rp_buf[c]) {
+        HIPCHK(hipFree(h->rp_buf[c]));
+        h->rp_buf[c] = nullptr;
+    }
+    d.rp_choice = nullptr;
+    d.rp_u = nullptr;
+    d.rp_sweep0 = d.rp_nsweeps = 0;
+    if (count) {
+        const size_t bytes_u = sizeof(double) * count, bytes = bytes_u + count;
+        void *q = nullptr;
+        if (hipMalloc(&q, bytes) != hipSuccess) return fail(h, KDQ_EHIP, "kdq_set_replay: device allocation failed");
+        h->rp_buf[c] = q;
+        HIPCHK(hipMemcpy(q, u, bytes_u, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy((char *)q + bytes_u, choice, count, hipMemcpyHostToDevice));
+        d.rp_u = (const double *)q;
+        d.rp_choice = (const uint8_t *)q + bytes_u;
+        d.rp_sweep0 = h->sweep;
+        d.rp_nsweeps = (uint32_t)nsweeps;
+    }
+    HIPCHK(hipMemcpy(h->d_chains + c, &d, sizeof(ChainDev), hipMemcpyHostToDevice));
+    return KDQ_OK;
 }
 
 int kdq_get_G(kdq_handle h, int32_t c, double *G) {
@@ -1505,6 +1608,7 @@ int kdq_wrap(kdq_handle h, int32_t lt, int32_t forward) {
 
 int kdq_update_layer(kdq_handle h, int32_t lt, int32_t dir, uint32_t sweep) {
     if (!h || lt < 1 || lt > h->nt || (dir != 0 && dir != 1)) return fail(h, KDQ_EARG, "kdq_update_layer: bad argument");
+    if (int r = check_replay(h, sweep, 1, "kdq_update_layer")) return r;
     if (h->kind == KDQ_MODEL_TV) {
         for (int k = 0; k < 3; ++k)   // forward pass visits families 3,2,1, backward 1,2,3
             if (int r = update_family_r2(h, lt, dir, sweep, dir == 0 ? 2 - k : k)) return r;
@@ -1898,10 +2002,10 @@ int scee_blocks(kdq_handle h, int idx) {
         HIPCHK(launch_diag(h->stream, h->nch, h->max_ns, h->d_chains, BMi, DIM_NS, 0.0, 1));
         for (int lt = lo; lt <= hi; ++lt) {
             GemmArgs g = gargs(h, h->t_eK, BM, h->t_T, DIM_NS, DIM_NS, DIM_NS);
-            if (int r = gemm(h, false, true, 0, 0, g, KDQ_PROF_CHAIN)) return r;
+            if (int r = gemm_m(h, true, false, false, true, 0, 0, g, KDQ_PROF_CHAIN)) return r;
             HIPCHK(launch_copy_mat(h->stream, h->nch, h->max_ns, h->max_ns, h->d_chains, h->t_T, BM, DIM_NS, DIM_NS, true));
             GemmArgs gi = gargs(h, BMi, h->t_eKinv, h->t_Tm, DIM_NS, DIM_NS, DIM_NS);
-            if (int r = gemm(h, true, false, 0, 0, gi, KDQ_PROF_CHAIN)) return r;
+            if (int r = gemm_m(h, false, true, true, false, 0, 0, gi, KDQ_PROF_CHAIN)) return r;
             HIPCHK(launch_copy_mat(h->stream, h->nch, h->max_ns, h->max_ns, h->d_chains, h->t_Tm, BMi, DIM_NS, DIM_NS, true));
             for (int j = 2; j >= 0; --j) {
                 if (int r = bondrot(h, BM, j, lt, +1, 1, DIM_NS, KDQ_PROF_CHAIN)) return r;
@@ -1918,11 +2022,11 @@ int scee_blocks(kdq_handle h, int idx) {
         Opnd *out = last_even ? BM : h->t_T;
         GemmArgs g = gargs(h, h->t_eK, src, out, DIM_NS, DIM_NS, DIM_NS);   // D eK X
         g.rowS = hs_diag(h, lt, +1);
-        if (int r = gemm(h, false, true, 0, 0, g, KDQ_PROF_CHAIN)) return r;
+        if (int r = gemm_m(h, true, false, false, true, 0, 0, g, KDQ_PROF_CHAIN)) return r;
         Opnd *outi = last_even ? BMi : h->t_Tm;
         GemmArgs gi = gargs(h, srci, h->t_eKinv, outi, DIM_NS, DIM_NS, DIM_NS);   // X eK⁻¹ D⁻¹
         gi.colS = hs_diag(h, lt, -1);
-        if (int r = gemm(h, true, false, 0, 0, gi, KDQ_PROF_CHAIN)) return r;
+        if (int r = gemm_m(h, false, true, true, false, 0, 0, gi, KDQ_PROF_CHAIN)) return r;
         src = out;
         srci = outi;
     }
@@ -2125,10 +2229,10 @@ int scee_layer_r2(kdq_handle h, int lt, int dir, uint32_t sweep, int family) {
 int scee_wrapK_tv(kdq_handle h, bool forward) {
     if (int r = wrap_K(h, h->t_G[h->cur], h->t_T, forward, KDQ_PROF_WRAP)) return r;
     GemmArgs gl = gargs(h, forward ? h->t_eK : h->t_eKinv, xcur(h, SX_GT0), xspare(h, SX_GT0), DIM_NS, DIM_NS, DIM_NS);
-    if (int r = gemm(h, false, true, 0, 0, gl, KDQ_PROF_WRAP)) return r;
+    if (int r = gemm_m(h, true, false, false, true, 0, 0, gl, KDQ_PROF_WRAP)) return r;
     xflip(h, SX_GT0);
     GemmArgs gr = gargs(h, xcur(h, SX_G0T), forward ? h->t_eKinv : h->t_eK, xspare(h, SX_G0T), DIM_NS, DIM_NS, DIM_NS);
-    if (int r = gemm(h, true, false, 0, 0, gr, KDQ_PROF_WRAP)) return r;
+    if (int r = gemm_m(h, false, true, true, false, 0, 0, gr, KDQ_PROF_WRAP)) return r;
     xflip(h, SX_G0T);
     return KDQ_OK;
 }
@@ -2156,10 +2260,10 @@ int scee_wraps(kdq_handle h, int lt, bool forward) {
         gr = gargs(h, xcur(h, SX_G0T), h->t_eK, xspare(h, SX_G0T), DIM_NS, DIM_NS, DIM_NS);
         gr.inS = hs_diag(h, lt, +1);
     }
-    if (int r = gemm(h, false, true, 0, 0, gl, KDQ_PROF_WRAP)) return r;
+    if (int r = gemm_m(h, true, false, false, true, 0, 0, gl, KDQ_PROF_WRAP)) return r;
     xflip(h, SX_GT0);
     if (int r = wrap_B(h, h->t_G[h->cur], h->t_T, lt, forward)) return r;
-    if (int r = gemm(h, true, false, 0, 0, gr, KDQ_PROF_WRAP)) return r;
+    if (int r = gemm_m(h, false, true, true, false, 0, 0, gr, KDQ_PROF_WRAP)) return r;
     xflip(h, SX_G0T);
     return KDQ_OK;
 }
@@ -2169,8 +2273,8 @@ int scee_init(kdq_handle h) {
     const int NN = h->nn;
     for (int idx = 1; idx <= NN - 1; ++idx)
         if (int r = scee_blocks(h, idx)) return r;
-    HIPCHK(launch_real_to_cplx(h->stream, h->nch, h->max_ns, h->max_np, h->d_chains, h->t_Pt, h->t_UR[0], DIM_NS, DIM_NP));
-    HIPCHK(launch_real_to_cplx(h->stream, h->nch, h->max_ns, h->max_np, h->d_chains, h->t_Pt, h->t_ULH[NN - 1], DIM_NS, DIM_NP));
+    HIPCHK(load_P(h, h->t_UR[0]));
+    HIPCHK(load_P(h, h->t_ULH[NN - 1]));
     for (int i = 1; i <= NN - 1; ++i) {
         if (int r = scee_update_BL(h, NN - i)) return r;
         if (int r = scee_update_BR(h, i + 1)) return r;

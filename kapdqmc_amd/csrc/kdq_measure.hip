@@ -25,6 +25,7 @@ __global__ __launch_bounds__(NT) void k_measure_tu(MeasArgs a) {
     for (int q = tid; q < n * n; q += NT) {
         const int i = q % n, j = q / n;
         ek += ch.K[q] * G.re[i + j * ld];
+        if (ch.Kim) ek -= ch.Kim[q] * G.im[i + j * ld];   // Re Σ K∘G with complex hopping
     }
     for (int i = tid; i < n; i += NT) {
         const double gr = G.re[i + i * ld], gi = G.im[i + i * ld];

@@ -95,7 +95,7 @@ __global__ __launch_bounds__(256) void k_decide_r2(DecideR2Args a) {
         int chv;
         double u;
         const uint32_t sweep = a.sweep_dev ? *a.sweep_dev : a.sweep;
-        proposal_draw(ch.seed, ch.chain_id, 0, sweep, tag, (uint32_t)(e0 + tid), chv, u);
+        sweep_draw(ch, sweep, lt, a.dir, (uint32_t)(e0 + tid), chv, u);
         chs[tid] = chv;
         us[tid] = u;
         sv[tid] = ch.s[(size_t)(e0 + tid) + (size_t)(lt - 1) * ch.n_elem];

@@ -213,7 +213,7 @@ __global__ __launch_bounds__(KBT >= 32 ? 1024 : KBT * KBT) void k_decide_tu(Deci
             int chv;
             double u;
             const uint32_t sweep = a.sweep_dev ? *a.sweep_dev : a.sweep;
-            proposal_draw(ch.seed, ch.chain_id, 0, sweep, tag, (uint32_t)(s0 + tid), chv, u);
+            sweep_draw(ch, sweep, lt, a.dir, (uint32_t)(s0 + tid), chv, u);
             const int si = ch.s[(size_t)(s0 + tid) + (size_t)(lt - 1) * ch.n_elem];
             const int sx = chv + 1 + ((chv + 1) >= si ? 1 : 0);
             const double x = ch.alpha[lt - 1] * (ch.eta[sx - 1] - ch.eta[si - 1]);

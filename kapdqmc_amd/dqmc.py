@@ -34,6 +34,45 @@ def field_shape(m):
     return (m.Ns * m.nnidx.shape[1], m.Nt)
 
 
+def elements_per_slice(m) -> int:
+    """Field elements of one slice: t-U Ns, t-V 3 Ns/2, t-U-V Ns z."""
+    return int(np.prod(field_shape(m)[:-1]))
+
+
+def replay_from_visiting_order(m, choice, u):
+    """Arranges proposal draws given in the reference's visiting order into the replay layout
+    of kdq_set_replay, [sweeps, 2, Nt, n_elem].
+
+    The visiting order of one sweep (one `sampler` draw and one `rand(rng)` per proposal, in
+    that order): the forward pass lt = 1..Nt, then the backward pass lt = Nt..1
+    (src/tU/phy_update.jl:53,101); within a slice the field elements in order, except t-V,
+    whose forward pass visits the bond families 3, 2, 1 and its backward pass 1, 2, 3, each
+    over its bonds i = 1..Ns/2 (src/tV/phy_update.jl:64,131). `choice` is the index of the
+    sampled value among the other three in increasing order (0, 1, 2)."""
+    E, nt = elements_per_slice(m), m.Nt
+    c = np.asarray(choice, dtype=np.uint8).ravel()
+    uu = np.asarray(u, dtype=np.float64).ravel()
+    per = 2 * nt * E
+    if c.size != uu.size or c.size % per:
+        raise ValueError(f"draws must hold whole sweeps of {per} proposals")
+    ns = c.size // per
+    order = np.arange(E)
+    if m.kind == "tV":
+        nb = m.Ns // 2
+        fwd = np.concatenate([np.arange(nb) + j * nb for j in (2, 1, 0)])
+        bwd = np.concatenate([np.arange(nb) + j * nb for j in (0, 1, 2)])
+    else:
+        fwd = bwd = order
+    oc = np.zeros((ns, 2, nt, E), dtype=np.uint8)
+    ou = np.zeros((ns, 2, nt, E))
+    vc, vu = c.reshape(ns, 2, nt, E), uu.reshape(ns, 2, nt, E)
+    for d, perm in ((0, fwd), (1, bwd)):
+        lts = np.arange(nt) if d == 0 else np.arange(nt)[::-1]     # visiting position -> lt - 1
+        oc[:, d, lts[:, None], perm[None, :]] = vc[:, d]
+        ou[:, d, lts[:, None], perm[None, :]] = vu[:, d]
+    return oc, ou
+
+
 def _descriptors(models, seeds, chains, delay, sweep0, precision=L.PREC_FP64):
     """kdq_model_desc / kdq_run_desc arrays for a list of model structs (plus the numpy
     arrays they point into, which must outlive the call)."""
@@ -69,6 +108,14 @@ def _descriptors(models, seeds, chains, delay, sweep0, precision=L.PREC_FP64):
         d.ht = m.Ht
         for k, v in arrs.items():
             setattr(d, k, _dp(v))
+        mats = dict(K=m.K, eK=m.eK, eKinv=m.eKinv, halfeK=m.HalfeK, halfeKinv=m.HalfeKinv, Pt=m.Pt)
+        if any(np.iscomplexobj(v) and np.any(np.imag(v) != 0) for v in mats.values()):
+            if kind != "tU":
+                raise ValueError("complex model matrices are supported for the t-U model only")
+            for k, v in mats.items():           # complex hopping (flux != 0): imaginary parts too
+                im = f(np.imag(v))
+                keep.append(im)
+                setattr(d, k + "_im", _dp(im))
         r = runs[c]
         r.seed, r.chain, r.sweep0, r.precision, r.delay = seeds[c], chains[c], sweep0, precision, delay
     return descs, runs, keep
@@ -174,6 +221,18 @@ class DQMCBatch:
         out = np.zeros((self.n, self.nobs))
         self._chk(self.L.kdq_measure(self.h, lt, _dp(out)))
         return out
+
+    def set_replay(self, c: int, choice, u):
+        """Replayed draws for chain c (kdq_set_replay): arrays [sweeps, 2, Nt, n_elem] (see
+        replay_from_visiting_order), used by the next `sweeps` sweeps instead of the Philox
+        stream; empty arrays return the chain to Philox."""
+        ch = np.ascontiguousarray(choice, dtype=np.uint8).ravel()
+        uu = np.ascontiguousarray(u, dtype=np.float64).ravel()
+        per = 2 * self.models[c].Nt * elements_per_slice(self.models[c])
+        if ch.size != uu.size or ch.size % per:
+            raise ValueError("replay arrays must hold whole sweeps")
+        self._chk(self.L.kdq_set_replay(self.h, c, ch.ctypes.data_as(C.POINTER(C.c_uint8)), _dp(uu),
+                                        ch.size // per))
 
     def measure_noalloc(self, lt: int) -> np.ndarray:
         """measure() through kdq_measure_noalloc: the working matrices live in the library's

@@ -57,21 +57,22 @@ def nn2idx(lattice: str, site, idx: int):
 
 
 def nnK_Matrix(lattice: str, site, t=(1.0, 1.0, 1.0), flux: float = 0.0) -> np.ndarray:
-    """Nearest-neighbour hopping matrix (Geometry.jl:178-207). Column-major-equivalent
-    dense array (numpy order is irrelevant for a symmetric matrix)."""
+    """Nearest-neighbour hopping matrix (Geometry.jl:178-207): real for flux = 0; complex
+    otherwise, with the square lattices' bonds carrying e^{±i flux/4} (the first two
+    neighbours e^{+i flux/4}, the last two e^{-i flux/4}) and the honeycomb's the real t."""
     ns = 2 * site[0] * site[1]
-    if flux != 0.0:
-        raise NotImplementedError("flux != 0 (complex hopping) is not supported by the GPU path")
-    K = np.zeros((ns, ns))
+    flux1, flux2 = np.exp(1j * flux / 4), np.exp(-1j * flux / 4)
+    K = np.zeros((ns, ns)) if flux == 0 else np.zeros((ns, ns), dtype=np.complex128)
     for i in range(1, ns + 1):
         nn = nn2idx(lattice, site, i)
         if lattice.startswith("SQUARE"):
-            for j in nn:
-                K[i - 1, j - 1] = 1.0
+            for j, nj in enumerate(nn):
+                v = flux1 if j < 2 else flux2
+                K[i - 1, nj - 1] = v if flux != 0 else 1.0
         else:
             for j, nj in enumerate(nn):
                 K[i - 1, nj - 1] = t[j]
-    if np.linalg.norm(K - K.T) >= 1e-8:
+    if np.linalg.norm(K - K.conj().T) >= 1e-8:
         raise AssertionError("K is not Hermitian!")
     return K
 

@@ -49,8 +49,9 @@ def _quench_profile(H1, H2, Theta_relax, Theta_quench, dt, Nt):
 
 
 def _exp_sym(A: np.ndarray, dt: float):
+    """exp(∓Δτ A / 2), exp(∓Δτ A) of a real symmetric or complex Hermitian A (syevd!)."""
     E, V = np.linalg.eigh(A)
-    f = lambda c: (V * np.exp(c * E)) @ V.T
+    f = lambda c: (V * np.exp(c * E)) @ V.conj().T
     return f(-dt / 2), f(-dt), f(dt / 2), f(dt)
 
 
@@ -86,7 +87,9 @@ def tU_Hubbard_Para(*, Ht, Hu1, Hu2, Δt, Θrelax, Θquench, Lattice: str, site,
     """t-U Hubbard model constants (src/tU/model.jl:39-96).
 
     The trial state is the lowest Ns/2 eigenvectors of K + 1e-5·diag(-1,+1,...) (the
-    reference's staggered potential, model.jl:68-76), or the "V" product state.
+    reference's staggered potential, model.jl:68-76), or the "V" product state. flux != 0
+    gives complex Hermitian K, eK, eKinv, HalfeK, HalfeKinv and a complex Pt (the reference
+    stores them ComplexF64 in any case, model.jl:24-34).
     """
     site = list(site)
     Nt = round(2 * (Θrelax + Θquench) / Δt)
@@ -97,7 +100,7 @@ def tU_Hubbard_Para(*, Ht, Hu1, Hu2, Δt, Θrelax, Θquench, Lattice: str, site,
     HalfeK, eK, HalfeKinv, eKinv = _exp_sym(Ht * K, Δt)
     if np.linalg.norm(eK @ eKinv - np.eye(Ns)) >= 1e-10:
         raise AssertionError("eK*eKinv does not equal identity!")
-    Pt = np.zeros((Ns, Ns // 2))
+    Pt = np.zeros((Ns, Ns // 2), dtype=K.dtype)   # ComplexF64 with complex hopping (model.jl:68)
     if Initial == "H0":
         KK = K + 1e-5 * np.diag(np.tile([-1.0, 1.0], Ns // 2))
         _, V = np.linalg.eigh(KK)

@@ -65,6 +65,7 @@ def _declare(L):
                              C.c_double, C.c_double, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
                              C.c_uint64, C.c_uint32]
     L.orc_tu_new.restype = C.c_void_p
+    L.orc_tu_set_imag.argtypes = [C.c_void_p] + [_dp] * 6
     L.orc_tu_free.argtypes = [C.c_void_p]
     L.orc_tu_initial_s.argtypes = [C.c_void_p]
     L.orc_tu_init_chain.argtypes = [C.c_void_p]
@@ -72,6 +73,8 @@ def _declare(L):
     L.orc_tu_Gtau.argtypes = [C.c_void_p, C.c_int, _dp]
     L.orc_tu_WrapKV.argtypes = [C.c_void_p, _dp, _dp, C.c_int, C.c_char]
     L.orc_tu_update_layer.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    for nm in ("orc_tu_set_replay", "orc_tv_set_replay", "orc_tuv_set_replay"):
+        getattr(L, nm).argtypes = [C.c_void_p, C.POINTER(C.c_uint8), C.POINTER(C.c_double), C.c_uint32]
     L.orc_tu_measure.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
     L.orc_tu_BM_F.argtypes = [C.c_void_p, C.c_int, _dp]
     L.orc_tu_forward_slices.argtypes = [C.c_void_p, C.c_int]
@@ -289,6 +292,14 @@ def tu_G4(model, s, tau1, tau2):
 
 
 # -------------------------------------------------------------------------- t-U
+def _n_elem(model) -> int:
+    """Field elements per slice: t-U Ns, t-V 3 Ns/2, t-U-V Ns z."""
+    nn = getattr(model, "nnidx", None)
+    if nn is None:
+        return model.Ns
+    return int(np.prod(np.asarray(nn).shape[:-1]))
+
+
 class OracleTU:
     """Oracle state for one t-U chain built from a host model (kapdqmc_amd.models)."""
 
@@ -296,15 +307,21 @@ class OracleTU:
         self.model = model
         L = lib()
         f = lambda a: _fortran(a, np.float64)
+        r = lambda a: f(np.real(a))
         self._keep = [f(model.α), f(model.η), f(model.γ),
-                      f(model.K), f(model.eK), f(model.eKinv), f(model.HalfeK), f(model.HalfeKinv),
-                      f(model.Pt), np.ascontiguousarray(model.nodes, dtype=np.int64)]
+                      r(model.K), r(model.eK), r(model.eKinv), r(model.HalfeK), r(model.HalfeKinv),
+                      r(model.Pt), np.ascontiguousarray(model.nodes, dtype=np.int64)]
         a, eta, gam, K, eK, eKi, hK, hKi, Pt, nodes = self._keep
         site = (C.c_int * 2)(*model.site)
         self.h = L.orc_tu_new(model.Ns, model.Nt, model.BatchSize, len(nodes),
                               nodes.ctypes.data_as(C.POINTER(C.c_int64)), LATTICE_CODE[model.Lattice],
                               site, model.Ht, model.Δt, _p(a), _p(eta), _p(gam), _p(K), _p(eK),
                               _p(eKi), _p(hK), _p(hKi), _p(Pt), seed, chain)
+        mats = (model.K, model.eK, model.eKinv, model.HalfeK, model.HalfeKinv, model.Pt)
+        if any(np.iscomplexobj(x) and np.any(np.imag(x) != 0) for x in mats):   # complex hopping
+            ims = [f(np.imag(x)) for x in mats]
+            self._keep += ims
+            L.orc_tu_set_imag(self.h, *[_p(x) for x in ims])
         self.ns, self.nt = model.Ns, model.Nt
 
     def __del__(self):
@@ -329,6 +346,19 @@ class OracleTU:
     @property
     def _st(self):
         return C.cast(self.h, C.POINTER(self._S)).contents
+
+    def set_replay(self, choice, u):
+        """Replayed proposal draws (uint8 choice in {0,1,2}, float64 u) shaped
+        [sweeps, 2, Nt, n_elem] (C order), used by the next sweeps instead of Philox; the
+        layout of kdq_set_replay."""
+        ch = np.ascontiguousarray(choice, dtype=np.uint8).ravel()
+        uu = np.ascontiguousarray(u, dtype=np.float64).ravel()
+        per = 2 * self.model.Nt * _n_elem(self.model)
+        if ch.size != uu.size or ch.size % per:
+            raise ValueError("replay arrays must hold whole sweeps")
+        self._replay = (ch, uu)
+        lib().orc_tu_set_replay(self.h, ch.ctypes.data_as(C.POINTER(C.c_uint8)),
+                            uu.ctypes.data_as(C.POINTER(C.c_double)), ch.size // per)
 
     def initial_s(self):
         lib().orc_tu_initial_s(self.h)
@@ -455,6 +485,19 @@ class OracleTV:
     def _st(self):
         return C.cast(self.h, C.POINTER(self._S)).contents
 
+    def set_replay(self, choice, u):
+        """Replayed proposal draws (uint8 choice in {0,1,2}, float64 u) shaped
+        [sweeps, 2, Nt, n_elem] (C order), used by the next sweeps instead of Philox; the
+        layout of kdq_set_replay."""
+        ch = np.ascontiguousarray(choice, dtype=np.uint8).ravel()
+        uu = np.ascontiguousarray(u, dtype=np.float64).ravel()
+        per = 2 * self.model.Nt * _n_elem(self.model)
+        if ch.size != uu.size or ch.size % per:
+            raise ValueError("replay arrays must hold whole sweeps")
+        self._replay = (ch, uu)
+        lib().orc_tv_set_replay(self.h, ch.ctypes.data_as(C.POINTER(C.c_uint8)),
+                            uu.ctypes.data_as(C.POINTER(C.c_double)), ch.size // per)
+
     def initial_s(self):
         lib().orc_tv_initial_s(self.h)
 
@@ -558,6 +601,19 @@ class OracleTUV:
     @property
     def _st(self):
         return C.cast(self.h, C.POINTER(self._S)).contents
+
+    def set_replay(self, choice, u):
+        """Replayed proposal draws (uint8 choice in {0,1,2}, float64 u) shaped
+        [sweeps, 2, Nt, n_elem] (C order), used by the next sweeps instead of Philox; the
+        layout of kdq_set_replay."""
+        ch = np.ascontiguousarray(choice, dtype=np.uint8).ravel()
+        uu = np.ascontiguousarray(u, dtype=np.float64).ravel()
+        per = 2 * self.model.Nt * _n_elem(self.model)
+        if ch.size != uu.size or ch.size % per:
+            raise ValueError("replay arrays must hold whole sweeps")
+        self._replay = (ch, uu)
+        lib().orc_tuv_set_replay(self.h, ch.ctypes.data_as(C.POINTER(C.c_uint8)),
+                            uu.ctypes.data_as(C.POINTER(C.c_double)), ch.size // per)
 
     def initial_s(self):
         lib().orc_tuv_initial_s(self.h)

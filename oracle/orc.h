@@ -64,6 +64,18 @@ int orc_zinv(int n, zdouble *A, int lda);                               /* getrf
 void orc_get_G(int ns, int np, const zdouble *BL, int ldbl, const zdouble *BR, int ldbr,
                zdouble *G, double *logabsdet);
 
+/* Replayed proposal draws: entry ((s * 2 + dir) * nt + lt - 1) * n_elem + e is the draw of
+ * field element e at slice lt of pass dir (0 forward, 1 backward) of sweep sweep0 + s; the
+ * caller fills it in the reference's visiting order (UpdatePhyLayer!: sampler, then uniform).
+ * u == NULL: the Philox stream. */
+typedef struct orc_replay {
+    const uint8_t *choice;
+    const double *u;
+    uint32_t sweep0, nsweeps;
+} orc_replay;
+void orc_sweep_draw(const orc_replay *r, int nt, int n_elem, uint64_t seed, uint32_t chain, uint32_t sweep,
+                    int lt, int dir, uint32_t elem, int *choice, double *u);
+
 /* ---------------- t-U model (src/tU) ------------------------------------------------- */
 typedef struct orc_tu {
     int ns, np, nt, bs, nn_nodes, lattice, site[2];
@@ -80,6 +92,7 @@ typedef struct orc_tu {
     uint32_t chain, sweep;  /* RNG stream position */
     long long n_accept, n_propose;
     long long n_ties;       /* decisions with |u - p| < ORC_NEAR_TIE */
+    orc_replay rp;
 } orc_tu;
 
 orc_tu *orc_tu_new(int ns, int nt, int bs, int nn_nodes, const int64_t *nodes, int lattice,
@@ -87,12 +100,15 @@ orc_tu *orc_tu_new(int ns, int nt, int bs, int nn_nodes, const int64_t *nodes, i
                    const double *eta, const double *gamma, const double *K, const double *eK,
                    const double *eKinv, const double *HalfeK, const double *HalfeKinv,
                    const double *Pt, uint64_t seed, uint32_t chain);
+void orc_tu_set_imag(orc_tu *m, const double *K_im, const double *eK_im, const double *eKinv_im,
+                     const double *HalfeK_im, const double *HalfeKinv_im, const double *Pt_im);
 void orc_tu_free(orc_tu *m);
 void orc_tu_initial_s(orc_tu *m);
 void orc_tu_BM_F(orc_tu *m, int idx, zdouble *BM);
 void orc_tu_BMinv_F(orc_tu *m, int idx, zdouble *BM);
 void orc_tu_WrapKV(orc_tu *m, zdouble *G, const zdouble *D_in, int forward, char LR);
 void orc_tu_update_layer(orc_tu *m, int lt, int dir);
+void orc_tu_set_replay(orc_tu *m, const uint8_t *choice, const double *u, uint32_t nsweeps);
 void orc_tu_measure(orc_tu *m, int lt, const zdouble *G, double out[6]);
 void orc_tu_init_chain(orc_tu *m);
 void orc_tu_phy_update(orc_tu *m, int sweeps, int record, double *rows);
@@ -115,6 +131,7 @@ typedef struct orc_tv {
     uint32_t chain, sweep;
     long long n_accept, n_propose, n_negative;
     long long n_ties;
+    orc_replay rp;
 } orc_tv;
 
 orc_tv *orc_tv_new(int ns, int nt, int bs, int nn_nodes, const int64_t *nodes, int lattice,
@@ -126,6 +143,7 @@ void orc_tv_free(orc_tv *m);
 void orc_tv_initial_s(orc_tv *m);
 void orc_tv_BM_F(orc_tv *m, int idx, double *BM);
 void orc_tv_update_family(orc_tv *m, int lt, int dir, int j);
+void orc_tv_set_replay(orc_tv *m, const uint8_t *choice, const double *u, uint32_t nsweeps);
 void orc_tv_measure(orc_tv *m, int lt, const double *G, double out[10]);
 void orc_tv_init_chain(orc_tv *m);
 void orc_tv_phy_update(orc_tv *m, int sweeps, int record, double *rows);
@@ -147,6 +165,7 @@ typedef struct orc_tuv {
     uint32_t chain, sweep;
     long long n_accept, n_propose;
     long long n_ties;
+    orc_replay rp;
 } orc_tuv;
 
 orc_tuv *orc_tuv_new(int ns, int nt, int bs, int nn_nodes, const int64_t *nodes, int lattice,
@@ -158,6 +177,7 @@ void orc_tuv_free(orc_tuv *m);
 void orc_tuv_initial_s(orc_tuv *m);
 void orc_tuv_BM_F(orc_tuv *m, int idx, zdouble *BM);
 void orc_tuv_update_layer(orc_tuv *m, int lt, int dir);
+void orc_tuv_set_replay(orc_tuv *m, const uint8_t *choice, const double *u, uint32_t nsweeps);
 void orc_tuv_measure(orc_tuv *m, int lt, const zdouble *G, double out[7]);
 void orc_tuv_init_chain(orc_tuv *m);
 void orc_tuv_phy_update(orc_tuv *m, int sweeps, int record, double *rows);

@@ -39,6 +39,25 @@ void orc_draw(uint64_t seed, uint32_t chain, uint32_t stream, uint32_t sweep, ui
     *u = ((double)(x[1] >> 5) * 67108864.0 + (double)(x[2] >> 6)) * (1.0 / 9007199254740992.0);
 }
 
+/* One proposal of the plain sweep: the replayed draw when a buffer is set, else Philox
+ * stream 0. A sweep outside the buffer reads as a rejected proposal (u = +inf). */
+void orc_sweep_draw(const orc_replay *r, int nt, int n_elem, uint64_t seed, uint32_t chain, uint32_t sweep,
+                    int lt, int dir, uint32_t elem, int *choice, double *u) {
+    if (r && r->u) {
+        uint32_t s = sweep - r->sweep0;
+        if (s < r->nsweeps) {
+            size_t q = (((size_t)s * 2 + (size_t)dir) * (size_t)nt + (size_t)(lt - 1)) * (size_t)n_elem + elem;
+            *choice = r->choice[q] % 3;
+            *u = r->u[q];
+        } else {
+            *choice = 0;
+            *u = INFINITY;
+        }
+        return;
+    }
+    orc_draw(seed, chain, 0, sweep, (uint32_t)lt | ((uint32_t)dir << 31), elem, choice, u);
+}
+
 /* Initial_s replacement (src/tU/GreenMatrix.jl:1-8 draws i.i.d. uniform over {1,2,3,4}):
  * stream tag 15 is reserved for initial fields. */
 int orc_init_value(uint64_t seed, uint32_t chain, uint32_t stream, uint32_t slice, uint32_t elem) {

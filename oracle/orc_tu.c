@@ -148,11 +148,10 @@ void orc_tu_update_layer(orc_tu *m, int lt, int dir) {
     int n = m->ns;
     zdouble *G = m->G;
     zdouble *zN = m->tmpN;
-    uint32_t tag = (uint32_t)lt | ((uint32_t)dir << 31);
     for (int i = 0; i < n; ++i) {
         int si = m->s[IX(i, lt - 1, n)];
         int choice; double u;
-        orc_draw(m->seed, m->chain, 0, m->sweep, tag, (uint32_t)i, &choice, &u);
+        orc_sweep_draw(&m->rp, m->nt, m->ns, m->seed, m->chain, m->sweep, lt, dir, (uint32_t)i, &choice, &u);
         /* Random.Sampler over the other three values, increasing order */
         int allowed[3], c = 0;
         for (int v = 1; v <= 4; ++v) if (v != si) allowed[c++] = v;
@@ -395,4 +394,27 @@ void orc_tu_Gtau(orc_tu *m, int tau, zdouble *out) {
     orc_zqr_q(n, np, BRm, n, NULL);
     orc_get_G(n, np, BLm, np, BRm, n, out, NULL);
     free(BLm); free(BRm); free(T1); free(T2); free(D);
+}
+
+void orc_tu_set_replay(orc_tu *m, const uint8_t *choice, const double *u, uint32_t nsweeps) {
+    m->rp.choice = nsweeps ? choice : NULL;
+    m->rp.u = nsweeps ? u : NULL;
+    m->rp.sweep0 = m->sweep;
+    m->rp.nsweeps = nsweeps;
+}
+
+/* Imaginary parts of the model matrices for complex hopping (flux != 0; the reference keeps
+ * K, eK, eKinv, HalfeK, HalfeKinv and Pt as ComplexF64, src/tU/model.jl:24-34); the real
+ * parts came with orc_tu_new. */
+void orc_tu_set_imag(orc_tu *m, const double *K_im, const double *eK_im, const double *eKinv_im,
+                     const double *HalfeK_im, const double *HalfeKinv_im, const double *Pt_im) {
+    size_t nn = (size_t)m->ns * m->ns, np = (size_t)m->ns * m->np;
+    for (size_t q = 0; q < nn; ++q) {
+        m->K[q] += I * K_im[q];
+        m->eK[q] += I * eK_im[q];
+        m->eKinv[q] += I * eKinv_im[q];
+        m->HalfeK[q] += I * HalfeK_im[q];
+        m->HalfeKinv[q] += I * HalfeKinv_im[q];
+    }
+    for (size_t q = 0; q < np; ++q) m->Pt[q] += I * Pt_im[q];
 }

@@ -111,14 +111,13 @@ void orc_tuv_BM_F(orc_tuv *m, int idx, zdouble *BM) {
 void orc_tuv_update_layer(orc_tuv *m, int lt, int dir) {
     int n = m->ns;
     zdouble *G = m->G;
-    uint32_t tag = (uint32_t)lt | ((uint32_t)dir << 31);
     for (int i = 0; i < m->nbd; ++i) {
         int x = m->bx[i], y = m->by[i];
         int sub[2] = {x, y};
         size_t sidx = IX(i, lt - 1, m->nbd);
         int si = m->s[sidx];
         int choice; double u;
-        orc_draw(m->seed, m->chain, 0, m->sweep, tag, (uint32_t)i, &choice, &u);
+        orc_sweep_draw(&m->rp, m->nt, m->nbd, m->seed, m->chain, m->sweep, lt, dir, (uint32_t)i, &choice, &u);
         int allowed[3], c = 0;
         for (int v = 1; v <= 4; ++v) if (v != si) allowed[c++] = v;
         int sx = allowed[choice];
@@ -312,4 +311,11 @@ void orc_tuv_phy_update(orc_tuv *m, int sweeps, int record, double *rows) {
         m->sweep += 1;
     }
     free(D);
+}
+
+void orc_tuv_set_replay(orc_tuv *m, const uint8_t *choice, const double *u, uint32_t nsweeps) {
+    m->rp.choice = nsweeps ? choice : NULL;
+    m->rp.u = nsweeps ? u : NULL;
+    m->rp.sweep0 = m->sweep;
+    m->rp.nsweeps = nsweeps;
 }

@@ -183,14 +183,13 @@ void orc_tv_update_family(orc_tv *m, int lt, int dir, int j) {
     double *G = m->G;
     const double h = sqrt(2.0) / 2.0;
     const double uv[2][2] = {{-h, -h}, {-h, h}};
-    uint32_t tag = (uint32_t)lt | ((uint32_t)dir << 31);
     for (int i = 0; i < m->nb; ++i) {
         int x = m->bx[i + j * m->nb], y = m->by[i + j * m->nb];
         int sub[2] = {x, y};
         size_t si_ = SIDX(m, i, j, lt - 1);
         int si = m->s[si_];
         int choice; double u;
-        orc_draw(m->seed, m->chain, 0, m->sweep, tag, (uint32_t)(i + j * m->nb), &choice, &u);
+        orc_sweep_draw(&m->rp, m->nt, 3 * m->nb, m->seed, m->chain, m->sweep, lt, dir, (uint32_t)(i + j * m->nb), &choice, &u);
         int allowed[3], c = 0;
         for (int v = 1; v <= 4; ++v) if (v != si) allowed[c++] = v;
         int sx = allowed[choice];
@@ -423,4 +422,11 @@ void orc_tv_Gtau(orc_tv *m, int tau, double *out) {
     }
     dget_G(n, np, L, R, out);
     free(R); free(L); free(T); free(D); free(Rt); free(Lt);
+}
+
+void orc_tv_set_replay(orc_tv *m, const uint8_t *choice, const double *u, uint32_t nsweeps) {
+    m->rp.choice = nsweeps ? choice : NULL;
+    m->rp.u = nsweeps ? u : NULL;
+    m->rp.sweep0 = m->sweep;
+    m->rp.nsweeps = nsweeps;
 }

@@ -64,3 +64,22 @@ def test_no_cpu_fallback_when_library_missing(tmp_path):
             _lib.load(str(tmp_path / "missing.so"))
         finally:
             _lib._lib = _lib._lib_saved
+
+
+def test_complex_model_matrices_validated_before_any_device_call(lib):
+    """Imaginary parts come all six or none, t-U only, FP64 only (checked before HIP)."""
+    import numpy as np
+    from kapdqmc_amd import _lib
+    from kapdqmc_amd.dqmc import _descriptors
+    from kapdqmc_amd.models import tU_Hubbard_Para
+    m = tU_Hubbard_Para(Ht=1.0, Hu1=4.0, Hu2=4.0, Δt=0.1, Θrelax=0.5, Θquench=0.0, Lattice="SQUARE90",
+                        site=[3, 2], BatchSize=5, flux=np.pi)
+    h = C.c_void_p()
+    md, rd, keep = _descriptors([m], [1], [0], 8, 0, precision=_lib.PREC_MIXED)
+    assert bool(md[0].K_im) and bool(md[0].Pt_im)          # the imaginary parts are passed, not dropped
+    assert lib.kdq_create(md, rd, 1, C.byref(h)) == _lib.KDQ_EARG
+    assert b"FP64" in lib.kdq_last_error(None)
+    md, rd, keep = _descriptors([m], [1], [0], 8, 0)
+    md[0].Pt_im = None
+    assert lib.kdq_create(md, rd, 1, C.byref(h)) == _lib.KDQ_EARG
+    assert b"all six" in lib.kdq_last_error(None)
