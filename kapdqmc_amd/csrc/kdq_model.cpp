@@ -298,7 +298,14 @@ bool coupling_profile(double h1, double h2, double th_relax, double th_quench, d
         const int nq = (int)std::nearbyint(th_quench / dt);
         const int nr = (int)std::nearbyint(th_relax / dt);
         std::vector<double> lin;
-        for (int i = 1; i <= nq; ++i) lin.push_back(h1 + (h2 - h1) * (double)i / nq);
+        // LinRange(h1, h2, nq + 1)[2:end]: element i is (1 - t) h1 + t h2 with t = i / nq, as
+        // Julia's lerp evaluates it (two products and a sum, no fused multiply-add)
+        for (int i = 1; i <= nq; ++i) {
+            const double t = (double)i / nq;
+            volatile double a = (1.0 - t) * h1;
+            volatile double b = t * h2;
+            lin.push_back(a + b);
+        }
         prof.assign(nr, h1);
         prof.insert(prof.end(), lin.begin(), lin.end());
         prof.insert(prof.end(), lin.rbegin(), lin.rend());

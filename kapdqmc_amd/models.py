@@ -34,7 +34,8 @@ def _quench_profile(H1, H2, Theta_relax, Theta_quench, dt, Nt):
     """Piecewise-linear coupling profile (src/tU/model.jl:41-52)."""
     if Theta_quench > 0.0 and abs(H1 - H2) > 0:
         nq = round(Theta_quench / dt)
-        lin = np.linspace(H1, H2, nq + 1)[1:]
+        t = np.arange(1, nq + 1) / nq              # LinRange(H1, H2, nq + 1)[2:end] as Julia's lerp
+        lin = (1.0 - t) * H1 + t * H2
         nr = round(Theta_relax / dt)
         prof = np.concatenate([np.full(nr, H1), lin, lin[::-1], np.full(nr, H1)])
     else:

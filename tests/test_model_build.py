@@ -45,6 +45,23 @@ def test_tu_equals_host_constructor(lattice, initial):
                           site=[4, 3], BatchSize=3, Initial=initial))
 
 
+@pytest.mark.parametrize("h1,h2", [(0.0, 3.8), (1.3, 3.8)])
+def test_quench_profile_is_julia_linrange(h1, h2):
+    """A step count that does not divide the range (the reference's quench test Hu 0 -> 3.8,
+    here over Θquench = 0.7, 7 slices): the profile is Julia's LinRange element for element,
+    (1 - t) h1 + t h2 with t = i / nq, in both constructors; numpy.linspace differs from it in
+    the last bit at some i, so the equality below would catch it."""
+    kw = dict(Ht=1.0, Hu1=h1, Hu2=h2, Δt=0.1, Θrelax=0.4, Θquench=0.7, Lattice="HoneyComb120",
+              site=[3, 3], BatchSize=10, Initial="H0")
+    m = check_same("tU", kw)
+    t = np.arange(1, 8) / 7
+    lin = np.array([(1.0 - x) * h1 + x * h2 for x in t])   # Julia: T((1-t)*a + t*b)
+    prof = np.concatenate([np.full(4, h1), lin, lin[::-1], np.full(4, h1)])
+    assert np.array_equal(m.α, np.sqrt(0.1 * prof / 2))
+    if h1 == 0.0:
+        assert (np.linspace(h1, h2, 8)[1:] != lin).any()
+
+
 @pytest.mark.parametrize("lattice", ["HoneyComb120", "HoneyComb60"])
 @pytest.mark.parametrize("initial", ["H0", "V"])
 def test_tv_and_tuv_equal_host_constructors(lattice, initial):
