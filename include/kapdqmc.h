@@ -189,6 +189,21 @@ int kdq_measure_noalloc(kdq_handle h, int32_t lt, double *out);
  * orthonormalised chains; the sweep state (chains, G) is not changed. */
 int kdq_G4(kdq_handle h, int32_t c, int32_t tau1, int32_t tau2, double *G1, double *G2, double *G12, double *G21);
 
+/* ---- failure detection (no counterpart in the reference beyond its commented checks) ----
+ * At every stabilisation node of a sweep the G recomputed from the chain (get_G!) is compared
+ * with the wrapped and updated G it replaces: the reference's commented wrap-error check
+ * (src/tU/phy_update.jl:54-58,92-97). out receives nchains x 4 doubles: [max |G_wrapped -
+ * G_recomputed| over the nodes since the last call (FP64 handles; mixed precision reports its
+ * error through kdq_get_precision_error), non-finite G (1/0), singular overlap (1/0),
+ * propagator lost rank (1/0)]; the maximum is reset by the call. A chain whose G turns
+ * non-finite (a non-finite model constant or field value) is flagged and the call that saw it
+ * returns KDQ_ENUM naming it, after the other chains finished their sweeps unaffected (rows of
+ * the others are valid); kdq_set_fields clears the chain's flags. Sweep handles. */
+int kdq_get_health(kdq_handle h, double *out);
+/* Projector check of every chain's current G (any slice): out receives nchains x 2 doubles,
+ * [max |G² - G|, |tr G - (ns - ns/2)|] (G is the projector I - U_R (U_L U_R)⁻¹ U_L). */
+int kdq_check_G(kdq_handle h, double *out);
+
 /* Counters: out[0] = proposals, out[1] = accepted, per chain (nchains x 2). */
 int kdq_get_counters(kdq_handle h, double *out);
 

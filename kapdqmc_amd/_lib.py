@@ -20,7 +20,7 @@ OPTIONS = {"graph": 0, "delay": 1, "gemm_tile": 2, "defer": 3, "pipe": 4}   # kd
 # Every symbol include/kapdqmc.h declares (checked by tests/test_capi.py).
 EXPORTS = ("kdq_create", "kdq_destroy", "kdq_last_error", "kdq_num_chains", "kdq_num_obs", "kdq_set_fields",
            "kdq_get_fields", "kdq_initial_s", "kdq_phy_update", "kdq_get_G", "kdq_set_G", "kdq_wrap",
-           "kdq_update_layer", "kdq_init_chain", "kdq_measure", "kdq_measure_noalloc", "kdq_set_replay", "kdq_get_counters", "kdq_get_near_ties", "kdq_profile",
+           "kdq_update_layer", "kdq_init_chain", "kdq_measure", "kdq_measure_noalloc", "kdq_set_replay", "kdq_get_health", "kdq_check_G", "kdq_get_counters", "kdq_get_near_ties", "kdq_profile",
            "kdq_profile_read", "kdq_get_logdet", "kdq_set_option", "kdq_get_precision_error", "kdq_log_weight", "kdq_scee_create", "kdq_scee_set_fields",
            "kdq_scee_get_fields", "kdq_scee_init", "kdq_scee_sweeps", "kdq_scee_get_G4", "kdq_scee_detg",
            "kdq_dop_create", "kdq_G4", "kdq_model_build", "kdq_model_free", "kdq_net_create", "kdq_net_destroy", "kdq_net_last_error", "kdq_net_num_params", "kdq_net_get_params",
@@ -101,6 +101,8 @@ def load(path: str = LIB_PATH):
     L.kdq_init_chain.argtypes = [vp]
     L.kdq_measure.argtypes = [vp, i32, dp]
     L.kdq_measure_noalloc.argtypes = [vp, i32, dp]
+    L.kdq_get_health.argtypes = [vp, dp]
+    L.kdq_check_G.argtypes = [vp, dp]
     L.kdq_get_counters.argtypes = [vp, dp]
     L.kdq_get_near_ties.argtypes = [vp, dp]
     L.kdq_profile.argtypes = [vp, u32]

@@ -96,6 +96,8 @@ struct kdq_handle_s {
     int sl_move = 0;
     int *d_info = nullptr;        // [nch] sticky: nonzero after a singular overlap
     int *d_qinfo = nullptr;       // [nch] sticky: nonzero after a rank-deficient propagator
+    int *d_poison = nullptr;      // [nch] sticky: nonzero after a non-finite G at a stabilisation node
+    double *d_wraperr = nullptr;  // [nch] max |G_wrapped - G_recomputed| at the nodes since the last read
     int *d_piv = nullptr;         // [nch][max_np] LU pivots of the overlap
     int cur = 0;           // which G buffer is current
     int idx = 1;           // current node index (1-based, as the reference)
@@ -914,7 +916,14 @@ int check_health(kdq_handle h) {
     HIPCHK(hipMemcpyAsync(info.data(), h->d_info, sizeof(int) * h->nch, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipMemcpyAsync(qinfo.data(), h->d_qinfo, sizeof(int) * h->nch, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
+    std::vector<int> pois(h->nch, 0);
+    if (h->d_poison)
+        HIPCHK(hipMemcpyAsync(pois.data(), h->d_poison, sizeof(int) * h->nch, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
     for (int c = 0; c < h->nch; ++c) {
+        if (pois[c])
+            return fail(h, KDQ_ENUM, "non-finite Green's function in chain " + std::to_string(c) +
+                                         " (the other chains are unaffected; kdq_set_fields resets the chain)");
         if (info[c]) return fail(h, KDQ_ENUM, "singular overlap U_L U_R in chain " + std::to_string(c));
         if (qinfo[c])
             return fail(h, KDQ_ENUM, "propagator lost rank between stabilisations in chain " + std::to_string(c) +
@@ -930,6 +939,25 @@ bool in_window(kdq_handle h, int thidx, bool forward) {
     const int d = h->idx - thidx;
     if (h->kind == KDQ_MODEL_TU) return forward ? (d <= 0 && d >= -1) : (d >= 0 && d <= 1);
     return d >= -1 && d <= 1;
+}
+
+// Stabilisation at node idx: G recomputed from the chain (get_G!). FP64: into the spare
+// buffer, compared with the wrapped and updated G it replaces (max |difference| per chain, the
+// reference's commented wrap-error check, src/tU/phy_update.jl:54-58,92-97; a non-finite entry
+// in either marks the chain), then made current. Mixed precision: the FP32 copy is compared and
+// replaced by refresh_g32.
+int node_refresh(kdq_handle h, int idx) {
+    if (h->mixed) {
+        if (int r = get_G(h, idx)) return r;
+        HIPCHK(launch_node_check(h->stream, h->nch, h->max_ns, h->d_chains, h->t_G[h->cur], h->t_G[h->cur], DIM_NS,
+                                 h->d_wraperr, h->d_poison));
+        return refresh_g32(h, true);
+    }
+    if (int r = get_G(h, idx, h->t_G[1 - h->cur])) return r;
+    HIPCHK(launch_node_check(h->stream, h->nch, h->max_ns, h->d_chains, h->t_G[h->cur], h->t_G[1 - h->cur], DIM_NS,
+                             h->d_wraperr, h->d_poison));
+    h->cur ^= 1;
+    return KDQ_OK;
 }
 
 // One sweep (forward and backward over all slices, stabilisation at the nodes, measurements
@@ -952,8 +980,7 @@ int sweep_body(kdq_handle h, int record, uint32_t sweep) {
         if (is_node(h, lt)) {
             h->idx += 1;
             if (int r = propagate_R(h, h->t_UR[h->idx - 2], h->t_UR[h->idx - 1], h->idx - 1)) return r;
-            if (int r = get_G(h, h->idx)) return r;
-            if (int r = refresh_g32(h, true)) return r;
+            if (int r = node_refresh(h, h->idx)) return r;
         }
     }
     for (int lt = nt; lt >= 1; --lt) {
@@ -970,8 +997,7 @@ int sweep_body(kdq_handle h, int record, uint32_t sweep) {
         if (is_node(h, lt - 1)) {
             h->idx -= 1;
             if (int r = propagate_L(h, h->t_ULH[h->idx], h->t_ULH[h->idx - 1], h->idx)) return r;
-            if (int r = get_G(h, h->idx)) return r;
-            if (int r = refresh_g32(h, true)) return r;
+            if (int r = node_refresh(h, h->idx)) return r;
         }
     }
     if (h->sweep_dev) HIPCHK(launch_bump(h->stream, h->d_sweep));
@@ -1446,14 +1472,20 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
         h->d_info = (int *)q2;
         h->d_qinfo = (int *)q3;
         h->d_piv = (int *)q4;
-        void *q5 = nullptr, *q6 = nullptr;
+        void *q5 = nullptr, *q6 = nullptr, *q7 = nullptr, *q8 = nullptr;
         if (hipMalloc(&q5, sizeof(uint32_t)) == hipSuccess) h->dev_misc.push_back(q5); else r = 1;
         if (hipMalloc(&q6, sizeof(double) * nchains) == hipSuccess) h->dev_misc.push_back(q6); else r = 1;
+        if (hipMalloc(&q7, sizeof(int) * nchains) == hipSuccess) h->dev_misc.push_back(q7); else r = 1;
+        if (hipMalloc(&q8, sizeof(double) * nchains) == hipSuccess) h->dev_misc.push_back(q8); else r = 1;
         h->d_sweep = (uint32_t *)q5;
         h->d_lw = (double *)q6;
+        h->d_poison = (int *)q7;
+        h->d_wraperr = (double *)q8;
         if (!r) {
             hipMemset(q2, 0, sizeof(int) * nchains);
             hipMemset(q3, 0, sizeof(int) * nchains);
+            hipMemset(q7, 0, sizeof(int) * nchains);
+            hipMemset(q8, 0, sizeof(double) * nchains);
         }
     }
     if (r || hipHostMalloc((void **)&h->h_meas, sizeof(double) * KDQ_MEAS_SLOTS * nchains, hipHostMallocDefault) != hipSuccess) {
@@ -1488,8 +1520,59 @@ int kdq_set_fields(kdq_handle h, int32_t c, const uint8_t *s) {
     for (size_t i = 0; i < n; ++i)
         if (s[i] < 1 || s[i] > 4) return fail(h, KDQ_EARG, "kdq_set_fields: field values must be 1..4");
     HIPCHK(hipMemcpyAsync(h->ch[c].s, s, n, hipMemcpyHostToDevice, h->stream));
+    static const int zero = 0;   // new fields: the chain's sticky failure flags start over
+    HIPCHK(hipMemcpyAsync(h->d_info + c, &zero, sizeof(int), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->d_qinfo + c, &zero, sizeof(int), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->d_poison + c, &zero, sizeof(int), hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     h->sl_logw_valid = false;
+    return KDQ_OK;
+}
+
+int kdq_get_health(kdq_handle h, double *out) {
+    if (!h || !out) return fail(h, KDQ_EARG, "kdq_get_health: bad argument");
+    const int n = h->nch;
+    std::vector<double> err(n);
+    std::vector<int> info(n), qinfo(n), pois(n);
+    HIPCHK(hipMemcpyAsync(err.data(), h->d_wraperr, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(pois.data(), h->d_poison, sizeof(int) * n, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(info.data(), h->d_info, sizeof(int) * n, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(qinfo.data(), h->d_qinfo, sizeof(int) * n, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    std::vector<double> zero(n, 0.0);
+    HIPCHK(hipMemcpy(h->d_wraperr, zero.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    for (int c = 0; c < n; ++c) {
+        out[4 * c] = err[c];
+        out[4 * c + 1] = pois[c] ? 1.0 : 0.0;
+        out[4 * c + 2] = info[c] ? 1.0 : 0.0;
+        out[4 * c + 3] = qinfo[c] ? 1.0 : 0.0;
+    }
+    return KDQ_OK;
+}
+
+int kdq_check_G(kdq_handle h, double *out) {
+    if (!h || !out) return fail(h, KDQ_EARG, "kdq_check_G: bad argument");
+    if (h->scee) return fail(h, KDQ_EARG, "kdq_check_G: sweep handles only (not swap-replica handles)");
+    const int n = h->nch;
+    Opnd *G = h->t_G[h->cur];
+    if (h->mixed)
+        HIPCHK(launch_to_f64(h->stream, n, h->max_ns, h->d_chains, h->t_G32[h->cur32], h->t_Gm, DIM_NS, 1));
+    else
+        HIPCHK(launch_copy_mat(h->stream, n, h->max_ns, h->max_ns, h->d_chains, G, h->t_Gm, DIM_NS, DIM_NS, true));
+    GemmArgs g = gargs(h, h->t_Gm, h->t_Gm, h->t_Tm, DIM_NS, DIM_NS, DIM_NS);   // G²
+    if (int r = gemm(h, true, true, 0, 0, g)) return r;
+    HIPCHK(hipMemsetAsync(h->d_lw, 0, sizeof(double) * n, h->stream));
+    HIPCHK(launch_node_check(h->stream, n, h->max_ns, h->d_chains, h->t_Tm, h->t_Gm, DIM_NS, h->d_lw, nullptr));
+    std::vector<double> e(n), tr(n);
+    HIPCHK(hipMemcpyAsync(e.data(), h->d_lw, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(launch_trace(h->stream, n, h->d_chains, h->t_Gm, h->d_lw));
+    HIPCHK(hipMemcpyAsync(tr.data(), h->d_lw, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    for (int c = 0; c < n; ++c) {
+        const int ns = h->ch[c].ns;
+        out[2 * c] = std::isfinite(tr[c]) ? e[c] : tr[c];
+        out[2 * c + 1] = std::fabs(tr[c] - (ns - ns / 2));
+    }
     return KDQ_OK;
 }
 

@@ -90,6 +90,9 @@ hipError_t launch_flush_f32(hipStream_t st, int nchains, int max_ns, const Flush
 // Mixed precision: dst = FP32(src) for per-chain n x n complex matrices; when maxerr is set,
 // max |dst_before - src| is folded into maxerr[c] first (FP32 propagation error against the
 // FP64 recompute). to_f64 is the reverse copy.
+hipError_t launch_node_check(hipStream_t st, int nchains, int max_n, const ChainDev *chains, const Opnd *wrapped,
+                             const Opnd *fresh, int ndim, double *maxerr, int *poison);
+hipError_t launch_trace(hipStream_t st, int nchains, const ChainDev *chains, const Opnd *G, double *out);
 hipError_t launch_to_f32(hipStream_t st, int nchains, int max_n, const ChainDev *chains, const Opnd *src,
                          const OpndF *dst, int ndim, double *maxerr);
 hipError_t launch_to_f64(hipStream_t st, int nchains, int max_n, const ChainDev *chains, const OpndF *src,

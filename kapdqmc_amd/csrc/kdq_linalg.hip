@@ -52,6 +52,43 @@ __global__ void k_to_f32(const ChainDev *chains, const Opnd *src, const OpndF *d
         atomicMax(reinterpret_cast<unsigned long long *>(maxerr + c), (unsigned long long)__double_as_longlong(err));
 }
 
+// Health check at a stabilisation node: max |wrapped - fresh| per chain into maxerr (the
+// reference's commented wrap-error check, src/tU/phy_update.jl:54-58,92-97) and a sticky flag
+// when either matrix holds a non-finite entry. Also used for max |G² - G| (wrapped = G², fresh = G).
+__global__ void k_node_check(const ChainDev *chains, const Opnd *wrapped, const Opnd *fresh, int ndim,
+                             double *maxerr, int *poison) {
+    const int c = blockIdx.z;
+    const int n = dim_of(chains[c], ndim);
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63), j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    double err = 0.0;
+    int bad = 0;
+    if (i < n && j < n) {
+        const Opnd w = wrapped[c], f = fresh[c];
+        const size_t ow = (size_t)i + (size_t)j * w.ld, of = (size_t)i + (size_t)j * f.ld;
+        const double wr = w.re[ow], wi = w.im[ow], fr = f.re[of], fi = f.im[of];
+        bad = !(isfinite(wr) && isfinite(wi) && isfinite(fr) && isfinite(fi));
+        err = bad ? 0.0 : fmax(fabs(wr - fr), fabs(wi - fi));
+    }
+    err = wave_max(err);
+    const unsigned long long anybad = __ballot(bad);
+    if ((threadIdx.x & 63) == 0) {
+        if (err > 0.0)
+            atomicMax(reinterpret_cast<unsigned long long *>(maxerr + c), (unsigned long long)__double_as_longlong(err));
+        if (anybad && poison) atomicOr(poison + c, 1);
+    }
+}
+
+// out[c] = Re tr G of the current G (projector check: N - Np)
+__global__ void k_trace(const ChainDev *chains, const Opnd *G, double *out) {
+    const int c = blockIdx.x;
+    const int n = chains[c].ns;
+    const Opnd g = G[c];
+    double v = 0.0;
+    for (int i = threadIdx.x; i < n; i += 64) v += g.re[(size_t)i + (size_t)i * g.ld];
+    v = wave_sum(v);
+    if (threadIdx.x == 0) out[c] = v;
+}
+
 __global__ void k_to_f64(const ChainDev *chains, const OpndF *src, const Opnd *dst, int ndim, int cplx) {
     const int c = blockIdx.z;
     const int n = dim_of(chains[c], ndim);
@@ -105,6 +142,18 @@ hipError_t launch_to_f32(hipStream_t st, int nchains, int max_n, const ChainDev 
                          const OpndF *dst, int ndim, double *maxerr) {
     dim3 grid((max_n + 63) / 64, (max_n + 3) / 4, nchains);
     hipLaunchKernelGGL(k_to_f32, grid, dim3(256), 0, st, chains, src, dst, ndim, maxerr);
+    return hipGetLastError();
+}
+
+hipError_t launch_node_check(hipStream_t st, int nchains, int max_n, const ChainDev *chains, const Opnd *wrapped,
+                             const Opnd *fresh, int ndim, double *maxerr, int *poison) {
+    dim3 grid((max_n + 63) / 64, (max_n + 3) / 4, nchains);
+    hipLaunchKernelGGL(k_node_check, grid, dim3(256), 0, st, chains, wrapped, fresh, ndim, maxerr, poison);
+    return hipGetLastError();
+}
+
+hipError_t launch_trace(hipStream_t st, int nchains, const ChainDev *chains, const Opnd *G, double *out) {
+    hipLaunchKernelGGL(k_trace, dim3(nchains), dim3(64), 0, st, chains, G, out);
     return hipGetLastError();
 }
 

@@ -234,6 +234,19 @@ class DQMCBatch:
         self._chk(self.L.kdq_set_replay(self.h, c, ch.ctypes.data_as(C.POINTER(C.c_uint8)), _dp(uu),
                                         ch.size // per))
 
+    def health(self) -> np.ndarray:
+        """[chains, 4]: max |G_wrapped - G_recomputed| at the stabilisation nodes since the last
+        call, non-finite G, singular overlap, lost rank (kdq_get_health)."""
+        out = np.zeros((self.n, 4))
+        self._chk(self.L.kdq_get_health(self.h, _dp(out)))
+        return out
+
+    def check_G(self) -> np.ndarray:
+        """[chains, 2]: max |G² - G| and |tr G - (N - Np)| of the current G (kdq_check_G)."""
+        out = np.zeros((self.n, 2))
+        self._chk(self.L.kdq_check_G(self.h, _dp(out)))
+        return out
+
     def measure_noalloc(self, lt: int) -> np.ndarray:
         """measure() through kdq_measure_noalloc: the working matrices live in the library's
         per-device static array (at most 32 MiB for the whole batch), nothing is allocated."""
