@@ -98,6 +98,7 @@ struct kdq_handle_s {
     int *d_qinfo = nullptr;       // [nch] sticky: nonzero after a rank-deficient propagator
     int *d_poison = nullptr;      // [nch] sticky: nonzero after a non-finite G at a stabilisation node
     double *d_wraperr = nullptr;  // [nch] max |G_wrapped - G_recomputed| at the nodes since the last read
+    double *d_mpart = nullptr;    // tile partial sums of the measurement
     int *d_piv = nullptr;         // [nch][max_np] LU pivots of the overlap
     int cur = 0;           // which G buffer is current
     int idx = 1;           // current node index (1-based, as the reference)
@@ -861,7 +862,7 @@ int wrap_slice(kdq_handle h, Opnd *Gt, Opnd *Tt, int lt, bool forward, int fam) 
 
 // phy_measure: wrap a copy of G to τ = Nt/2, HalfeK sandwich, observables (accumulated).
 // Gm, Tm: per-chain N x N complex working matrices; meas: accumulators (nullptr: the chains').
-int measure_into(kdq_handle h, int lt, Opnd *Gm, Opnd *Tm, double *meas) {
+int measure_into(kdq_handle h, int lt, Opnd *Gm, Opnd *Tm, double *meas, double *part) {
     if (h->mixed)
         HIPCHK(launch_to_f64(h->stream, h->nch, h->max_ns, h->d_chains, h->t_G32[h->cur32], Gm, DIM_NS, 1));
     else
@@ -878,14 +879,13 @@ int measure_into(kdq_handle h, int lt, Opnd *Gm, Opnd *Tm, double *meas) {
     if (int r = gemm_m(h, true, false, false, true, 0, 0, g1, KDQ_PROF_MEAS_WRAP)) return r;
     GemmArgs g2 = gargs(h, Tm, h->t_hKinv, Gm, DIM_NS, DIM_NS, DIM_NS);
     if (int r = gemm_m(h, false, true, true, false, 0, 0, g2, KDQ_PROF_MEAS_WRAP)) return r;
-    MeasArgs m{h->d_chains, Gm, meas};
+    MeasArgs m{h->d_chains, Gm, meas, part, meas_tiles(h->max_ns)};
     ProfScope ps(h, KDQ_PROF_MEASURE);
-    if (h->kind == KDQ_MODEL_TU) HIPCHK(launch_measure_tu(h->stream, h->nch, m));
-    else HIPCHK(launch_measure_model(h->stream, h->nch, h->kind, m));
+    HIPCHK(launch_measure(h->stream, h->nch, m));
     return KDQ_OK;
 }
 
-int measure(kdq_handle h, int lt) { return measure_into(h, lt, h->t_Gm, h->t_Tm, nullptr); }
+int measure(kdq_handle h, int lt) { return measure_into(h, lt, h->t_Gm, h->t_Tm, nullptr, h->d_mpart); }
 
 int elems(kdq_handle h, int c) {
     const int n = h->ch[c].ns;
@@ -1472,7 +1472,9 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
         h->d_info = (int *)q2;
         h->d_qinfo = (int *)q3;
         h->d_piv = (int *)q4;
-        void *q5 = nullptr, *q6 = nullptr, *q7 = nullptr, *q8 = nullptr;
+        void *q5 = nullptr, *q6 = nullptr, *q7 = nullptr, *q8 = nullptr, *q9 = nullptr;
+        if (hipMalloc(&q9, sizeof(double) * meas_part_doubles(nchains, h->max_ns)) == hipSuccess) h->dev_misc.push_back(q9); else r = 1;
+        h->d_mpart = (double *)q9;
         if (hipMalloc(&q5, sizeof(uint32_t)) == hipSuccess) h->dev_misc.push_back(q5); else r = 1;
         if (hipMalloc(&q6, sizeof(double) * nchains) == hipSuccess) h->dev_misc.push_back(q6); else r = 1;
         if (hipMalloc(&q7, sizeof(int) * nchains) == hipSuccess) h->dev_misc.push_back(q7); else r = 1;
@@ -1785,15 +1787,17 @@ int scratch_base(kdq_handle h, int dev, unsigned char **out) {
 }
 
 // Scratch layout: [Opnd tables Gm[nch], Tm[nch]] [accumulators nch x KDQ_MEAS_SLOTS]
-// [per chain: Gm re, Gm im, Tm re, Tm im, each ns x ns doubles padded to 256 bytes].
+// [tile partial sums of the measurement] [per chain: Gm re, Gm im, Tm re, Tm im, each ns x ns
+// doubles padded to 256 bytes].
 struct NoallocLayout {
-    size_t off_meas, off_mats, total;
+    size_t off_meas, off_part, off_mats, total;
 };
 
 NoallocLayout noalloc_layout(kdq_handle h) {
     NoallocLayout L{};
     L.off_meas = align256(2 * sizeof(Opnd) * (size_t)h->nch);
-    L.off_mats = L.off_meas + align256(sizeof(double) * KDQ_MEAS_SLOTS * (size_t)h->nch);
+    L.off_part = L.off_meas + align256(sizeof(double) * KDQ_MEAS_SLOTS * (size_t)h->nch);
+    L.off_mats = L.off_part + align256(sizeof(double) * meas_part_doubles(h->nch, h->max_ns));
     size_t t = L.off_mats;
     for (int c = 0; c < h->nch; ++c) t += 4 * align256(sizeof(double) * (size_t)h->ch[c].ns * h->ch[c].ns);
     L.total = t;
@@ -1843,7 +1847,7 @@ int kdq_measure_noalloc(kdq_handle h, int32_t lt, double *out) {
     double *meas = (double *)(base + L.off_meas);
     const uint32_t mask = h->prof_mask;   // kernel timing would create events: off for this call
     h->prof_mask = 0;
-    const int r = measure_into(h, lt, tGm, tTm, meas);
+    const int r = measure_into(h, lt, tGm, tTm, meas, (double *)(base + L.off_part));
     h->prof_mask = mask;
     if (r) return r;
     const int S = KDQ_MEAS_SLOTS, no = h->nobs;

@@ -155,8 +155,13 @@ struct MeasArgs {
     const ChainDev *chains;
     const Opnd *G;
     double *meas;   // nullptr: accumulate into chains[c].meas; else into meas + c * KDQ_MEAS_SLOTS
+    double *part;   // tile partial sums, meas_part_doubles(nchains, max_ns) doubles
+    int T;          // tiles per dimension, meas_tiles(max_ns)
 };
-hipError_t launch_measure_tu(hipStream_t st, int nchains, const MeasArgs &m);
+// phy_measure of every chain (any model) on the sandwiched G0: tile reduction + per-chain finish.
+hipError_t launch_measure(hipStream_t st, int nchains, const MeasArgs &m);
+int meas_tiles(int max_ns);
+size_t meas_part_doubles(int nchains, int max_ns);
 
 // Rank-2 local update (t-V bond flips, t-U-V directed-bond flips) of one block.
 struct BlockDesc {
@@ -185,7 +190,6 @@ struct BondRotArgs {
 };
 hipError_t launch_bondrot(hipStream_t st, int nchains, int max_nb, int max_cols, const BondRotArgs &a);
 // Observables of t-V (kind 1) and t-U-V (kind 2).
-hipError_t launch_measure_model(hipStream_t st, int nchains, int kind, const MeasArgs &m);
 
 // Small helpers.
 hipError_t launch_copy_mat(hipStream_t st, int nchains, int max_m, int max_n, const ChainDev *chains,

@@ -401,154 +401,6 @@ __global__ void k_bondrot(BondRotArgs a) {
         }
 }
 
-constexpr int NT = 256;
-
-KDQ_DEV void block_reduce_add(double *v, int nv, double *red, double *dst, double scale_last) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int k = 0; k < nv; ++k) {
-        const double s = wave_sum(v[k]);
-        if (lane == 0) red[k * (NT / 64) + wave] = s;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int k = 0; k < nv; ++k) {
-            double s = 0.0;
-            for (int w = 0; w < NT / 64; ++w) s += red[k * (NT / 64) + w];
-            dst[k] += s;
-        }
-        dst[KDQ_MEAS_SLOTS - 1] += scale_last;
-    }
-}
-
-// phy_measure of t-V (src/tV/phy_update.jl:257-313) on the sandwiched G0 (real):
-// Ek = Ht Σ K∘G0, Ev = Σ_bonds (1-G_xx)(1-G_yy) - G_xy G_yx, R0/R1[4] x 4/Ns².
-__global__ __launch_bounds__(NT) void k_measure_tv(MeasArgs a) {
-    const ChainDev &ch = a.chains[blockIdx.x];
-    const int n = ch.ns;
-    const Opnd G = a.G[blockIdx.x];
-    const size_t ld = G.ld;
-    double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int q = threadIdx.x; q < n * n; q += NT) v[0] += ch.K[q] * G.re[(q % n) + (q / n) * ld];
-#define g(p, q) G.re[(size_t)(p) + (size_t)(q) * ld]
-    for (int k = threadIdx.x; k < ch.n_elem; k += NT) {
-        const int x = ch.ex[k], y = ch.ey[k];
-        v[1] += (1 - g(x, x)) * (1 - g(y, y)) - g(x, y) * g(y, x);
-    }
-    const int L1 = ch.L1, L2 = ch.L2, nc = L1 * L2;
-    const double tp1 = 2.0 * M_PI / L1, tp2 = 2.0 * M_PI / L2;
-    for (int p = threadIdx.x; p < nc * nc; p += NT) {
-        const int cell = p % nc, rr = p / nc;
-        const int ix = cell % L1, iy = cell / L1, rx = rr % L1 + 1, ry = rr / L1 + 1;
-        const int a0 = 2 * (ix + iy * L1), b0 = 2 * ((ix + rx) % L1 + ((iy + ry) % L2) * L1);
-        double t[4];
-        if (a0 == b0) {
-            t[0] = 1 - g(a0, a0);
-            t[1] = 1 - g(a0 + 1, a0 + 1);
-        } else {
-            t[0] = (1 - g(a0, a0)) * (1 - g(b0, b0)) - g(a0, b0) * g(b0, a0);
-            t[1] = (1 - g(a0 + 1, a0 + 1)) * (1 - g(b0 + 1, b0 + 1)) - g(a0 + 1, b0 + 1) * g(b0 + 1, a0 + 1);
-        }
-        t[2] = (1 - g(a0 + 1, a0 + 1)) * (1 - g(b0, b0)) - g(a0 + 1, b0) * g(b0, a0 + 1);
-        t[3] = (1 - g(a0, a0)) * (1 - g(b0 + 1, b0 + 1)) - g(a0, b0 + 1) * g(b0 + 1, a0);
-        const double w = cos(tp1 * rx + tp2 * ry);
-        for (int k = 0; k < 4; ++k) {
-            v[2 + k] += t[k];
-            v[6 + k] += w * t[k];
-        }
-    }
-#undef g
-    const double f = 4.0 / ((double)n * n);
-    v[0] *= ch.ht;
-    for (int k = 2; k < 10; ++k) v[k] *= f;
-    __shared__ double red[10 * (NT / 64)];
-    block_reduce_add(v, 10, red, a.meas ? a.meas + (size_t)blockIdx.x * KDQ_MEAS_SLOTS : ch.meas, 1.0);
-}
-
-// phy_measure of t-U-V (src/tUV/phy_update.jl:205-355): Ek, Eu, Ev, CDW0/1, SDW0/1.
-__global__ __launch_bounds__(NT) void k_measure_tuv(MeasArgs a) {
-    const ChainDev &ch = a.chains[blockIdx.x];
-    const int n = ch.ns;
-    const Opnd G = a.G[blockIdx.x];
-    const size_t ld = G.ld;
-    const bool cx = ch.is_complex != 0;
-    double v[7] = {0, 0, 0, 0, 0, 0, 0};
-#define GR(p, q) G.re[(size_t)(p) + (size_t)(q) * ld]
-#define GI(p, q) G.im[(size_t)(p) + (size_t)(q) * ld]
-    for (int q = threadIdx.x; q < n * n; q += NT) v[0] += ch.K[q] * GR(q % n, q / n);
-    for (int i = threadIdx.x; i < n; i += NT) {
-        const double gr = GR(i, i), gi = GI(i, i);
-        if (cx) v[1] += (1.0 - gr) * gr - gi * gi;           // Re[(1-g) conj(g)]
-        else v[1] += (1.0 - gr) * (1.0 - gr);
-    }
-    for (int k = threadIdx.x; k < ch.n_elem; k += NT) {
-        const int x = ch.ex[k], y = ch.ey[k];
-        double t0r, t0i, t1r, t1i;
-        if (cx) {
-            // (1-gxx)(1-gyy) - gxy gyx  +  conj(gxx gyy - gxy gyx)  +  (1-gyy) conj(gxx) + (1-gxx) conj(gyy)
-            cmul(1.0 - GR(x, x), -GI(x, x), 1.0 - GR(y, y), -GI(y, y), t0r, t0i);
-            cmul(GR(x, y), GI(x, y), GR(y, x), GI(y, x), t1r, t1i);
-            v[2] += t0r - t1r;
-            cmul(GR(x, x), GI(x, x), GR(y, y), GI(y, y), t0r, t0i);
-            v[2] += t0r - t1r;
-            cmul(1.0 - GR(y, y), -GI(y, y), GR(x, x), -GI(x, x), t0r, t0i);
-            v[2] += t0r;
-            cmul(1.0 - GR(x, x), -GI(x, x), GR(y, y), -GI(y, y), t0r, t0i);
-            v[2] += t0r;
-        } else {
-            v[2] += 2.0 * ((1.0 - GR(x, x)) * (1.0 - GR(y, y)) - GR(x, y) * GR(y, x));
-            v[2] += 2.0 * (1.0 - GR(y, y)) * (1.0 - GR(x, x));
-        }
-    }
-    const int L1 = ch.L1, L2 = ch.L2, nc = L1 * L2;
-    const double tp1 = 2.0 * M_PI / L1, tp2 = 2.0 * M_PI / L2;
-    for (int p = threadIdx.x; p < nc * nc; p += NT) {
-        const int cell = p % nc, rr = p / nc;
-        const int ix = cell % L1, iy = cell / L1, rx = rr % L1 + 1, ry = rr / L1 + 1;
-        const int a0 = 2 * (ix + iy * L1), b0 = 2 * ((ix + rx) % L1 + ((iy + ry) % L2) * L1);
-        const int a1 = a0 + 1, b1 = b0 + 1;
-        const double d = (a0 == b0) ? 1.0 : 0.0;
-        double t1 = 0.0, t2 = 0.0, xr, xi;
-        if (!cx) {
-            t1 += 2.0 * ((1 - GR(a0, a0)) * (1 - GR(b0, b0)) + (d - GR(a0, b0)) * GR(b0, a0));
-            t1 += 2.0 * ((1 - GR(a1, a1)) * (1 - GR(b1, b1)) + (d - GR(a1, b1)) * GR(b1, a1));
-            t1 -= 2.0 * ((1 - GR(a1, a1)) * (1 - GR(b0, b0)) - GR(a1, b0) * GR(b0, a1));
-            t1 -= 2.0 * ((1 - GR(a0, a0)) * (1 - GR(b1, b1)) - GR(a0, b1) * GR(b1, a0));
-            t2 += 2.0 * (1 - GR(a0, a0)) * (1 - GR(b0, b0));
-            t2 += 2.0 * (1 - GR(a1, a1)) * (1 - GR(b1, b1));
-            t2 -= 2.0 * (1 - GR(a1, a1)) * (1 - GR(b0, b0));
-            t2 -= 2.0 * (1 - GR(a0, a0)) * (1 - GR(b1, b1));
-        } else {
-            // first expression of each line pair only (the `+adjoint(` lines are discarded)
-            cmul(1 - GR(a0, a0), -GI(a0, a0), 1 - GR(b0, b0), -GI(b0, b0), xr, xi); t1 += xr;
-            cmul(d - GR(a0, b0), -GI(a0, b0), GR(b0, a0), GI(b0, a0), xr, xi); t1 += xr;
-            cmul(1 - GR(a1, a1), -GI(a1, a1), 1 - GR(b1, b1), -GI(b1, b1), xr, xi); t1 += xr;
-            cmul(d - GR(a1, b1), -GI(a1, b1), GR(b1, a1), GI(b1, a1), xr, xi); t1 += xr;
-            cmul(1 - GR(a1, a1), -GI(a1, a1), 1 - GR(b0, b0), -GI(b0, b0), xr, xi); t1 -= xr;
-            cmul(GR(a1, b0), GI(a1, b0), GR(b0, a1), GI(b0, a1), xr, xi); t1 += xr;
-            cmul(1 - GR(a0, a0), -GI(a0, a0), 1 - GR(b1, b1), -GI(b1, b1), xr, xi); t1 -= xr;
-            cmul(GR(a0, b1), GI(a0, b1), GR(b1, a0), GI(b1, a0), xr, xi); t1 += xr;
-            cmul(1 - GR(a0, a0), -GI(a0, a0), GR(b0, b0), -GI(b0, b0), xr, xi); t2 += xr;
-            cmul(1 - GR(a1, a1), -GI(a1, a1), GR(b1, b1), -GI(b1, b1), xr, xi); t2 += xr;
-            cmul(1 - GR(a0, a0), -GI(a0, a0), GR(b1, b1), -GI(b1, b1), xr, xi); t2 -= xr;
-            cmul(1 - GR(a1, a1), -GI(a1, a1), GR(b0, b0), -GI(b0, b0), xr, xi); t2 -= xr;
-        }
-        const double w = cos(tp1 * rx + tp2 * ry);
-        v[3] += t1 + t2;
-        v[4] += w * (t1 + t2);
-        v[5] += t1 - t2;
-        v[6] += w * (t1 - t2);
-    }
-#undef GR
-#undef GI
-    const double n2 = (double)n * n;
-    v[0] *= 2.0 * ch.ht;
-    v[1] *= ch.U;
-    v[2] *= ch.V;
-    for (int k = 3; k < 7; ++k) v[k] /= n2;
-    __shared__ double red[7 * (NT / 64)];
-    block_reduce_add(v, 7, red, a.meas ? a.meas + (size_t)blockIdx.x * KDQ_MEAS_SLOTS : ch.meas, 1.0);
-}
-
 }  // namespace
 
 hipError_t launch_decide_r2(hipStream_t st, int nchains, const DecideR2Args &a) {
@@ -562,8 +414,3 @@ hipError_t launch_bondrot(hipStream_t st, int nchains, int max_nb, int max_cols,
     return hipGetLastError();
 }
 
-hipError_t launch_measure_model(hipStream_t st, int nchains, int kind, const MeasArgs &m) {
-    if (kind == 1) hipLaunchKernelGGL(k_measure_tv, dim3(nchains), dim3(NT), 0, st, m);
-    else hipLaunchKernelGGL(k_measure_tuv, dim3(nchains), dim3(NT), 0, st, m);
-    return hipGetLastError();
-}
