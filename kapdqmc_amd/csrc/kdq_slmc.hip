@@ -385,6 +385,203 @@ __global__ __launch_bounds__(STEP_THREADS) void k_slmc_step(StepArgs a) {
     }
 }
 
+// ---- layer 2 of training on BF16 matrix cores (C = 16 or 32 channels)
+//
+// The 3 x 3 x 3 convolution as an implicit GEMM: out[m][q] = Σ_kk A[m][kk] in[kk / 27][tap(q, kk % 27)]
+// with kk = ci·27 + k, on MFMA f32_16x16x32_bf16 (lane l holds A[l%16][8(l/16) + v],
+// B[8(l/16) + v][l%16] and D[4(l/16) + v][l%16], checked on the device). The weights are BF16
+// already; each FP32 activation or gradient x is split into x_hi = bf16(x) and x_lo =
+// bf16(x − x_hi), and A·x = A·x_hi + A·x_lo (products of two split operands add the
+// hi·lo and lo·hi terms), so the result carries FP32-level accuracy (≈ 2^-16 relative per
+// term) with FP32 accumulation. Training only: the energies of the inner chain and the global
+// moves come from the per-position VALU code, which reproduces them bit for bit.
+typedef __attribute__((__vector_size__(8 * sizeof(short)))) short bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
+
+KDQ_DEV void split_bf16(float x, bf16x8 &hi, bf16x8 &lo, int v) {
+    const uint16_t h = f2bf16(x);
+    hi[v] = (short)h;
+    lo[v] = (short)f2bf16(x - bf16f(h));
+}
+
+// position (x, y, t) of q and its tap k: the same result as tap_pos, with the coordinates
+// decomposed once by the caller and the periodic wrap done by compare (|dx|, |dy| <= 1)
+struct Coord {
+    int x, y, t;
+};
+KDQ_DEV Coord coord_of(const Geo &g, int q) { return Coord{q % g.X, (q / g.X) % g.Y, q / (g.X * g.Y)}; }
+KDQ_DEV int tap_at(const Geo &g, Coord c, int k, bool &ok) {
+    const int tt = c.t + k / 9 - 1;
+    int yy = c.y + (k / 3) % 3 - 1, xx = c.x + k % 3 - 1;
+    yy += yy < 0 ? g.Y : (yy >= g.Y ? -g.Y : 0);
+    xx += xx < 0 ? g.X : (xx >= g.X ? -g.X : 0);
+    ok = tt >= 0 && tt < g.T;
+    return (tt * g.Y + yy) * g.X + xx;
+}
+
+constexpr int MF_NQ = 64;   // positions per workgroup (4 waves x 16)
+
+// The reduction index is tap-major, kk = k·C + c, so each lane's run of 8 consecutive kk shares
+// one tap and reads 8 channels at one position.
+// MODE 0: forward, A[co][k·C + ci] = W[co][ci][k], out = tanh(bias + acc).
+// MODE 1: backward data, A[ci][k·C + co] = W[co][ci][k] (taps mirrored), out = acc · (1 − h²).
+template <int C, int MODE>
+__global__ __launch_bounds__(256) void k_conv_mfma(const float *in, const uint16_t *W, const float *bias,
+                                                   const float *h, float *out, Geo g, int B) {
+    constexpr int K = C * 27, KP = (K + 31) / 32 * 32, MT = C / 16;
+    __shared__ uint16_t As[C][KP + 8];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int b = blockIdx.y, q0 = blockIdx.x * MF_NQ;
+    for (int e = tid; e < C * KP; e += 256) {
+        const int m = e / KP, kk = e % KP;
+        uint16_t v = 0;
+        if (kk < K) {
+            const int k = kk / C, c2 = kk % C;
+            v = MODE == 0 ? W[((size_t)m * C + c2) * 27 + k] : W[((size_t)c2 * C + m) * 27 + k];
+        }
+        As[m][kk] = v;
+    }
+    __syncthreads();
+    const int j = lane & 15, grp = lane >> 4;
+    const int q = q0 + wave * 16 + j;
+    const bool qok = q < g.P;
+    const Coord cq = coord_of(g, qok ? q : 0);
+    const float *src = in + (size_t)b * C * g.P;
+    f32x4 acc[MT];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) acc[mt] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < KP; k0 += 32) {
+        const int kk0 = k0 + 8 * grp, k = kk0 / C, c0 = kk0 % C;
+        bool ok = false;
+        int p = 0;
+        if (qok && kk0 < K) p = tap_at(g, cq, MODE == 0 ? k : 26 - k, ok);
+        bf16x8 bh, bl;
+#pragma unroll
+        for (int v = 0; v < 8; ++v) split_bf16(ok ? src[(size_t)(c0 + v) * g.P + p] : 0.f, bh, bl, v);
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+            bf16x8 a;
+#pragma unroll
+            for (int v = 0; v < 8; ++v) a[v] = (short)As[mt * 16 + j][kk0 + v];
+            acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bh, acc[mt], 0, 0, 0);
+            acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bl, acc[mt], 0, 0, 0);
+        }
+    }
+    if (!qok) return;
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int m = mt * 16 + 4 * grp + v;
+            const size_t o = ((size_t)b * C + m) * g.P + q;
+            if (MODE == 0) {
+                out[o] = tanhf(bias[m] + acc[mt][v]);
+            } else {
+                const float hv = h[o];
+                out[o] = acc[mt][v] * (1.f - hv * hv);
+            }
+        }
+}
+
+// Weight and bias gradient partials of a layer with CO outputs and CI inputs:
+// part[chunk][co][n] = Σ_{positions of the chunk} dout[co][q] x_n(q), with x_n(q) = in[ci][tap(q, k)]
+// for n = ci·27 + k < CI·27 and x_n = 1 for the bias column n = CI·27; M = CO, N = CI·27 + 1 in
+// blocks of 64 columns, K = positions (flattened over the configurations).
+constexpr int WG_CHUNK = 2048, WG_SUB = 256;
+template <int CO, int CI>
+__global__ __launch_bounds__(256) void k_wgrad_mfma(const float *dout, const float *in, float *part, Geo g, int B) {
+    constexpr int N = CI * 27 + 1, MT = CO / 16;
+    // dout of WG_SUB positions, split once and shared by the four waves
+    __shared__ uint16_t Ah[CO][WG_SUB + 8], Al[CO][WG_SUB + 8];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, j = lane & 15, grp = lane >> 4;
+    const int n = blockIdx.x * 64 + wave * 16 + j;   // this lane's B column
+    const bool nok = n < N, isb = n == CI * 27;
+    const int ci = nok && !isb ? n / 27 : 0, k = nok && !isb ? n % 27 : 0;
+    const size_t total = (size_t)B * g.P;
+    f32x4 acc[MT];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) acc[mt] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int s0 = 0; s0 < WG_CHUNK; s0 += WG_SUB) {
+        const size_t sb = (size_t)blockIdx.y * WG_CHUNK + s0;
+        {
+            const size_t qg = sb + tid;   // one position per thread (WG_SUB = blockDim)
+            const bool in_range = qg < total;
+            const int bb = in_range ? (int)(qg / g.P) : 0, qq = in_range ? (int)(qg % g.P) : 0;
+            __syncthreads();   // the previous sub-chunk's reads are done
+            for (int co = 0; co < CO; ++co) {
+                const float x = in_range ? dout[((size_t)bb * CO + co) * g.P + qq] : 0.f;
+                const uint16_t h = f2bf16(x);
+                Ah[co][tid] = h;
+                Al[co][tid] = f2bf16(x - bf16f(h));
+            }
+            __syncthreads();
+        }
+        for (int k0 = 0; k0 < WG_SUB; k0 += 32) {
+            bf16x8 bh, bl;
+            // the lane's 8 consecutive positions: decompose the first, then step x with carries
+            const size_t q0g = sb + k0 + 8 * grp;
+            int b0 = (int)(q0g / g.P), r0 = (int)(q0g % g.P);
+            Coord c = coord_of(g, r0);
+#pragma unroll
+            for (int v = 0; v < 8; ++v) {
+                float x = 0.f;
+                if (q0g + v < total && nok) {
+                    if (isb) {
+                        x = 1.f;
+                    } else {
+                        bool ok;
+                        const int p = tap_at(g, c, k, ok);
+                        if (ok) x = in[((size_t)b0 * CI + ci) * g.P + p];
+                    }
+                }
+                split_bf16(x, bh, bl, v);
+                if (++r0 == g.P) {
+                    r0 = 0;
+                    ++b0;
+                    c = Coord{0, 0, 0};
+                } else if (++c.x == g.X) {
+                    c.x = 0;
+                    if (++c.y == g.Y) {
+                        c.y = 0;
+                        ++c.t;
+                    }
+                }
+            }
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+                bf16x8 ah, al;
+#pragma unroll
+                for (int v = 0; v < 8; ++v) {
+                    ah[v] = (short)Ah[mt * 16 + j][k0 + 8 * grp + v];
+                    al[v] = (short)Al[mt * 16 + j][k0 + 8 * grp + v];
+                }
+                acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc[mt], 0, 0, 0);
+                acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc[mt], 0, 0, 0);
+                acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc[mt], 0, 0, 0);
+            }
+        }
+    }
+    if (!nok) return;
+    float *pc = part + (size_t)blockIdx.y * CO * N;
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) pc[(size_t)(mt * 16 + 4 * grp + v) * N + n] = acc[mt][v];
+}
+
+// Σ over the chunks in double, fixed order: dw[co][n] (n < ncol − 1) and db[co] (n = ncol − 1)
+__global__ void k_wgrad_reduce(const float *part, int nchunk, int cout, int ncol, float *dw, float *db) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= cout * ncol) return;
+    double s = 0.0;
+    for (int c = 0; c < nchunk; ++c) s += part[(size_t)c * cout * ncol + e];
+    const int co = e / ncol, n = e % ncol;
+    if (n < ncol - 1)
+        dw[(size_t)co * (ncol - 1) + n] = (float)s;
+    else
+        db[co] = (float)s;
+}
+
 inline int nb(size_t n, int t) { return (int)((n + t - 1) / t); }
 
 }  // namespace
@@ -399,6 +596,8 @@ struct kdq_net_s {
     float *d_X0 = nullptr, *d_h1 = nullptr, *d_h2 = nullptr, *d_o = nullptr;
     float *d_d2 = nullptr, *d_d1 = nullptr, *d_do = nullptr;
     double *d_part = nullptr, *d_E = nullptr, *d_gb = nullptr;
+    float *d_wpart = nullptr;        // weight-gradient partials of the matrix-core layer 2 (C = 16, 32)
+    int nwchunk = 0;
     uint8_t *d_fields = nullptr;     // [maxB][ns nt] fields uploaded by the host entry points
     uint8_t **d_fptr = nullptr;      // [maxB] device pointers into d_fields
     uint64_t *d_seed = nullptr;      // [maxB] inner-chain seeds of kdq_net_sample
@@ -472,13 +671,21 @@ int net_energy_sum(kdq_net n, int B, hipStream_t st) {
 
 // forward pass of the B configurations whose fields the device table fp points to ->
 // h1, h2, o and E
-int net_forward(kdq_net n, uint8_t *const *fp, int B, hipStream_t st) {
+// (matrix: layer 2 on the BF16 matrix cores, C = 16 or 32; training only)
+int net_forward(kdq_net n, uint8_t *const *fp, int B, hipStream_t st, bool matrix = false) {
     const Geo g = n->g;
     const int C = n->C;
     const NetW w = net_weights(n);
     hipLaunchKernelGGL(k_layer1, dim3(nb((size_t)B * C * g.P, 256)), dim3(256), 0, st, fp, w, n->d_h1, g, n->n, B);
-    hipLaunchKernelGGL(k_conv, dim3(nb((size_t)B * C * g.P, 256)), dim3(256), 0, st, n->d_h1, w.W2, w.b2, n->d_h2, g, B,
-                       C, C, 1);
+    if (matrix && C == 32)
+        hipLaunchKernelGGL((k_conv_mfma<32, 0>), dim3(nb(g.P, MF_NQ), B), dim3(256), 0, st, n->d_h1, w.W2, w.b2,
+                           (const float *)nullptr, n->d_h2, g, B);
+    else if (matrix && C == 16)
+        hipLaunchKernelGGL((k_conv_mfma<16, 0>), dim3(nb(g.P, MF_NQ), B), dim3(256), 0, st, n->d_h1, w.W2, w.b2,
+                           (const float *)nullptr, n->d_h2, g, B);
+    else
+        hipLaunchKernelGGL(k_conv, dim3(nb((size_t)B * C * g.P, 256)), dim3(256), 0, st, n->d_h1, w.W2, w.b2, n->d_h2, g,
+                           B, C, C, 1);
     hipLaunchKernelGGL(k_conv, dim3(nb((size_t)B * g.P, 256)), dim3(256), 0, st, n->d_h2, w.W3, (const float *)nullptr,
                        n->d_o, g, B, C, 1, 0);
     NCHK(hipGetLastError());
@@ -532,7 +739,8 @@ int net_steps(kdq_net n, uint8_t **fp, const uint64_t *d_seed, const uint32_t *d
 int net_grad(kdq_net n, const uint8_t *fields, const double *logw, int B, double *loss) {
     if (int r = net_upload(n, fields, B)) return r;
     if (int r = net_embed(n, B)) return r;
-    if (int r = net_forward(n, n->d_fptr, B, n->stream)) return r;
+    const bool matrix = n->d_wpart != nullptr;
+    if (int r = net_forward(n, n->d_fptr, B, n->stream, matrix)) return r;
     std::vector<double> E(B), gb(B);
     NCHK(hipMemcpyAsync(E.data(), n->d_E, sizeof(double) * B, hipMemcpyDeviceToHost, n->stream));
     NCHK(hipStreamSynchronize(n->stream));
@@ -558,13 +766,40 @@ int net_grad(kdq_net n, const uint8_t *fields, const double *logw, int B, double
     hipLaunchKernelGGL(k_conv_bwd_data, dim3(nb((size_t)B * C * g.P, 256)), dim3(256), 0, n->stream, n->d_do,
                        n->d_wb + n->oW3, n->d_h2, n->d_d2, g, B, C, 1);
     // layer 2
-    hipLaunchKernelGGL(k_conv_bwd_w, dim3(C * 27 + 1, C), dim3(RT), 0, n->stream, n->d_d2, n->d_h1, gr + n->oW2,
-                       gr + n->ob2, g, B, C, C);
-    hipLaunchKernelGGL(k_conv_bwd_data, dim3(nb((size_t)B * C * g.P, 256)), dim3(256), 0, n->stream, n->d_d2,
-                       n->d_wb + n->oW2, n->d_h1, n->d_d1, g, B, C, C);
+    if (matrix) {
+        const int N = C * 27 + 1;
+        const dim3 gw(nb(N, 64), n->nwchunk), gd(nb(g.P, MF_NQ), B);
+        if (C == 32) {
+            hipLaunchKernelGGL((k_wgrad_mfma<32, 32>), gw, dim3(256), 0, n->stream, n->d_d2, n->d_h1, n->d_wpart, g, B);
+            hipLaunchKernelGGL((k_conv_mfma<32, 1>), gd, dim3(256), 0, n->stream, n->d_d2, n->d_wb + n->oW2,
+                               (const float *)nullptr, n->d_h1, n->d_d1, g, B);
+        } else {
+            hipLaunchKernelGGL((k_wgrad_mfma<16, 16>), gw, dim3(256), 0, n->stream, n->d_d2, n->d_h1, n->d_wpart, g, B);
+            hipLaunchKernelGGL((k_conv_mfma<16, 1>), gd, dim3(256), 0, n->stream, n->d_d2, n->d_wb + n->oW2,
+                               (const float *)nullptr, n->d_h1, n->d_d1, g, B);
+        }
+        hipLaunchKernelGGL(k_wgrad_reduce, dim3(nb((size_t)C * N, 256)), dim3(256), 0, n->stream, n->d_wpart,
+                           n->nwchunk, C, N, gr + n->oW2, gr + n->ob2);
+    } else {
+        hipLaunchKernelGGL(k_conv_bwd_w, dim3(C * 27 + 1, C), dim3(RT), 0, n->stream, n->d_d2, n->d_h1, gr + n->oW2,
+                           gr + n->ob2, g, B, C, C);
+        hipLaunchKernelGGL(k_conv_bwd_data, dim3(nb((size_t)B * C * g.P, 256)), dim3(256), 0, n->stream, n->d_d2,
+                           n->d_wb + n->oW2, n->d_h1, n->d_d1, g, B, C, C);
+    }
     // layer 1
-    hipLaunchKernelGGL(k_conv_bwd_w, dim3(CIN * 27 + 1, C), dim3(RT), 0, n->stream, n->d_d1, n->d_X0, gr + n->oW1,
-                       gr + n->ob1, g, B, CIN, C);
+    if (matrix) {
+        const int N = CIN * 27 + 1;
+        const dim3 gw(nb(N, 64), n->nwchunk);
+        if (C == 32)
+            hipLaunchKernelGGL((k_wgrad_mfma<32, CIN>), gw, dim3(256), 0, n->stream, n->d_d1, n->d_X0, n->d_wpart, g, B);
+        else
+            hipLaunchKernelGGL((k_wgrad_mfma<16, CIN>), gw, dim3(256), 0, n->stream, n->d_d1, n->d_X0, n->d_wpart, g, B);
+        hipLaunchKernelGGL(k_wgrad_reduce, dim3(nb((size_t)C * N, 256)), dim3(256), 0, n->stream, n->d_wpart,
+                           n->nwchunk, C, N, gr + n->oW1, gr + n->ob1);
+    } else {
+        hipLaunchKernelGGL(k_conv_bwd_w, dim3(CIN * 27 + 1, C), dim3(RT), 0, n->stream, n->d_d1, n->d_X0, gr + n->oW1,
+                           gr + n->ob1, g, B, CIN, C);
+    }
     NCHK(hipGetLastError());
     return KDQ_OK;
 }
@@ -604,6 +839,10 @@ int kdq_net_create(int32_t L1, int32_t L2, int32_t nt, int32_t channels, uint64_
     r |= nalloc(n, &n->d_fields, B * (size_t)n->n * nt);
     r |= nalloc(n, &n->d_fptr, B); r |= nalloc(n, &n->d_seed, B); r |= nalloc(n, &n->d_chain, B);
     r |= nalloc(n, &n->d_nacc, B);
+    if (C == 16 || C == 32) {
+        n->nwchunk = nb(B * P, WG_CHUNK);
+        r |= nalloc(n, &n->d_wpart, (size_t)n->nwchunk * C * (C * 27 + 1));
+    }
     if (r) {
         nfail(n, "kdq_net_create: device allocation failed");
         delete n;

@@ -71,21 +71,28 @@ def _net_setup(slmc, L1, L2, nt, C, B, scale, seed=7):
     return net, p, geo, f, sn, rng
 
 
-def test_net_energy_and_grad_vs_oracle(mods):
+@pytest.mark.parametrize("C", [4, 16, 32])
+def test_net_energy_and_grad_vs_oracle(mods, C):
+    """C = 16 and 32 train layer 2 on the BF16 matrix cores (split-operand products); every
+    layer's gradient block is checked on its own, so a transposed or mirrored tap in one of
+    the three layer-2 kernels fails here."""
     _, slmc = mods
-    print("case: net E / grad vs numpy, 3x2 lattice nt=6 C=4", flush=True)
-    net, p, geo, f, sn, rng = _net_setup(slmc, 3, 2, 6, 4, 5, 0.3)
+    print(f"case: net E / grad vs numpy, 3x2 lattice nt=6 C={C}", flush=True)
+    net, p, geo, f, sn, rng = _net_setup(slmc, 3, 2, 6, C, 5, 0.3 if C == 4 else 0.1)
     fields = [np.asfortranarray(x) for x in f]
     E = net.energy(fields)
-    pb = sn.bf16_weights(p, 4)          # the kernels read BF16 weights
+    pb = sn.bf16_weights(p, C)          # the kernels read BF16 weights
     assert np.abs(pb - p).max() > 0    # (so the rounding is exercised)
-    Eo = sn.energy(geo, pb, 4, f)
+    Eo = sn.energy(geo, pb, C, f)
     assert np.abs(E - Eo).max() < 1e-4 * np.abs(Eo).max()
     lw = rng.normal(0, 3, len(f))
     loss, g = net.loss_grad(fields, lw)
-    lo, go = sn.loss_grad(geo, pb, 4, f, lw)
+    lo, go = sn.loss_grad(geo, pb, C, f, lw)
     assert abs(loss - lo) < 1e-4 * max(lo, 1.0)
     assert np.linalg.norm(g - go) < 2e-3 * np.linalg.norm(go)
+    for name, (off, size) in sn.param_layout(C)[0].items():
+        sl = slice(off, off + size)
+        assert np.linalg.norm(g[sl] - go[sl]) < 2e-3 * np.linalg.norm(go[sl]), (C, name)
     net.close()
 
 
