@@ -255,6 +255,104 @@ def slmc_leg(m, C, seed0, delay, warmup, sweeps, inner, rounds, channels):
                "double_occupancy_mean": float(D.mean())}
 
 
+def bench_slmc_ess(args):
+    """Self-learning moves at the 15x15 scale (t-U honeycomb N=450, U=5, dtau=0.1, 2Theta=24, 16
+    FP64 chains, 32-channel network): warm-up sweeps, then --slmc-rounds rounds of one sweep
+    of every chain whose (field, exact log W) pairs train the network (--slmc-epochs Adam steps
+    per round on minibatches of the pairs so far; the loss on each round's new pairs before any
+    step on them is the held-out loss), then two recorded legs from the same chains:
+    --slmc-sweeps plain sweeps, and --slmc-sweeps sweeps each followed by one global move of
+    --slmc-inner inner steps (exact FP64 acceptance). Reports per leg τ_int of the double
+    occupancy D and of the antiferromagnetic structure factor (SDW0), and ESS/s from the
+    larger τ_int."""
+    import numpy as np
+    import torch
+    from kapdqmc_amd import slmc
+    from kapdqmc_amd.dqmc import DQMCBatch
+    from kapdqmc_amd.models import tU_Hubbard_Para
+    from kapdqmc_amd.rng import initial_fields
+    C = args.chains
+    channels = args.slmc_channels or 32
+    m = tU_Hubbard_Para(Ht=1.0, Hu1=5.0, Hu2=5.0, Δt=0.1, Θrelax=12.0, Θquench=0.0, Lattice="HoneyComb120",
+                        site=[15, 15], BatchSize=10)
+    N = m.Ns
+    b = DQMCBatch([m] * C, [args.seed + c for c in range(C)], list(range(C)), delay=args.delay or 32)
+    for c in range(C):
+        b.set_fields(c, initial_fields(args.seed + c, c, N, m.Nt))
+    t0 = time.perf_counter()
+    for k in range(args.warmup):
+        b.phy_update(1, False)
+        print(f"bench: slmc-ess warm-up sweep {k + 1} of {args.warmup}", file=sys.stderr, flush=True)
+    net = slmc.EffectiveModel.for_model(m, channels=channels, seed=1, max_batch=C)
+    buf_f, buf_w, losses = [], [], []
+    rng = np.random.default_rng(0)
+    t1 = time.perf_counter()
+    held_out = []
+    for r in range(args.slmc_rounds):
+        b.phy_update(1, False)
+        lw = b.log_weight()
+        new_f = [b.get_fields(c) for c in range(C)]
+        if r:
+            held_out.append(net.loss_grad(new_f, lw)[0])   # the new pairs before any step on them
+        buf_f += new_f
+        buf_w += list(lw)
+        for _ in range(args.slmc_epochs):
+            idx = rng.choice(len(buf_f), size=min(C, len(buf_f)), replace=False)
+            loss = net.train_step([buf_f[i] for i in idx], np.array([buf_w[i] for i in idx]), args.slmc_lr)
+        losses.append(loss)
+        if (r + 1) % 10 == 0 or r + 1 == args.slmc_rounds:
+            print(f"bench: slmc-ess training round {r + 1} of {args.slmc_rounds}, loss {loss:.4g}, held-out "
+                  f"{held_out[-1] if held_out else float('nan'):.4g}", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t1
+
+    def leg(moves):
+        rows, acc = [], []
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        for k in range(args.slmc_sweeps):
+            rows.append(b.phy_update(1, True))
+            if moves:
+                acc.append(float(np.mean(b.slmc_move(net, args.slmc_inner))))
+            print(f"bench: slmc-ess {'moves' if moves else 'plain'} leg sweep {k + 1} of {args.slmc_sweeps}",
+                  file=sys.stderr, flush=True)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - ts
+        rows = np.concatenate(rows, axis=1)
+        taus = {}
+        for name, col, scale in (("D", 1, 1.0 / N), ("SDW0", 4, 1.0)):
+            taus[name], _ = tau_report(rows[:, :, col] * scale)
+        tau = max(taus.values())
+        sw = C * args.slmc_sweeps / dt
+        out = {"chain_sweeps_per_s": sw, "ms_per_sweep_all_chains": dt / args.slmc_sweeps * 1e3,
+               "tau_int": taus, "effective_samples_per_s": sw / (2 * tau),
+               "D_mean": float(rows[:, :, 1].mean() / N), "SDW0_mean": float(rows[:, :, 4].mean())}
+        if moves:
+            out["global_move_acceptance"] = float(np.mean(acc))
+            out["global_move_acceptance_per_sweep"] = acc
+        return out
+
+    plain = leg(False)
+    moved = leg(True)
+    b.close()
+    net.close()
+    print(json.dumps({
+        "metric": "SLMC effective samples/s at 15x15 t-U (moves on vs off)",
+        "value": moved["effective_samples_per_s"], "unit": "ESS/s", "n_gpus": 1, "steps": args.slmc_sweeps,
+        "higher_is_better": True, "dtype": "f64 sweep / bf16-weight fp32 network",
+        "data": "synthetic (Philox-seeded HS fields)",
+        "config": {"workload": f"t-U honeycomb 15x15 N=450 Np=225 U=5 dtau=0.1 2Theta=24 Nt=240 BS=10, {C} FP64 "
+                               f"chains, conv net 3 layers x {channels} channels",
+                   "chains": C, "warmup_sweeps": args.warmup, "training_rounds": args.slmc_rounds,
+                   "training_pairs": len(buf_f), "inner_steps_per_move": args.slmc_inner},
+        "training_s": t_train, "setup_s": t1 - t0,
+        "training_loss_per_round": [float(x) for x in losses[::max(1, len(losses) // 50)]],
+        "held_out_loss_per_round": [float(x) for x in held_out[::max(1, len(held_out) // 50)]],
+        "log_weight_variance_first_round": float(np.var(buf_w[:C])),
+        "plain": plain, "with_moves": moved,
+        "ess_ratio_moves_vs_plain": moved["effective_samples_per_s"] / plain["effective_samples_per_s"]}))
+
+
 def bench_scee(args):
     """Swap-replica Rényi-2 workload (ctrl_SCEEicr, src/tV/SCEE.jl, src/tUV/SCEE.jl): 9 x 9
     honeycomb (N = 162, Np = 81), Δτ = 0.05, 2Θ = 20 (Nt = 400), stabilisation every 10
@@ -480,7 +578,8 @@ def main():
     ap.add_argument("--slmc-sweeps", type=int, default=60,
                     help="sweeps of the mixed+batched+SLMC leg, each followed by a global move (0: skip)")
     ap.add_argument("--slmc-rounds", type=int, default=8, help="training rounds of the SLMC leg's network")
-    ap.add_argument("--slmc-channels", type=int, default=8, help="hidden channels of the SLMC leg's network")
+    ap.add_argument("--slmc-channels", type=int, default=None,
+                    help="hidden channels of the network (SLMC leg: 8, slmc-ess: 32)")
     ap.add_argument("--slmc-leg-inner", type=int, default=1, help="inner steps per global move (SLMC leg)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -493,13 +592,16 @@ def main():
     ap.add_argument("--profile-all", action="store_true", help="time every kernel family (slower)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-sized buffer 5 times (scale for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
-    ap.add_argument("--workload", choices=("tu", "uneven", "scee-tv", "scee-tuv", "slmc", "mixed-curve"),
+    ap.add_argument("--workload", choices=("tu", "uneven", "scee-tv", "scee-tuv", "slmc", "slmc-ess", "mixed-curve"),
                     default="tu",
                     help="tu: the headline t-U sweep (default); uneven: 64 t-U chains of L = 6/9/12 with "
                          "2Theta = 2L, mixed vs FP64; scee-tv / scee-tuv: swap-replica Renyi-2 on 9x9 "
-                         "t-V / t-U-V; slmc: self-learning global moves on 15x15 t-U; mixed-curve: "
+                         "t-V / t-U-V; slmc: self-learning global-move costs on 15x15 t-U; slmc-ess: "
+                         "trained 15x15 moves, tau_int and ESS/s with moves on vs off; mixed-curve: "
                          "mixed-precision error vs recompute interval")
-    ap.add_argument("--slmc-inner", type=int, default=3, help="inner steps per global move (slmc)")
+    ap.add_argument("--slmc-inner", type=int, default=3, help="inner steps per global move (slmc, slmc-ess)")
+    ap.add_argument("--slmc-epochs", type=int, default=8, help="Adam steps per training round (slmc-ess)")
+    ap.add_argument("--slmc-lr", type=float, default=1e-3, help="Adam learning rate (slmc-ess)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only (independent chains shard by chain "
@@ -513,11 +615,13 @@ def main():
         sys.exit(1)
     torch.cuda.set_device(0)
     if args.chains is None:
-        args.chains = {"tu": 64, "uneven": 64, "slmc": 16, "mixed-curve": 16}.get(args.workload, 32)
+        args.chains = {"tu": 64, "uneven": 64, "slmc": 16, "slmc-ess": 16, "mixed-curve": 16}.get(args.workload, 32)
     if args.workload in ("scee-tv", "scee-tuv"):
         return bench_scee(args)
     if args.workload == "slmc":
         return bench_slmc(args)
+    if args.workload == "slmc-ess":
+        return bench_slmc_ess(args)
     if args.workload == "uneven":
         return bench_uneven(args)
     if args.workload == "mixed-curve":
@@ -616,7 +720,7 @@ def main():
     slmc_l = None
     if args.slmc_sweeps > 0:
         Ds, slmc_l = slmc_leg(m, C, args.seed + 2000, bat["delay"], args.warmup, args.slmc_sweeps,
-                              args.slmc_leg_inner, args.slmc_rounds, args.slmc_channels)
+                              args.slmc_leg_inner, args.slmc_rounds, args.slmc_channels or 8)
         d64, ds = D_all.mean(axis=1), Ds.mean(axis=1)
         sig = np.sqrt(d64.var(ddof=1) / C + ds.var(ddof=1) / C)
         slmc_l["double_occupancy_fp64_minus_slmc_in_sigma"] = float((d64.mean() - ds.mean()) / sig)
