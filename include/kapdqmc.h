@@ -108,7 +108,13 @@ typedef struct kdq_run_desc {
 typedef struct kdq_handle_s *kdq_handle;
 
 /* Creates a batch of `nchains` chains on the current HIP device. All chains of a batch
- * must share the model kind, nt and nodes; ns, couplings and seeds may differ.
+ * must share the model kind, precision and delay; ns, couplings, seeds, nt and nodes may
+ * differ. Chains of equal (nt, nodes) advance together in the same launches; a batch with
+ * several (nt, nodes) groups keeps one stream per group, and kdq_phy_update sweeps the
+ * groups concurrently (one host thread each). Chain-indexed calls and per-chain outputs
+ * keep the batch's chain order; the per-slice calls (kdq_wrap, kdq_update_layer,
+ * kdq_measure, kdq_measure_noalloc) and kdq_slmc_move need one nt and return KDQ_EARG on
+ * such a batch. Swap-replica handles (kdq_scee_create, kdq_dop_create) need one nt.
  * Replaces: the buffer set-up at the top of phy_update (src/tU/phy_update.jl:7-37). */
 int kdq_create(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t nchains,
                kdq_handle *out);

@@ -11,11 +11,13 @@
 #include "../../include/kapdqmc.h"
 #include "kdq_kernels.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -131,6 +133,12 @@ struct kdq_handle_s {
     hipGraphExec_t gexec = nullptr;
     int g_record = -1;
     std::vector<void *> rp_buf;               // [nch] replayed draws (kdq_set_replay), nullptr: Philox
+    // A batch whose chains differ in nt or nodes: one sub-handle per (nt, nodes) group, each
+    // with its own stream; this handle only routes calls. gch[g] lists the batch indices of
+    // group g's chains, gmap[c] = (group, index within the group).
+    std::vector<kdq_handle> grp;
+    std::vector<std::vector<int>> gch;
+    std::vector<std::pair<int, int>> gmap;
 
     ~kdq_handle_s() {
         if (stream) hipStreamSynchronize(stream);
@@ -1093,9 +1101,121 @@ const char *kdq_last_error(kdq_handle h) {
 }
 
 int kdq_create(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t nchains, kdq_handle *out) {
-    return create_impl(models, runs, nchains, nullptr, out);
+    if (!models || !runs || !out || nchains < 1) return create_impl(models, runs, nchains, nullptr, out);
+    // group the chains by (nt, nodes); one group is the plain batch
+    std::vector<std::vector<int>> groups;
+    for (int c = 0; c < nchains; ++c) {
+        const kdq_model_desc &m = models[c];
+        if (!m.nodes || m.n_nodes < 1) return create_impl(models, runs, nchains, nullptr, out);   // its own message
+        int g = 0;
+        for (; g < (int)groups.size(); ++g) {
+            const kdq_model_desc &f = models[groups[g][0]];
+            if (f.nt == m.nt && f.n_nodes == m.n_nodes && std::equal(m.nodes, m.nodes + m.n_nodes, f.nodes)) break;
+        }
+        if (g == (int)groups.size()) groups.emplace_back();
+        groups[g].push_back(c);
+    }
+    if (groups.size() == 1) return create_impl(models, runs, nchains, nullptr, out);
+    *out = nullptr;
+    for (int c = 1; c < nchains; ++c) {   // what the groups must still share
+        if (models[c].kind != models[0].kind)
+            return fail(nullptr, KDQ_EARG, "kdq_create: all chains of a batch must share the model kind");
+        if (runs[c].delay != runs[0].delay) return fail(nullptr, KDQ_EARG, "kdq_create: delay must be equal across chains");
+        if (runs[c].precision != runs[0].precision)
+            return fail(nullptr, KDQ_EARG, "kdq_create: precision must be equal across chains");
+    }
+    kdq_handle h = new kdq_handle_s();
+    h->nch = nchains;
+    h->kind = models[0].kind;
+    h->gmap.resize(nchains);
+    for (const auto &grp : groups) {
+        std::vector<kdq_model_desc> gm;
+        std::vector<kdq_run_desc> gr;
+        for (int c : grp) {
+            gm.push_back(models[c]);
+            gr.push_back(runs[c]);
+        }
+        kdq_handle sub = nullptr;
+        if (int r = create_impl(gm.data(), gr.data(), (int)grp.size(), nullptr, &sub)) {
+            kdq_destroy(h);
+            return r;
+        }
+        for (int l = 0; l < (int)grp.size(); ++l) h->gmap[grp[l]] = {(int)h->grp.size(), l};
+        h->grp.push_back(sub);
+        h->gch.push_back(grp);
+    }
+    h->nobs = h->grp[0]->nobs;
+    h->delay = h->grp[0]->delay;
+    *out = h;
+    return KDQ_OK;
 }
 }  // extern "C"
+
+namespace {
+// ---- routing for batches of unequal nt (kdq_create with more than one (nt, nodes) group)
+int g_chain(kdq_handle h, int c, kdq_handle &sub, int &lc, const char *who) {
+    if (c < 0 || c >= h->nch) return fail(h, KDQ_EARG, std::string(who) + ": bad argument");
+    sub = h->grp[h->gmap[c].first];
+    lc = h->gmap[c].second;
+    return KDQ_OK;
+}
+
+template <class F>
+int g_each(kdq_handle h, F f) {
+    for (kdq_handle sub : h->grp)
+        if (int r = f(sub)) return fail(h, r, sub->err);
+    return KDQ_OK;
+}
+
+// per-chain outputs of width w: each group's rows land at its chains' batch indices
+template <class F>
+int g_gather(kdq_handle h, double *out, int w, F f) {
+    for (size_t g = 0; g < h->grp.size(); ++g) {
+        kdq_handle sub = h->grp[g];
+        std::vector<double> buf((size_t)sub->nch * w);
+        if (int r = f(sub, buf.data())) return fail(h, r, sub->err);
+        for (int l = 0; l < sub->nch; ++l)
+            for (int k = 0; k < w; ++k) out[(size_t)h->gch[g][l] * w + k] = buf[(size_t)l * w + k];
+    }
+    return KDQ_OK;
+}
+
+// The groups sweep concurrently, one host thread each (each group replays its own captured
+// sweep on its own stream); rows come back in the batch's chain order.
+int g_phy_update(kdq_handle h, int sweeps, int record, double *rows) {
+    const size_t G = h->grp.size();
+    const int no = h->nobs;
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    std::vector<std::vector<double>> buf(G);
+    std::vector<int> rc(G, KDQ_OK);
+    std::vector<std::thread> th;
+    for (size_t g = 0; g < G; ++g) {
+        if (record) buf[g].resize((size_t)h->grp[g]->nch * sweeps * no);
+        th.emplace_back([&, g] {
+            if (hipSetDevice(dev) != hipSuccess) {
+                rc[g] = fail(h->grp[g], KDQ_EHIP, "kdq_phy_update: hipSetDevice failed in a group's thread");
+                return;
+            }
+            rc[g] = kdq_phy_update(h->grp[g], sweeps, record, record ? buf[g].data() : nullptr);
+        });
+    }
+    for (auto &t : th) t.join();
+    for (size_t g = 0; g < G; ++g)
+        if (rc[g]) return fail(h, rc[g], h->grp[g]->err);
+    if (record)
+        for (size_t g = 0; g < G; ++g)
+            for (int l = 0; l < h->grp[g]->nch; ++l)
+                std::memcpy(rows + (size_t)h->gch[g][l] * sweeps * no, buf[g].data() + (size_t)l * sweeps * no,
+                            sizeof(double) * sweeps * no);
+    return KDQ_OK;
+}
+
+int g_per_slice(kdq_handle h, const char *who) {
+    return fail(h, KDQ_EARG, std::string(who) + ": per-slice calls need one nt; this batch has chains of " +
+                                 std::to_string(h->grp.size()) + " different nt (use one handle per nt)");
+}
+}  // namespace
 
 namespace {
 // Builds a handle over `nchains` entries; streams[c] (null: all 0) is the Philox stream of
@@ -1507,6 +1627,8 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
 
 extern "C" {
 int kdq_destroy(kdq_handle h) {
+    if (h)
+        for (kdq_handle sub : h->grp) kdq_destroy(sub);
     if (h && h->scee) scee_free(h);
     delete h;
     return KDQ_OK;
@@ -1518,6 +1640,13 @@ int kdq_num_obs(kdq_handle h) { return h ? h->nobs : 0; }
 
 int kdq_set_fields(kdq_handle h, int32_t c, const uint8_t *s) {
     if (!h || !s || c < 0 || c >= h->nch) return fail(h, KDQ_EARG, "kdq_set_fields: bad argument");
+    if (!h->grp.empty()) {
+        kdq_handle sub;
+        int l;
+        g_chain(h, c, sub, l, "kdq_set_fields");
+        if (int r = kdq_set_fields(sub, l, s)) return fail(h, r, sub->err);
+        return KDQ_OK;
+    }
     const size_t n = (size_t)elems(h, c) * h->nt;
     for (size_t i = 0; i < n; ++i)
         if (s[i] < 1 || s[i] > 4) return fail(h, KDQ_EARG, "kdq_set_fields: field values must be 1..4");
@@ -1533,6 +1662,7 @@ int kdq_set_fields(kdq_handle h, int32_t c, const uint8_t *s) {
 
 int kdq_get_health(kdq_handle h, double *out) {
     if (!h || !out) return fail(h, KDQ_EARG, "kdq_get_health: bad argument");
+    if (!h->grp.empty()) return g_gather(h, out, 4, [](kdq_handle g, double *o) { return kdq_get_health(g, o); });
     const int n = h->nch;
     std::vector<double> err(n);
     std::vector<int> info(n), qinfo(n), pois(n);
@@ -1554,6 +1684,7 @@ int kdq_get_health(kdq_handle h, double *out) {
 
 int kdq_check_G(kdq_handle h, double *out) {
     if (!h || !out) return fail(h, KDQ_EARG, "kdq_check_G: bad argument");
+    if (!h->grp.empty()) return g_gather(h, out, 2, [](kdq_handle g, double *o) { return kdq_check_G(g, o); });
     if (h->scee) return fail(h, KDQ_EARG, "kdq_check_G: sweep handles only (not swap-replica handles)");
     const int n = h->nch;
     Opnd *G = h->t_G[h->cur];
@@ -1580,6 +1711,13 @@ int kdq_check_G(kdq_handle h, double *out) {
 
 int kdq_get_fields(kdq_handle h, int32_t c, uint8_t *s) {
     if (!h || !s || c < 0 || c >= h->nch) return fail(h, KDQ_EARG, "kdq_get_fields: bad argument");
+    if (!h->grp.empty()) {
+        kdq_handle sub;
+        int l;
+        g_chain(h, c, sub, l, "kdq_get_fields");
+        if (int r = kdq_get_fields(sub, l, s)) return fail(h, r, sub->err);
+        return KDQ_OK;
+    }
     HIPCHK(hipMemcpyAsync(s, h->ch[c].s, (size_t)elems(h, c) * h->nt, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     return KDQ_OK;
@@ -1587,6 +1725,7 @@ int kdq_get_fields(kdq_handle h, int32_t c, uint8_t *s) {
 
 int kdq_initial_s(kdq_handle h) {
     if (!h) return fail(h, KDQ_EARG, "kdq_initial_s: null handle");
+    if (!h->grp.empty()) return g_each(h, [](kdq_handle g) { return kdq_initial_s(g); });
     HIPCHK(launch_initial_s(h->stream, h->nch, h->max_elem, h->nt, h->d_chains));
     HIPCHK(hipStreamSynchronize(h->stream));
     return KDQ_OK;
@@ -1610,6 +1749,7 @@ static int check_replay(kdq_handle h, uint32_t first, uint32_t count, const char
 int kdq_phy_update(kdq_handle h, int32_t sweeps, int32_t record, double *rows) {
     if (!h || sweeps < 0) return fail(h, KDQ_EARG, "kdq_phy_update: bad argument");
     if (record && !rows) return fail(h, KDQ_EARG, "kdq_phy_update: rows must not be NULL when record != 0");
+    if (!h->grp.empty()) return g_phy_update(h, sweeps, record, rows);
     if (int r = check_replay(h, h->sweep, (uint32_t)sweeps, "kdq_phy_update")) return r;
     return phy_update(h, sweeps, record, rows);
 }
@@ -1617,6 +1757,13 @@ int kdq_phy_update(kdq_handle h, int32_t sweeps, int32_t record, double *rows) {
 int kdq_set_replay(kdq_handle h, int32_t c, const uint8_t *choice, const double *u, int64_t nsweeps) {
     if (!h || c < 0 || c >= h->nch || nsweeps < 0 || (nsweeps > 0 && (!choice || !u)))
         return fail(h, KDQ_EARG, "kdq_set_replay: bad argument");
+    if (!h->grp.empty()) {
+        kdq_handle sub;
+        int l;
+        g_chain(h, c, sub, l, "kdq_set_replay");
+        if (int r = kdq_set_replay(sub, l, choice, u, nsweeps)) return fail(h, r, sub->err);
+        return KDQ_OK;
+    }
     if (h->scee) return fail(h, KDQ_EARG, "kdq_set_replay: sweep handles only (not swap-replica handles)");
     if (nsweeps > 0xFFFFFFF) return fail(h, KDQ_EARG, "kdq_set_replay: too many sweeps");
     ChainDev &d = h->cd_host[c];
@@ -1650,6 +1797,13 @@ int kdq_set_replay(kdq_handle h, int32_t c, const uint8_t *choice, const double 
 
 int kdq_get_G(kdq_handle h, int32_t c, double *G) {
     if (!h || !G || c < 0 || c >= h->nch) return fail(h, KDQ_EARG, "kdq_get_G: bad argument");
+    if (!h->grp.empty()) {
+        kdq_handle sub;
+        int l;
+        g_chain(h, c, sub, l, "kdq_get_G");
+        if (int r = kdq_get_G(sub, l, G)) return fail(h, r, sub->err);
+        return KDQ_OK;
+    }
     const int n = h->ch[c].ns;
     const size_t nn = (size_t)n * n;
     std::vector<double> tmp(2 * nn);
@@ -1671,6 +1825,13 @@ int kdq_get_G(kdq_handle h, int32_t c, double *G) {
 
 int kdq_set_G(kdq_handle h, int32_t c, const double *G) {
     if (!h || !G || c < 0 || c >= h->nch) return fail(h, KDQ_EARG, "kdq_set_G: bad argument");
+    if (!h->grp.empty()) {
+        kdq_handle sub;
+        int l;
+        g_chain(h, c, sub, l, "kdq_set_G");
+        if (int r = kdq_set_G(sub, l, G)) return fail(h, r, sub->err);
+        return KDQ_OK;
+    }
     const int n = h->ch[c].ns;
     const size_t nn = (size_t)n * n;
     std::vector<double> tmp(2 * nn);
@@ -1685,6 +1846,7 @@ int kdq_set_G(kdq_handle h, int32_t c, const double *G) {
 }
 
 int kdq_wrap(kdq_handle h, int32_t lt, int32_t forward) {
+    if (h && !h->grp.empty()) return g_per_slice(h, "kdq_wrap");
     if (!h || lt < 1 || lt > h->nt) return fail(h, KDQ_EARG, "kdq_wrap: bad argument");
     if (int r = wrap_slice(h, h->t_G[h->cur], h->t_T, lt, forward != 0, KDQ_PROF_WRAP)) return r;
     HIPCHK(hipStreamSynchronize(h->stream));
@@ -1692,6 +1854,7 @@ int kdq_wrap(kdq_handle h, int32_t lt, int32_t forward) {
 }
 
 int kdq_update_layer(kdq_handle h, int32_t lt, int32_t dir, uint32_t sweep) {
+    if (h && !h->grp.empty()) return g_per_slice(h, "kdq_update_layer");
     if (!h || lt < 1 || lt > h->nt || (dir != 0 && dir != 1)) return fail(h, KDQ_EARG, "kdq_update_layer: bad argument");
     if (int r = check_replay(h, sweep, 1, "kdq_update_layer")) return r;
     if (h->kind == KDQ_MODEL_TV) {
@@ -1708,6 +1871,13 @@ int kdq_G4(kdq_handle h, int32_t c, int32_t tau1, int32_t tau2, double *G1, doub
            double *G21) {
     if (!h || h->scee || c < 0 || c >= h->nch || !G1 || !G2 || !G12 || !G21)
         return fail(h, KDQ_EARG, "kdq_G4: bad argument");
+    if (!h->grp.empty()) {
+        kdq_handle sub;
+        int l;
+        g_chain(h, c, sub, l, "kdq_G4");
+        if (int r = kdq_G4(sub, l, tau1, tau2, G1, G2, G12, G21)) return fail(h, r, sub->err);
+        return KDQ_OK;
+    }
     if (h->kind != KDQ_MODEL_TU) return fail(h, KDQ_EARG, "kdq_G4: time-displaced G is for t-U chains");
     if (tau1 < 0 || tau1 > h->nt || tau2 < 0 || tau2 > h->nt) return fail(h, KDQ_EARG, "kdq_G4: slice out of range");
     const bool swap = tau1 < tau2;
@@ -1742,11 +1912,13 @@ int kdq_G4(kdq_handle h, int32_t c, int32_t tau1, int32_t tau2, double *G1, doub
 
 int kdq_init_chain(kdq_handle h) {
     if (!h) return fail(h, KDQ_EARG, "kdq_init_chain: null handle");
+    if (!h->grp.empty()) return g_each(h, [](kdq_handle g) { return kdq_init_chain(g); });
     if (int r = init_chain(h)) return r;
     return check_health(h);
 }
 
 int kdq_measure(kdq_handle h, int32_t lt, double *out) {
+    if (h && !h->grp.empty()) return g_per_slice(h, "kdq_measure");
     if (!h || !out || lt < 1 || lt > h->nt) return fail(h, KDQ_EARG, "kdq_measure: bad argument");
     HIPCHK(launch_zero_meas(h->stream, h->nch, h->d_chains));
     if (int r = measure(h, lt)) return r;
@@ -1827,6 +1999,7 @@ __global__ void k_noalloc_layout(const ChainDev *chains, int nch, unsigned char 
 extern "C" {
 
 int kdq_measure_noalloc(kdq_handle h, int32_t lt, double *out) {
+    if (h && !h->grp.empty()) return g_per_slice(h, "kdq_measure_noalloc");
     if (!h || !out || lt < 1 || lt > h->nt) return fail(h, KDQ_EARG, "kdq_measure_noalloc: bad argument");
     if (h->scee) return fail(h, KDQ_EARG, "kdq_measure_noalloc: sweep handles only (not swap-replica handles)");
     int dev = -1, sdev = -1;
@@ -1862,6 +2035,7 @@ int kdq_measure_noalloc(kdq_handle h, int32_t lt, double *out) {
 
 int kdq_profile(kdq_handle h, uint32_t mask) {
     if (!h) return fail(h, KDQ_EARG, "kdq_profile: null handle");
+    if (!h->grp.empty()) return g_each(h, [mask](kdq_handle g) { return kdq_profile(g, mask); });
     harvest(h);
     h->prof_mask = mask;
     for (int f = 0; f < KDQ_PROF_FAMILIES; ++f) {
@@ -1873,6 +2047,22 @@ int kdq_profile(kdq_handle h, uint32_t mask) {
 
 int kdq_profile_read(kdq_handle h, double *ms, int64_t *launches) {
     if (!h || !ms || !launches) return fail(h, KDQ_EARG, "kdq_profile_read: bad argument");
+    if (!h->grp.empty()) {   // summed over the groups
+        for (int f = 0; f < KDQ_PROF_FAMILIES; ++f) {
+            ms[f] = 0.0;
+            launches[f] = 0;
+        }
+        return g_each(h, [&](kdq_handle g) {
+            double m[KDQ_PROF_FAMILIES];
+            int64_t n[KDQ_PROF_FAMILIES];
+            if (int r = kdq_profile_read(g, m, n)) return r;
+            for (int f = 0; f < KDQ_PROF_FAMILIES; ++f) {
+                ms[f] += m[f];
+                launches[f] += n[f];
+            }
+            return (int)KDQ_OK;
+        });
+    }
     HIPCHK(hipStreamSynchronize(h->stream));
     harvest(h);
     for (int f = 0; f < KDQ_PROF_FAMILIES; ++f) {
@@ -1884,16 +2074,21 @@ int kdq_profile_read(kdq_handle h, double *ms, int64_t *launches) {
 
 int kdq_set_option(kdq_handle h, int32_t key, int32_t value) {
     if (!h) return fail(h, KDQ_EARG, "kdq_set_option: null handle");
+    if (!h->grp.empty()) {
+        if (int r = g_each(h, [key, value](kdq_handle g) { return kdq_set_option(g, key, value); })) return r;
+        if (key == KDQ_OPT_DELAY) h->delay = value;
+        return KDQ_OK;
+    }
     switch (key) {
         case KDQ_OPT_GRAPH:
             h->use_graph = value ? 1 : 0;
             break;
         case KDQ_OPT_DELAY: {
-            const bool tv_scee = h->scee && h->kind == KDQ_MODEL_TV;
-            const int kmax = h->scee ? (tv_scee ? 8 : 16) : 64;
-            if ((h->kind != KDQ_MODEL_TU && !tv_scee) || value < 1 || value > kmax)
+            const bool r2_scee = h->scee && h->kind != KDQ_MODEL_TU;
+            const int kmax = h->scee ? (r2_scee ? 8 : 16) : 64;
+            if ((h->kind != KDQ_MODEL_TU && !r2_scee) || value < 1 || value > kmax)
                 return fail(h, KDQ_EARG, "kdq_set_option: delay is adjustable for t-U chains (1..64, 1..16 with "
-                                         "replicas) and t-V replica chains (1..8 bonds)");
+                                         "replicas) and t-V / t-U-V replica chains (1..8 bonds or elements)");
             h->delay = value;
             if (h->scee) scee_set_kb(h, value);
             break;
@@ -1918,6 +2113,8 @@ int kdq_set_option(kdq_handle h, int32_t key, int32_t value) {
 
 int kdq_get_precision_error(kdq_handle h, double *out) {
     if (!h || !out) return fail(h, KDQ_EARG, "kdq_get_precision_error: bad argument");
+    if (!h->grp.empty())
+        return g_gather(h, out, 1, [](kdq_handle g, double *o) { return kdq_get_precision_error(g, o); });
     if (!h->mixed) {
         for (int c = 0; c < h->nch; ++c) out[c] = 0.0;
         return KDQ_OK;
@@ -1929,6 +2126,7 @@ int kdq_get_precision_error(kdq_handle h, double *out) {
 }
 
 int kdq_slmc_move(kdq_handle h, kdq_net net, int32_t nsteps, double *accepted) {
+    if (h && !h->grp.empty()) return fail(h, KDQ_EARG, "kdq_slmc_move: the network takes one lattice and nt");
     if (!h || !net || !accepted || nsteps < 0 || h->kind != KDQ_MODEL_TU || h->scee)
         return fail(h, KDQ_EARG, "kdq_slmc_move: t-U batches with a network only");
     int nsn = 0, ntn = 0;
@@ -2005,12 +2203,14 @@ int kdq_slmc_move(kdq_handle h, kdq_net net, int32_t nsteps, double *accepted) {
 
 int kdq_log_weight(kdq_handle h, double *out) {
     if (!h || !out || h->kind != KDQ_MODEL_TU || h->scee) return fail(h, KDQ_EARG, "kdq_log_weight: t-U batches only");
+    if (!h->grp.empty()) return g_gather(h, out, 1, [](kdq_handle g, double *o) { return kdq_log_weight(g, o); });
     if (int r = log_weight(h, out)) return r;
     return check_health(h);
 }
 
 int kdq_get_logdet(kdq_handle h, double *out) {
     if (!h || !out) return fail(h, KDQ_EARG, "kdq_get_logdet: bad argument");
+    if (!h->grp.empty()) return g_gather(h, out, 1, [](kdq_handle g, double *o) { return kdq_get_logdet(g, o); });
     HIPCHK(hipMemcpyAsync(out, h->d_logdet, sizeof(double) * h->nch, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     return KDQ_OK;
@@ -2018,6 +2218,7 @@ int kdq_get_logdet(kdq_handle h, double *out) {
 
 int kdq_get_counters(kdq_handle h, double *out) {
     if (!h || !out) return fail(h, KDQ_EARG, "kdq_get_counters: bad argument");
+    if (!h->grp.empty()) return g_gather(h, out, 2, [](kdq_handle g, double *o) { return kdq_get_counters(g, o); });
     for (int c = 0; c < h->nch; ++c)
         HIPCHK(hipMemcpyAsync(out + 2 * c, h->ch[c].counters, 2 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
@@ -2026,6 +2227,7 @@ int kdq_get_counters(kdq_handle h, double *out) {
 
 int kdq_get_near_ties(kdq_handle h, double *out) {
     if (!h || !out) return fail(h, KDQ_EARG, "kdq_get_near_ties: bad argument");
+    if (!h->grp.empty()) return g_gather(h, out, 1, [](kdq_handle g, double *o) { return kdq_get_near_ties(g, o); });
     for (int c = 0; c < h->nch; ++c)
         HIPCHK(hipMemcpyAsync(out + c, h->ch[c].counters + 2, sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
@@ -2262,19 +2464,21 @@ int scee_layer(kdq_handle h, int lt, int dir, uint32_t sweep) {
     return KDQ_OK;
 }
 
-// bonds per delayed block of the t-V replica decisions (0: element by element): 8, or 4,
-// as the handle's delay allows and the block's LDS fits
+// bonds (t-V) or elements (t-U-V) per delayed block of the replica decisions (0: element by
+// element): 8, or 4, as the handle's delay allows and the block's LDS fits
 int scee_r2_block(kdq_handle h) {
     const SceeState *S = h->scee;
-    if (h->kind != KDQ_MODEL_TV) return 0;
-    for (int kb : {8, 4})
-        if (S->kb >= kb && scee_r2_block_lds_bytes(kb, S->nx_max) <= 150 * 1024) return kb;
+    for (int kb : {8, 4}) {
+        if (S->kb < kb) continue;
+        if (h->kind == KDQ_MODEL_TV && scee_r2_block_lds_bytes(kb, S->nx_max) <= 150 * 1024) return kb;
+        if (h->kind == KDQ_MODEL_TUV && scee_r2_block_uv_lds_bytes(kb, S->nx_max) <= 150 * 1024) return kb;
+    }
     return 0;
 }
 
-// UpdateSCEELayer! of the rank-2 models: t-V bond family `family`, or every element (-1).
-// t-V: delayed blocks of bonds, each followed by the four-matrix flush; otherwise (t-U-V, or
-// delay below 4) the element kernel updates the four matrices in place
+// UpdateSCEELayer! of the rank-2 models: t-V bond family `family`, or every element (-1, t-U-V).
+// Delayed blocks of bonds (t-V) or elements (t-U-V), each followed by the four-matrix flush;
+// with a delay below 4 the element kernel updates the four matrices in place
 int scee_layer_r2(kdq_handle h, int lt, int dir, uint32_t sweep, int family) {
     SceeState *S = h->scee;
     SceeR2Args a{};
@@ -2290,9 +2494,10 @@ int scee_layer_r2(kdq_handle h, int lt, int dir, uint32_t sweep, int family) {
     a.sweep = sweep;
     a.nx_max = S->nx_max;
     a.max_ns = h->max_ns;
-    const int kb = family >= 0 ? scee_r2_block(h) : 0;
+    const int kb = scee_r2_block(h);
     if (kb) {
-        for (int s0 = 0; s0 < h->max_ns / 2; s0 += kb) {
+        const bool uv = family < 0;   // t-U-V: blocks of consecutive elements of the slice
+        for (int s0 = 0; s0 < (uv ? h->max_elem : h->max_ns / 2); s0 += kb) {
             a.s0 = s0;
             a.Gt = h->t_G[h->cur];
             a.Gt0 = xcur(h, SX_GT0);
@@ -2300,7 +2505,8 @@ int scee_layer_r2(kdq_handle h, int lt, int dir, uint32_t sweep, int family) {
             a.G0 = xcur(h, SX_G0);
             {
                 ProfScope ps(h, KDQ_PROF_DECIDE);
-                HIPCHK(launch_scee_r2_block(h->stream, a, kb));
+                if (uv) HIPCHK(launch_scee_r2_block_uv(h->stream, a, kb));
+                else HIPCHK(launch_scee_r2_block(h->stream, a, kb));
             }
             if (int r = scee_flush(h, 2 * kb)) return r;
         }

@@ -258,6 +258,8 @@ size_t scee_r2_lds_bytes(int nx_max, int max_ns);
 // LDS does not fit (then the element kernel is used)
 hipError_t launch_scee_r2_block(hipStream_t st, const SceeR2Args &a, int kb);
 size_t scee_r2_block_lds_bytes(int kb, int nx_max);
+hipError_t launch_scee_r2_block_uv(hipStream_t st, const SceeR2Args &a, int kb);
+size_t scee_r2_block_uv_lds_bytes(int kb, int nx_max);
 // X[i][i] += v for the per-chain n x n matrices (dimension code ndim); set = 1 writes I.
 hipError_t launch_diag(hipStream_t st, int nchains, int max_n, const ChainDev *chains, const Opnd *X, int ndim,
                        double v, int set);

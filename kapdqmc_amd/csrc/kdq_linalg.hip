@@ -55,26 +55,42 @@ __global__ void k_to_f32(const ChainDev *chains, const Opnd *src, const OpndF *d
 // Health check at a stabilisation node: max |wrapped - fresh| per chain into maxerr (the
 // reference's commented wrap-error check, src/tU/phy_update.jl:54-58,92-97) and a sticky flag
 // when either matrix holds a non-finite entry. Also used for max |G² - G| (wrapped = G², fresh = G).
+// One workgroup per 64 x 32 tile (each wave walks 8 columns), one atomic per workgroup.
+constexpr int NC_COLS = 32;
 __global__ void k_node_check(const ChainDev *chains, const Opnd *wrapped, const Opnd *fresh, int ndim,
                              double *maxerr, int *poison) {
     const int c = blockIdx.z;
     const int n = dim_of(chains[c], ndim);
-    const int i = blockIdx.x * 64 + (threadIdx.x & 63), j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int i = blockIdx.x * 64 + lane;
     double err = 0.0;
     int bad = 0;
-    if (i < n && j < n) {
+    if (i < n) {
         const Opnd w = wrapped[c], f = fresh[c];
-        const size_t ow = (size_t)i + (size_t)j * w.ld, of = (size_t)i + (size_t)j * f.ld;
-        const double wr = w.re[ow], wi = w.im[ow], fr = f.re[of], fi = f.im[of];
-        bad = !(isfinite(wr) && isfinite(wi) && isfinite(fr) && isfinite(fi));
-        err = bad ? 0.0 : fmax(fabs(wr - fr), fabs(wi - fi));
+        for (int jj = wave; jj < NC_COLS; jj += 4) {
+            const int j = blockIdx.y * NC_COLS + jj;
+            if (j >= n) break;
+            const size_t ow = (size_t)i + (size_t)j * w.ld, of = (size_t)i + (size_t)j * f.ld;
+            const double wr = w.re[ow], wi = w.im[ow], fr = f.re[of], fi = f.im[of];
+            const bool b = !(isfinite(wr) && isfinite(wi) && isfinite(fr) && isfinite(fi));
+            bad |= b;
+            if (!b) err = fmax(err, fmax(fabs(wr - fr), fabs(wi - fi)));
+        }
     }
     err = wave_max(err);
     const unsigned long long anybad = __ballot(bad);
-    if ((threadIdx.x & 63) == 0) {
-        if (err > 0.0)
-            atomicMax(reinterpret_cast<unsigned long long *>(maxerr + c), (unsigned long long)__double_as_longlong(err));
-        if (anybad && poison) atomicOr(poison + c, 1);
+    __shared__ double se[4];
+    __shared__ int sb[4];
+    if (lane == 0) {
+        se[wave] = err;
+        sb[wave] = anybad != 0;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double e = fmax(fmax(se[0], se[1]), fmax(se[2], se[3]));
+        if (e > 0.0)
+            atomicMax(reinterpret_cast<unsigned long long *>(maxerr + c), (unsigned long long)__double_as_longlong(e));
+        if ((sb[0] | sb[1] | sb[2] | sb[3]) && poison) atomicOr(poison + c, 1);
     }
 }
 
@@ -147,7 +163,7 @@ hipError_t launch_to_f32(hipStream_t st, int nchains, int max_n, const ChainDev 
 
 hipError_t launch_node_check(hipStream_t st, int nchains, int max_n, const ChainDev *chains, const Opnd *wrapped,
                              const Opnd *fresh, int ndim, double *maxerr, int *poison) {
-    dim3 grid((max_n + 63) / 64, (max_n + 3) / 4, nchains);
+    dim3 grid((max_n + 63) / 64, (max_n + NC_COLS - 1) / NC_COLS, nchains);
     hipLaunchKernelGGL(k_node_check, grid, dim3(256), 0, st, chains, wrapped, fresh, ndim, maxerr, poison);
     return hipGetLastError();
 }

@@ -1422,6 +1422,534 @@ __global__ __launch_bounds__(BT) void k_scee_r2_block(SceeR2Args a) {
 #undef O0
 }
 
+// ---- delayed blocks of the t-U-V rank-2 decisions (UpdateSCEELayer!, src/tUV/SCEE.jl:355-402)
+// A block holds KB consecutive elements (directed bonds) of a slice. Consecutive elements share
+// sites, so the block's sites S are the distinct sites of its elements (at most 2·KB), and one
+// site may be changed more than once. The decisions are those of k_scee_r2 element by element,
+// replica 1 then 2, from the working blocks Gt[S,S], Gt0[S,X], G0t[X,S] (LDS), the region
+// blocks M[r][X] and gmInv, all kept current with the sequential rank-2 updates. The four full
+// matrices are left to the four-matrix flush, whose form G ← G0 − G0[:,A] W (E_A − G0[A,:])
+// (Big-matrix rows and columns as in k_scee_r2_block) holds for any sequence of rank-2
+// updates, repeated sites included: with A the accepted elements' site pairs in acceptance
+// order (repeats kept), G_(t-1)[:,a] = G0[:,(A,a)] [−x; I] and E_a − G_(t-1)[a,:] =
+// [y, I] (E_(A,a) − G0[(A,a),:]), where x = W (E_A[:,a] − G0[A,a]) and y = G0[a,A] W are read
+// from the block-start copy G0[S,S], so W' = [[W − x w y, −x w], [w y, w]] with w = r⁻¹Δ.
+template <int KB>
+__global__ __launch_bounds__(BT) void k_scee_r2_block_uv(SceeR2Args a) {
+    constexpr int KS = 2 * KB;
+    const int c = blockIdx.x, nch = a.nch, tid = threadIdx.x;
+    const SceeRegion &R = a.reg[c];
+    const ChainDev &ch0 = a.chains[c];
+    const int nA = R.nA, nB = R.nB, nX = nA + nB, NX = max(a.nx_max, 1);
+    const int lt = a.lt, mA = nA * nA, mAB = mA + nB * nB;
+    const int e0 = a.s0;
+    const int kb = min(KB, ch0.n_elem - a.s0);
+    if (kb <= 0) {
+        if (tid < 2) *a.chains[c + tid * nch].acc_cnt = 0;
+        return;
+    }
+    extern __shared__ double lds[];
+    double *gtr = lds, *gti = gtr + 2 * KS * KS;                 // Gt[S,S] current [r][i][j]
+    double *g0r = gti + 2 * KS * KS, *g0i = g0r + 2 * KS * KS;  // Gt[S,S] at the block start
+    double *t0r = g0i + 2 * KS * KS, *t0i = t0r + 2 * KS * NX;  // Gt0[S,X] [r][i][x]
+    double *o0r = t0i + 2 * KS * NX, *o0i = o0r + 2 * KS * NX;  // G0t[X,S] [r][x][i]
+    double *zr = o0i + 2 * KS * NX, *zi = zr + 2 * NX;          // zN [t][q]
+    double *ar_ = zi + 2 * NX, *ai_ = ar_ + 2 * NX;             // a [u][q]
+    double *br_ = ai_ + 2 * NX, *bi_ = br_ + 2 * NX;            // b [t][q]
+    double *nr_ = bi_ + 2 * NX, *ni_ = nr_ + 2 * NX;            // gmInv a [u][q]
+    double *wr_ = ni_ + 2 * NX, *wi_ = wr_ + 2 * NX;            // Tau⁻¹ b [t][q]
+    double *cxr = wi_ + 2 * NX, *cxi = cxr + 2 * NX;            // G0t[X, s_t] [t][x]
+    double *rxr = cxi + 2 * NX, *rxi = rxr + 2 * NX;            // Gt0[s_t, X] [t][x]
+    double *csr = rxi + 2 * NX, *csi = csr + 2 * KS;            // Gt[S, s_t]  [t][i]
+    double *rsr = csi + 2 * KS, *rsi = rsr + 2 * KS;            // Gt[s_t, S]  [t][j]
+    double *xr = rsi + 2 * KS, *xi = xr + 2 * KS;               // x [t][k] (W rows x 2)
+    double *yr = xi + 2 * KS, *yi = yr + 2 * KS;                // y [k][u] (2 x W columns)
+    double *Wr = yi + 2 * KS, *Wi = Wr + 2 * KS * KS;           // W [r][t][u]
+    double *red = Wi + 2 * KS * KS;                             // BT/64 x 16
+    __shared__ double dgr[2][KB][2], dgi[2][KB][2];             // diagonal Δ of each element's proposal
+    __shared__ double pgam[2][KB], pu[2][KB];                   // γ'/γ, uniform draw
+    __shared__ int sv[2][KB], site[KS], loc[KB][2], accl[2][KS];
+    __shared__ int nsl_s;
+    __shared__ double rmr[2][2], rmi[2][2], tinv[2][2][2][2], prob_s;
+    __shared__ int acc_s;
+#define GT(r, i, j) ((r) * KS * KS + (i) * KS + (j))
+#define T0(r, i, x) ((r) * KS * NX + (i) * NX + (x))
+#define O0(r, x, i) ((r) * NX * KS + (x) * KS + (i))
+    const uint32_t tag = (uint32_t)lt | ((uint32_t)a.dir << 31);
+    const uint32_t sweep = a.sweep_dev ? *a.sweep_dev : a.sweep;
+    if (tid == 0) {   // distinct sites of the block, in first-appearance order
+        int ns = 0;
+        for (int q = 0; q < kb; ++q)
+            for (int k = 0; k < 2; ++k) {
+                const int sg = k ? ch0.ey[e0 + q] : ch0.ex[e0 + q];
+                int j = 0;
+                while (j < ns && site[j] != sg) ++j;
+                if (j == ns) site[ns++] = sg;
+                loc[q][k] = j;
+            }
+        for (int j = ns; j < KS; ++j) site[j] = 0;
+        nsl_s = ns;
+    }
+    if (tid < 2 * KB) {
+        const int r = tid / KB, q = tid % KB;
+        if (q < kb) {
+            const ChainDev &chr = a.chains[c + r * nch];
+            const int el = e0 + q;
+            int chv;
+            double u;
+            proposal_draw(chr.seed, chr.chain_id, chr.stream, sweep, tag, (uint32_t)el, chv, u);
+            const int si = chr.s[(size_t)el + (size_t)(lt - 1) * chr.n_elem];
+            const int sx = chv + 1 + ((chv + 1) >= si ? 1 : 0);
+            // Δ = diag(e^{ds} − 1, e^{a ds} − 1), phases e^{i v} for the complex case
+            const double ds = chr.eta[sx - 1] - chr.eta[si - 1], v[2] = {ds, chr.a * ds};
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                if (chr.is_complex) {
+                    double sn, cs;
+                    sincos(v[k], &sn, &cs);
+                    dgr[r][q][k] = cs - 1.0;
+                    dgi[r][q][k] = sn;
+                } else {
+                    dgr[r][q][k] = exp(v[k]) - 1.0;
+                    dgi[r][q][k] = 0.0;
+                }
+            }
+            pgam[r][q] = chr.gamma[sx - 1] / chr.gamma[si - 1];
+            pu[r][q] = u;
+            sv[r][q] = si | (sx << 8);
+        }
+    }
+    __syncthreads();
+    const int nsl = nsl_s;
+    for (int e = tid; e < 2 * KS * KS; e += BT) {
+        const int r = e / (KS * KS), i = (e / KS) % KS, j = e % KS;
+        double vr = 0.0, vi = 0.0;
+        if (i < nsl && j < nsl) {
+            const Opnd G = a.Gt[c + r * nch];
+            const size_t o = (size_t)site[i] + (size_t)site[j] * G.ld;
+            vr = G.re[o];
+            vi = G.im[o];
+        }
+        gtr[e] = vr;
+        gti[e] = vi;
+        g0r[e] = vr;
+        g0i[e] = vi;
+        Wr[e] = 0.0;
+        Wi[e] = 0.0;
+    }
+    for (int e = tid; e < 2 * KS * nX; e += BT) {
+        const int r = e / (KS * nX), rem = e % (KS * nX), x = rem / KS, i = rem % KS;
+        double tr = 0.0, ti = 0.0, orr = 0.0, oi = 0.0;
+        if (i < nsl) {
+            const Opnd T = a.Gt0[c + r * nch], O = a.G0t[c + r * nch];
+            const int sx = R.idx[x];
+            const size_t ot = (size_t)site[i] + (size_t)sx * T.ld, oo = (size_t)sx + (size_t)site[i] * O.ld;
+            tr = T.re[ot]; ti = T.im[ot];
+            orr = O.re[oo]; oi = O.im[oo];
+        }
+        t0r[T0(r, i, x)] = tr; t0i[T0(r, i, x)] = ti;
+        o0r[O0(r, x, i)] = orr; o0i[O0(r, x, i)] = oi;
+    }
+    for (int e = tid; e < 2 * mAB; e += BT) {
+        const int r = e / mAB, f = e % mAB;
+        const int X = f < mA ? 0 : 1, nr = X ? nB : nA, base = X ? nA : 0, g = X ? f - mA : f;
+        const int p = g % nr, q = g / nr;
+        const Opnd G0 = a.G0[c + r * nch];
+        const size_t o = (size_t)R.idx[base + p] + (size_t)R.idx[base + q] * G0.ld;
+        double *M = R.M[r][X];
+        M[p + (size_t)q * nr] = G0.re[o];
+        M[(size_t)nr * nr + p + (size_t)q * nr] = G0.im[o];
+    }
+    __syncthreads();
+
+    int na[2] = {0, 0}, nties[2] = {0, 0};
+    for (int qb = 0; qb < kb; ++qb) {
+        const int l0 = loc[qb][0], l1 = loc[qb][1];
+        auto ls = [&](int k) { return k ? l1 : l0; };
+        for (int r = 0; r < 2; ++r) {
+            const int o = 1 - r;
+            if (tid == 0) {
+                // rr = I + Δ (I - Gt[sub, sub]) = (I + Δ) - Δ Gt[sub, sub] (Δ diagonal; the
+                // element kernel's order of operations); w = rr⁻¹ Δ
+                double rr[2][2], ri[2][2];
+#pragma unroll
+                for (int p = 0; p < 2; ++p)
+#pragma unroll
+                    for (int q = 0; q < 2; ++q) {
+                        const double sr = p == q ? dgr[r][qb][p] + 1.0 : 0.0, sim = p == q ? dgi[r][qb][p] : 0.0;
+                        double tr, ti;
+                        cmul(dgr[r][qb][p], dgi[r][qb][p], gtr[GT(r, ls(p), ls(q))], gti[GT(r, ls(p), ls(q))], tr, ti);
+                        rr[p][q] = sr - tr;
+                        ri[p][q] = sim - ti;
+                    }
+                double ivr[2][2], ivi[2][2], dr, di;
+                c22_inv(rr, ri, ivr, ivi, dr, di);
+#pragma unroll
+                for (int p = 0; p < 2; ++p)
+#pragma unroll
+                    for (int q = 0; q < 2; ++q) cmul(ivr[p][q], ivi[p][q], dgr[r][qb][q], dgi[r][qb][q], rmr[p][q], rmi[p][q]);
+                const double pw = a.kind == 1 ? dr : dr * dr + di * di;
+                prob_s = pw * pgam[r][qb];
+                acc_s = 0;
+            }
+            __syncthreads();
+            // ---- Grover factors of both regions (as k_scee_r2, from the working blocks)
+            for (int it = tid; it < 2 * nX; it += BT) {
+                const int t = it / nX, q = it % nX;
+                const int X = q < nA ? 0 : 1, k0 = X ? nA : 0, k1 = X ? nX : nA, nr = X ? nB : nA;
+                const double *Mo = R.M[o][X], *Moi = Mo + (size_t)nr * nr;
+                const int ql = q - k0;
+                double sr = 0.0, sim = 0.0, pr = 0.0, pi = 0.0;
+                if (r == 0) {
+#pragma unroll 8
+                    for (int k = k0; k < k1; ++k) {
+                        const int kl = k - k0;
+                        const double mr = 2.0 * Mo[kl + (size_t)ql * nr] - (k == q ? 1.0 : 0.0);
+                        const double mi = 2.0 * Moi[kl + (size_t)ql * nr];
+                        double tr, ti;
+                        cmul(t0r[T0(r, ls(t), k)], t0i[T0(r, ls(t), k)], mr, mi, tr, ti);
+                        sr += tr;
+                        sim += ti;
+                    }
+#pragma unroll
+                    for (int k = 0; k < 2; ++k) {
+                        double tr, ti;
+                        cmul(o0r[O0(r, q, ls(k))], o0i[O0(r, q, ls(k))], rmr[k][t], rmi[k][t], tr, ti);
+                        pr += tr;
+                        pi += ti;
+                    }
+                } else {
+#pragma unroll 8
+                    for (int k = k0; k < k1; ++k) {
+                        const int kl = k - k0;
+                        const double mr = 2.0 * Mo[ql + (size_t)kl * nr] - (k == q ? 1.0 : 0.0);
+                        const double mi = 2.0 * Moi[ql + (size_t)kl * nr];
+                        double tr, ti;
+                        cmul(mr, mi, o0r[O0(r, k, ls(t))], o0i[O0(r, k, ls(t))], tr, ti);
+                        pr += tr;
+                        pi += ti;
+                    }
+#pragma unroll
+                    for (int k = 0; k < 2; ++k) {
+                        double tr, ti;
+                        cmul(rmr[t][k], rmi[t][k], t0r[T0(r, ls(k), q)], t0i[T0(r, ls(k), q)], tr, ti);
+                        sr += tr;
+                        sim += ti;
+                    }
+                }
+                zr[t * NX + q] = sr;
+                zi[t * NX + q] = sim;
+                ar_[t * NX + q] = pr;
+                ai_[t * NX + q] = pi;
+            }
+            __syncthreads();
+            for (int it = tid; it < 2 * nX; it += BT) {
+                const int t = it / nX, q = it % nX;
+                const int X = q < nA ? 0 : 1, off = X ? nA : 0, m = X ? nB : nA;
+                const double *gr = R.gmInv[X], *gi = gr + (size_t)m * m;
+                double sr = 0.0, sim = 0.0;
+#pragma unroll 8
+                for (int k = 0; k < m; ++k) {
+                    const size_t og = (size_t)k + (size_t)(q - off) * m;
+                    double tr, ti;
+                    cmul(zr[t * NX + off + k], zi[t * NX + off + k], gr[og], gi[og], tr, ti);
+                    sr += tr;
+                    sim += ti;
+                }
+                br_[t * NX + q] = sr;
+                bi_[t * NX + q] = sim;
+            }
+            __syncthreads();
+            double v[8] = {0, 0, 0, 0, 0, 0, 0, 0}, w8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            for (int q = tid; q < nX; q += BT) {
+                const bool inA = q < nA;
+#pragma unroll
+                for (int t = 0; t < 2; ++t)
+#pragma unroll
+                    for (int uu = 0; uu < 2; ++uu) {
+                        double tr, ti;
+                        cmul(br_[t * NX + q], bi_[t * NX + q], ar_[uu * NX + q], ai_[uu * NX + q], tr, ti);
+                        const int k = 2 * (t * 2 + uu);
+                        v[k] += inA ? tr : 0.0;
+                        v[k + 1] += inA ? ti : 0.0;
+                        w8[k] += inA ? 0.0 : tr;
+                        w8[k + 1] += inA ? 0.0 : ti;
+                    }
+            }
+            block_sum8<BT>(v, red);
+            block_sum8<BT>(w8, red + 8 * (BT / 64));
+            if (tid == 0) {
+                const int m[2] = {nA, nB};
+                double dX[2];
+#pragma unroll
+                for (int X = 0; X < 2; ++X) {
+                    double tr[2][2], ti[2][2], ir[2][2], ii[2][2], dr = 1.0, di = 0.0;
+#pragma unroll
+                    for (int t = 0; t < 2; ++t)
+#pragma unroll
+                        for (int uu = 0; uu < 2; ++uu) {
+                            const int k = 2 * (t * 2 + uu);
+                            tr[t][uu] = (X ? w8[k] : v[k]) + (t == uu ? 1.0 : 0.0);
+                            ti[t][uu] = X ? w8[k + 1] : v[k + 1];
+                        }
+                    if (m[X]) c22_inv(tr, ti, ir, ii, dr, di);
+                    else {
+                        ir[0][0] = ir[1][1] = 1.0;
+                        ir[0][1] = ir[1][0] = 0.0;
+                        ii[0][0] = ii[0][1] = ii[1][0] = ii[1][1] = 0.0;
+                    }
+#pragma unroll
+                    for (int t = 0; t < 2; ++t)
+#pragma unroll
+                        for (int uu = 0; uu < 2; ++uu) {
+                            tinv[X][0][t][uu] = ir[t][uu];
+                            tinv[X][1][t][uu] = ii[t][uu];
+                        }
+                    dX[X] = m[X] ? (a.kind == 1 ? dr : dr * dr + di * di) : 1.0;
+                }
+                const double u = pu[r][qb];
+                const double p = prob_s * pow(dX[0], R.lambda) * pow(dX[1], 1.0 - R.lambda);
+                const int ok = u < p;
+                if (fabs(u - p) < KDQ_NEAR_TIE) ++nties[r];
+                acc_s = ok;
+                if (ok) {
+                    R.detg[0] *= dX[0];
+                    R.detg[1] *= dX[1];
+                }
+            }
+            __syncthreads();
+            if (!acc_s) continue;   // block-uniform
+            const int n0 = na[r];
+            // ---- accepted: GMupdate! operands, the working rows / columns at the element's
+            // sites (before the update), and the bordering vectors x, y from the block start
+            for (int it = tid; it < 2 * nX; it += BT) {
+                const int uu = it / nX, q = it % nX;
+                const int X = q < nA ? 0 : 1, off = X ? nA : 0, m = X ? nB : nA;
+                const double *gr = R.gmInv[X], *gi = gr + (size_t)m * m;
+                double sr = 0.0, sim = 0.0;
+#pragma unroll 8
+                for (int k = 0; k < m; ++k) {
+                    const size_t og = (size_t)(q - off) + (size_t)k * m;
+                    double tr, ti;
+                    cmul(gr[og], gi[og], ar_[uu * NX + off + k], ai_[uu * NX + off + k], tr, ti);
+                    sr += tr;
+                    sim += ti;
+                }
+                nr_[uu * NX + q] = sr;
+                ni_[uu * NX + q] = sim;
+                double wr = 0.0, wi = 0.0;
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    double tr, ti;
+                    cmul(tinv[X][0][uu][k], tinv[X][1][uu][k], br_[k * NX + q], bi_[k * NX + q], tr, ti);
+                    wr += tr;
+                    wi += ti;
+                }
+                wr_[uu * NX + q] = wr;
+                wi_[uu * NX + q] = wi;
+                cxr[uu * NX + q] = o0r[O0(r, q, ls(uu))];
+                cxi[uu * NX + q] = o0i[O0(r, q, ls(uu))];
+                rxr[uu * NX + q] = t0r[T0(r, ls(uu), q)];
+                rxi[uu * NX + q] = t0i[T0(r, ls(uu), q)];
+            }
+            if (tid < 2 * KS) {
+                const int t = tid / KS, i = tid % KS;
+                csr[t * KS + i] = gtr[GT(r, i, ls(t))];
+                csi[t * KS + i] = gti[GT(r, i, ls(t))];
+                rsr[t * KS + i] = gtr[GT(r, ls(t), i)];
+                rsi[t * KS + i] = gti[GT(r, ls(t), i)];
+            }
+            if (tid < 2 * n0) {
+                // x[t][k] = Σ_u W[t][u] (δ(A_u, a_k) − G0[A_u, a_k]); y[k][u] = Σ_t G0[a_k, A_t] W[t][u]
+                const int t = tid / 2, k = tid % 2;
+                double sxr = 0.0, sxi = 0.0, syr = 0.0, syi = 0.0;
+                for (int u = 0; u < n0; ++u) {
+                    const int au = accl[r][u];
+                    double tr, ti;
+                    cmul(Wr[GT(r, t, u)], Wi[GT(r, t, u)], (au == ls(k) ? 1.0 : 0.0) - g0r[GT(r, au, ls(k))],
+                         -g0i[GT(r, au, ls(k))], tr, ti);
+                    sxr += tr;
+                    sxi += ti;
+                    // y[k][t]: here t plays the column index of W
+                    cmul(g0r[GT(r, ls(k), au)], g0i[GT(r, ls(k), au)], Wr[GT(r, u, t)], Wi[GT(r, u, t)], tr, ti);
+                    syr += tr;
+                    syi += ti;
+                }
+                xr[t * 2 + k] = sxr;
+                xi[t * 2 + k] = sxi;
+                yr[k * KS + t] = syr;
+                yi[k * KS + t] = syi;
+            }
+            __syncthreads();
+            for (int it = tid; it < mAB; it += BT) {
+                const int X = it < mA ? 0 : 1, nr = X ? nB : nA, base = X ? nA : 0, g = X ? it - mA : it;
+                const int p = g % nr, q = g / nr;
+                double sr = 0.0, sim = 0.0, mr = 0.0, mi = 0.0;
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    double tr, ti;
+                    cmul(nr_[k * NX + base + p], ni_[k * NX + base + p], wr_[k * NX + base + q],
+                         wi_[k * NX + base + q], tr, ti);
+                    sr += tr;
+                    sim += ti;
+                    double zr_ = 0.0, zi_ = 0.0;
+#pragma unroll
+                    for (int k2 = 0; k2 < 2; ++k2) {
+                        double ur, ui;
+                        cmul(rmr[k][k2], rmi[k][k2], rxr[k2 * NX + base + q], rxi[k2 * NX + base + q], ur, ui);
+                        zr_ += ur;
+                        zi_ += ui;
+                    }
+                    cmul(cxr[k * NX + base + p], cxi[k * NX + base + p], zr_, zi_, tr, ti);
+                    mr += tr;
+                    mi += ti;
+                }
+                double *gv = R.gmInv[X];
+                gv[p + (size_t)q * nr] -= sr;
+                gv[(size_t)nr * nr + p + (size_t)q * nr] -= sim;
+                double *M = R.M[r][X];
+                M[p + (size_t)q * nr] += mr;
+                M[(size_t)nr * nr + p + (size_t)q * nr] += mi;
+            }
+            // working blocks: with f = w (E_s - Gt[s, S]) and g = w Gt0[s, X],
+            // Gt[S,S] -= Gt[S,s] f, G0t[X,S] -= G0t[X,s] f, Gt0[S,X] += Gt[S,s] g
+            for (int e = tid; e < KS * KS; e += BT) {
+                const int i = e / KS, j = e % KS;
+                double sr = 0.0, sim = 0.0;
+#pragma unroll
+                for (int t = 0; t < 2; ++t) {
+                    double fr = 0.0, fi = 0.0;
+#pragma unroll
+                    for (int k = 0; k < 2; ++k) {
+                        double tr, ti;
+                        cmul(rmr[t][k], rmi[t][k], (j == ls(k) ? 1.0 : 0.0) - rsr[k * KS + j], -rsi[k * KS + j], tr, ti);
+                        fr += tr;
+                        fi += ti;
+                    }
+                    double tr, ti;
+                    cmul(csr[t * KS + i], csi[t * KS + i], fr, fi, tr, ti);
+                    sr += tr;
+                    sim += ti;
+                }
+                gtr[GT(r, i, j)] -= sr;
+                gti[GT(r, i, j)] -= sim;
+            }
+            for (int e = tid; e < nX * KS; e += BT) {
+                const int x = e / KS, j = e % KS;
+                double sr = 0.0, sim = 0.0;
+#pragma unroll
+                for (int t = 0; t < 2; ++t) {
+                    double fr = 0.0, fi = 0.0;
+#pragma unroll
+                    for (int k = 0; k < 2; ++k) {
+                        double tr, ti;
+                        cmul(rmr[t][k], rmi[t][k], (j == ls(k) ? 1.0 : 0.0) - rsr[k * KS + j], -rsi[k * KS + j], tr, ti);
+                        fr += tr;
+                        fi += ti;
+                    }
+                    double tr, ti;
+                    cmul(cxr[t * NX + x], cxi[t * NX + x], fr, fi, tr, ti);
+                    sr += tr;
+                    sim += ti;
+                }
+                o0r[O0(r, x, j)] -= sr;
+                o0i[O0(r, x, j)] -= sim;
+            }
+            for (int e = tid; e < KS * nX; e += BT) {
+                const int i = e / nX, x = e % nX;
+                double sr = 0.0, sim = 0.0;
+#pragma unroll
+                for (int t = 0; t < 2; ++t) {
+                    double gr = 0.0, gi = 0.0;
+#pragma unroll
+                    for (int k = 0; k < 2; ++k) {
+                        double tr, ti;
+                        cmul(rmr[t][k], rmi[t][k], rxr[k * NX + x], rxi[k * NX + x], tr, ti);
+                        gr += tr;
+                        gi += ti;
+                    }
+                    double tr, ti;
+                    cmul(csr[t * KS + i], csi[t * KS + i], gr, gi, tr, ti);
+                    sr += tr;
+                    sim += ti;
+                }
+                t0r[T0(r, i, x)] += sr;
+                t0i[T0(r, i, x)] += sim;
+            }
+            // W' = [[W − x w y, −x w], [w y, w]] (rows / columns n0, n0 + 1 new)
+            for (int e = tid; e < (n0 + 2) * (n0 + 2); e += BT) {
+                const int t = e % (n0 + 2), u = e / (n0 + 2);
+                double vr = 0.0, vi = 0.0;
+                if (t < n0 && u < n0) {
+                    vr = Wr[GT(r, t, u)];
+                    vi = Wi[GT(r, t, u)];
+#pragma unroll
+                    for (int k = 0; k < 2; ++k)
+#pragma unroll
+                        for (int k2 = 0; k2 < 2; ++k2) {
+                            double pr, pi, tr, ti;
+                            cmul(xr[t * 2 + k], xi[t * 2 + k], rmr[k][k2], rmi[k][k2], pr, pi);
+                            cmul(pr, pi, yr[k2 * KS + u], yi[k2 * KS + u], tr, ti);
+                            vr -= tr;
+                            vi -= ti;
+                        }
+                } else if (t < n0) {
+#pragma unroll
+                    for (int k = 0; k < 2; ++k) {
+                        double tr, ti;
+                        cmul(xr[t * 2 + k], xi[t * 2 + k], rmr[k][u - n0], rmi[k][u - n0], tr, ti);
+                        vr -= tr;
+                        vi -= ti;
+                    }
+                } else if (u < n0) {
+#pragma unroll
+                    for (int k = 0; k < 2; ++k) {
+                        double tr, ti;
+                        cmul(rmr[t - n0][k], rmi[t - n0][k], yr[k * KS + u], yi[k * KS + u], tr, ti);
+                        vr += tr;
+                        vi += ti;
+                    }
+                } else {
+                    vr = rmr[t - n0][u - n0];
+                    vi = rmi[t - n0][u - n0];
+                }
+                Wr[GT(r, t, u)] = vr;
+                Wi[GT(r, t, u)] = vi;
+            }
+            if (tid == 0) {
+                accl[r][n0] = l0;
+                accl[r][n0 + 1] = l1;
+                sv[r][qb] |= 1 << 16;
+            }
+            __syncthreads();
+            na[r] += 2;
+        }
+    }
+    for (int r = 0; r < 2; ++r) {
+        const ChainDev &chr = a.chains[c + r * nch];
+        if (tid == 0) {
+            *chr.acc_cnt = na[r];
+            chr.counters[0] += kb;
+            chr.counters[1] += na[r] / 2;
+            chr.counters[2] += nties[r];
+        }
+        if (tid < na[r]) chr.acc_idx[tid] = site[accl[r][tid]];
+        for (int e = tid; e < na[r] * na[r]; e += BT) {
+            const int t = e % na[r], u = e / na[r];
+            chr.W[t + u * 64] = Wr[r * KS * KS + t * KS + u];
+            chr.W[64 * 64 + t + u * 64] = Wi[r * KS * KS + t * KS + u];
+        }
+        if (tid < kb) {
+            const int v = sv[r][tid];
+            chr.s[(size_t)(e0 + tid) + (size_t)(lt - 1) * chr.n_elem] =
+                (uint8_t)((v >> 16) ? (v >> 8) & 0xff : v & 0xff);
+        }
+    }
+#undef GT
+#undef T0
+#undef O0
+}
+
 __global__ void k_scee_detg_signed(const SceeRegion *reg, int nch, const double *la, const double *lb,
                                    const double *pa, const double *pb) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1471,6 +1999,23 @@ hipError_t launch_scee_r2_block(hipStream_t st, const SceeR2Args &a, int kb) {
     if (hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes)) return e;
     if (kb == 8) hipLaunchKernelGGL(k_scee_r2_block<8>, dim3(a.nch), dim3(BT), bytes, st, a);
     else hipLaunchKernelGGL(k_scee_r2_block<4>, dim3(a.nch), dim3(BT), bytes, st, a);
+    return hipGetLastError();
+}
+
+size_t scee_r2_block_uv_lds_bytes(int kb, int nx_max) {
+    const size_t ks = 2 * (size_t)kb, nx = nx_max > 1 ? nx_max : 1;
+    return (12 * ks * ks + 8 * ks * nx + 28 * nx + 16 * ks + 16 * (BT / 64)) * sizeof(double);
+}
+
+hipError_t launch_scee_r2_block_uv(hipStream_t st, const SceeR2Args &a, int kb) {
+    if (kb != 4 && kb != 8) return hipErrorInvalidValue;
+    const size_t bytes = scee_r2_block_uv_lds_bytes(kb, a.nx_max);
+    if (bytes > 150 * 1024) return hipErrorInvalidValue;
+    const void *f = kb == 8 ? reinterpret_cast<const void *>(k_scee_r2_block_uv<8>)
+                            : reinterpret_cast<const void *>(k_scee_r2_block_uv<4>);
+    if (hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes)) return e;
+    if (kb == 8) hipLaunchKernelGGL(k_scee_r2_block_uv<8>, dim3(a.nch), dim3(BT), bytes, st, a);
+    else hipLaunchKernelGGL(k_scee_r2_block_uv<4>, dim3(a.nch), dim3(BT), bytes, st, a);
     return hipGetLastError();
 }
 

@@ -124,7 +124,9 @@ def _descriptors(models, seeds, chains, delay, sweep0, precision=L.PREC_FP64):
 class DQMCBatch:
     """A batch of chains of one model kind (t-U, t-V or t-U-V) on the current GPU.
 
-    models: list of model structs (same kind, Nt and nodes); seeds: list of int.
+    models: list of model structs of one kind; seeds: list of int. Chains may differ in Ns,
+    couplings and Nt: chains of one Nt share launches, and the Nt groups sweep concurrently
+    (per-slice calls — wrap, update_layer, measure — need one Nt).
     delay: local-update block size k in sites (t-U, ≤ 64) or field elements (t-V / t-U-V,
     ≤ 32); 1 = one Sherman-Morrison update per flip.
     """

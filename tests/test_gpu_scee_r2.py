@@ -139,3 +139,34 @@ def test_tv_scee_delayed_blocks(dq, oracle_mod, delay):
     assert not np.array_equal(b.get_fields(0, 0), s1)
     assert b.near_ties().sum() == 0
     b.close()
+
+
+@pytest.mark.parametrize("delay", [1, 4, 8])
+@pytest.mark.parametrize("U", [4.0, -4.0])
+def test_tuv_scee_delayed_blocks(dq, oracle_mod, delay, U):
+    """t-U-V decisions in delayed blocks of `delay` consecutive elements (directed bonds, so a
+    block's sites repeat and W is bordered from the block-start G; 1: element by element)
+    against the oracle's sequential G4update! on a 3x3 lattice (54 elements per slice: whole
+    and partial blocks), complex (U > 0) and real (U < 0) cases: fields bit for bit, the four
+    matrices to 1e-9, O to 1e-8."""
+    print(f"case: SCEE t-U-V L=3 U={U} delay={delay}", flush=True)
+    m = tuv(U=U)
+    iA, iB = regions(m)
+    seed = 41
+    s1, s2 = dq.Initial_s(m, seed), dq.Initial_s(m, seed + 100)
+    o = oracle_mod.OracleSCEE2(m, iA.astype(np.int32), iB.astype(np.int32), 0.5, 2, s1, s2, seed=seed)
+    b = dq.SCEEBatch([m], iA, iB, 0.5, 2, [seed], [0], delay=delay)
+    b.set_fields(0, 0, s1)
+    b.set_fields(0, 1, s2)
+    Oo = o.sweeps(2)
+    Og = b.sweeps(2)[0]
+    for r in (0, 1):
+        assert np.array_equal(b.get_fields(0, r), o.fields(r)), r
+        for nm, x, y in zip(("Gt", "G0", "Gt0", "G0t"), b.G4(0, r), o.G4(r)):
+            assert rel(x, y) < 1e-9, (r, nm, rel(x, y))
+    assert np.allclose(Og, Oo, rtol=1e-8, atol=0), (Og, Oo)
+    d = b.detg()[0]
+    assert abs(d[0] / o.detg(0) - 1) < 1e-8 and abs(d[1] / o.detg(1) - 1) < 1e-8
+    assert not np.array_equal(b.get_fields(0, 0), s1)
+    assert b.near_ties().sum() == 0
+    b.close()
