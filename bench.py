@@ -492,61 +492,49 @@ def bench_mixed_curve(args):
 def bench_uneven(args):
     """Uneven batch: 64 t-U chains, chain c on an L x L honeycomb with L = (6, 9, 12)[c % 3]
     (N = 72 / 162 / 288), U/t = (2, 4, 6)[(c // 3) % 3], seed c, Δτ = 0.1, 2Θ = 2L (Nt = 120 /
-    180 / 240), stabilisation every 10 slices. Chains of one Nt share a handle (one stream);
-    the three handles run concurrently from three host threads. Mixed precision is timed and
-    compared with FP64 on the same seeds: max |G_FP32 - G_FP64| per chain at the
-    stabilisations, and the double occupancy per group in units of the chain spread."""
-    import threading
+    180 / 240), stabilisation every 10 slices, all in one handle (its three Nt groups sweep
+    concurrently, each on its own stream). Mixed precision is timed and compared with FP64 on
+    the same seeds: max |G_FP32 - G_FP64| per chain at the stabilisations, and the double
+    occupancy per L in units of the chain spread."""
     import numpy as np
     import torch
     from kapdqmc_amd.dqmc import DQMCBatch
     from kapdqmc_amd.models import tU_Hubbard_Para
     C = args.chains
-    groups = {}
+    ms, Ls = [], []
     for c in range(C):
         L, U = (6, 9, 12)[c % 3], (2.0, 4.0, 6.0)[(c // 3) % 3]
-        m = tU_Hubbard_Para(Ht=1.0, Hu1=U, Hu2=U, Δt=0.1, Θrelax=float(L), Θquench=0.0, Lattice="HoneyComb120",
-                            site=[L, L], BatchSize=10)
-        groups.setdefault(L, []).append((c, m))
+        ms.append(tU_Hubbard_Para(Ht=1.0, Hu1=U, Hu2=U, Δt=0.1, Θrelax=float(L), Θquench=0.0,
+                                  Lattice="HoneyComb120", site=[L, L], BatchSize=10))
+        Ls.append(L)
 
     def run(precision):
-        hs = {L: DQMCBatch([m for _, m in g], [c for c, _ in g], [c for c, _ in g], delay=32, precision=precision)
-              for L, g in groups.items()}
-        for b in hs.values():
-            b.initial_s()
-            b.phy_update(args.warmup, True)
-            b.phy_update(0, True)   # graph capture
-            b.precision_error()     # reset the error record
+        b = DQMCBatch(ms, list(range(C)), list(range(C)), delay=32, precision=precision)
+        b.initial_s()
+        b.phy_update(args.warmup, True)
+        b.phy_update(0, True)   # graph capture
+        b.precision_error()     # reset the error record
         torch.cuda.synchronize()
-        rows, errs = {}, {}
-
-        def go(L):
-            rows[L] = hs[L].phy_update(args.steps, True)
-            errs[L] = hs[L].precision_error()
-
         t0 = time.perf_counter()
-        th = [threading.Thread(target=go, args=(L,)) for L in hs]
-        for t in th:
-            t.start()
-        for t in th:
-            t.join()
+        rows = b.phy_update(args.steps, True)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
-        for b in hs.values():
-            b.close()
-        return dt, rows, errs
+        err = b.precision_error()
+        b.close()
+        return dt, rows, err
 
     dt64, rows64, _ = run("fp64")
     print(f"bench: uneven FP64 {args.steps} sweeps in {dt64:.2f} s", file=sys.stderr, flush=True)
     dtm, rowsm, errs = run("mixed")
     print(f"bench: uneven mixed {args.steps} sweeps in {dtm:.2f} s", file=sys.stderr, flush=True)
     per_group = {}
-    for L, g in groups.items():
+    for L in (6, 9, 12):
+        idx = [c for c in range(C) if Ls[c] == L]
         N = 2 * L * L
-        d64, dm = rows64[L][:, :, 1].mean(axis=1) / N, rowsm[L][:, :, 1].mean(axis=1) / N
-        sig = np.sqrt(d64.var(ddof=1) / len(g) + dm.var(ddof=1) / len(g))
-        per_group[f"L={L}"] = {"chains": len(g), "N": N, "Nt": 20 * L,
-                               "max_abs_G_fp32_minus_fp64_per_chain": [float(x) for x in errs[L]],
+        d64, dm = rows64[idx, :, 1].mean(axis=1) / N, rowsm[idx, :, 1].mean(axis=1) / N
+        sig = np.sqrt(d64.var(ddof=1) / len(idx) + dm.var(ddof=1) / len(idx))
+        per_group[f"L={L}"] = {"chains": len(idx), "N": N, "Nt": 20 * L,
+                               "max_abs_G_fp32_minus_fp64_per_chain": [float(errs[c]) for c in idx],
                                "double_occupancy_fp64": float(d64.mean()), "double_occupancy_mixed": float(dm.mean()),
                                "difference_in_sigma": float((d64.mean() - dm.mean()) / sig)}
     print(json.dumps({
@@ -555,7 +543,7 @@ def bench_uneven(args):
         "higher_is_better": True, "dtype": "mixed (f32 G between stabilisations, f64 decisions / QR / LU)",
         "data": "synthetic (Philox-seeded HS fields)",
         "config": {"workload": f"{C} t-U chains, L = (6, 9, 12)[c % 3], U = (2, 4, 6)[(c // 3) % 3], seed c, "
-                               "dtau = 0.1, 2Theta = 2L, BS = 10; one handle per Nt, three concurrent streams",
+                               "dtau = 0.1, 2Theta = 2L, BS = 10; one handle, three Nt groups on concurrent streams",
                    "chains": C},
         "fp64_chain_sweeps_per_s": C * args.steps / dt64, "groups": per_group}))
 
@@ -580,7 +568,7 @@ def main():
     ap.add_argument("--slmc-rounds", type=int, default=8, help="training rounds of the SLMC leg's network")
     ap.add_argument("--slmc-channels", type=int, default=None,
                     help="hidden channels of the network (SLMC leg: 8, slmc-ess: 32)")
-    ap.add_argument("--slmc-leg-inner", type=int, default=1, help="inner steps per global move (SLMC leg)")
+    ap.add_argument("--slmc-leg-inner", type=int, default=16, help="inner steps per global move (SLMC leg)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="timed sweeps with host launches")

@@ -1191,7 +1191,7 @@ int g_phy_update(kdq_handle h, int sweeps, int record, double *rows) {
     std::vector<int> rc(G, KDQ_OK);
     std::vector<std::thread> th;
     for (size_t g = 0; g < G; ++g) {
-        if (record) buf[g].resize((size_t)h->grp[g]->nch * sweeps * no);
+        if (record) buf[g].resize(std::max<size_t>(1, (size_t)h->grp[g]->nch * sweeps * no));   // non-null at 0 sweeps
         th.emplace_back([&, g] {
             if (hipSetDevice(dev) != hipSuccess) {
                 rc[g] = fail(h->grp[g], KDQ_EHIP, "kdq_phy_update: hipSetDevice failed in a group's thread");

@@ -66,6 +66,7 @@ def test_mixed_nt_batch_equals_separate_handles(dq):
     ms = [tu(L, U) for L, U in zip(LS, US)]
     b = dq.DQMCBatch(ms, SEEDS, list(range(5)), delay=8)
     b.initial_s()
+    b.phy_update(0, True)          # zero sweeps (graph capture only) is a valid call
     rb = b.phy_update(3, True)
     sep = {}
     for L in sorted(set(LS)):
