@@ -41,6 +41,42 @@ def model_cfg2():
                            Lattice="HoneyComb120", site=[12, 12], BatchSize=10)
 
 
+def small_case(cpu_sweeps: int = 20, gpu_sweeps: int = 20):
+    """The reference's own CPU-runnable case: t-U honeycomb 3x3 (N = 18), U = 4, Δτ = 0.1,
+    2Θ = 10 (Nt = 100), stabilisation every 10 slices, record = true. The oracle (C FP64; at
+    N = 18 its products stay below the OpenMP threshold, so it runs on one thread) against the
+    GPU path as one chain per handle and as 64 chains in one handle."""
+    import numpy as np
+    import torch
+    from kapdqmc_amd.dqmc import DQMCBatch
+    from kapdqmc_amd.models import tU_Hubbard_Para
+    from kapdqmc_amd.rng import initial_fields
+    from oracle import oracle
+    m = tU_Hubbard_Para(Ht=1.0, Hu1=4.0, Hu2=4.0, Δt=0.1, Θrelax=5.0, Θquench=0.0,
+                        Lattice="HoneyComb120", site=[3, 3], BatchSize=10)
+    o = oracle.OracleTU(m, seed=1)
+    o.initial_s()
+    o.phy_update(1, True)
+    t0 = time.perf_counter()
+    o.phy_update(cpu_sweeps, True)
+    t_cpu = (time.perf_counter() - t0) / cpu_sweeps
+    out = {"workload": "t-U honeycomb 3x3 N=18 U=4 dtau=0.1 2Theta=10 Nt=100 BS=10, record=true",
+           "cpu_oracle_1_thread_sweeps_per_s": 1.0 / t_cpu}
+    for C in (1, 64):
+        b = DQMCBatch([m] * C, list(range(C)), list(range(C)), delay=8)
+        for c in range(C):
+            b.set_fields(c, initial_fields(c, c, m.Ns, m.Nt))
+        b.phy_update(2, True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        b.phy_update(gpu_sweeps, True)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        b.close()
+        out[f"gpu_{C}_chain{'s' if C > 1 else ''}_chain_sweeps_per_s"] = C * gpu_sweeps / dt
+    return out
+
+
 def family_work(fam: str, N: int, Np: int):
     """Algorithmic work of one launch of a kernel family: (bound, amount, unit)."""
     if fam in ("wrap", "meas_wrap"):
@@ -736,6 +772,7 @@ def main():
     if not args.no_cpu_baseline:
         try:
             cpu = cpu_baseline(m, tau, args.cpu_slices)
+            cpu["small_case"] = small_case()
         except Exception as e:  # the baseline is reported, never required
             cpu = {"error": repr(e)}
 

@@ -1,9 +1,10 @@
 """The benchmark configurations at their full sizes through the GPU path (C ABI), with the
 oracle where it finishes in seconds and size-independent properties elsewhere.
 
-* Uneven batch (64 t-U chains, L = 6/9/12 by c mod 3, U = 2/4/6, seed c, 2Θ = 2L, one handle per
-  Nt): one FP64 sweep of the L = 6 chains against the oracle (fields bit-exact, G 1e-10), and
-  the mixed-precision batch's max |G_FP32 − G_FP64| per chain at the recomputes.
+* Uneven batch (64 t-U chains, L = 6/9/12 by c mod 3, U = 2/4/6, seed c, 2Θ = 2L, all in one
+  handle): one FP64 sweep, eight L = 6 chains and one L = 9 chain against the oracle (fields
+  bit-exact, G 1e-10), and the mixed-precision batch's max |G_FP32 − G_FP64| per chain at the
+  recomputes.
 * Swap replicas at 9x9 (N = 162), Δτ = 0.05, 2Θ = 20 (Nt = 400), |A| = 81 (72 sites of the
   cells x = 1..4 and the 9 A sites at x = 5), B empty: the set-up (G(τ), G(Θ), G(τ,Θ), G(Θ,τ)
   of both replicas and det g) against the oracle for t-V and t-U-V; after a t-V sweep, G(τ)
@@ -38,31 +39,34 @@ def uneven_groups():
 
 
 def test_uneven_batch_fp64_against_oracle_and_mixed_error(oracle_mod):
-    print("case: uneven 64-chain batch L=6/9/12, 2Theta=2L", flush=True)
+    print("case: uneven 64-chain batch L=6/9/12, 2Theta=2L, one handle", flush=True)
     from kapdqmc_amd.dqmc import DQMCBatch, Initial_s
     groups = uneven_groups()
-    g6 = groups[6]
-    b = DQMCBatch([m for _, m in g6], [c for c, _ in g6], [c for c, _ in g6], delay=32)
-    for k, (c, m) in enumerate(g6):
-        b.set_fields(k, Initial_s(m, c, c))
+    ms = [None] * 64
+    for g in groups.values():
+        for c, m in g:
+            ms[c] = m
+    b = DQMCBatch(ms, list(range(64)), list(range(64)), delay=32)
+    for c, m in enumerate(ms):
+        b.set_fields(c, Initial_s(m, c, c))
     rb = b.phy_update(1, True)
-    for k, (c, m) in enumerate(g6[:8]):
+    for c in [c for c, _ in groups[6][:8]] + [groups[9][0][0]]:   # eight L = 6 chains, one L = 9
+        m = ms[c]
         o = oracle_mod.OracleTU(m, seed=c, chain=c)
         o.s = Initial_s(m, c, c)
         ro = o.phy_update(1, True)
-        assert np.array_equal(b.get_fields(k), o.s), c
-        assert rel(b.get_G(k), o.G) < 1e-10
-        assert np.allclose(rb[k], ro, rtol=1e-9, atol=1e-11)
+        assert np.array_equal(b.get_fields(c), o.s), c
+        assert rel(b.get_G(c), o.G) < 1e-10
+        assert np.allclose(rb[c], ro, rtol=1e-9, atol=1e-11)
     b.close()
-    errs = {}
+    mx = DQMCBatch(ms, list(range(64)), list(range(64)), delay=32, precision="mixed")
+    for c, m in enumerate(ms):
+        mx.set_fields(c, Initial_s(m, c, c))
+    mx.phy_update(1, False)
+    e_all = mx.precision_error()
+    mx.close()
     for L, g in groups.items():
-        mx = DQMCBatch([m for _, m in g], [c for c, _ in g], [c for c, _ in g], delay=32, precision="mixed")
-        for k, (c, m) in enumerate(g):
-            mx.set_fields(k, Initial_s(m, c, c))
-        mx.phy_update(1, False)
-        errs[L] = mx.precision_error()
-        mx.close()
-    for L, e in errs.items():
+        e = e_all[[c for c, _ in g]]
         print(f"L={L}: max|G_FP32-G_FP64| per chain: median {np.median(e):.2e}, max {e.max():.2e}", flush=True)
         assert np.all(e > 0) and np.all(e < 1e-2)
 
