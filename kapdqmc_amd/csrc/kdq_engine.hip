@@ -125,6 +125,10 @@ struct kdq_handle_s {
     // boundary, and N x N work matrices
     std::vector<Opnd *> g4_ul;
     Opnd *g4_m[6] = {};
+    // t-U measurements of a sweep: slot pairs (X_k, X̃_k) of the products that take G to the
+    // symmetric slice (slot 0: HalfeK, HalfeK⁻¹), which recursion they hold and how far
+    std::vector<Opnd *> mq, mqt;
+    int mq_dir = 0, mq_have = 0;
     hipStream_t side = nullptr;               // their stream
     hipEvent_t ev_app = nullptr, ev_dec[2] = {nullptr, nullptr};   // their dependencies
     uint32_t *d_sweep = nullptr;              // device sweep counter read by graph replays
@@ -895,6 +899,139 @@ int measure_into(kdq_handle h, int lt, Opnd *Gm, Opnd *Tm, double *meas, double 
 
 int measure(kdq_handle h, int lt) { return measure_into(h, lt, h->t_Gm, h->t_Tm, nullptr, h->d_mpart); }
 
+// ---- t-U measurements inside a sweep (phy_measure at the slices of the two windows).
+// measure_into wraps a copy of G slice by slice to c = Nt/2: |lt − c| wrap pairs per
+// measurement, 100 per pass of the N = 288 bench. The same G_sym = HalfeK B(c, lt) G B(c, lt)⁻¹
+// HalfeK⁻¹ comes from two complex products with X = HalfeK B(c, lt) and X̃ = its inverse
+// times HalfeK⁻¹, and those follow one slice at a time (one pair of real-by-complex products
+// per slice) by two recursions over slots k = |lt − c|:
+//   down: X_k = X_(k−1) D_s eK,  X̃_k = eK⁻¹ D_s⁻¹ X̃_(k−1)  with s = c − k + 1  (slices below c)
+//   up:   Y_k = Y_(k−1) eK⁻¹ D_s⁻¹,  Ỹ_k = D_s eK Ỹ_(k−1)  with s = c + k    (slices above c)
+// (B_s = D_s eK as in wrap_B), X_0 = Y_0 = HalfeK, X̃_0 = Ỹ_0 = HalfeK⁻¹. A slot must be built from
+// the fields the measurement sees: going forward, the slices between lt and c are not yet
+// updated below c (built at the window's first slice) and already final above c (built
+// slice by slice); going backward the other way round, and the measured slice itself, just
+// updated, is wrapped on the copy first. Products of at most one block of slices (κ ≤ e^{0.6 k}
+// at Δτ = 0.1) keep the rounding near that of the stepwise wraps.
+bool is_node(kdq_handle h, int t);
+
+int mq_alloc(kdq_handle h) {
+    if (!h->mq.empty()) return KDQ_OK;
+    // the largest |lt − c| a window reaches (the windows as sweep_body visits them)
+    const int c = h->nt / 2, thidx = h->nn / 2 + 1;
+    int kmax = 1, idx = 1;
+    for (int lt = 1; lt <= h->nt; ++lt) {
+        const int d = idx - thidx;
+        if (d <= 0 && d >= -1) kmax = std::max(kmax, std::abs(lt - c));
+        if (is_node(h, lt)) ++idx;
+    }
+    for (int lt = h->nt; lt >= 1; --lt) {
+        const int d = idx - thidx;
+        if (d >= 0 && d <= 1) kmax = std::max(kmax, std::abs(lt - c));
+        if (is_node(h, lt - 1)) --idx;
+    }
+    const int slots = kmax + 1;
+    std::vector<std::vector<Opnd>> vq(slots), vqt(slots);
+    for (int c2 = 0; c2 < h->nch; ++c2) {
+        ChainHost &C = h->ch[c2];
+        const int n = C.ns;
+        for (int k = 0; k < slots; ++k)
+            for (auto *v : {&vq, &vqt}) {
+                double *p = nullptr;
+                if (dalloc(C, &p, 2 * (size_t)n * n) != hipSuccess)
+                    return fail(h, KDQ_EHIP, "measurement propagators: device allocation failed");
+                (*v)[k].push_back(cplx(p, n, (size_t)n * n));
+            }
+    }
+    h->mq.resize(slots);
+    h->mqt.resize(slots);
+    for (int k = 0; k < slots; ++k) {
+        if (int r = make_table(h, &h->mq[k], vq[k])) return r;
+        if (int r = make_table(h, &h->mqt[k], vqt[k])) return r;
+    }
+    // slot 0: HalfeK and HalfeK⁻¹ as complex matrices
+    if (h->cK) {
+        HIPCHK(launch_copy_mat(h->stream, h->nch, h->max_ns, h->max_ns, h->d_chains, h->t_hK, h->mq[0], DIM_NS, DIM_NS, true));
+        HIPCHK(launch_copy_mat(h->stream, h->nch, h->max_ns, h->max_ns, h->d_chains, h->t_hKinv, h->mqt[0], DIM_NS, DIM_NS, true));
+    } else {
+        HIPCHK(launch_real_to_cplx(h->stream, h->nch, h->max_ns, h->max_ns, h->d_chains, h->t_hK, h->mq[0], DIM_NS, DIM_NS));
+        HIPCHK(launch_real_to_cplx(h->stream, h->nch, h->max_ns, h->max_ns, h->d_chains, h->t_hKinv, h->mqt[0], DIM_NS, DIM_NS));
+    }
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return KDQ_OK;
+}
+
+// slots 1..k of recursion dir (1: down, 2: up) from the current fields
+int mq_ensure(kdq_handle h, int dir, int k) {
+    const int c = h->nt / 2;
+    if (h->mq_dir != dir) {
+        h->mq_dir = dir;
+        h->mq_have = 0;
+    }
+    for (int j = h->mq_have + 1; j <= k; ++j) {
+        if (dir == 1) {
+            const int sl = c - j + 1;
+            GemmArgs g1 = gargs(h, h->mq[j - 1], h->t_eK, h->mq[j], DIM_NS, DIM_NS, DIM_NS);
+            g1.inS = hs_diag(h, sl, +1);
+            if (int r = gemm_m(h, false, true, true, false, 0, 0, g1, KDQ_PROF_MEAS_WRAP)) return r;
+            GemmArgs g2 = gargs(h, h->t_eKinv, h->mqt[j - 1], h->mqt[j], DIM_NS, DIM_NS, DIM_NS);
+            g2.inS = hs_diag(h, sl, -1);
+            if (int r = gemm_m(h, true, false, false, true, 0, 0, g2, KDQ_PROF_MEAS_WRAP)) return r;
+        } else {
+            const int sl = c + j;
+            GemmArgs g1 = gargs(h, h->mq[j - 1], h->t_eKinv, h->mq[j], DIM_NS, DIM_NS, DIM_NS);
+            g1.colS = hs_diag(h, sl, -1);
+            if (int r = gemm_m(h, false, true, true, false, 0, 0, g1, KDQ_PROF_MEAS_WRAP)) return r;
+            GemmArgs g2 = gargs(h, h->t_eK, h->mqt[j - 1], h->mqt[j], DIM_NS, DIM_NS, DIM_NS);
+            g2.rowS = hs_diag(h, sl, +1);
+            if (int r = gemm_m(h, true, false, false, true, 0, 0, g2, KDQ_PROF_MEAS_WRAP)) return r;
+        }
+    }
+    h->mq_have = std::max(h->mq_have, k);
+    return KDQ_OK;
+}
+
+// observables of X G X̃ (wrap_first: G wrapped backward through slice lt on the copy first)
+int measure_q(kdq_handle h, int lt, int k, bool wrap_first) {
+    Opnd *Gm = h->t_Gm, *Tm = h->t_Tm;
+    if (h->mixed)
+        HIPCHK(launch_to_f64(h->stream, h->nch, h->max_ns, h->d_chains, h->t_G32[h->cur32], Gm, DIM_NS, 1));
+    else
+        HIPCHK(launch_copy_mat(h->stream, h->nch, h->max_ns, h->max_ns, h->d_chains, h->t_G[h->cur], Gm, DIM_NS, DIM_NS, true));
+    if (wrap_first)
+        if (int r = wrap_slice(h, Gm, Tm, lt, false, KDQ_PROF_MEAS_WRAP)) return r;
+    GemmArgs g1 = gargs(h, h->mq[k], Gm, Tm, DIM_NS, DIM_NS, DIM_NS);
+    if (int r = gemm(h, true, true, 0, 0, g1, KDQ_PROF_MEAS_WRAP)) return r;
+    GemmArgs g2 = gargs(h, Tm, h->mqt[k], Gm, DIM_NS, DIM_NS, DIM_NS);
+    if (int r = gemm(h, true, true, 0, 0, g2, KDQ_PROF_MEAS_WRAP)) return r;
+    MeasArgs m{h->d_chains, Gm, nullptr, h->d_mpart, meas_tiles(h->max_ns)};
+    ProfScope ps(h, KDQ_PROF_MEASURE);
+    HIPCHK(launch_measure(h->stream, h->nch, m));
+    return KDQ_OK;
+}
+
+// phy_measure at slice lt of a t-U sweep (after the slice's update): stepwise wraps where
+// they cost no more (|lt − c| ≤ 1, or 2 going backward above c), the slot products elsewhere
+int measure_sweep(kdq_handle h, int lt, bool forward) {
+    if (h->kind != KDQ_MODEL_TU || h->mq.empty()) return measure(h, lt);
+    const int c = h->nt / 2;
+    if (lt <= c) {
+        const int k = c - lt;
+        if (!forward || k >= 2)   // backward: slot k - 1 .. k follow slice by slice
+            if (int r = mq_ensure(h, 1, k)) return r;
+        return k <= 1 ? measure(h, lt) : measure_q(h, lt, k, false);
+    }
+    if (forward) {
+        const int k = lt - c;
+        if (int r = mq_ensure(h, 2, k)) return r;
+        return k <= 1 ? measure(h, lt) : measure_q(h, lt, k, false);
+    }
+    const int k = lt - 1 - c;   // slices c+1 .. lt-1, not yet updated in this pass
+    if (k <= 1) return measure(h, lt);
+    if (int r = mq_ensure(h, 2, k)) return r;
+    return measure_q(h, lt, k, true);
+}
+
 int elems(kdq_handle h, int c) {
     const int n = h->ch[c].ns;
     return h->kind == KDQ_MODEL_TU ? n : (h->kind == KDQ_MODEL_TV ? 3 * (n / 2) : 3 * n);
@@ -976,6 +1113,7 @@ int sweep_body(kdq_handle h, int record, uint32_t sweep) {
     const bool tv = h->kind == KDQ_MODEL_TV;
     (void)NN;
     if (record) HIPCHK(launch_zero_meas(h->stream, h->nch, h->d_chains));
+    h->mq_dir = 0;
     for (int lt = 1; lt <= nt; ++lt) {
         if (tv) {
             if (int r = slice_tv_forward(h, h->t_G[h->cur], h->t_T, lt, true, sweep, KDQ_PROF_WRAP)) return r;
@@ -984,13 +1122,14 @@ int sweep_body(kdq_handle h, int record, uint32_t sweep) {
             if (int r = update_layer(h, lt, 0, sweep)) return r;
         }
         if (record && in_window(h, thidx, true))
-            if (int r = measure(h, lt)) return r;
+            if (int r = tv ? measure(h, lt) : measure_sweep(h, lt, true)) return r;
         if (is_node(h, lt)) {
             h->idx += 1;
             if (int r = propagate_R(h, h->t_UR[h->idx - 2], h->t_UR[h->idx - 1], h->idx - 1)) return r;
             if (int r = node_refresh(h, h->idx)) return r;
         }
     }
+    h->mq_dir = 0;
     for (int lt = nt; lt >= 1; --lt) {
         if (tv) {
             if (int r = slice_tv_backward(h, h->t_G[h->cur], h->t_T, lt, true, sweep, KDQ_PROF_WRAP)) return r;
@@ -999,7 +1138,7 @@ int sweep_body(kdq_handle h, int record, uint32_t sweep) {
         } else {
             if (int r = update_layer(h, lt, 1, sweep)) return r;
             if (record && in_window(h, thidx, false))
-                if (int r = measure(h, lt)) return r;
+                if (int r = measure_sweep(h, lt, false)) return r;
             if (int r = h->mixed ? wrap_B_f32(h, lt, false) : wrap_B(h, h->t_G[h->cur], h->t_T, lt, false)) return r;
         }
         if (is_node(h, lt - 1)) {
@@ -1063,6 +1202,8 @@ constexpr long kMaxGraphLaunches = 20000;
 int phy_update(kdq_handle h, int sweeps, int record, double *rows) {
     const int no = h->nobs;
     h->sl_logw_valid = false;
+    if (record && h->kind == KDQ_MODEL_TU && !h->scee)
+        if (int r = mq_alloc(h)) return r;
     if (int r = init_chain(h)) return r;
     std::vector<double> tmp(no * h->nch);
     const bool graph = h->use_graph && h->prof_mask == 0 && sweep_launch_estimate(h) <= kMaxGraphLaunches;
@@ -1190,6 +1331,12 @@ int g_phy_update(kdq_handle h, int sweeps, int record, double *rows) {
     std::vector<std::vector<double>> buf(G);
     std::vector<int> rc(G, KDQ_OK);
     std::vector<std::thread> th;
+    // device allocations before the threads: one group's allocation must not fall into
+    // another group's graph capture
+    if (record)
+        for (kdq_handle sub : h->grp)
+            if (sub->kind == KDQ_MODEL_TU && !sub->scee)
+                if (int r = mq_alloc(sub)) return fail(h, r, sub->err);
     for (size_t g = 0; g < G; ++g) {
         if (record) buf[g].resize(std::max<size_t>(1, (size_t)h->grp[g]->nch * sweeps * no));   // non-null at 0 sweeps
         th.emplace_back([&, g] {
