@@ -385,10 +385,11 @@ __global__ __launch_bounds__(STEP_THREADS) void k_slmc_step(StepArgs a) {
     }
 }
 
-// ---- layer 2 of training on BF16 matrix cores (C = 16 or 32 channels)
+// ---- training on BF16 matrix cores (C = 16 or 32 channels): layer 2 forward and backward-data,
+// weight and bias gradients of layers 1 and 2
 //
-// The 3 x 3 x 3 convolution as an implicit GEMM: out[m][q] = Σ_kk A[m][kk] in[kk / 27][tap(q, kk % 27)]
-// with kk = ci·27 + k, on MFMA f32_16x16x32_bf16 (lane l holds A[l%16][8(l/16) + v],
+// The 3 x 3 x 3 convolution as an implicit GEMM: out[m][q] = Σ_(c, k) A[m][(c, k)] in[c][tap(q, k)],
+// the reduction running over (input channel, tap) pairs, on MFMA f32_16x16x32_bf16 (lane l holds A[l%16][8(l/16) + v],
 // B[8(l/16) + v][l%16] and D[4(l/16) + v][l%16], checked on the device). The weights are BF16
 // already; each FP32 activation or gradient x is split into x_hi = bf16(x) and x_lo =
 // bf16(x − x_hi), and A·x = A·x_hi + A·x_lo (products of two split operands add the
