@@ -860,6 +860,65 @@ __global__ __launch_bounds__(RT) void k_scee_r2(SceeR2Args a) {
 }
 
 
+// GMupdate! of both regions and the replica's region blocks of G0 after an accepted rank-2
+// change: gmInv_X -= (gmInv a)(Tau⁻¹ b), M_r += G0t[X, s] w Gt0[s, X]. Read-modify-writes of
+// global memory: each thread loads UB entries' old values before it stores any, so the round
+// trips overlap (one after another they serialise on the possible aliasing of the stores).
+constexpr int UB = 8;
+KDQ_DEV void region_update(const SceeRegion &R, int r, int nA, int nB, int NX, const double *nr_,
+                           const double *ni_, const double *wr_, const double *wi_, const double *cxr,
+                           const double *cxi, const double *rxr, const double *rxi, const double (*rmr)[2],
+                           const double (*rmi)[2]) {
+    const int mA = nA * nA, mAB = mA + nB * nB;
+    for (int it0 = threadIdx.x; it0 < mAB; it0 += BT * UB) {
+        double gvr[UB], gvi[UB], mvr[UB], mvi[UB];
+#pragma unroll
+        for (int u = 0; u < UB; ++u) {
+            const int it = it0 + u * BT;
+            if (it >= mAB) continue;
+            const int X = it < mA ? 0 : 1, nr = X ? nB : nA, g = X ? it - mA : it;
+            const size_t o = (size_t)(g % nr) + (size_t)(g / nr) * nr, pl = (size_t)nr * nr;
+            gvr[u] = R.gmInv[X][o];
+            gvi[u] = R.gmInv[X][pl + o];
+            mvr[u] = R.M[r][X][o];
+            mvi[u] = R.M[r][X][pl + o];
+        }
+#pragma unroll
+        for (int u = 0; u < UB; ++u) {
+            const int it = it0 + u * BT;
+            if (it >= mAB) continue;
+            const int X = it < mA ? 0 : 1, nr = X ? nB : nA, base = X ? nA : 0, g = X ? it - mA : it;
+            const int p = g % nr, q = g / nr;
+            double sr = 0.0, sim = 0.0, mr = 0.0, mi = 0.0;
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                double tr, ti;
+                cmul(nr_[k * NX + base + p], ni_[k * NX + base + p], wr_[k * NX + base + q], wi_[k * NX + base + q], tr,
+                     ti);
+                sr += tr;
+                sim += ti;
+                // (w Gt0[s, X_q])_k
+                double zr_ = 0.0, zi_ = 0.0;
+#pragma unroll
+                for (int k2 = 0; k2 < 2; ++k2) {
+                    double ur, ui;
+                    cmul(rmr[k][k2], rmi[k][k2], rxr[k2 * NX + base + q], rxi[k2 * NX + base + q], ur, ui);
+                    zr_ += ur;
+                    zi_ += ui;
+                }
+                cmul(cxr[k * NX + base + p], cxi[k * NX + base + p], zr_, zi_, tr, ti);
+                mr += tr;
+                mi += ti;
+            }
+            const size_t o = (size_t)p + (size_t)q * nr, pl = (size_t)nr * nr;
+            R.gmInv[X][o] = gvr[u] - sr;
+            R.gmInv[X][pl + o] = gvi[u] - sim;
+            R.M[r][X][o] = mvr[u] + mr;
+            R.M[r][X][pl + o] = mvi[u] + mi;
+        }
+    }
+}
+
 // ---- delayed blocks of the t-V rank-2 decisions (UpdateSCEELayer!, src/tV/SCEE.jl:447-494)
 // A block holds KB consecutive bonds of one family. The bonds of a family are disjoint, so the
 // block's 2·KB sites S are distinct. Gt[S,S], Gt0[S,X] and G0t[X,S] of both replicas live in
@@ -1203,37 +1262,7 @@ __global__ __launch_bounds__(BT) void k_scee_r2_block(SceeR2Args a) {
             __syncthreads();
             // inverse Grover matrices: gmInv_X -= (gmInv a)(Tau⁻¹ b); the replica's region blocks
             // of G0: M_r += G0t[X, s] w Gt0[s, X]
-            for (int it = tid; it < mAB; it += BT) {
-                const int X = it < mA ? 0 : 1, nr = X ? nB : nA, base = X ? nA : 0, g = X ? it - mA : it;
-                const int p = g % nr, q = g / nr;
-                double sr = 0.0, sim = 0.0, mr = 0.0, mi = 0.0;
-#pragma unroll
-                for (int k = 0; k < 2; ++k) {
-                    double tr, ti;
-                    cmul(nr_[k * NX + base + p], ni_[k * NX + base + p], wr_[k * NX + base + q],
-                         wi_[k * NX + base + q], tr, ti);
-                    sr += tr;
-                    sim += ti;
-                    // (w Gt0[s, X_q])_k
-                    double zr_ = 0.0, zi_ = 0.0;
-#pragma unroll
-                    for (int k2 = 0; k2 < 2; ++k2) {
-                        double ur, ui;
-                        cmul(rmr[k][k2], rmi[k][k2], rxr[k2 * NX + base + q], rxi[k2 * NX + base + q], ur, ui);
-                        zr_ += ur;
-                        zi_ += ui;
-                    }
-                    cmul(cxr[k * NX + base + p], cxi[k * NX + base + p], zr_, zi_, tr, ti);
-                    mr += tr;
-                    mi += ti;
-                }
-                double *gv = R.gmInv[X];
-                gv[p + (size_t)q * nr] -= sr;
-                gv[(size_t)nr * nr + p + (size_t)q * nr] -= sim;
-                double *M = R.M[r][X];
-                M[p + (size_t)q * nr] += mr;
-                M[(size_t)nr * nr + p + (size_t)q * nr] += mi;
-            }
+            region_update(R, r, nA, nB, NX, nr_, ni_, wr_, wi_, cxr, cxi, rxr, rxi, rmr, rmi);
             // working blocks: with f = w (E_s - Gt[s, S]) (2 x KS) and g = w Gt0[s, X] (2 x nX),
             // Gt[S,S] -= Gt[S,s] f, G0t[X,S] -= G0t[X,s] f, Gt0[S,X] += Gt[S,s] g
             for (int e = tid; e < KS * KS; e += BT) {
@@ -1781,36 +1810,7 @@ __global__ __launch_bounds__(BT) void k_scee_r2_block_uv(SceeR2Args a) {
                 yi[k * KS + t] = syi;
             }
             __syncthreads();
-            for (int it = tid; it < mAB; it += BT) {
-                const int X = it < mA ? 0 : 1, nr = X ? nB : nA, base = X ? nA : 0, g = X ? it - mA : it;
-                const int p = g % nr, q = g / nr;
-                double sr = 0.0, sim = 0.0, mr = 0.0, mi = 0.0;
-#pragma unroll
-                for (int k = 0; k < 2; ++k) {
-                    double tr, ti;
-                    cmul(nr_[k * NX + base + p], ni_[k * NX + base + p], wr_[k * NX + base + q],
-                         wi_[k * NX + base + q], tr, ti);
-                    sr += tr;
-                    sim += ti;
-                    double zr_ = 0.0, zi_ = 0.0;
-#pragma unroll
-                    for (int k2 = 0; k2 < 2; ++k2) {
-                        double ur, ui;
-                        cmul(rmr[k][k2], rmi[k][k2], rxr[k2 * NX + base + q], rxi[k2 * NX + base + q], ur, ui);
-                        zr_ += ur;
-                        zi_ += ui;
-                    }
-                    cmul(cxr[k * NX + base + p], cxi[k * NX + base + p], zr_, zi_, tr, ti);
-                    mr += tr;
-                    mi += ti;
-                }
-                double *gv = R.gmInv[X];
-                gv[p + (size_t)q * nr] -= sr;
-                gv[(size_t)nr * nr + p + (size_t)q * nr] -= sim;
-                double *M = R.M[r][X];
-                M[p + (size_t)q * nr] += mr;
-                M[(size_t)nr * nr + p + (size_t)q * nr] += mi;
-            }
+            region_update(R, r, nA, nB, NX, nr_, ni_, wr_, wi_, cxr, cxi, rxr, rxi, rmr, rmi);
             // working blocks: with f = w (E_s - Gt[s, S]) and g = w Gt0[s, X],
             // Gt[S,S] -= Gt[S,s] f, G0t[X,S] -= G0t[X,s] f, Gt0[S,X] += Gt[S,s] g
             for (int e = tid; e < KS * KS; e += BT) {
