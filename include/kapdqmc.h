@@ -245,8 +245,26 @@ int kdq_get_near_ties(kdq_handle h, double *out);
  *                     wait for the flush to rewrite G. 0: decisions and flushes alternate on
  *                     one stream. Same Markov chain; G agrees to rounding. Sweeps with host
  *                     launches (graph option 0, kernel timing on, kdq_update_layer) always
- *                     use the one-stream order. */
-enum { KDQ_OPT_GRAPH = 0, KDQ_OPT_DELAY = 1, KDQ_OPT_GEMM_TILE = 2, KDQ_OPT_DEFER = 3, KDQ_OPT_PIPE = 4 };
+ *                     use the one-stream order.
+ *  KDQ_OPT_FLUSH      form of the rank-|A| rewrite of G after a block of decisions (every
+ *                     model, swap-replica four-matrix flushes included). 0 (default): one
+ *                     launch, a workgroup per strip of 32 columns forms its part of
+ *                     W (E_A - G[A,:]) and streams the strip's rows. 1: two launches (the
+ *                     product and the gathered columns into scratch, then the rewrite).
+ *                     Identical results.
+ *  KDQ_OPT_ASIDE      t-U chains, captured sweeps. 1 (default): the product of each updated
+ *                     slice into the stabilised chain (U_R forward, U_L backward) runs on a
+ *                     second stream as soon as the slice's fields are final, beside the next
+ *                     slices' wraps and updates; the stabilisation node waits for it. 0: the
+ *                     node forms the block's products itself. Identical results.
+ *  KDQ_OPT_CIRC       1 (default): when every chain's eK and eKinv are block-circulant over
+ *                     its L x L periodic cells (ns = 2 L², checked at kdq_create to 1e-12 of
+ *                     the largest entry; true for every lattice the model constructors build),
+ *                     the kinetic products of the wraps and the chain propagation run through
+ *                     cell momenta (length-L DFTs and 2 x 2 blocks per momentum) instead of
+ *                     dense GEMMs. 0: dense GEMMs. Results agree to rounding. */
+enum { KDQ_OPT_GRAPH = 0, KDQ_OPT_DELAY = 1, KDQ_OPT_GEMM_TILE = 2, KDQ_OPT_DEFER = 3, KDQ_OPT_PIPE = 4,
+       KDQ_OPT_FLUSH = 5, KDQ_OPT_ASIDE = 6, KDQ_OPT_CIRC = 7 };
 int kdq_set_option(kdq_handle h, int32_t key, int32_t value);
 
 /* Mixed precision (kdq_run_desc.precision = KDQ_PREC_MIXED, t-U): G is carried in FP32

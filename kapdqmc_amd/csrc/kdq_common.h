@@ -72,6 +72,12 @@ struct ChainDev {
     const double *rp_u;
     uint32_t rp_sweep0, rp_nsweeps;
     const double *Kim;     // imaginary part of K (complex hopping), nullptr when K is real
+    // block-circulant kinetic factors: with ns = 2 L² sites in L x L periodic cells (site
+    // 2·(x + L·y) + b), eK and eK⁻¹ act as 2 x 2 matrices per cell momentum k. circ holds, for
+    // eK then eK⁻¹, Λ(k) = (1/L²) Σ_d E(d) e^{+i k·d} ([k][a][b] complex, k = kx + L·ky) with
+    // E_ab(d) = the entry of row a (cell 0), column 2d + b. cL = L, or 0 (dense products only).
+    const double *circ;
+    int cL, pad2;
 };
 
 #define KDQ_MEAS_SLOTS 16

@@ -40,6 +40,7 @@ struct ChainHost {
     float *FZ32 = nullptr, *FC32 = nullptr;
     double *DU = nullptr, *DV = nullptr;     // deferred-flush factors of a t-U slice (N x N each)
     double *DI = nullptr;                    // inverses of the 16 x 16 diagonal blocks (Np x 32)
+    double *circ = nullptr;                  // Λ(k) of eK and eK⁻¹ (block-circulant kinetic factors)
     float *DU32 = nullptr, *DV32 = nullptr;
     int *rk = nullptr, *rank = nullptr;
     int *ex = nullptr, *ey = nullptr, *inc = nullptr;
@@ -62,6 +63,8 @@ struct kdq_handle_s {
     int nch = 0, nt = 0, nn = 0, max_ns = 0, max_np = 0, delay = 16;
     int kind = KDQ_MODEL_TU, nobs = 6, max_elem = 0, max_nb = 0;
     bool cK = false;   // complex model matrices (t-U with flux): eK, eK⁻¹, HalfeK, HalfeK⁻¹, K, P
+    int circ_L = 0;    // largest cell edge when every chain's eK, eK⁻¹ are block-circulant (else 0)
+    bool circ = true;  // kinetic products through cell momenta where circ_L > 0 (KDQ_OPT_CIRC)
     // rank-2 block tables (t-V: per family, t-U-V: one group)
     BlockDesc *d_blocks = nullptr;
     int *d_block_offset = nullptr, *d_nblocks = nullptr;
@@ -121,6 +124,7 @@ struct kdq_handle_s {
     // options (kdq_set_option) and the captured sweep
     int use_graph = 1, gemm_tile = 0;
     bool pipe = true;                         // t-U decisions of block b+1 beside block b's flush
+    int flush2 = 0;                           // 1: the two-launch flush (prep + apply)
     // time-displaced G (kdq_G4) scratch, allocated on first use: left chain per block
     // boundary, and N x N work matrices
     std::vector<Opnd *> g4_ul;
@@ -131,6 +135,10 @@ struct kdq_handle_s {
     int mq_dir = 0, mq_have = 0;
     hipStream_t side = nullptr;               // their stream
     hipEvent_t ev_app = nullptr, ev_dec[2] = {nullptr, nullptr};   // their dependencies
+    bool aside = true;                        // t-U chain products beside the sweep (captured sweeps)
+    hipStream_t aux = nullptr;                // their stream
+    hipEvent_t ev_fields = nullptr, ev_chain = nullptr;   // a slice's fields final / a block's products done
+    Opnd *chain_cur = nullptr;                // the last product of the block in progress
     uint32_t *d_sweep = nullptr;              // device sweep counter read by graph replays
     const uint32_t *sweep_dev = nullptr;      // non-null while a sweep is being captured
     hipGraph_t graph = nullptr;
@@ -160,10 +168,14 @@ struct kdq_handle_s {
         }
         for (hipEvent_t e : ev_pool) hipEventDestroy(e);
         if (side) hipStreamSynchronize(side);
+        if (aux) hipStreamSynchronize(aux);
         if (ev_app) hipEventDestroy(ev_app);
         for (hipEvent_t e : ev_dec)
             if (e) hipEventDestroy(e);
+        if (ev_fields) hipEventDestroy(ev_fields);
+        if (ev_chain) hipEventDestroy(ev_chain);
         if (side) hipStreamDestroy(side);
+        if (aux) hipStreamDestroy(aux);
         if (stream) hipStreamDestroy(stream);
     }
 };
@@ -315,6 +327,49 @@ int gemm_f(kdq_handle h, bool ac, bool bc, int opa, int opb, const GemmArgsF &g,
     return KDQ_OK;
 }
 
+// ---- kinetic products: through cell momenta (k_circ) when eK, eK⁻¹ are block-circulant ----
+// t-U only. The t-V and t-U-V measurements wrap a copy of G over up to a block of slices
+// without stabilisation (src/tV/phy_update.jl:207-314), which amplifies any difference in the
+// last bits of the kinetic factors by ~1e5 on some observables; through cell momenta they act
+// as exactly circulant matrices, which the dense eK from the eigen-decomposition is only to
+// ~1e-15, so those models keep the dense products that round like the reference's.
+bool use_circ(kdq_handle h) { return h->circ && h->circ_L > 0 && h->kind == KDQ_MODEL_TU; }
+
+// Y = line ⊙ (out ⊙ op(E) (in ⊙ X)) over the ldim columns of X, E = eK (mat 0) or eK⁻¹ (mat 1);
+// opa as gemm_m (1: transposed, the conjugate transpose for complex hopping). The dense path:
+// gemm_m with A = E, rowS = out, inS = in, colS = line.
+int kin_left(kdq_handle h, int mat, int opa, const Opnd *X, const Opnd *Y, int ldim, DiagSpec in, DiagSpec out,
+             DiagSpec line, int fam) {
+    if (use_circ(h)) {
+        ProfScope ps(h, fam);
+        CircArgs a{h->d_chains, X, Y, mat, opa == 0 ? 0 : (h->cK ? 2 : 1), 0, ldim, in, out, line};
+        HIPCHK(launch_circ(h->stream, h->nch, maxdim(h, ldim), h->circ_L, a));
+        return KDQ_OK;
+    }
+    GemmArgs g = gargs(h, mat ? h->t_eKinv : h->t_eK, X, Y, DIM_NS, ldim, DIM_NS);
+    g.rowS = out;
+    g.inS = in;
+    g.colS = line;
+    return gemm_m(h, true, false, false, true, opa, 0, g, fam);
+}
+
+// Y = line ⊙ ((X ⊙ in) E ⊙ out) over the rows of the ns x ns matrix X (line: per row, in / out:
+// per column). The dense path: gemm_m with B = E, rowS = line, inS = in, colS = out.
+int kin_right(kdq_handle h, int mat, const Opnd *X, const Opnd *Y, DiagSpec in, DiagSpec out, DiagSpec line,
+              int fam) {
+    if (use_circ(h)) {
+        ProfScope ps(h, fam);
+        CircArgs a{h->d_chains, X, Y, mat, 0, 1, DIM_NS, in, out, line};
+        HIPCHK(launch_circ(h->stream, h->nch, h->max_ns, h->circ_L, a));
+        return KDQ_OK;
+    }
+    GemmArgs g = gargs(h, X, mat ? h->t_eKinv : h->t_eK, Y, DIM_NS, DIM_NS, DIM_NS);
+    g.rowS = line;
+    g.inS = in;
+    g.colS = out;
+    return gemm_m(h, false, true, true, false, 0, 0, g, fam);
+}
+
 GemmArgsF gargsf(kdq_handle h, const OpndF *A, const OpndF *B, const OpndF *C) {
     GemmArgsF g{};
     g.chains = h->d_chains;
@@ -338,29 +393,21 @@ int bondrot(kdq_handle h, Opnd *X, int family, int lt, int sign, int mode, int n
 
 // t-V kinetic wrap: G <- eK G eKinv (forward) or eKinv G eK (backward)
 int wrap_K(kdq_handle h, Opnd *Gt, Opnd *Tt, bool forward, int fam) {
-    GemmArgs g1 = gargs(h, forward ? h->t_eK : h->t_eKinv, Gt, Tt, DIM_NS, DIM_NS, DIM_NS);
-    if (int r = gemm_m(h, true, false, false, true, 0, 0, g1, fam)) return r;
-    GemmArgs g2 = gargs(h, Tt, forward ? h->t_eKinv : h->t_eK, Gt, DIM_NS, DIM_NS, DIM_NS);
-    return gemm_m(h, false, true, true, false, 0, 0, g2, fam);
+    const DiagSpec none{};
+    if (int r = kin_left(h, forward ? 0 : 1, 0, Gt, Tt, DIM_NS, none, none, none, fam)) return r;
+    return kin_right(h, forward ? 1 : 0, Tt, Gt, none, none, none, fam);
 }
 
 // G <- D eK G eKinv D⁻¹ (forward) or eKinv D⁻¹ G D eK (backward), D = cis(α η_s) at slice lt.
 int wrap_B(kdq_handle h, Opnd *Gt, Opnd *Tt, int lt, bool forward, int fam = KDQ_PROF_WRAP) {
-    if (forward) {
-        GemmArgs g1 = gargs(h, h->t_eK, Gt, Tt, DIM_NS, DIM_NS, DIM_NS);
-        if (int r = gemm_m(h, true, false, false, true, 0, 0, g1, fam)) return r;
-        GemmArgs g2 = gargs(h, Tt, h->t_eKinv, Gt, DIM_NS, DIM_NS, DIM_NS);
-        g2.rowS = hs_diag(h, lt, +1);
-        g2.colS = hs_diag(h, lt, -1);
-        return gemm_m(h, false, true, true, false, 0, 0, g2, fam);
+    const DiagSpec none{};
+    if (forward) {   // T = eK G, then G = D T eKinv D⁻¹
+        if (int r = kin_left(h, 0, 0, Gt, Tt, DIM_NS, none, none, none, fam)) return r;
+        return kin_right(h, 1, Tt, Gt, none, hs_diag(h, lt, -1), hs_diag(h, lt, +1), fam);
     }
     // D' = D(-): T = eKinv · (D' G) · D'⁻¹, then G = T eK
-    GemmArgs g1 = gargs(h, h->t_eKinv, Gt, Tt, DIM_NS, DIM_NS, DIM_NS);
-    g1.inS = hs_diag(h, lt, -1);
-    g1.colS = hs_diag(h, lt, +1);
-    if (int r = gemm_m(h, true, false, false, true, 0, 0, g1, fam)) return r;
-    GemmArgs g2 = gargs(h, Tt, h->t_eK, Gt, DIM_NS, DIM_NS, DIM_NS);
-    return gemm_m(h, false, true, true, false, 0, 0, g2, fam);
+    if (int r = kin_left(h, 1, 0, Gt, Tt, DIM_NS, hs_diag(h, lt, -1), none, hs_diag(h, lt, +1), fam)) return r;
+    return kin_right(h, 0, Tt, Gt, none, none, none, fam);
 }
 
 // WrapKV!(…, "B") on the FP32 copy of G (mixed precision): the same two GEMMs as wrap_B in
@@ -415,9 +462,8 @@ int prop_R_range(kdq_handle h, Opnd *src, Opnd *dst, int lo, int hi) {
     Opnd *cur = src;
     for (int lt = lo; lt <= hi; ++lt) {
         Opnd *out = (lt == hi && cur != dst) ? dst : ((cur == h->t_P0) ? h->t_P1 : h->t_P0);
-        GemmArgs g = gargs(h, h->t_eK, cur, out, DIM_NS, DIM_NP, DIM_NS);
-        g.rowS = hs_diag(h, lt, +1);
-        if (int r = gemm_m(h, true, false, false, true, 0, 0, g, KDQ_PROF_CHAIN)) return r;
+        if (int r = kin_left(h, 0, 0, cur, out, DIM_NP, DiagSpec{}, hs_diag(h, lt, +1), DiagSpec{}, KDQ_PROF_CHAIN))
+            return r;
         cur = out;
     }
     if (cur != dst) HIPCHK(launch_copy_mat(h->stream, h->nch, h->max_ns, h->max_np, h->d_chains, cur, dst, DIM_NS, DIM_NP, true));
@@ -430,9 +476,8 @@ int prop_L_range(kdq_handle h, Opnd *src, Opnd *dst, int lo, int hi) {
     Opnd *cur = src;
     for (int lt = hi; lt >= lo; --lt) {
         Opnd *out = (lt == lo && cur != dst) ? dst : ((cur == h->t_P0) ? h->t_P1 : h->t_P0);
-        GemmArgs g = gargs(h, h->t_eK, cur, out, DIM_NS, DIM_NP, DIM_NS);
-        g.inS = hs_diag(h, lt, +1, 1);
-        if (int r = gemm_m(h, true, false, false, true, 1, 0, g, KDQ_PROF_CHAIN)) return r;
+        if (int r = kin_left(h, 0, 1, cur, out, DIM_NP, hs_diag(h, lt, +1, 1), DiagSpec{}, DiagSpec{}, KDQ_PROF_CHAIN))
+            return r;
         cur = out;
     }
     if (cur != dst) HIPCHK(launch_copy_mat(h->stream, h->nch, h->max_ns, h->max_np, h->d_chains, cur, dst, DIM_NS, DIM_NP, true));
@@ -445,9 +490,8 @@ int propagate_R(kdq_handle h, Opnd *src, Opnd *dst, int idx, double *logacc = nu
     Opnd *cur = src;
     for (int lt = lo; lt <= hi; ++lt) {
         Opnd *out = (lt == hi) ? dst : ((cur == h->t_P0) ? h->t_P1 : h->t_P0);
-        GemmArgs g = gargs(h, h->t_eK, cur, out, DIM_NS, DIM_NP, DIM_NS);
-        if (h->kind != KDQ_MODEL_TV) g.rowS = hs_diag(h, lt, +1);
-        if (int r = gemm_m(h, true, false, false, true, 0, 0, g, KDQ_PROF_CHAIN)) return r;
+        const DiagSpec rs = h->kind != KDQ_MODEL_TV ? hs_diag(h, lt, +1) : DiagSpec{};
+        if (int r = kin_left(h, 0, 0, cur, out, DIM_NP, DiagSpec{}, rs, DiagSpec{}, KDQ_PROF_CHAIN)) return r;
         if (h->kind == KDQ_MODEL_TV)   // B(lt) = V1 V2 V3 eK: V3 first
             for (int j = 2; j >= 0; --j)
                 if (int r = bondrot(h, out, j, lt, +1, 1, DIM_NP, KDQ_PROF_CHAIN)) return r;
@@ -471,16 +515,55 @@ int propagate_L(kdq_handle h, Opnd *src, Opnd *dst, int idx) {
             HIPCHK(launch_copy_mat(h->stream, h->nch, h->max_ns, h->max_np, h->d_chains, cur, tmp, DIM_NS, DIM_NP, true));
             for (int j = 0; j < 3; ++j)
                 if (int r = bondrot(h, tmp, j, lt, +1, 1, DIM_NP, KDQ_PROF_CHAIN)) return r;
-            GemmArgs g = gargs(h, h->t_eK, tmp, out, DIM_NS, DIM_NP, DIM_NS);
-            if (int r = gemm_m(h, true, false, false, true, 1, 0, g, KDQ_PROF_CHAIN)) return r;
+            if (int r = kin_left(h, 0, 1, tmp, out, DIM_NP, DiagSpec{}, DiagSpec{}, DiagSpec{}, KDQ_PROF_CHAIN))
+                return r;
         } else {
-            GemmArgs g = gargs(h, h->t_eK, cur, out, DIM_NS, DIM_NP, DIM_NS);
-            g.inS = hs_diag(h, lt, +1, 1);   // conj(D): D⁻¹ for unimodular phases, D for real ones
-            if (int r = gemm_m(h, true, false, false, true, 1, 0, g, KDQ_PROF_CHAIN)) return r;
+            // conj(D): D⁻¹ for unimodular phases, D for real ones
+            if (int r = kin_left(h, 0, 1, cur, out, DIM_NP, hs_diag(h, lt, +1, 1), DiagSpec{}, DiagSpec{},
+                                 KDQ_PROF_CHAIN))
+                return r;
         }
         cur = out;
     }
     if (lo > hi) HIPCHK(launch_copy_mat(h->stream, h->nch, h->max_ns, h->max_np, h->d_chains, src, dst, DIM_NS, DIM_NP, true));
+    return orthonormalise(h, dst);
+}
+
+// Chain products beside the sweep (captured t-U sweeps with the pipelined blocks): the product
+// of slice lt into the block's chain — U_R <- B(lt) U_R going forward, U_Lᴴ <- (U_L B(lt))ᴴ
+// going backward, the operands and order of propagate_R / propagate_L — is issued on the aux
+// stream once the slice's fields are final (after its local updates, so after every decision
+// of the slice), and overlaps the wraps and updates of the block's later slices. The node
+// waits for the block's last product (chain_join) and orthonormalises on the handle's stream.
+// Same products in the same order: identical results to the in-line form.
+bool chain_aside(kdq_handle h) {
+    return h->kind == KDQ_MODEL_TU && h->aside && h->sweep_dev && !h->prof_mask && !h->scee && h->aux;
+}
+
+int chain_step(kdq_handle h, int lt, bool forward) {
+    const int b = forward ? h->idx : h->idx - 1;   // the node block holding slice lt
+    const int lo = (int)h->nodes[b - 1] + 1, hi = (int)h->nodes[b];
+    Opnd *src = forward ? h->t_UR[b - 1] : h->t_ULH[b];
+    Opnd *dst = forward ? h->t_UR[b] : h->t_ULH[b - 1];
+    const bool first = forward ? lt == lo : lt == hi, last = forward ? lt == hi : lt == lo;
+    Opnd *cur = first ? src : h->chain_cur;
+    Opnd *out = last ? dst : ((cur == h->t_P0) ? h->t_P1 : h->t_P0);
+    HIPCHK(hipEventRecord(h->ev_fields, h->stream));
+    HIPCHK(hipStreamWaitEvent(h->aux, h->ev_fields, 0));
+    const DiagSpec none{};
+    std::swap(h->stream, h->aux);   // the product goes to the aux stream
+    const int r = forward ? kin_left(h, 0, 0, cur, out, DIM_NP, none, hs_diag(h, lt, +1), none, KDQ_PROF_CHAIN)
+                          : kin_left(h, 0, 1, cur, out, DIM_NP, hs_diag(h, lt, +1, 1), none, none, KDQ_PROF_CHAIN);
+    std::swap(h->stream, h->aux);
+    if (r) return r;
+    h->chain_cur = out;
+    if (last) HIPCHK(hipEventRecord(h->ev_chain, h->aux));
+    return KDQ_OK;
+}
+
+// the node after a block of chain_step products: dst = QR_Q of the block's product
+int chain_join(kdq_handle h, Opnd *dst) {
+    HIPCHK(hipStreamWaitEvent(h->stream, h->ev_chain, 0));
     return orthonormalise(h, dst);
 }
 
@@ -698,6 +781,7 @@ int update_family_r2(kdq_handle h, int lt, int dir, uint32_t sweep, int family) 
         FlushArgs f{h->d_chains, h->t_G[h->cur], h->t_G[1 - h->cur], 64, 0, nullptr, nullptr, h->t_FZ, h->t_FC};   // two slots per element
         {
             ProfScope ps(h, KDQ_PROF_FLUSH);
+            f.two_launch = h->flush2;
             HIPCHK(launch_flush(h->stream, h->nch, h->max_ns, f));
         }
         h->cur ^= 1;
@@ -770,11 +854,13 @@ int update_layer_pipe(kdq_handle h, int lt, int dir, uint32_t sweep) {
         if (mx) {
             FlushArgsF f{h->d_chains, h->t_G32[h->cur32], h->t_G32[1 - h->cur32], kb, 0, nullptr, nullptr,
                          h->t_FZ32, h->t_FC32, slot};
+            f.two_launch = h->flush2;
             HIPCHK(launch_flush_f32(h->stream, h->nch, h->max_ns, f));
             h->cur32 ^= 1;
         } else {
             FlushArgs f{h->d_chains, h->t_G[h->cur], h->t_G[1 - h->cur], kb, 0, nullptr, nullptr, h->t_FZ, h->t_FC,
                         slot};
+            f.two_launch = h->flush2;
             HIPCHK(launch_flush(h->stream, h->nch, h->max_ns, f));
             h->cur ^= 1;
         }
@@ -824,6 +910,7 @@ int update_layer(kdq_handle h, int lt, int dir, uint32_t sweep) {
         if (h->mixed) {
             FlushArgsF f{h->d_chains, h->t_G32[h->cur32], h->t_G32[1 - h->cur32], kb, 0, nullptr, nullptr, h->t_FZ32, h->t_FC32};
             ProfScope ps(h, KDQ_PROF_FLUSH);
+            f.two_launch = h->flush2;
             HIPCHK(launch_flush_f32(h->stream, h->nch, h->max_ns, f));
             h->cur32 ^= 1;
             continue;
@@ -831,6 +918,7 @@ int update_layer(kdq_handle h, int lt, int dir, uint32_t sweep) {
         FlushArgs f{h->d_chains, h->t_G[h->cur], h->t_G[1 - h->cur], kb, 0, nullptr, nullptr, h->t_FZ, h->t_FC};
         {
             ProfScope ps(h, KDQ_PROF_FLUSH);
+            f.two_launch = h->flush2;
             HIPCHK(launch_flush(h->stream, h->nch, h->max_ns, f));
         }
         h->cur ^= 1;
@@ -1113,6 +1201,7 @@ int sweep_body(kdq_handle h, int record, uint32_t sweep) {
     const bool tv = h->kind == KDQ_MODEL_TV;
     (void)NN;
     if (record) HIPCHK(launch_zero_meas(h->stream, h->nch, h->d_chains));
+    const bool aside = chain_aside(h);
     h->mq_dir = 0;
     for (int lt = 1; lt <= nt; ++lt) {
         if (tv) {
@@ -1120,12 +1209,18 @@ int sweep_body(kdq_handle h, int record, uint32_t sweep) {
         } else {
             if (int r = h->mixed ? wrap_B_f32(h, lt, true) : wrap_B(h, h->t_G[h->cur], h->t_T, lt, true)) return r;
             if (int r = update_layer(h, lt, 0, sweep)) return r;
+            if (aside)
+                if (int r = chain_step(h, lt, true)) return r;
         }
         if (record && in_window(h, thidx, true))
             if (int r = tv ? measure(h, lt) : measure_sweep(h, lt, true)) return r;
         if (is_node(h, lt)) {
             h->idx += 1;
-            if (int r = propagate_R(h, h->t_UR[h->idx - 2], h->t_UR[h->idx - 1], h->idx - 1)) return r;
+            if (aside) {
+                if (int r = chain_join(h, h->t_UR[h->idx - 1])) return r;
+            } else if (int r = propagate_R(h, h->t_UR[h->idx - 2], h->t_UR[h->idx - 1], h->idx - 1)) {
+                return r;
+            }
             if (int r = node_refresh(h, h->idx)) return r;
         }
     }
@@ -1137,13 +1232,19 @@ int sweep_body(kdq_handle h, int record, uint32_t sweep) {
                 if (int r = measure(h, lt - 1)) return r;
         } else {
             if (int r = update_layer(h, lt, 1, sweep)) return r;
+            if (aside)
+                if (int r = chain_step(h, lt, false)) return r;
             if (record && in_window(h, thidx, false))
                 if (int r = measure_sweep(h, lt, false)) return r;
             if (int r = h->mixed ? wrap_B_f32(h, lt, false) : wrap_B(h, h->t_G[h->cur], h->t_T, lt, false)) return r;
         }
         if (is_node(h, lt - 1)) {
             h->idx -= 1;
-            if (int r = propagate_L(h, h->t_ULH[h->idx], h->t_ULH[h->idx - 1], h->idx)) return r;
+            if (aside) {
+                if (int r = chain_join(h, h->t_ULH[h->idx - 1])) return r;
+            } else if (int r = propagate_L(h, h->t_ULH[h->idx], h->t_ULH[h->idx - 1], h->idx)) {
+                return r;
+            }
             if (int r = node_refresh(h, h->idx)) return r;
         }
     }
@@ -1239,6 +1340,53 @@ const char *kdq_last_error(kdq_handle h) {
     if (h) return h->err.c_str();
     std::lock_guard<std::mutex> lk(g_err_mu);
     return g_err.c_str();
+}
+
+// Block-circulant kinetic factors: with ns = 2 L² (site 2·(x + L·y) + b on an L x L periodic
+// cell lattice), checks that eK and eK⁻¹ satisfy E[(c,a),(c',b)] = E_ab(c' - c) to 1e-12 of
+// their largest entry (the model constructors' eigen-decompositions leave ~1e-15) and returns L
+// with tab = [eK, eK⁻¹][k][a][b] complex Λ(k) = (1/L²) Σ_d E_ab(d) e^{+2πi (kx dx + ky dy)/L};
+// 0 when the matrices are not block-circulant or L has no kernel instantiation.
+int circ_table(const kdq_model_desc &m, bool cplx, std::vector<double> &tab) {
+    const int L = m.site[0];
+    if (m.site[1] != L || L < 1 || m.ns != 2 * L * L || !circ_supported(L)) return 0;
+    const int ns = m.ns, nc = L * L;
+    const size_t nn = (size_t)ns * ns;
+    tab.assign((size_t)2 * nc * 8, 0.0);
+    for (int mat = 0; mat < 2; ++mat) {
+        const double *re = mat ? m.eKinv : m.eK;
+        const double *im = cplx ? (mat ? m.eKinv_im : m.eK_im) : nullptr;
+        double big = 0.0;
+        for (size_t q = 0; q < nn; ++q) big = std::max(big, std::abs(re[q]) + (im ? std::abs(im[q]) : 0.0));
+        const double tol = 1e-12 * big;
+        for (int j = 0; j < ns; ++j)
+            for (int i = 0; i < ns; ++i) {
+                const int ci = i / 2, a = i % 2, cj = j / 2, b = j % 2;
+                const int dx = ((cj % L) - (ci % L) + L) % L, dy = ((cj / L) - (ci / L) + L) % L;
+                const size_t o = (size_t)i + (size_t)j * ns, o0 = (size_t)a + (size_t)(2 * (dx + L * dy) + b) * ns;
+                if (std::abs(re[o] - re[o0]) > tol || (im && std::abs(im[o] - im[o0]) > tol)) return 0;
+            }
+        for (int k = 0; k < nc; ++k) {
+            const int kx = k % L, ky = k / L;
+            for (int ab = 0; ab < 4; ++ab) {
+                const int a = ab >> 1, b = ab & 1;
+                double sr = 0.0, si = 0.0;
+                for (int d = 0; d < nc; ++d) {
+                    const int dx = d % L, dy = d / L;
+                    const size_t o = (size_t)a + (size_t)(2 * d + b) * ns;
+                    const double er = re[o], ei = im ? im[o] : 0.0;
+                    const double ph = 2.0 * M_PI * (double)((kx * dx + ky * dy) % L) / L;
+                    const double cr = std::cos(ph), cs = std::sin(ph);
+                    sr += er * cr - ei * cs;
+                    si += er * cs + ei * cr;
+                }
+                double *t = tab.data() + ((size_t)mat * nc + k) * 8 + 2 * ab;
+                t[0] = sr / nc;
+                t[1] = si / nc;
+            }
+        }
+    }
+    return L;
 }
 
 int kdq_create(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t nchains, kdq_handle *out) {
@@ -1428,6 +1576,9 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_app, hipEventDisableTiming);
     for (int k = 0; k < 2 && e == hipSuccess; ++k) e = hipEventCreateWithFlags(&h->ev_dec[k], hipEventDisableTiming);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->aux, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_fields, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_chain, hipEventDisableTiming);
     if (e != hipSuccess) {
         int r = fail(h, KDQ_EHIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
         delete h;
@@ -1441,6 +1592,7 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
     std::vector<BlockDesc> blk_all;
     std::vector<int> blk_offset, blk_count;
     std::vector<std::vector<Opnd>> v_UR(h->nn), v_ULH(h->nn);
+    std::vector<int> circ_L(nchains, 0);
     for (int c = 0; c < nchains; ++c) {
         const kdq_model_desc &m = models[c];
         ChainHost &C = h->ch[c];
@@ -1570,6 +1722,18 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
             hipMemcpy(C.K + nn, m.K_im, nn * 8, hipMemcpyHostToDevice);
             hipMemcpy(C.Pt + nP, m.Pt_im, nP * 8, hipMemcpyHostToDevice);
         }
+        {
+            std::vector<double> tab;
+            circ_L[c] = circ_table(m, h->cK, tab);
+            if (circ_L[c]) {
+                if (dalloc(C, &C.circ, tab.size()) != hipSuccess) {
+                    int r = fail(h, KDQ_EHIP, "kdq_create: device allocation failed");
+                    delete h;
+                    return r;
+                }
+                hipMemcpy(C.circ, tab.data(), tab.size() * 8, hipMemcpyHostToDevice);
+            }
+        }
         if (m.alpha) hipMemcpy(C.alpha, m.alpha, (size_t)m.nt * 8, hipMemcpyHostToDevice);
         else {
             std::vector<double> ones((size_t)m.nt, 1.0);
@@ -1670,6 +1834,12 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
             v_UR[k].push_back(cplx(C.UR[k], n, nP));
             v_ULH[k].push_back(cplx(C.ULH[k], n, nP));
         }
+    }
+    // kinetic products through cell momenta when every chain's factors are block-circulant
+    h->circ_L = *std::min_element(circ_L.begin(), circ_L.end()) > 0 ? *std::max_element(circ_L.begin(), circ_L.end()) : 0;
+    for (int c = 0; c < nchains; ++c) {
+        cd[c].cL = h->circ_L ? circ_L[c] : 0;
+        cd[c].circ = h->circ_L ? h->ch[c].circ : nullptr;
     }
     {
         void *q = nullptr;
@@ -2251,6 +2421,16 @@ int kdq_set_option(kdq_handle h, int32_t key, int32_t value) {
         case KDQ_OPT_PIPE:
             h->pipe = value != 0;
             break;
+        case KDQ_OPT_CIRC:
+            h->circ = value != 0;
+            break;
+        case KDQ_OPT_ASIDE:
+            h->aside = value != 0;
+            break;
+        case KDQ_OPT_FLUSH:
+            if (value < 0 || value > 2) return fail(h, KDQ_EARG, "kdq_set_option: flush form is 0 (one launch) or 1 (two launches)");
+            h->flush2 = value;
+            break;
         default:
             return fail(h, KDQ_EARG, "kdq_set_option: unknown option");
     }
@@ -2589,7 +2769,10 @@ int scee_flush(kdq_handle h, int kmax = 0) {
         {h->d_chains, xcur(h, SX_G0), xspare(h, SX_G0), kb, 1, xcur(h, SX_G0T), xcur(h, SX_GT0), h->t_FZ, h->t_FC},   // G0
     };
     ProfScope ps(h, KDQ_PROF_FLUSH);
-    for (int k = 0; k < 4; ++k) HIPCHK(launch_flush(h->stream, h->nch, h->max_ns, f[k]));
+    for (int k = 0; k < 4; ++k) {
+        f[k].two_launch = h->flush2;
+        HIPCHK(launch_flush(h->stream, h->nch, h->max_ns, f[k]));
+    }
     h->cur ^= 1;
     for (int k = 0; k < 3; ++k) xflip(h, k);
     return KDQ_OK;

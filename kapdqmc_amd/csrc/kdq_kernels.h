@@ -28,6 +28,21 @@ hipError_t launch_gemm(hipStream_t st, int nchains, int max_m, int max_n, int ma
 hipError_t launch_gemm_f32(hipStream_t st, int nchains, int max_m, int max_n, int max_k, bool a_cplx, bool b_cplx,
                            int opa, int opb, const GemmArgsF &g, int tile = 0);
 
+// Kinetic product through cell momenta (block-circulant eK / eK⁻¹, ChainDev::circ, cL > 0 for
+// every chain): per chain, Y = line ⊙ (out ⊙ op(E) (in ⊙ X)) over the nline_dim columns of X
+// (rows = 0), or Y = line ⊙ ((X ⊙ in) op(E) ⊙ out) over its rows (rows = 1, op(E) acting from
+// the right), E = eK (mat 0) or eK⁻¹ (mat 1), op 0: E, 1: Eᵀ, 2: Eᴴ. The diagonal factors are
+// those of the GEMM path (DiagSpec; kind 0 = none). Y may be X.
+struct CircArgs {
+    const ChainDev *chains;
+    const Opnd *X;
+    const Opnd *Y;
+    int mat, op, rows, nline_dim;
+    DiagSpec in, out, line;
+};
+bool circ_supported(int L);   // cell edges with a kernel instantiation
+hipError_t launch_circ(hipStream_t st, int nchains, int max_lines, int max_L, const CircArgs &a);
+
 // Local update of one block of sites [s0, s0 + kb) at slice lt (t-U model).
 struct DecideArgs {
     const ChainDev *chains;
@@ -67,9 +82,13 @@ hipError_t launch_defer_xy(hipStream_t st, int nchains, int max_ns, const DeferA
 //   mode 0: Xout = Xin - Cin[:,A] · W · (E_A - Rin[A,:])
 //   mode 1: Xout = Xin + Cin[:,A] · W · Rin[A,:]
 // The plain sweep uses mode 0 with Cin = Rin = Xin = G. Null Cin / Rin mean Xin.
-// Two launches: Z = W · Y (|A| x N) and the gathered columns ±Cin[:,A] (N x |A|) into the
-// scratch operands Zs (ld 64) and Cs, then Xout = Xin - Cs · Zs of rank |A| (the accepted
-// count, read on the device).
+// Default (two_launch = 0): one launch; each workgroup owns a strip of columns J of one
+// chain, forms Z[:,J] = W · Y[:,J] in LDS and streams the strip's rows through
+// Xout = Xin - Cin[:,A] · Z[:,J] (the gathered columns read straight from Cin). Xout must
+// alias none of Xin, Cin, Rin (every caller writes a spare buffer).
+// two_launch = 1: Z = W · Y (|A| x N) and the gathered columns ±Cin[:,A] (N x |A|) into the
+// scratch operands Zs (ld 64) and Cs, then Xout = Xin - Cs · Zs of rank |A|. Both forms read
+// the accepted count on the device and give identical results (same sums, same order).
 template <class T>
 struct FlushArgsT {
     const ChainDev *chains;
@@ -78,9 +97,10 @@ struct FlushArgsT {
     int kmax, mode;     // largest accepted set the preceding decisions can produce (<= 64)
     const typename OpndOf<T>::type *Cin;
     const typename OpndOf<T>::type *Rin;
-    const typename OpndOf<T>::type *Zs;     // scratch: 64 x N per chain
-    const typename OpndOf<T>::type *Cs;     // scratch: N x 64 per chain
+    const typename OpndOf<T>::type *Zs;     // scratch: 64 x N per chain (two-launch form)
+    const typename OpndOf<T>::type *Cs;     // scratch: N x 64 per chain (two-launch form)
     int slot;           // the rank, accepted sites and W of slot `slot`
+    int two_launch;     // 1: the prep + apply form
 };
 typedef FlushArgsT<double> FlushArgs;
 typedef FlushArgsT<float> FlushArgsF;

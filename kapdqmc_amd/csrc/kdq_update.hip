@@ -406,8 +406,8 @@ __global__ __launch_bounds__(256) void k_flush_prep(FlushArgsT<T> f) {
         for (int u = 0; u < na; ++u) {
             const T ar = Wr[t * K + u], ai = Wi[t * K + u];
             const T br = Yr[u * PC + jj], bi = Yi[u * PC + jj];
-            sr += ar * br - ai * bi;
-            si += ar * bi + ai * br;
+            sr = fma(ar, br, fma(-ai, bi, sr));   // explicit order: the one-launch form repeats it
+            si = fma(ar, bi, fma(ai, br, si));
         }
         const size_t oz = (size_t)t + (size_t)j * Z.ld;
         Z.re[oz] = sr;
@@ -532,6 +532,196 @@ __global__ __launch_bounds__(256) void k_flush_apply(FlushArgsT<T> f, int nch, i
                 Go.im[oo] = xi[a][b] - ai[a][b];
             }
         }
+}
+
+// Flush in one launch: workgroup (chain c, strip sx) owns columns J = [32 sx, 32 sx + 32).
+//  1. Y[u, J] = edge·δ(A_u, J) - sgn·Rin[A_u, J] (the strip's entries of the accepted rows)
+//     into LDS, then Z[:, J] = sgn · W · Y[:, J] (thread: one column, ranks t = t_g + 8q),
+//     with W staged in LDS 16 columns at a time.
+//  2. The strip's rows in chunks of SRM = 96: thread (ri = tid % 32, cg = tid / 32) owns rows
+//     ri + 32a (a < 3) and columns cg + 8b (b < 4); the chunk's entries of the gathered
+//     columns Cin[I, A] pass through LDS 16 ranks at a time, each stage loaded into registers
+//     while the previous one is used, and Xout = Xin - Cin[I,A]·Z.
+// The sign sits on Z here and on the gathered columns in the two-launch form: multiplying by
+// ±1 is exact, and both forms run the same fma sequences, so they give the same bits. No
+// scratch operands and no second launch.
+constexpr int SW = 32, SRA = 3, SRM = 32 * SRA, SCP = AK * SRM / 256;   // SCP: staged C entries per thread
+template <class T>
+size_t strip_lds_bytes(int K) {
+    typedef typename Cplx2<T>::type C2;
+    const size_t z = (size_t)K * (SW + 1), y = (size_t)K * (SW + 1), u = y > (size_t)AK * SRM ? y : (size_t)AK * SRM;
+    return (z + u) * sizeof(C2);
+}
+template <class T>
+__global__ __launch_bounds__(256, 3) void k_flush_strip(FlushArgsT<T> f, int nch, int strips, int rparts, int rpw,
+                                                        int xcd_map) {
+    typedef typename Cplx2<T>::type C2;
+    int c, tile;
+    if (xcd_map) {   // every tile of chain c on XCD c % 8 (one L2 holds the chain's operands)
+        const int L = blockIdx.x, x = L % NXCD, k = L / NXCD;
+        c = x + NXCD * (k / (strips * rparts));
+        tile = k % (strips * rparts);
+        if (c >= nch) return;
+    } else {
+        c = blockIdx.y;
+        tile = blockIdx.x;
+    }
+    const int sx = tile % strips, rp = tile / strips;
+    const ChainDev &ch = f.chains[c];
+    const int n = ch.ns, j0 = sx * SW, rbeg = rp * rpw, rend = min(n, rbeg + rpw);
+    if (j0 >= n || rbeg >= n) return;   // uniform per workgroup
+    const auto Gi = f.Gin[c], Go = f.Gout[c];
+    const auto Ci = f.Cin ? f.Cin[c] : Gi, Ri = f.Rin ? f.Rin[c] : Gi;
+    const int tid = threadIdx.x, ri = tid & 31, cg = tid >> 5;
+    T xr[SRA][4], xi[SRA][4];
+    auto load_x = [&](int r0) {
+#pragma unroll
+        for (int a = 0; a < SRA; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int i = r0 + ri + 32 * a, j = j0 + cg + 8 * b;
+                const bool ok = i < rend && j < n;
+                const size_t oi = (size_t)i + (size_t)j * Gi.ld;
+                xr[a][b] = ok ? Gi.re[oi] : T(0);
+                xi[a][b] = ok ? Gi.im[oi] : T(0);
+            }
+    };
+    const int K = f.kmax;
+    const int na = min(ch.acc_cnt[f.slot], K);   // the decisions accept at most kmax sites
+    const T sgn = f.mode == 0 ? T(1) : T(-1);
+    const T edge = f.mode == 0 ? T(1) : T(0);
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    C2 *Zs = reinterpret_cast<C2 *>(lds_raw);   // [t][jj], stride SW + 1 (W chunks [t][u] before)
+    C2 *Us = Zs + (size_t)K * (SW + 1);           // Y [u][jj] (stride SW + 1), then C stages [t][r] (stride SRM)
+    __shared__ int aidx[KB];
+    if (na > 0) {
+        if (tid < na) aidx[tid] = ch.acc_idx[f.slot * KB + tid];
+        __syncthreads();
+        for (int e = tid; e < na * SW; e += 256) {
+            const int u = e % na, jj = e / na, j = j0 + jj;   // u fastest: the accepted rows are neighbours
+            C2 y;
+            y.x = T(0);
+            y.y = T(0);
+            if (j < n) {
+                const size_t off = (size_t)aidx[u] + (size_t)j * Ri.ld;
+                y.x = (aidx[u] == j ? edge : T(0)) - sgn * Ri.re[off];
+                y.y = -sgn * Ri.im[off];
+            }
+            Us[u * (SW + 1) + jj] = y;
+        }
+        const double *Wg = ch.W + (size_t)f.slot * 2 * KB * KB;
+        const int jj = tid % SW, tg = tid / SW;
+        T sr[KB / 8], si[KB / 8];
+#pragma unroll
+        for (int q = 0; q < KB / 8; ++q) {
+            sr[q] = T(0);
+            si[q] = T(0);
+        }
+        C2 *Ws = Zs;   // W[t, u0 + uu] at Ws[t * 17 + uu] (padded: the stores run down a column)
+        for (int u0 = 0; u0 < na; u0 += 16) {
+            const int uc = min(16, na - u0);
+            __syncthreads();   // Y stored (first chunk); the previous chunk's reads done
+            for (int e = tid; e < na * 16; e += 256) {
+                const int t = e % na, uu = e / na;
+                C2 w;
+                w.x = T(0);
+                w.y = T(0);
+                if (uu < uc) {
+                    w.x = (T)Wg[t + (u0 + uu) * KB];
+                    w.y = (T)Wg[KB * KB + t + (u0 + uu) * KB];
+                }
+                Ws[t * 17 + uu] = w;
+            }
+            __syncthreads();
+            for (int uu = 0; uu < uc; ++uu) {
+                const C2 y = Us[(u0 + uu) * (SW + 1) + jj];
+#pragma unroll
+                for (int q = 0; q < KB / 8; ++q) {
+                    const int t = tg + 8 * q;
+                    if (t < na) {
+                        const C2 w = Ws[t * 17 + uu];
+                        sr[q] = fma(w.x, y.x, fma(-w.y, y.y, sr[q]));
+                        si[q] = fma(w.x, y.y, fma(w.y, y.x, si[q]));
+                    }
+                }
+            }
+        }
+        __syncthreads();   // W chunks and Y are dead
+#pragma unroll
+        for (int q = 0; q < KB / 8; ++q) {
+            const int t = tg + 8 * q;
+            if (t < na) {
+                C2 z;
+                z.x = sgn * sr[q];
+                z.y = sgn * si[q];
+                Zs[t * (SW + 1) + jj] = z;
+            }
+        }
+    }
+    // stages (row chunk r0, ranks t0 .. t0 + 15) in order; stage s+1 is loaded into registers
+    // while stage s is used
+    const int nrk = (na + AK - 1) / AK;
+    C2 cst[SCP];
+    auto load_c = [&](int r0, int t0) {
+        const int tk = min(AK, na - t0);
+#pragma unroll
+        for (int q = 0; q < SCP; ++q) {
+            const int e = tid + 256 * q, r = e % SRM, t = e / SRM;
+            cst[q].x = T(0);
+            cst[q].y = T(0);
+            if (t < tk && r0 + r < n) {
+                const size_t oc = (size_t)(r0 + r) + (size_t)aidx[t0 + t] * Ci.ld;
+                cst[q].x = Ci.re[oc];
+                cst[q].y = Ci.im[oc];
+            }
+        }
+    };
+    if (nrk > 0) load_c(rbeg, 0);
+    for (int r0 = rbeg; r0 < rend; r0 += SRM) {
+        T ar[SRA][4], ai[SRA][4];
+#pragma unroll
+        for (int a = 0; a < SRA; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                ar[a][b] = T(0);
+                ai[a][b] = T(0);
+            }
+        for (int k = 0; k < nrk; ++k) {
+            const int t0 = k * AK, tk = min(AK, na - t0);
+            __syncthreads();   // Z stored / the previous stage's reads done
+#pragma unroll
+            for (int q = 0; q < SCP; ++q) Us[tid + 256 * q] = cst[q];
+            __syncthreads();
+            if (k + 1 < nrk) load_c(r0, t0 + AK);
+            else if (r0 + SRM < rend) load_c(r0 + SRM, 0);
+            for (int t = 0; t < tk; ++t) {
+                C2 cv[SRA], zv[4];
+#pragma unroll
+                for (int a = 0; a < SRA; ++a) cv[a] = Us[t * SRM + ri + 32 * a];
+#pragma unroll
+                for (int b = 0; b < 4; ++b) zv[b] = Zs[(t0 + t) * (SW + 1) + cg + 8 * b];
+#pragma unroll
+                for (int a = 0; a < SRA; ++a)
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) {
+                        ar[a][b] = fma(cv[a].x, zv[b].x, fma(-cv[a].y, zv[b].y, ar[a][b]));
+                        ai[a][b] = fma(cv[a].x, zv[b].y, fma(cv[a].y, zv[b].x, ai[a][b]));
+                    }
+            }
+        }
+        load_x(r0);   // after the products: the registers go to the accumulators and stages
+#pragma unroll
+        for (int a = 0; a < SRA; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int i = r0 + ri + 32 * a, j = j0 + cg + 8 * b;
+                if (i < rend && j < n) {
+                    const size_t oo = (size_t)i + (size_t)j * Go.ld;
+                    Go.re[oo] = xr[a][b] - ar[a][b];
+                    Go.im[oo] = xi[a][b] - ai[a][b];
+                }
+            }
+    }
 }
 
 // Deferred flush, per block: chunk of XR rows (for X) and XR columns (for T = G[A, J]) of one
@@ -698,7 +888,25 @@ hipError_t launch_decide_tu(hipStream_t st, int nchains, const DecideArgs &a) {
 
 template <class T>
 hipError_t flush_t(hipStream_t st, int nchains, int max_ns, const FlushArgsT<T> &f) {
-    if (f.kmax < 1 || f.kmax > KB || !f.Zs || !f.Cs || f.slot < 0 || f.slot > 1) return hipErrorInvalidValue;
+    if (f.kmax < 1 || f.kmax > KB || f.slot < 0 || f.slot > 1) return hipErrorInvalidValue;
+    if (f.two_launch != 1) {
+        const size_t bytes = strip_lds_bytes<T>(f.kmax);
+        if (hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_flush_strip<T>),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes))
+            return e;
+        const int strips = (max_ns + SW - 1) / SW, rpw = f.two_launch == 2 ? ((max_ns + SRM - 1) / SRM) * SRM : SRM,
+                  rparts = (max_ns + rpw - 1) / rpw;
+        if (nchains >= NXCD) {
+            const int groups = (nchains + NXCD - 1) / NXCD;
+            hipLaunchKernelGGL(k_flush_strip<T>, dim3(NXCD * groups * strips * rparts), dim3(256), bytes, st, f, nchains,
+                               strips, rparts, rpw, 1);
+        } else {
+            hipLaunchKernelGGL(k_flush_strip<T>, dim3(strips * rparts, nchains), dim3(256), bytes, st, f, nchains, strips,
+                               rparts, rpw, 0);
+        }
+        return hipGetLastError();
+    }
+    if (!f.Zs || !f.Cs) return hipErrorInvalidValue;
     const size_t K = (size_t)f.kmax;
     const size_t bytes = (2 * K * K + 2 * K * PC) * sizeof(T);
     if (hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_flush_prep<T>),
