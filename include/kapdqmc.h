@@ -247,16 +247,16 @@ int kdq_get_near_ties(kdq_handle h, double *out);
  *                     launches (graph option 0, kernel timing on, kdq_update_layer) always
  *                     use the one-stream order.
  *  KDQ_OPT_FLUSH      form of the rank-|A| rewrite of G after a block of decisions (every
- *                     model, swap-replica four-matrix flushes included). 0 (default): one
- *                     launch, a workgroup per strip of 32 columns forms its part of
- *                     W (E_A - G[A,:]) and streams the strip's rows. 1: two launches (the
- *                     product and the gathered columns into scratch, then the rewrite).
- *                     Identical results.
- *  KDQ_OPT_ASIDE      t-U chains, captured sweeps. 1 (default): the product of each updated
- *                     slice into the stabilised chain (U_R forward, U_L backward) runs on a
- *                     second stream as soon as the slice's fields are final, beside the next
- *                     slices' wraps and updates; the stabilisation node waits for it. 0: the
- *                     node forms the block's products itself. Identical results.
+ *                     model, swap-replica four-matrix flushes included). 0 (default): two
+ *                     launches (the product W (E_A - G[A,:]) and the gathered columns into
+ *                     scratch, then the rewrite). 1: one launch, a workgroup per strip of 32
+ *                     columns forms its part of the product and streams the strip's rows
+ *                     (faster alone, slower beside the pipelined decisions). Identical results.
+ *  KDQ_OPT_ASIDE      t-U chains, captured sweeps. 0 (default): the stabilisation node forms
+ *                     its block's chain products. 1: the product of each updated slice into
+ *                     the stabilised chain (U_R forward, U_L backward) runs on a third stream
+ *                     as soon as the slice's fields are final, beside the next slices' wraps
+ *                     and updates, and the node waits for it. Identical results.
  *  KDQ_OPT_CIRC       1 (default): when every chain's eK and eKinv are block-circulant over
  *                     its L x L periodic cells (ns = 2 L², checked at kdq_create to 1e-12 of
  *                     the largest entry; true for every lattice the model constructors build),

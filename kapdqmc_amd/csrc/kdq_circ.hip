@@ -6,10 +6,12 @@
 // site s = 2·(x + L·y) + b, E[(c,a), (c',b)] = E_ab(c' - c). A product E·X then factors as
 //     X̂_b(k) = Σ_c e^{-i k·c} X[(c,b)],   Ŷ_a(k) = Σ_b Λ_ab(k) X̂_b(k),   Y[(c,a)] = Σ_k e^{+i k·c} Ŷ_a(k)
 // with Λ(k) = (1/L²) Σ_d E(d) e^{+i k·d} (ChainDev::circ). The cell transforms are two passes of
-// length-L DFTs (along x, then y). Each workgroup holds LPW lines of X in LDS (columns for a
-// left product E·X, rows for a right product X·E = (Eᵀ Xᵀ)ᵀ), transforms them, applies the 2 x 2
-// blocks and transforms back: one read and one write of X, and 4 passes of 2L² outputs with L
-// complex terms each per line (8L³ complex MACs) against (2L²)² for the dense product.
+// length-L FFTs (along x, then y; radix 2, 3, 4, 5 butterflies and one Cooley-Tukey step for
+// L = 6, 8, 9, 12, 15). Each workgroup holds LPW = 96 / L lines of X in LDS (columns for a left
+// product E·X, rows for a right product X·E = (Eᵀ Xᵀ)ᵀ), so that its 192 threads take the 2L
+// transforms of every line one thread each; it transforms them, applies the 2 x 2 blocks and
+// transforms back: one read and one write of X and O(L² log L) flops per line, against (2L²)²
+// for the dense product.
 // op selects E (0), Eᵀ (1: Λ_ba(-k)) or Eᴴ (2: conj Λ_ba(k)). The diagonal factors of the GEMM
 // path are applied in the same places: `in` scales the line entries before the product,
 // `out` the products, `line` the whole line (per column of a left product, per row of a right
@@ -18,87 +20,207 @@
 
 namespace {
 
-constexpr int CT = 256;   // threads per workgroup
+constexpr int CT = 192;   // threads per workgroup: LPW(L) = 96 / L lines x 2L DFTs, one thread each
 
-// One length-L DFT of `base[m * stride]`, m < L, in place: forward (FWD) with e^{-2πi km/L},
-// else e^{+2πi km/L}; twiddle w[e] = e^{-2πi e/L} (wr, wi).
+// Length-L DFTs in registers, in place and in natural order: forward (FWD) X_k = Σ_n x_n
+// e^{-2πi kn/L}, backward with e^{+2πi kn/L}. Radices 2, 3, 4 and 5 by their butterflies;
+// composite L = P·Q by one Cooley-Tukey step (Q transforms of length P over n = Q n1 + n2, the
+// twiddles e^{∓2πi n2 k1/L}, P transforms of length Q; X_{k1 + P k2}) with the twiddle table
+// w[e] = e^{-2πi e/L} (wr, wi) of the full length.
+template <bool FWD>
+KDQ_DEV void mul_i(double &re, double &im) {   // z <- ∓i z
+    const double t = re;
+    if (FWD) {
+        re = im;
+        im = -t;
+    } else {
+        re = -im;
+        im = t;
+    }
+}
+
 template <int L, bool FWD>
-KDQ_DEV void dft1(double2 *base, int stride, const double (&wr)[L], const double (&wi)[L]) {
-    double xr[L], xi[L];
+KDQ_DEV void fft(double *xr, double *xi, const double *wr, const double *wi);
+
+template <int P, int Q, bool FWD>
+KDQ_DEV void fft_pq(double *xr, double *xi, const double *wr, const double *wi) {
+    constexpr int L = P * Q;
+    double yr[L], yi[L];   // y[n2 P + k1]
 #pragma unroll
-    for (int m = 0; m < L; ++m) {
-        const double2 v = base[m * stride];
-        xr[m] = v.x;
-        xi[m] = v.y;
-    }
+    for (int n2 = 0; n2 < Q; ++n2) {
+        double ar[P], ai[P], sr[P], si[P];
 #pragma unroll
-    for (int k = 0; k < L; ++k) {
-        double sr = xr[0], si = xi[0];
-#pragma unroll
-        for (int m = 1; m < L; ++m) {
-            const int e = (k * m) % L;
-            const double cr = wr[e], ci = FWD ? wi[e] : -wi[e];
-            sr = fma(xr[m], cr, fma(-xi[m], ci, sr));
-            si = fma(xr[m], ci, fma(xi[m], cr, si));
+        for (int n1 = 0; n1 < P; ++n1) {
+            ar[n1] = xr[Q * n1 + n2];
+            ai[n1] = xi[Q * n1 + n2];
         }
-        base[k * stride] = make_double2(sr, si);
-    }
-}
-
-// all length-L DFTs of one pass over the LPW lines in LDS (line stride NPAD): along x
-// (elements 2·(x + L·y) + b, stride 2) or along y (stride 2L)
-template <int L, bool FWD, bool ALONG_Y>
-KDQ_DEV void dft_pass(double2 *buf, int nlines, int npad, const double (&wr)[L], const double (&wi)[L]) {
-    const int per_line = 2 * L;   // (y or x, b) pairs
-    for (int q = threadIdx.x; q < nlines * per_line; q += CT) {
-        const int l = q / per_line, r = q % per_line, b = r & 1, o = r >> 1;
-        double2 *base = buf + (size_t)l * npad + (ALONG_Y ? 2 * o + b : 2 * L * o + b);
-        dft1<L, FWD>(base, ALONG_Y ? 2 * L : 2, wr, wi);
-    }
-}
-
-template <int L, int LPW>
-KDQ_DEV void circ_body(const CircArgs &a, const ChainDev &ch, int c, int line0, unsigned char *lds) {
-    constexpr int NC = L * L, NS = 2 * NC, NPAD = NS + 1;
-    const int nl = dim_of(ch, a.nline_dim);
-    const int nlw = min(LPW, nl - line0);
-    double2 *buf = reinterpret_cast<double2 *>(lds);   // [LPW][NPAD]
-    double2 *lam = buf + LPW * NPAD;                    // [NC][2][2]
-    double2 *pin = lam + 4 * NC, *pout = pin + NS;      // position factors
-    double2 *pline = pout + NS;                         // [LPW] line factors
-    const int tid = threadIdx.x;
-    double wr[L], wi[L];
+        // the length-P twiddles are every Q-th entry of the length-L table
 #pragma unroll
-    for (int e = 0; e < L; ++e) {
-        double sn, cs;
-        sincospi(2.0 * e / L, &sn, &cs);
-        wr[e] = cs;
-        wi[e] = -sn;
-    }
-    // the 2 x 2 blocks of E, Eᵀ or Eᴴ acting on a line; a right product X·E acts on the rows
-    // of X with Eᵀ
-    const int eop = a.rows ? 1 : a.op;
-    const double *tab = ch.circ + (size_t)a.mat * NC * 8;
-    for (int e = tid; e < 4 * NC; e += CT) {
-        const int k = e >> 2, ab = e & 3, ia = ab >> 1, ib = ab & 1;
-        int ks = k, sa = ia, sb = ib;
-        if (eop != 0) {   // Λᵀ: (b, a) at -k; Λᴴ: conj (b, a) at k
-            sa = ib;
-            sb = ia;
-            if (eop == 1) {
-                const int kx = k % L, ky = k / L;
-                ks = ((L - kx) % L) + L * ((L - ky) % L);
+        for (int e = 0; e < P; ++e) {
+            sr[e] = wr[Q * e];
+            si[e] = wi[Q * e];
+        }
+        fft<P, FWD>(ar, ai, sr, si);
+#pragma unroll
+        for (int k1 = 0; k1 < P; ++k1) {
+            const int e = (n2 * k1) % L;
+            if (e == 0) {
+                yr[n2 * P + k1] = ar[k1];
+                yi[n2 * P + k1] = ai[k1];
+            } else {
+                const double cr = wr[e], ci = FWD ? wi[e] : -wi[e];
+                yr[n2 * P + k1] = ar[k1] * cr - ai[k1] * ci;
+                yi[n2 * P + k1] = ar[k1] * ci + ai[k1] * cr;
             }
         }
-        const double *t = tab + (size_t)ks * 8 + 2 * (2 * sa + sb);
-        lam[e] = make_double2(t[0], eop == 2 ? -t[1] : t[1]);
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < P; ++k1) {
+        double br[Q], bi[Q], sr[Q], si[Q];
+#pragma unroll
+        for (int n2 = 0; n2 < Q; ++n2) {
+            br[n2] = yr[n2 * P + k1];
+            bi[n2] = yi[n2 * P + k1];
+        }
+#pragma unroll
+        for (int e = 0; e < Q; ++e) {
+            sr[e] = wr[P * e];
+            si[e] = wi[P * e];
+        }
+        fft<Q, FWD>(br, bi, sr, si);
+#pragma unroll
+        for (int k2 = 0; k2 < Q; ++k2) {
+            xr[k1 + P * k2] = br[k2];
+            xi[k1 + P * k2] = bi[k2];
+        }
+    }
+}
+
+template <int L, bool FWD>
+KDQ_DEV void fft(double *xr, double *xi, const double *wr, const double *wi) {
+    if constexpr (L == 1) {
+    } else if constexpr (L == 2) {
+        const double ar = xr[0], ai = xi[0];
+        xr[0] = ar + xr[1];
+        xi[0] = ai + xi[1];
+        xr[1] = ar - xr[1];
+        xi[1] = ai - xi[1];
+    } else if constexpr (L == 3) {
+        constexpr double h = 0.86602540378443864676;   // sin(2π/3)
+        const double s1r = xr[1] + xr[2], s1i = xi[1] + xi[2];
+        double dr = h * (xr[1] - xr[2]), di = h * (xi[1] - xi[2]);
+        mul_i<FWD>(dr, di);
+        const double tr = xr[0] - 0.5 * s1r, ti = xi[0] - 0.5 * s1i;
+        xr[0] += s1r;
+        xi[0] += s1i;
+        xr[1] = tr + dr;
+        xi[1] = ti + di;
+        xr[2] = tr - dr;
+        xi[2] = ti - di;
+    } else if constexpr (L == 4) {
+        const double t0r = xr[0] + xr[2], t0i = xi[0] + xi[2], t1r = xr[0] - xr[2], t1i = xi[0] - xi[2];
+        const double t2r = xr[1] + xr[3], t2i = xi[1] + xi[3];
+        double t3r = xr[1] - xr[3], t3i = xi[1] - xi[3];
+        mul_i<FWD>(t3r, t3i);
+        xr[0] = t0r + t2r;
+        xi[0] = t0i + t2i;
+        xr[2] = t0r - t2r;
+        xi[2] = t0i - t2i;
+        xr[1] = t1r + t3r;
+        xi[1] = t1i + t3i;
+        xr[3] = t1r - t3r;
+        xi[3] = t1i - t3i;
+    } else if constexpr (L == 5) {
+        constexpr double c1 = 0.30901699437494742410, c2 = -0.80901699437494742410;   // cos(2π/5), cos(4π/5)
+        constexpr double s1 = 0.95105651629515357212, s2 = 0.58778525229247312917;    // sin(2π/5), sin(4π/5)
+        const double b1r = xr[1] + xr[4], b1i = xi[1] + xi[4], b2r = xr[2] + xr[3], b2i = xi[2] + xi[3];
+        const double d1r = xr[1] - xr[4], d1i = xi[1] - xi[4], d2r = xr[2] - xr[3], d2i = xi[2] - xi[3];
+        const double t1r = xr[0] + c1 * b1r + c2 * b2r, t1i = xi[0] + c1 * b1i + c2 * b2i;
+        const double t2r = xr[0] + c2 * b1r + c1 * b2r, t2i = xi[0] + c2 * b1i + c1 * b2i;
+        double u1r = s1 * d1r + s2 * d2r, u1i = s1 * d1i + s2 * d2i;
+        double u2r = s2 * d1r - s1 * d2r, u2i = s2 * d1i - s1 * d2i;
+        mul_i<FWD>(u1r, u1i);
+        mul_i<FWD>(u2r, u2i);
+        xr[0] += b1r + b2r;
+        xi[0] += b1i + b2i;
+        xr[1] = t1r + u1r;
+        xi[1] = t1i + u1i;
+        xr[4] = t1r - u1r;
+        xi[4] = t1i - u1i;
+        xr[2] = t2r + u2r;
+        xi[2] = t2i + u2i;
+        xr[3] = t2r - u2r;
+        xi[3] = t2i - u2i;
+    } else if constexpr (L == 6) {
+        fft_pq<2, 3, FWD>(xr, xi, wr, wi);
+    } else if constexpr (L == 8) {
+        fft_pq<2, 4, FWD>(xr, xi, wr, wi);
+    } else if constexpr (L == 9) {
+        fft_pq<3, 3, FWD>(xr, xi, wr, wi);
+    } else if constexpr (L == 12) {
+        fft_pq<4, 3, FWD>(xr, xi, wr, wi);
+    } else if constexpr (L == 15) {
+        fft_pq<3, 5, FWD>(xr, xi, wr, wi);
+    } else {
+        static_assert(L == 0, "no transform of this length");
+    }
+}
+
+// One DFT pass over the LPW lines in LDS (line stride npad), in place: along x (elements
+// 2·(x + L·y) + b of a row, stride 2) or along y (stride 2L + 1: rows of 2L elements are padded
+// to 2L + 1, so that the x pass of neighbouring rows falls on different banks). One thread per
+// (line, row or column, b).
+template <int L, bool FWD, bool ALONG_Y>
+KDQ_DEV void dft_pass(double2 *buf, int nlines, int npad, const double (&wr)[L], const double (&wi)[L]) {
+    const int q = threadIdx.x, l = q / (2 * L), r = q % (2 * L), b = r & 1, o = r >> 1;
+    if (l < nlines) {
+        double2 *base = buf + (size_t)l * npad + (ALONG_Y ? 2 * o + b : (2 * L + 1) * o + b);
+        const int stride = ALONG_Y ? 2 * L + 1 : 2;
+        double xr[L], xi[L];
+#pragma unroll
+        for (int m = 0; m < L; ++m) {
+            const double2 v = base[m * stride];
+            xr[m] = v.x;
+            xi[m] = v.y;
+        }
+        fft<L, FWD>(xr, xi, wr, wi);
+#pragma unroll
+        for (int m = 0; m < L; ++m) base[m * stride] = make_double2(xr[m], xi[m]);
+    }
+    __syncthreads();
+}
+
+KDQ_HD int lpw_of(int L) { return L >= 96 ? 1 : 96 / L; }
+
+template <int L>
+KDQ_DEV void circ_body(const CircArgs &a, const ChainDev &ch, int c, unsigned char *lds) {
+    constexpr int NC = L * L, NS = 2 * NC, NPAD = (2 * L + 1) * L + 1, LPW = 96 / L;
+    const int nl = dim_of(ch, a.nline_dim);
+    const int line0 = blockIdx.x * LPW;
+    if (line0 >= nl) return;   // uniform per workgroup
+    const int nlw = min(LPW, nl - line0);
+    const bool use_in = a.in.kind != 0, use_out = a.out.kind != 0;
+    double2 *buf = reinterpret_cast<double2 *>(lds);   // [LPW][NPAD]: p = 2(x + L y) + b at p + y
+    double2 *pline = buf + LPW * NPAD;                  // [LPW] line factors
+    double2 *pin = pline + LPW;                         // [NS] position factors (when used)
+    double2 *pout = pin + (use_in ? NS : 0);
+    const int tid = threadIdx.x;
+    __shared__ double2 tws[16];   // w[e] = e^{-2πi e/L}, evaluated once per workgroup
+    if (tid < L) {
+        double sn, cs;
+        sincospi(2.0 * tid / L, &sn, &cs);
+        tws[tid] = make_double2(cs, -sn);
     }
     for (int p = tid; p < NS; p += CT) {
         double r, i;
-        diag_value(ch, a.in, p, r, i);
-        pin[p] = make_double2(r, i);
-        diag_value(ch, a.out, p, r, i);
-        pout[p] = make_double2(r, i);
+        if (use_in) {
+            diag_value(ch, a.in, p, r, i);
+            pin[p] = make_double2(r, i);
+        }
+        if (use_out) {
+            diag_value(ch, a.out, p, r, i);
+            pout[p] = make_double2(r, i);
+        }
     }
     if (tid < nlw) {
         double r, i;
@@ -106,91 +228,90 @@ KDQ_DEV void circ_body(const CircArgs &a, const ChainDev &ch, int c, int line0, 
         pline[tid] = make_double2(r, i);
     }
     __syncthreads();
+    double wr[L], wi[L];
+#pragma unroll
+    for (int e = 0; e < L; ++e) {
+        wr[e] = tws[e].x;
+        wi[e] = tws[e].y;
+    }
     const Opnd X = a.X[c], Y = a.Y[c];
     // load (and scale by `in`)
-    if (!a.rows) {   // lines are columns: positions fastest (contiguous)
-        for (int e = tid; e < LPW * NS; e += CT) {
-            const int l = e / NS, p = e % NS;
-            double2 v = make_double2(0.0, 0.0);
-            if (l < nlw) {
-                const size_t o = (size_t)p + (size_t)(line0 + l) * X.ld;
-                const double xr = X.re[o], xi = X.im[o], fr = pin[p].x, fi = pin[p].y;
+    for (int e = tid; e < LPW * NS; e += CT) {
+        // columns: positions fastest (contiguous); rows: the lines of one column together
+        const int l = a.rows ? e % LPW : e / NS, p = a.rows ? e / LPW : e % NS;
+        double2 v = make_double2(0.0, 0.0);
+        if (l < nlw) {
+            const size_t o = a.rows ? (size_t)(line0 + l) + (size_t)p * X.ld : (size_t)p + (size_t)(line0 + l) * X.ld;
+            const double xr = X.re[o], xi = X.im[o];
+            if (use_in) {
+                const double fr = pin[p].x, fi = pin[p].y;
                 v = make_double2(xr * fr - xi * fi, xr * fi + xi * fr);
+            } else {
+                v = make_double2(xr, xi);
             }
-            buf[l * NPAD + p] = v;
         }
-    } else {         // lines are rows: the LPW rows of one column together
-        for (int e = tid; e < LPW * NS; e += CT) {
-            const int l = e % LPW, p = e / LPW;
-            double2 v = make_double2(0.0, 0.0);
-            if (l < nlw) {
-                const size_t o = (size_t)(line0 + l) + (size_t)p * X.ld;
-                const double xr = X.re[o], xi = X.im[o], fr = pin[p].x, fi = pin[p].y;
-                v = make_double2(xr * fr - xi * fi, xr * fi + xi * fr);
-            }
-            buf[l * NPAD + p] = v;
-        }
+        buf[l * NPAD + p + p / (2 * L)] = v;
     }
     __syncthreads();
     dft_pass<L, true, false>(buf, nlw, NPAD, wr, wi);
-    __syncthreads();
     dft_pass<L, true, true>(buf, nlw, NPAD, wr, wi);
-    __syncthreads();
+    // the 2 x 2 blocks of E, Eᵀ or Eᴴ acting on a line (a right product X·E acts on the rows
+    // of X with Eᵀ): Λ(k), Λ_ba(-k) or conj Λ_ba(k), read from the chain's table
+    const int eop = a.rows ? 1 : a.op;
+    const double *tab = ch.circ + (size_t)a.mat * NC * 8;
     for (int e = tid; e < nlw * NC; e += CT) {
         const int l = e / NC, k = e % NC;
-        double2 *u = buf + (size_t)l * NPAD + 2 * k;
+        int ks = k;
+        if (eop == 1) {
+            const int kx = k % L, ky = k / L;
+            ks = ((L - kx) % L) + L * ((L - ky) % L);
+        }
+        const double *t = tab + (size_t)ks * 8;   // (a, b) at t + 2·(2a + b)
+        const double cj = eop == 2 ? -1.0 : 1.0;
+        const bool sw = eop != 0;
+        const double l00r = t[0], l00i = cj * t[1], l11r = t[6], l11i = cj * t[7];
+        const double l01r = sw ? t[4] : t[2], l01i = cj * (sw ? t[5] : t[3]);
+        const double l10r = sw ? t[2] : t[4], l10i = cj * (sw ? t[3] : t[5]);
+        double2 *u = buf + (size_t)l * NPAD + 2 * k + k / L;
         const double2 u0 = u[0], u1 = u[1];
-        const double2 l00 = lam[4 * k], l01 = lam[4 * k + 1], l10 = lam[4 * k + 2], l11 = lam[4 * k + 3];
-        u[0] = make_double2(l00.x * u0.x - l00.y * u0.y + l01.x * u1.x - l01.y * u1.y,
-                            l00.x * u0.y + l00.y * u0.x + l01.x * u1.y + l01.y * u1.x);
-        u[1] = make_double2(l10.x * u0.x - l10.y * u0.y + l11.x * u1.x - l11.y * u1.y,
-                            l10.x * u0.y + l10.y * u0.x + l11.x * u1.y + l11.y * u1.x);
+        u[0] = make_double2(l00r * u0.x - l00i * u0.y + l01r * u1.x - l01i * u1.y,
+                            l00r * u0.y + l00i * u0.x + l01r * u1.y + l01i * u1.x);
+        u[1] = make_double2(l10r * u0.x - l10i * u0.y + l11r * u1.x - l11i * u1.y,
+                            l10r * u0.y + l10i * u0.x + l11r * u1.y + l11i * u1.x);
     }
     __syncthreads();
     dft_pass<L, false, true>(buf, nlw, NPAD, wr, wi);
-    __syncthreads();
     dft_pass<L, false, false>(buf, nlw, NPAD, wr, wi);
-    __syncthreads();
     // store (scaled by `out` and the line factor)
-    if (!a.rows) {
-        for (int e = tid; e < LPW * NS; e += CT) {
-            const int l = e / NS, p = e % NS;
-            if (l >= nlw) continue;
-            const double2 v = buf[l * NPAD + p], f = pout[p], g = pline[l];
-            const double tr = v.x * f.x - v.y * f.y, ti = v.x * f.y + v.y * f.x;
-            const size_t o = (size_t)p + (size_t)(line0 + l) * Y.ld;
-            Y.re[o] = tr * g.x - ti * g.y;
-            Y.im[o] = tr * g.y + ti * g.x;
+    for (int e = tid; e < LPW * NS; e += CT) {
+        const int l = a.rows ? e % LPW : e / NS, p = a.rows ? e / LPW : e % NS;
+        if (l >= nlw) continue;
+        const double2 v = buf[l * NPAD + p + p / (2 * L)], g = pline[l];
+        double tr = v.x, ti = v.y;
+        if (use_out) {
+            const double2 f = pout[p];
+            tr = v.x * f.x - v.y * f.y;
+            ti = v.x * f.y + v.y * f.x;
         }
-    } else {
-        for (int e = tid; e < LPW * NS; e += CT) {
-            const int l = e % LPW, p = e / LPW;
-            if (l >= nlw) continue;
-            const double2 v = buf[l * NPAD + p], f = pout[p], g = pline[l];
-            const double tr = v.x * f.x - v.y * f.y, ti = v.x * f.y + v.y * f.x;
-            const size_t o = (size_t)(line0 + l) + (size_t)p * Y.ld;
-            Y.re[o] = tr * g.x - ti * g.y;
-            Y.im[o] = tr * g.y + ti * g.x;
-        }
+        const size_t o = a.rows ? (size_t)(line0 + l) + (size_t)p * Y.ld : (size_t)p + (size_t)(line0 + l) * Y.ld;
+        Y.re[o] = tr * g.x - ti * g.y;
+        Y.im[o] = tr * g.y + ti * g.x;
     }
 }
 
-template <int LPW>
 __global__ __launch_bounds__(CT) void k_circ(CircArgs a) {
     const int c = blockIdx.y;
     const ChainDev &ch = a.chains[c];
-    const int line0 = blockIdx.x * LPW;
-    if (line0 >= dim_of(ch, a.nline_dim)) return;   // uniform per workgroup
     extern __shared__ __align__(16) unsigned char lds_raw[];
     switch (ch.cL) {
-        case 2: circ_body<2, LPW>(a, ch, c, line0, lds_raw); break;
-        case 3: circ_body<3, LPW>(a, ch, c, line0, lds_raw); break;
-        case 4: circ_body<4, LPW>(a, ch, c, line0, lds_raw); break;
-        case 6: circ_body<6, LPW>(a, ch, c, line0, lds_raw); break;
-        case 8: circ_body<8, LPW>(a, ch, c, line0, lds_raw); break;
-        case 9: circ_body<9, LPW>(a, ch, c, line0, lds_raw); break;
-        case 12: circ_body<12, LPW>(a, ch, c, line0, lds_raw); break;
-        case 15: circ_body<15, LPW>(a, ch, c, line0, lds_raw); break;
+        case 2: circ_body<2>(a, ch, c, lds_raw); break;
+        case 3: circ_body<3>(a, ch, c, lds_raw); break;
+        case 4: circ_body<4>(a, ch, c, lds_raw); break;
+        case 6: circ_body<6>(a, ch, c, lds_raw); break;
+        case 8: circ_body<8>(a, ch, c, lds_raw); break;
+        case 9: circ_body<9>(a, ch, c, lds_raw); break;
+        case 12: circ_body<12>(a, ch, c, lds_raw); break;
+        case 15: circ_body<15>(a, ch, c, lds_raw); break;
         default: break;   // not reached: the host takes the dense path unless every cL is listed
     }
 }
@@ -199,18 +320,18 @@ __global__ __launch_bounds__(CT) void k_circ(CircArgs a) {
 
 bool circ_supported(int L) { return L == 2 || L == 3 || L == 4 || L == 6 || L == 8 || L == 9 || L == 12 || L == 15; }
 
-size_t circ_lds_bytes(int L, int lpw) {
-    const size_t nc = (size_t)L * L, ns = 2 * nc;
-    return sizeof(double2) * ((size_t)lpw * (ns + 1) + 4 * nc + 2 * ns + lpw);
+int circ_lines_per_wg(int L) { return lpw_of(L); }
+
+size_t circ_lds_bytes(int L, bool in, bool out) {
+    const size_t ns = 2 * (size_t)L * L, lpw = (size_t)lpw_of(L), npad = (2 * (size_t)L + 1) * L + 1;
+    return sizeof(double2) * (lpw * npad + lpw + (in ? ns : 0) + (out ? ns : 0));
 }
 
-hipError_t launch_circ(hipStream_t st, int nchains, int max_lines, int max_L, const CircArgs &a) {
-    if (!circ_supported(max_L) || a.op < 0 || a.op > 2 || (a.rows && a.op != 0)) return hipErrorInvalidValue;
-    constexpr int LPW = 8;
-    const size_t bytes = circ_lds_bytes(max_L, LPW);
-    if (hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_circ<LPW>),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes))
+hipError_t launch_circ(hipStream_t st, int nchains, int grid_x, size_t lds_bytes, const CircArgs &a) {
+    if (a.op < 0 || a.op > 2 || (a.rows && a.op != 0) || grid_x < 1) return hipErrorInvalidValue;
+    if (hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_circ),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes))
         return e;
-    hipLaunchKernelGGL(k_circ<LPW>, dim3((max_lines + LPW - 1) / LPW, nchains), dim3(CT), bytes, st, a);
+    hipLaunchKernelGGL(k_circ, dim3(grid_x, nchains), dim3(CT), lds_bytes, st, a);
     return hipGetLastError();
 }

@@ -41,6 +41,7 @@ struct ChainHost {
     double *DU = nullptr, *DV = nullptr;     // deferred-flush factors of a t-U slice (N x N each)
     double *DI = nullptr;                    // inverses of the 16 x 16 diagonal blocks (Np x 32)
     double *circ = nullptr;                  // Λ(k) of eK and eK⁻¹ (block-circulant kinetic factors)
+    int cL = 0;                              // their cell edge (0: dense products)
     float *DU32 = nullptr, *DV32 = nullptr;
     int *rk = nullptr, *rank = nullptr;
     int *ex = nullptr, *ey = nullptr, *inc = nullptr;
@@ -124,7 +125,7 @@ struct kdq_handle_s {
     // options (kdq_set_option) and the captured sweep
     int use_graph = 1, gemm_tile = 0;
     bool pipe = true;                         // t-U decisions of block b+1 beside block b's flush
-    int flush2 = 0;                           // 1: the two-launch flush (prep + apply)
+    int flush1 = 0;                           // 1: the one-launch (strip) flush
     // time-displaced G (kdq_G4) scratch, allocated on first use: left chain per block
     // boundary, and N x N work matrices
     std::vector<Opnd *> g4_ul;
@@ -135,7 +136,7 @@ struct kdq_handle_s {
     int mq_dir = 0, mq_have = 0;
     hipStream_t side = nullptr;               // their stream
     hipEvent_t ev_app = nullptr, ev_dec[2] = {nullptr, nullptr};   // their dependencies
-    bool aside = true;                        // t-U chain products beside the sweep (captured sweeps)
+    bool aside = false;                       // t-U chain products beside the sweep (captured sweeps)
     hipStream_t aux = nullptr;                // their stream
     hipEvent_t ev_fields = nullptr, ev_chain = nullptr;   // a slice's fields final / a block's products done
     Opnd *chain_cur = nullptr;                // the last product of the block in progress
@@ -335,6 +336,17 @@ int gemm_f(kdq_handle h, bool ac, bool bc, int opa, int opb, const GemmArgsF &g,
 // ~1e-15, so those models keep the dense products that round like the reference's.
 bool use_circ(kdq_handle h) { return h->circ && h->circ_L > 0 && h->kind == KDQ_MODEL_TU; }
 
+// grid width and LDS of a k_circ launch over every chain's lines (ldim: DIM_NS or DIM_NP)
+void circ_launch_shape(kdq_handle h, int ldim, bool in, bool out, int &gx, size_t &bytes) {
+    gx = 1;
+    bytes = 0;
+    for (int c = 0; c < h->nch; ++c) {
+        const int L = h->ch[c].cL, lines = ldim == DIM_NP ? h->ch[c].np : h->ch[c].ns, lpw = circ_lines_per_wg(L);
+        gx = std::max(gx, (lines + lpw - 1) / lpw);
+        bytes = std::max(bytes, circ_lds_bytes(L, in, out));
+    }
+}
+
 // Y = line ⊙ (out ⊙ op(E) (in ⊙ X)) over the ldim columns of X, E = eK (mat 0) or eK⁻¹ (mat 1);
 // opa as gemm_m (1: transposed, the conjugate transpose for complex hopping). The dense path:
 // gemm_m with A = E, rowS = out, inS = in, colS = line.
@@ -343,7 +355,10 @@ int kin_left(kdq_handle h, int mat, int opa, const Opnd *X, const Opnd *Y, int l
     if (use_circ(h)) {
         ProfScope ps(h, fam);
         CircArgs a{h->d_chains, X, Y, mat, opa == 0 ? 0 : (h->cK ? 2 : 1), 0, ldim, in, out, line};
-        HIPCHK(launch_circ(h->stream, h->nch, maxdim(h, ldim), h->circ_L, a));
+        int gx = 1;
+        size_t bytes = 0;
+        circ_launch_shape(h, ldim, in.kind != 0, out.kind != 0, gx, bytes);
+        HIPCHK(launch_circ(h->stream, h->nch, gx, bytes, a));
         return KDQ_OK;
     }
     GemmArgs g = gargs(h, mat ? h->t_eKinv : h->t_eK, X, Y, DIM_NS, ldim, DIM_NS);
@@ -360,7 +375,10 @@ int kin_right(kdq_handle h, int mat, const Opnd *X, const Opnd *Y, DiagSpec in, 
     if (use_circ(h)) {
         ProfScope ps(h, fam);
         CircArgs a{h->d_chains, X, Y, mat, 0, 1, DIM_NS, in, out, line};
-        HIPCHK(launch_circ(h->stream, h->nch, h->max_ns, h->circ_L, a));
+        int gx = 1;
+        size_t bytes = 0;
+        circ_launch_shape(h, DIM_NS, in.kind != 0, out.kind != 0, gx, bytes);
+        HIPCHK(launch_circ(h->stream, h->nch, gx, bytes, a));
         return KDQ_OK;
     }
     GemmArgs g = gargs(h, X, mat ? h->t_eKinv : h->t_eK, Y, DIM_NS, DIM_NS, DIM_NS);
@@ -781,7 +799,7 @@ int update_family_r2(kdq_handle h, int lt, int dir, uint32_t sweep, int family) 
         FlushArgs f{h->d_chains, h->t_G[h->cur], h->t_G[1 - h->cur], 64, 0, nullptr, nullptr, h->t_FZ, h->t_FC};   // two slots per element
         {
             ProfScope ps(h, KDQ_PROF_FLUSH);
-            f.two_launch = h->flush2;
+            f.one_launch = h->flush1;
             HIPCHK(launch_flush(h->stream, h->nch, h->max_ns, f));
         }
         h->cur ^= 1;
@@ -854,13 +872,13 @@ int update_layer_pipe(kdq_handle h, int lt, int dir, uint32_t sweep) {
         if (mx) {
             FlushArgsF f{h->d_chains, h->t_G32[h->cur32], h->t_G32[1 - h->cur32], kb, 0, nullptr, nullptr,
                          h->t_FZ32, h->t_FC32, slot};
-            f.two_launch = h->flush2;
+            f.one_launch = h->flush1;
             HIPCHK(launch_flush_f32(h->stream, h->nch, h->max_ns, f));
             h->cur32 ^= 1;
         } else {
             FlushArgs f{h->d_chains, h->t_G[h->cur], h->t_G[1 - h->cur], kb, 0, nullptr, nullptr, h->t_FZ, h->t_FC,
                         slot};
-            f.two_launch = h->flush2;
+            f.one_launch = h->flush1;
             HIPCHK(launch_flush(h->stream, h->nch, h->max_ns, f));
             h->cur ^= 1;
         }
@@ -910,7 +928,7 @@ int update_layer(kdq_handle h, int lt, int dir, uint32_t sweep) {
         if (h->mixed) {
             FlushArgsF f{h->d_chains, h->t_G32[h->cur32], h->t_G32[1 - h->cur32], kb, 0, nullptr, nullptr, h->t_FZ32, h->t_FC32};
             ProfScope ps(h, KDQ_PROF_FLUSH);
-            f.two_launch = h->flush2;
+            f.one_launch = h->flush1;
             HIPCHK(launch_flush_f32(h->stream, h->nch, h->max_ns, f));
             h->cur32 ^= 1;
             continue;
@@ -918,7 +936,7 @@ int update_layer(kdq_handle h, int lt, int dir, uint32_t sweep) {
         FlushArgs f{h->d_chains, h->t_G[h->cur], h->t_G[1 - h->cur], kb, 0, nullptr, nullptr, h->t_FZ, h->t_FC};
         {
             ProfScope ps(h, KDQ_PROF_FLUSH);
-            f.two_launch = h->flush2;
+            f.one_launch = h->flush1;
             HIPCHK(launch_flush(h->stream, h->nch, h->max_ns, f));
         }
         h->cur ^= 1;
@@ -1838,7 +1856,7 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
     // kinetic products through cell momenta when every chain's factors are block-circulant
     h->circ_L = *std::min_element(circ_L.begin(), circ_L.end()) > 0 ? *std::max_element(circ_L.begin(), circ_L.end()) : 0;
     for (int c = 0; c < nchains; ++c) {
-        cd[c].cL = h->circ_L ? circ_L[c] : 0;
+        cd[c].cL = h->ch[c].cL = h->circ_L ? circ_L[c] : 0;
         cd[c].circ = h->circ_L ? h->ch[c].circ : nullptr;
     }
     {
@@ -2428,8 +2446,8 @@ int kdq_set_option(kdq_handle h, int32_t key, int32_t value) {
             h->aside = value != 0;
             break;
         case KDQ_OPT_FLUSH:
-            if (value < 0 || value > 2) return fail(h, KDQ_EARG, "kdq_set_option: flush form is 0 (one launch) or 1 (two launches)");
-            h->flush2 = value;
+            if (value != 0 && value != 1) return fail(h, KDQ_EARG, "kdq_set_option: flush form is 0 (two launches) or 1 (one launch)");
+            h->flush1 = value;
             break;
         default:
             return fail(h, KDQ_EARG, "kdq_set_option: unknown option");
@@ -2770,7 +2788,7 @@ int scee_flush(kdq_handle h, int kmax = 0) {
     };
     ProfScope ps(h, KDQ_PROF_FLUSH);
     for (int k = 0; k < 4; ++k) {
-        f[k].two_launch = h->flush2;
+        f[k].one_launch = h->flush1;
         HIPCHK(launch_flush(h->stream, h->nch, h->max_ns, f[k]));
     }
     h->cur ^= 1;

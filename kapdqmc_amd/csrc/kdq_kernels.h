@@ -41,7 +41,11 @@ struct CircArgs {
     DiagSpec in, out, line;
 };
 bool circ_supported(int L);   // cell edges with a kernel instantiation
-hipError_t launch_circ(hipStream_t st, int nchains, int max_lines, int max_L, const CircArgs &a);
+int circ_lines_per_wg(int L);  // lines of X per workgroup for cell edge L
+size_t circ_lds_bytes(int L, bool in, bool out);   // its dynamic LDS (in / out factors used)
+// grid_x: the largest workgroup count over the chains (lines / lines per workgroup),
+// lds_bytes: the largest circ_lds_bytes over the chains
+hipError_t launch_circ(hipStream_t st, int nchains, int grid_x, size_t lds_bytes, const CircArgs &a);
 
 // Local update of one block of sites [s0, s0 + kb) at slice lt (t-U model).
 struct DecideArgs {
@@ -82,13 +86,13 @@ hipError_t launch_defer_xy(hipStream_t st, int nchains, int max_ns, const DeferA
 //   mode 0: Xout = Xin - Cin[:,A] · W · (E_A - Rin[A,:])
 //   mode 1: Xout = Xin + Cin[:,A] · W · Rin[A,:]
 // The plain sweep uses mode 0 with Cin = Rin = Xin = G. Null Cin / Rin mean Xin.
-// Default (two_launch = 0): one launch; each workgroup owns a strip of columns J of one
-// chain, forms Z[:,J] = W · Y[:,J] in LDS and streams the strip's rows through
-// Xout = Xin - Cin[:,A] · Z[:,J] (the gathered columns read straight from Cin). Xout must
-// alias none of Xin, Cin, Rin (every caller writes a spare buffer).
-// two_launch = 1: Z = W · Y (|A| x N) and the gathered columns ±Cin[:,A] (N x |A|) into the
-// scratch operands Zs (ld 64) and Cs, then Xout = Xin - Cs · Zs of rank |A|. Both forms read
-// the accepted count on the device and give identical results (same sums, same order).
+// Default (one_launch = 0): two launches; Z = W · Y (|A| x N) and the gathered columns
+// ±Cin[:,A] (N x |A|) into the scratch operands Zs (ld 64) and Cs, then Xout = Xin - Cs · Zs
+// of rank |A|. one_launch = 1: each workgroup owns a strip of columns J of one chain, forms
+// Z[:,J] = W · Y[:,J] in LDS and streams the strip's rows through Xout = Xin - Cin[:,A] · Z[:,J]
+// (the gathered columns read straight from Cin); Xout must alias none of Xin, Cin, Rin (every
+// caller writes a spare buffer). Both forms read the accepted count on the device and give
+// identical results (same sums, same order).
 template <class T>
 struct FlushArgsT {
     const ChainDev *chains;
@@ -100,7 +104,7 @@ struct FlushArgsT {
     const typename OpndOf<T>::type *Zs;     // scratch: 64 x N per chain (two-launch form)
     const typename OpndOf<T>::type *Cs;     // scratch: N x 64 per chain (two-launch form)
     int slot;           // the rank, accepted sites and W of slot `slot`
-    int two_launch;     // 1: the prep + apply form
+    int one_launch;     // 1: the strip form
 };
 typedef FlushArgsT<double> FlushArgs;
 typedef FlushArgsT<float> FlushArgsF;

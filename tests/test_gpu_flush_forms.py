@@ -6,8 +6,8 @@ measurement rows must agree bit for bit: t-U FP64 (batches below and above 8 cha
 take the two launch grids of the strip kernel, block sizes 1..64), t-U mixed precision
 (FP32 G), t-V and t-U-V sweeps (rank-2 elements, up to 64 ranks), and the swap-replica
 four-matrix flushes of t-U, t-V and t-U-V (modes 0 and 1, gathered columns and rows from
-other matrices). The oracle comparisons of the default (one-launch) form are the sweep and
-swap-replica parity tests."""
+other matrices). The oracle comparisons of the default (two-launch) form are the sweep and
+swap-replica parity tests; "flush" 1 selects the one-launch form."""
 import numpy as np
 import pytest
 
@@ -57,7 +57,7 @@ def same(a, b, what):
 def test_tu_forms_agree_few_chains(delay):
     print(f"case: flush forms, t-U 3 chains (L = 3/4/6), delay {delay}", flush=True)
     models = [tu(3), tu(4, U=2.0), tu(6, U=6.0)]
-    same(sweep(models, delay, 0), sweep(models, delay, 1), f"k={delay}")
+    same(sweep(models, delay, 1), sweep(models, delay, 0), f"k={delay}")
 
 
 def test_tu_forms_agree_many_chains():

@@ -83,3 +83,25 @@ def test_pipe_graph_replays_repeat():
         assert np.array_equal(f1[c], f0[c])
         assert np.array_equal(g1[c], g0[c])
     assert np.array_equal(r1, r0)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_chain_products_aside_equal_inline(precision):
+    # the chain products of each slice issued on a third stream as soon as the slice's fields
+    # are final (option "aside") are the node's own products in the same order: bit-identical
+    from kapdqmc_amd import dqmc
+    print(f"case: chain products beside the sweep vs at the node, {precision}", flush=True)
+    models = [tu(4), tu(6, U=2.0), tu(3, U=6.0)]
+    res = []
+    for aside in (1, 0):
+        b = dqmc.DQMCBatch(models, [5, 6, 7], [0, 1, 2], delay=16, precision=precision)
+        b.set_option("aside", aside)
+        b.initial_s()
+        rows = b.phy_update(3, True)
+        res.append(([b.get_fields(c) for c in range(b.n)], [b.get_G(c) for c in range(b.n)], rows))
+        b.close()
+    (f1, g1, r1), (f0, g0, r0) = res
+    for c in range(len(models)):
+        assert np.array_equal(f1[c], f0[c])
+        assert np.array_equal(g1[c], g0[c])
+    assert np.array_equal(r1, r0)
