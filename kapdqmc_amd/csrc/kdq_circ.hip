@@ -13,9 +13,10 @@
 // transforms back: one read and one write of X and O(L² log L) flops per line, against (2L²)²
 // for the dense product.
 // op selects E (0), Eᵀ (1: Λ_ba(-k)) or Eᴴ (2: conj Λ_ba(k)). The diagonal factors of the GEMM
-// path are applied in the same places: `in` scales the line entries before the product,
-// `out` the products, `line` the whole line (per column of a left product, per row of a right
-// product). Every line is read completely before it is written, so Y may be X.
+// path (the t-U HS phases, DiagSpec kind 0 or 1) are applied in the same places: `in` scales the
+// line entries before the product, `out` the products, `line` the whole line (per column of a
+// left product, per row of a right product). Every line is read completely before it is
+// written, so Y may be X.
 #include "kdq_kernels.h"
 
 namespace {
@@ -190,42 +191,79 @@ KDQ_DEV void dft_pass(double2 *buf, int nlines, int npad, const double (&wr)[L],
     __syncthreads();
 }
 
+// The y pass with the 2 x 2 blocks between its two directions, in registers: thread (line,
+// kx = o, b) transforms its column forward, takes the other component (b ^ 1) of every
+// momentum from the neighbouring lane, forms y_b(k) = M_bb(k) x_b(k) + M_b,1-b(k) x_1-b(k) and
+// transforms back. M = Λ (eop 0), Λ(-k)ᵀ (eop 1) or Λᴴ (eop 2), from the chain's table
+// (Λ_ij(k) at tab + 8k + 2(2i + j)). Every thread of the workgroup calls it (the lane exchange
+// and one barrier).
+template <int L>
+KDQ_DEV void y_lambda_pass(double2 *buf, int nlines, int npad, const double (&wr)[L], const double (&wi)[L],
+                           const double *tab, int eop) {
+    const int q = threadIdx.x, l = q / (2 * L), r = q % (2 * L), b = r & 1, o = r >> 1;
+    const bool act = l < nlines;
+    double2 *base = buf + (size_t)l * npad + 2 * o + b;
+    constexpr int stride = 2 * L + 1;
+    double xr[L], xi[L];
+#pragma unroll
+    for (int m = 0; m < L; ++m) {
+        double2 v = make_double2(0.0, 0.0);
+        if (act) v = base[m * stride];
+        xr[m] = v.x;
+        xi[m] = v.y;
+    }
+    fft<L, true>(xr, xi, wr, wi);
+    const double cj = eop == 2 ? -1.0 : 1.0;
+    const int own = 2 * (2 * b + b), oth = eop == 0 ? 2 * (2 * b + (1 - b)) : 2 * (2 * (1 - b) + b);
+#pragma unroll
+    for (int m = 0; m < L; ++m) {
+        const double pr = __shfl_xor(xr[m], 1), pi = __shfl_xor(xi[m], 1);
+        int ks = o + L * m;
+        if (eop == 1) ks = ((L - o) % L) + L * ((L - m) % L);
+        const double *t = tab + (size_t)ks * 8;
+        const double ar = t[own], ai = cj * t[own + 1], br = t[oth], bi = cj * t[oth + 1];
+        const double yr = ar * xr[m] - ai * xi[m] + br * pr - bi * pi;
+        const double yi = ar * xi[m] + ai * xr[m] + br * pi + bi * pr;
+        xr[m] = yr;
+        xi[m] = yi;
+    }
+    fft<L, false>(xr, xi, wr, wi);
+    if (act) {
+#pragma unroll
+        for (int m = 0; m < L; ++m) base[m * stride] = make_double2(xr[m], xi[m]);
+    }
+    __syncthreads();
+}
+
 KDQ_HD int lpw_of(int L) { return L >= 96 ? 1 : 96 / L; }
 
 template <int L>
-KDQ_DEV void circ_body(const CircArgs &a, const ChainDev &ch, int c, unsigned char *lds) {
+KDQ_DEV void circ_body(const CircArgs &a, const ChainDev &ch, int c, int grp, unsigned char *lds) {
     constexpr int NC = L * L, NS = 2 * NC, NPAD = (2 * L + 1) * L + 1, LPW = 96 / L;
     const int nl = dim_of(ch, a.nline_dim);
-    const int line0 = blockIdx.x * LPW;
+    const int line0 = grp * LPW;
     if (line0 >= nl) return;   // uniform per workgroup
     const int nlw = min(LPW, nl - line0);
     const bool use_in = a.in.kind != 0, use_out = a.out.kind != 0;
     double2 *buf = reinterpret_cast<double2 *>(lds);   // [LPW][NPAD]: p = 2(x + L y) + b at p + y
-    double2 *pline = buf + LPW * NPAD;                  // [LPW] line factors
-    double2 *pin = pline + LPW;                         // [NS] position factors (when used)
-    double2 *pout = pin + (use_in ? NS : 0);
     const int tid = threadIdx.x;
-    __shared__ double2 tws[16];   // w[e] = e^{-2πi e/L}, evaluated once per workgroup
+    // the t-U HS factors take four values per slice (one per field value): cis(±α η_v) of the
+    // `in`, `out` and `line` factors, evaluated once per workgroup (kind 0 = 1)
+    __shared__ double2 tws[16];   // w[e] = e^{-2πi e/L}
+    __shared__ double2 fac[3][4];
     if (tid < L) {
         double sn, cs;
         sincospi(2.0 * tid / L, &sn, &cs);
         tws[tid] = make_double2(cs, -sn);
-    }
-    for (int p = tid; p < NS; p += CT) {
-        double r, i;
-        if (use_in) {
-            diag_value(ch, a.in, p, r, i);
-            pin[p] = make_double2(r, i);
+    } else if (tid >= 32 && tid < 44) {
+        const int w = (tid - 32) >> 2, v = (tid - 32) & 3;
+        const DiagSpec &d = w == 0 ? a.in : (w == 1 ? a.out : a.line);
+        double r = 1.0, i = 0.0;
+        if (d.kind == 1) {
+            sincos(d.sign * ch.alpha[d.lt - 1] * ch.eta[v], &i, &r);
+            if (d.conj) i = -i;
         }
-        if (use_out) {
-            diag_value(ch, a.out, p, r, i);
-            pout[p] = make_double2(r, i);
-        }
-    }
-    if (tid < nlw) {
-        double r, i;
-        diag_value(ch, a.line, line0 + tid, r, i);
-        pline[tid] = make_double2(r, i);
+        fac[w][v] = make_double2(r, i);
     }
     __syncthreads();
     double wr[L], wi[L];
@@ -234,62 +272,51 @@ KDQ_DEV void circ_body(const CircArgs &a, const ChainDev &ch, int c, unsigned ch
         wr[e] = tws[e].x;
         wi[e] = tws[e].y;
     }
+    const uint8_t *sin_ = ch.s + (size_t)(a.in.lt > 0 ? a.in.lt - 1 : 0) * ch.n_elem;
+    const uint8_t *sout = ch.s + (size_t)(a.out.lt > 0 ? a.out.lt - 1 : 0) * ch.n_elem;
+    const uint8_t *sline = ch.s + (size_t)(a.line.lt > 0 ? a.line.lt - 1 : 0) * ch.n_elem;
     const Opnd X = a.X[c], Y = a.Y[c];
-    // load (and scale by `in`)
-    for (int e = tid; e < LPW * NS; e += CT) {
-        // columns: positions fastest (contiguous); rows: the lines of one column together
-        const int l = a.rows ? e % LPW : e / NS, p = a.rows ? e / LPW : e % NS;
-        double2 v = make_double2(0.0, 0.0);
-        if (l < nlw) {
-            const size_t o = a.rows ? (size_t)(line0 + l) + (size_t)p * X.ld : (size_t)p + (size_t)(line0 + l) * X.ld;
-            const double xr = X.re[o], xi = X.im[o];
-            if (use_in) {
-                const double fr = pin[p].x, fi = pin[p].y;
-                v = make_double2(xr * fr - xi * fi, xr * fi + xi * fr);
-            } else {
-                v = make_double2(xr, xi);
+    // load (and scale by `in`), four loads of a thread in flight before their LDS stores
+    constexpr int NE = LPW * NS, UB = 4;
+    for (int e0 = tid; e0 < NE; e0 += UB * CT) {
+        double2 v[UB];
+#pragma unroll
+        for (int u = 0; u < UB; ++u) {
+            // columns: positions fastest (contiguous); rows: the lines of one column together
+            const int e = e0 + u * CT, l = a.rows ? e % LPW : e / NS, p = a.rows ? e / LPW : e % NS;
+            v[u] = make_double2(0.0, 0.0);
+            if (e < NE && l < nlw) {
+                const size_t o = a.rows ? (size_t)(line0 + l) + (size_t)p * X.ld : (size_t)p + (size_t)(line0 + l) * X.ld;
+                v[u] = make_double2(X.re[o], X.im[o]);
             }
         }
-        buf[l * NPAD + p + p / (2 * L)] = v;
+#pragma unroll
+        for (int u = 0; u < UB; ++u) {
+            const int e = e0 + u * CT, l = a.rows ? e % LPW : e / NS, p = a.rows ? e / LPW : e % NS;
+            if (e >= NE) break;
+            double2 w = v[u];
+            if (use_in) {
+                const double2 f = fac[0][sin_[p] - 1];
+                w = make_double2(v[u].x * f.x - v[u].y * f.y, v[u].x * f.y + v[u].y * f.x);
+            }
+            buf[l * NPAD + p + p / (2 * L)] = w;
+        }
     }
     __syncthreads();
     dft_pass<L, true, false>(buf, nlw, NPAD, wr, wi);
-    dft_pass<L, true, true>(buf, nlw, NPAD, wr, wi);
-    // the 2 x 2 blocks of E, Eᵀ or Eᴴ acting on a line (a right product X·E acts on the rows
-    // of X with Eᵀ): Λ(k), Λ_ba(-k) or conj Λ_ba(k), read from the chain's table
-    const int eop = a.rows ? 1 : a.op;
-    const double *tab = ch.circ + (size_t)a.mat * NC * 8;
-    for (int e = tid; e < nlw * NC; e += CT) {
-        const int l = e / NC, k = e % NC;
-        int ks = k;
-        if (eop == 1) {
-            const int kx = k % L, ky = k / L;
-            ks = ((L - kx) % L) + L * ((L - ky) % L);
-        }
-        const double *t = tab + (size_t)ks * 8;   // (a, b) at t + 2·(2a + b)
-        const double cj = eop == 2 ? -1.0 : 1.0;
-        const bool sw = eop != 0;
-        const double l00r = t[0], l00i = cj * t[1], l11r = t[6], l11i = cj * t[7];
-        const double l01r = sw ? t[4] : t[2], l01i = cj * (sw ? t[5] : t[3]);
-        const double l10r = sw ? t[2] : t[4], l10i = cj * (sw ? t[3] : t[5]);
-        double2 *u = buf + (size_t)l * NPAD + 2 * k + k / L;
-        const double2 u0 = u[0], u1 = u[1];
-        u[0] = make_double2(l00r * u0.x - l00i * u0.y + l01r * u1.x - l01i * u1.y,
-                            l00r * u0.y + l00i * u0.x + l01r * u1.y + l01i * u1.x);
-        u[1] = make_double2(l10r * u0.x - l10i * u0.y + l11r * u1.x - l11i * u1.y,
-                            l10r * u0.y + l10i * u0.x + l11r * u1.y + l11i * u1.x);
-    }
-    __syncthreads();
-    dft_pass<L, false, true>(buf, nlw, NPAD, wr, wi);
+    // forward y, the 2 x 2 blocks of E, Eᵀ or Eᴴ acting on a line (a right product X·E acts on
+    // the rows of X with Eᵀ), backward y
+    y_lambda_pass<L>(buf, nlw, NPAD, wr, wi, ch.circ + (size_t)a.mat * NC * 8, a.rows ? 1 : a.op);
     dft_pass<L, false, false>(buf, nlw, NPAD, wr, wi);
     // store (scaled by `out` and the line factor)
     for (int e = tid; e < LPW * NS; e += CT) {
         const int l = a.rows ? e % LPW : e / NS, p = a.rows ? e / LPW : e % NS;
         if (l >= nlw) continue;
-        const double2 v = buf[l * NPAD + p + p / (2 * L)], g = pline[l];
+        const double2 v = buf[l * NPAD + p + p / (2 * L)];
+        const double2 g = a.line.kind ? fac[2][sline[line0 + l] - 1] : make_double2(1.0, 0.0);
         double tr = v.x, ti = v.y;
         if (use_out) {
-            const double2 f = pout[p];
+            const double2 f = fac[1][sout[p] - 1];
             tr = v.x * f.x - v.y * f.y;
             ti = v.x * f.y + v.y * f.x;
         }
@@ -299,19 +326,33 @@ KDQ_DEV void circ_body(const CircArgs &a, const ChainDev &ch, int c, unsigned ch
     }
 }
 
-__global__ __launch_bounds__(CT) void k_circ(CircArgs a) {
-    const int c = blockIdx.y;
+// Workgroup -> (chain, line group). Workgroups go to the 8 XCDs round-robin by linear id; with
+// xcd_map set (batches of at least 8 chains) every group of chain c runs on XCD c % 8, and
+// neighbouring groups one after another there: the row pass reads half of each 128-byte line
+// per group, and the neighbouring group finds the other half in the same L2.
+constexpr int NXCD = 8;
+__global__ __launch_bounds__(CT, 3) void k_circ(CircArgs a, int nch, int gx, int xcd_map) {
+    int c, grp;
+    if (xcd_map) {
+        const int L = blockIdx.x, x = L % NXCD, k = L / NXCD;
+        c = x + NXCD * (k / gx);
+        grp = k % gx;
+        if (c >= nch) return;
+    } else {
+        c = blockIdx.y;
+        grp = blockIdx.x;
+    }
     const ChainDev &ch = a.chains[c];
     extern __shared__ __align__(16) unsigned char lds_raw[];
     switch (ch.cL) {
-        case 2: circ_body<2>(a, ch, c, lds_raw); break;
-        case 3: circ_body<3>(a, ch, c, lds_raw); break;
-        case 4: circ_body<4>(a, ch, c, lds_raw); break;
-        case 6: circ_body<6>(a, ch, c, lds_raw); break;
-        case 8: circ_body<8>(a, ch, c, lds_raw); break;
-        case 9: circ_body<9>(a, ch, c, lds_raw); break;
-        case 12: circ_body<12>(a, ch, c, lds_raw); break;
-        case 15: circ_body<15>(a, ch, c, lds_raw); break;
+        case 2: circ_body<2>(a, ch, c, grp, lds_raw); break;
+        case 3: circ_body<3>(a, ch, c, grp, lds_raw); break;
+        case 4: circ_body<4>(a, ch, c, grp, lds_raw); break;
+        case 6: circ_body<6>(a, ch, c, grp, lds_raw); break;
+        case 8: circ_body<8>(a, ch, c, grp, lds_raw); break;
+        case 9: circ_body<9>(a, ch, c, grp, lds_raw); break;
+        case 12: circ_body<12>(a, ch, c, grp, lds_raw); break;
+        case 15: circ_body<15>(a, ch, c, grp, lds_raw); break;
         default: break;   // not reached: the host takes the dense path unless every cL is listed
     }
 }
@@ -322,9 +363,9 @@ bool circ_supported(int L) { return L == 2 || L == 3 || L == 4 || L == 6 || L ==
 
 int circ_lines_per_wg(int L) { return lpw_of(L); }
 
-size_t circ_lds_bytes(int L, bool in, bool out) {
-    const size_t ns = 2 * (size_t)L * L, lpw = (size_t)lpw_of(L), npad = (2 * (size_t)L + 1) * L + 1;
-    return sizeof(double2) * (lpw * npad + lpw + (in ? ns : 0) + (out ? ns : 0));
+size_t circ_lds_bytes(int L) {
+    const size_t lpw = (size_t)lpw_of(L), npad = (2 * (size_t)L + 1) * L + 1;
+    return sizeof(double2) * lpw * npad;
 }
 
 hipError_t launch_circ(hipStream_t st, int nchains, int grid_x, size_t lds_bytes, const CircArgs &a) {
@@ -332,6 +373,11 @@ hipError_t launch_circ(hipStream_t st, int nchains, int grid_x, size_t lds_bytes
     if (hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_circ),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes))
         return e;
-    hipLaunchKernelGGL(k_circ, dim3(grid_x, nchains), dim3(CT), lds_bytes, st, a);
+    if (nchains >= NXCD) {
+        const int groups = (nchains + NXCD - 1) / NXCD;
+        hipLaunchKernelGGL(k_circ, dim3(NXCD * groups * grid_x), dim3(CT), lds_bytes, st, a, nchains, grid_x, 1);
+    } else {
+        hipLaunchKernelGGL(k_circ, dim3(grid_x, nchains), dim3(CT), lds_bytes, st, a, nchains, grid_x, 0);
+    }
     return hipGetLastError();
 }

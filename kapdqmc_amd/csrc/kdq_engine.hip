@@ -337,13 +337,13 @@ int gemm_f(kdq_handle h, bool ac, bool bc, int opa, int opb, const GemmArgsF &g,
 bool use_circ(kdq_handle h) { return h->circ && h->circ_L > 0 && h->kind == KDQ_MODEL_TU; }
 
 // grid width and LDS of a k_circ launch over every chain's lines (ldim: DIM_NS or DIM_NP)
-void circ_launch_shape(kdq_handle h, int ldim, bool in, bool out, int &gx, size_t &bytes) {
+void circ_launch_shape(kdq_handle h, int ldim, int &gx, size_t &bytes) {
     gx = 1;
     bytes = 0;
     for (int c = 0; c < h->nch; ++c) {
         const int L = h->ch[c].cL, lines = ldim == DIM_NP ? h->ch[c].np : h->ch[c].ns, lpw = circ_lines_per_wg(L);
         gx = std::max(gx, (lines + lpw - 1) / lpw);
-        bytes = std::max(bytes, circ_lds_bytes(L, in, out));
+        bytes = std::max(bytes, circ_lds_bytes(L));
     }
 }
 
@@ -357,7 +357,7 @@ int kin_left(kdq_handle h, int mat, int opa, const Opnd *X, const Opnd *Y, int l
         CircArgs a{h->d_chains, X, Y, mat, opa == 0 ? 0 : (h->cK ? 2 : 1), 0, ldim, in, out, line};
         int gx = 1;
         size_t bytes = 0;
-        circ_launch_shape(h, ldim, in.kind != 0, out.kind != 0, gx, bytes);
+        circ_launch_shape(h, ldim, gx, bytes);
         HIPCHK(launch_circ(h->stream, h->nch, gx, bytes, a));
         return KDQ_OK;
     }
@@ -377,7 +377,7 @@ int kin_right(kdq_handle h, int mat, const Opnd *X, const Opnd *Y, DiagSpec in, 
         CircArgs a{h->d_chains, X, Y, mat, 0, 1, DIM_NS, in, out, line};
         int gx = 1;
         size_t bytes = 0;
-        circ_launch_shape(h, DIM_NS, in.kind != 0, out.kind != 0, gx, bytes);
+        circ_launch_shape(h, DIM_NS, gx, bytes);
         HIPCHK(launch_circ(h->stream, h->nch, gx, bytes, a));
         return KDQ_OK;
     }
@@ -1075,22 +1075,21 @@ int mq_ensure(kdq_handle h, int dir, int k) {
         h->mq_have = 0;
     }
     for (int j = h->mq_have + 1; j <= k; ++j) {
-        if (dir == 1) {
+        const DiagSpec none{};
+        if (dir == 1) {   // X_j = X_{j-1} D eK, X̃_j = eK⁻¹ D⁻¹ X̃_{j-1} at slice c - j + 1
             const int sl = c - j + 1;
-            GemmArgs g1 = gargs(h, h->mq[j - 1], h->t_eK, h->mq[j], DIM_NS, DIM_NS, DIM_NS);
-            g1.inS = hs_diag(h, sl, +1);
-            if (int r = gemm_m(h, false, true, true, false, 0, 0, g1, KDQ_PROF_MEAS_WRAP)) return r;
-            GemmArgs g2 = gargs(h, h->t_eKinv, h->mqt[j - 1], h->mqt[j], DIM_NS, DIM_NS, DIM_NS);
-            g2.inS = hs_diag(h, sl, -1);
-            if (int r = gemm_m(h, true, false, false, true, 0, 0, g2, KDQ_PROF_MEAS_WRAP)) return r;
-        } else {
+            if (int r = kin_right(h, 0, h->mq[j - 1], h->mq[j], hs_diag(h, sl, +1), none, none, KDQ_PROF_MEAS_WRAP))
+                return r;
+            if (int r = kin_left(h, 1, 0, h->mqt[j - 1], h->mqt[j], DIM_NS, hs_diag(h, sl, -1), none, none,
+                                 KDQ_PROF_MEAS_WRAP))
+                return r;
+        } else {          // X_j = X_{j-1} eK⁻¹ D⁻¹, X̃_j = D eK X̃_{j-1} at slice c + j
             const int sl = c + j;
-            GemmArgs g1 = gargs(h, h->mq[j - 1], h->t_eKinv, h->mq[j], DIM_NS, DIM_NS, DIM_NS);
-            g1.colS = hs_diag(h, sl, -1);
-            if (int r = gemm_m(h, false, true, true, false, 0, 0, g1, KDQ_PROF_MEAS_WRAP)) return r;
-            GemmArgs g2 = gargs(h, h->t_eK, h->mqt[j - 1], h->mqt[j], DIM_NS, DIM_NS, DIM_NS);
-            g2.rowS = hs_diag(h, sl, +1);
-            if (int r = gemm_m(h, true, false, false, true, 0, 0, g2, KDQ_PROF_MEAS_WRAP)) return r;
+            if (int r = kin_right(h, 1, h->mq[j - 1], h->mq[j], none, hs_diag(h, sl, -1), none, KDQ_PROF_MEAS_WRAP))
+                return r;
+            if (int r = kin_left(h, 0, 0, h->mqt[j - 1], h->mqt[j], DIM_NS, none, hs_diag(h, sl, +1), none,
+                                 KDQ_PROF_MEAS_WRAP))
+                return r;
         }
     }
     h->mq_have = std::max(h->mq_have, k);

@@ -32,7 +32,7 @@ hipError_t launch_gemm_f32(hipStream_t st, int nchains, int max_m, int max_n, in
 // every chain): per chain, Y = line ⊙ (out ⊙ op(E) (in ⊙ X)) over the nline_dim columns of X
 // (rows = 0), or Y = line ⊙ ((X ⊙ in) op(E) ⊙ out) over its rows (rows = 1, op(E) acting from
 // the right), E = eK (mat 0) or eK⁻¹ (mat 1), op 0: E, 1: Eᵀ, 2: Eᴴ. The diagonal factors are
-// those of the GEMM path (DiagSpec; kind 0 = none). Y may be X.
+// those of the GEMM path, t-U HS phases only (DiagSpec kind 0 = none, 1). Y may be X.
 struct CircArgs {
     const ChainDev *chains;
     const Opnd *X;
@@ -42,7 +42,7 @@ struct CircArgs {
 };
 bool circ_supported(int L);   // cell edges with a kernel instantiation
 int circ_lines_per_wg(int L);  // lines of X per workgroup for cell edge L
-size_t circ_lds_bytes(int L, bool in, bool out);   // its dynamic LDS (in / out factors used)
+size_t circ_lds_bytes(int L);  // its dynamic LDS
 // grid_x: the largest workgroup count over the chains (lines / lines per workgroup),
 // lds_bytes: the largest circ_lds_bytes over the chains
 hipError_t launch_circ(hipStream_t st, int nchains, int grid_x, size_t lds_bytes, const CircArgs &a);
