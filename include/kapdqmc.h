@@ -262,9 +262,16 @@ int kdq_get_near_ties(kdq_handle h, double *out);
  *                     the largest entry; true for every lattice the model constructors build),
  *                     the kinetic products of the wraps and the chain propagation run through
  *                     cell momenta (length-L DFTs and 2 x 2 blocks per momentum) instead of
- *                     dense GEMMs. 0: dense GEMMs. Results agree to rounding. */
+ *                     dense GEMMs. 0: dense GEMMs. Results agree to rounding.
+ *  KDQ_OPT_QSPLIT     t-U sweeps. 1 (default): at a stabilisation node the
+ *                     recomputed G is formed from the chain after the first (shifted)
+ *                     Cholesky-QR pass, already well conditioned and of the same span, while
+ *                     the second pass, which restores orthonormality for the next block's
+ *                     propagation, runs beside it on a second stream. 0: both passes, then G.
+ *                     G agrees to rounding; kdq_get_logdet then refers to the first-pass
+ *                     factors. */
 enum { KDQ_OPT_GRAPH = 0, KDQ_OPT_DELAY = 1, KDQ_OPT_GEMM_TILE = 2, KDQ_OPT_DEFER = 3, KDQ_OPT_PIPE = 4,
-       KDQ_OPT_FLUSH = 5, KDQ_OPT_ASIDE = 6, KDQ_OPT_CIRC = 7 };
+       KDQ_OPT_FLUSH = 5, KDQ_OPT_ASIDE = 6, KDQ_OPT_CIRC = 7, KDQ_OPT_QSPLIT = 8 };
 int kdq_set_option(kdq_handle h, int32_t key, int32_t value);
 
 /* Mixed precision (kdq_run_desc.precision = KDQ_PREC_MIXED, t-U): G is carried in FP32

@@ -40,6 +40,7 @@ struct ChainHost {
     float *FZ32 = nullptr, *FC32 = nullptr;
     double *DU = nullptr, *DV = nullptr;     // deferred-flush factors of a t-U slice (N x N each)
     double *DI = nullptr;                    // inverses of the 16 x 16 diagonal blocks (Np x 32)
+    double *S2 = nullptr, *DI2 = nullptr;    // Gram matrix and block inverses of a concurrent second pass
     double *circ = nullptr;                  // Λ(k) of eK and eK⁻¹ (block-circulant kinetic factors)
     int cL = 0;                              // their cell edge (0: dense products)
     float *DU32 = nullptr, *DV32 = nullptr;
@@ -84,6 +85,9 @@ struct kdq_handle_s {
     OpndF *t_FZ32 = nullptr, *t_FC32 = nullptr;
     Opnd *t_DU = nullptr, *t_DV = nullptr;     // deferred-flush factors (t-U)
     Opnd *t_DI = nullptr;                      // diagonal-block inverses of the Np x Np factors
+    Opnd *t_S2 = nullptr, *t_DI2 = nullptr;    // the same for a second Cholesky-QR pass on the aux stream
+    bool qsplit = true;                        // t-U captured sweeps: that second pass beside get_G!
+    hipEvent_t ev_q1 = nullptr, ev_q2 = nullptr;
     OpndF *t_DU32 = nullptr, *t_DV32 = nullptr;
     bool defer = false;                        // t-U slices flush once (U, V factors)
     std::vector<Opnd *> t_UR, t_ULH;
@@ -175,6 +179,8 @@ struct kdq_handle_s {
             if (e) hipEventDestroy(e);
         if (ev_fields) hipEventDestroy(ev_fields);
         if (ev_chain) hipEventDestroy(ev_chain);
+        if (ev_q1) hipEventDestroy(ev_q1);
+        if (ev_q2) hipEventDestroy(ev_q2);
         if (side) hipStreamDestroy(side);
         if (aux) hipStreamDestroy(aux);
         if (stream) hipStreamDestroy(stream);
@@ -459,20 +465,54 @@ int refresh_g32(kdq_handle h, bool track) {
 
 // U <- orthonormal basis of span(U) for the N x Np propagator, in place: two Cholesky-QR
 // passes (S = UᴴU, S + σI = L Lᴴ, U <- U L⁻ᴴ), the first shifted by σ = c·u·‖U‖_F².
-int orthonormalise(kdq_handle h, Opnd *U, double *logacc = nullptr) {
+// one pass on the handle's stream with the Gram / block-inverse buffers S, DI
+int ortho_pass(kdq_handle h, Opnd *U, int pass, double *logacc, Opnd *S, Opnd *DI) {
     const double u = 1.1102230246251565e-16;
     const double n = h->max_ns, k = h->max_np;
     const double shift = 11.0 * (n * k + k * (k + 1.0)) * u;
-    for (int pass = 0; pass < 2; ++pass) {
-        GemmArgs g = gargs(h, U, U, h->t_S, DIM_NP, DIM_NP, DIM_NS);
-        if (int r = gemm(h, true, true, 2, 0, g, KDQ_PROF_QR)) return r;
-        ProfScope ps(h, KDQ_PROF_QR);
-        CholArgs ca{h->d_chains, h->t_S, DIM_NP, 0, pass == 0 ? shift : 0.0, h->d_qinfo, logacc, h->t_DI};
-        HIPCHK(launch_chol(h->stream, h->nch, h->max_np, ca));
-        TrsmArgs ta{h->d_chains, U, h->t_S, DIM_NS, DIM_NP, h->t_DI};
-        HIPCHK(launch_trsm_rlh(h->stream, h->nch, h->max_ns, h->max_np, ta));
-    }
+    GemmArgs g = gargs(h, U, U, S, DIM_NP, DIM_NP, DIM_NS);
+    if (int r = gemm(h, true, true, 2, 0, g, KDQ_PROF_QR)) return r;
+    ProfScope ps(h, KDQ_PROF_QR);
+    CholArgs ca{h->d_chains, S, DIM_NP, 0, pass == 0 ? shift : 0.0, h->d_qinfo, logacc, DI};
+    HIPCHK(launch_chol(h->stream, h->nch, h->max_np, ca));
+    TrsmArgs ta{h->d_chains, U, S, DIM_NS, DIM_NP, DI};
+    HIPCHK(launch_trsm_rlh(h->stream, h->nch, h->max_ns, h->max_np, ta));
     return KDQ_OK;
+}
+
+int orthonormalise(kdq_handle h, Opnd *U, double *logacc = nullptr) {
+    for (int pass = 0; pass < 2; ++pass)
+        if (int r = ortho_pass(h, U, pass, logacc, h->t_S, h->t_DI)) return r;
+    return KDQ_OK;
+}
+
+// The second pass beside get_G! (t-U sweeps, captured or not): G = I - U_R (U_L U_R)⁻¹ U_L depends
+// only on span(U), and after the shifted first pass U is already well conditioned (κ = O(1)
+// for κ(U) well inside u^(-1/2)·10^4, Fukaya et al. 2020), so the node's G is formed from a copy
+// of the first-pass factor while the second pass, which only restores orthonormality for the
+// next block's propagation, runs on the aux stream with its own Gram and block-inverse
+// buffers. Returns the copy (t_P2); the caller waits for ev_q2 before U is used again.
+bool qsplit_on(kdq_handle h) { return h->kind == KDQ_MODEL_TU && h->qsplit && !h->scee && h->aux && h->t_S2; }
+
+Opnd *orthonormalise_split(kdq_handle h, Opnd *U, int &rc) {
+    rc = ortho_pass(h, U, 0, nullptr, h->t_S, h->t_DI);
+    if (rc) return nullptr;
+    hipError_t e = launch_copy_mat(h->stream, h->nch, h->max_ns, h->max_np, h->d_chains, U, h->t_P2, DIM_NS, DIM_NP, true);
+    if (e == hipSuccess) e = hipEventRecord(h->ev_q1, h->stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(h->aux, h->ev_q1, 0);
+    if (e != hipSuccess) {
+        rc = fail(h, KDQ_EHIP, std::string("orthonormalise_split: ") + hipGetErrorString(e));
+        return nullptr;
+    }
+    std::swap(h->stream, h->aux);   // the second pass goes to the aux stream
+    rc = ortho_pass(h, U, 1, nullptr, h->t_S2, h->t_DI2);
+    std::swap(h->stream, h->aux);
+    if (rc) return nullptr;
+    if ((e = hipEventRecord(h->ev_q2, h->aux)) != hipSuccess) {
+        rc = fail(h, KDQ_EHIP, std::string("orthonormalise_split: ") + hipGetErrorString(e));
+        return nullptr;
+    }
+    return h->t_P2;
 }
 
 // dst = B(hi) ... B(lo) · src for U_R (N x Np), not orthonormalised (t-U); src may be dst
@@ -503,7 +543,7 @@ int prop_L_range(kdq_handle h, Opnd *src, Opnd *dst, int lo, int hi) {
 }
 
 // dst = QR_Q(B_block(idx) · src) for U_R: slices nodes[idx-1]+1 .. nodes[idx] (1-based idx)
-int propagate_R(kdq_handle h, Opnd *src, Opnd *dst, int idx, double *logacc = nullptr) {
+int propagate_R(kdq_handle h, Opnd *src, Opnd *dst, int idx, double *logacc = nullptr, bool ortho = true) {
     const int lo = (int)h->nodes[idx - 1] + 1, hi = (int)h->nodes[idx];
     Opnd *cur = src;
     for (int lt = lo; lt <= hi; ++lt) {
@@ -516,13 +556,13 @@ int propagate_R(kdq_handle h, Opnd *src, Opnd *dst, int idx, double *logacc = nu
         cur = out;
     }
     if (lo > hi) HIPCHK(launch_copy_mat(h->stream, h->nch, h->max_ns, h->max_np, h->d_chains, src, dst, DIM_NS, DIM_NP, true));
-    return orthonormalise(h, dst, logacc);
+    return ortho ? orthonormalise(h, dst, logacc) : KDQ_OK;
 }
 
 // dst = QR_Q((U_L · B_block(idx))ᴴ) with U_L stored as U_Lᴴ: U_Lᴴ <- eKᵀ (conj(D_l) ⊙ U_Lᴴ)
 // (t-V: U_Lᴴ <- eKᵀ V3 V2 V1 U_Lᴴ)
 // for l = nodes[idx] down to nodes[idx-1]+1.
-int propagate_L(kdq_handle h, Opnd *src, Opnd *dst, int idx) {
+int propagate_L(kdq_handle h, Opnd *src, Opnd *dst, int idx, bool ortho = true) {
     const int lo = (int)h->nodes[idx - 1] + 1, hi = (int)h->nodes[idx];
     Opnd *cur = src;
     for (int lt = hi; lt >= lo; --lt) {
@@ -544,7 +584,7 @@ int propagate_L(kdq_handle h, Opnd *src, Opnd *dst, int idx) {
         cur = out;
     }
     if (lo > hi) HIPCHK(launch_copy_mat(h->stream, h->nch, h->max_ns, h->max_np, h->d_chains, src, dst, DIM_NS, DIM_NP, true));
-    return orthonormalise(h, dst);
+    return ortho ? orthonormalise(h, dst) : KDQ_OK;
 }
 
 // Chain products beside the sweep (captured t-U sweeps with the pipelined blocks): the product
@@ -580,9 +620,9 @@ int chain_step(kdq_handle h, int lt, bool forward) {
 }
 
 // the node after a block of chain_step products: dst = QR_Q of the block's product
-int chain_join(kdq_handle h, Opnd *dst) {
+int chain_join(kdq_handle h, Opnd *dst, bool ortho = true) {
     HIPCHK(hipStreamWaitEvent(h->stream, h->ev_chain, 0));
-    return orthonormalise(h, dst);
+    return ortho ? orthonormalise(h, dst) : KDQ_OK;
 }
 
 // get_G!: G = I - U_R (U_L U_R)⁻¹ U_L into the current G buffer
@@ -599,8 +639,8 @@ int get_G_from(kdq_handle h, Opnd *UR, Opnd *ULH, Opnd *dst) {
     return gemm(h, true, true, 0, 0, g3, KDQ_PROF_GETG);
 }
 
-int get_G(kdq_handle h, int idx, Opnd *dst = nullptr) {
-    Opnd *ULH = h->t_ULH[idx - 1], *UR = h->t_UR[idx - 1];
+int get_G(kdq_handle h, int idx, Opnd *dst = nullptr, Opnd *UR_ = nullptr, Opnd *ULH_ = nullptr) {
+    Opnd *ULH = ULH_ ? ULH_ : h->t_ULH[idx - 1], *UR = UR_ ? UR_ : h->t_UR[idx - 1];
     GemmArgs g1 = gargs(h, ULH, UR, h->t_M, DIM_NP, DIM_NP, DIM_NS);
     if (int r = gemm(h, true, true, 2, 0, g1, KDQ_PROF_GETG)) return r;
     {
@@ -1196,14 +1236,14 @@ bool in_window(kdq_handle h, int thidx, bool forward) {
 // reference's commented wrap-error check, src/tU/phy_update.jl:54-58,92-97; a non-finite entry
 // in either marks the chain), then made current. Mixed precision: the FP32 copy is compared and
 // replaced by refresh_g32.
-int node_refresh(kdq_handle h, int idx) {
+int node_refresh(kdq_handle h, int idx, Opnd *UR = nullptr, Opnd *ULH = nullptr) {
     if (h->mixed) {
-        if (int r = get_G(h, idx)) return r;
+        if (int r = get_G(h, idx, nullptr, UR, ULH)) return r;
         HIPCHK(launch_node_check(h->stream, h->nch, h->max_ns, h->d_chains, h->t_G[h->cur], h->t_G[h->cur], DIM_NS,
                                  h->d_wraperr, h->d_poison));
         return refresh_g32(h, true);
     }
-    if (int r = get_G(h, idx, h->t_G[1 - h->cur])) return r;
+    if (int r = get_G(h, idx, h->t_G[1 - h->cur], UR, ULH)) return r;
     HIPCHK(launch_node_check(h->stream, h->nch, h->max_ns, h->d_chains, h->t_G[h->cur], h->t_G[1 - h->cur], DIM_NS,
                              h->d_wraperr, h->d_poison));
     h->cur ^= 1;
@@ -1218,7 +1258,7 @@ int sweep_body(kdq_handle h, int record, uint32_t sweep) {
     const bool tv = h->kind == KDQ_MODEL_TV;
     (void)NN;
     if (record) HIPCHK(launch_zero_meas(h->stream, h->nch, h->d_chains));
-    const bool aside = chain_aside(h);
+    const bool aside = chain_aside(h), split = qsplit_on(h);
     h->mq_dir = 0;
     for (int lt = 1; lt <= nt; ++lt) {
         if (tv) {
@@ -1233,12 +1273,20 @@ int sweep_body(kdq_handle h, int record, uint32_t sweep) {
             if (int r = tv ? measure(h, lt) : measure_sweep(h, lt, true)) return r;
         if (is_node(h, lt)) {
             h->idx += 1;
+            Opnd *U = h->t_UR[h->idx - 1];
             if (aside) {
-                if (int r = chain_join(h, h->t_UR[h->idx - 1])) return r;
-            } else if (int r = propagate_R(h, h->t_UR[h->idx - 2], h->t_UR[h->idx - 1], h->idx - 1)) {
+                if (int r = chain_join(h, U, !split)) return r;
+            } else if (int r = propagate_R(h, h->t_UR[h->idx - 2], U, h->idx - 1, nullptr, !split)) {
                 return r;
             }
-            if (int r = node_refresh(h, h->idx)) return r;
+            Opnd *Uq = nullptr;   // the first-pass copy the node's G is formed from (split)
+            if (split) {
+                int rc = 0;
+                Uq = orthonormalise_split(h, U, rc);
+                if (rc) return rc;
+            }
+            if (int r = node_refresh(h, h->idx, Uq, nullptr)) return r;
+            if (split) HIPCHK(hipStreamWaitEvent(h->stream, h->ev_q2, 0));
         }
     }
     h->mq_dir = 0;
@@ -1257,12 +1305,20 @@ int sweep_body(kdq_handle h, int record, uint32_t sweep) {
         }
         if (is_node(h, lt - 1)) {
             h->idx -= 1;
+            Opnd *U = h->t_ULH[h->idx - 1];
             if (aside) {
-                if (int r = chain_join(h, h->t_ULH[h->idx - 1])) return r;
-            } else if (int r = propagate_L(h, h->t_ULH[h->idx], h->t_ULH[h->idx - 1], h->idx)) {
+                if (int r = chain_join(h, U, !split)) return r;
+            } else if (int r = propagate_L(h, h->t_ULH[h->idx], U, h->idx, !split)) {
                 return r;
             }
-            if (int r = node_refresh(h, h->idx)) return r;
+            Opnd *Uq = nullptr;
+            if (split) {
+                int rc = 0;
+                Uq = orthonormalise_split(h, U, rc);
+                if (rc) return rc;
+            }
+            if (int r = node_refresh(h, h->idx, nullptr, Uq)) return r;
+            if (split) HIPCHK(hipStreamWaitEvent(h->stream, h->ev_q2, 0));
         }
     }
     if (h->sweep_dev) HIPCHK(launch_bump(h->stream, h->d_sweep));
@@ -1596,6 +1652,8 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->aux, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_fields, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_chain, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_q1, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_q2, hipEventDisableTiming);
     if (e != hipSuccess) {
         int r = fail(h, KDQ_EHIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
         delete h;
@@ -1604,7 +1662,7 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
     std::vector<ChainDev> cd(nchains);
     std::vector<Opnd> v_eK, v_eKinv, v_hK, v_hKinv, v_Pt, v_G0, v_G1, v_T, v_Gm, v_Tm, v_P0, v_P1, v_P2, v_M, v_X, v_S;
     std::vector<OpndF> v_G32a, v_G32b, v_T32, v_eK32, v_eKinv32, v_FZ32, v_FC32;
-    std::vector<Opnd> v_FZ, v_FC, v_DU, v_DV, v_DI;
+    std::vector<Opnd> v_FZ, v_FC, v_DU, v_DV, v_DI, v_S2, v_DI2;
     std::vector<OpndF> v_DU32, v_DV32;
     std::vector<BlockDesc> blk_all;
     std::vector<int> blk_offset, blk_count;
@@ -1690,6 +1748,10 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
         ok &= dalloc(C, &C.FC, 2 * 64 * (size_t)n) == hipSuccess;
         ok &= dalloc(C, &C.rk, (size_t)ns_all + 2) == hipSuccess;
         ok &= dalloc(C, &C.DI, 2 * 32 * (size_t)np) == hipSuccess;   // blocks run over the largest chain
+        if (m.kind == KDQ_MODEL_TU) {
+            ok &= dalloc(C, &C.S2, 2 * pp) == hipSuccess;
+            ok &= dalloc(C, &C.DI2, 2 * 32 * (size_t)np) == hipSuccess;
+        }
         ok &= dalloc(C, &C.rank, 1) == hipSuccess;
         if (m.kind == KDQ_MODEL_TU && !h->mixed) {
             ok &= dalloc(C, &C.DU, 2 * nn) == hipSuccess;
@@ -1833,6 +1895,8 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
         v_S.push_back(cplx(C.S, np, pp));
         v_FZ.push_back(Opnd{C.FZ, C.FZ + 64 * (size_t)n, 64, 0});
         v_DI.push_back(Opnd{C.DI, C.DI + 32 * (size_t)np, np, 0});
+        v_S2.push_back(Opnd{C.S2, C.S2 ? C.S2 + pp : nullptr, np, 0});
+        v_DI2.push_back(Opnd{C.DI2, C.DI2 ? C.DI2 + 32 * (size_t)np : nullptr, np, 0});
         v_DU.push_back(Opnd{C.DU, C.DU ? C.DU + nn : nullptr, n, 0});
         v_DV.push_back(Opnd{C.DV, C.DV ? C.DV + nn : nullptr, n, 0});
         v_DU32.push_back(OpndF{C.DU32, C.DU32 ? C.DU32 + nn : nullptr, n, 0});
@@ -1895,6 +1959,7 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
     r |= make_table(h, &h->t_FZ, v_FZ); r |= make_table(h, &h->t_FC, v_FC);
     r |= make_table(h, &h->t_DU, v_DU); r |= make_table(h, &h->t_DV, v_DV);
     r |= make_table(h, &h->t_DI, v_DI);
+    r |= make_table(h, &h->t_S2, v_S2); r |= make_table(h, &h->t_DI2, v_DI2);
     r |= make_table_f(h, &h->t_DU32, v_DU32); r |= make_table_f(h, &h->t_DV32, v_DV32);
     if (h->mixed) {
         r |= make_table_f(h, &h->t_G32[0], v_G32a); r |= make_table_f(h, &h->t_G32[1], v_G32b);
@@ -2440,6 +2505,9 @@ int kdq_set_option(kdq_handle h, int32_t key, int32_t value) {
             break;
         case KDQ_OPT_CIRC:
             h->circ = value != 0;
+            break;
+        case KDQ_OPT_QSPLIT:
+            h->qsplit = value != 0;
             break;
         case KDQ_OPT_ASIDE:
             h->aside = value != 0;

@@ -442,27 +442,107 @@ __global__ __launch_bounds__(NL) void k_lu(LuArgs a) {
             s_phr = phr;
             s_phi = phi;
         }
-        // the panel's interchanges on every column outside it (one thread per column)
-        for (int col = tid; col < n; col += NL) {
-            if (col >= k0 && col < k0 + kb) continue;
-            for (int j = 0; j < kb; ++j) {
-                const int p = lpiv[j];
-                if (p == j) continue;
-                const size_t oa = (k0 + j) + (size_t)col * ld, ob = (k0 + p) + (size_t)col * ld;
-                const double ar = Re[oa], ai = Im[oa];
-                Re[oa] = Re[ob];
-                Im[oa] = Im[ob];
-                Re[ob] = ar;
-                Im[ob] = ai;
+        // the panel's interchanges on every column outside it, as one row permutation of the
+        // (at most 2 kb) rows they touch: src[t] is the original row (panel-relative) that ends in
+        // row rows[t]; every entry is loaded before any is stored
+        // The rows: 0 .. kb-1 (thread t < kb), then each distinct pivot row >= kb (thread kb + j
+        // for the first j that names it), compacted in order; every thread of the first 2 kb
+        // follows its row back through the swaps with the pivots in registers.
+        __shared__ int rows[2 * NB], src[2 * NB], nrows;
+        if (tid < 2 * NB) {
+            int lp[NB];
+#pragma unroll
+            for (int j = 0; j < NB; ++j) lp[j] = j < kb ? lpiv[j] : j;
+            int row = -1, pos = -1;
+            if (tid < kb) {
+                row = tid;
+                pos = tid;
+            } else if (tid < 2 * kb) {
+                const int j = tid - kb, p = lpiv[j];
+                bool rep = p >= kb;
+                int before = 0;   // representatives among the earlier pivots
+#pragma unroll
+                for (int i = 0; i < NB; ++i) {
+                    if (i < j) {
+                        rep = rep && lp[i] != p;
+                        bool ri = lp[i] >= kb;
+#pragma unroll
+                        for (int i2 = 0; i2 < NB; ++i2)
+                            if (i2 < i) ri = ri && lp[i2] != lp[i];
+                        before += ri ? 1 : 0;
+                    }
+                }
+                if (rep) {
+                    row = p;
+                    pos = kb + before;
+                }
+            }
+            if (row >= 0) {
+                int r = row;
+#pragma unroll
+                for (int j = NB - 1; j >= 0; --j) {
+                    if (j >= kb) continue;
+                    if (r == j) r = lp[j];
+                    else if (r == lp[j]) r = j;
+                }
+                rows[pos] = row;
+                src[pos] = r;
+            }
+            if (tid == 0) {   // the count of distinct pivot rows >= kb
+                int cnt = kb;
+#pragma unroll
+                for (int i = 0; i < NB; ++i) {
+                    bool ri = i < kb && lp[i] >= kb;
+#pragma unroll
+                    for (int i2 = 0; i2 < NB; ++i2)
+                        if (i2 < i) ri = ri && lp[i2] != lp[i];
+                    cnt += ri ? 1 : 0;
+                }
+                nrows = cnt;
             }
         }
         __syncthreads();
-        // U12 = L11⁻¹ A12 (unit lower), one thread per column, kept in LDS and stored
+        {
+            // whole columns per round (cpi of them, nr rows each), so that every row of a column
+            // is read before any of its rows is written; the same number of rounds in every thread
+            const int nr = nrows, nout = n - kb;   // columns outside the panel
+            const int cpi = nr > 0 ? (NL * LB) / nr : nout;
+            for (int c0 = 0; c0 < nout && nr > 0; c0 += cpi) {
+                double vr[LB], vi[LB];
+#pragma unroll
+                for (int b = 0; b < LB; ++b) {
+                    const int e = tid + b * NL, t = e % nr, cq = c0 + e / nr, col = cq < k0 ? cq : cq + kb;
+                    const bool ok = e < cpi * nr && cq < nout;
+                    const size_t o = ok ? (k0 + src[t]) + (size_t)col * ld : 0;
+                    vr[b] = ok ? Re[o] : 0.0;
+                    vi[b] = ok ? Im[o] : 0.0;
+                }
+                __syncthreads();
+#pragma unroll
+                for (int b = 0; b < LB; ++b) {
+                    const int e = tid + b * NL, t = e % nr, cq = c0 + e / nr, col = cq < k0 ? cq : cq + kb;
+                    if (e >= cpi * nr || cq >= nout) continue;
+                    const size_t o = (k0 + rows[t]) + (size_t)col * ld;
+                    Re[o] = vr[b];
+                    Im[o] = vi[b];
+                }
+                __syncthreads();
+            }
+        }
+        __syncthreads();
+        // U12 = L11⁻¹ A12 (unit lower), one thread per column: its kb entries loaded together
+        // into the thread's LDS column (no global store between them), solved there, stored
         const int ncol = n - k0 - kb;
         for (int q = tid; q < ncol; q += NL) {
             const size_t col = (size_t)(k0 + kb + q) * ld;
+#pragma unroll
+            for (int j = 0; j < NB; ++j)
+                if (j < kb) {
+                    Ur[q * PS + j] = Re[k0 + j + col];
+                    Ui[q * PS + j] = Im[k0 + j + col];
+                }
             for (int j = 0; j < kb; ++j) {
-                double sr = Re[k0 + j + col], si = Im[k0 + j + col];
+                double sr = Ur[q * PS + j], si = Ui[q * PS + j];
                 for (int i = 0; i < j; ++i) {
                     double tr_, ti_;
                     cmul(Pr[j * PS + i], Pi[j * PS + i], Ur[q * PS + i], Ui[q * PS + i], tr_, ti_);
