@@ -356,8 +356,10 @@ __global__ __launch_bounds__(KBT >= 32 ? 1024 : KBT * KBT) void k_decide_tu(Deci
 // Flush, first launch: per chain and chunk of PC columns, Z[:, J] = W · Y[:, J] with
 // Y = edge·E_A - sgn·Rin[A, :], and rows J of the gathered columns Cs = sgn·Cin[:, A]. The
 // accepted sites of a block are consecutive indices, so the row gather Rin[A, j] reads one or
-// two cache lines per column.
-constexpr int PC = 64;
+// two cache lines per column. The launch is latency-bound (three dependent global reads, then
+// the product), so chunks are narrow: 18 workgroups per chain at N = 288 (64-column chunks
+// measured 431, 32: 412, 16: 407, 8: 413 ms per sweep of the 64-chain N = 288 batch).
+constexpr int PC = 16;
 template <class T>
 __global__ __launch_bounds__(256) void k_flush_prep(FlushArgsT<T> f) {
     const int c = blockIdx.y;
