@@ -453,11 +453,12 @@ __global__ __launch_bounds__(256) void k_flush_apply(FlushArgsT<T> f, int nch, i
     const auto Gi = f.Gin[c], Go = f.Gout[c], Z = f.Zs[c], Cg = f.Cs[c];
     __shared__ C2 Cs2[AK][AM];
     __shared__ C2 Zs2[AK][AN + 1];   // padded: the staging stores run down a column (t)
-    T xr[2][4], xi[2][4], ar[2][4], ai[2][4];
+    constexpr int RA = AM / 16, RB = AN / 16;
+    T xr[RA][RB], xi[RA][RB], ar[RA][RB], ai[RA][RB];
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < RA; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
+        for (int b = 0; b < RB; ++b) {
             const int i = i0 + ri + 16 * a, j = j0 + cg + 16 * b;
             const bool ok = i < n && j < n;
             const size_t oi = (size_t)i + (size_t)j * Gi.ld;
@@ -508,15 +509,15 @@ __global__ __launch_bounds__(256) void k_flush_apply(FlushArgsT<T> f, int nch, i
         }
         __syncthreads();
         for (int t = 0; t < tk; ++t) {
-            C2 cv[2], zv[4];
+            C2 cv[RA], zv[RB];
 #pragma unroll
-            for (int a = 0; a < 2; ++a) cv[a] = Cs2[t][ri + 16 * a];
+            for (int a = 0; a < RA; ++a) cv[a] = Cs2[t][ri + 16 * a];
 #pragma unroll
-            for (int b = 0; b < 4; ++b) zv[b] = Zs2[t][cg + 16 * b];
+            for (int b = 0; b < RB; ++b) zv[b] = Zs2[t][cg + 16 * b];
 #pragma unroll
-            for (int a = 0; a < 2; ++a)
+            for (int a = 0; a < RA; ++a)
 #pragma unroll
-                for (int b = 0; b < 4; ++b) {
+                for (int b = 0; b < RB; ++b) {
                     ar[a][b] = fma(cv[a].x, zv[b].x, fma(-cv[a].y, zv[b].y, ar[a][b]));
                     ai[a][b] = fma(cv[a].x, zv[b].y, fma(cv[a].y, zv[b].x, ai[a][b]));
                 }
@@ -524,9 +525,9 @@ __global__ __launch_bounds__(256) void k_flush_apply(FlushArgsT<T> f, int nch, i
         __syncthreads();
     }
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < RA; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
+        for (int b = 0; b < RB; ++b) {
             const int i = i0 + ri + 16 * a, j = j0 + cg + 16 * b;
             if (i < n && j < n) {
                 const size_t oo = (size_t)i + (size_t)j * Go.ld;
