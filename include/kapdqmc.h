@@ -269,9 +269,14 @@ int kdq_get_near_ties(kdq_handle h, double *out);
  *                     the second pass, which restores orthonormality for the next block's
  *                     propagation, runs beside it on a second stream. 0: both passes, then G.
  *                     G agrees to rounding; kdq_get_logdet then refers to the first-pass
- *                     factors. */
+ *                     factors.
+ *  KDQ_OPT_MASIDE     t-U sweeps (not while kernel timing is on). 1 (default): a measurement's
+ *                     products and observable tiles run on a second stream beside the next
+ *                     slices (its copy of G and any wrap reading fields the sweep may still
+ *                     change stay in line); the next measurement and the end of the sweep wait
+ *                     for it. 0: in line. Identical results. */
 enum { KDQ_OPT_GRAPH = 0, KDQ_OPT_DELAY = 1, KDQ_OPT_GEMM_TILE = 2, KDQ_OPT_DEFER = 3, KDQ_OPT_PIPE = 4,
-       KDQ_OPT_FLUSH = 5, KDQ_OPT_ASIDE = 6, KDQ_OPT_CIRC = 7, KDQ_OPT_QSPLIT = 8 };
+       KDQ_OPT_FLUSH = 5, KDQ_OPT_ASIDE = 6, KDQ_OPT_CIRC = 7, KDQ_OPT_QSPLIT = 8, KDQ_OPT_MASIDE = 9 };
 int kdq_set_option(kdq_handle h, int32_t key, int32_t value);
 
 /* Mixed precision (kdq_run_desc.precision = KDQ_PREC_MIXED, t-U): G is carried in FP32

@@ -87,6 +87,9 @@ struct kdq_handle_s {
     Opnd *t_DI = nullptr;                      // diagonal-block inverses of the Np x Np factors
     Opnd *t_S2 = nullptr, *t_DI2 = nullptr;    // the same for a second Cholesky-QR pass on the aux stream
     bool qsplit = true;                        // t-U captured sweeps: that second pass beside get_G!
+    bool maside = true;                        // t-U sweeps: a measurement's products and tiles on aux
+    bool mpending = false;                     // such a measurement not yet joined
+    hipEvent_t ev_mfork = nullptr, ev_mdone = nullptr;
     hipEvent_t ev_q1 = nullptr, ev_q2 = nullptr;
     OpndF *t_DU32 = nullptr, *t_DV32 = nullptr;
     bool defer = false;                        // t-U slices flush once (U, V factors)
@@ -181,6 +184,8 @@ struct kdq_handle_s {
         if (ev_chain) hipEventDestroy(ev_chain);
         if (ev_q1) hipEventDestroy(ev_q1);
         if (ev_q2) hipEventDestroy(ev_q2);
+        if (ev_mfork) hipEventDestroy(ev_mfork);
+        if (ev_mdone) hipEventDestroy(ev_mdone);
         if (side) hipStreamDestroy(side);
         if (aux) hipStreamDestroy(aux);
         if (stream) hipStreamDestroy(stream);
@@ -1020,7 +1025,39 @@ int wrap_slice(kdq_handle h, Opnd *Gt, Opnd *Tt, int lt, bool forward, int fam) 
 
 // phy_measure: wrap a copy of G to τ = Nt/2, HalfeK sandwich, observables (accumulated).
 // Gm, Tm: per-chain N x N complex working matrices; meas: accumulators (nullptr: the chains').
-int measure_into(kdq_handle h, int lt, Opnd *Gm, Opnd *Tm, double *meas, double *part) {
+// Measurements beside the sweep (t-U): the front of a measurement — the copy of G and any
+// wrap that reads fields the sweep may still change — runs on the handle's stream; its back —
+// the products with HalfeK or the slot products and the observable tiles, which read only the
+// copy, the slot matrices and the model — goes to the aux stream (meas_fork / meas_unfork).
+// The next measurement, which rewrites the copy and may rewrite slot matrices, and the end of
+// the sweep wait for it (meas_join). Same kernels on the same operands: identical results.
+bool meas_aside(kdq_handle h) {
+    return h->kind == KDQ_MODEL_TU && h->maside && !h->prof_mask && !h->scee && h->aux;
+}
+
+int meas_join(kdq_handle h) {
+    if (h->mpending) {
+        HIPCHK(hipStreamWaitEvent(h->stream, h->ev_mdone, 0));
+        h->mpending = false;
+    }
+    return KDQ_OK;
+}
+
+int meas_fork(kdq_handle h) {
+    HIPCHK(hipEventRecord(h->ev_mfork, h->stream));
+    HIPCHK(hipStreamWaitEvent(h->aux, h->ev_mfork, 0));
+    std::swap(h->stream, h->aux);
+    return KDQ_OK;
+}
+
+int meas_unfork(kdq_handle h) {
+    std::swap(h->stream, h->aux);
+    HIPCHK(hipEventRecord(h->ev_mdone, h->aux));
+    h->mpending = true;
+    return KDQ_OK;
+}
+
+int measure_into(kdq_handle h, int lt, Opnd *Gm, Opnd *Tm, double *meas, double *part, bool fork = false) {
     if (h->mixed)
         HIPCHK(launch_to_f64(h->stream, h->nch, h->max_ns, h->d_chains, h->t_G32[h->cur32], Gm, DIM_NS, 1));
     else
@@ -1033,14 +1070,21 @@ int measure_into(kdq_handle h, int lt, Opnd *Gm, Opnd *Tm, double *meas, double 
         for (int t = lt + 1; t <= nt / 2; ++t)
             if (int r = wrap_slice(h, Gm, Tm, t, true, KDQ_PROF_MEAS_WRAP)) return r;
     }
+    if (fork)
+        if (int r = meas_fork(h)) return r;
     GemmArgs g1 = gargs(h, h->t_hK, Gm, Tm, DIM_NS, DIM_NS, DIM_NS);
-    if (int r = gemm_m(h, true, false, false, true, 0, 0, g1, KDQ_PROF_MEAS_WRAP)) return r;
+    int r = gemm_m(h, true, false, false, true, 0, 0, g1, KDQ_PROF_MEAS_WRAP);
     GemmArgs g2 = gargs(h, Tm, h->t_hKinv, Gm, DIM_NS, DIM_NS, DIM_NS);
-    if (int r = gemm_m(h, false, true, true, false, 0, 0, g2, KDQ_PROF_MEAS_WRAP)) return r;
-    MeasArgs m{h->d_chains, Gm, meas, part, meas_tiles(h->max_ns)};
-    ProfScope ps(h, KDQ_PROF_MEASURE);
-    HIPCHK(launch_measure(h->stream, h->nch, m));
-    return KDQ_OK;
+    if (!r) r = gemm_m(h, false, true, true, false, 0, 0, g2, KDQ_PROF_MEAS_WRAP);
+    if (!r) {
+        MeasArgs m{h->d_chains, Gm, meas, part, meas_tiles(h->max_ns)};
+        ProfScope ps(h, KDQ_PROF_MEASURE);
+        if (hipError_t e = launch_measure(h->stream, h->nch, m)) r = fail(h, KDQ_EHIP, hipGetErrorString(e));
+    }
+    if (fork) {
+        if (int r2 = meas_unfork(h)) return r2;
+    }
+    return r;
 }
 
 int measure(kdq_handle h, int lt) { return measure_into(h, lt, h->t_Gm, h->t_Tm, nullptr, h->d_mpart); }
@@ -1137,7 +1181,7 @@ int mq_ensure(kdq_handle h, int dir, int k) {
 }
 
 // observables of X G X̃ (wrap_first: G wrapped backward through slice lt on the copy first)
-int measure_q(kdq_handle h, int lt, int k, bool wrap_first) {
+int measure_q(kdq_handle h, int lt, int k, bool wrap_first, bool fork = false) {
     Opnd *Gm = h->t_Gm, *Tm = h->t_Tm;
     if (h->mixed)
         HIPCHK(launch_to_f64(h->stream, h->nch, h->max_ns, h->d_chains, h->t_G32[h->cur32], Gm, DIM_NS, 1));
@@ -1145,36 +1189,47 @@ int measure_q(kdq_handle h, int lt, int k, bool wrap_first) {
         HIPCHK(launch_copy_mat(h->stream, h->nch, h->max_ns, h->max_ns, h->d_chains, h->t_G[h->cur], Gm, DIM_NS, DIM_NS, true));
     if (wrap_first)
         if (int r = wrap_slice(h, Gm, Tm, lt, false, KDQ_PROF_MEAS_WRAP)) return r;
+    if (fork)
+        if (int r = meas_fork(h)) return r;
     GemmArgs g1 = gargs(h, h->mq[k], Gm, Tm, DIM_NS, DIM_NS, DIM_NS);
-    if (int r = gemm(h, true, true, 0, 0, g1, KDQ_PROF_MEAS_WRAP)) return r;
+    int r = gemm(h, true, true, 0, 0, g1, KDQ_PROF_MEAS_WRAP);
     GemmArgs g2 = gargs(h, Tm, h->mqt[k], Gm, DIM_NS, DIM_NS, DIM_NS);
-    if (int r = gemm(h, true, true, 0, 0, g2, KDQ_PROF_MEAS_WRAP)) return r;
-    MeasArgs m{h->d_chains, Gm, nullptr, h->d_mpart, meas_tiles(h->max_ns)};
-    ProfScope ps(h, KDQ_PROF_MEASURE);
-    HIPCHK(launch_measure(h->stream, h->nch, m));
-    return KDQ_OK;
+    if (!r) r = gemm(h, true, true, 0, 0, g2, KDQ_PROF_MEAS_WRAP);
+    if (!r) {
+        MeasArgs m{h->d_chains, Gm, nullptr, h->d_mpart, meas_tiles(h->max_ns)};
+        ProfScope ps(h, KDQ_PROF_MEASURE);
+        if (hipError_t e = launch_measure(h->stream, h->nch, m)) r = fail(h, KDQ_EHIP, hipGetErrorString(e));
+    }
+    if (fork) {
+        if (int r2 = meas_unfork(h)) return r2;
+    }
+    return r;
 }
 
 // phy_measure at slice lt of a t-U sweep (after the slice's update): stepwise wraps where
 // they cost no more (|lt − c| ≤ 1, or 2 going backward above c), the slot products elsewhere
 int measure_sweep(kdq_handle h, int lt, bool forward) {
     if (h->kind != KDQ_MODEL_TU || h->mq.empty()) return measure(h, lt);
+    const bool fk = meas_aside(h);
+    if (fk)   // the previous measurement's back half still reads the copy and a slot matrix
+        if (int r = meas_join(h)) return r;
+    auto near = [&]() { return measure_into(h, lt, h->t_Gm, h->t_Tm, nullptr, h->d_mpart, fk); };
     const int c = h->nt / 2;
     if (lt <= c) {
         const int k = c - lt;
         if (!forward || k >= 2)   // backward: slot k - 1 .. k follow slice by slice
             if (int r = mq_ensure(h, 1, k)) return r;
-        return k <= 1 ? measure(h, lt) : measure_q(h, lt, k, false);
+        return k <= 1 ? near() : measure_q(h, lt, k, false, fk);
     }
     if (forward) {
         const int k = lt - c;
         if (int r = mq_ensure(h, 2, k)) return r;
-        return k <= 1 ? measure(h, lt) : measure_q(h, lt, k, false);
+        return k <= 1 ? near() : measure_q(h, lt, k, false, fk);
     }
     const int k = lt - 1 - c;   // slices c+1 .. lt-1, not yet updated in this pass
-    if (k <= 1) return measure(h, lt);
+    if (k <= 1) return near();
     if (int r = mq_ensure(h, 2, k)) return r;
-    return measure_q(h, lt, k, true);
+    return measure_q(h, lt, k, true, fk);
 }
 
 int elems(kdq_handle h, int c) {
@@ -1321,6 +1376,7 @@ int sweep_body(kdq_handle h, int record, uint32_t sweep) {
             if (split) HIPCHK(hipStreamWaitEvent(h->stream, h->ev_q2, 0));
         }
     }
+    if (int r = meas_join(h)) return r;
     if (h->sweep_dev) HIPCHK(launch_bump(h->stream, h->d_sweep));
     return KDQ_OK;
 }
@@ -1654,6 +1710,8 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
     if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_chain, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_q1, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_q2, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_mfork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_mdone, hipEventDisableTiming);
     if (e != hipSuccess) {
         int r = fail(h, KDQ_EHIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
         delete h;
@@ -2505,6 +2563,9 @@ int kdq_set_option(kdq_handle h, int32_t key, int32_t value) {
             break;
         case KDQ_OPT_CIRC:
             h->circ = value != 0;
+            break;
+        case KDQ_OPT_MASIDE:
+            h->maside = value != 0;
             break;
         case KDQ_OPT_QSPLIT:
             h->qsplit = value != 0;

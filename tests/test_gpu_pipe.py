@@ -127,3 +127,25 @@ def test_node_second_pass_beside_get_G(precision):
         assert np.array_equal(f1[c], f0[c])
         assert rel(g1[c], g0[c]) < (1e-11 if precision == "fp64" else 1e-5), (c, rel(g1[c], g0[c]))
     assert np.allclose(r1, r0, rtol=1e-9, atol=1e-11) if precision == "fp64" else np.allclose(r1, r0, rtol=1e-4, atol=1e-6)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_measurements_beside_the_sweep(precision):
+    # option "maside": a measurement's products and tiles on another stream; same kernels on
+    # the same operands, so the rows and everything else are bit-identical
+    from kapdqmc_amd import dqmc
+    print(f"case: measurements beside the sweep, {precision}", flush=True)
+    models = [tu(4, theta=2.0), tu(6, U=2.0, theta=2.0), tu(3, U=6.0, theta=2.0, bs=3)]
+    res = []
+    for ms in (1, 0):
+        b = dqmc.DQMCBatch(models, [5, 6, 7], [0, 1, 2], delay=16, precision=precision)
+        b.set_option("maside", ms)
+        b.initial_s()
+        rows = b.phy_update(3, True)
+        res.append(([b.get_fields(c) for c in range(b.n)], [b.get_G(c) for c in range(b.n)], rows))
+        b.close()
+    (f1, g1, r1), (f0, g0, r0) = res
+    for c in range(len(models)):
+        assert np.array_equal(f1[c], f0[c])
+        assert np.array_equal(g1[c], g0[c])
+    assert np.array_equal(r1, r0)
