@@ -194,9 +194,9 @@ KDQ_DEV void dft_pass(double2 *buf, int nlines, int npad, const double (&wr)[L],
 // The y pass with the 2 x 2 blocks between its two directions, in registers: thread (line,
 // kx = o, b) transforms its column forward, takes the other component (b ^ 1) of every
 // momentum from the neighbouring lane, forms y_b(k) = M_bb(k) x_b(k) + M_b,1-b(k) x_1-b(k) and
-// transforms back. M = Λ (eop 0), Λ(-k)ᵀ (eop 1) or Λᴴ (eop 2), from the chain's table
-// (Λ_ij(k) at tab + 8k + 2(2i + j)). Every thread of the workgroup calls it (the lane exchange
-// and one barrier).
+// transforms back. M = Λ (eop 0), Λ(-k)ᵀ (eop 1) or Λᴴ (eop 2), from the chain's table staged in
+// LDS (Λ_ij(k) at tab + 8k + 2(2i + j)). Every thread of the workgroup calls it (the lane
+// exchange and one barrier).
 template <int L>
 KDQ_DEV void y_lambda_pass(double2 *buf, int nlines, int npad, const double (&wr)[L], const double (&wi)[L],
                            const double *tab, int eop) {
@@ -246,7 +246,12 @@ KDQ_DEV void circ_body(const CircArgs &a, const ChainDev &ch, int c, int grp, un
     const int nlw = min(LPW, nl - line0);
     const bool use_in = a.in.kind != 0, use_out = a.out.kind != 0;
     double2 *buf = reinterpret_cast<double2 *>(lds);   // [LPW][NPAD]: p = 2(x + L y) + b at p + y
+    double *lamt = reinterpret_cast<double *>(buf + LPW * NPAD);   // the chain's Λ table of E [k][a][b]
     const int tid = threadIdx.x;
+    {   // staged while the lines load; read by the y pass
+        const double *tab = ch.circ + (size_t)a.mat * NC * 8;
+        for (int e = tid; e < NC * 8; e += CT) lamt[e] = tab[e];
+    }
     // the t-U HS factors take four values per slice (one per field value): cis(±α η_v) of the
     // `in`, `out` and `line` factors, evaluated once per workgroup (kind 0 = 1)
     __shared__ double2 tws[16];   // w[e] = e^{-2πi e/L}
@@ -306,7 +311,7 @@ KDQ_DEV void circ_body(const CircArgs &a, const ChainDev &ch, int c, int grp, un
     dft_pass<L, true, false>(buf, nlw, NPAD, wr, wi);
     // forward y, the 2 x 2 blocks of E, Eᵀ or Eᴴ acting on a line (a right product X·E acts on
     // the rows of X with Eᵀ), backward y
-    y_lambda_pass<L>(buf, nlw, NPAD, wr, wi, ch.circ + (size_t)a.mat * NC * 8, a.rows ? 1 : a.op);
+    y_lambda_pass<L>(buf, nlw, NPAD, wr, wi, lamt, a.rows ? 1 : a.op);
     dft_pass<L, false, false>(buf, nlw, NPAD, wr, wi);
     // store (scaled by `out` and the line factor)
     for (int e = tid; e < LPW * NS; e += CT) {
@@ -365,7 +370,7 @@ int circ_lines_per_wg(int L) { return lpw_of(L); }
 
 size_t circ_lds_bytes(int L) {
     const size_t lpw = (size_t)lpw_of(L), npad = (2 * (size_t)L + 1) * L + 1;
-    return sizeof(double2) * lpw * npad;
+    return sizeof(double2) * lpw * npad + sizeof(double) * 8 * (size_t)L * L;
 }
 
 hipError_t launch_circ(hipStream_t st, int nchains, int grid_x, size_t lds_bytes, const CircArgs &a) {
