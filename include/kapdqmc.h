@@ -274,9 +274,15 @@ int kdq_get_near_ties(kdq_handle h, double *out);
  *                     products and observable tiles run on a second stream beside the next
  *                     slices (its copy of G and any wrap reading fields the sweep may still
  *                     change stay in line); the next measurement and the end of the sweep wait
- *                     for it. 0: in line. Identical results. */
+ *                     for it. 0: in line. Identical results.
+ *  KDQ_OPT_GINV       The overlap solve of get_G! (X = (U_L U_R)⁻¹ U_L). 1 (default): the inverse
+ *                     by Gauss-Jordan elimination with partial pivoting held in registers of
+ *                     one workgroup per chain, then a GEMM (Np <= 144; larger Np take the LU).
+ *                     0: blocked LU with partial pivoting and its triangular solves. G agrees
+ *                     to rounding. */
 enum { KDQ_OPT_GRAPH = 0, KDQ_OPT_DELAY = 1, KDQ_OPT_GEMM_TILE = 2, KDQ_OPT_DEFER = 3, KDQ_OPT_PIPE = 4,
-       KDQ_OPT_FLUSH = 5, KDQ_OPT_ASIDE = 6, KDQ_OPT_CIRC = 7, KDQ_OPT_QSPLIT = 8, KDQ_OPT_MASIDE = 9 };
+       KDQ_OPT_FLUSH = 5, KDQ_OPT_ASIDE = 6, KDQ_OPT_CIRC = 7, KDQ_OPT_QSPLIT = 8, KDQ_OPT_MASIDE = 9,
+       KDQ_OPT_GINV = 10 };
 int kdq_set_option(kdq_handle h, int32_t key, int32_t value);
 
 /* Mixed precision (kdq_run_desc.precision = KDQ_PREC_MIXED, t-U): G is carried in FP32

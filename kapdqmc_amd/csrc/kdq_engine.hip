@@ -88,6 +88,7 @@ struct kdq_handle_s {
     Opnd *t_S2 = nullptr, *t_DI2 = nullptr;    // the same for a second Cholesky-QR pass on the aux stream
     bool qsplit = true;                        // t-U captured sweeps: that second pass beside get_G!
     bool maside = true;                        // t-U sweeps: a measurement's products and tiles on aux
+    bool ginv = true;                          // get_G!: register Gauss-Jordan inverse + GEMM (Np <= 144)
     bool mpending = false;                     // such a measurement not yet joined
     hipEvent_t ev_mfork = nullptr, ev_mdone = nullptr;
     hipEvent_t ev_q1 = nullptr, ev_q2 = nullptr;
@@ -632,13 +633,26 @@ int chain_join(kdq_handle h, Opnd *dst, bool ortho = true) {
 
 // get_G!: G = I - U_R (U_L U_R)⁻¹ U_L into the current G buffer
 // G = I - U_R (U_L U_R)⁻¹ U_L for explicit chain operands (log|det| not recorded)
-int get_G_from(kdq_handle h, Opnd *UR, Opnd *ULH, Opnd *dst) {
-    GemmArgs g1 = gargs(h, ULH, UR, h->t_M, DIM_NP, DIM_NP, DIM_NS);
-    if (int r = gemm(h, true, true, 2, 0, g1, KDQ_PROF_GETG)) return r;
-    LuArgs la{h->d_chains, h->t_M, DIM_NP, h->max_np, h->d_piv, nullptr, h->d_info, nullptr, h->t_DI};
+// X = M⁻¹ U_L for M in t_M (overwritten): the register Gauss-Jordan inverse and one GEMM
+// (Np <= 144, option on), else the blocked LU and its solve
+int overlap_solve(kdq_handle h, Opnd *ULH, double *logdet) {
+    if (h->ginv && inv_supported(h->max_np)) {
+        InvArgs ia{h->d_chains, h->t_M, h->t_M, DIM_NP, 0, logdet, h->d_info};
+        HIPCHK(launch_inv(h->stream, h->nch, h->max_np, ia));
+        GemmArgs g2 = gargs(h, h->t_M, ULH, h->t_X, DIM_NP, DIM_NS, DIM_NP);
+        return gemm(h, true, true, 0, 2, g2, KDQ_PROF_GETG);
+    }
+    LuArgs la{h->d_chains, h->t_M, DIM_NP, h->max_np, h->d_piv, logdet, h->d_info, nullptr, h->t_DI};
     HIPCHK(launch_lu(h->stream, h->nch, h->max_np, la));
     LuSolveArgs sa{h->d_chains, h->t_M, ULH, h->t_X, h->d_piv, DIM_NP, DIM_NS, h->max_np, 0, h->t_DI};
     HIPCHK(launch_lu_solve(h->stream, h->nch, h->max_np, h->max_ns, sa));
+    return KDQ_OK;
+}
+
+int get_G_from(kdq_handle h, Opnd *UR, Opnd *ULH, Opnd *dst) {
+    GemmArgs g1 = gargs(h, ULH, UR, h->t_M, DIM_NP, DIM_NP, DIM_NS);
+    if (int r = gemm(h, true, true, 2, 0, g1, KDQ_PROF_GETG)) return r;
+    if (int r = overlap_solve(h, ULH, nullptr)) return r;
     GemmArgs g3 = gargs(h, UR, h->t_X, dst, DIM_NS, DIM_NS, DIM_NP);
     g3.mode = 1;
     return gemm(h, true, true, 0, 0, g3, KDQ_PROF_GETG);
@@ -650,10 +664,7 @@ int get_G(kdq_handle h, int idx, Opnd *dst = nullptr, Opnd *UR_ = nullptr, Opnd 
     if (int r = gemm(h, true, true, 2, 0, g1, KDQ_PROF_GETG)) return r;
     {
         ProfScope ps(h, KDQ_PROF_INV);
-        LuArgs la{h->d_chains, h->t_M, DIM_NP, h->max_np, h->d_piv, h->d_logdet, h->d_info, nullptr, h->t_DI};
-        HIPCHK(launch_lu(h->stream, h->nch, h->max_np, la));
-        LuSolveArgs sa{h->d_chains, h->t_M, ULH, h->t_X, h->d_piv, DIM_NP, DIM_NS, h->max_np, 0, h->t_DI};
-        HIPCHK(launch_lu_solve(h->stream, h->nch, h->max_np, h->max_ns, sa));
+        if (int r = overlap_solve(h, ULH, h->d_logdet)) return r;
     }
     GemmArgs g3 = gargs(h, UR, h->t_X, dst ? dst : h->t_G[h->cur], DIM_NS, DIM_NS, DIM_NP);
     g3.mode = 1;
@@ -2566,6 +2577,9 @@ int kdq_set_option(kdq_handle h, int32_t key, int32_t value) {
             break;
         case KDQ_OPT_MASIDE:
             h->maside = value != 0;
+            break;
+        case KDQ_OPT_GINV:
+            h->ginv = value != 0;
             break;
         case KDQ_OPT_QSPLIT:
             h->qsplit = value != 0;

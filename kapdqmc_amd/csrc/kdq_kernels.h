@@ -172,6 +172,20 @@ struct LuSolveArgs {
     const Opnd *Dinv;  // when set: the diagonal-block inverses written by k_lu
 };
 hipError_t launch_lu_solve(hipStream_t st, int nchains, int max_n, int max_r, const LuSolveArgs &s);
+
+// X = A⁻¹ for the per-chain n x n complex A (n <= 144) by Gauss-Jordan elimination with partial
+// pivoting in registers; X may be A. logdet (if set) receives log|det A|, info (sticky) j+1 for
+// an exactly zero pivot at step j.
+struct InvArgs {
+    const ChainDev *chains;
+    const Opnd *A;
+    const Opnd *X;
+    int ndim, pad;
+    double *logdet;
+    int *info;
+};
+bool inv_supported(int max_n);
+hipError_t launch_inv(hipStream_t st, int nchains, int max_n, const InvArgs &a);
 size_t factor_lds_bytes(int which, int max_n);
 
 // Observables of phy_measure on the sandwiched G0 (t-U); accumulates into chains[c].meas.
