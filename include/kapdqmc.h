@@ -279,10 +279,15 @@ int kdq_get_near_ties(kdq_handle h, double *out);
  *                     by Gauss-Jordan elimination with partial pivoting held in registers of
  *                     one workgroup per chain, then a GEMM (Np <= 144; larger Np take the LU).
  *                     0: blocked LU with partial pivoting and its triangular solves. G agrees
- *                     to rounding. */
+ *                     to rounding.
+ *  KDQ_OPT_QINV       Each Cholesky-QR pass of the orthonormalisation. 1 (default): the Cholesky
+ *                     factor of UᴴU and its inverse in one register pass of one workgroup per
+ *                     chain, then U L⁻ᴴ as a GEMM into a second buffer (Np <= 144). 0: the
+ *                     blocked Cholesky and the in-place triangular solve. Same span; the
+ *                     orthonormal basis agrees to rounding. */
 enum { KDQ_OPT_GRAPH = 0, KDQ_OPT_DELAY = 1, KDQ_OPT_GEMM_TILE = 2, KDQ_OPT_DEFER = 3, KDQ_OPT_PIPE = 4,
        KDQ_OPT_FLUSH = 5, KDQ_OPT_ASIDE = 6, KDQ_OPT_CIRC = 7, KDQ_OPT_QSPLIT = 8, KDQ_OPT_MASIDE = 9,
-       KDQ_OPT_GINV = 10 };
+       KDQ_OPT_GINV = 10, KDQ_OPT_QINV = 11 };
 int kdq_set_option(kdq_handle h, int32_t key, int32_t value);
 
 /* Mixed precision (kdq_run_desc.precision = KDQ_PREC_MIXED, t-U): G is carried in FP32

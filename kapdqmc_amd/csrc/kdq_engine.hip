@@ -34,7 +34,7 @@ struct ChainHost {
     int ns = 0, np = 0;
     std::vector<void *> allocs;
     double *eK, *eKinv, *hK, *hKinv, *K, *Pt, *alpha;
-    double *G[2], *T, *Gm, *Tm, *P0, *P1, *P2, *M, *X, *S;
+    double *G[2], *T, *Gm, *Tm, *P0, *P1, *P2, *M, *X, *S, *QT;
     float *G32[2] = {nullptr, nullptr}, *T32 = nullptr, *eK32 = nullptr, *eKinv32 = nullptr;   // mixed precision
     double *FZ = nullptr, *FC = nullptr;     // flush scratch: W·Y (64 x N) and gathered columns (N x 64)
     float *FZ32 = nullptr, *FC32 = nullptr;
@@ -86,6 +86,8 @@ struct kdq_handle_s {
     Opnd *t_DU = nullptr, *t_DV = nullptr;     // deferred-flush factors (t-U)
     Opnd *t_DI = nullptr;                      // diagonal-block inverses of the Np x Np factors
     Opnd *t_S2 = nullptr, *t_DI2 = nullptr;    // the same for a second Cholesky-QR pass on the aux stream
+    Opnd *t_QT = nullptr;                      // N x Np: the first Cholesky-QR pass's result (register factor path)
+    bool qinv = true;                          // Cholesky-QR by the register factor-inverse + GEMM (Np <= 144)
     bool qsplit = true;                        // t-U captured sweeps: that second pass beside get_G!
     bool maside = true;                        // t-U sweeps: a measurement's products and tiles on aux
     bool ginv = true;                          // get_G!: register Gauss-Jordan inverse + GEMM (Np <= 144)
@@ -486,7 +488,30 @@ int ortho_pass(kdq_handle h, Opnd *U, int pass, double *logacc, Opnd *S, Opnd *D
     return KDQ_OK;
 }
 
+// The same pass out of place, dst = src L⁻ᴴ: the register Cholesky factor-inverse (X = L⁻¹
+// into S) and one GEMM src · Xᴴ (Np <= 144)
+bool qinv_on(kdq_handle h) { return h->qinv && inv_supported(h->max_np); }
+
+int ortho_pass_to(kdq_handle h, Opnd *src, Opnd *dst, int pass, double *logacc, Opnd *S) {
+    const double u = 1.1102230246251565e-16;
+    const double n = h->max_ns, k = h->max_np;
+    const double shift = 11.0 * (n * k + k * (k + 1.0)) * u;
+    GemmArgs g = gargs(h, src, src, S, DIM_NP, DIM_NP, DIM_NS);
+    if (int r = gemm(h, true, true, 2, 0, g, KDQ_PROF_QR)) return r;
+    {
+        ProfScope ps(h, KDQ_PROF_QR);
+        CholInvArgs ca{h->d_chains, S, S, DIM_NP, 0, pass == 0 ? shift : 0.0, h->d_qinfo, logacc};
+        HIPCHK(launch_cholinv(h->stream, h->nch, h->max_np, ca));
+    }
+    GemmArgs g2 = gargs(h, src, S, dst, DIM_NS, DIM_NP, DIM_NP);
+    return gemm(h, true, true, 0, 2, g2, KDQ_PROF_QR);
+}
+
 int orthonormalise(kdq_handle h, Opnd *U, double *logacc = nullptr) {
+    if (qinv_on(h)) {
+        if (int r = ortho_pass_to(h, U, h->t_QT, 0, logacc, h->t_S)) return r;
+        return ortho_pass_to(h, h->t_QT, U, 1, logacc, h->t_S);
+    }
     for (int pass = 0; pass < 2; ++pass)
         if (int r = ortho_pass(h, U, pass, logacc, h->t_S, h->t_DI)) return r;
     return KDQ_OK;
@@ -497,10 +522,31 @@ int orthonormalise(kdq_handle h, Opnd *U, double *logacc = nullptr) {
 // for κ(U) well inside u^(-1/2)·10^4, Fukaya et al. 2020), so the node's G is formed from a copy
 // of the first-pass factor while the second pass, which only restores orthonormality for the
 // next block's propagation, runs on the aux stream with its own Gram and block-inverse
-// buffers. Returns the copy (t_P2); the caller waits for ev_q2 before U is used again.
+// buffers. Returns the copy (t_P2; with the register factor path the first pass writes t_QT,
+// which the second pass maps back to U, so no copy is made); the caller waits for ev_q2
+// before U or that buffer is used again.
 bool qsplit_on(kdq_handle h) { return h->kind == KDQ_MODEL_TU && h->qsplit && !h->scee && h->aux && h->t_S2; }
 
 Opnd *orthonormalise_split(kdq_handle h, Opnd *U, int &rc) {
+    if (qinv_on(h)) {   // first pass U -> t_QT, which get_G! reads while the second maps it back to U
+        rc = ortho_pass_to(h, U, h->t_QT, 0, nullptr, h->t_S);
+        if (rc) return nullptr;
+        hipError_t e = hipEventRecord(h->ev_q1, h->stream);
+        if (e == hipSuccess) e = hipStreamWaitEvent(h->aux, h->ev_q1, 0);
+        if (e != hipSuccess) {
+            rc = fail(h, KDQ_EHIP, std::string("orthonormalise_split: ") + hipGetErrorString(e));
+            return nullptr;
+        }
+        std::swap(h->stream, h->aux);
+        rc = ortho_pass_to(h, h->t_QT, U, 1, nullptr, h->t_S2);
+        std::swap(h->stream, h->aux);
+        if (rc) return nullptr;
+        if ((e = hipEventRecord(h->ev_q2, h->aux)) != hipSuccess) {
+            rc = fail(h, KDQ_EHIP, std::string("orthonormalise_split: ") + hipGetErrorString(e));
+            return nullptr;
+        }
+        return h->t_QT;
+    }
     rc = ortho_pass(h, U, 0, nullptr, h->t_S, h->t_DI);
     if (rc) return nullptr;
     hipError_t e = launch_copy_mat(h->stream, h->nch, h->max_ns, h->max_np, h->d_chains, U, h->t_P2, DIM_NS, DIM_NP, true);
@@ -1729,7 +1775,7 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
         return r;
     }
     std::vector<ChainDev> cd(nchains);
-    std::vector<Opnd> v_eK, v_eKinv, v_hK, v_hKinv, v_Pt, v_G0, v_G1, v_T, v_Gm, v_Tm, v_P0, v_P1, v_P2, v_M, v_X, v_S;
+    std::vector<Opnd> v_eK, v_eKinv, v_hK, v_hKinv, v_Pt, v_G0, v_G1, v_T, v_Gm, v_Tm, v_P0, v_P1, v_P2, v_QT, v_M, v_X, v_S;
     std::vector<OpndF> v_G32a, v_G32b, v_T32, v_eK32, v_eKinv32, v_FZ32, v_FC32;
     std::vector<Opnd> v_FZ, v_FC, v_DU, v_DV, v_DI, v_S2, v_DI2;
     std::vector<OpndF> v_DU32, v_DV32;
@@ -1762,6 +1808,7 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
         ok &= dalloc(C, &C.P0, 2 * nP) == hipSuccess;
         ok &= dalloc(C, &C.P1, 2 * nP) == hipSuccess;
         ok &= dalloc(C, &C.P2, 2 * nP) == hipSuccess;
+        ok &= dalloc(C, &C.QT, 2 * nP) == hipSuccess;
         ok &= dalloc(C, &C.M, 2 * pp) == hipSuccess;
         ok &= dalloc(C, &C.S, 2 * pp) == hipSuccess;
         ok &= dalloc(C, &C.X, 2 * nP) == hipSuccess;
@@ -1960,6 +2007,7 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
         v_G0.push_back(cplx(C.G[0], n, nn)); v_G1.push_back(cplx(C.G[1], n, nn));
         v_T.push_back(cplx(C.T, n, nn)); v_Gm.push_back(cplx(C.Gm, n, nn)); v_Tm.push_back(cplx(C.Tm, n, nn));
         v_P0.push_back(cplx(C.P0, n, nP)); v_P1.push_back(cplx(C.P1, n, nP)); v_P2.push_back(cplx(C.P2, n, nP));
+        v_QT.push_back(cplx(C.QT, n, nP));
         v_M.push_back(cplx(C.M, np, pp)); v_X.push_back(cplx(C.X, np, nP));
         v_S.push_back(cplx(C.S, np, pp));
         v_FZ.push_back(Opnd{C.FZ, C.FZ + 64 * (size_t)n, 64, 0});
@@ -2010,6 +2058,7 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
     r |= make_table(h, &h->t_G[0], v_G0); r |= make_table(h, &h->t_G[1], v_G1);
     r |= make_table(h, &h->t_T, v_T); r |= make_table(h, &h->t_Gm, v_Gm); r |= make_table(h, &h->t_Tm, v_Tm);
     r |= make_table(h, &h->t_P0, v_P0); r |= make_table(h, &h->t_P1, v_P1); r |= make_table(h, &h->t_P2, v_P2);
+    r |= make_table(h, &h->t_QT, v_QT);
     if (!blk_all.empty()) {
         void *q1 = nullptr, *q2 = nullptr, *q3 = nullptr;
         if (hipMalloc(&q1, sizeof(BlockDesc) * blk_all.size()) == hipSuccess) h->dev_misc.push_back(q1); else r = 1;
@@ -2580,6 +2629,9 @@ int kdq_set_option(kdq_handle h, int32_t key, int32_t value) {
             break;
         case KDQ_OPT_GINV:
             h->ginv = value != 0;
+            break;
+        case KDQ_OPT_QINV:
+            h->qinv = value != 0;
             break;
         case KDQ_OPT_QSPLIT:
             h->qsplit = value != 0;

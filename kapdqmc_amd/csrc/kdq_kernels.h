@@ -186,6 +186,20 @@ struct InvArgs {
 };
 bool inv_supported(int max_n);
 hipError_t launch_inv(hipStream_t st, int nchains, int max_n, const InvArgs &a);
+
+// X = L⁻¹ (lower, zeros above) for S + σI = L Lᴴ, S Hermitian n x n (n <= 144), σ = shift ·
+// trace S, in one register pass; X may be S. logdiag (if set) += Σ log L_jj, info (sticky) j+1
+// for a non-positive pivot.
+struct CholInvArgs {
+    const ChainDev *chains;
+    const Opnd *A;
+    const Opnd *X;
+    int ndim, pad;
+    double shift;
+    int *info;
+    double *logdiag;
+};
+hipError_t launch_cholinv(hipStream_t st, int nchains, int max_n, const CholInvArgs &a);
 size_t factor_lds_bytes(int which, int max_n);
 
 // Observables of phy_measure on the sandwiched G0 (t-U); accumulates into chains[c].meas.
