@@ -19,9 +19,9 @@
 
 namespace {
 
-// f(std::integral_constant<int, 0>{}), ..., f(.. N - 1 ..): the column slot of a step is then a
-// compile-time index into the register arrays (a loop the compiler might not unroll would turn
-// them into scratch memory)
+// f(std::integral_constant<int, 0>{}), ..., f(.. N - 1 ..): a compile-time column slot per
+// block of 16 steps (a loop the compiler might not unroll would turn the register arrays into
+// scratch memory)
 template <class F, int... I>
 KDQ_DEV void static_for_impl(F &&f, std::integer_sequence<int, I...>) {
     (f(std::integral_constant<int, I>{}), ...);
@@ -35,139 +35,209 @@ constexpr int IT = 512;
 constexpr int SENT = 1 << 30;
 
 template <int RS, int CS>
+struct RegLds {
+    double colr[2][32 * RS], coli[2][32 * RS];   // the step's multiplier column (by row)
+    double rowr[16][CS], rowi[16][CS];           // a row of the matrix passed to its half
+    double s1[2], s2[2];                          // the step's scalars
+    int sp[2];                                    // the step's pivot row
+    double pd[16 * CS];                           // per step: |pivot|² or the Cholesky diagonal
+    double ir[16 * CS], ii[16 * CS];              // per step: 1/pivot (1/d for the Cholesky pass)
+    int pvt[16 * CS], qv[32 * RS];                // pivot row of step j, step of pivot row r
+    double red[IT / 64];
+    int bad;
+};
+
+// Loads the n x n matrix into the register layout (zeros outside it); returns the trace.
+// LOWER: only the slots holding some entry on or below the diagonal (32 i + 31 >= 16 k); the
+// others are 0 and stay 0 (the Cholesky pass never needs them, so they take no registers).
+KDQ_DEV constexpr bool low_slot(int i, int k) { return 32 * i + 31 >= 16 * k; }
+
+template <int RS, int CS, bool LOWER>
+KDQ_DEV double load_regs(const Opnd &A, int n, int tr, int tc, double (&xr)[RS][CS], double (&xi)[RS][CS]) {
+    double t = 0.0;
+#pragma unroll
+    for (int i = 0; i < RS; ++i) {
+        const int r = tr + 32 * i;
+#pragma unroll
+        for (int k = 0; k < CS; ++k) {
+            const int col = tc + 16 * k;
+            const bool ok = r < n && col < n && (!LOWER || low_slot(i, k));
+            const size_t o = ok ? (size_t)r + (size_t)col * A.ld : 0;
+            xr[i][k] = ok ? A.re[o] : 0.0;
+            xi[i][k] = ok ? A.im[o] : 0.0;
+            if (ok && r == col) t += xr[i][k];
+        }
+    }
+    return t;
+}
+
+// x[i][K] -= d_i u for the row slots i >= i0 (i0 uniform)
+template <int RS, int CS, bool LOWER>
+KDQ_DEV void upd_slot(double (&xr)[RS][CS], double (&xi)[RS][CS], const double (&dr)[RS], const double (&di)[RS],
+                      int K, double ur, double ui, int i0) {
+#pragma unroll
+    for (int i = 0; i < RS; ++i)
+        if (i >= i0 && (!LOWER || low_slot(i, K))) {
+            xr[i][K] = fma(-dr[i], ur, fma(di[i], ui, xr[i][K]));
+            xi[i][K] = fma(-dr[i], ui, fma(-di[i], ur, xi[i][K]));
+        }
+}
+
+// Gauss-Jordan pivot search of step j, called by the half holding column j (slot K): the
+// pivot row p among the rows not pivoted yet; the column to LDS (0 at row p); 1/piv.
+template <int RS, int CS>
+KDQ_DEV void inv_pivot(RegLds<RS, CS> &S, const double (&xr)[RS][CS], const double (&xi)[RS][CS], unsigned elim,
+                       int K, int tr, int hb, int j) {
+    const int b = j & 1;
+    double best = -1.0;
+    int bi = SENT;
+#pragma unroll
+    for (int i = 0; i < RS; ++i)
+        if (!((elim >> i) & 1u)) {
+            const double v = xr[i][K] * xr[i][K] + xi[i][K] * xi[i][K];
+            if (bi == SENT || v > best) {
+                best = v;
+                bi = tr + 32 * i;
+            }
+        }
+#pragma unroll
+    for (int off = 16; off > 0; off >>= 1) {
+        const double ob = __shfl_xor(best, off, 64);
+        const int oi = __shfl_xor(bi, off, 64);
+        if (oi != SENT && (bi == SENT || ob > best || (ob == best && oi < bi))) {
+            best = ob;
+            bi = oi;
+        }
+    }
+    const int p = __shfl(bi, hb, 64), ps = p >> 5, pl = p & 31;
+    double vr = xr[0][K], vi = xi[0][K];
+#pragma unroll
+    for (int i = 1; i < RS; ++i)
+        if (ps == i) {
+            vr = xr[i][K];
+            vi = xi[i][K];
+        }
+    vr = __shfl(vr, hb + pl, 64);
+    vi = __shfl(vi, hb + pl, 64);
+#pragma unroll
+    for (int i = 0; i < RS; ++i) {
+        const int r = tr + 32 * i;
+        S.colr[b][r] = r == p ? 0.0 : xr[i][K];
+        S.coli[b][r] = r == p ? 0.0 : xi[i][K];
+    }
+    if (tr == 0) {
+        const double p2 = vr * vr + vi * vi;
+        const bool ok = p2 > 0.0;   // an exactly singular step leaves the matrix as it is
+        S.sp[b] = p;
+        S.pvt[j] = p;
+        S.qv[p] = j;
+        S.pd[j] = p2;
+        S.s1[b] = S.ir[j] = ok ? vr / p2 : 0.0;
+        S.s2[b] = S.ii[j] = ok ? -vi / p2 : 0.0;
+    }
+}
+
+// Cholesky column of step j, called by the half holding column j (slot K): d = sqrt of the
+// current diagonal, the multipliers l_r = x_rj / d (rows > j, else 0) to LDS, 1/d
+template <int RS, int CS>
+KDQ_DEV void chol_pivot(RegLds<RS, CS> &S, const double (&xr)[RS][CS], const double (&xi)[RS][CS], int K, int tr,
+                        int hb, int j) {
+    const int b = j & 1, js = j >> 5, jl = j & 31;
+    double v = xr[0][K];
+#pragma unroll
+    for (int i = 1; i < RS; ++i)
+        if (js == i) v = xr[i][K];
+    const double dj = __shfl(v, hb + jl, 64);
+    const double d = sqrt(fmax(dj, 1e-300)), id = 1.0 / d;
+#pragma unroll
+    for (int i = 0; i < RS; ++i) {
+        const int r = tr + 32 * i;
+        S.colr[b][r] = r > j ? xr[i][K] * id : 0.0;
+        S.coli[b][r] = r > j ? xi[i][K] * id : 0.0;
+    }
+    if (tr == 0) {
+        S.s1[b] = id;
+        S.ir[j] = id;
+        S.pd[j] = dj;
+    }
+}
+
+// block sum of v (every thread calls it)
+template <int RS, int CS>
+KDQ_DEV double block_sum(RegLds<RS, CS> &S, double v, int tid) {
+    v = wave_sum(v);
+    __syncthreads();
+    if ((tid & 63) == 0) S.red[tid >> 6] = v;
+    __syncthreads();
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < IT / 64; ++w) t += S.red[w];
+    return t;
+}
+
+// In-place Gauss-Jordan with the pivot row kept unnormalised: at step j (pivot p, piv = m_pj)
+// the column owners set column j to 0 outside row p and to 1 at row p, and every entry takes
+// m_rc -= d_r u_c with u = row p / piv (u_j = 1/piv, formed by the lane holding row p) and
+// d = column j (d_p = 0). Rows r != p get the Gauss-Jordan update; row p stays piv · u (its
+// column j: 1 = piv / piv), and keeps that factor through later steps (its multipliers carry
+// it too), so the store scales it by 1/piv. No register is written under a per-lane condition
+// on the pivot row (the compiler would evaluate such code for every slot).
+template <int RS, int CS>
 __global__ __launch_bounds__(IT) void k_inv(InvArgs a) {
     const int c = blockIdx.x;
     const ChainDev &ch = a.chains[c];
     const int n = dim_of(ch, a.ndim);
     const Opnd A = a.A[c], X = a.X[c];
     const int tid = threadIdx.x, tr = tid & 31, tc = tid >> 5, hb = tid & 32;
-    __shared__ double colr[2][32 * RS], coli[2][32 * RS];   // column j of step j, 0 at the pivot row
-    __shared__ double rowr[16][CS], rowi[16][CS];           // the normalised pivot row, per half
-    __shared__ double sir[2], sii[2];                       // 1 / pivot
-    __shared__ int sp[2], pvt[16 * CS], qv[32 * RS], bad;
-    __shared__ double red[IT / 64];
+    __shared__ RegLds<RS, CS> S;
     double xr[RS][CS], xi[RS][CS];
+    load_regs<RS, CS, false>(A, n, tr, tc, xr, xi);
     unsigned elim = 0;   // bit i: row tr + 32 i pivoted already (or outside the matrix)
 #pragma unroll
-    for (int i = 0; i < RS; ++i) {
-        const int r = tr + 32 * i;
-        if (r >= n) elim |= 1u << i;
-#pragma unroll
-        for (int k = 0; k < CS; ++k) {
-            const int col = tc + 16 * k;
-            const bool ok = r < n && col < n;
-            const size_t o = ok ? (size_t)r + (size_t)col * A.ld : 0;
-            xr[i][k] = ok ? A.re[o] : 0.0;
-            xi[i][k] = ok ? A.im[o] : 0.0;
-        }
-    }
-    if (tid == 0) bad = 0;
-    double ldet = 0.0;
+    for (int i = 0; i < RS; ++i)
+        if (tr + 32 * i >= n) elim |= 1u << i;
 #pragma unroll
     for (int kj = 0; kj < CS; ++kj) {
         for (int jj = 0; jj < 16; ++jj) {
             const int j = 16 * kj + jj;
             if (j >= n) break;
             const int b = j & 1;
-            if (tc == jj) {
-                double best = -1.0;
-                int bi = SENT;
-#pragma unroll
-                for (int i = 0; i < RS; ++i)
-                    if (!((elim >> i) & 1u)) {
-                        const double v = xr[i][kj] * xr[i][kj] + xi[i][kj] * xi[i][kj];
-                        if (bi == SENT || v > best) {
-                            best = v;
-                            bi = tr + 32 * i;
-                        }
-                    }
-#pragma unroll
-                for (int off = 16; off > 0; off >>= 1) {
-                    const double ob = __shfl_xor(best, off, 64);
-                    const int oi = __shfl_xor(bi, off, 64);
-                    if (oi != SENT && (bi == SENT || ob > best || (ob == best && oi < bi))) {
-                        best = ob;
-                        bi = oi;
-                    }
-                }
-                const int p = __shfl(bi, hb, 64), ps = p >> 5, pl = p & 31;
-                double vr = xr[0][kj], vi = xi[0][kj];
-#pragma unroll
-                for (int i = 1; i < RS; ++i)
-                    if (ps == i) {
-                        vr = xr[i][kj];
-                        vi = xi[i][kj];
-                    }
-                vr = __shfl(vr, hb + pl, 64);
-                vi = __shfl(vi, hb + pl, 64);
-#pragma unroll
-                for (int i = 0; i < RS; ++i) {
-                    const int r = tr + 32 * i;
-                    colr[b][r] = r == p ? 0.0 : xr[i][kj];
-                    coli[b][r] = r == p ? 0.0 : xi[i][kj];
-                }
-                if (tr == 0) {
-                    const double p2 = vr * vr + vi * vi;
-                    sp[b] = p;
-                    pvt[j] = p;
-                    qv[p] = j;
-                    if (p2 > 0.0) {
-                        sir[b] = vr / p2;
-                        sii[b] = -vi / p2;
-                        ldet += 0.5 * log(p2);
-                    } else {   // exactly singular: flagged, the step leaves the matrix as it is
-                        sir[b] = 0.0;
-                        sii[b] = 0.0;
-                        if (!bad) bad = j + 1;
-                    }
-                }
-            }
+            if (tc == jj) inv_pivot<RS, CS>(S, xr, xi, elim, kj, tr, hb, j);
             __syncthreads();
-            const int p = sp[b], ps = p >> 5, pl = p & 31;
-            const double ir = sir[b], ii = sii[b];
+            const int p = __builtin_amdgcn_readfirstlane(S.sp[b]), ps = p >> 5, pl = p & 31;
+            const double ir = S.s1[b], ii = S.s2[b];
             if (tr == pl) elim |= 1u << ps;
+            // u = row p / piv (u_j = 1/piv) from the lane holding row p, to its half's LDS row
 #pragma unroll
             for (int i = 0; i < RS; ++i)
-                if (ps == i && tr == pl) {
+                if (ps == i) {
+                    if (tr == pl) {
 #pragma unroll
-                    for (int k = 0; k < CS; ++k) {
-                        double nr, ni;
-                        if (tc + 16 * k == j) {
-                            nr = ir;
-                            ni = ii;
-                        } else {
-                            cmul(xr[i][k], xi[i][k], ir, ii, nr, ni);
+                        for (int k = 0; k < CS; ++k) {
+                            double nr = ir, ni = ii;
+                            if (tc + 16 * k != j) cmul(xr[i][k], xi[i][k], ir, ii, nr, ni);
+                            S.rowr[tc][k] = nr;
+                            S.rowi[tc][k] = ni;
                         }
-                        xr[i][k] = nr;
-                        xi[i][k] = ni;
-                        rowr[tc][k] = nr;
-                        rowi[tc][k] = ni;
                     }
                 }
             __builtin_amdgcn_wave_barrier();
             if (tc == jj) {
 #pragma unroll
-                for (int i = 0; i < RS; ++i)
-                    if (tr + 32 * i != p) {
-                        xr[i][kj] = 0.0;
-                        xi[i][kj] = 0.0;
-                    }
+                for (int i = 0; i < RS; ++i) {
+                    xr[i][kj] = tr + 32 * i == p ? 1.0 : 0.0;
+                    xi[i][kj] = 0.0;
+                }
             }
             double dr[RS], di[RS];
 #pragma unroll
             for (int i = 0; i < RS; ++i) {
-                dr[i] = colr[b][tr + 32 * i];
-                di[i] = coli[b][tr + 32 * i];
+                dr[i] = S.colr[b][tr + 32 * i];
+                di[i] = S.coli[b][tr + 32 * i];
             }
 #pragma unroll
-            for (int k = 0; k < CS; ++k) {
-                const double ur = rowr[tc][k], ui = rowi[tc][k];
-#pragma unroll
-                for (int i = 0; i < RS; ++i) {
-                    xr[i][k] = fma(-dr[i], ur, fma(di[i], ui, xr[i][k]));
-                    xi[i][k] = fma(-dr[i], ui, fma(-di[i], ur, xi[i][k]));
-                }
-            }
+            for (int k = 0; k < CS; ++k)
+                upd_slot<RS, CS, false>(xr, xi, dr, di, k, S.rowr[tc][k], S.rowi[tc][k], 0);
             __builtin_amdgcn_wave_barrier();
         }
     }
@@ -175,28 +245,35 @@ __global__ __launch_bounds__(IT) void k_inv(InvArgs a) {
 #pragma unroll
     for (int i = 0; i < RS; ++i) {
         const int r = tr + 32 * i;
+        if (r < n) {
+            const int q = S.qv[r];
+            const double sr = S.ir[q], si = S.ii[q];
 #pragma unroll
-        for (int k = 0; k < CS; ++k) {
-            const int col = tc + 16 * k;
-            if (r < n && col < n) {
-                const size_t o = (size_t)qv[r] + (size_t)pvt[col] * X.ld;
-                X.re[o] = xr[i][k];
-                X.im[o] = xi[i][k];
+            for (int k = 0; k < CS; ++k) {
+                const int col = tc + 16 * k;
+                if (col < n) {
+                    double vr, vi;
+                    cmul(xr[i][k], xi[i][k], sr, si, vr, vi);
+                    const size_t o = (size_t)q + (size_t)S.pvt[col] * X.ld;
+                    X.re[o] = vr;
+                    X.im[o] = vi;
+                }
             }
         }
     }
-    if (a.logdet) {
-        double s = wave_sum(ldet);
-        if ((tid & 63) == 0) red[tid >> 6] = s;
-        __syncthreads();
-        if (tid == 0) {
-            double t = 0.0;
-#pragma unroll
-            for (int w = 0; w < IT / 64; ++w) t += red[w];
-            a.logdet[c] = t;
-        }
+    if (tid == 0) S.bad = 0x7fffffff;
+    __syncthreads();
+    double l = 0.0;
+    for (int j = tid; j < n; j += IT) {
+        const double p2 = S.pd[j];
+        if (p2 > 0.0) l += 0.5 * log(p2);
+        else atomicMin(&S.bad, j + 1);
     }
-    if (tid == 0 && bad && a.info && !a.info[c]) a.info[c] = bad;
+    const double ld = block_sum<RS, CS>(S, l, tid);
+    if (tid == 0) {
+        if (a.logdet) a.logdet[c] = ld;
+        if (S.bad != 0x7fffffff && a.info && !a.info[c]) a.info[c] = S.bad;
+    }
 }
 
 // X = L⁻¹ for the Cholesky factor of S + σI = L Lᴴ (S Hermitian n x n, n <= 144, σ = shift ·
@@ -205,8 +282,10 @@ __global__ __launch_bounds__(IT) void k_inv(InvArgs a) {
 // one rank-1 update to the rows below j with the vector u = (row j of the partial inverse
 // scaled by 1/d for columns <= j, conj(l) for columns > j): the right-hand columns continue the
 // Cholesky factorisation, the left-hand ones the forward elimination of [L | I], which needs
-// L's column j just when the factorisation makes it. X is lower triangular (zeros above);
-// log d_j summed into logdiag, info (sticky) j+1 for a non-positive pivot.
+// L's column j just when the factorisation makes it. Row j is left unscaled in the registers
+// (later steps neither read nor change it) and scaled by 1/d when stored. X is lower
+// triangular (zeros above); log d_j summed into logdiag, info (sticky) j+1 for a non-positive
+// pivot.
 template <int RS, int CS>
 __global__ __launch_bounds__(IT) void k_cholinv(CholInvArgs a) {
     const int c = blockIdx.x;
@@ -214,86 +293,38 @@ __global__ __launch_bounds__(IT) void k_cholinv(CholInvArgs a) {
     const int n = dim_of(ch, a.ndim);
     const Opnd A = a.A[c], X = a.X[c];
     const int tid = threadIdx.x, tr = tid & 31, tc = tid >> 5, hb = tid & 32;
-    __shared__ double colr[2][32 * RS + 16 * CS], coli[2][32 * RS + 16 * CS];   // l (0 at rows <= j)
-    __shared__ double rowr[16][CS], rowi[16][CS];   // row j of the partial inverse, per half
-    __shared__ double sid[2];                       // 1 / d
-    __shared__ int bad;
-    __shared__ double red[IT / 64];
+    __shared__ RegLds<RS, CS> S;
     double xr[RS][CS], xi[RS][CS];
-    double tr0 = 0.0;
-#pragma unroll
-    for (int i = 0; i < RS; ++i) {
-        const int r = tr + 32 * i;
-#pragma unroll
-        for (int k = 0; k < CS; ++k) {
-            const int col = tc + 16 * k;
-            const bool ok = r < n && col < n;
-            const size_t o = ok ? (size_t)r + (size_t)col * A.ld : 0;
-            xr[i][k] = ok ? A.re[o] : 0.0;
-            xi[i][k] = ok ? A.im[o] : 0.0;
-            if (ok && r == col) tr0 += xr[i][k];
-        }
-    }
-    if (tid == 0) bad = 0;
+    const double tr0 = load_regs<RS, CS, true>(A, n, tr, tc, xr, xi);
     {
-        const double s = wave_sum(tr0);
-        if ((tid & 63) == 0) red[tid >> 6] = s;
-        __syncthreads();
-        double t = 0.0;
-#pragma unroll
-        for (int w = 0; w < IT / 64; ++w) t += red[w];
-        const double sigma = a.shift * t;
+        const double sigma = a.shift * block_sum<RS, CS>(S, tr0, tid);
 #pragma unroll
         for (int i = 0; i < RS; ++i)
 #pragma unroll
             for (int k = 0; k < CS; ++k)
                 if (tr + 32 * i == tc + 16 * k) xr[i][k] += sigma;
     }
-    double lsum = 0.0;
     static_for<CS>([&](auto kjc) {
         constexpr int kj = decltype(kjc)::value;
         for (int jj = 0; jj < 16; ++jj) {
             const int j = 16 * kj + jj;
             if (j >= n) break;
             const int b = j & 1, js = j >> 5, jl = j & 31;
-            if (tc == jj) {
-                double v = xr[0][kj];
-#pragma unroll
-                for (int i = 1; i < RS; ++i)
-                    if (js == i) v = xr[i][kj];
-                const double dj = __shfl(v, hb + jl, 64);
-                const double d = sqrt(fmax(dj, 1e-300)), id = 1.0 / d;
-#pragma unroll
-                for (int i = 0; i < RS; ++i) {
-                    const int r = tr + 32 * i;
-                    colr[b][r] = r > j ? xr[i][kj] * id : 0.0;
-                    coli[b][r] = r > j ? xi[i][kj] * id : 0.0;
-                }
-                if (tr == 0) {
-                    sid[b] = id;
-                    lsum += log(d);
-                    if (!(dj > 0.0) && !bad) bad = j + 1;
-                }
-            }
+            if (tc == jj) chol_pivot<RS, CS>(S, xr, xi, kj, tr, hb, j);
             __syncthreads();
-            const double id = sid[b];
+            const double id = S.s1[b];
+            // row j of the partial inverse scaled by 1/d (columns <= j; 1/d at j) to its half's LDS row
 #pragma unroll
             for (int i = 0; i < RS; ++i)
-                if (js == i && tr == jl) {
+                if (js == i) {
+                    if (tr == jl) {
 #pragma unroll
-                    for (int k = 0; k < CS; ++k) {
-                        const int col = tc + 16 * k;
-                        double nr = 0.0, ni = 0.0;
-                        if (col < j) {
-                            nr = xr[i][k] * id;
-                            ni = xi[i][k] * id;
-                        } else if (col == j) {
-                            nr = id;
+                        for (int k = 0; k < CS; ++k) {
+                            if (!low_slot(i, k)) continue;
+                            const int col = tc + 16 * k;
+                            S.rowr[tc][k] = col < j ? xr[i][k] * id : (col == j ? id : 0.0);
+                            S.rowi[tc][k] = col < j ? xi[i][k] * id : 0.0;
                         }
-                        xr[i][k] = nr;
-                        xi[i][k] = ni;
-                        rowr[tc][k] = nr;
-                        rowi[tc][k] = ni;
                     }
                 }
             __builtin_amdgcn_wave_barrier();
@@ -308,26 +339,16 @@ __global__ __launch_bounds__(IT) void k_cholinv(CholInvArgs a) {
             double dr[RS], di[RS];
 #pragma unroll
             for (int i = 0; i < RS; ++i) {
-                dr[i] = colr[b][tr + 32 * i];
-                di[i] = coli[b][tr + 32 * i];
+                dr[i] = S.colr[b][tr + 32 * i];
+                di[i] = S.coli[b][tr + 32 * i];
             }
+            const int i0 = (j + 1) >> 5;   // row slots holding rows > j
 #pragma unroll
             for (int k = 0; k < CS; ++k) {
                 const int col = tc + 16 * k;
-                double ur, ui;
-                if (col > j) {
-                    ur = colr[b][col];
-                    ui = -coli[b][col];
-                } else {
-                    ur = rowr[tc][k];
-                    ui = rowi[tc][k];
-                }
-#pragma unroll
-                for (int i = 0; i < RS; ++i)
-                    if (32 * i + 31 > j) {   // rows below j only (the multiplier is 0 elsewhere)
-                        xr[i][k] = fma(-dr[i], ur, fma(di[i], ui, xr[i][k]));
-                        xi[i][k] = fma(-dr[i], ui, fma(-di[i], ur, xi[i][k]));
-                    }
+                const double ur = col > j ? S.colr[b][col] : S.rowr[tc][k];
+                const double ui = col > j ? -S.coli[b][col] : S.rowi[tc][k];
+                upd_slot<RS, CS, true>(xr, xi, dr, di, k, ur, ui, i0);
             }
             __builtin_amdgcn_wave_barrier();
         }
@@ -336,28 +357,33 @@ __global__ __launch_bounds__(IT) void k_cholinv(CholInvArgs a) {
 #pragma unroll
     for (int i = 0; i < RS; ++i) {
         const int r = tr + 32 * i;
+        if (r < n) {
+            const double sc = S.ir[r];   // row r was left unscaled: 1/d_r
 #pragma unroll
-        for (int k = 0; k < CS; ++k) {
-            const int col = tc + 16 * k;
-            if (r < n && col < n) {
-                const size_t o = (size_t)r + (size_t)col * X.ld;
-                X.re[o] = r >= col ? xr[i][k] : 0.0;
-                X.im[o] = r >= col ? xi[i][k] : 0.0;
+            for (int k = 0; k < CS; ++k) {
+                const int col = tc + 16 * k;
+                if (col < n) {
+                    const size_t o = (size_t)r + (size_t)col * X.ld;
+                    const bool low = low_slot(i, k) && r >= col;
+                    X.re[o] = low ? (r == col ? sc : xr[i][k] * sc) : 0.0;
+                    X.im[o] = low && r != col ? xi[i][k] * sc : 0.0;
+                }
             }
         }
     }
-    if (a.logdiag) {
-        const double s = wave_sum(lsum);
-        if ((tid & 63) == 0) red[tid >> 6] = s;
-        __syncthreads();
-        if (tid == 0) {
-            double t = 0.0;
-#pragma unroll
-            for (int w = 0; w < IT / 64; ++w) t += red[w];
-            a.logdiag[c] += t;
-        }
+    if (tid == 0) S.bad = 0x7fffffff;
+    __syncthreads();
+    double l = 0.0;
+    for (int j = tid; j < n; j += IT) {
+        const double dj = S.pd[j];
+        l += log(sqrt(fmax(dj, 1e-300)));
+        if (!(dj > 0.0)) atomicMin(&S.bad, j + 1);
     }
-    if (tid == 0 && bad && a.info && !a.info[c]) a.info[c] = bad;
+    const double ls = block_sum<RS, CS>(S, l, tid);
+    if (tid == 0) {
+        if (a.logdiag) a.logdiag[c] += ls;
+        if (S.bad != 0x7fffffff && a.info && !a.info[c]) a.info[c] = S.bad;
+    }
 }
 
 }  // namespace
