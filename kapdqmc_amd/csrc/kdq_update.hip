@@ -24,15 +24,16 @@ namespace {
 constexpr int KB = 64;   // maximum block size
 constexpr int LP = KB + 1;
 
-// Layout sized to the block: KBT in {8, 16, 32, 64} (smallest >= k), NTH = min(1024, KBT²)
-// threads, thread (r = tid % KBT, cg = tid / KBT) owns row r, columns cg·CPT .. cg·CPT+CPT-1
-// (CPT = KBT²/NTH) of both the working submatrix and W. The accept path is bound by VALU
-// issue, so no wave works on padding: at k = 32 each of 1024 threads owns one entry.
+// Layout sized to the block: KBT in {8, 16, 32, 64} (smallest >= k), NTH threads (KBT² up to
+// 1024; 256 at KBT = 32), thread (r = tid % KBT, cg = tid / KBT) owns row r, columns
+// cg·CPT .. cg·CPT+CPT-1 (CPT = KBT²/NTH) of both the working submatrix and W. No wave works on
+// padding. At k = 32, 256 threads (4 entries each) decide as fast as 1024 and leave most of
+// their compute unit to the flush kernels running beside them in the pipelined sweep (345 vs
+// 348 ms per sweep of the 64-chain N = 288 batch, same box; 512 threads: no different).
 // Everything a proposal needs except the current diagonal (Δ, γ'/γ, Δ/(1+Δ), the uniform,
 // the new value) is computed for all k sites in parallel before the sequential loop.
-template <int KBT>
-__global__ __launch_bounds__(KBT >= 32 ? 1024 : KBT * KBT) void k_decide_tu(DecideArgs a) {
-    constexpr int NTH = KBT >= 32 ? 1024 : KBT * KBT;
+template <int KBT, int NTH = (KBT >= 32 ? 1024 : KBT * KBT)>
+__global__ __launch_bounds__(NTH) void k_decide_tu(DecideArgs a) {
     constexpr int CPT = KBT * KBT / NTH;
     const int c = blockIdx.x;
     const ChainDev &ch = a.chains[c];
@@ -884,7 +885,7 @@ hipError_t launch_decide_tu(hipStream_t st, int nchains, const DecideArgs &a) {
     if (a.kb < 1 || a.kb > KB || a.slot < 0 || a.slot > 1) return hipErrorInvalidValue;
     if (a.kb <= 8) hipLaunchKernelGGL(k_decide_tu<8>, dim3(nchains), dim3(64), 0, st, a);
     else if (a.kb <= 16) hipLaunchKernelGGL(k_decide_tu<16>, dim3(nchains), dim3(256), 0, st, a);
-    else if (a.kb <= 32) hipLaunchKernelGGL(k_decide_tu<32>, dim3(nchains), dim3(1024), 0, st, a);
+    else if (a.kb <= 32) hipLaunchKernelGGL((k_decide_tu<32, 256>), dim3(nchains), dim3(256), 0, st, a);
     else hipLaunchKernelGGL(k_decide_tu<64>, dim3(nchains), dim3(1024), 0, st, a);
     return hipGetLastError();
 }
