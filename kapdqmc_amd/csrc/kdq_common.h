@@ -49,6 +49,7 @@ struct ChainDev {
     // local-update scratch
     int *acc_cnt;          // [2]: accepted count of a block, slot by block parity (t-U sweep)
     int *acc_idx;          // [2][64]: accepted sites of a block, by slot
+    int *pw_fld;           // [64]: field values of a prewrapped block 0 (old | new << 8 | accepted << 16), written back by its flush
     double *W;             // [2 slots][2 planes (re, im)] of 64 x 64, ld 64
     double *counters;      // [3]: proposals, accepted, near ties (|u - p| < KDQ_NEAR_TIE)
     double *meas;          // [KDQ_MEAS_SLOTS]: accumulated observables, count in the last slot

@@ -48,7 +48,7 @@ struct ChainHost {
     int *ex = nullptr, *ey = nullptr, *inc = nullptr;
     std::vector<double *> UR, ULH;
     uint8_t *s;
-    int *acc_cnt, *acc_idx, *info;
+    int *acc_cnt, *acc_idx, *info, *pw_fld;
     double *W, *counters, *meas, *logdet;
 };
 
@@ -91,6 +91,9 @@ struct kdq_handle_s {
     bool qsplit = true;                        // t-U captured sweeps: that second pass beside get_G!
     bool maside = true;                        // t-U sweeps: a measurement's products and tiles on aux
     bool ginv = true;                          // get_G!: register Gauss-Jordan inverse + GEMM (Np <= 144)
+    bool prewrap = true;                       // pipelined t-U layers: block 0 decided beside the wrap's 2nd product
+    bool pw_next = false;                      // the next layer's block 0 was launched with the wrap before it
+    double *d_pwpart = nullptr;                // prewrap partial products: [chain][N/32][2][32][32]
     bool mpending = false;                     // such a measurement not yet joined
     hipEvent_t ev_mfork = nullptr, ev_mdone = nullptr;
     hipEvent_t ev_q1 = nullptr, ev_q2 = nullptr;
@@ -940,7 +943,7 @@ int defer_flush(kdq_handle h) {
 //   main:       prep_0 apply_0 prep_1 apply_1 ... prep_b waits for decide_b
 // so decide_{b+1} is issued while prep_b runs and shares the device with apply_b. Accepted
 // counts, sites and W alternate between two slots by block parity.
-int update_layer_pipe(kdq_handle h, int lt, int dir, uint32_t sweep) {
+int update_layer_pipe(kdq_handle h, int lt, int dir, uint32_t sweep, bool prewrapped = false) {
     const int kb = h->delay;
     const bool mx = h->mixed;
     auto decide = [&](int s0, int slot, bool pre, const Opnd *Gb, const OpndF *Gbf) -> hipError_t {
@@ -955,11 +958,14 @@ int update_layer_pipe(kdq_handle h, int lt, int dir, uint32_t sweep) {
         return launch_decide_tu(h->side, h->nch, a);
     };
     const int nb = (h->max_ns + kb - 1) / kb;
-    // G_0 (this layer's G after the wrap) is ready
+    // G_0 (this layer's G after the wrap) is ready; block 0 was already decided beside the
+    // wrap's second product when prewrapped (wrap_prewrapped)
     HIPCHK(hipEventRecord(h->ev_app, h->stream));
     HIPCHK(hipStreamWaitEvent(h->side, h->ev_app, 0));
-    HIPCHK(decide(0, 0, false, h->t_G[h->cur], mx ? h->t_G32[h->cur32] : nullptr));
-    HIPCHK(hipEventRecord(h->ev_dec[0], h->side));
+    if (!prewrapped) {
+        HIPCHK(decide(0, 0, false, h->t_G[h->cur], mx ? h->t_G32[h->cur32] : nullptr));
+        HIPCHK(hipEventRecord(h->ev_dec[0], h->side));
+    }
     for (int b = 0; b < nb; ++b) {
         const int slot = b & 1;
         // G_b, the input of this block's flush
@@ -981,6 +987,10 @@ int update_layer_pipe(kdq_handle h, int lt, int dir, uint32_t sweep) {
             FlushArgs f{h->d_chains, h->t_G[h->cur], h->t_G[1 - h->cur], kb, 0, nullptr, nullptr, h->t_FZ, h->t_FC,
                         slot};
             f.one_launch = h->flush1;
+            if (prewrapped && b == 0) {   // block 0's accepted fields, held back while the wrap read them
+                f.commit_lt = lt;
+                f.commit_s0 = 0;
+            }
             HIPCHK(launch_flush(h->stream, h->nch, h->max_ns, f));
             h->cur ^= 1;
         }
@@ -989,14 +999,61 @@ int update_layer_pipe(kdq_handle h, int lt, int dir, uint32_t sweep) {
     return KDQ_OK;
 }
 
+// pipelined only in captured sweeps: with host launches the cross-stream events cost more
+// than the overlap gains (measured: 705 against 672 ms per sweep, 64 chains at N = 288)
+bool pipe_on(kdq_handle h) {
+    const bool defer = h->defer && !h->scee;
+    return h->kind == KDQ_MODEL_TU && h->pipe && h->sweep_dev && !defer && !h->scee && !h->prof_mask &&
+           h->max_ns > h->delay;
+}
+
+bool prewrap_on(kdq_handle h) {
+    return pipe_on(h) && h->prewrap && !h->mixed && !h->flush1 && h->delay <= 32 && h->d_pwpart;
+}
+
+// The wrap of slice lw (WrapKV!, "B"; forward before layer lt = lw, backward after layer lw and
+// before layer lt = lw - 1) as its two products with block 0 of layer lt decided in between, on
+// the side stream: its G[S,S] of the wrapped G comes from the first product T and the dense
+// factor of the second (k_decide_tu, prewrap), so the decisions overlap the second product
+// instead of following the whole wrap. The layer is then run with update_layer_pipe(...,
+// prewrapped = true).
+int wrap_prewrapped(kdq_handle h, int lw, bool forward, int lt, int dir, uint32_t sweep) {
+    Opnd *Gt = h->t_G[h->cur], *Tt = h->t_T;
+    const DiagSpec none{};
+    DecideArgs a{};
+    a.chains = h->d_chains;
+    a.G = Gt;
+    a.lt = lt; a.dir = dir; a.s0 = 0; a.kb = h->delay; a.sweep = sweep;
+    a.sweep_dev = h->sweep_dev;
+    a.slot = 0;
+    a.prewrap = 1;
+    a.T = Tt;
+    if (forward) {   // T = eK G, then G = D T eK⁻¹ D⁻¹
+        if (int r = kin_left(h, 0, 0, Gt, Tt, DIM_NS, none, none, none, KDQ_PROF_WRAP)) return r;
+        a.E = h->t_eKinv;
+        a.pw_line = hs_diag(h, lw, +1);
+        a.pw_out = hs_diag(h, lw, -1);
+    } else {         // T = eK⁻¹ (D' G) D'⁻¹, then G = T eK
+        if (int r = kin_left(h, 1, 0, Gt, Tt, DIM_NS, hs_diag(h, lw, -1), none, hs_diag(h, lw, +1), KDQ_PROF_WRAP))
+            return r;
+        a.E = h->t_eK;
+    }
+    a.pw_part = h->d_pwpart;
+    a.pw_nkc = (h->max_ns + 31) / 32;
+    HIPCHK(hipEventRecord(h->ev_app, h->stream));
+    HIPCHK(hipStreamWaitEvent(h->side, h->ev_app, 0));
+    HIPCHK(launch_prewrap_part(h->side, h->nch, a));
+    HIPCHK(launch_decide_tu(h->side, h->nch, a));
+    HIPCHK(hipEventRecord(h->ev_dec[0], h->side));
+    if (forward) return kin_right(h, 1, Tt, Gt, none, hs_diag(h, lw, -1), hs_diag(h, lw, +1), KDQ_PROF_WRAP);
+    return kin_right(h, 0, Tt, Gt, none, none, none, KDQ_PROF_WRAP);
+}
+
 int update_layer(kdq_handle h, int lt, int dir, uint32_t sweep) {
     if (h->kind == KDQ_MODEL_TUV) return update_family_r2(h, lt, dir, sweep, 0);
     const int kb = h->delay;
     const bool defer = h->defer && !h->scee;
-    // pipelined only in captured sweeps: with host launches the cross-stream events cost more
-    // than the overlap gains (measured: 705 against 672 ms per sweep, 64 chains at N = 288)
-    if (h->kind == KDQ_MODEL_TU && h->pipe && h->sweep_dev && !defer && !h->scee && !h->prof_mask && h->max_ns > kb)
-        return update_layer_pipe(h, lt, dir, sweep);
+    if (pipe_on(h)) return update_layer_pipe(h, lt, dir, sweep);
     int blk = 0;
     for (int s0 = 0; s0 < h->max_ns; s0 += kb, ++blk) {
         DecideArgs a{};
@@ -1366,6 +1423,7 @@ int node_refresh(kdq_handle h, int idx, Opnd *UR = nullptr, Opnd *ULH = nullptr)
 // in the reference's windows) for every chain. Only launches: no host reads, no
 // synchronisation, no memset, so the sequence can be captured into a graph.
 int sweep_body(kdq_handle h, int record, uint32_t sweep) {
+    h->pw_next = false;
     const int NN = h->nn, nt = h->nt, thidx = NN / 2 + 1;
     const bool tv = h->kind == KDQ_MODEL_TV;
     (void)NN;
@@ -1376,8 +1434,13 @@ int sweep_body(kdq_handle h, int record, uint32_t sweep) {
         if (tv) {
             if (int r = slice_tv_forward(h, h->t_G[h->cur], h->t_T, lt, true, sweep, KDQ_PROF_WRAP)) return r;
         } else {
-            if (int r = h->mixed ? wrap_B_f32(h, lt, true) : wrap_B(h, h->t_G[h->cur], h->t_T, lt, true)) return r;
-            if (int r = update_layer(h, lt, 0, sweep)) return r;
+            if (prewrap_on(h)) {
+                if (int r = wrap_prewrapped(h, lt, true, lt, 0, sweep)) return r;
+                if (int r = update_layer_pipe(h, lt, 0, sweep, true)) return r;
+            } else {
+                if (int r = h->mixed ? wrap_B_f32(h, lt, true) : wrap_B(h, h->t_G[h->cur], h->t_T, lt, true)) return r;
+                if (int r = update_layer(h, lt, 0, sweep)) return r;
+            }
             if (aside)
                 if (int r = chain_step(h, lt, true)) return r;
         }
@@ -1408,12 +1471,23 @@ int sweep_body(kdq_handle h, int record, uint32_t sweep) {
             if (record && in_window(h, thidx, false))
                 if (int r = measure(h, lt - 1)) return r;
         } else {
-            if (int r = update_layer(h, lt, 1, sweep)) return r;
+            if (h->pw_next) {
+                h->pw_next = false;
+                if (int r = update_layer_pipe(h, lt, 1, sweep, true)) return r;
+            } else if (int r = update_layer(h, lt, 1, sweep)) {
+                return r;
+            }
             if (aside)
                 if (int r = chain_step(h, lt, false)) return r;
             if (record && in_window(h, thidx, false))
                 if (int r = measure_sweep(h, lt, false)) return r;
-            if (int r = h->mixed ? wrap_B_f32(h, lt, false) : wrap_B(h, h->t_G[h->cur], h->t_T, lt, false)) return r;
+            // the next layer's block 0 beside this wrap unless a node replaces G in between
+            if (prewrap_on(h) && lt > 1 && !is_node(h, lt - 1)) {
+                if (int r = wrap_prewrapped(h, lt, false, lt - 1, 1, sweep)) return r;
+                h->pw_next = true;
+            } else if (int r = h->mixed ? wrap_B_f32(h, lt, false) : wrap_B(h, h->t_G[h->cur], h->t_T, lt, false)) {
+                return r;
+            }
         }
         if (is_node(h, lt - 1)) {
             h->idx -= 1;
@@ -1858,6 +1932,7 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
         }
         ok &= dalloc(C, &C.acc_cnt, 2) == hipSuccess;
         ok &= dalloc(C, &C.acc_idx, 2 * 64) == hipSuccess;
+        ok &= dalloc(C, &C.pw_fld, 64) == hipSuccess;
         ok &= dalloc(C, &C.info, 1) == hipSuccess;
         ok &= dalloc(C, &C.W, 2 * 2 * 64 * 64) == hipSuccess;
         ok &= dalloc(C, &C.FZ, 2 * 64 * (size_t)n) == hipSuccess;
@@ -1984,7 +2059,7 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
         d.seed = runs[c].seed;
         d.chain_id = runs[c].chain;
         d.stream = streams ? streams[c] : 0;
-        d.acc_cnt = C.acc_cnt; d.acc_idx = C.acc_idx; d.W = C.W;
+        d.acc_cnt = C.acc_cnt; d.acc_idx = C.acc_idx; d.pw_fld = C.pw_fld; d.W = C.W;
         d.rk = C.rk; d.rank = C.rank;
         d.counters = C.counters; d.meas = C.meas;
         d.kind = m.kind;
@@ -2120,6 +2195,12 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
         h->d_lw = (double *)q6;
         h->d_poison = (int *)q7;
         h->d_wraperr = (double *)q8;
+        if (h->kind == KDQ_MODEL_TU) {   // prewrap partial products (t-U pipelined layers)
+            void *q10 = nullptr;
+            const size_t pw = (size_t)nchains * ((h->max_ns + 31) / 32) * 2048;
+            if (hipMalloc(&q10, sizeof(double) * pw) == hipSuccess) h->dev_misc.push_back(q10); else r = 1;
+            h->d_pwpart = (double *)q10;
+        }
         if (!r) {
             hipMemset(q2, 0, sizeof(int) * nchains);
             hipMemset(q3, 0, sizeof(int) * nchains);
@@ -2632,6 +2713,9 @@ int kdq_set_option(kdq_handle h, int32_t key, int32_t value) {
             break;
         case KDQ_OPT_QINV:
             h->qinv = value != 0;
+            break;
+        case KDQ_OPT_PREWRAP:
+            h->prewrap = value != 0;
             break;
         case KDQ_OPT_QSPLIT:
             h->qsplit = value != 0;

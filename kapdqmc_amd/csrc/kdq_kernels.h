@@ -66,8 +66,19 @@ struct DecideArgs {
     // 1 - slot) has not been applied to G yet (its flush runs concurrently); the block reads
     // G[S,S] + G[S,A] W G[A,S]
     int slot, pre;
+    // prewrap (with pre = 0, FP64): G is not read; the block's G[S,S] of the wrapped G is formed
+    // as line ⊙ (T E) ⊙ out on S x S from the wrap's first product T and the dense kinetic
+    // factor E of its second (real, or complex when E.im is set), so the decisions can run
+    // beside that second product
+    int prewrap, pw_nkc;         // pw_nkc: chunks of 32 of the contraction index (N / 32, rounded up)
+    const Opnd *T, *E;
+    DiagSpec pw_line, pw_out;
+    double *pw_part;             // partial products per chain and chunk: [c][chunk][re, im][32][32]
 };
 hipError_t launch_decide_tu(hipStream_t st, int nchains, const DecideArgs &a);
+// prewrap: the partial products T[S, K] E[K, S] per chunk K of 32 (launch before the decisions,
+// same stream)
+hipError_t launch_prewrap_part(hipStream_t st, int nchains, const DecideArgs &a);
 
 // Deferred flush, per block after its decisions: with r = rk[blk], A the accepted set and W
 // its factor, append X = G[:,A] (columns r.. of U) and Y = W (E_A - G[A,:]) (rows r.. of V),
@@ -105,6 +116,8 @@ struct FlushArgsT {
     const typename OpndOf<T>::type *Cs;     // scratch: N x 64 per chain (two-launch form)
     int slot;           // the rank, accepted sites and W of slot `slot`
     int one_launch;     // 1: the strip form
+    int commit_lt, commit_s0;   // > 0: write back the field values a prewrapped block 0 left in
+                                // chains[c].pw_fld (slice commit_lt, sites commit_s0..), two-launch form
 };
 typedef FlushArgsT<double> FlushArgs;
 typedef FlushArgsT<float> FlushArgsF;

@@ -62,6 +62,34 @@ __global__ __launch_bounds__(NTH) void k_decide_tu(DecideArgs a) {
     const int lt = a.lt;
     const int r = tid % KBT, c0 = (tid / KBT) * CPT;
     double wr[CPT], wi[CPT], Wr[CPT], Wi[CPT];
+    if (a.prewrap) {
+        // the block of the wrapped G: line ⊙ (T[S,:] E[:,S]) ⊙ out, the product summed over the
+        // chunk partials of k_pw_part in chunk order
+        const double *P = a.pw_part + (size_t)c * a.pw_nkc * 2048;
+        double lr = 1.0, li = 0.0;
+        if (r < kb) diag_value(ch, a.pw_line, s0 + r, lr, li);
+#pragma unroll
+        for (int q = 0; q < CPT; ++q) {
+            const int cc = c0 + q;
+            double sr = 0.0, si = 0.0;
+            if (r < 32 && cc < 32)
+                for (int kc = 0; kc < a.pw_nkc; ++kc) {
+                    sr += P[(size_t)kc * 2048 + r * 32 + cc];
+                    si += P[(size_t)kc * 2048 + 1024 + r * 32 + cc];
+                }
+            double orr = 1.0, oi = 0.0;
+            if (cc < kb) diag_value(ch, a.pw_out, s0 + cc, orr, oi);
+            double tr_, ti_;
+            cmul(lr, li, sr, si, tr_, ti_);
+            double vr, vi;
+            cmul(tr_, ti_, orr, oi, vr, vi);
+            const bool ok = r < kb && cc < kb;
+            wr[q] = ok ? vr : 0.0;
+            wi[q] = ok ? vi : 0.0;
+            Wr[q] = 0.0;
+            Wi[q] = 0.0;
+        }
+    } else
 #pragma unroll
     for (int q = 0; q < CPT; ++q) {
         const int cc = c0 + q;
@@ -346,9 +374,63 @@ __global__ __launch_bounds__(NTH) void k_decide_tu(DecideArgs a) {
             }
         }
     }
-    if (tid < kb) {
+    if (a.prewrap) {   // the wrap's second product may still read this slice's fields
+        if (tid < KBT) ch.pw_fld[tid] = tid < kb ? sv[tid] : 0;
+    } else if (tid < kb) {
         const int v = sv[tid];
         ch.s[(size_t)(s0 + tid) + (size_t)(lt - 1) * ch.n_elem] = (uint8_t)((v >> 16) ? (v >> 8) & 0xff : v & 0xff);
+    }
+}
+
+// prewrap, first part: per chain (blockIdx.y) and chunk K = [32 kc, 32 kc + 32) of the contraction
+// index (blockIdx.x), P[kc] = T[S, K] E[K, S] for the block's sites S = s0 .. s0 + 31 (rows
+// and columns beyond the block are 0), both operands staged in LDS with one global round trip;
+// thread (r = tid % 32, c0 = 4 (tid / 32)) sums four entries over k in order.
+__global__ __launch_bounds__(256) void k_pw_part(DecideArgs a) {
+    const int c = blockIdx.y, kc = blockIdx.x;
+    const ChainDev &ch = a.chains[c];
+    const int n = ch.ns, s0 = a.s0, kb = min(a.kb, n - s0), k0 = 32 * kc;
+    const int kn = min(32, n - k0);
+    double *P = a.pw_part + ((size_t)c * a.pw_nkc + kc) * 2048;
+    const int tid = threadIdx.x, r = tid % 32, c0 = (tid / 32) * 4;
+    if (kn <= 0 || kb <= 0) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            P[r * 32 + c0 + q] = 0.0;
+            P[1024 + r * 32 + c0 + q] = 0.0;
+        }
+        return;
+    }
+    const Opnd T = a.T[c], E = a.E[c];
+    const bool ecx = E.im != nullptr;
+    __shared__ double Tr[32][33], Ti[32][33], Er[32][33], Ei[32][33];
+    for (int e = tid; e < 32 * 32; e += 256) {
+        const int i = e % 32, k = e / 32;   // consecutive threads: consecutive rows of a column of T
+        const bool ok = i < kb && k < kn;
+        const size_t o = (size_t)(s0 + i) + (size_t)(k0 + k) * T.ld;
+        Tr[i][k] = ok ? T.re[o] : 0.0;
+        Ti[i][k] = ok ? T.im[o] : 0.0;
+        const int kk = e % 32, j = e / 32;  // consecutive entries of a column of E
+        const bool ok2 = j < kb && kk < kn;
+        const size_t o2 = (size_t)(k0 + kk) + (size_t)(s0 + j) * E.ld;
+        Er[kk][j] = ok2 ? E.re[o2] : 0.0;
+        Ei[kk][j] = ok2 && ecx ? E.im[o2] : 0.0;
+    }
+    __syncthreads();
+    double sr[4] = {0.0, 0.0, 0.0, 0.0}, si[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int k = 0; k < kn; ++k) {
+        const double tr_ = Tr[r][k], ti_ = Ti[r][k];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const double er = Er[k][c0 + q], ei = Ei[k][c0 + q];
+            sr[q] = fma(tr_, er, fma(-ti_, ei, sr[q]));
+            si[q] = fma(tr_, ei, fma(ti_, er, si[q]));
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        P[r * 32 + c0 + q] = sr[q];
+        P[1024 + r * 32 + c0 + q] = si[q];
     }
 }
 
@@ -367,6 +449,10 @@ __global__ __launch_bounds__(256) void k_flush_prep(FlushArgsT<T> f) {
     const ChainDev &ch = f.chains[c];
     const int n = ch.ns, na = ch.acc_cnt[f.slot];
     const int j0 = blockIdx.x * PC;
+    if (f.commit_lt > 0 && blockIdx.x == 0 && threadIdx.x < KB) {   // accepted fields of a prewrapped block 0
+        const int v = ch.pw_fld[threadIdx.x], s = f.commit_s0 + threadIdx.x;
+        if ((v >> 16) && s < n) ch.s[(size_t)s + (size_t)(f.commit_lt - 1) * ch.n_elem] = (uint8_t)((v >> 8) & 0xff);
+    }
     if (na == 0 || j0 >= n) return;   // rank 0: the GEMM copies Xin
     const auto Gi = f.Gin[c];
     const auto Ci = f.Cin ? f.Cin[c] : Gi, Ri = f.Rin ? f.Rin[c] : Gi;
@@ -926,6 +1012,12 @@ hipError_t flush_t(hipStream_t st, int nchains, int max_ns, const FlushArgsT<T> 
     } else {
         hipLaunchKernelGGL(k_flush_apply<T>, dim3(tx, ty, nchains), dim3(256), 0, st, f, nchains, tx, ty, 0);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_prewrap_part(hipStream_t st, int nchains, const DecideArgs &a) {
+    if (!a.pw_part || a.pw_nkc < 1 || a.kb > 32) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_pw_part, dim3(a.pw_nkc, nchains), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 

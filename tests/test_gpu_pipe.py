@@ -149,3 +149,28 @@ def test_measurements_beside_the_sweep(precision):
         assert np.array_equal(f1[c], f0[c])
         assert np.array_equal(g1[c], g0[c])
     assert np.array_equal(r1, r0)
+
+
+@pytest.mark.parametrize("delay", [16, 32])
+def test_first_block_beside_the_wrap(delay):
+    # option "prewrap": each layer's block 0 decided between the wrap's two products from the
+    # first product and the dense kinetic factor, its accepted fields written back by its flush;
+    # the block's G[S,S] rounds differently, so G agrees to rounding and fields exactly
+    from kapdqmc_amd import dqmc
+    print(f"case: block 0 beside the wrap's second product, delay {delay}", flush=True)
+    models = [tu(4, theta=2.0), tu(6, U=2.0, theta=2.0), tu(3, U=6.0, theta=2.0, bs=3)]
+    res = []
+    for pw in (1, 0):
+        b = dqmc.DQMCBatch(models, [5, 6, 7], [0, 1, 2], delay=delay)
+        b.set_option("prewrap", pw)
+        b.initial_s()
+        rows = b.phy_update(3, True)
+        res.append(([b.get_fields(c) for c in range(b.n)], [b.get_G(c) for c in range(b.n)], rows,
+                    b.near_ties()))
+        b.close()
+    (f1, g1, r1, n1), (f0, g0, r0, n0) = res
+    assert np.array_equal(n1, n0)
+    for c in range(len(models)):
+        assert np.array_equal(f1[c], f0[c]), c
+        assert rel(g1[c], g0[c]) < 1e-10, (c, rel(g1[c], g0[c]))
+    assert np.allclose(r1, r0, rtol=1e-9, atol=1e-11)
