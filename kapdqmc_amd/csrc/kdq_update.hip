@@ -110,8 +110,9 @@ __global__ __launch_bounds__(NTH) void k_decide_tu(DecideArgs a) {
     // a rank-rk product not yet applied to G (deferred flush, or the previous block's flush in
     // the pipelined sweep): the block works on G[S,S] - U[S,:] V[:,S]
     if ((a.deferred && a.blk > 0) || a.pre) {
-        constexpr int DCH = 16;
+        constexpr int DCH = KBT >= 32 ? 32 : 16;
         __shared__ double Usr[KBT][DCH + 1], Usi[KBT][DCH + 1], Vsr[DCH][KBT + 1], Vsi[DCH][KBT + 1];
+        __shared__ double Upr[KBT][DCH + 1], Upi[KBT][DCH + 1];   // pipelined: G[S, A] chunk 0, prefetched
         const int ps = 1 - a.slot;
         const int rk = a.pre ? ch.acc_cnt[ps] : ch.rk[a.blk];
         const int *Ap = ch.acc_idx + ps * KB;
@@ -153,6 +154,14 @@ __global__ __launch_bounds__(NTH) void k_decide_tu(DecideArgs a) {
                     Vsr[u][j] = yr;
                     Vsi[u][j] = yi;
                 }
+                if (u0 == 0)   // the first chunk of U = G[S, A] for the second phase, in the same round trip
+                    for (int e = tid; e < KBT * DCH; e += NTH) {
+                        const int i = e % KBT, t = e / KBT;
+                        double ur = 0.0, ui = 0.0;
+                        if (i < kb && t < min(DCH, rk)) g_at(s0 + i, Ap[t], ur, ui);
+                        Upr[i][t] = ur;
+                        Upi[i][t] = ui;
+                    }
                 __syncthreads();
 #pragma unroll
                 for (int q = 0; q < CPT; ++q)
@@ -170,6 +179,9 @@ __global__ __launch_bounds__(NTH) void k_decide_tu(DecideArgs a) {
         // 1 - slot), so U = G[S, A] and V = -W G[A, S] from phase 1 (E_A vanishes on S)
         for (int t0 = 0; t0 < rk; t0 += DCH) {
             const int tc = min(DCH, rk - t0);
+            const bool pref = a.pre && t0 == 0;   // staged with the first phase
+            double (*UR)[DCH + 1] = pref ? Upr : Usr, (*UI)[DCH + 1] = pref ? Upi : Usi;
+            if (!pref)
             for (int e = tid; e < KBT * DCH; e += NTH) {
                 const int i = e % KBT, t = e / KBT;
                 double ur = 0.0, ui = 0.0;
@@ -226,8 +238,8 @@ __global__ __launch_bounds__(NTH) void k_decide_tu(DecideArgs a) {
                 const int cc = c0 + q;
                 double sr = 0.0, si = 0.0;
                 for (int t = 0; t < tc; ++t) {
-                    sr += Usr[r][t] * Vsr[t][cc] - Usi[r][t] * Vsi[t][cc];
-                    si += Usr[r][t] * Vsi[t][cc] + Usi[r][t] * Vsr[t][cc];
+                    sr += UR[r][t] * Vsr[t][cc] - UI[r][t] * Vsi[t][cc];
+                    si += UR[r][t] * Vsi[t][cc] + UI[r][t] * Vsr[t][cc];
                 }
                 wr[q] -= sr;
                 wi[q] -= si;
@@ -455,8 +467,8 @@ __global__ __launch_bounds__(256) void k_flush_prep(FlushArgsT<T> f) {
     }
     if (na == 0 || j0 >= n) return;   // rank 0: the GEMM copies Xin
     const auto Gi = f.Gin[c];
-    const auto Ci = f.Cin ? f.Cin[c] : Gi, Ri = f.Rin ? f.Rin[c] : Gi;
-    const auto Z = f.Zs[c], Cg = f.Cs[c];
+    const auto Ri = f.Rin ? f.Rin[c] : Gi;
+    const auto Z = f.Zs[c];
     const T sgn = f.mode == 0 ? T(1) : T(-1);
     const T edge = f.mode == 0 ? T(1) : T(0);
     const int K = f.kmax;
@@ -480,9 +492,6 @@ __global__ __launch_bounds__(256) void k_flush_prep(FlushArgsT<T> f) {
             const size_t off = (size_t)aidx[u] + (size_t)j * Ri.ld;
             yr = (aidx[u] == j ? edge : T(0)) - sgn * Ri.re[off];
             yi = -sgn * Ri.im[off];
-            const size_t oc = (size_t)j + (size_t)aidx[u] * Ci.ld, og = (size_t)j + (size_t)u * Cg.ld;
-            Cg.re[og] = sgn * Ci.re[oc];
-            Cg.im[og] = sgn * Ci.im[oc];
         }
         Yr[u * PC + jj] = yr;
         Yi[u * PC + jj] = yi;
@@ -537,7 +546,10 @@ __global__ __launch_bounds__(256) void k_flush_apply(FlushArgsT<T> f, int nch, i
     const int i0 = by * AM, j0 = bx * AN;
     if (i0 >= n || j0 >= n) return;
     const int tid = threadIdx.x, ri = tid & 15, cg = tid >> 4;
-    const auto Gi = f.Gin[c], Go = f.Gout[c], Z = f.Zs[c], Cg = f.Cs[c];
+    const auto Gi = f.Gin[c], Go = f.Gout[c], Z = f.Zs[c];
+    const auto Ci = f.Cin ? f.Cin[c] : Gi;
+    const T sgn = f.mode == 0 ? T(1) : T(-1);
+    const int *aidx = ch.acc_idx + f.slot * KB;   // entries at or beyond the count hold valid indices
     __shared__ C2 Cs2[AK][AM];
     __shared__ C2 Zs2[AK][AN + 1];   // padded: the staging stores run down a column (t)
     constexpr int RA = AM / 16, RB = AN / 16;
@@ -560,12 +572,12 @@ __global__ __launch_bounds__(256) void k_flush_apply(FlushArgsT<T> f, int nch, i
         // scratch whatever it is, and entries at or beyond the count are zeroed by a select
         C2 cst[(AK * AM) / 256], zst[(AK * AN) / 256];
 #pragma unroll
-        for (int q = 0; q < (AK * AM) / 256; ++q) {
+        for (int q = 0; q < (AK * AM) / 256; ++q) {   // Cs = sgn · Cin[:, A], gathered here
             const int e = tid + 256 * q, r = e & (AM - 1), t = e / AM;
             const int ir = min(i0 + r, n - 1);
-            const size_t oc = (size_t)ir + (size_t)(t0 + t) * Cg.ld;
-            cst[q].x = Cg.re[oc];
-            cst[q].y = Cg.im[oc];
+            const size_t oc = (size_t)ir + (size_t)aidx[t0 + t] * Ci.ld;
+            cst[q].x = sgn * Ci.re[oc];
+            cst[q].y = sgn * Ci.im[oc];
         }
 #pragma unroll
         for (int q = 0; q < (AK * AN) / 256; ++q) {

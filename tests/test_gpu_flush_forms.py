@@ -37,6 +37,7 @@ def sweep(models, delay, form, precision="fp64", sweeps=2):
     b = dqmc.DQMCBatch(models, list(range(11, 11 + len(models))), list(range(len(models))), delay=delay,
                        precision=precision)
     b.set_option("flush", form)
+    b.set_option("prewrap", 0)   # the one-launch form has no prewrapped block 0
     b.initial_s()
     rows = b.phy_update(sweeps, True)
     out = ([b.get_fields(c) for c in range(b.n)], [b.get_G(c) for c in range(b.n)], rows)
