@@ -1,18 +1,24 @@
-// Inverse of the per-chain n x n overlap M = U_L U_R of get_G! (src/public/Gupdate.jl:139-157,
-// where the reference inverts it through LAPACK's LU) by Gauss-Jordan elimination with partial
-// pivoting, the whole matrix held in registers of one 512-thread workgroup per chain.
+// Register-resident factorisations of the stabilisation, one 512-thread workgroup per chain
+// with the whole n x n matrix (n <= 144) in registers:
+//   k_inv      the inverse of the overlap M = U_L U_R of get_G! (src/public/Gupdate.jl:139-157,
+//              where the reference inverts it through LAPACK's LU) by Gauss-Jordan elimination
+//              with partial pivoting; log|det M|;
+//   k_cholinv  the Cholesky factor of the Gram matrix UᴴU of a Cholesky-QR pass (the role of
+//              the QR at src/tU/phy_update.jl:41-47,81-87) and its inverse in the same pass.
 //
 // Layout: thread t is column class tc = t / 32 (0..15) and row class tr = t % 32; it holds
 // M[tr + 32 i][tc + 16 k] for i < RS, k < CS (n <= 32 RS, n <= 16 CS), so the 32 holders of a
 // column are one half of a wave64 and the pivot search is a shuffle butterfly without LDS.
 // Step j (column j): the half holding column j finds the pivot row p among the rows not yet
 // pivoted (largest |m|², first index on ties), publishes column j (0 at row p), p and 1/m_pj;
-// one barrier; then every thread normalises its part of row p if it holds it (row p lives in
-// one lane of each half, which passes it to its half through a per-half LDS row), sets its
-// column-j entries outside row p to 0 and applies the rank-1 update m_rc -= m_rj m'_pc. Rows
-// are not interchanged (the pivot order is kept instead): the registers then hold
-// M⁻¹[q(r)][p(c)] at (r, c), with p(j) the pivot row of step j and q its inverse, and are
-// stored that way. log|det M| = Σ log|pivot|. n³ complex multiply-adds, 2 waves per SIMD.
+// one barrier; the lane holding row p passes u = row p / m_pj to its half through a per-half
+// LDS row; the column owners set column j to 0 outside row p (1 at row p) and every thread
+// applies the rank-1 update m_rc -= m_rj u_c. Rows are not interchanged (the pivot order is
+// kept instead) and row p keeps the factor m_pj until the store: the registers then hold
+// m_pj M⁻¹[q(r)][p(c)] at (r, c) for the pivot row r = p(j), with p(j) the pivot row of step j
+// and q its inverse, and are stored scaled and permuted. log|det M| = Σ log|pivot|.
+// n³ complex multiply-adds, 2 waves per SIMD; every wave64 VALU instruction (FP64 FMA
+// included) takes 4 cycles, so no register is written under a per-lane condition on the pivot.
 #include "kdq_kernels.h"
 
 #include <utility>
