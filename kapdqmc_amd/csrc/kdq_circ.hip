@@ -237,14 +237,42 @@ KDQ_DEV void y_lambda_pass(double2 *buf, int nlines, int npad, const double (&wr
 
 KDQ_HD int lpw_of(int L) { return L >= 96 ? 1 : 96 / L; }
 
-template <int L>
+// complex product with a fixed fma order (the load, the store and the steps of a chained
+// product must round alike)
+KDQ_DEV double2 cmul_f(double2 v, double2 f) {
+    return make_double2(fma(v.x, f.x, -v.y * f.y), fma(v.x, f.y, v.y * f.x));
+}
+
+// the four values cis(±α η_v) of the `in`, `out` and `line` factors of step `step` (threads 32..43)
+KDQ_DEV void set_fac(const CircArgs &a, const ChainDev &ch, double2 (*f)[4], int step, int tid) {
+    if (tid >= 32 && tid < 44) {
+        const int w = (tid - 32) >> 2, v = (tid - 32) & 3;
+        const DiagSpec &d = w == 0 ? a.in : (w == 1 ? a.out : a.line);
+        double r = 1.0, i = 0.0;
+        if (d.kind == 1) {
+            const int lt = d.lt + step * a.dlt;
+            sincos(d.sign * ch.alpha[lt - 1] * ch.eta[v], &i, &r);
+            if (d.conj) i = -i;
+        }
+        f[w][v] = make_double2(r, i);
+    }
+}
+
+// the fields of a factor's slice at step `step`
+KDQ_DEV const uint8_t *step_fields(const CircArgs &a, const ChainDev &ch, const DiagSpec &d, int step) {
+    const int lt = d.lt > 0 ? d.lt + step * a.dlt : 1;
+    return ch.s + (size_t)(lt - 1) * ch.n_elem;
+}
+
+template <int L, bool MULTI>
 KDQ_DEV void circ_body(const CircArgs &a, const ChainDev &ch, int c, int grp, unsigned char *lds) {
     constexpr int NC = L * L, NS = 2 * NC, NPAD = (2 * L + 1) * L + 1, LPW = 96 / L;
     const int nl = dim_of(ch, a.nline_dim);
     const int line0 = grp * LPW;
     if (line0 >= nl) return;   // uniform per workgroup
     const int nlw = min(LPW, nl - line0);
-    const bool use_in = a.in.kind != 0, use_out = a.out.kind != 0;
+    const bool use_in = a.in.kind != 0, use_out = a.out.kind != 0, use_line = a.line.kind != 0;
+    const int nsteps = MULTI && a.nsteps > 1 ? a.nsteps : 1;
     double2 *buf = reinterpret_cast<double2 *>(lds);   // [LPW][NPAD]: p = 2(x + L y) + b at p + y
     double *lamt = reinterpret_cast<double *>(buf + LPW * NPAD);   // the chain's Λ table of E [k][a][b]
     const int tid = threadIdx.x;
@@ -253,81 +281,82 @@ KDQ_DEV void circ_body(const CircArgs &a, const ChainDev &ch, int c, int grp, un
         for (int e = tid; e < NC * 8; e += CT) lamt[e] = tab[e];
     }
     // the t-U HS factors take four values per slice (one per field value): cis(±α η_v) of the
-    // `in`, `out` and `line` factors, evaluated once per workgroup (kind 0 = 1)
-    __shared__ double2 tws[16];   // w[e] = e^{-2πi e/L}
-    __shared__ double2 fac[3][4];
+    // `in`, `out` and `line` factors of a step, evaluated once per workgroup and step (kind 0 = 1)
+    __shared__ double twr[16], twi[16];   // w[e] = e^{-2πi e/L}, read from LDS by the butterflies
+    __shared__ double2 fac[2][3][4];
     if (tid < L) {
         double sn, cs;
         sincospi(2.0 * tid / L, &sn, &cs);
-        tws[tid] = make_double2(cs, -sn);
-    } else if (tid >= 32 && tid < 44) {
-        const int w = (tid - 32) >> 2, v = (tid - 32) & 3;
-        const DiagSpec &d = w == 0 ? a.in : (w == 1 ? a.out : a.line);
-        double r = 1.0, i = 0.0;
-        if (d.kind == 1) {
-            sincos(d.sign * ch.alpha[d.lt - 1] * ch.eta[v], &i, &r);
-            if (d.conj) i = -i;
-        }
-        fac[w][v] = make_double2(r, i);
+        twr[tid] = cs;
+        twi[tid] = -sn;
     }
+    set_fac(a, ch, fac[0], 0, tid);
     __syncthreads();
-    double wr[L], wi[L];
-#pragma unroll
-    for (int e = 0; e < L; ++e) {
-        wr[e] = tws[e].x;
-        wi[e] = tws[e].y;
-    }
-    const uint8_t *sin_ = ch.s + (size_t)(a.in.lt > 0 ? a.in.lt - 1 : 0) * ch.n_elem;
-    const uint8_t *sout = ch.s + (size_t)(a.out.lt > 0 ? a.out.lt - 1 : 0) * ch.n_elem;
-    const uint8_t *sline = ch.s + (size_t)(a.line.lt > 0 ? a.line.lt - 1 : 0) * ch.n_elem;
+    const double(&wr)[L] = *reinterpret_cast<const double(*)[L]>(twr);
+    const double(&wi)[L] = *reinterpret_cast<const double(*)[L]>(twi);
     const Opnd X = a.X[c], Y = a.Y[c];
     // load (and scale by `in`), four loads of a thread in flight before their LDS stores
-    constexpr int NE = LPW * NS, UB = 4;
-    for (int e0 = tid; e0 < NE; e0 += UB * CT) {
-        double2 v[UB];
+    {
+        const uint8_t *sin_ = step_fields(a, ch, a.in, 0);
+        constexpr int NE = LPW * NS, UB = 4;
+        for (int e0 = tid; e0 < NE; e0 += UB * CT) {
+            double2 v[UB];
 #pragma unroll
-        for (int u = 0; u < UB; ++u) {
-            // columns: positions fastest (contiguous); rows: the lines of one column together
-            const int e = e0 + u * CT, l = a.rows ? e % LPW : e / NS, p = a.rows ? e / LPW : e % NS;
-            v[u] = make_double2(0.0, 0.0);
-            if (e < NE && l < nlw) {
-                const size_t o = a.rows ? (size_t)(line0 + l) + (size_t)p * X.ld : (size_t)p + (size_t)(line0 + l) * X.ld;
-                v[u] = make_double2(X.re[o], X.im[o]);
+            for (int u = 0; u < UB; ++u) {
+                // columns: positions fastest (contiguous); rows: the lines of one column together
+                const int e = e0 + u * CT, l = a.rows ? e % LPW : e / NS, p = a.rows ? e / LPW : e % NS;
+                v[u] = make_double2(0.0, 0.0);
+                if (e < NE && l < nlw) {
+                    const size_t o = a.rows ? (size_t)(line0 + l) + (size_t)p * X.ld : (size_t)p + (size_t)(line0 + l) * X.ld;
+                    v[u] = make_double2(X.re[o], X.im[o]);
+                }
             }
-        }
 #pragma unroll
-        for (int u = 0; u < UB; ++u) {
-            const int e = e0 + u * CT, l = a.rows ? e % LPW : e / NS, p = a.rows ? e / LPW : e % NS;
-            if (e >= NE) break;
-            double2 w = v[u];
-            if (use_in) {
-                const double2 f = fac[0][sin_[p] - 1];
-                w = make_double2(v[u].x * f.x - v[u].y * f.y, v[u].x * f.y + v[u].y * f.x);
+            for (int u = 0; u < UB; ++u) {
+                const int e = e0 + u * CT, l = a.rows ? e % LPW : e / NS, p = a.rows ? e / LPW : e % NS;
+                if (e >= NE) break;
+                const double2 w = use_in ? cmul_f(v[u], fac[0][0][sin_[p] - 1]) : v[u];
+                buf[l * NPAD + p + p / (2 * L)] = w;
             }
-            buf[l * NPAD + p + p / (2 * L)] = w;
         }
     }
     __syncthreads();
-    dft_pass<L, true, false>(buf, nlw, NPAD, wr, wi);
-    // forward y, the 2 x 2 blocks of E, Eᵀ or Eᴴ acting on a line (a right product X·E acts on
-    // the rows of X with Eᵀ), backward y
-    y_lambda_pass<L>(buf, nlw, NPAD, wr, wi, lamt, a.rows ? 1 : a.op);
-    dft_pass<L, false, false>(buf, nlw, NPAD, wr, wi);
-    // store (scaled by `out` and the line factor)
+    for (int st = 0; st < nsteps; ++st) {
+        const int cur = st & 1;
+        dft_pass<L, true, false>(buf, nlw, NPAD, wr, wi);
+        // forward y, the 2 x 2 blocks of E, Eᵀ or Eᴴ acting on a line (a right product X·E acts on
+        // the rows of X with Eᵀ), backward y
+        y_lambda_pass<L>(buf, nlw, NPAD, wr, wi, lamt, a.rows ? 1 : a.op);
+        dft_pass<L, false, false>(buf, nlw, NPAD, wr, wi);
+        if (MULTI && st + 1 < nsteps) {   // this step's `out` and `line`, the next step's `in`, in LDS
+            set_fac(a, ch, fac[cur ^ 1], st + 1, tid);
+            __syncthreads();
+            const uint8_t *so = step_fields(a, ch, a.out, st), *sl = step_fields(a, ch, a.line, st),
+                          *si = step_fields(a, ch, a.in, st + 1);
+            for (int e = tid; e < LPW * NS; e += CT) {
+                const int l = e / NS, p = e % NS;
+                if (l >= nlw) continue;
+                double2 v = buf[l * NPAD + p + p / (2 * L)];
+                if (use_out) v = cmul_f(v, fac[cur][1][so[p] - 1]);
+                v = cmul_f(v, use_line ? fac[cur][2][sl[line0 + l] - 1] : make_double2(1.0, 0.0));
+                if (use_in) v = cmul_f(v, fac[cur ^ 1][0][si[p] - 1]);
+                buf[l * NPAD + p + p / (2 * L)] = v;
+            }
+            __syncthreads();
+        }
+    }
+    // store (scaled by `out` and the line factor of the last step)
+    const int last = (nsteps - 1) & 1;
+    const uint8_t *sout = step_fields(a, ch, a.out, nsteps - 1), *sline = step_fields(a, ch, a.line, nsteps - 1);
     for (int e = tid; e < LPW * NS; e += CT) {
         const int l = a.rows ? e % LPW : e / NS, p = a.rows ? e / LPW : e % NS;
         if (l >= nlw) continue;
-        const double2 v = buf[l * NPAD + p + p / (2 * L)];
-        const double2 g = a.line.kind ? fac[2][sline[line0 + l] - 1] : make_double2(1.0, 0.0);
-        double tr = v.x, ti = v.y;
-        if (use_out) {
-            const double2 f = fac[1][sout[p] - 1];
-            tr = v.x * f.x - v.y * f.y;
-            ti = v.x * f.y + v.y * f.x;
-        }
+        double2 v = buf[l * NPAD + p + p / (2 * L)];
+        if (use_out) v = cmul_f(v, fac[last][1][sout[p] - 1]);
+        v = cmul_f(v, use_line ? fac[last][2][sline[line0 + l] - 1] : make_double2(1.0, 0.0));
         const size_t o = a.rows ? (size_t)(line0 + l) + (size_t)p * Y.ld : (size_t)p + (size_t)(line0 + l) * Y.ld;
-        Y.re[o] = tr * g.x - ti * g.y;
-        Y.im[o] = tr * g.y + ti * g.x;
+        Y.re[o] = v.x;
+        Y.im[o] = v.y;
     }
 }
 
@@ -336,7 +365,8 @@ KDQ_DEV void circ_body(const CircArgs &a, const ChainDev &ch, int c, int grp, un
 // neighbouring groups one after another there: the row pass reads half of each 128-byte line
 // per group, and the neighbouring group finds the other half in the same L2.
 constexpr int NXCD = 8;
-__global__ __launch_bounds__(CT, 3) void k_circ(CircArgs a, int nch, int gx, int xcd_map) {
+template <bool MULTI>
+KDQ_DEV void circ_dispatch(const CircArgs &a, int nch, int gx, int xcd_map) {
     int c, grp;
     if (xcd_map) {
         const int L = blockIdx.x, x = L % NXCD, k = L / NXCD;
@@ -350,16 +380,25 @@ __global__ __launch_bounds__(CT, 3) void k_circ(CircArgs a, int nch, int gx, int
     const ChainDev &ch = a.chains[c];
     extern __shared__ __align__(16) unsigned char lds_raw[];
     switch (ch.cL) {
-        case 2: circ_body<2>(a, ch, c, grp, lds_raw); break;
-        case 3: circ_body<3>(a, ch, c, grp, lds_raw); break;
-        case 4: circ_body<4>(a, ch, c, grp, lds_raw); break;
-        case 6: circ_body<6>(a, ch, c, grp, lds_raw); break;
-        case 8: circ_body<8>(a, ch, c, grp, lds_raw); break;
-        case 9: circ_body<9>(a, ch, c, grp, lds_raw); break;
-        case 12: circ_body<12>(a, ch, c, grp, lds_raw); break;
-        case 15: circ_body<15>(a, ch, c, grp, lds_raw); break;
+        case 2: circ_body<2, MULTI>(a, ch, c, grp, lds_raw); break;
+        case 3: circ_body<3, MULTI>(a, ch, c, grp, lds_raw); break;
+        case 4: circ_body<4, MULTI>(a, ch, c, grp, lds_raw); break;
+        case 6: circ_body<6, MULTI>(a, ch, c, grp, lds_raw); break;
+        case 8: circ_body<8, MULTI>(a, ch, c, grp, lds_raw); break;
+        case 9: circ_body<9, MULTI>(a, ch, c, grp, lds_raw); break;
+        case 12: circ_body<12, MULTI>(a, ch, c, grp, lds_raw); break;
+        case 15: circ_body<15, MULTI>(a, ch, c, grp, lds_raw); break;
         default: break;   // not reached: the host takes the dense path unless every cL is listed
     }
+}
+
+__global__ __launch_bounds__(CT, 3) void k_circ(CircArgs a, int nch, int gx, int xcd_map) {
+    circ_dispatch<false>(a, nch, gx, xcd_map);
+}
+
+// a product of several factors (a.nsteps > 1) in one launch, the lines kept in LDS between them
+__global__ __launch_bounds__(CT, 2) void k_circ_chain(CircArgs a, int nch, int gx, int xcd_map) {
+    circ_dispatch<true>(a, nch, gx, xcd_map);
 }
 
 }  // namespace
@@ -374,15 +413,17 @@ size_t circ_lds_bytes(int L) {
 }
 
 hipError_t launch_circ(hipStream_t st, int nchains, int grid_x, size_t lds_bytes, const CircArgs &a) {
-    if (a.op < 0 || a.op > 2 || (a.rows && a.op != 0) || grid_x < 1) return hipErrorInvalidValue;
-    if (hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_circ),
+    if (a.op < 0 || a.op > 2 || (a.rows && a.op != 0) || grid_x < 1 || (a.nsteps > 1 && a.rows))
+        return hipErrorInvalidValue;
+    auto kern = a.nsteps > 1 ? k_circ_chain : k_circ;
+    if (hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes))
         return e;
     if (nchains >= NXCD) {
         const int groups = (nchains + NXCD - 1) / NXCD;
-        hipLaunchKernelGGL(k_circ, dim3(NXCD * groups * grid_x), dim3(CT), lds_bytes, st, a, nchains, grid_x, 1);
+        hipLaunchKernelGGL(kern, dim3(NXCD * groups * grid_x), dim3(CT), lds_bytes, st, a, nchains, grid_x, 1);
     } else {
-        hipLaunchKernelGGL(k_circ, dim3(grid_x, nchains), dim3(CT), lds_bytes, st, a, nchains, grid_x, 0);
+        hipLaunchKernelGGL(kern, dim3(grid_x, nchains), dim3(CT), lds_bytes, st, a, nchains, grid_x, 0);
     }
     return hipGetLastError();
 }

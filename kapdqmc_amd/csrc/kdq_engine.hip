@@ -385,6 +385,20 @@ int kin_left(kdq_handle h, int mat, int opa, const Opnd *X, const Opnd *Y, int l
     return gemm_m(h, true, false, false, true, opa, 0, g, fam);
 }
 
+// The product of nsteps kin_left factors over the columns of X (cell-momentum path only): step
+// s with the diagonal factors of slice lt + s·dlt (in, out, line as given for s = 0), the columns
+// kept in LDS between the steps; the same arithmetic as nsteps kin_left launches.
+int kin_left_chain(kdq_handle h, int mat, int opa, const Opnd *X, const Opnd *Y, int ldim, DiagSpec in, DiagSpec out,
+                   DiagSpec line, int nsteps, int dlt, int fam) {
+    ProfScope ps(h, fam);
+    CircArgs a{h->d_chains, X, Y, mat, opa == 0 ? 0 : (h->cK ? 2 : 1), 0, ldim, in, out, line, nsteps, dlt};
+    int gx = 1;
+    size_t bytes = 0;
+    circ_launch_shape(h, ldim, gx, bytes);
+    HIPCHK(launch_circ(h->stream, h->nch, gx, bytes, a));
+    return KDQ_OK;
+}
+
 // Y = line ⊙ ((X ⊙ in) E ⊙ out) over the rows of the ns x ns matrix X (line: per row, in / out:
 // per column). The dense path: gemm_m with B = E, rowS = line, inS = in, colS = out.
 int kin_right(kdq_handle h, int mat, const Opnd *X, const Opnd *Y, DiagSpec in, DiagSpec out, DiagSpec line,
@@ -600,6 +614,12 @@ int prop_L_range(kdq_handle h, Opnd *src, Opnd *dst, int lo, int hi) {
 // dst = QR_Q(B_block(idx) · src) for U_R: slices nodes[idx-1]+1 .. nodes[idx] (1-based idx)
 int propagate_R(kdq_handle h, Opnd *src, Opnd *dst, int idx, double *logacc = nullptr, bool ortho = true) {
     const int lo = (int)h->nodes[idx - 1] + 1, hi = (int)h->nodes[idx];
+    if (h->kind == KDQ_MODEL_TU && use_circ(h) && hi > lo) {   // the block's factors in one launch
+        if (int r = kin_left_chain(h, 0, 0, src, dst, DIM_NP, DiagSpec{}, hs_diag(h, lo, +1), DiagSpec{}, hi - lo + 1,
+                                   +1, KDQ_PROF_CHAIN))
+            return r;
+        return ortho ? orthonormalise(h, dst, logacc) : KDQ_OK;
+    }
     Opnd *cur = src;
     for (int lt = lo; lt <= hi; ++lt) {
         Opnd *out = (lt == hi) ? dst : ((cur == h->t_P0) ? h->t_P1 : h->t_P0);
@@ -619,6 +639,12 @@ int propagate_R(kdq_handle h, Opnd *src, Opnd *dst, int idx, double *logacc = nu
 // for l = nodes[idx] down to nodes[idx-1]+1.
 int propagate_L(kdq_handle h, Opnd *src, Opnd *dst, int idx, bool ortho = true) {
     const int lo = (int)h->nodes[idx - 1] + 1, hi = (int)h->nodes[idx];
+    if (h->kind == KDQ_MODEL_TU && use_circ(h) && hi > lo) {   // the block's factors in one launch
+        if (int r = kin_left_chain(h, 0, 1, src, dst, DIM_NP, hs_diag(h, hi, +1, 1), DiagSpec{}, DiagSpec{}, hi - lo + 1,
+                                   -1, KDQ_PROF_CHAIN))
+            return r;
+        return ortho ? orthonormalise(h, dst) : KDQ_OK;
+    }
     Opnd *cur = src;
     for (int lt = hi; lt >= lo; --lt) {
         Opnd *out = (lt == lo) ? dst : ((cur == h->t_P0) ? h->t_P1 : h->t_P0);

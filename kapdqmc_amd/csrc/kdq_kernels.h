@@ -39,6 +39,10 @@ struct CircArgs {
     const Opnd *Y;
     int mat, op, rows, nline_dim;
     DiagSpec in, out, line;
+    // nsteps > 1 (lines = columns): the product of nsteps such factors, step s with the
+    // diagonal factors of slice lt + s·dlt, the lines held in LDS between the steps (the same
+    // arithmetic as nsteps single launches, Y_s = X_{s+1}); 0 or 1: one factor
+    int nsteps, dlt;
 };
 bool circ_supported(int L);   // cell edges with a kernel instantiation
 int circ_lines_per_wg(int L);  // lines of X per workgroup for cell edge L
