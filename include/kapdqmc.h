@@ -289,10 +289,14 @@ int kdq_get_near_ties(kdq_handle h, double *out);
  *                     before a layer runs as its two products and the layer's first block is
  *                     decided between them, from the first product and the dense kinetic factor
  *                     (its G[S,S] rounds differently from the second product's), beside the
- *                     second product. 0: the first block after the whole wrap. */
+ *                     second product. 0: the first block after the whole wrap.
+ *  KDQ_OPT_MCHAIN     t-U sweep measurements with block-circulant factors. 1 (default): G_sym =
+ *                     HalfeK B(c, lt) G B(c, lt)⁻¹ HalfeK⁻¹ as two chained cell-momentum passes
+ *                     (every slice factor in LDS, one launch per side). 0: products with the
+ *                     slot matrices built slice by slice. The rows agree to rounding. */
 enum { KDQ_OPT_GRAPH = 0, KDQ_OPT_DELAY = 1, KDQ_OPT_GEMM_TILE = 2, KDQ_OPT_DEFER = 3, KDQ_OPT_PIPE = 4,
        KDQ_OPT_FLUSH = 5, KDQ_OPT_ASIDE = 6, KDQ_OPT_CIRC = 7, KDQ_OPT_QSPLIT = 8, KDQ_OPT_MASIDE = 9,
-       KDQ_OPT_GINV = 10, KDQ_OPT_QINV = 11, KDQ_OPT_PREWRAP = 12 };
+       KDQ_OPT_GINV = 10, KDQ_OPT_QINV = 11, KDQ_OPT_PREWRAP = 12, KDQ_OPT_MCHAIN = 13 };
 int kdq_set_option(kdq_handle h, int32_t key, int32_t value);
 
 /* Mixed precision (kdq_run_desc.precision = KDQ_PREC_MIXED, t-U): G is carried in FP32

@@ -15,7 +15,7 @@ KDQ_OK, KDQ_EARG, KDQ_EHIP, KDQ_ENUM, KDQ_ESTATE = 0, -1, -2, -3, -4
 MODEL_TU, MODEL_TV, MODEL_TUV = 0, 1, 2
 LATTICE_CODE = {"HoneyComb120": 0, "HoneyComb60": 1, "SQUARE90": 2, "SQUARE45": 3}
 PREC_FP64, PREC_MIXED = 0, 1
-OPTIONS = {"graph": 0, "delay": 1, "gemm_tile": 2, "defer": 3, "pipe": 4, "flush": 5, "aside": 6, "circ": 7, "qsplit": 8, "maside": 9, "ginv": 10, "qinv": 11, "prewrap": 12}   # kdq_set_option keys
+OPTIONS = {"graph": 0, "delay": 1, "gemm_tile": 2, "defer": 3, "pipe": 4, "flush": 5, "aside": 6, "circ": 7, "qsplit": 8, "maside": 9, "ginv": 10, "qinv": 11, "prewrap": 12, "mchain": 13}   # kdq_set_option keys
 
 # Every symbol include/kapdqmc.h declares (checked by tests/test_capi.py).
 EXPORTS = ("kdq_create", "kdq_destroy", "kdq_last_error", "kdq_num_chains", "kdq_num_obs", "kdq_set_fields",

@@ -277,7 +277,8 @@ KDQ_DEV void circ_body(const CircArgs &a, const ChainDev &ch, int c, int grp, un
     double *lamt = reinterpret_cast<double *>(buf + LPW * NPAD);   // the chain's Λ table of E [k][a][b]
     const int tid = threadIdx.x;
     {   // staged while the lines load; read by the y pass
-        const double *tab = ch.circ + (size_t)a.mat * NC * 8;
+        const int m0 = MULTI && a.plain_last && nsteps == 1 ? a.mat_last : a.mat;
+        const double *tab = ch.circ + (size_t)m0 * NC * 8;
         for (int e = tid; e < NC * 8; e += CT) lamt[e] = tab[e];
     }
     // the t-U HS factors take four values per slice (one per field value): cis(±α η_v) of the
@@ -296,6 +297,7 @@ KDQ_DEV void circ_body(const CircArgs &a, const ChainDev &ch, int c, int grp, un
     const double(&wi)[L] = *reinterpret_cast<const double(*)[L]>(twi);
     const Opnd X = a.X[c], Y = a.Y[c];
     // load (and scale by `in`), four loads of a thread in flight before their LDS stores
+    const bool plain0 = MULTI && a.plain_last && nsteps == 1;   // the only step is the plain one
     {
         const uint8_t *sin_ = step_fields(a, ch, a.in, 0);
         constexpr int NE = LPW * NS, UB = 4;
@@ -315,7 +317,7 @@ KDQ_DEV void circ_body(const CircArgs &a, const ChainDev &ch, int c, int grp, un
             for (int u = 0; u < UB; ++u) {
                 const int e = e0 + u * CT, l = a.rows ? e % LPW : e / NS, p = a.rows ? e / LPW : e % NS;
                 if (e >= NE) break;
-                const double2 w = use_in ? cmul_f(v[u], fac[0][0][sin_[p] - 1]) : v[u];
+                const double2 w = use_in && !plain0 ? cmul_f(v[u], fac[0][0][sin_[p] - 1]) : v[u];
                 buf[l * NPAD + p + p / (2 * L)] = w;
             }
         }
@@ -323,12 +325,20 @@ KDQ_DEV void circ_body(const CircArgs &a, const ChainDev &ch, int c, int grp, un
     __syncthreads();
     for (int st = 0; st < nsteps; ++st) {
         const int cur = st & 1;
+        if (MULTI && a.plain_last && st == nsteps - 1 && st > 0 && a.mat_last != a.mat) {
+            // the closing matrix's table in place of the step matrix's (the y pass of the
+            // previous step has read it: its dft_pass ended with a barrier)
+            const double *tab = ch.circ + (size_t)a.mat_last * NC * 8;
+            for (int e = tid; e < NC * 8; e += CT) lamt[e] = tab[e];
+            __syncthreads();
+        }
         dft_pass<L, true, false>(buf, nlw, NPAD, wr, wi);
         // forward y, the 2 x 2 blocks of E, Eᵀ or Eᴴ acting on a line (a right product X·E acts on
         // the rows of X with Eᵀ), backward y
         y_lambda_pass<L>(buf, nlw, NPAD, wr, wi, lamt, a.rows ? 1 : a.op);
         dft_pass<L, false, false>(buf, nlw, NPAD, wr, wi);
         if (MULTI && st + 1 < nsteps) {   // this step's `out` and `line`, the next step's `in`, in LDS
+            const bool next_plain = a.plain_last && st + 1 == nsteps - 1;
             set_fac(a, ch, fac[cur ^ 1], st + 1, tid);
             __syncthreads();
             const uint8_t *so = step_fields(a, ch, a.out, st), *sl = step_fields(a, ch, a.line, st),
@@ -339,21 +349,22 @@ KDQ_DEV void circ_body(const CircArgs &a, const ChainDev &ch, int c, int grp, un
                 double2 v = buf[l * NPAD + p + p / (2 * L)];
                 if (use_out) v = cmul_f(v, fac[cur][1][so[p] - 1]);
                 v = cmul_f(v, use_line ? fac[cur][2][sl[line0 + l] - 1] : make_double2(1.0, 0.0));
-                if (use_in) v = cmul_f(v, fac[cur ^ 1][0][si[p] - 1]);
+                if (use_in && !next_plain) v = cmul_f(v, fac[cur ^ 1][0][si[p] - 1]);
                 buf[l * NPAD + p + p / (2 * L)] = v;
             }
             __syncthreads();
         }
     }
-    // store (scaled by `out` and the line factor of the last step)
+    // store (scaled by `out` and the line factor of the last step, unless it is plain)
     const int last = (nsteps - 1) & 1;
+    const bool plain = MULTI && a.plain_last;
     const uint8_t *sout = step_fields(a, ch, a.out, nsteps - 1), *sline = step_fields(a, ch, a.line, nsteps - 1);
     for (int e = tid; e < LPW * NS; e += CT) {
         const int l = a.rows ? e % LPW : e / NS, p = a.rows ? e / LPW : e % NS;
         if (l >= nlw) continue;
         double2 v = buf[l * NPAD + p + p / (2 * L)];
-        if (use_out) v = cmul_f(v, fac[last][1][sout[p] - 1]);
-        v = cmul_f(v, use_line ? fac[last][2][sline[line0 + l] - 1] : make_double2(1.0, 0.0));
+        if (use_out && !plain) v = cmul_f(v, fac[last][1][sout[p] - 1]);
+        v = cmul_f(v, use_line && !plain ? fac[last][2][sline[line0 + l] - 1] : make_double2(1.0, 0.0));
         const size_t o = a.rows ? (size_t)(line0 + l) + (size_t)p * Y.ld : (size_t)p + (size_t)(line0 + l) * Y.ld;
         Y.re[o] = v.x;
         Y.im[o] = v.y;
@@ -413,9 +424,10 @@ size_t circ_lds_bytes(int L) {
 }
 
 hipError_t launch_circ(hipStream_t st, int nchains, int grid_x, size_t lds_bytes, const CircArgs &a) {
-    if (a.op < 0 || a.op > 2 || (a.rows && a.op != 0) || grid_x < 1 || (a.nsteps > 1 && a.rows))
+    if (a.op < 0 || a.op > 2 || (a.rows && a.op != 0) || grid_x < 1 || a.mat < 0 || a.mat > 3 || a.mat_last < 0 ||
+        a.mat_last > 3)
         return hipErrorInvalidValue;
-    auto kern = a.nsteps > 1 ? k_circ_chain : k_circ;
+    auto kern = a.nsteps > 1 || a.plain_last ? k_circ_chain : k_circ;
     if (hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes))
         return e;

@@ -92,6 +92,8 @@ struct kdq_handle_s {
     bool maside = true;                        // t-U sweeps: a measurement's products and tiles on aux
     bool ginv = true;                          // get_G!: register Gauss-Jordan inverse + GEMM (Np <= 144)
     bool prewrap = true;                       // pipelined t-U layers: block 0 decided beside the wrap's 2nd product
+    bool mchain = true;                        // t-U sweep measurements as chained cell-momentum passes
+    bool mchain_ok = true;                     // every chain's HalfeK, HalfeK⁻¹ block-circulant too
     bool pw_next = false;                      // the next layer's block 0 was launched with the wrap before it
     double *d_pwpart = nullptr;                // prewrap partial products: [chain][N/32][2][32][32]
     bool mpending = false;                     // such a measurement not yet joined
@@ -1346,9 +1348,59 @@ int measure_q(kdq_handle h, int lt, int k, bool wrap_first, bool fork = false) {
     return r;
 }
 
-// phy_measure at slice lt of a t-U sweep (after the slice's update): stepwise wraps where
-// they cost no more (|lt − c| ≤ 1, or 2 going backward above c), the slot products elsewhere
+// phy_measure at slice lt of a t-U sweep with the cell-momentum products: G_sym =
+// HalfeK B(c, lt) G B(c, lt)⁻¹ HalfeK⁻¹ (c = Nt/2; B(c, lt) = B_c ... B_(lt+1) below c and
+// B_(c+1)⁻¹ ... B_lt⁻¹ above, with the fields the measurement sees) as one chained launch over
+// the columns of G (the slices' factors, then HalfeK) and one over the rows of the result (their
+// inverses in the same order, then HalfeK⁻¹) — the stepwise wraps of measure_into with every
+// left factor first. Both passes read fields of slices the sweep updates next, so they run in
+// line; the tile reduction of G_sym runs beside the sweep (option maside).
+int measure_chain(kdq_handle h, int lt) {
+    const bool fk = meas_aside(h);
+    if (fk)   // the previous measurement's reduction still reads Gm
+        if (int r = meas_join(h)) return r;
+    Opnd *Gm = h->t_Gm, *Tm = h->t_Tm, *src = h->t_G[h->cur];
+    if (h->mixed) {
+        HIPCHK(launch_to_f64(h->stream, h->nch, h->max_ns, h->d_chains, h->t_G32[h->cur32], Gm, DIM_NS, 1));
+        src = Gm;
+    }
+    const int c = h->nt / 2, k = lt < c ? c - lt : lt - c;
+    const DiagSpec none{};
+    CircArgs l{h->d_chains, src, Tm, 0, 0, 0, DIM_NS, none, none, none, k + 1, 0, 1, 2};
+    CircArgs r{h->d_chains, Tm, Gm, 0, 0, 1, DIM_NS, none, none, none, k + 1, 0, 1, 3};
+    if (lt < c) {          // B_(lt+1) .. B_c, B_s = D_s eK; inverses eK⁻¹ D_s⁻¹ from the right
+        l.mat = 0; l.out = hs_diag(h, lt + 1, +1); l.dlt = +1;
+        r.mat = 1; r.out = hs_diag(h, lt + 1, -1); r.dlt = +1;
+    } else if (lt > c) {   // B_lt⁻¹ .. B_(c+1)⁻¹, B_s⁻¹ = eK⁻¹ D_s⁻¹; then D_s eK from the right
+        l.mat = 1; l.in = hs_diag(h, lt, -1); l.dlt = -1;
+        r.mat = 0; r.in = hs_diag(h, lt, +1); r.dlt = -1;
+    }
+    {
+        ProfScope ps(h, KDQ_PROF_MEAS_WRAP);
+        int gx = 1;
+        size_t bytes = 0;
+        circ_launch_shape(h, DIM_NS, gx, bytes);
+        HIPCHK(launch_circ(h->stream, h->nch, gx, bytes, l));
+        HIPCHK(launch_circ(h->stream, h->nch, gx, bytes, r));
+    }
+    if (fk)
+        if (int r2 = meas_fork(h)) return r2;
+    int rc = KDQ_OK;
+    {
+        MeasArgs m{h->d_chains, Gm, nullptr, h->d_mpart, meas_tiles(h->max_ns)};
+        ProfScope ps(h, KDQ_PROF_MEASURE);
+        if (hipError_t e = launch_measure(h->stream, h->nch, m)) rc = fail(h, KDQ_EHIP, hipGetErrorString(e));
+    }
+    if (fk)
+        if (int r2 = meas_unfork(h)) return r2;
+    return rc;
+}
+
+// phy_measure at slice lt of a t-U sweep (after the slice's update): chained cell-momentum
+// passes when the factors are block-circulant; otherwise stepwise wraps where they cost no more
+// (|lt − c| ≤ 1, or 2 going backward above c), the slot products elsewhere
 int measure_sweep(kdq_handle h, int lt, bool forward) {
+    if (use_circ(h) && h->mchain && h->mchain_ok) return measure_chain(h, lt);
     if (h->kind != KDQ_MODEL_TU || h->mq.empty()) return measure(h, lt);
     const bool fk = meas_aside(h);
     if (fk)   // the previous measurement's back half still reads the copy and a slot matrix
@@ -1633,25 +1685,44 @@ const char *kdq_last_error(kdq_handle h) {
 // their largest entry (the model constructors' eigen-decompositions leave ~1e-15) and returns L
 // with tab = [eK, eK⁻¹][k][a][b] complex Λ(k) = (1/L²) Σ_d E_ab(d) e^{+2πi (kx dx + ky dy)/L};
 // 0 when the matrices are not block-circulant or L has no kernel instantiation.
-int circ_table(const kdq_model_desc &m, bool cplx, std::vector<double> &tab) {
+int circ_table(const kdq_model_desc &m, bool cplx, std::vector<double> &tab, bool &half) {
     const int L = m.site[0];
     if (m.site[1] != L || L < 1 || m.ns != 2 * L * L || !circ_supported(L)) return 0;
     const int ns = m.ns, nc = L * L;
     const size_t nn = (size_t)ns * ns;
-    tab.assign((size_t)2 * nc * 8, 0.0);
-    for (int mat = 0; mat < 2; ++mat) {
-        const double *re = mat ? m.eKinv : m.eK;
-        const double *im = cplx ? (mat ? m.eKinv_im : m.eK_im) : nullptr;
+    // eK, eK⁻¹ and the measurement's HalfeK, HalfeK⁻¹ (mats 0..3), each checked block-circulant;
+    // HalfeK's pair only serves the chained measurement (half = false when it is not circulant)
+    tab.assign((size_t)4 * nc * 8, 0.0);
+    half = true;
+    for (int mat = 0; mat < 4; ++mat) {
+        const double *res[4] = {m.eK, m.eKinv, m.halfeK, m.halfeKinv};
+        const double *ims[4] = {m.eK_im, m.eKinv_im, m.halfeK_im, m.halfeKinv_im};
+        const double *re = res[mat];
+        const double *im = cplx ? ims[mat] : nullptr;
+        if (!re) {
+            if (mat < 2) return 0;
+            half = false;
+            break;
+        }
         double big = 0.0;
         for (size_t q = 0; q < nn; ++q) big = std::max(big, std::abs(re[q]) + (im ? std::abs(im[q]) : 0.0));
         const double tol = 1e-12 * big;
-        for (int j = 0; j < ns; ++j)
+        bool circulant = true;
+        for (int j = 0; j < ns && circulant; ++j)
             for (int i = 0; i < ns; ++i) {
                 const int ci = i / 2, a = i % 2, cj = j / 2, b = j % 2;
                 const int dx = ((cj % L) - (ci % L) + L) % L, dy = ((cj / L) - (ci / L) + L) % L;
                 const size_t o = (size_t)i + (size_t)j * ns, o0 = (size_t)a + (size_t)(2 * (dx + L * dy) + b) * ns;
-                if (std::abs(re[o] - re[o0]) > tol || (im && std::abs(im[o] - im[o0]) > tol)) return 0;
+                if (std::abs(re[o] - re[o0]) > tol || (im && std::abs(im[o] - im[o0]) > tol)) {
+                    circulant = false;
+                    break;
+                }
             }
+        if (!circulant) {
+            if (mat < 2) return 0;
+            half = false;
+            break;
+        }
         for (int k = 0; k < nc; ++k) {
             const int kx = k % L, ky = k / L;
             for (int ab = 0; ab < 4; ++ab) {
@@ -2020,7 +2091,9 @@ int create_impl(const kdq_model_desc *models, const kdq_run_desc *runs, int32_t 
         }
         {
             std::vector<double> tab;
-            circ_L[c] = circ_table(m, h->cK, tab);
+            bool half = false;
+            circ_L[c] = circ_table(m, h->cK, tab, half);
+            if (!half) h->mchain_ok = false;
             if (circ_L[c]) {
                 if (dalloc(C, &C.circ, tab.size()) != hipSuccess) {
                     int r = fail(h, KDQ_EHIP, "kdq_create: device allocation failed");
@@ -2742,6 +2815,9 @@ int kdq_set_option(kdq_handle h, int32_t key, int32_t value) {
             break;
         case KDQ_OPT_PREWRAP:
             h->prewrap = value != 0;
+            break;
+        case KDQ_OPT_MCHAIN:
+            h->mchain = value != 0;
             break;
         case KDQ_OPT_QSPLIT:
             h->qsplit = value != 0;

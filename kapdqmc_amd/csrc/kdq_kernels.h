@@ -31,7 +31,7 @@ hipError_t launch_gemm_f32(hipStream_t st, int nchains, int max_m, int max_n, in
 // Kinetic product through cell momenta (block-circulant eK / eK⁻¹, ChainDev::circ, cL > 0 for
 // every chain): per chain, Y = line ⊙ (out ⊙ op(E) (in ⊙ X)) over the nline_dim columns of X
 // (rows = 0), or Y = line ⊙ ((X ⊙ in) op(E) ⊙ out) over its rows (rows = 1, op(E) acting from
-// the right), E = eK (mat 0) or eK⁻¹ (mat 1), op 0: E, 1: Eᵀ, 2: Eᴴ. The diagonal factors are
+// the right), E = eK (mat 0), eK⁻¹ (mat 1), HalfeK (2) or HalfeK⁻¹ (3), op 0: E, 1: Eᵀ, 2: Eᴴ. The diagonal factors are
 // those of the GEMM path, t-U HS phases only (DiagSpec kind 0 = none, 1). Y may be X.
 struct CircArgs {
     const ChainDev *chains;
@@ -43,6 +43,9 @@ struct CircArgs {
     // diagonal factors of slice lt + s·dlt, the lines held in LDS between the steps (the same
     // arithmetic as nsteps single launches, Y_s = X_{s+1}); 0 or 1: one factor
     int nsteps, dlt;
+    // plain_last: the last step is the product with matrix mat_last alone (no diagonal factors),
+    // e.g. HalfeK (2) or HalfeK⁻¹ (3) closing the measurement's propagation
+    int plain_last, mat_last;
 };
 bool circ_supported(int L);   // cell edges with a kernel instantiation
 int circ_lines_per_wg(int L);  // lines of X per workgroup for cell edge L

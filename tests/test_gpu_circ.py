@@ -7,7 +7,9 @@ kernel instantiation and both directions; sweeps with the two paths: fields bit-
 within 1e-11 relative, rows within 1e-9 (the paths round differently; the oracle parity bars);
 sweeps of t-U on every lattice, complex hopping (π flux on SQUARE90); t-V and t-U-V, a batch
 whose eK is not block-circulant and a cell edge without an instantiation take the dense path
-(bit-identical results with the option on and off)."""
+(bit-identical results with the option on and off). The sweep measurements as two chained
+cell-momentum passes (option "mchain") against the slot-matrix products, to the same bars, and
+a model whose HalfeK is not block-circulant keeps the slot products (bit-identical)."""
 import copy
 
 import numpy as np
@@ -54,11 +56,13 @@ def test_wrap_against_oracle(oracle_mod, L, forward):
     assert rel(out[1], ref) < 1e-12
 
 
-def sweep(models, circ, sweeps=2, delay=16, precision="fp64"):
+def sweep(models, circ, sweeps=2, delay=16, precision="fp64", extra=()):
     from kapdqmc_amd.dqmc import DQMCBatch
     b = DQMCBatch(models, list(range(3, 3 + len(models))), list(range(len(models))), delay=delay,
                   precision=precision)
     b.set_option("circ", circ)
+    for k, v in extra:
+        b.set_option(k, v)
     b.initial_s()
     rows = b.phy_update(sweeps, True)
     out = ([b.get_fields(c) for c in range(b.n)], [b.get_G(c) for c in range(b.n)], rows,
@@ -135,3 +139,32 @@ def test_dense_fallbacks():
     m.eK = m.eK + 1e-6 * (R + R.T)
     m.eKinv = np.linalg.inv(m.eK)
     agree(sweep([m], 1), sweep([m], 0), exact=True)
+
+
+@pytest.mark.parametrize("case", ["uneven", "flux", "mixed"])
+def test_chained_measurements(case):
+    print(f"case: chained vs slot-product measurements, {case}", flush=True)
+    prec = "mixed" if case == "mixed" else "fp64"
+    if case == "flux":
+        models = [tu(6, lattice="SQUARE90", flux=np.pi)]
+    else:
+        models = [tu(3), tu(4, U=2.0), tu(6, U=6.0)]
+    a = sweep(models, 1, precision=prec, extra=(("mchain", 1),))
+    b = sweep(models, 1, precision=prec, extra=(("mchain", 0),))
+    if case == "mixed":
+        for c in range(len(models)):
+            assert np.array_equal(a[0][c], b[0][c]), c
+            assert np.array_equal(a[1][c], b[1][c]), c
+        assert np.allclose(a[2], b[2], rtol=1e-9, atol=1e-11), np.abs(a[2] - b[2]).max()
+    else:
+        agree(a, b)
+
+
+def test_chained_measurements_need_circulant_halfek():
+    print("case: perturbed HalfeK keeps the slot-product measurements", flush=True)
+    m = copy.deepcopy(tu(4))
+    rs = np.random.default_rng(1)
+    R = rs.standard_normal(m.HalfeK.shape)
+    m.HalfeK = m.HalfeK + 1e-6 * (R + R.T)
+    m.HalfeKinv = np.linalg.inv(m.HalfeK)
+    agree(sweep([m], 1, extra=(("mchain", 1),)), sweep([m], 1, extra=(("mchain", 0),)), exact=True)
