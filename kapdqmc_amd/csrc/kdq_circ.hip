@@ -5,19 +5,26 @@
 // invariant under cell translations, so eK = exp(-Δτ K) and eK⁻¹ are block-circulant: with
 // site s = 2·(x + L·y) + b, E[(c,a), (c',b)] = E_ab(c' - c). A product E·X then factors as
 //     X̂_b(k) = Σ_c e^{-i k·c} X[(c,b)],   Ŷ_a(k) = Σ_b Λ_ab(k) X̂_b(k),   Y[(c,a)] = Σ_k e^{+i k·c} Ŷ_a(k)
-// with Λ(k) = (1/L²) Σ_d E(d) e^{+i k·d} (ChainDev::circ). The cell transforms are two passes of
-// length-L FFTs (along x, then y; radix 2, 3, 4, 5 butterflies and one Cooley-Tukey step for
-// L = 6, 8, 9, 12, 15). Each workgroup holds LPW = 96 / L lines of X in LDS (columns for a left
-// product E·X, rows for a right product X·E = (Eᵀ Xᵀ)ᵀ), so that its 192 threads take the 2L
-// transforms of every line one thread each; it transforms them, applies the 2 x 2 blocks and
-// transforms back: one read and one write of X and O(L² log L) flops per line, against (2L²)²
-// for the dense product.
+// with Λ(k) = (1/L²) Σ_d E(d) e^{+i k·d} (ChainDev::circ). The cell transforms are length-L FFTs
+// (radix 2, 3, 4, 5 butterflies and one Cooley-Tukey step for L = 6, 8, 9, 12, 15) along y, then
+// x. A workgroup takes LPW = 96 / L lines of X (columns for a left product E·X, rows for a right
+// product X·E = (Eᵀ Xᵀ)ᵀ) with its 192 threads, one per (line, x, b): each thread loads its L
+// elements of a line straight from X, transforms them along y in registers and puts them in
+// LDS; one pass over LDS takes the x transforms, the 2 x 2 blocks and the inverse x transforms
+// (one thread per (line, ky, b)); each thread then takes its elements back, transforms them
+// backward along y and stores them. One read and one write of X and O(L² log L) flops per line,
+// against (2L²)² for the dense product.
 // op selects E (0), Eᵀ (1: Λ_ba(-k)) or Eᴴ (2: conj Λ_ba(k)). The diagonal factors of the GEMM
 // path (the t-U HS phases, DiagSpec kind 0 or 1) are applied in the same places: `in` scales the
 // line entries before the product, `out` the products, `line` the whole line (per column of a
 // left product, per row of a right product). Every line is read completely before it is
 // written, so Y may be X.
 #include "kdq_kernels.h"
+
+// Multiply-adds fuse only within one source expression (not across statements, as the
+// backend's default would), so the single-product kernel and the chained kernel, which inline
+// the same transforms into different surroundings, round every step alike.
+#pragma clang fp contract(on)
 
 namespace {
 
@@ -107,17 +114,17 @@ KDQ_DEV void fft(double *xr, double *xi, const double *wr, const double *wi) {
         xr[1] = ar - xr[1];
         xi[1] = ai - xi[1];
     } else if constexpr (L == 3) {
-        constexpr double h = 0.86602540378443864676;   // sin(2π/3)
+        // x0 + x1 + x2 and (x0 - (x1 + x2)/2) ± (∓i) sin(2π/3) (x1 - x2), the last as fmas
+        constexpr double h = FWD ? 0.86602540378443864676 : -0.86602540378443864676;   // ±sin(2π/3)
         const double s1r = xr[1] + xr[2], s1i = xi[1] + xi[2];
-        double dr = h * (xr[1] - xr[2]), di = h * (xi[1] - xi[2]);
-        mul_i<FWD>(dr, di);
+        const double dr = xr[1] - xr[2], di = xi[1] - xi[2];
         const double tr = xr[0] - 0.5 * s1r, ti = xi[0] - 0.5 * s1i;
         xr[0] += s1r;
         xi[0] += s1i;
-        xr[1] = tr + dr;
-        xi[1] = ti + di;
-        xr[2] = tr - dr;
-        xi[2] = ti - di;
+        xr[1] = fma(h, di, tr);
+        xi[1] = fma(-h, dr, ti);
+        xr[2] = fma(-h, di, tr);
+        xi[2] = fma(h, dr, ti);
     } else if constexpr (L == 4) {
         const double t0r = xr[0] + xr[2], t0i = xi[0] + xi[2], t1r = xr[0] - xr[2], t1i = xi[0] - xi[2];
         const double t2r = xr[1] + xr[3], t2i = xi[1] + xi[3];
@@ -159,7 +166,7 @@ KDQ_DEV void fft(double *xr, double *xi, const double *wr, const double *wi) {
     } else if constexpr (L == 9) {
         fft_pq<3, 3, FWD>(xr, xi, wr, wi);
     } else if constexpr (L == 12) {
-        fft_pq<4, 3, FWD>(xr, xi, wr, wi);
+        fft_pq<3, 4, FWD>(xr, xi, wr, wi);
     } else if constexpr (L == 15) {
         fft_pq<3, 5, FWD>(xr, xi, wr, wi);
     } else {
@@ -167,48 +174,24 @@ KDQ_DEV void fft(double *xr, double *xi, const double *wr, const double *wi) {
     }
 }
 
-// One DFT pass over the LPW lines in LDS (line stride npad), in place: along x (elements
-// 2·(x + L·y) + b of a row, stride 2) or along y (stride 2L + 1: rows of 2L elements are padded
-// to 2L + 1, so that the x pass of neighbouring rows falls on different banks). One thread per
-// (line, row or column, b).
-template <int L, bool FWD, bool ALONG_Y>
-KDQ_DEV void dft_pass(double2 *buf, int nlines, int npad, const double (&wr)[L], const double (&wi)[L]) {
-    const int q = threadIdx.x, l = q / (2 * L), r = q % (2 * L), b = r & 1, o = r >> 1;
-    if (l < nlines) {
-        double2 *base = buf + (size_t)l * npad + (ALONG_Y ? 2 * o + b : (2 * L + 1) * o + b);
-        const int stride = ALONG_Y ? 2 * L + 1 : 2;
-        double xr[L], xi[L];
-#pragma unroll
-        for (int m = 0; m < L; ++m) {
-            const double2 v = base[m * stride];
-            xr[m] = v.x;
-            xi[m] = v.y;
-        }
-        fft<L, FWD>(xr, xi, wr, wi);
-#pragma unroll
-        for (int m = 0; m < L; ++m) base[m * stride] = make_double2(xr[m], xi[m]);
-    }
-    __syncthreads();
-}
-
-// The y pass with the 2 x 2 blocks between its two directions, in registers: thread (line,
-// kx = o, b) transforms its column forward, takes the other component (b ^ 1) of every
-// momentum from the neighbouring lane, forms y_b(k) = M_bb(k) x_b(k) + M_b,1-b(k) x_1-b(k) and
-// transforms back. M = Λ (eop 0), Λ(-k)ᵀ (eop 1) or Λᴴ (eop 2), from the chain's table staged in
-// LDS (Λ_ij(k) at tab + 8k + 2(2i + j)). Every thread of the workgroup calls it (the lane
-// exchange and one barrier).
+// The middle pass of a product, in place on the LPW lines in LDS (line stride npad; element
+// 2·(x + L·y) + b of a line at (2L + 1)·y + 2x + b, rows of 2L padded to 2L + 1), after the y
+// transforms: thread (line, ky = o, b) transforms its row forward along x, takes the other
+// component (b ^ 1) of every momentum from the neighbouring lane, forms y_b(k) = M_bb(k) x_b(k)
+// + M_b,1-b(k) x_1-b(k) and transforms back. M = Λ (eop 0), Λ(-k)ᵀ (eop 1) or Λᴴ (eop 2), from
+// the chain's table staged in LDS (Λ_ij(k) at tab + 8k + 2(2i + j), k = kx + L·ky). Every
+// thread of the workgroup calls it (the lane exchange).
 template <int L>
-KDQ_DEV void y_lambda_pass(double2 *buf, int nlines, int npad, const double (&wr)[L], const double (&wi)[L],
+KDQ_DEV void x_lambda_pass(double2 *buf, int nlines, int npad, const double (&wr)[L], const double (&wi)[L],
                            const double *tab, int eop) {
     const int q = threadIdx.x, l = q / (2 * L), r = q % (2 * L), b = r & 1, o = r >> 1;
     const bool act = l < nlines;
-    double2 *base = buf + (size_t)l * npad + 2 * o + b;
-    constexpr int stride = 2 * L + 1;
+    double2 *base = buf + (size_t)l * npad + (2 * L + 1) * o + b;
     double xr[L], xi[L];
 #pragma unroll
     for (int m = 0; m < L; ++m) {
         double2 v = make_double2(0.0, 0.0);
-        if (act) v = base[m * stride];
+        if (act) v = base[2 * m];
         xr[m] = v.x;
         xi[m] = v.y;
     }
@@ -218,8 +201,8 @@ KDQ_DEV void y_lambda_pass(double2 *buf, int nlines, int npad, const double (&wr
 #pragma unroll
     for (int m = 0; m < L; ++m) {
         const double pr = __shfl_xor(xr[m], 1), pi = __shfl_xor(xi[m], 1);
-        int ks = o + L * m;
-        if (eop == 1) ks = ((L - o) % L) + L * ((L - m) % L);
+        int ks = m + L * o;
+        if (eop == 1) ks = ((L - m) % L) + L * ((L - o) % L);
         const double *t = tab + (size_t)ks * 8;
         const double ar = t[own], ai = cj * t[own + 1], br = t[oth], bi = cj * t[oth + 1];
         const double yr = ar * xr[m] - ai * xi[m] + br * pr - bi * pi;
@@ -230,9 +213,8 @@ KDQ_DEV void y_lambda_pass(double2 *buf, int nlines, int npad, const double (&wr
     fft<L, false>(xr, xi, wr, wi);
     if (act) {
 #pragma unroll
-        for (int m = 0; m < L; ++m) base[m * stride] = make_double2(xr[m], xi[m]);
+        for (int m = 0; m < L; ++m) base[2 * m] = make_double2(xr[m], xi[m]);
     }
-    __syncthreads();
 }
 
 KDQ_HD int lpw_of(int L) { return L >= 96 ? 1 : 96 / L; }
@@ -264,110 +246,170 @@ KDQ_DEV const uint8_t *step_fields(const CircArgs &a, const ChainDev &ch, const 
     return ch.s + (size_t)(lt - 1) * ch.n_elem;
 }
 
+// a plane element at a byte offset from the plane's (workgroup-uniform) base
+KDQ_DEV double ld_at(const double *base, uint32_t off) {
+    return *reinterpret_cast<const double *>(reinterpret_cast<const char *>(base) + off);
+}
+KDQ_DEV void st_at(double *base, uint32_t off, double v) {
+    *reinterpret_cast<double *>(reinterpret_cast<char *>(base) + off) = v;
+}
+
+// Thread -> (line, x, b) of the y transforms: for columns the 2L threads of a line together (a
+// line's 2L elements at one y are contiguous in memory), for rows b and the lines fastest (the
+// lines of one column are contiguous). Threads past 2L·LPW have x >= L or l >= LPW.
+template <int L>
+KDQ_DEV void y_thread(bool rows, int &l, int &x, int &b) {
+    constexpr int LPW = 96 / L;
+    const int q = threadIdx.x;
+    if (rows) {
+        b = q & 1;
+        l = (q >> 1) % LPW;
+        x = (q >> 1) / LPW;
+    } else {
+        l = q / (2 * L);
+        x = (q % (2 * L)) >> 1;
+        b = q & 1;
+    }
+}
+
+// One product (or a chain of nsteps products) on a group of LPW lines. Each thread loads its
+// L elements of a line (fixed x and b, every y) from X, applies `in` and transforms them along
+// y in registers, and writes them to LDS; the x transforms with the 2 x 2 blocks run as one
+// pass over LDS; each thread reads its elements back, transforms them backward along y and
+// stores them (scaled by `out` and `line`) — or, between the steps of a chain, applies this
+// step's `out` and `line` and the next step's `in` in registers and starts the next step.
 template <int L, bool MULTI>
 KDQ_DEV void circ_body(const CircArgs &a, const ChainDev &ch, int c, int grp, unsigned char *lds) {
-    constexpr int NC = L * L, NS = 2 * NC, NPAD = (2 * L + 1) * L + 1, LPW = 96 / L;
+    constexpr int NC = L * L, NPAD = (2 * L + 1) * L + 1, LPW = 96 / L, YS = 2 * L + 1;
     const int nl = dim_of(ch, a.nline_dim);
     const int line0 = grp * LPW;
     if (line0 >= nl) return;   // uniform per workgroup
     const int nlw = min(LPW, nl - line0);
     const bool use_in = a.in.kind != 0, use_out = a.out.kind != 0, use_line = a.line.kind != 0;
     const int nsteps = MULTI && a.nsteps > 1 ? a.nsteps : 1;
-    double2 *buf = reinterpret_cast<double2 *>(lds);   // [LPW][NPAD]: p = 2(x + L y) + b at p + y
+    double2 *buf = reinterpret_cast<double2 *>(lds);   // [LPW][NPAD]
     double *lamt = reinterpret_cast<double *>(buf + LPW * NPAD);   // the chain's Λ table of E [k][a][b]
     const int tid = threadIdx.x;
-    {   // staged while the lines load; read by the y pass
-        const int m0 = MULTI && a.plain_last && nsteps == 1 ? a.mat_last : a.mat;
-        const double *tab = ch.circ + (size_t)m0 * NC * 8;
-        for (int e = tid; e < NC * 8; e += CT) lamt[e] = tab[e];
-    }
+    int l, x, b;
+    y_thread<L>(a.rows, l, x, b);
+    const bool act = l < nlw && x < L;
+    const Opnd X = a.X[c], Y = a.Y[c];
+    // byte offsets of a thread's elements from the planes' base (32 bits: a matrix plane holds at
+    // most a few MB), element m at off + m·step
+    auto off0 = [&](const Opnd &M) {
+        const uint32_t p = 2 * x + b, ln = line0 + l;
+        return 8u * (a.rows ? ln + p * (uint32_t)M.ld : p + ln * (uint32_t)M.ld);
+    };
+    auto step = [&](const Opnd &M) { return 16u * L * (a.rows ? (uint32_t)M.ld : 1u); };
     // the t-U HS factors take four values per slice (one per field value): cis(±α η_v) of the
     // `in`, `out` and `line` factors of a step, evaluated once per workgroup and step (kind 0 = 1)
     __shared__ double twr[16], twi[16];   // w[e] = e^{-2πi e/L}, read from LDS by the butterflies
     __shared__ double2 fac[2][3][4];
-    if (tid < L) {
-        double sn, cs;
-        sincospi(2.0 * tid / L, &sn, &cs);
-        twr[tid] = cs;
-        twi[tid] = -sn;
-    }
-    set_fac(a, ch, fac[0], 0, tid);
-    __syncthreads();
     const double(&wr)[L] = *reinterpret_cast<const double(*)[L]>(twr);
     const double(&wi)[L] = *reinterpret_cast<const double(*)[L]>(twi);
-    const Opnd X = a.X[c], Y = a.Y[c];
-    // load (and scale by `in`), four loads of a thread in flight before their LDS stores
-    const bool plain0 = MULTI && a.plain_last && nsteps == 1;   // the only step is the plain one
+    double2 *ybase = buf + (size_t)l * NPAD + 2 * x + b;
+    auto tables = [&]() {
+        if (tid < L) {
+            double sn, cs;
+            sincospi(2.0 * tid / L, &sn, &cs);
+            twr[tid] = cs;
+            twi[tid] = -sn;
+        }
+        set_fac(a, ch, fac[0], 0, tid);
+    };
+    // the twiddles and the first step's factors: after the loads are issued where the registers
+    // allow it (L <= 12), before them at L = 15
+    if constexpr (L > 12) tables();
     {
-        const uint8_t *sin_ = step_fields(a, ch, a.in, 0);
-        constexpr int NE = LPW * NS, UB = 4;
-        for (int e0 = tid; e0 < NE; e0 += UB * CT) {
-            double2 v[UB];
+        double xr[L], xi[L];
+        if (act) {   // the loads in flight while the tables are built and staged
+            const uint32_t o0 = off0(X), sx = step(X);
 #pragma unroll
-            for (int u = 0; u < UB; ++u) {
-                // columns: positions fastest (contiguous); rows: the lines of one column together
-                const int e = e0 + u * CT, l = a.rows ? e % LPW : e / NS, p = a.rows ? e / LPW : e % NS;
-                v[u] = make_double2(0.0, 0.0);
-                if (e < NE && l < nlw) {
-                    const size_t o = a.rows ? (size_t)(line0 + l) + (size_t)p * X.ld : (size_t)p + (size_t)(line0 + l) * X.ld;
-                    v[u] = make_double2(X.re[o], X.im[o]);
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < UB; ++u) {
-                const int e = e0 + u * CT, l = a.rows ? e % LPW : e / NS, p = a.rows ? e / LPW : e % NS;
-                if (e >= NE) break;
-                const double2 w = use_in && !plain0 ? cmul_f(v[u], fac[0][0][sin_[p] - 1]) : v[u];
-                buf[l * NPAD + p + p / (2 * L)] = w;
+            for (int m = 0; m < L; ++m) {
+                xr[m] = ld_at(X.re, o0 + m * sx);
+                xi[m] = ld_at(X.im, o0 + m * sx);
             }
         }
+        if constexpr (L <= 12) tables();
+        {
+            const int m0 = MULTI && a.plain_last && nsteps == 1 ? a.mat_last : a.mat;
+            const double *tab = ch.circ + (size_t)m0 * NC * 8;
+            for (int e = tid; e < NC * 8; e += CT) lamt[e] = tab[e];
+        }
+        __syncthreads();
+        if (act) {   // `in` (unless the only step is the plain one), forward along y, to LDS
+            if (use_in && !(MULTI && a.plain_last && nsteps == 1)) {
+                const uint8_t *sin_ = step_fields(a, ch, a.in, 0);
+#pragma unroll
+                for (int m = 0; m < L; ++m) {
+                    const double2 w = cmul_f(make_double2(xr[m], xi[m]), fac[0][0][sin_[2 * (x + L * m) + b] - 1]);
+                    xr[m] = w.x;
+                    xi[m] = w.y;
+                }
+            }
+            fft<L, true>(xr, xi, wr, wi);
+#pragma unroll
+            for (int m = 0; m < L; ++m) ybase[m * YS] = make_double2(xr[m], xi[m]);
+        }
     }
-    __syncthreads();
     for (int st = 0; st < nsteps; ++st) {
         const int cur = st & 1;
         if (MULTI && a.plain_last && st == nsteps - 1 && st > 0 && a.mat_last != a.mat) {
-            // the closing matrix's table in place of the step matrix's (the y pass of the
-            // previous step has read it: its dft_pass ended with a barrier)
+            // the closing matrix's table in place of the step matrix's (every thread has left the
+            // previous step's x pass: a barrier followed it)
             const double *tab = ch.circ + (size_t)a.mat_last * NC * 8;
             for (int e = tid; e < NC * 8; e += CT) lamt[e] = tab[e];
-            __syncthreads();
         }
-        dft_pass<L, true, false>(buf, nlw, NPAD, wr, wi);
-        // forward y, the 2 x 2 blocks of E, Eᵀ or Eᴴ acting on a line (a right product X·E acts on
-        // the rows of X with Eᵀ), backward y
-        y_lambda_pass<L>(buf, nlw, NPAD, wr, wi, lamt, a.rows ? 1 : a.op);
-        dft_pass<L, false, false>(buf, nlw, NPAD, wr, wi);
-        if (MULTI && st + 1 < nsteps) {   // this step's `out` and `line`, the next step's `in`, in LDS
+        __syncthreads();
+        // forward x, the 2 x 2 blocks of E, Eᵀ or Eᴴ acting on a line (a right product X·E acts on
+        // the rows of X with Eᵀ), backward x
+        x_lambda_pass<L>(buf, nlw, NPAD, wr, wi, lamt, a.rows ? 1 : a.op);
+        // the next step's factors (every thread has used fac[cur ^ 1] before the barrier above)
+        const bool more = MULTI && st + 1 < nsteps;
+        if (more) set_fac(a, ch, fac[cur ^ 1], st + 1, tid);
+        __syncthreads();
+        if (!act) continue;
+        double xr[L], xi[L];
+#pragma unroll
+        for (int m = 0; m < L; ++m) {
+            const double2 v = ybase[m * YS];
+            xr[m] = v.x;
+            xi[m] = v.y;
+        }
+        fft<L, false>(xr, xi, wr, wi);
+        if (more) {   // this step's `out` and `line`, the next step's `in`, forward along y, to LDS
             const bool next_plain = a.plain_last && st + 1 == nsteps - 1;
-            set_fac(a, ch, fac[cur ^ 1], st + 1, tid);
-            __syncthreads();
             const uint8_t *so = step_fields(a, ch, a.out, st), *sl = step_fields(a, ch, a.line, st),
                           *si = step_fields(a, ch, a.in, st + 1);
-            for (int e = tid; e < LPW * NS; e += CT) {
-                const int l = e / NS, p = e % NS;
-                if (l >= nlw) continue;
-                double2 v = buf[l * NPAD + p + p / (2 * L)];
+            const double2 fl = use_line ? fac[cur][2][sl[line0 + l] - 1] : make_double2(1.0, 0.0);
+#pragma unroll
+            for (int m = 0; m < L; ++m) {
+                const int p = 2 * (x + L * m) + b;
+                double2 v = make_double2(xr[m], xi[m]);
                 if (use_out) v = cmul_f(v, fac[cur][1][so[p] - 1]);
-                v = cmul_f(v, use_line ? fac[cur][2][sl[line0 + l] - 1] : make_double2(1.0, 0.0));
+                v = cmul_f(v, fl);
                 if (use_in && !next_plain) v = cmul_f(v, fac[cur ^ 1][0][si[p] - 1]);
-                buf[l * NPAD + p + p / (2 * L)] = v;
+                xr[m] = v.x;
+                xi[m] = v.y;
             }
-            __syncthreads();
+            fft<L, true>(xr, xi, wr, wi);
+#pragma unroll
+            for (int m = 0; m < L; ++m) ybase[m * YS] = make_double2(xr[m], xi[m]);
+        } else {   // store (scaled by `out` and the line factor of the last step, unless it is plain)
+            const bool plain = MULTI && a.plain_last;
+            const uint8_t *sout = step_fields(a, ch, a.out, st), *sline = step_fields(a, ch, a.line, st);
+            const double2 fl = use_line && !plain ? fac[cur][2][sline[line0 + l] - 1] : make_double2(1.0, 0.0);
+            const uint32_t o0 = off0(Y), sy = step(Y);
+#pragma unroll
+            for (int m = 0; m < L; ++m) {
+                const int p = 2 * (x + L * m) + b;
+                double2 v = make_double2(xr[m], xi[m]);
+                if (use_out && !plain) v = cmul_f(v, fac[cur][1][sout[p] - 1]);
+                v = cmul_f(v, fl);
+                st_at(Y.re, o0 + m * sy, v.x);
+                st_at(Y.im, o0 + m * sy, v.y);
+            }
         }
-    }
-    // store (scaled by `out` and the line factor of the last step, unless it is plain)
-    const int last = (nsteps - 1) & 1;
-    const bool plain = MULTI && a.plain_last;
-    const uint8_t *sout = step_fields(a, ch, a.out, nsteps - 1), *sline = step_fields(a, ch, a.line, nsteps - 1);
-    for (int e = tid; e < LPW * NS; e += CT) {
-        const int l = a.rows ? e % LPW : e / NS, p = a.rows ? e / LPW : e % NS;
-        if (l >= nlw) continue;
-        double2 v = buf[l * NPAD + p + p / (2 * L)];
-        if (use_out && !plain) v = cmul_f(v, fac[last][1][sout[p] - 1]);
-        v = cmul_f(v, use_line && !plain ? fac[last][2][sline[line0 + l] - 1] : make_double2(1.0, 0.0));
-        const size_t o = a.rows ? (size_t)(line0 + l) + (size_t)p * Y.ld : (size_t)p + (size_t)(line0 + l) * Y.ld;
-        Y.re[o] = v.x;
-        Y.im[o] = v.y;
     }
 }
 
@@ -376,7 +418,9 @@ KDQ_DEV void circ_body(const CircArgs &a, const ChainDev &ch, int c, int grp, un
 // neighbouring groups one after another there: the row pass reads half of each 128-byte line
 // per group, and the neighbouring group finds the other half in the same L2.
 constexpr int NXCD = 8;
-template <bool MULTI>
+// BIG: the instantiation for cell edge 15 too (batches whose largest edge is 15; it needs more
+// registers than the smaller edges, so those launch without it)
+template <bool MULTI, bool BIG>
 KDQ_DEV void circ_dispatch(const CircArgs &a, int nch, int gx, int xcd_map) {
     int c, grp;
     if (xcd_map) {
@@ -398,18 +442,22 @@ KDQ_DEV void circ_dispatch(const CircArgs &a, int nch, int gx, int xcd_map) {
         case 8: circ_body<8, MULTI>(a, ch, c, grp, lds_raw); break;
         case 9: circ_body<9, MULTI>(a, ch, c, grp, lds_raw); break;
         case 12: circ_body<12, MULTI>(a, ch, c, grp, lds_raw); break;
-        case 15: circ_body<15, MULTI>(a, ch, c, grp, lds_raw); break;
+        case 15:
+            if constexpr (BIG) circ_body<15, MULTI>(a, ch, c, grp, lds_raw);
+            break;
         default: break;   // not reached: the host takes the dense path unless every cL is listed
     }
 }
 
+template <bool BIG>
 __global__ __launch_bounds__(CT, 3) void k_circ(CircArgs a, int nch, int gx, int xcd_map) {
-    circ_dispatch<false>(a, nch, gx, xcd_map);
+    circ_dispatch<false, BIG>(a, nch, gx, xcd_map);
 }
 
 // a product of several factors (a.nsteps > 1) in one launch, the lines kept in LDS between them
+template <bool BIG>
 __global__ __launch_bounds__(CT, 2) void k_circ_chain(CircArgs a, int nch, int gx, int xcd_map) {
-    circ_dispatch<true>(a, nch, gx, xcd_map);
+    circ_dispatch<true, BIG>(a, nch, gx, xcd_map);
 }
 
 }  // namespace
@@ -423,11 +471,12 @@ size_t circ_lds_bytes(int L) {
     return sizeof(double2) * lpw * npad + sizeof(double) * 8 * (size_t)L * L;
 }
 
-hipError_t launch_circ(hipStream_t st, int nchains, int grid_x, size_t lds_bytes, const CircArgs &a) {
+hipError_t launch_circ(hipStream_t st, int nchains, int grid_x, size_t lds_bytes, int lmax, const CircArgs &a) {
     if (a.op < 0 || a.op > 2 || (a.rows && a.op != 0) || grid_x < 1 || a.mat < 0 || a.mat > 3 || a.mat_last < 0 ||
-        a.mat_last > 3)
+        a.mat_last > 3 || lmax < 1 || lmax > 15)
         return hipErrorInvalidValue;
-    auto kern = a.nsteps > 1 || a.plain_last ? k_circ_chain : k_circ;
+    const bool big = lmax > 12, multi = a.nsteps > 1 || a.plain_last;
+    auto kern = multi ? (big ? k_circ_chain<true> : k_circ_chain<false>) : (big ? k_circ<true> : k_circ<false>);
     if (hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes))
         return e;

@@ -356,13 +356,15 @@ int gemm_f(kdq_handle h, bool ac, bool bc, int opa, int opb, const GemmArgsF &g,
 bool use_circ(kdq_handle h) { return h->circ && h->circ_L > 0 && h->kind == KDQ_MODEL_TU; }
 
 // grid width and LDS of a k_circ launch over every chain's lines (ldim: DIM_NS or DIM_NP)
-void circ_launch_shape(kdq_handle h, int ldim, int &gx, size_t &bytes) {
+void circ_launch_shape(kdq_handle h, int ldim, int &gx, size_t &bytes, int &lmax) {
     gx = 1;
     bytes = 0;
+    lmax = 1;
     for (int c = 0; c < h->nch; ++c) {
         const int L = h->ch[c].cL, lines = ldim == DIM_NP ? h->ch[c].np : h->ch[c].ns, lpw = circ_lines_per_wg(L);
         gx = std::max(gx, (lines + lpw - 1) / lpw);
         bytes = std::max(bytes, circ_lds_bytes(L));
+        lmax = std::max(lmax, L);
     }
 }
 
@@ -376,8 +378,9 @@ int kin_left(kdq_handle h, int mat, int opa, const Opnd *X, const Opnd *Y, int l
         CircArgs a{h->d_chains, X, Y, mat, opa == 0 ? 0 : (h->cK ? 2 : 1), 0, ldim, in, out, line};
         int gx = 1;
         size_t bytes = 0;
-        circ_launch_shape(h, ldim, gx, bytes);
-        HIPCHK(launch_circ(h->stream, h->nch, gx, bytes, a));
+        int lmax = 1;
+        circ_launch_shape(h, ldim, gx, bytes, lmax);
+        HIPCHK(launch_circ(h->stream, h->nch, gx, bytes, lmax, a));
         return KDQ_OK;
     }
     GemmArgs g = gargs(h, mat ? h->t_eKinv : h->t_eK, X, Y, DIM_NS, ldim, DIM_NS);
@@ -396,8 +399,9 @@ int kin_left_chain(kdq_handle h, int mat, int opa, const Opnd *X, const Opnd *Y,
     CircArgs a{h->d_chains, X, Y, mat, opa == 0 ? 0 : (h->cK ? 2 : 1), 0, ldim, in, out, line, nsteps, dlt};
     int gx = 1;
     size_t bytes = 0;
-    circ_launch_shape(h, ldim, gx, bytes);
-    HIPCHK(launch_circ(h->stream, h->nch, gx, bytes, a));
+    int lmax = 1;
+    circ_launch_shape(h, ldim, gx, bytes, lmax);
+    HIPCHK(launch_circ(h->stream, h->nch, gx, bytes, lmax, a));
     return KDQ_OK;
 }
 
@@ -410,8 +414,9 @@ int kin_right(kdq_handle h, int mat, const Opnd *X, const Opnd *Y, DiagSpec in, 
         CircArgs a{h->d_chains, X, Y, mat, 0, 1, DIM_NS, in, out, line};
         int gx = 1;
         size_t bytes = 0;
-        circ_launch_shape(h, DIM_NS, gx, bytes);
-        HIPCHK(launch_circ(h->stream, h->nch, gx, bytes, a));
+        int lmax = 1;
+        circ_launch_shape(h, DIM_NS, gx, bytes, lmax);
+        HIPCHK(launch_circ(h->stream, h->nch, gx, bytes, lmax, a));
         return KDQ_OK;
     }
     GemmArgs g = gargs(h, X, mat ? h->t_eKinv : h->t_eK, Y, DIM_NS, DIM_NS, DIM_NS);
@@ -1379,9 +1384,10 @@ int measure_chain(kdq_handle h, int lt) {
         ProfScope ps(h, KDQ_PROF_MEAS_WRAP);
         int gx = 1;
         size_t bytes = 0;
-        circ_launch_shape(h, DIM_NS, gx, bytes);
-        HIPCHK(launch_circ(h->stream, h->nch, gx, bytes, l));
-        HIPCHK(launch_circ(h->stream, h->nch, gx, bytes, r));
+        int lmax = 1;
+        circ_launch_shape(h, DIM_NS, gx, bytes, lmax);
+        HIPCHK(launch_circ(h->stream, h->nch, gx, bytes, lmax, l));
+        HIPCHK(launch_circ(h->stream, h->nch, gx, bytes, lmax, r));
     }
     if (fk)
         if (int r2 = meas_fork(h)) return r2;

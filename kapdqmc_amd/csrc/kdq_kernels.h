@@ -51,8 +51,8 @@ bool circ_supported(int L);   // cell edges with a kernel instantiation
 int circ_lines_per_wg(int L);  // lines of X per workgroup for cell edge L
 size_t circ_lds_bytes(int L);  // its dynamic LDS
 // grid_x: the largest workgroup count over the chains (lines / lines per workgroup),
-// lds_bytes: the largest circ_lds_bytes over the chains
-hipError_t launch_circ(hipStream_t st, int nchains, int grid_x, size_t lds_bytes, const CircArgs &a);
+// lds_bytes: the largest circ_lds_bytes over the chains, lmax: the largest cell edge
+hipError_t launch_circ(hipStream_t st, int nchains, int grid_x, size_t lds_bytes, int lmax, const CircArgs &a);
 
 // Local update of one block of sites [s0, s0 + kb) at slice lt (t-U model).
 struct DecideArgs {
