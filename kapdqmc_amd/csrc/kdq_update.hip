@@ -62,6 +62,30 @@ __global__ __launch_bounds__(NTH) void k_decide_tu(DecideArgs a) {
     const int lt = a.lt;
     const int r = tid % KBT, c0 = (tid / KBT) * CPT;
     double wr[CPT], wi[CPT], Wr[CPT], Wi[CPT];
+    // the proposals (draws, phases, weight ratios) first: they depend on nothing below, so their
+    // latency overlaps the loads of the working submatrix and the pending-block correction
+    if (tid < KBT) {
+        accl[tid] = 0;   // entries beyond the accepted count stay valid indices
+        if (tid < kb) {
+            int chv;
+            double u;
+            const uint32_t sweep = a.sweep_dev ? *a.sweep_dev : a.sweep;
+            sweep_draw(ch, sweep, lt, a.dir, (uint32_t)(s0 + tid), chv, u);
+            const int si = ch.s[(size_t)(s0 + tid) + (size_t)(lt - 1) * ch.n_elem];
+            const int sx = chv + 1 + ((chv + 1) >= si ? 1 : 0);
+            const double x = ch.alpha[lt - 1] * (ch.eta[sx - 1] - ch.eta[si - 1]);
+            double sn, cs;
+            sincos(x, &sn, &cs);
+            dR[tid] = cs - 1.0;
+            dI[tid] = sn;
+            pg[tid] = ch.gamma[sx - 1] / ch.gamma[si - 1];
+            // Δ/(1+Δ) = 1 - 1/(1+Δ), with 1+Δ = cis(x)
+            qxr[tid] = 1.0 - cs;
+            qxi[tid] = sn;
+            us[tid] = u;
+            sv[tid] = si | (sx << 8);
+        }
+    }
     if (a.prewrap) {
         // the block of the wrapped G: line ⊙ (T[S,:] E[:,S]) ⊙ out, the product summed over the
         // chunk partials of k_pw_part in chunk order
@@ -245,29 +269,6 @@ __global__ __launch_bounds__(NTH) void k_decide_tu(DecideArgs a) {
                 wi[q] -= si;
             }
             __syncthreads();
-        }
-    }
-    const uint32_t tag = (uint32_t)lt | ((uint32_t)a.dir << 31);
-    if (tid < KBT) {
-        accl[tid] = 0;   // entries beyond the accepted count stay valid indices
-        if (tid < kb) {
-            int chv;
-            double u;
-            const uint32_t sweep = a.sweep_dev ? *a.sweep_dev : a.sweep;
-            sweep_draw(ch, sweep, lt, a.dir, (uint32_t)(s0 + tid), chv, u);
-            const int si = ch.s[(size_t)(s0 + tid) + (size_t)(lt - 1) * ch.n_elem];
-            const int sx = chv + 1 + ((chv + 1) >= si ? 1 : 0);
-            const double x = ch.alpha[lt - 1] * (ch.eta[sx - 1] - ch.eta[si - 1]);
-            double sn, cs;
-            sincos(x, &sn, &cs);
-            dR[tid] = cs - 1.0;
-            dI[tid] = sn;
-            pg[tid] = ch.gamma[sx - 1] / ch.gamma[si - 1];
-            // Δ/(1+Δ) = 1 - 1/(1+Δ), with 1+Δ = cis(x)
-            qxr[tid] = 1.0 - cs;
-            qxi[tid] = sn;
-            us[tid] = u;
-            sv[tid] = si | (sx << 8);
         }
     }
 #pragma unroll
